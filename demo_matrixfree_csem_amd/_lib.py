@@ -1,0 +1,180 @@
+"""ctypes view of libcsem.so: the reference-shaped host entry points
+(include/csem.h) and the device C-ABI (include/csem_dev.h).
+
+Loading fails loudly when the library has not been built: there is no
+other implementation behind these calls.
+"""
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libcsem.so")
+
+c_cplx = C.c_double * 2
+vp = C.c_void_p
+
+
+class EmfT(C.Structure):
+    """emf_t (include/csem.h), field for field."""
+    _fields_ = [
+        ("mode", C.c_int), ("verb", C.c_int), ("reciprocity", C.c_int), ("fvm", C.c_int), ("airbc", C.c_int),
+        ("nfreq", C.c_int), ("freqs", C.POINTER(C.c_float)),
+        ("rho_air", C.c_float), ("lextend", C.c_float), ("rho_water", C.c_float),
+        ("rhomin", C.c_float), ("rhomax", C.c_float),
+        ("nchsrc", C.c_int), ("nchrec", C.c_int), ("chsrc", vp), ("chrec", vp),
+        ("istretch", C.c_int), ("rstretch", C.c_float),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nb", C.c_int),
+        ("n1", C.c_int), ("n2", C.c_int), ("n3", C.c_int),
+        ("lv", C.c_int), ("lE", C.c_int), ("lH", C.c_int), ("lEx", C.c_int), ("lEy", C.c_int), ("lEz", C.c_int),
+        ("lHx", C.c_int), ("lHy", C.c_int), ("lHz", C.c_int),
+        ("x1node", vp), ("x2node", vp), ("x3node", vp),
+        ("x1", C.POINTER(C.c_double)), ("x2", C.POINTER(C.c_double)), ("x3", C.POINTER(C.c_double)),
+        ("x1s", C.POINTER(C.c_double)), ("x2s", C.POINTER(C.c_double)), ("x3s", C.POINTER(C.c_double)),
+        ("d1", C.POINTER(C.c_double)), ("d2", C.POINTER(C.c_double)), ("d3", C.POINTER(C.c_double)),
+        ("d1s", C.POINTER(C.c_double)), ("d2s", C.POINTER(C.c_double)), ("d3s", C.POINTER(C.c_double)),
+        ("d1min", C.c_double), ("d2min", C.c_double), ("d3min", C.c_double),
+        ("x1min", C.c_double), ("x2min", C.c_double), ("x3min", C.c_double),
+        ("x1max", C.c_double), ("x2max", C.c_double), ("x3max", C.c_double),
+        ("n1fft", C.c_int), ("n2fft", C.c_int),
+        ("d1fft", C.c_double), ("d2fft", C.c_double), ("d3fft", C.c_double),
+        ("sH1kxky", C.POINTER(C.POINTER(C.c_double))), ("sH2kxky", C.POINTER(C.POINTER(C.c_double))),
+        ("rho11", vp), ("rho22", vp), ("rho33", vp),
+        ("sigma11", C.POINTER(C.POINTER(C.POINTER(C.c_double)))),
+        ("sigma22", C.POINTER(C.POINTER(C.POINTER(C.c_double)))),
+        ("sigma33", C.POINTER(C.POINTER(C.POINTER(C.c_double)))),
+        ("E1", vp), ("E2", vp), ("E3", vp),
+        ("dcal_fd", C.POINTER(C.POINTER(C.POINTER(C.c_float)))),
+        ("ne", C.c_int), ("nh", C.c_int),
+        ("xe", vp), ("xh", vp), ("be", vp), ("v", vp), ("bv", vp), ("xx", vp), ("yy", vp),
+        ("I_omega_mu0", c_cplx),
+        ("preco", C.c_int), ("algopt", C.c_int), ("niter", C.c_int), ("nrestart", C.c_int), ("mp", C.c_int),
+        ("tol", C.c_float), ("param_extended", C.c_int),
+    ]
+
+
+class AcqT(C.Structure):
+    _fields_ = [
+        ("nsrc", C.c_int), ("nrec", C.c_int), ("nsubsrc", C.c_int), ("nsubrec", C.c_int),
+        ("lensrc", C.c_float), ("lenrec", C.c_float),
+        ("src_x1", C.POINTER(C.c_float)), ("src_x2", C.POINTER(C.c_float)), ("src_x3", C.POINTER(C.c_float)),
+        ("src_azimuth", vp), ("src_dip", vp),
+        ("rec_x1", C.POINTER(C.c_float)), ("rec_x2", C.POINTER(C.c_float)), ("rec_x3", C.POINTER(C.c_float)),
+        ("rec_azimuth", vp), ("rec_dip", vp),
+        ("x1min", C.c_float), ("x1max", C.c_float), ("x2min", C.c_float), ("x2max", C.c_float),
+        ("x3min", C.c_float), ("x3max", C.c_float),
+        ("nsrc_total", C.c_int), ("nrec_total", C.c_int), ("shot_idx", vp),
+    ]
+
+
+class FreqT(C.Structure):
+    _fields_ = [("iomu_re", C.c_double), ("iomu_im", C.c_double),
+                ("iwmu_re", C.c_double), ("iwmu_im", C.c_double)]
+
+
+# every symbol the headers in include/ declare
+HOST_SYMBOLS = [
+    "emf_init", "emf_free", "acq_init", "acq_free", "extend_model_init", "extend_model_free",
+    "fdfd_init", "fdfd_free", "fdfd_apply", "lusgs_apply", "airbc_init", "airbc_free", "airbc_apply",
+    "bicgstab", "gmres", "fbicgstab", "fgmres", "cdotprod", "inject_source", "extract_emf", "write_data",
+    "find_index", "create_nugrid", "initargs", "getparint", "getparfloat", "getpardouble", "getparstring",
+    "getparstringarray", "getnparint", "getnparfloat", "getnpardouble", "getnparstring", "countparname",
+    "countparval", "countnparval", "alloc1", "alloc2", "alloc3", "free1", "free2", "free3", "alloc1int",
+    "free1int", "alloc1float", "alloc3float", "free1float", "free3float", "alloc1double", "alloc3double",
+    "free1double", "free3double", "alloc1complex", "alloc2complex", "free1complex", "free2complex",
+    "alloc3complexf", "free3complexf", "err", "warn", "csem_last_status", "csem_set_exit_on_error",
+    "csem_active_model", "csem_last_iterations", "csem_fdfd_prepare", "csem_model_from_emf",
+    "iproc", "nproc", "ierr",
+]
+DEV_SYMBOLS = [
+    "csem_last_error", "csem_freq_consts", "csem_model_create", "csem_model_destroy", "csem_model_len",
+    "csem_model_full_len", "csem_model_bytes", "csem_apply", "csem_lusgs", "csem_curl_h", "csem_airbc",
+    "csem_fft2", "csem_dot", "csem_bicgstab", "csem_fgmres", "csem_gmres", "csem_scatter_add", "csem_gather",
+    "csem_malloc", "csem_free", "csem_memcpy_h2d", "csem_memcpy_d2h", "csem_memset", "csem_sync",
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libcsem.so is not built (%s); run python -m demo_matrixfree_csem_amd.build"
+                           % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    ip, dp, lp = C.c_int, C.c_double, C.c_long
+    L.csem_last_error.restype = C.c_char_p
+    L.csem_freq_consts.argtypes = [C.c_float, C.POINTER(FreqT)]
+    L.csem_model_destroy.argtypes = [vp]
+    for f in ("csem_model_len", "csem_model_full_len", "csem_model_bytes"):
+        getattr(L, f).restype = lp
+        getattr(L, f).argtypes = [vp]
+    L.csem_apply.argtypes = [vp, C.POINTER(FreqT), vp, vp, vp]
+    L.csem_lusgs.argtypes = [vp, C.POINTER(FreqT), vp, vp, vp]
+    L.csem_curl_h.argtypes = [vp, C.POINTER(FreqT), vp, vp, vp]
+    L.csem_airbc.argtypes = [vp, vp, vp, vp, vp]
+    L.csem_fft2.argtypes = [vp, vp, ip, vp]
+    L.csem_dot.argtypes = [lp, vp, vp, vp, vp]
+    L.csem_bicgstab.argtypes = [vp, C.POINTER(FreqT), vp, vp, ip, dp, ip, vp, vp]
+    L.csem_fgmres.argtypes = [vp, C.POINTER(FreqT), vp, vp, ip, dp, ip, ip, vp, vp]
+    L.csem_gmres.argtypes = [vp, C.POINTER(FreqT), vp, vp, ip, dp, ip, vp, vp]
+    L.csem_scatter_add.argtypes = [lp, vp, vp, vp, vp]
+    L.csem_gather.argtypes = [ip, vp, vp, vp, vp, vp, vp, vp]
+    L.csem_malloc.argtypes = [C.POINTER(vp), lp]
+    L.csem_free.argtypes = [vp]
+    L.csem_memcpy_h2d.argtypes = [vp, vp, lp, vp]
+    L.csem_memcpy_d2h.argtypes = [vp, vp, lp, vp]
+    L.csem_memset.argtypes = [vp, ip, lp, vp]
+    L.csem_sync.argtypes = [vp]
+    L.csem_active_model.restype = vp
+    L.csem_fdfd_prepare.argtypes = [C.POINTER(EmfT)]
+    L.csem_model_from_emf.argtypes = [C.POINTER(EmfT), C.POINTER(vp)]
+    for f in ("emf_init", "emf_free", "fdfd_init", "fdfd_free", "extend_model_free"):
+        getattr(L, f).argtypes = [C.POINTER(EmfT)]
+    L.acq_init.argtypes = [C.POINTER(AcqT), C.POINTER(EmfT)]
+    L.extend_model_init.argtypes = [C.POINTER(AcqT), C.POINTER(EmfT)]
+    L.acq_free.argtypes = [C.POINTER(AcqT)]
+    L.inject_source.argtypes = [C.POINTER(AcqT), C.POINTER(EmfT), vp]
+    L.extract_emf.argtypes = [C.POINTER(AcqT), C.POINTER(EmfT), vp, ip]
+    L.write_data.argtypes = [C.POINTER(AcqT), C.POINTER(EmfT), C.c_char_p, vp]
+    L.initargs.argtypes = [ip, C.POINTER(C.c_char_p)]
+    L.fdfd_apply.argtypes = [ip, vp, vp]
+    L.lusgs_apply.argtypes = [ip, vp, vp]
+    L.airbc_apply.argtypes = [C.POINTER(EmfT), vp, vp, vp]
+    op_t = C.CFUNCTYPE(None, ip, vp, vp)
+    L.fbicgstab.argtypes = [ip, vp, vp, vp, vp, ip, dp, ip]
+    L.bicgstab.argtypes = [ip, vp, vp, vp, ip, dp, ip]
+    L.fgmres.argtypes = [ip, vp, vp, vp, vp, ip, dp, ip, ip]
+    L.gmres.argtypes = [ip, vp, vp, vp, ip, dp, ip, ip]
+    L.cdotprod.restype = _C128
+    L.cdotprod.argtypes = [ip, vp, vp]
+    L.find_index.argtypes = [ip, vp, dp]
+    L.create_nugrid.restype = dp
+    L.create_nugrid.argtypes = [ip, dp, dp, vp, ip]
+    L.alloc1complex.restype = vp
+    L.alloc1complex.argtypes = [C.c_size_t]
+    L.alloc3complexf.restype = C.POINTER(C.POINTER(C.POINTER(C.c_float)))
+    L.alloc3complexf.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t]
+    L.csem_set_exit_on_error.argtypes = [ip]
+    _lib = L
+    return L
+
+
+class _C128(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+def last_error():
+    return lib().csem_last_error().decode(errors="replace")
+
+
+def check(rc, what=""):
+    if rc < 0:
+        raise RuntimeError("%s failed (%d): %s" % (what, rc, last_error()))
+    return rc
+
+
+def fn_addr(name):
+    """Address of an exported function, for op_t arguments."""
+    return C.cast(getattr(lib(), name), vp).value

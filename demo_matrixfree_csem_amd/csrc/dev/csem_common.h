@@ -1,0 +1,105 @@
+// csem_common.h -- shared device helpers for the MI355X solve path.
+//
+// Complex arithmetic is written out in real arithmetic with the exact
+// expression shapes gcc uses for C99 complex on x86-64 (no FMA contraction;
+// every file including this one is compiled with -ffp-contract=off):
+//   a*b   -> (ar*br - ai*bi, ar*bi + ai*br)
+//   a/r   -> (ar/r, ai/r)               (real divisor)
+//   r*a   -> (r*ar, r*ai)               (real factor)
+//   a/b   -> Smith's algorithm as in libgcc's __divdc3 (normal range)
+// so device results equal the reference's bit for bit on the same inputs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double2 z2;
+
+__host__ __device__ __forceinline__ z2 zmk(double x, double y) { return make_double2(x, y); }
+__host__ __device__ __forceinline__ z2 zadd(z2 a, z2 b) { return zmk(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ z2 zsub(z2 a, z2 b) { return zmk(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ z2 zmul(z2 a, z2 b) { return zmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ __forceinline__ z2 zrmul(double r, z2 a) { return zmk(r * a.x, r * a.y); }
+__host__ __device__ __forceinline__ z2 zmulr(z2 a, double r) { return zmk(a.x * r, a.y * r); }
+__host__ __device__ __forceinline__ z2 zdivr(z2 a, double r) { return zmk(a.x / r, a.y / r); }
+// complex minus real, real part only
+__host__ __device__ __forceinline__ z2 zsubr(z2 a, double r) { return zmk(a.x - r, a.y); }
+
+// Smith division, libgcc2.c __divdc3 for operands in the normal range.
+__host__ __device__ __forceinline__ z2 zdiv(z2 n, z2 q)
+{
+  double a = n.x, b = n.y, c = q.x, d = q.y, ratio, denom;
+  if (fabs(c) < fabs(d)) {
+    ratio = c / d;
+    denom = (c * ratio) + d;
+    return zmk(((a * ratio) + b) / denom, ((b * ratio) - a) / denom);
+  }
+  ratio = d / c;
+  denom = (d * ratio) + c;
+  return zmk(((b * ratio) + a) / denom, (b - (a * ratio)) / denom);
+}
+
+// Single precision counterparts for the mixed-precision inner solver.
+typedef float2 c2;
+__host__ __device__ __forceinline__ c2 cmk(float x, float y) { return make_float2(x, y); }
+__host__ __device__ __forceinline__ c2 cadd(c2 a, c2 b) { return cmk(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ c2 csub(c2 a, c2 b) { return cmk(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ c2 cmul(c2 a, c2 b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ __forceinline__ c2 crmul(float r, c2 a) { return cmk(r * a.x, r * a.y); }
+__host__ __device__ __forceinline__ c2 cdivr(c2 a, float r) { return cmk(a.x / r, a.y / r); }
+__host__ __device__ __forceinline__ c2 cdiv(c2 n, c2 q)
+{
+  float a = n.x, b = n.y, c = q.x, d = q.y, ratio, denom;
+  if (fabsf(c) < fabsf(d)) {
+    ratio = c / d;
+    denom = (c * ratio) + d;
+    return cmk(((a * ratio) + b) / denom, ((b * ratio) - a) / denom);
+  }
+  ratio = d / c;
+  denom = (d * ratio) + c;
+  return cmk(((b * ratio) + a) / denom, (b - (a * ratio)) / denom);
+}
+
+// Deterministic reduction layout shared by every kernel that produces dot
+// product partials: chunks of RED_CHUNK = RED_T * RED_E consecutive
+// elements, thread t of a 256-thread block sums elements
+// chunk*RED_CHUNK + m*RED_T + t for m = 0..RED_E-1 in order, then the
+// block folds v[t] += v[t+h] for h = 128, 64, ..., 1.
+#define RED_T 256
+#define RED_E 16
+#define RED_CHUNK (RED_T * RED_E)
+
+// Block fold of one double per thread (blockDim == 256), result in lane 0
+// of wave 0.  `sh` must hold 256 doubles.
+__device__ __forceinline__ double block_fold(double v, double *sh)
+{
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  if (t < 128) sh[t] = sh[t] + sh[t + 128];
+  __syncthreads();
+  if (t < 64) {
+    v = sh[t] + sh[t + 64];
+    // within wave 0: v[t] += v[t+h], h = 32..1
+    for (int h = 32; h >= 1; h >>= 1) v = v + __shfl_down(v, h, 64);
+  }
+  __syncthreads();
+  return v;
+}
+
+struct Dims {
+  int n1, n2, n3;      // operator grid (n3 = operator depth)
+  long lEx, lEy, lEz;  // operator component lengths
+};
+
+__host__ __device__ __forceinline__ long id1(int n1, int n2, int k, int j, int i) { return i + (long)n1 * (j + (long)(n2 + 1) * k); }
+__host__ __device__ __forceinline__ long id2(int n1, int n2, int k, int j, int i) { return i + (long)(n1 + 1) * (j + (long)n2 * k); }
+__host__ __device__ __forceinline__ long id3(int n1, int n2, int k, int j, int i) { return i + (long)(n1 + 1) * (j + (long)(n2 + 1) * k); }
+
+#define HIP_OK(call)                                                       \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess) return csem_fail(-1, "%s: %s (%s:%d)", #call,    \
+                                           hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+extern "C" int csem_fail(int code, const char *fmt, ...);

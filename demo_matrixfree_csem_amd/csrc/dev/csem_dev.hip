@@ -1,0 +1,740 @@
+// csem_dev.hip -- device model, launchers and the device-resident FP64
+// BiCGStab of the MI355X solve path (C-ABI in include/csem_dev.h).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+#include <algorithm>
+
+#include "csem_kernels.h"
+#include "../../../include/csem_dev.h"
+
+static char g_err[1024];
+
+extern "C" int csem_fail(int code, const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+extern "C" const char *csem_last_error(void) { return g_err; }
+
+#define PI_REF 3.14159265358979323846264
+
+extern "C" int csem_freq_consts(float freq, csem_freq *out)
+{
+  // I*2.*PI*f*mu0 (main.c:95) has a zero real part and imaginary part
+  // ((2*PI)*f)*mu0; 1./that goes through complex division (fdfd.c:100).
+  double w = ((2. * PI_REF) * (double)freq) * (4. * PI_REF * 1e-7);
+  z2 iomu = zmk(0.0, w);
+  z2 iwmu = zdiv(zmk(1.0, 0.0), iomu);
+  out->iomu_re = iomu.x; out->iomu_im = iomu.y;
+  out->iwmu_re = iwmu.x; out->iwmu_im = iwmu.y;
+  return 0;
+}
+
+struct csem_model {
+  int n1, n2, n3f, n3, nb, airbc, fvm, kofs, nfft, kc;
+  long lEx, lEy, lEz, lE, fEx, fEy, fEz, fE;
+  double *d1, *d2, *d3, *d1s, *d2s, *d3s;
+  double *s11, *s22, *s33;
+  z2 *W;
+  z2 *sH1, *sH2, *tw, *buf, *bufx, *bufy, *hz0, *hxg, *hyg;
+  int *jx, *jy, *rows, nrowx, nrowy;
+  double *wx, *wy;
+  int *gxj1, *gxj2, *gyj1, *gyj2;
+  double *gxw1, *gxw2, *gyw1, *gyw2;
+  double *partials, *scal;
+  long nch;
+  z2 *kv;  // Krylov work vectors (lazily allocated)
+  int nkv;
+  long bytes;
+  std::vector<void *> allocs;
+};
+
+template <class T>
+static int dalloc(csem_model *m, T **p, long count)
+{
+  long bytes = count * (long)sizeof(T);
+  if (bytes <= 0) bytes = 256;
+  hipError_t e = hipMalloc((void **)p, bytes);
+  if (e != hipSuccess) return csem_fail(-2, "hipMalloc(%ld): %s", bytes, hipGetErrorString(e));
+  m->allocs.push_back((void *)*p);
+  m->bytes += bytes;
+  return 0;
+}
+
+template <class T>
+static int upload(csem_model *m, T **p, const void *src, long count)
+{
+  int rc = dalloc(m, p, count);
+  if (rc) return rc;
+  HIP_OK(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static bool g_attr_done = false;
+static int set_lds_attrs()
+{
+  if (g_attr_done) return 0;
+  HIP_OK(hipFuncSetAttribute((const void *)k_air_cols<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+  HIP_OK(hipFuncSetAttribute((const void *)k_fft_cols<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+  HIP_OK(hipFuncSetAttribute((const void *)k_air_rows_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+  HIP_OK(hipFuncSetAttribute((const void *)k_air_rows_inv, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+  HIP_OK(hipFuncSetAttribute((const void *)k_fft_rows, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+  g_attr_done = true;
+  return 0;
+}
+
+extern "C" void csem_model_destroy(csem_model *m)
+{
+  if (!m) return;
+  for (void *p : m->allocs) hipFree(p);
+  delete m;
+}
+
+extern "C" int csem_model_create(const csem_model_desc *d, csem_model **out)
+{
+  *out = NULL;
+  if (d->n1 < 2 || d->n2 < 2 || d->n3f < 2) return csem_fail(-3, "grid too small: %d %d %d", d->n1, d->n2, d->n3f);
+  if (d->airbc && d->nb < 1) return csem_fail(-3, "airbc=1 needs nb>0");
+  if (d->n3f - 1 > 1024 || d->n2 - 1 > 1024) return csem_fail(-3, "sweep rows exceed 1024 threads");
+  csem_model *m = new csem_model();
+  int rc;
+  m->n1 = d->n1; m->n2 = d->n2; m->n3f = d->n3f; m->nb = d->nb; m->airbc = d->airbc; m->fvm = d->fvm;
+  m->kofs = d->airbc ? d->nb : 0;
+  m->n3 = d->n3f - m->kofs;
+  m->kc = 8;
+  const int n1 = m->n1, n2 = m->n2, n3 = m->n3, n3f = m->n3f;
+  m->lEx = (long)n1 * (n2 + 1) * (n3 + 1);
+  m->lEy = (long)(n1 + 1) * n2 * (n3 + 1);
+  m->lEz = (long)(n1 + 1) * (n2 + 1) * n3;
+  m->lE = m->lEx + m->lEy + m->lEz;
+  m->fEx = (long)n1 * (n2 + 1) * (n3f + 1);
+  m->fEy = (long)(n1 + 1) * n2 * (n3f + 1);
+  m->fEz = (long)(n1 + 1) * (n2 + 1) * n3f;
+  m->fE = m->fEx + m->fEy + m->fEz;
+#define UP(dst, src, cnt) if ((rc = upload(m, &m->dst, src, cnt))) goto fail
+  UP(d1, d->d1, n1 + 1); UP(d2, d->d2, n2 + 1); UP(d3, d->d3, n3f + 1);
+  UP(d1s, d->d1s, n1 + 1); UP(d2s, d->d2s, n2 + 1); UP(d3s, d->d3s, n3f + 1);
+  UP(s11, d->sigma11, m->fEx); UP(s22, d->sigma22, m->fEy); UP(s33, d->sigma33, m->fEz);
+  if ((rc = dalloc(m, &m->W, m->fE))) goto fail;
+  if (hipMemset(m->W, 0, m->fE * sizeof(z2)) != hipSuccess) { rc = csem_fail(-1, "memset W"); goto fail; }
+  m->nch = (m->fE + RED_CHUNK - 1) / RED_CHUNK;
+  if ((rc = dalloc(m, &m->partials, m->nch * 4))) goto fail;
+  if ((rc = dalloc(m, &m->scal, 64))) goto fail;
+  if (hipMemset(m->scal, 0, 64 * sizeof(double)) != hipSuccess) { rc = csem_fail(-1, "memset scal"); goto fail; }
+  if (m->airbc) {
+    const int N = d->nfft;
+    if (N < 4 || (N & (N - 1)) || N > 2048) { rc = csem_fail(-3, "nfft=%d must be a power of two in [4, 2048]", N); goto fail; }
+    m->nfft = N;
+    long NN = (long)N * N;
+    UP(sH1, d->sH1, NN); UP(sH2, d->sH2, NN); UP(tw, d->twiddle, N);
+    UP(jx, d->jx, N); UP(jy, d->jy, N); UP(wx, d->wx, N); UP(wy, d->wy, N);
+    UP(gxj1, d->gxj1, n1); UP(gxj2, d->gxj2, n2); UP(gxw1, d->gxw1, n1); UP(gxw2, d->gxw2, n2);
+    UP(gyj1, d->gyj1, n1); UP(gyj2, d->gyj2, n2); UP(gyw1, d->gyw1, n1); UP(gyw2, d->gyw2, n2);
+    // rows of the inverse transforms that the back-interpolation reads
+    std::vector<int> rx, ry;
+    for (int i2 = 0; i2 < n2; i2++) { rx.push_back(d->gxj2[i2]); rx.push_back(d->gxj2[i2] + 1); }
+    for (int i2 = 0; i2 < n2; i2++) { ry.push_back(d->gyj2[i2]); ry.push_back(d->gyj2[i2] + 1); }
+    std::sort(rx.begin(), rx.end()); rx.erase(std::unique(rx.begin(), rx.end()), rx.end());
+    std::sort(ry.begin(), ry.end()); ry.erase(std::unique(ry.begin(), ry.end()), ry.end());
+    for (int r : rx) if (r < 0 || r >= N) { rc = csem_fail(-3, "ghost row %d outside transform", r); goto fail; }
+    for (int r : ry) if (r < 0 || r >= N) { rc = csem_fail(-3, "ghost row %d outside transform", r); goto fail; }
+    for (int i1 = 0; i1 < n1; i1++)
+      if (d->gxj1[i1] < 0 || d->gxj1[i1] + 1 >= N || d->gyj1[i1] < 0 || d->gyj1[i1] + 1 >= N) {
+        rc = csem_fail(-3, "ghost column outside transform"); goto fail;
+      }
+    for (int q = 0; q < N; q++)
+      if ((d->jx[q] >= 0 && d->jx[q] + 1 >= n1) || (d->jy[q] >= 0 && d->jy[q] + 1 >= n2)) {
+        rc = csem_fail(-3, "forward interpolation index outside Hz plane"); goto fail;
+      }
+    m->nrowx = (int)rx.size(); m->nrowy = (int)ry.size();
+    std::vector<int> rows(rx);
+    rows.insert(rows.end(), ry.begin(), ry.end());
+    UP(rows, rows.data(), (long)rows.size());
+    if ((rc = dalloc(m, &m->buf, NN))) goto fail;
+    if ((rc = dalloc(m, &m->bufx, NN))) goto fail;
+    if ((rc = dalloc(m, &m->bufy, NN))) goto fail;
+    if ((rc = dalloc(m, &m->hz0, (long)n1 * n2))) goto fail;
+    if ((rc = dalloc(m, &m->hxg, (long)(n1 + 1) * n2))) goto fail;
+    if ((rc = dalloc(m, &m->hyg, (long)n1 * (n2 + 1)))) goto fail;
+  }
+#undef UP
+  if ((rc = set_lds_attrs())) goto fail;
+  *out = m;
+  return 0;
+fail:
+  csem_model_destroy(m);
+  return rc;
+}
+
+extern "C" long csem_model_len(const csem_model *m) { return m->lE; }
+extern "C" long csem_model_full_len(const csem_model *m) { return m->fE; }
+extern "C" long csem_model_bytes(const csem_model *m) { return m->bytes; }
+
+static inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+static ApplyArgs apply_args(const csem_model *m, const csem_freq *f, const z2 *x, z2 *y)
+{
+  ApplyArgs a;
+  const int n1 = m->n1, n2 = m->n2, ks = m->kofs;
+  a.x = x; a.y = y;
+  a.s11 = m->s11 + (long)ks * n1 * (n2 + 1);
+  a.s22 = m->s22 + (long)ks * (n1 + 1) * n2;
+  a.s33 = m->s33 + (long)ks * (n1 + 1) * (n2 + 1);
+  a.d1 = m->d1; a.d2 = m->d2; a.d3 = m->d3 + ks;
+  a.d1s = m->d1s; a.d2s = m->d2s; a.d3s = m->d3s + ks;
+  a.hxg = m->hxg; a.hyg = m->hyg;
+  a.iwmu = zmk(f->iwmu_re, f->iwmu_im);
+  a.iomu = zmk(f->iomu_re, f->iomu_im);
+  a.n1 = n1; a.n2 = n2; a.n3 = m->n3; a.fvm = m->fvm; a.airbc = m->airbc; a.kc = m->kc;
+  a.lEx = m->lEx; a.lEy = m->lEy;
+  return a;
+}
+
+static AirArgs air_args(const csem_model *m)
+{
+  AirArgs a;
+  a.N = m->nfft; a.n1 = m->n1; a.n2 = m->n2;
+  a.hz = m->hz0; a.buf = m->buf; a.bufx = m->bufx; a.bufy = m->bufy;
+  a.sH1 = m->sH1; a.sH2 = m->sH2; a.tw = m->tw;
+  a.jx = m->jx; a.jy = m->jy; a.wx = m->wx; a.wy = m->wy;
+  a.rows = m->rows; a.nrowx = m->nrowx; a.nrowy = m->nrowy;
+  a.gxj1 = m->gxj1; a.gxj2 = m->gxj2; a.gyj1 = m->gyj1; a.gyj2 = m->gyj2;
+  a.gxw1 = m->gxw1; a.gxw2 = m->gxw2; a.gyw1 = m->gyw1; a.gyw2 = m->gyw2;
+  a.hxg = m->hxg; a.hyg = m->hyg;
+  return a;
+}
+
+static int run_airbc(csem_model *m, hipStream_t s)
+{
+  AirArgs a = air_args(m);
+  const int N = m->nfft;
+  const size_t row_lds = 2 * (size_t)N * sizeof(z2);
+  k_air_rows_fwd<<<N, 256, row_lds, s>>>(a);
+  k_air_cols<2><<<N / 2, 256, 3 * 2 * (size_t)N * sizeof(z2), s>>>(a);
+  k_air_rows_inv<<<m->nrowx + m->nrowy, 256, row_lds, s>>>(a);
+  const int ng = (m->n1 + 1) * m->n2 + m->n1 * (m->n2 + 1);
+  k_air_ghost<<<(ng + 255) / 256, 256, 0, s>>>(a);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+static int run_apply(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStream_t s)
+{
+  ApplyArgs a = apply_args(m, f, x, y);
+  if (m->airbc) {
+    const int nh = m->n1 * m->n2;
+    k_hz0<<<(nh + 255) / 256, 256, 0, s>>>(a, m->hz0);
+    int rc = run_airbc(m, s);
+    if (rc) return rc;
+  }
+  dim3 blk(64, 4);
+  dim3 grd((m->n1 + 1 + 63) / 64, (m->n2 + 1 + 3) / 4, (m->n3 + 1 + m->kc - 1) / m->kc);
+  k_apply<64, 4><<<grd, blk, 0, s>>>(a);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStream_t s)
+{
+  LuArgs a;
+  a.x = x; a.W = m->W; a.y = y;
+  a.s11 = m->s11; a.s22 = m->s22; a.s33 = m->s33;
+  a.d1 = m->d1; a.d2 = m->d2; a.d3 = m->d3; a.d1s = m->d1s; a.d2s = m->d2s; a.d3s = m->d3s;
+  a.iwmu = zmk(f->iwmu_re, f->iwmu_im);
+  a.iomu = zmk(f->iomu_re, f->iomu_im);
+  a.fvm = m->fvm; a.n1 = m->n1; a.n2 = m->n2; a.n3f = m->n3f; a.kofs = m->kofs; a.n3op = m->n3;
+  a.fEx = m->fEx; a.fEy = m->fEy; a.lEx = m->lEx; a.lEy = m->lEy;
+  const int tk = ((m->n3f - 1 + 63) / 64) * 64;  // rows k = 1..n3f-1
+  const int tj = ((m->n2 - 1 + 63) / 64) * 64;   // rows j = 1..n2-1
+  const size_t lk = 2 * tk * sizeof(z2), lj = 2 * tj * sizeof(z2);
+  k_sgs_ex<true><<<m->n1, tk, lk, s>>>(a);
+  k_sgs_ey<true><<<m->n2, tk, lk, s>>>(a);
+  k_sgs_ez<true><<<m->n3f, tj, lj, s>>>(a);
+  k_sgs_ez<false><<<m->n3f, tj, lj, s>>>(a);
+  k_sgs_ey<false><<<m->n2, tk, lk, s>>>(a);
+  k_sgs_ex<false><<<m->n1, tk, lk, s>>>(a);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int csem_apply(csem_model *m, const csem_freq *f, const void *x, void *y, void *stream)
+{
+  return run_apply(m, f, (const z2 *)x, (z2 *)y, S(stream));
+}
+
+extern "C" int csem_lusgs(csem_model *m, const csem_freq *f, const void *x, void *y, void *stream)
+{
+  return run_lusgs(m, f, (const z2 *)x, (z2 *)y, S(stream));
+}
+
+extern "C" int csem_curl_h(csem_model *m, const csem_freq *f, const void *x, void *h, void *stream)
+{
+  ApplyArgs a = apply_args(m, f, (const z2 *)x, NULL);
+  long tot = (long)(m->n1 + 1) * (m->n2 + 1) * (m->n3 + 1);
+  long lHx = (long)(m->n1 + 1) * m->n2 * m->n3, lHy = (long)m->n1 * (m->n2 + 1) * m->n3;
+  k_curl_h<<<(unsigned)((tot + 255) / 256), 256, 0, S(stream)>>>(a, (z2 *)h, lHx, lHy);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int csem_airbc(csem_model *m, const void *hz0, void *hxg, void *hyg, void *stream)
+{
+  if (!m->airbc) return csem_fail(-3, "model has no airwave boundary");
+  hipStream_t s = S(stream);
+  HIP_OK(hipMemcpyAsync(m->hz0, hz0, (size_t)m->n1 * m->n2 * sizeof(z2), hipMemcpyDeviceToDevice, s));
+  int rc = run_airbc(m, s);
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(hxg, m->hxg, (size_t)(m->n1 + 1) * m->n2 * sizeof(z2), hipMemcpyDeviceToDevice, s));
+  HIP_OK(hipMemcpyAsync(hyg, m->hyg, (size_t)m->n1 * (m->n2 + 1) * sizeof(z2), hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+extern "C" int csem_fft2(csem_model *m, void *data, int dir, void *stream)
+{
+  if (!m->airbc) return csem_fail(-3, "model has no transform tables");
+  const int N = m->nfft;
+  hipStream_t s = S(stream);
+  z2 *d = (z2 *)data;
+  if (dir < 0) {
+    k_fft_rows<<<N, 256, 2 * (size_t)N * sizeof(z2), s>>>(d, N, m->tw, -1);
+    k_fft_cols<2><<<N / 2, 256, 2 * 2 * (size_t)N * sizeof(z2), s>>>(d, N, m->tw, -1);
+  } else {
+    k_fft_cols<2><<<N / 2, 256, 2 * 2 * (size_t)N * sizeof(z2), s>>>(d, N, m->tw, +1);
+    k_fft_rows<<<N, 256, 2 * (size_t)N * sizeof(z2), s>>>(d, N, m->tw, +1);
+  }
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+// --------------------------------------------------------------- dots
+
+static int dot_launch(long n, const z2 *a0, const z2 *b0, const z2 *a1, const z2 *b1, double *partials,
+                      double *out, hipStream_t s)
+{
+  DotArgs g;
+  g.n = n; g.a[0] = a0; g.b[0] = b0; g.a[1] = a1; g.b[1] = b1; g.partials = partials;
+  long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
+  if (a1) {
+    k_dot_partial<2><<<(unsigned)nch, RED_T, 0, s>>>(g);
+    k_dot_final<2><<<1, RED_T, 0, s>>>(partials, nch, out);
+  } else {
+    k_dot_partial<1><<<(unsigned)nch, RED_T, 0, s>>>(g);
+    k_dot_final<1><<<1, RED_T, 0, s>>>(partials, nch, out);
+  }
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+static int dot_partial_launch(long n, const z2 *a0, const z2 *b0, const z2 *a1, const z2 *b1, double *partials,
+                              hipStream_t s)
+{
+  DotArgs g;
+  g.n = n; g.a[0] = a0; g.b[0] = b0; g.a[1] = a1; g.b[1] = b1; g.partials = partials;
+  long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
+  if (a1) k_dot_partial<2><<<(unsigned)nch, RED_T, 0, s>>>(g);
+  else k_dot_partial<1><<<(unsigned)nch, RED_T, 0, s>>>(g);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+static double *g_dot_partials = NULL;
+static long g_dot_cap = 0;
+
+extern "C" int csem_dot(long n, const void *a, const void *b, void *out, void *stream)
+{
+  long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
+  if (nch > g_dot_cap) {
+    if (g_dot_partials) hipFree(g_dot_partials);
+    g_dot_cap = 0;
+    HIP_OK(hipMalloc((void **)&g_dot_partials, nch * 4 * sizeof(double)));
+    g_dot_cap = nch;
+  }
+  return dot_launch(n, (const z2 *)a, (const z2 *)b, NULL, NULL, g_dot_partials, (double *)out, S(stream));
+}
+
+// ---------------------------------------------------------- BiCGStab
+
+// Device scalar slots (doubles)
+enum { SC_RHO = 0, SC_ALPHA = 2, SC_OMEGA = 4, SC_BETA = 6, SC_RS0 = 8, SC_RS = 9, SC_RELRES = 10,
+       SC_CONV = 11, SC_TOL = 12, SC_DOT = 16 };
+
+// r = b - v; p = r; r0 = r; partials of <r,r0>, <r,r> (solver.c:108-115)
+__global__ __launch_bounds__(RED_T) void k_bicg_init(long n, const z2 *b, const z2 *v, z2 *r, z2 *p, z2 *r0,
+                                                     double *partials)
+{
+  __shared__ double sh[RED_T];
+  double acc[4] = {0, 0, 0, 0};
+  const long base = (long)blockIdx.x * RED_CHUNK + threadIdx.x;
+  for (int mm = 0; mm < RED_E; mm++) {
+    long e = base + (long)mm * RED_T;
+    if (e < n) {
+      z2 rr = zsub(b[e], v[e]);
+      r[e] = rr; p[e] = rr; r0[e] = rr;
+      double c = rr.x, d = -rr.y;
+      acc[0] = acc[0] + (rr.x * c - rr.y * d);
+      acc[1] = acc[1] + (rr.x * d + rr.y * c);
+      acc[2] = acc[2] + (rr.x * c - rr.y * d);
+      acc[3] = acc[3] + (rr.x * d + rr.y * c);
+    }
+  }
+  chunk_fold_store<2>(acc, partials, sh);
+}
+
+__global__ __launch_bounds__(RED_T) void k_bicg_fin_init(const double *partials, long nch, double *sc)
+{
+  __shared__ double sh[RED_T];
+  final_fold<2>(partials, nch, sc + SC_DOT, sh);
+  if (threadIdx.x == 0) {
+    sc[SC_RHO] = sc[SC_DOT];
+    sc[SC_RHO + 1] = sc[SC_DOT + 1];
+    double rs = sc[SC_DOT + 2];
+    sc[SC_RS] = rs;
+    sc[SC_RS0] = rs;
+    sc[SC_RELRES] = sqrt(rs / rs);
+    sc[SC_CONV] = 0;
+  }
+}
+
+// alpha = rho_old / <v, r0> (solver.c:145-146)
+__global__ __launch_bounds__(RED_T) void k_bicg_fin_alpha(const double *partials, long nch, double *sc)
+{
+  __shared__ double sh[RED_T];
+  final_fold<1>(partials, nch, sc + SC_DOT, sh);
+  if (threadIdx.x == 0) {
+    z2 al = zdiv(zmk(sc[SC_RHO], sc[SC_RHO + 1]), zmk(sc[SC_DOT], sc[SC_DOT + 1]));
+    sc[SC_ALPHA] = al.x;
+    sc[SC_ALPHA + 1] = al.y;
+  }
+}
+
+// omega = <s,t> / Re<t,t> (solver.c:151)
+__global__ __launch_bounds__(RED_T) void k_bicg_fin_omega(const double *partials, long nch, double *sc)
+{
+  __shared__ double sh[RED_T];
+  final_fold<2>(partials, nch, sc + SC_DOT, sh);
+  if (threadIdx.x == 0) {
+    double tt = sc[SC_DOT + 2];
+    sc[SC_OMEGA] = sc[SC_DOT] / tt;
+    sc[SC_OMEGA + 1] = sc[SC_DOT + 1] / tt;
+  }
+}
+
+// relres = sqrt(Re<r,r>/rs0); beta = (rho_new/rho_old)*alpha/omega
+// (solver.c:157-168)
+__global__ __launch_bounds__(RED_T) void k_bicg_fin_conv(const double *partials, long nch, double *sc)
+{
+  __shared__ double sh[RED_T];
+  final_fold<2>(partials, nch, sc + SC_DOT, sh);
+  if (threadIdx.x == 0) {
+    double rs = sc[SC_DOT];
+    double relres = sqrt(rs / sc[SC_RS0]);
+    sc[SC_RS] = rs;
+    sc[SC_RELRES] = relres;
+    if (relres < sc[SC_TOL]) {
+      sc[SC_CONV] = 1;
+    } else {
+      z2 rho_new = zmk(sc[SC_DOT + 2], sc[SC_DOT + 3]);
+      z2 rho_old = zmk(sc[SC_RHO], sc[SC_RHO + 1]);
+      z2 al = zmk(sc[SC_ALPHA], sc[SC_ALPHA + 1]), om = zmk(sc[SC_OMEGA], sc[SC_OMEGA + 1]);
+      z2 be = zdiv(zmul(zdiv(rho_new, rho_old), al), om);
+      sc[SC_BETA] = be.x;
+      sc[SC_BETA + 1] = be.y;
+      sc[SC_RHO] = rho_new.x;
+      sc[SC_RHO + 1] = rho_new.y;
+    }
+  }
+}
+
+// s = r - alpha v (solver.c:147)
+__global__ void k_bicg_s(long n, const z2 *r, const z2 *v, z2 *s, const double *sc)
+{
+  long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  z2 al = zmk(sc[SC_ALPHA], sc[SC_ALPHA + 1]);
+  s[e] = zsub(r[e], zmul(al, v[e]));
+}
+
+// x += alpha xa + omega xb; r = s - omega t; partials of <r,r>, <r,r0>
+// (solver.c:153-164)
+__global__ __launch_bounds__(RED_T) void k_bicg_xr(long n, z2 *x, const z2 *xa, const z2 *xb, const z2 *s,
+                                                   const z2 *t, z2 *r, const z2 *r0, const double *sc,
+                                                   double *partials)
+{
+  __shared__ double sh[RED_T];
+  double acc[4] = {0, 0, 0, 0};
+  const z2 al = zmk(sc[SC_ALPHA], sc[SC_ALPHA + 1]), om = zmk(sc[SC_OMEGA], sc[SC_OMEGA + 1]);
+  const long base = (long)blockIdx.x * RED_CHUNK + threadIdx.x;
+  for (int mm = 0; mm < RED_E; mm++) {
+    long e = base + (long)mm * RED_T;
+    if (e < n) {
+      x[e] = zadd(x[e], zadd(zmul(al, xa[e]), zmul(om, xb[e])));
+      z2 rr = zsub(s[e], zmul(om, t[e]));
+      r[e] = rr;
+      double c = rr.x, d = -rr.y;
+      acc[0] = acc[0] + (rr.x * c - rr.y * d);
+      acc[1] = acc[1] + (rr.x * d + rr.y * c);
+      z2 w = r0[e];
+      c = w.x; d = -w.y;
+      acc[2] = acc[2] + (rr.x * c - rr.y * d);
+      acc[3] = acc[3] + (rr.x * d + rr.y * c);
+    }
+  }
+  chunk_fold_store<2>(acc, partials, sh);
+}
+
+// p = r + beta (p - omega v) (solver.c:166)
+__global__ void k_bicg_p(long n, z2 *p, const z2 *r, const z2 *v, const double *sc)
+{
+  long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  z2 be = zmk(sc[SC_BETA], sc[SC_BETA + 1]), om = zmk(sc[SC_OMEGA], sc[SC_OMEGA + 1]);
+  p[e] = zadd(r[e], zmul(be, zsub(p[e], zmul(om, v[e]))));
+}
+
+static int ensure_kv(csem_model *m, int nv)
+{
+  if (m->nkv >= nv) return 0;
+  if (m->kv) {
+    hipFree(m->kv);
+    m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void *)m->kv), m->allocs.end());
+    m->bytes -= m->nkv * m->lE * (long)sizeof(z2);
+    m->kv = NULL;
+    m->nkv = 0;
+  }
+  int rc = dalloc(m, &m->kv, (long)nv * m->lE);
+  if (rc) return rc;
+  m->nkv = nv;
+  return 0;
+}
+
+// Flexible right-preconditioned BiCGStab (solver.c:93-178) or, with
+// precond=0, BiCGStab (solver.c:16-87), all vectors and scalars resident.
+extern "C" int csem_bicgstab(csem_model *m, const csem_freq *f, void *xv, const void *bv, int niter,
+                             double tol, int precond, double *hist, void *stream)
+{
+  hipStream_t s = S(stream);
+  const long n = m->lE, nch = (n + RED_CHUNK - 1) / RED_CHUNK;
+  int rc = ensure_kv(m, 7);
+  if (rc) return rc;
+  z2 *x = (z2 *)xv;
+  const z2 *b = (const z2 *)bv;
+  z2 *r = m->kv, *r0 = r + n, *p = r0 + n, *pp = p + n, *v = pp + n, *sv = v + n, *t = sv + n;
+  double *sc = m->scal, *pa = m->partials;
+  const unsigned ge = (unsigned)((n + 255) / 256);
+  double hs[16];
+  HIP_OK(hipMemcpyAsync(sc + SC_TOL, &tol, sizeof(double), hipMemcpyHostToDevice, s));
+  if ((rc = run_apply(m, f, x, v, s))) return rc;
+  k_bicg_init<<<(unsigned)nch, RED_T, 0, s>>>(n, b, v, r, p, r0, pa);
+  k_bicg_fin_init<<<1, RED_T, 0, s>>>(pa, nch, sc);
+  HIP_OK(hipMemcpyAsync(hs, sc, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  if (hs[SC_RS] == 0.) return 0;
+  double relres = hs[SC_RELRES];
+  if (hist) hist[0] = relres;
+  int iter;
+  for (iter = 0; iter < niter; ++iter) {
+    if (hist) hist[iter] = relres;
+    const z2 *pin = p;
+    if (precond) {
+      if ((rc = run_lusgs(m, f, p, pp, s))) return rc;
+      pin = pp;
+    }
+    if ((rc = run_apply(m, f, pin, v, s))) return rc;
+    if ((rc = dot_partial_launch(n, v, r0, NULL, NULL, pa, s))) return rc;
+    k_bicg_fin_alpha<<<1, RED_T, 0, s>>>(pa, nch, sc);
+    k_bicg_s<<<ge, 256, 0, s>>>(n, r, v, sv, sc);
+    const z2 *xb;
+    if (precond) {
+      if ((rc = run_lusgs(m, f, sv, r, s))) return rc;
+      if ((rc = run_apply(m, f, r, t, s))) return rc;
+      if ((rc = dot_partial_launch(n, sv, t, t, t, pa, s))) return rc;
+      xb = r;
+    } else {
+      if ((rc = run_apply(m, f, sv, t, s))) return rc;
+      if ((rc = dot_partial_launch(n, t, sv, t, t, pa, s))) return rc;
+      xb = sv;
+    }
+    k_bicg_fin_omega<<<1, RED_T, 0, s>>>(pa, nch, sc);
+    k_bicg_xr<<<(unsigned)nch, RED_T, 0, s>>>(n, x, pin, xb, sv, t, r, r0, sc, pa);
+    k_bicg_fin_conv<<<1, RED_T, 0, s>>>(pa, nch, sc);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(hs, sc, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    relres = hs[SC_RELRES];
+    if (hs[SC_CONV] != 0) {
+      if (hist) hist[iter + 1] = relres;
+      break;
+    }
+    k_bicg_p<<<ge, 256, 0, s>>>(n, p, r, v, sc);
+  }
+  HIP_OK(hipGetLastError());
+  return iter;
+}
+
+// ------------------------------------------------- vector primitives
+
+__global__ void k_vec_sub(long n, const z2 *a, const z2 *b, z2 *o)
+{
+  long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e < n) o[e] = zsub(a[e], b[e]);
+}
+__global__ void k_vec_divr(long n, const z2 *a, double r, z2 *o)
+{
+  long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e < n) o[e] = zdivr(a[e], r);
+}
+__global__ void k_vec_divz(long n, const z2 *a, z2 q, z2 *o)
+{
+  long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e < n) o[e] = zdiv(a[e], q);
+}
+// y = y - h*x, h read from device memory (solver.c:447)
+__global__ void k_vec_subscaled_dev(long n, z2 *y, const double *h, const z2 *x)
+{
+  long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e < n) y[e] = zsub(y[e], zmul(zmk(h[0], h[1]), x[e]));
+}
+// y = y + c*x (solver.c:497)
+__global__ void k_vec_addscaled(long n, z2 *y, z2 c, const z2 *x)
+{
+  long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e < n) y[e] = zadd(y[e], zmul(c, x[e]));
+}
+__global__ void k_scatter_add(long nnz, const long *idx, const z2 *val, z2 *b)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (long q = 0; q < nnz; q++) b[idx[q]] = zadd(b[idx[q]], val[q]);
+}
+// one thread per receiver: s = 0; s += ((sig*x)*wa)*wb)*wc for the 8
+// corners in the order of inject_extract.c:151-158 / 199-206; then s*scale
+__global__ void k_gather(int nrec, const long *idx, const double *wts, const double *sig, const double *scale,
+                         const z2 *x, z2 *out)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  z2 s = zmk(0, 0);
+  for (int q = 0; q < 8; q++) {
+    long e = idx[8 * r + q];
+    double sg = sig[8 * r + q];
+    z2 v = zrmul(sg, x[e]);  // 1.0 for Ex/Ey, which multiply by nothing: exact
+    v = zmulr(zmulr(zmulr(v, wts[24 * r + 3 * q]), wts[24 * r + 3 * q + 1]), wts[24 * r + 3 * q + 2]);
+    s = zadd(s, v);
+  }
+  out[r] = zmulr(s, scale[r]);
+}
+
+#define GE(n) (unsigned)(((n) + 255) / 256), 256, 0, S(stream)
+
+extern "C" int csem_vec_sub(long n, const void *a, const void *b, void *o, void *stream)
+{
+  k_vec_sub<<<GE(n)>>>(n, (const z2 *)a, (const z2 *)b, (z2 *)o);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+extern "C" int csem_vec_divr(long n, const void *a, double r, void *o, void *stream)
+{
+  k_vec_divr<<<GE(n)>>>(n, (const z2 *)a, r, (z2 *)o);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+extern "C" int csem_vec_divz(long n, const void *a, double re, double im, void *o, void *stream)
+{
+  k_vec_divz<<<GE(n)>>>(n, (const z2 *)a, zmk(re, im), (z2 *)o);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+extern "C" int csem_vec_subscaled_dev(long n, void *y, const void *h, const void *x, void *stream)
+{
+  k_vec_subscaled_dev<<<GE(n)>>>(n, (z2 *)y, (const double *)h, (const z2 *)x);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+extern "C" int csem_vec_addscaled(long n, void *y, double re, double im, const void *x, void *stream)
+{
+  k_vec_addscaled<<<GE(n)>>>(n, (z2 *)y, zmk(re, im), (const z2 *)x);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int csem_scatter_add(long nnz, const long *idx, const double *val, void *b, void *stream)
+{
+  if (nnz <= 0) return 0;
+  long *di;
+  z2 *dv;
+  HIP_OK(hipMalloc((void **)&di, nnz * sizeof(long)));
+  HIP_OK(hipMalloc((void **)&dv, nnz * sizeof(z2)));
+  HIP_OK(hipMemcpyAsync(di, idx, nnz * sizeof(long), hipMemcpyHostToDevice, S(stream)));
+  HIP_OK(hipMemcpyAsync(dv, val, nnz * sizeof(z2), hipMemcpyHostToDevice, S(stream)));
+  k_scatter_add<<<1, 64, 0, S(stream)>>>(nnz, di, dv, (z2 *)b);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(S(stream)));
+  hipFree(di);
+  hipFree(dv);
+  return 0;
+}
+
+extern "C" int csem_gather(int nrec, const long *idx, const double *wts, const double *sig, const double *scale,
+                           const void *x, double *out, void *stream)
+{
+  if (nrec <= 0) return 0;
+  char *blob;
+  size_t bi = 8 * (size_t)nrec * sizeof(long), bw = 24 * (size_t)nrec * sizeof(double);
+  size_t bs = 8 * (size_t)nrec * sizeof(double), bc = (size_t)nrec * sizeof(double), bo = (size_t)nrec * sizeof(z2);
+  HIP_OK(hipMalloc((void **)&blob, bi + bw + bs + bc + bo));
+  hipStream_t s = S(stream);
+  HIP_OK(hipMemcpyAsync(blob, idx, bi, hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(blob + bi, wts, bw, hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(blob + bi + bw, sig, bs, hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(blob + bi + bw + bs, scale, bc, hipMemcpyHostToDevice, s));
+  k_gather<<<(nrec + 63) / 64, 64, 0, s>>>(nrec, (const long *)blob, (const double *)(blob + bi),
+                                           (const double *)(blob + bi + bw), (const double *)(blob + bi + bw + bs),
+                                           (const z2 *)x, (z2 *)(blob + bi + bw + bs + bc));
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(out, blob + bi + bw + bs + bc, bo, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  hipFree(blob);
+  return 0;
+}
+
+// ------------------------------------------------------ memory helpers
+
+extern "C" int csem_malloc(void **p, long bytes)
+{
+  HIP_OK(hipMalloc(p, bytes > 0 ? bytes : 256));
+  return 0;
+}
+extern "C" int csem_free(void *p)
+{
+  HIP_OK(hipFree(p));
+  return 0;
+}
+extern "C" int csem_memcpy_h2d(void *dst, const void *src, long bytes, void *stream)
+{
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(stream)));
+  HIP_OK(hipStreamSynchronize(S(stream)));
+  return 0;
+}
+extern "C" int csem_memcpy_d2h(void *dst, const void *src, long bytes, void *stream)
+{
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(stream)));
+  HIP_OK(hipStreamSynchronize(S(stream)));
+  return 0;
+}
+extern "C" int csem_memset(void *p, int v, long bytes, void *stream)
+{
+  HIP_OK(hipMemsetAsync(p, v, bytes, S(stream)));
+  return 0;
+}
+extern "C" int csem_sync(void *stream)
+{
+  HIP_OK(hipStreamSynchronize(S(stream)));
+  return 0;
+}

@@ -1,0 +1,670 @@
+/* csem_entry.c -- the reference's operator, preconditioner, airwave and
+ * Krylov entry points, running on the MI355X device library.
+ *
+ * fdfd_init builds the device model from the extended model (one active
+ * model per process, like the reference's static emf pointer, fdfd.c:23).
+ * fdfd_apply / lusgs_apply / airbc_apply / cdotprod take host pointers, as
+ * the reference's op_t calls do, and stage them through device buffers.
+ * fbicgstab / bicgstab / fgmres / gmres recognise the device operators
+ * (Aop == fdfd_apply, invMop == lusgs_apply) and then run the whole solve
+ * device-resident: one upload of b and x, one download of x.
+ * inject_source / extract_emf touch 8 grid values per source / receiver and
+ * keep the reference's host-array interface.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../../../include/csem.h"
+
+#define CSEM_PI 3.14159265358979323846264 /* cstd.h:35-37 */
+#define CSEM_MU0 (4. * CSEM_PI * 1e-7)    /* cstd.h:38-40 */
+
+void csem_set_status(int st);
+
+/* vector primitives exported by the device library */
+int csem_vec_sub(long n, const void *a, const void *b, void *o, void *stream);
+int csem_vec_divr(long n, const void *a, double r, void *o, void *stream);
+int csem_vec_divz(long n, const void *a, double re, double im, void *o, void *stream);
+int csem_vec_subscaled_dev(long n, void *y, const void *h, const void *x, void *stream);
+int csem_vec_addscaled(long n, void *y, double re, double im, const void *x, void *stream);
+
+static struct {
+  emf_t *emf;
+  csem_model *model;
+  void *dx, *dy, *dh, *dg1, *dg2, *dred;
+  long nvec;
+  int last_iters;
+  /* airwave host tables (airbc_init) */
+  double *twiddle;
+  int *jx, *jy, *gxj1, *gxj2, *gyj1, *gyj2;
+  double *wx, *wy, *gxw1, *gxw2, *gyw1, *gyw2;
+} G;
+
+csem_model *csem_active_model(void) { return G.model; }
+int csem_last_iterations(void) { return G.last_iters; }
+
+static void die_dev(const char *what, int rc)
+{
+  csem_set_status(rc);
+  err("%s failed (%d): %s", what, rc, csem_last_error());
+}
+
+#define DEV(call)                            \
+  do {                                       \
+    int rc_ = (call);                        \
+    if (rc_) {                               \
+      die_dev(#call, rc_);                   \
+      return;                                \
+    }                                        \
+  } while (0)
+
+static int is_pow2(int n) { return n >= 4 && (n & (n - 1)) == 0; }
+
+/* airbc_init_free.c:24-109: transform size, uniform spacing, the
+ * frequency-independent multipliers and the interpolation tables. */
+void airbc_init(emf_t *emf)
+{
+  int N, i1, i2, j1, j2, n1 = emf->n1, n2 = emf->n2;
+  double dkx, dky, kx, ky, kz, x1, x2, s1, s2;
+  if (!getparint("n1fft", &emf->n1fft)) emf->n1fft = 1024;
+  if (!getparint("n2fft", &emf->n2fft)) emf->n2fft = 1024;
+  if (emf->n1fft != emf->n2fft || !is_pow2(emf->n1fft) || emf->n1fft > 1024)
+    err("airwave transform must be square with a power-of-two size <= 1024 (n1fft=%d n2fft=%d)", emf->n1fft,
+        emf->n2fft);
+  N = emf->n1fft;
+  emf->d1fft = (emf->x1[n1] - emf->x1[0]) / (N - 1);
+  emf->d2fft = (emf->x2[n2] - emf->x2[0]) / (N - 1);
+  emf->d3fft = emf->x3node[1] - emf->x3node[0];
+  if (emf->verb) {
+    printf("[n1fft, n2fft]=[%d, %d]\n", emf->n1fft, emf->n2fft);
+    printf("[d1fft, d2fft, d3fft]=[%g, %g, %g]\n", emf->d1fft, emf->d2fft, emf->d3fft);
+  }
+  emf->sH1kxky = alloc2complex(N, N);
+  emf->sH2kxky = alloc2complex(N, N);
+  dkx = 2.0 * CSEM_PI / (emf->d1fft * N);
+  dky = 2.0 * CSEM_PI / (emf->d2fft * N);
+  for (i2 = 0; i2 < N; i2++) {
+    ky = (i2 <= N / 2) ? i2 * dky : (i2 - N) * dky;
+    for (i1 = 0; i1 < N; i1++) {
+      kx = (i1 <= N / 2) ? i1 * dkx : (i1 - N) * dkx;
+      kz = sqrt(kx * kx + ky * ky);
+      /* upward continuation by half a cell and d/dx, d/dy of the
+       * quasi-static air field, shifted half a cell for the staggering */
+      emf->sH1kxky[i2][i1] = exp(-kz * 0.5 * emf->d3fft) * I * kx / (kz + 1.e-15);
+      emf->sH2kxky[i2][i1] = exp(-kz * 0.5 * emf->d3fft) * I * ky / (kz + 1.e-15);
+      emf->sH1kxky[i2][i1] *= cexp(-I * kx * 0.5 * emf->d1s[0]);
+      emf->sH2kxky[i2][i1] *= cexp(-I * ky * 0.5 * emf->d2s[0]);
+    }
+  }
+  G.twiddle = (double *)malloc(2 * N * sizeof(double));
+  for (i1 = 0; i1 < N; i1++) {
+    double a = -2.0 * CSEM_PI * (double)i1 / (double)N;
+    G.twiddle[2 * i1] = cos(a);
+    G.twiddle[2 * i1 + 1] = sin(a);
+  }
+  G.jx = alloc1int(N); G.jy = alloc1int(N); G.wx = alloc1double(N); G.wy = alloc1double(N);
+  for (i1 = 0; i1 < N; i1++) {
+    x1 = emf->x1s[0] + i1 * emf->d1fft;
+    G.jx[i1] = -1;
+    G.wx[i1] = 0;
+    if (x1 < emf->x1s[n1 - 1]) {
+      j1 = find_index(n1 + 1, emf->x1s, x1);
+      G.jx[i1] = j1;
+      G.wx[i1] = (x1 - emf->x1s[j1]) / (emf->x1s[j1 + 1] - emf->x1s[j1]);
+    }
+  }
+  for (i2 = 0; i2 < N; i2++) {
+    x2 = emf->x2s[0] + i2 * emf->d2fft;
+    G.jy[i2] = -1;
+    G.wy[i2] = 0;
+    if (x2 < emf->x2s[n2 - 1]) {
+      j2 = find_index(n2 + 1, emf->x2s, x2);
+      G.jy[i2] = j2;
+      G.wy[i2] = (x2 - emf->x2s[j2]) / (emf->x2s[j2 + 1] - emf->x2s[j2]);
+    }
+  }
+  /* back-interpolation cells (airbc_init_free.c:168-175, 195-202) */
+  G.gxj1 = alloc1int(n1); G.gxw1 = alloc1double(n1); G.gxj2 = alloc1int(n2); G.gxw2 = alloc1double(n2);
+  G.gyj1 = alloc1int(n1); G.gyw1 = alloc1double(n1); G.gyj2 = alloc1int(n2); G.gyw2 = alloc1double(n2);
+  for (i2 = 0; i2 < n2; i2++) {
+    s2 = (emf->x2s[i2] - emf->x2s[0]);
+    G.gxj2[i2] = s2 / emf->d2fft;
+    G.gxw2[i2] = s2 / emf->d2fft - G.gxj2[i2];
+    s2 = (emf->x2[i2] - emf->x2[0]);
+    G.gyj2[i2] = s2 / emf->d2fft;
+    G.gyw2[i2] = s2 / emf->d2fft - G.gyj2[i2];
+  }
+  for (i1 = 0; i1 < n1; i1++) {
+    s1 = (emf->x1[i1] - emf->x1[0]);
+    G.gxj1[i1] = s1 / emf->d1fft;
+    G.gxw1[i1] = s1 / emf->d1fft - G.gxj1[i1];
+    s1 = (emf->x1s[i1] - emf->x1s[0]);
+    G.gyj1[i1] = s1 / emf->d1fft;
+    G.gyw1[i1] = s1 / emf->d1fft - G.gyj1[i1];
+  }
+}
+
+void airbc_free(emf_t *emf)
+{
+  free2complex(emf->sH1kxky);
+  free2complex(emf->sH2kxky);
+  free(G.twiddle);
+  free1int(G.jx); free1int(G.jy); free1double(G.wx); free1double(G.wy);
+  free1int(G.gxj1); free1int(G.gxj2); free1int(G.gyj1); free1int(G.gyj2);
+  free1double(G.gxw1); free1double(G.gxw2); free1double(G.gyw1); free1double(G.gyw2);
+  G.twiddle = NULL;
+}
+
+static void ensure_staging(long n)
+{
+  if (G.nvec >= n) return;
+  if (G.dx) csem_free(G.dx);
+  if (G.dy) csem_free(G.dy);
+  G.dx = G.dy = NULL;
+  if (csem_malloc(&G.dx, n * 16) || csem_malloc(&G.dy, n * 16)) {
+    die_dev("staging allocation", -2);
+    return;
+  }
+  G.nvec = n;
+}
+
+/* Host half of fdfd_init (fdfd.c:30-63): operator truncation under the
+ * airwave boundary, the airwave tables and the vector lengths. */
+void csem_fdfd_prepare(emf_t *emf)
+{
+  G.emf = emf;
+  if (emf->preco == 3) err("preco=3 (A-phi preconditioner) is not provided by this library");
+  if (emf->airbc) {
+    emf->xx = emf->yy = NULL; /* LU-SGS scratch lives on the device */
+    emf->n3 = emf->nz + emf->nb;
+    airbc_init(emf);
+  }
+  emf->lEx = emf->n1 * (emf->n2 + 1) * (emf->n3 + 1);
+  emf->lEy = (emf->n1 + 1) * emf->n2 * (emf->n3 + 1);
+  emf->lEz = (emf->n1 + 1) * (emf->n2 + 1) * emf->n3;
+  emf->lE = emf->lEx + emf->lEy + emf->lEz;
+  emf->lHx = (emf->n1 + 1) * emf->n2 * emf->n3;
+  emf->lHy = emf->n1 * (emf->n2 + 1) * emf->n2; /* fdfd.c:59 sizes the k extent with n2 */
+  emf->lHz = emf->n1 * emf->n2 * (emf->n3 + 1);
+  emf->lH = emf->lHx + emf->lHy + emf->lHz;
+  emf->lv = (emf->n1 + 1) * (emf->n2 + 1) * (emf->n3 + 1);
+}
+
+/* Device half: the model the device library solves on, built from the
+ * extended model and the airwave tables csem_fdfd_prepare made. */
+int csem_model_from_emf(const emf_t *emf, csem_model **out)
+{
+  csem_model_desc d;
+  memset(&d, 0, sizeof d);
+  d.n1 = emf->n1; d.n2 = emf->n2; d.n3f = emf->nz + 2 * emf->nb; d.nb = emf->nb;
+  d.airbc = emf->airbc; d.fvm = emf->fvm;
+  d.d1 = emf->d1; d.d2 = emf->d2; d.d3 = emf->d3; d.d1s = emf->d1s; d.d2s = emf->d2s; d.d3s = emf->d3s;
+  d.sigma11 = &emf->sigma11[0][0][0];
+  d.sigma22 = &emf->sigma22[0][0][0];
+  d.sigma33 = &emf->sigma33[0][0][0];
+  if (emf->airbc) {
+    d.nfft = emf->n1fft;
+    d.sH1 = (const double *)&emf->sH1kxky[0][0];
+    d.sH2 = (const double *)&emf->sH2kxky[0][0];
+    d.twiddle = G.twiddle;
+    d.jx = G.jx; d.jy = G.jy; d.wx = G.wx; d.wy = G.wy;
+    d.gxj1 = G.gxj1; d.gxj2 = G.gxj2; d.gxw1 = G.gxw1; d.gxw2 = G.gxw2;
+    d.gyj1 = G.gyj1; d.gyj2 = G.gyj2; d.gyw1 = G.gyw1; d.gyw2 = G.gyw2;
+  }
+  return csem_model_create(&d, out);
+}
+
+/* fdfd.c:30-63 */
+void fdfd_init(emf_t *emf)
+{
+  csem_fdfd_prepare(emf);
+  if (G.model) csem_model_destroy(G.model);
+  G.model = NULL;
+  DEV(csem_model_from_emf(emf, &G.model));
+  csem_set_status(0);
+}
+
+void fdfd_free(emf_t *emf)
+{
+  if (emf->airbc) airbc_free(emf);
+  if (G.model) csem_model_destroy(G.model);
+  G.model = NULL;
+  if (G.dx) csem_free(G.dx);
+  if (G.dy) csem_free(G.dy);
+  if (G.dh) csem_free(G.dh);
+  if (G.dg1) csem_free(G.dg1);
+  if (G.dg2) csem_free(G.dg2);
+  if (G.dred) csem_free(G.dred);
+  G.dx = G.dy = G.dh = G.dg1 = G.dg2 = G.dred = NULL;
+  G.nvec = 0;
+}
+
+/* iomu = emf->I_omega_mu0 as the caller set it (main.c:95), iwmu = 1/iomu
+ * by C complex division as fdfd.c:100 forms it. */
+static csem_freq freq_of(const emf_t *emf)
+{
+  csem_freq f;
+  csem_complex iwmu = 1. / emf->I_omega_mu0;
+  f.iomu_re = creal(emf->I_omega_mu0);
+  f.iomu_im = cimag(emf->I_omega_mu0);
+  f.iwmu_re = creal(iwmu);
+  f.iwmu_im = cimag(iwmu);
+  return f;
+}
+
+static int need_model(const char *who)
+{
+  if (!G.model || !G.emf) {
+    csem_set_status(-4);
+    err("%s called before fdfd_init", who);
+    return 0;
+  }
+  return 1;
+}
+
+void fdfd_apply(int n, csem_complex *x, csem_complex *y)
+{
+  if (!need_model("fdfd_apply")) return;
+  csem_freq f = freq_of(G.emf);
+  ensure_staging(n);
+  DEV(csem_memcpy_h2d(G.dx, x, (long)n * 16, NULL));
+  DEV(csem_apply(G.model, &f, G.dx, G.dy, NULL));
+  DEV(csem_memcpy_d2h(y, G.dy, (long)n * 16, NULL));
+  csem_set_status(0);
+}
+
+void lusgs_apply(int n, csem_complex *x, csem_complex *y)
+{
+  if (!need_model("lusgs_apply")) return;
+  csem_freq f = freq_of(G.emf);
+  ensure_staging(n);
+  DEV(csem_memcpy_h2d(G.dx, x, (long)n * 16, NULL));
+  DEV(csem_lusgs(G.model, &f, G.dx, G.dy, NULL));
+  DEV(csem_memcpy_d2h(y, G.dy, (long)n * 16, NULL));
+  csem_set_status(0);
+}
+
+void airbc_apply(emf_t *emf, csem_complex *H3k, csem_complex *H1km1, csem_complex *H2km1)
+{
+  long nz = (long)emf->n1 * emf->n2, nx = (long)(emf->n1 + 1) * emf->n2, ny = (long)emf->n1 * (emf->n2 + 1);
+  if (!need_model("airbc_apply")) return;
+  if (!G.dh) DEV(csem_malloc(&G.dh, nz * 16));
+  if (!G.dg1) DEV(csem_malloc(&G.dg1, nx * 16));
+  if (!G.dg2) DEV(csem_malloc(&G.dg2, ny * 16));
+  DEV(csem_memcpy_h2d(G.dh, H3k, nz * 16, NULL));
+  DEV(csem_airbc(G.model, G.dh, G.dg1, G.dg2, NULL));
+  DEV(csem_memcpy_d2h(H1km1, G.dg1, nx * 16, NULL));
+  DEV(csem_memcpy_d2h(H2km1, G.dg2, ny * 16, NULL));
+  csem_set_status(0);
+}
+
+/* sum a_i conj(b_i); the device sums in a fixed chunked order, so the
+ * last bits can differ from a serial left-to-right sum */
+csem_complex cdotprod(int n, csem_complex *a, csem_complex *b)
+{
+  double out[2] = {0, 0};
+  ensure_staging(n);
+  if (!G.dred && csem_malloc(&G.dred, 256)) {
+    die_dev("cdotprod", -2);
+    return 0;
+  }
+  if (csem_memcpy_h2d(G.dx, a, (long)n * 16, NULL) || csem_memcpy_h2d(G.dy, b, (long)n * 16, NULL) ||
+      csem_dot(n, G.dx, G.dy, G.dred, NULL) || csem_memcpy_d2h(out, G.dred, 16, NULL)) {
+    die_dev("cdotprod", -1);
+    return 0;
+  }
+  csem_set_status(0);
+  return out[0] + I * out[1];
+}
+
+/* iterate_*.txt and stdout lines of solver.c:40-54,127-141 */
+static void log_history(const char *file, const char *tag, const double *hist, int nrow, int conv_iter)
+{
+  FILE *fp = fopen(file, "w");
+  int it;
+  if (fp) {
+    fprintf(fp, "iteration \t relres\n");
+    for (it = 0; it < nrow; it++) fprintf(fp, "%d \t %e\n", it, hist[it]);
+    fclose(fp);
+  }
+  for (it = 0; it < nrow; it++) printf("%s iter=%d relres=%e\n", tag, it, hist[it]);
+  if (conv_iter >= 0) printf("converged at iter=%d\n", conv_iter);
+}
+
+static void device_bicgstab(int n, csem_complex *x, csem_complex *b, int niter, double tol, int verb,
+                            int precond)
+{
+  void *dx = NULL, *db = NULL;
+  double *hist = (double *)calloc(niter + 2, sizeof(double));
+  csem_freq f = freq_of(G.emf);
+  int it;
+  if (csem_malloc(&dx, (long)n * 16) || csem_malloc(&db, (long)n * 16)) {
+    free(hist);
+    die_dev("solver allocation", -2);
+    return;
+  }
+  if (csem_memcpy_h2d(dx, x, (long)n * 16, NULL) || csem_memcpy_h2d(db, b, (long)n * 16, NULL)) {
+    die_dev("solver upload", -1);
+    goto out;
+  }
+  it = csem_bicgstab(G.model, &f, dx, db, niter, tol, precond, hist, NULL);
+  if (it < 0) {
+    die_dev("csem_bicgstab", it);
+    goto out;
+  }
+  if (csem_memcpy_d2h(x, dx, (long)n * 16, NULL)) {
+    die_dev("solver download", -1);
+    goto out;
+  }
+  G.last_iters = it;
+  if (verb)
+    log_history(precond ? "iterate_fbicgstab.txt" : "iterate_bicgstab.txt", precond ? "f-bicgstab" : "bicgstab",
+                hist, (it < niter ? it + 1 : niter), it < niter ? it : -1);
+  csem_set_status(0);
+out:
+  csem_free(dx);
+  csem_free(db);
+  free(hist);
+}
+
+void fbicgstab(int n, csem_complex *x, csem_complex *b, op_t Aop, op_t invMop, int niter, double tol, int verb)
+{
+  if (!need_model("fbicgstab")) return;
+  if (Aop != fdfd_apply || invMop != lusgs_apply) {
+    csem_set_status(-5);
+    err("fbicgstab: only the device operators (fdfd_apply, lusgs_apply) are provided");
+    return;
+  }
+  device_bicgstab(n, x, b, niter, tol, verb, 1);
+}
+
+void bicgstab(int n, csem_complex *x, csem_complex *b, op_t Aop, int niter, double tol, int verb)
+{
+  if (!need_model("bicgstab")) return;
+  if (Aop != fdfd_apply) {
+    csem_set_status(-5);
+    err("bicgstab: only the device operator fdfd_apply is provided");
+    return;
+  }
+  device_bicgstab(n, x, b, niter, tol, verb, 0);
+}
+
+/* Restarted (flexible) GMRES with the vectors on the device and the small
+ * Hessenberg / Givens recurrence on the host, in the arithmetic of
+ * solver.c:403-512 (fgmres) and solver.c:181-285 (gmres). */
+static int gmres_core(csem_model *model, const csem_freq *fq, void *x, const void *b, int nouter, double tol,
+                      int m, int precond, int flexible, double *hist)
+{
+  const long n = csem_model_len(model), nb = n * 16;
+  const csem_freq f = *fq;
+  void *w = NULL, *r = NULL, *V = NULL, *Z = NULL, *hd = NULL;
+  csem_complex *h = (csem_complex *)calloc((size_t)(m + 1) * m, sizeof(csem_complex));
+  csem_complex *g = (csem_complex *)calloc(m + 1, sizeof(csem_complex));
+  csem_complex *y = (csem_complex *)calloc(m + 1, sizeof(csem_complex));
+  csem_complex *c = (csem_complex *)calloc(m + 1, sizeof(csem_complex));
+  double *s = (double *)calloc(m + 1, sizeof(double));
+  double dv[2 * 64 + 2];
+  double beta, tmp, r0 = 1, relres;
+  csem_complex ss;
+  int i, j, iter, mm, conv = 0, steps = 0, rc = 0, mcur = m, early = 0;
+  if (m > 64 || m < 1) {
+    rc = -3;
+    goto out;
+  }
+#define CK(call)       \
+  if ((rc = (call))) { \
+    goto out;          \
+  }
+  CK(csem_malloc(&w, nb)); CK(csem_malloc(&r, nb));
+  CK(csem_malloc(&V, nb * (m + 1)));
+  if (precond) CK(csem_malloc(&Z, nb * m));
+  CK(csem_malloc(&hd, (long)(2 * 64 + 2) * sizeof(double)));
+#define VJ(k) ((char *)V + (long)(k) * nb)
+#define ZJ(k) (precond ? (char *)Z + (long)(k) * nb : VJ(k))
+/* storage element (column j, row i) of the Hessenberg: fgmres keeps
+ * h[j][i] (solver.c:412), gmres h[i][j] (solver.c:190) -- same numbers */
+#define HH(row, col) h[(long)(col) * (m + 1) + (row)]
+  for (iter = 0; iter < nouter; iter++) {
+    CK(csem_apply(model, &f, x, w, NULL));
+    CK(csem_vec_sub(n, b, w, r, NULL));
+    CK(csem_dot(n, r, r, hd, NULL));
+    CK(csem_memcpy_d2h(dv, hd, 16, NULL));
+    beta = sqrt(dv[0]);
+    if (beta == 0.) break;
+    CK(csem_vec_divr(n, r, beta, VJ(0), NULL));
+    memset(g, 0, (m + 1) * sizeof(csem_complex));
+    g[0] = beta;
+    memset(h, 0, (size_t)(m + 1) * m * sizeof(csem_complex));
+    if (iter == 0) {
+      r0 = beta;
+      if (hist) hist[0] = flexible ? 1.0 : beta / r0;
+    }
+    for (j = 0; j < mcur; j++) {
+      if (precond) CK(csem_lusgs(model, &f, VJ(j), ZJ(j), NULL));
+      CK(csem_apply(model, &f, ZJ(j), w, NULL));
+      for (i = 0; i <= j; i++) { /* modified Gram-Schmidt, solver.c:445-448 */
+        CK(csem_dot(n, w, VJ(i), (char *)hd + 16 * i, NULL));
+        CK(csem_vec_subscaled_dev(n, w, (char *)hd + 16 * i, VJ(i), NULL));
+      }
+      CK(csem_dot(n, w, w, (char *)hd + 16 * (j + 1), NULL));
+      CK(csem_memcpy_d2h(dv, hd, 16 * (j + 2), NULL));
+      for (i = 0; i <= j; i++) HH(i, j) = dv[2 * i] + I * dv[2 * i + 1];
+      HH(j + 1, j) = sqrt(dv[2 * (j + 1)]);
+      if (cabs(HH(j + 1, j)) == 0.) {
+        if (flexible) conv = 1;
+        else mcur = j + 1;
+        break;
+      }
+      CK(csem_vec_divz(n, w, creal(HH(j + 1, j)), cimag(HH(j + 1, j)), VJ(j + 1), NULL));
+      for (i = 0; i < j; i++) {
+        ss = conj(c[i]) * HH(i, j) + s[i] * HH(i + 1, j);
+        HH(i + 1, j) = -s[i] * HH(i, j) + c[i] * HH(i + 1, j);
+        HH(i, j) = ss;
+      }
+      tmp = sqrt(creal(HH(j, j) * conj(HH(j, j)) + HH(j + 1, j) * conj(HH(j + 1, j))));
+      s[j] = creal(HH(j + 1, j)) / tmp;
+      c[j] = HH(j, j) / tmp;
+      if (flexible) {
+        g[j + 1] = -s[j] * g[j];
+        g[j] = conj(c[j]) * g[j];
+        HH(j, j) = conj(c[j]) * HH(j, j) + s[j] * HH(j + 1, j);
+      } else {
+        HH(j, j) = conj(c[j]) * HH(j, j) + s[j] * HH(j + 1, j);
+        HH(j + 1, j) = 0.;
+        g[j + 1] = -s[j] * g[j];
+        g[j] = conj(c[j]) * g[j];
+      }
+      relres = cabs(g[j + 1]) / r0;
+      steps = iter * mcur + j + 1;
+      if (hist) hist[steps] = relres;
+      if (relres < tol) {
+        conv = 1;
+        if (!flexible) early = 1; /* solver.c:251 returns before updating x */
+        break;
+      }
+    }
+    if (early) break;
+    mm = flexible ? (mcur < j + 1 ? mcur : j + 1) : mcur;
+    y[mm - 1] = g[mm - 1] / HH(mm - 1, mm - 1);
+    for (i = mm - 2; i >= 0; i--) {
+      y[i] = g[i];
+      for (j = i + 1; j < mm; j++) y[i] -= HH(i, j) * y[j];
+      y[i] /= HH(i, i);
+    }
+    for (j = 0; j < mm; j++) CK(csem_vec_addscaled(n, x, creal(y[j]), cimag(y[j]), ZJ(j), NULL));
+    if (conv && flexible) break;
+  }
+  rc = steps;
+out:
+#undef CK
+#undef VJ
+#undef ZJ
+#undef HH
+  if (w) csem_free(w);
+  if (r) csem_free(r);
+  if (V) csem_free(V);
+  if (Z) csem_free(Z);
+  if (hd) csem_free(hd);
+  free(h); free(g); free(y); free(c); free(s);
+  return rc;
+}
+
+int csem_fgmres(csem_model *m, const csem_freq *f, void *x, const void *b, int nouter, double tol, int mrestart,
+                int precond, double *hist, void *stream)
+{
+  (void)stream;
+  int rc = gmres_core(m, f, x, b, nouter, tol, mrestart, precond, 1, hist);
+  if (rc == -3) return csem_fail(-3, "restart length %d outside [1, 64]", mrestart);
+  return rc;
+}
+
+int csem_gmres(csem_model *m, const csem_freq *f, void *x, const void *b, int nouter, double tol, int mrestart,
+               double *hist, void *stream)
+{
+  (void)stream;
+  int rc = gmres_core(m, f, x, b, nouter, tol, mrestart, 0, 0, hist);
+  if (rc == -3) return csem_fail(-3, "restart length %d outside [1, 64]", mrestart);
+  return rc;
+}
+
+static void host_gmres(int n, csem_complex *xh, csem_complex *bh, int nouter, double tol, int m, int verb,
+                       int flexible)
+{
+  void *x = NULL, *b = NULL;
+  double *hist = (double *)calloc((size_t)nouter * m + 2, sizeof(double));
+  csem_freq f = freq_of(G.emf);
+  int steps, i;
+  if (csem_malloc(&x, (long)n * 16) || csem_malloc(&b, (long)n * 16) ||
+      csem_memcpy_h2d(x, xh, (long)n * 16, NULL) || csem_memcpy_h2d(b, bh, (long)n * 16, NULL)) {
+    die_dev("gmres staging", -2);
+    goto out;
+  }
+  steps = flexible ? csem_fgmres(G.model, &f, x, b, nouter, tol, m, 1, hist, NULL)
+                   : csem_gmres(G.model, &f, x, b, nouter, tol, m, hist, NULL);
+  if (steps < 0) {
+    die_dev(flexible ? "csem_fgmres" : "csem_gmres", steps);
+    goto out;
+  }
+  if (csem_memcpy_d2h(xh, x, (long)n * 16, NULL)) {
+    die_dev("gmres download", -1);
+    goto out;
+  }
+  G.last_iters = steps;
+  if (verb) {
+    const char *file = flexible ? "iterate_fgmres.txt" : "iterate_gmres.txt";
+    FILE *fp = fopen(file, "w");
+    if (fp) {
+      fprintf(fp, "iteration \t error\n");
+      for (i = 0; i <= steps; i++) fprintf(fp, "%d \t %e\n", i, hist[i]);
+      fclose(fp);
+    }
+    for (i = 0; i <= steps; i++) printf("%s iter=%d relres=%e\n", flexible ? "f-gmres" : "gmres", i, hist[i]);
+  }
+  csem_set_status(0);
+out:
+  if (x) csem_free(x);
+  if (b) csem_free(b);
+  free(hist);
+}
+
+void fgmres(int n, csem_complex *x, csem_complex *b, op_t Aop, op_t invMop, int niter, double tol, int m, int verb)
+{
+  if (!need_model("fgmres")) return;
+  if (Aop != fdfd_apply || invMop != lusgs_apply) {
+    csem_set_status(-5);
+    err("fgmres: only the device operators (fdfd_apply, lusgs_apply) are provided");
+    return;
+  }
+  host_gmres(n, x, b, niter, tol, m, verb, 1);
+}
+
+void gmres(int n, csem_complex *x, csem_complex *b, op_t Aop, int niter, double tol, int m, int verb)
+{
+  if (!need_model("gmres")) return;
+  if (Aop != fdfd_apply) {
+    csem_set_status(-5);
+    err("gmres: only the device operator fdfd_apply is provided");
+    return;
+  }
+  host_gmres(n, x, b, niter, tol, m, verb, 0);
+}
+
+/* ---------------------------------------------------------------- sources */
+
+#define I1(k, j, i) ((i) + (long)emf->n1 * ((j) + (long)(emf->n2 + 1) * (k)))
+#define I2(k, j, i) ((i) + (long)(emf->n1 + 1) * ((j) + (long)emf->n2 * (k)))
+#define I3(k, j, i) ((i) + (long)(emf->n1 + 1) * ((j) + (long)(emf->n2 + 1) * (k)))
+
+/* inject_extract.c:24-120.  With the airwave boundary the plane index is
+ * shifted by nb before the conductivity lookup as well (the reference
+ * reads sigma nb planes above the source cell, inject_extract.c:55-63). */
+void inject_source(acq_t *acq, emf_t *emf, csem_complex *b)
+{
+  const int n1 = emf->nx + 2 * emf->nb, n2 = emf->ny + 2 * emf->nb, n3 = emf->nz + 2 * emf->nb;
+  int ic, is, i, j, k, q;
+  double w1, w2, w3, vol;
+  csem_complex sv, *bx = b, *by = b + emf->lEx, *bz = b + emf->lEx + emf->lEy;
+  for (ic = 0; ic < emf->nchsrc; ic++) {
+    int ch = !strcmp(emf->chsrc[ic], "Ex") ? 0 : !strcmp(emf->chsrc[ic], "Ey") ? 1 : !strcmp(emf->chsrc[ic], "Ez") ? 2 : -1;
+    if (ch < 0) continue;
+    for (is = 0; is < acq->nsrc; is++) {
+      const double *xa = ch == 0 ? emf->x1s : emf->x1, *ya = ch == 1 ? emf->x2s : emf->x2,
+                   *za = ch == 2 ? emf->x3s : emf->x3;
+      i = find_index(ch == 0 ? n1 : n1 + 1, (double *)xa, acq->src_x1[is]);
+      j = find_index(ch == 1 ? n2 : n2 + 1, (double *)ya, acq->src_x2[is]);
+      k = find_index(ch == 2 ? n3 : n3 + 1, (double *)za, acq->src_x3[is]);
+      w1 = (acq->src_x1[is] - xa[i]) / (xa[i + 1] - xa[i]);
+      w2 = (acq->src_x2[is] - ya[j]) / (ya[j + 1] - ya[j]);
+      w3 = (acq->src_x3[is] - za[k]) / (za[k + 1] - za[k]);
+      vol = (xa[i + 1] - xa[i]) * (ya[j + 1] - ya[j]) * (za[k + 1] - za[k]);
+      sv = 1. / vol;
+      if (emf->fvm) sv = emf->I_omega_mu0;
+      if (ch != 2) sv /= emf->rho_water;
+      if (emf->airbc) k -= emf->nb;
+      for (q = 0; q < 8; q++) {
+        int di = q & 1, dj = (q >> 1) & 1, dk = q >> 2;
+        double f1 = di ? w1 : (1. - w1), f2 = dj ? w2 : (1. - w2), f3 = dk ? w3 : (1. - w3);
+        if (ch == 0) bx[I1(k + dk, j + dj, i + di)] += sv / emf->sigma11[k + dk][j + dj][i + di] * f1 * f2 * f3;
+        else if (ch == 1) by[I2(k + dk, j + dj, i + di)] += sv / emf->sigma22[k + dk][j + dj][i + di] * f1 * f2 * f3;
+        else bz[I3(k + dk, j + dj, i + di)] += sv * f1 * f2 * f3;
+      }
+    }
+  }
+}
+
+/* inject_extract.c:123-213: trilinear Ex, Ey; for Ez the vertical current
+ * sigma33*Ez is interpolated (at the unshifted plane index) and scaled by
+ * rho_water.  H channels are left as they are (the reference has none). */
+void extract_emf(acq_t *acq, emf_t *emf, csem_complex *x, int ifreq)
+{
+  const int n1 = emf->nx + 2 * emf->nb, n2 = emf->ny + 2 * emf->nb, n3 = emf->nz + 2 * emf->nb;
+  int ic, ir, i, j, k, kk, q;
+  double w1, w2, w3;
+  csem_complex s, *Ex = x, *Ey = x + emf->lEx, *Ez = x + emf->lEx + emf->lEy;
+  for (ic = 0; ic < emf->nchrec; ic++) {
+    int ch = !strcmp(emf->chrec[ic], "Ex") ? 0 : !strcmp(emf->chrec[ic], "Ey") ? 1 : !strcmp(emf->chrec[ic], "Ez") ? 2 : -1;
+    if (ch < 0) continue;
+    for (ir = 0; ir < acq->nrec; ir++) {
+      const double *xa = ch == 0 ? emf->x1s : emf->x1, *ya = ch == 1 ? emf->x2s : emf->x2,
+                   *za = ch == 2 ? emf->x3s : emf->x3;
+      i = find_index(ch == 0 ? n1 : n1 + 1, (double *)xa, acq->rec_x1[ir]);
+      j = find_index(ch == 1 ? n2 : n2 + 1, (double *)ya, acq->rec_x2[ir]);
+      k = find_index(ch == 2 ? n3 : n3 + 1, (double *)za, acq->rec_x3[ir]);
+      w1 = (acq->rec_x1[ir] - xa[i]) / (xa[i + 1] - xa[i]);
+      w2 = (acq->rec_x2[ir] - ya[j]) / (ya[j + 1] - ya[j]);
+      w3 = (acq->rec_x3[ir] - za[k]) / (za[k + 1] - za[k]);
+      kk = k;
+      if (emf->airbc) k -= emf->nb;
+      s = 0;
+      for (q = 0; q < 8; q++) {
+        int di = q & 1, dj = (q >> 1) & 1, dk = q >> 2;
+        double f1 = di ? w1 : (1. - w1), f2 = dj ? w2 : (1. - w2), f3 = dk ? w3 : (1. - w3);
+        if (ch == 0) s += Ex[I1(k + dk, j + dj, i + di)] * f1 * f2 * f3;
+        else if (ch == 1) s += Ey[I2(k + dk, j + dj, i + di)] * f1 * f2 * f3;
+        else s += emf->sigma33[kk + dk][j + dj][i + di] * Ez[I3(k + dk, j + dj, i + di)] * f1 * f2 * f3;
+      }
+      emf->dcal_fd[ic][ifreq][ir] = (ch == 2) ? s * emf->rho_water : s;
+    }
+  }
+}

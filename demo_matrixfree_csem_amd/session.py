@@ -1,0 +1,179 @@
+"""Host mirror of the reference's driver (main.c:46-145) over libcsem.so.
+
+`Session(argv)` runs the reference's setup sequence -- initargs, emf_init,
+acq_init, extend_model_init, fdfd_init -- with the same key=value flags, so
+a test or a tool can then call the reference entry points (fdfd_apply,
+lusgs_apply, fbicgstab, ...) on numpy host arrays, or the device C-ABI on
+the model fdfd_init built.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import EmfT, AcqT, FreqT, lib, check
+
+
+def freq_consts(freq):
+    f = FreqT()
+    lib().csem_freq_consts(C.c_float(freq), C.byref(f))
+    return f
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+class Session:
+    """One extended model, as main.c builds it (one per process)."""
+
+    def __init__(self, argv, verb=0, device=True):
+        L = lib()
+        L.csem_set_exit_on_error(0)
+        args = [b"csem"] + [a.encode() if isinstance(a, str) else a for a in argv]
+        self._argv = (C.c_char_p * len(args))(*args)
+        L.initargs(len(args), self._argv)
+        self.emf, self.acq = EmfT(), AcqT()
+        e = self.emf
+        flags = dict(a.split("=", 1) for a in argv if "=" in a)
+        # main.c:62-69 defaults
+        e.mode = int(flags.get("mode", 0))
+        e.verb = int(flags.get("verb", verb))
+        e.niter = int(flags.get("niter", 1000))
+        e.preco = int(flags.get("preco", 1))
+        e.algopt = int(flags.get("algopt", 1))
+        e.tol = float(np.float32(float(flags.get("tol", 1e-8))))
+        e.nrestart = int(flags.get("nrestart", 10))
+        e.mp = int(flags.get("mp", 30))
+        for step, fn in (("emf_init", lambda: L.emf_init(C.byref(e))),
+                         ("acq_init", lambda: L.acq_init(C.byref(self.acq), C.byref(e))),
+                         ("extend_model_init", lambda: L.extend_model_init(C.byref(self.acq), C.byref(e))),
+                         ("fdfd_init", (lambda: L.fdfd_init(C.byref(e))) if device
+                          else (lambda: L.csem_fdfd_prepare(C.byref(e))))):
+            fn()
+            st = L.csem_last_status()
+            if st < 0:
+                raise RuntimeError("%s failed (%d): %s" % (step, st, _lib.last_error()))
+        self.model = L.csem_active_model() if device else None
+        self.n = e.lE
+        self.freqs = [e.freqs[i] for i in range(e.nfreq)]
+        self._keep = []
+
+    # ------------------------------------------------------------- facts
+    @property
+    def dims(self):
+        e = self.emf
+        return dict(n1=e.n1, n2=e.n2, n3=e.n3, nb=e.nb, nz=e.nz, lEx=e.lEx, lEy=e.lEy, lEz=e.lEz, lE=e.lE)
+
+    def array(self, name, count):
+        ptr = getattr(self.emf, name)
+        return np.ctypeslib.as_array(ptr, (count,)).copy()
+
+    def sigma(self, which):
+        e = self.emf
+        n1, n2, n3 = e.n1, e.n2, e.nz + 2 * e.nb
+        cnt = {11: n1 * (n2 + 1) * (n3 + 1), 22: (n1 + 1) * n2 * (n3 + 1), 33: (n1 + 1) * (n2 + 1) * n3}[which]
+        p = getattr(e, "sigma%d" % which)[0][0]
+        return np.ctypeslib.as_array(p, (cnt,)).copy()
+
+    # ------------------------------------------------------- frequency
+    def set_freq(self, freq):
+        """emf->I_omega_mu0 = I*2*PI*f*mu0 (main.c:95)."""
+        f = freq_consts(freq)
+        self.emf.I_omega_mu0[0] = f.iomu_re
+        self.emf.I_omega_mu0[1] = f.iomu_im
+        self.freq = freq
+        return f
+
+    # -------------------------------------------------- host entry points
+    def fdfd_apply(self, x):
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.zeros(self.n, np.complex128)
+        lib().fdfd_apply(self.n, _p(x), _p(y))
+        self._status("fdfd_apply")
+        return y
+
+    def lusgs_apply(self, x):
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.zeros(self.n, np.complex128)
+        lib().lusgs_apply(self.n, _p(x), _p(y))
+        self._status("lusgs_apply")
+        return y
+
+    def airbc_apply(self, hz):
+        e = self.emf
+        hz = np.ascontiguousarray(hz, np.complex128)
+        hx = np.zeros((e.n1 + 1) * e.n2, np.complex128)
+        hy = np.zeros(e.n1 * (e.n2 + 1), np.complex128)
+        lib().airbc_apply(C.byref(e), _p(hz), _p(hx), _p(hy))
+        self._status("airbc_apply")
+        return hx, hy
+
+    def cdotprod(self, a, b):
+        a = np.ascontiguousarray(a, np.complex128)
+        b = np.ascontiguousarray(b, np.complex128)
+        r = lib().cdotprod(len(a), _p(a), _p(b))
+        self._status("cdotprod")
+        return complex(r.re, r.im)
+
+    def inject(self):
+        b = np.zeros(self.n, np.complex128)
+        lib().inject_source(C.byref(self.acq), C.byref(self.emf), _p(b))
+        return b
+
+    def extract(self, x, ifreq=0):
+        """extract_emf into a fresh dcal_fd; returns complex64 (nchrec, nrec)."""
+        e, a = self.emf, self.acq
+        L = lib()
+        if not e.dcal_fd:
+            e.dcal_fd = L.alloc3complexf(a.nrec, e.nfreq, e.nchrec)
+        base = C.cast(e.dcal_fd[0][0], C.c_void_p).value
+        total = a.nrec * e.nfreq * e.nchrec
+        C.memset(base, 0, total * 8)
+        x = np.ascontiguousarray(x, np.complex128)
+        L.extract_emf(C.byref(a), C.byref(e), _p(x), ifreq)
+        arr = np.ctypeslib.as_array(C.cast(base, C.POINTER(C.c_float)), (total * 2,)).copy().view(np.complex64)
+        return arr.reshape(e.nchrec, e.nfreq, a.nrec)[:, ifreq, :]
+
+    def solve(self, b, tol, niter=None, algopt=None, preco=None, nrestart=None, x0=None):
+        """The solver call of main.c:104-118; returns (x, iterations)."""
+        e = self.emf
+        L = lib()
+        niter = e.niter if niter is None else niter
+        algopt = e.algopt if algopt is None else algopt
+        preco = e.preco if preco is None else preco
+        nrestart = e.nrestart if nrestart is None else nrestart
+        x = np.zeros(self.n, np.complex128) if x0 is None else np.array(x0, np.complex128)
+        b = np.ascontiguousarray(b, np.complex128)
+        A = _lib.fn_addr("fdfd_apply")
+        M = _lib.fn_addr("lusgs_apply")
+        if preco == 0 and algopt == 1:
+            L.bicgstab(self.n, _p(x), _p(b), A, niter, tol, 0)
+        elif preco == 0:
+            L.gmres(self.n, _p(x), _p(b), A, niter // nrestart, tol, nrestart, 0)
+        elif algopt == 1:
+            L.fbicgstab(self.n, _p(x), _p(b), A, M, niter, tol, 0)
+        else:
+            L.fgmres(self.n, _p(x), _p(b), A, M, niter // nrestart, tol, nrestart, 0)
+        self._status("solve")
+        return x, L.csem_last_iterations()
+
+    def _status(self, what):
+        st = lib().csem_last_status()
+        if st < 0:
+            raise RuntimeError("%s failed (%d): %s" % (what, st, _lib.last_error()))
+
+    def close(self):
+        if self.model:
+            lib().fdfd_free(C.byref(self.emf))
+            self.model = None
+
+    def multipliers(self):
+        """The airwave multipliers sH1, sH2 airbc_init computed (nfft x nfft)."""
+        e = self.emf
+        N = e.n1fft
+        out = []
+        for p in (e.sH1kxky, e.sH2kxky):
+            a = np.ctypeslib.as_array(p[0], (2 * N * N,)).copy().view(np.complex128)
+            out.append(a)
+        return out

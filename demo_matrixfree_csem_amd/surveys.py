@@ -1,0 +1,167 @@
+"""Synthetic marine CSEM surveys in the reference's input formats.
+
+A survey is a model (float32 node coordinates and float32 cell
+resistivities, x fastest, as emf_init reads them: emf_init_free.c:77-150)
+plus an acquisition (text source/receiver tables, acq_init_free.c:38-80).
+`write_files` writes exactly those files so that the reference's own
+command line (`fx1= ... frho11= ... fsrc= frec=`) and this package's
+driver both read them.
+
+Generators cover the measured workloads:
+  layered_64      64x64x64 uniform 100 m cells, water + 1 ohm-m + 100 ohm-m slab
+  layered_100     100^3 uniform 100 m cells, 8 seeded log-uniform layers
+  tiny            small seeded random models for parity tests
+"""
+from dataclasses import dataclass, field
+import os
+
+import numpy as np
+
+RHO_WATER = 0.3125
+
+
+@dataclass
+class Model:
+    x1node: np.ndarray
+    x2node: np.ndarray
+    x3node: np.ndarray
+    rho11: np.ndarray  # (nz, ny, nx) float32
+    rho22: np.ndarray
+    rho33: np.ndarray
+
+    @property
+    def shape(self):
+        return len(self.x1node) - 1, len(self.x2node) - 1, len(self.x3node) - 1
+
+
+@dataclass
+class Acquisition:
+    sources: np.ndarray  # (ns, 5) float32: x, y, z, azimuth, dip
+    receivers: np.ndarray  # (nr, 5)
+
+
+@dataclass
+class Survey:
+    name: str
+    model: Model
+    acq: Acquisition
+    freqs: list
+    flags: dict = field(default_factory=dict)
+
+    def argv(self, workdir=".", isrc=0):
+        """key=value flags for the reference command line (README.md:49-112)."""
+        f = dict(self.flags)
+        f["freqs"] = ",".join("%g" % v for v in self.freqs)
+        for k in ("fx1", "fx2", "fx3", "frho11", "frho22", "frho33"):
+            f[k] = os.path.join(workdir, k)
+        f["fsrc"] = os.path.join(workdir, "sources%d.txt" % isrc if isrc else "sources.txt")
+        f["frec"] = os.path.join(workdir, "receivers.txt")
+        return ["%s=%s" % (k, v) for k, v in f.items()]
+
+
+def _acq_table(rows):
+    a = np.asarray(rows, np.float32)
+    if a.ndim == 1:
+        a = a[None, :]
+    if a.shape[1] == 3:
+        a = np.concatenate([a, np.zeros((a.shape[0], 2), np.float32)], 1)
+    return a
+
+
+def write_table(path, rows, label):
+    with open(path, "w") as f:
+        f.write("          x1          x2          x3     azimuth         dip         %s\n" % label)
+        for i, r in enumerate(rows):
+            # %.9g round-trips float32 through the reference's fscanf("%f")
+            f.write("%16.9g %16.9g %16.9g %12.9g %12.9g %8d\n" % (tuple(float(v) for v in r[:5]) + (i + 1,)))
+
+
+def write_files(workdir, survey):
+    os.makedirs(workdir, exist_ok=True)
+    m = survey.model
+    for k, a in (("fx1", m.x1node), ("fx2", m.x2node), ("fx3", m.x3node)):
+        np.asarray(a, np.float32).tofile(os.path.join(workdir, k))
+    for k, a in (("frho11", m.rho11), ("frho22", m.rho22), ("frho33", m.rho33)):
+        np.ascontiguousarray(a, np.float32).tofile(os.path.join(workdir, k))
+    write_table(os.path.join(workdir, "sources.txt"), survey.acq.sources, "iTx")
+    for i in range(1, len(survey.acq.sources)):
+        write_table(os.path.join(workdir, "sources%d.txt" % i), survey.acq.sources[i:i + 1], "iTx")
+    write_table(os.path.join(workdir, "receivers.txt"), survey.acq.receivers, "iRx")
+
+
+def _centers(nodes):
+    nodes = np.asarray(nodes, np.float64)
+    return 0.5 * (nodes[1:] + nodes[:-1])
+
+
+def layered_64():
+    """64^3 uniform 100 m cells, seawater above 1000 m, 1 ohm-m below and a
+    100 ohm-m slab at z 2000-2100 m, |x|,|y| <= 1500 m; one Ex source at
+    (-2000, 0, 950), 33 receivers on y=0, z=1000; 0.25 Hz."""
+    x = (-3200 + 100 * np.arange(65)).astype(np.float32)
+    z = (100 * np.arange(65)).astype(np.float32)
+    xc, zc = _centers(x), _centers(z)
+    rho = np.where(zc[:, None, None] < 1000, RHO_WATER, 1.0) * np.ones((64, 64, 64))
+    slab = (np.abs(xc)[None, None, :] <= 1500) & (np.abs(xc)[None, :, None] <= 1500) & \
+           (zc[:, None, None] > 2000) & (zc[:, None, None] < 2100)
+    rho = np.where(slab, 100.0, rho).astype(np.float32)
+    rec = [(xr, 0.0, 1000.0) for xr in np.arange(-3200, 3201, 200)]
+    return Survey("layered_64", Model(x, x.copy(), z, rho, rho.copy(), rho.copy()),
+                  Acquisition(_acq_table([(-2000.0, 0.0, 950.0)]), _acq_table(rec)), [0.25],
+                  dict(fvm=0, airbc=1, preco=1, algopt=1, tol=1e-8, niter=1000, nb=10,
+                       lextend=40e3, chsrc="Ex", chrec="Ex,Ey,Ez"))
+
+
+def layered_100(seed=1):
+    """100^3 uniform 100 m cells on x,y in [-5000,5000], z in [0,10000]:
+    water (0.3125 ohm-m) above 1000 m, then 8 layers of 1125 m with seeded
+    log-uniform resistivity in [0.5, 200] ohm-m (by cell centre).  One Ex
+    source at (0,0,950), 51 receivers on y=0, z=1000; 0.25/0.75/1.25 Hz."""
+    rng = np.random.default_rng(seed)
+    x = (-5000 + 100 * np.arange(101)).astype(np.float32)
+    z = (100 * np.arange(101)).astype(np.float32)
+    zc = _centers(z)
+    rl = 10 ** rng.uniform(np.log10(0.5), np.log10(200.0), 8)
+    layer = np.clip(((zc - 1000) // 1125).astype(int), 0, 7)
+    prof = np.where(zc < 1000, RHO_WATER, rl[layer])
+    rho = (prof[:, None, None] * np.ones((100, 100, 100))).astype(np.float32)
+    rec = [(xr, 0.0, 1000.0) for xr in np.arange(-5000, 5001, 200)]
+    return Survey("layered_100", Model(x, x.copy(), z, rho, rho.copy(), rho.copy()),
+                  Acquisition(_acq_table([(0.0, 0.0, 950.0)]), _acq_table(rec)), [0.25, 0.75, 1.25],
+                  dict(fvm=0, airbc=1, preco=1, algopt=1, tol=1e-8, niter=1000, nb=10,
+                       chsrc="Ex", chrec="Ex,Ey,Ez"))
+
+
+def tiny(seed=0, nx=10, ny=9, nz=7, airbc=1, aniso=True, dz=100.0):
+    """A small seeded model with random cell sizes and resistivities, a
+    water layer on top (so rho_water is the water value), one Ex source
+    and a few receivers.  nz <= ny keeps the operator's H scratch layout
+    unaliased (the reference sizes it with n2 for the k extent)."""
+    rng = np.random.default_rng(seed)
+
+    def axis(n, d0, origin):
+        d = rng.uniform(0.6, 1.4, n) * d0
+        return (origin + np.concatenate([[0.0], np.cumsum(d)])).astype(np.float32)
+
+    x = axis(nx, 100.0, -50.0 * nx)
+    y = axis(ny, 100.0, -50.0 * ny)
+    z = (dz * np.arange(nz + 1)).astype(np.float32)
+    zc = _centers(z)
+    r11 = 10 ** rng.uniform(0, 1.5, (nz, ny, nx))
+    r22 = r11 if not aniso else 10 ** rng.uniform(0, 1.5, (nz, ny, nx))
+    r33 = 2 * r11 if aniso else r11
+    water = (zc < 2 * dz)[:, None, None]
+    r11 = np.where(water, RHO_WATER, r11).astype(np.float32)
+    r22 = np.where(water, RHO_WATER, r22).astype(np.float32)
+    r33 = np.where(water, RHO_WATER, r33).astype(np.float32)
+    xm, ym = float(x[0]), float(y[0])
+    xw, yw = float(x[-1] - x[0]), float(y[-1] - y[0])
+    src = [(xm + 0.31 * xw, ym + 0.47 * yw, 1.45 * dz)]
+    rec = [(xm + f * xw, ym + 0.52 * yw, 2.0 * dz) for f in (0.2, 0.45, 0.6, 0.85)]
+    return Survey("tiny%d" % seed, Model(x, y, z, r11, r22, r33),
+                  Acquisition(_acq_table(src), _acq_table(rec)), [0.5, 1.25],
+                  dict(fvm=0, airbc=airbc, preco=1, algopt=1, tol=1e-8, niter=200, nb=3,
+                       chsrc="Ex", chrec="Ex,Ey,Ez"))
+
+
+GENERATORS = {"layered_64": layered_64, "layered_100": layered_100, "tiny": tiny}

@@ -1,0 +1,110 @@
+/* csem_dev.h -- device-level C-ABI of the MI355X solve path.
+ *
+ * Plain C types only: pointers, sizes, doubles.  Device buffers are
+ * passed as `void *` device addresses; `stream` is a hipStream_t (NULL =
+ * the null stream).  Complex data is interleaved (re, im) double pairs,
+ * the layout of C99 `double _Complex` that the reference uses for its
+ * vectors (emf.h:44-47), so the host entry points in csem.h hand their
+ * buffers straight through.
+ *
+ * Vector layout (fdfd.c:13-21): E = [Ex | Ey | Ez], each x fastest,
+ *   Ex(k,j,i) = i + n1*(j + (n2+1)*k)       k in [0,n3], j in [0,n2], i in [0,n1)
+ *   Ey(k,j,i) = i + (n1+1)*(j + n2*k)       k in [0,n3], j in [0,n2), i in [0,n1]
+ *   Ez(k,j,i) = i + (n1+1)*(j + (n2+1)*k)   k in [0,n3), j in [0,n2], i in [0,n1]
+ * with n3 the operator depth (nz+nb under the airwave boundary, fdfd.c:49).
+ *
+ * All functions return 0 on success and a negative code on error; the
+ * message is available from csem_last_error().
+ */
+#ifndef CSEM_DEV_H
+#define CSEM_DEV_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct csem_model csem_model;
+
+/* Everything the device needs to know about one extended model.  All
+ * arrays are host arrays, copied at creation. */
+typedef struct {
+  int n1, n2, n3f;  /* extended grid: nx+2nb, ny+2nb, nz+2nb */
+  int nb, airbc, fvm;
+  const double *d1, *d2, *d3, *d1s, *d2s, *d3s; /* n+1 each, full grid (extend_model.c:8-21) */
+  const double *sigma11, *sigma22, *sigma33;    /* edge conductivities, full grid (extend_model.c:180-252) */
+  /* airwave boundary (airbc=1), airbc_init_free.c:24-109 */
+  int nfft;                  /* transform size (square, power of two) */
+  const double *sH1, *sH2;   /* nfft*nfft interleaved complex multipliers, [i2][i1] */
+  const double *twiddle;     /* nfft interleaved complex, exp(-2 pi i t / nfft) */
+  const int *jx, *jy;        /* forward interpolation cell per uniform index, -1 = outside */
+  const double *wx, *wy;
+  const int *gxj1, *gxj2;    /* ghost Hx: uniform cell per i1 in [0,n1), per i2 in [0,n2) */
+  const double *gxw1, *gxw2;
+  const int *gyj1, *gyj2;    /* ghost Hy: per i1 in [0,n1), per i2 in [0,n2) */
+  const double *gyw1, *gyw2;
+} csem_model_desc;
+
+/* Per-frequency constants: iomu = I*2*pi*f*mu0 (main.c:95), iwmu = 1/iomu
+ * (fdfd.c:100).  csem_freq_consts computes both exactly as the reference. */
+typedef struct {
+  double iomu_re, iomu_im, iwmu_re, iwmu_im;
+} csem_freq;
+
+const char *csem_last_error(void);
+int csem_freq_consts(float freq, csem_freq *out);
+
+int csem_model_create(const csem_model_desc *d, csem_model **out);
+void csem_model_destroy(csem_model *m);
+/* operator vector length lE, LU-SGS (full grid) length, device bytes held */
+long csem_model_len(const csem_model *m);
+long csem_model_full_len(const csem_model *m);
+long csem_model_bytes(const csem_model *m);
+
+/* y = A x, fdfd.c:86-205 (fused: H is never stored). */
+int csem_apply(csem_model *m, const csem_freq *f, const void *x, void *y, void *stream);
+/* y = M^-1 x, LU-SGS, fdfd.c:287-510 */
+int csem_lusgs(csem_model *m, const csem_freq *f, const void *x, void *y, void *stream);
+/* H = curl E / (i w mu0) on faces, fdfd.c:125-149: Hx (n1+1)*n2*n3,
+ * Hy n1*(n2+1)*n3, Hz n1*n2*(n3+1), stored contiguously Hx|Hy|Hz. */
+int csem_curl_h(csem_model *m, const csem_freq *f, const void *x, void *h, void *stream);
+/* ghost planes from Hz(z=0), airbc_init_free.c:130-209 */
+int csem_airbc(csem_model *m, const void *hz0, void *hxg, void *hyg, void *stream);
+/* 2-D transform of an nfft x nfft buffer in place, dir -1 forward (rows
+ * then columns), +1 inverse unnormalised (columns then rows). */
+int csem_fft2(csem_model *m, void *data, int dir, void *stream);
+/* out[0..1] = sum a_i conj(b_i) in the fixed chunked order; out is a
+ * device pointer to two doubles. */
+int csem_dot(long n, const void *a, const void *b, void *out, void *stream);
+
+/* Device-resident solves of A x = b.  x holds the initial guess on entry.
+ * precond: 0 none, 1 LU-SGS.  Returns the iteration index at convergence
+ * (as printed by "converged at iter=", solver.c:160) or niter, or < 0 on
+ * error.  hist (host, may be NULL, niter+2 doubles) receives relres. */
+int csem_bicgstab(csem_model *m, const csem_freq *f, void *x, const void *b, int niter,
+                  double tol, int precond, double *hist, void *stream);
+int csem_fgmres(csem_model *m, const csem_freq *f, void *x, const void *b, int nouter,
+                double tol, int mrestart, int precond, double *hist, void *stream);
+int csem_gmres(csem_model *m, const csem_freq *f, void *x, const void *b, int nouter,
+               double tol, int mrestart, double *hist, void *stream);
+
+/* Sparse source term: b[idx[q]] += val[q] for q in order (val interleaved
+ * complex, host arrays). */
+int csem_scatter_add(long nnz, const long *idx, const double *val, void *b, void *stream);
+/* Receiver interpolation: for each receiver r, out[r] = sum_{q<8}
+ * (s[q] * x[idx[8r+q]]) * w1 * w2 * w3 in the order of
+ * inject_extract.c:151-158, then times scale[r].  Host tables, device x,
+ * host out (interleaved complex doubles). */
+int csem_gather(int nrec, const long *idx, const double *wts, const double *sig,
+                const double *scale, const void *x, double *out, void *stream);
+
+/* Device buffer helpers for hosts without their own allocator. */
+int csem_malloc(void **p, long bytes);
+int csem_free(void *p);
+int csem_memcpy_h2d(void *dst, const void *src, long bytes, void *stream);
+int csem_memcpy_d2h(void *dst, const void *src, long bytes, void *stream);
+int csem_memset(void *p, int v, long bytes, void *stream);
+int csem_sync(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

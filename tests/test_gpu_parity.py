@@ -1,0 +1,157 @@
+"""HIP path against the oracle, through the C-ABI entry points.
+
+The oracle (oracle/csem_oracle.c) is pinned to the reference's own code
+elsewhere (tests/test_oracle_ref.py); here every device result must equal
+the oracle bit for bit on the same inputs, except where noted:
+  - the Krylov solves are compared with the oracle running the device's
+    chunked reduction order (bitwise), and with the reference's serial
+    order (receivers within 1e-9 relative, iterations within +-2).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from demo_matrixfree_csem_amd import surveys
+from demo_matrixfree_csem_amd.session import Session
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(workdir, airbc, seed=3, fvm=0, nfft=64, **kw):
+    s = surveys.tiny(seed, airbc=airbc, **kw)
+    s.flags.update(fvm=fvm, n1fft=nfft, n2fft=nfft)
+    surveys.write_files(workdir, s)
+    ses = Session(s.argv(workdir))
+    m = s.model
+    orc = O.Oracle(m.x1node, m.x2node, m.x3node, m.rho11, m.rho22, m.rho33, nb=s.flags["nb"],
+                   airbc=airbc, fvm=fvm, nfft=nfft, rho_air=float(s.flags.get("rho_air", 1e6)))
+    return s, ses, orc
+
+
+def _rand(n, seed):
+    r = np.random.default_rng(seed)
+    return r.standard_normal(n) + 1j * r.standard_normal(n)
+
+
+@pytest.mark.parametrize("airbc,fvm", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_apply_bitwise(workdir, airbc, fvm):
+    print("case apply airbc=%d fvm=%d" % (airbc, fvm), flush=True)
+    s, ses, orc = _pair(workdir, airbc, fvm=fvm)
+    try:
+        for f in (0.25, 1.25):
+            ses.set_freq(f)
+            orc.set_freq(f)
+            x = _rand(ses.n, 7)
+            y = ses.fdfd_apply(x)
+            yo = orc.apply(x)
+            assert np.array_equal(y, yo), np.abs(y - yo).max() / np.abs(yo).max()
+    finally:
+        ses.close()
+
+
+@pytest.mark.parametrize("airbc,fvm", [(0, 0), (1, 0), (1, 1)])
+def test_lusgs_bitwise(workdir, airbc, fvm):
+    print("case lusgs airbc=%d fvm=%d" % (airbc, fvm), flush=True)
+    s, ses, orc = _pair(workdir, airbc, fvm=fvm)
+    try:
+        ses.set_freq(0.75)
+        orc.set_freq(0.75)
+        for seed in (1, 2):
+            x = _rand(ses.n, seed)
+            y = ses.lusgs_apply(x)
+            yo = orc.lusgs(x)
+            assert np.array_equal(y, yo), np.abs(y - yo).max() / np.abs(yo).max()
+    finally:
+        ses.close()
+
+
+def test_airbc_bitwise(workdir):
+    print("case airbc", flush=True)
+    s, ses, orc = _pair(workdir, 1)
+    try:
+        e = ses.emf
+        hz = _rand(e.n1 * e.n2, 11)
+        hx, hy = ses.airbc_apply(hz)
+        ox, oy = orc.airbc(hz)
+        assert np.array_equal(hx, ox)
+        assert np.array_equal(hy, oy)
+    finally:
+        ses.close()
+
+
+def test_cdotprod_chunked_order(workdir):
+    print("case cdotprod", flush=True)
+    s, ses, orc = _pair(workdir, 0)
+    try:
+        orc.set_red(O.RED_CHUNKED)
+        for n in (1, 255, 4096, 4097, 70001):
+            a, b = _rand(n, n), _rand(n, n + 1)
+            assert ses.cdotprod(a, b) == orc.dot(a, b)
+    finally:
+        ses.close()
+
+
+@pytest.mark.parametrize("airbc", [0, 1])
+def test_fbicgstab_matches_oracle(workdir, airbc):
+    print("case fbicgstab airbc=%d" % airbc, flush=True)
+    kw = {} if airbc else {}
+    s, ses, orc = _pair(workdir, airbc, seed=2)
+    if not airbc:
+        pytest.skip("air-filled tiny model does not converge in a test-sized budget")
+    try:
+        for f in sorted(s.freqs):
+            ses.set_freq(f)
+            orc.set_freq(f)
+            tol = float(np.float32(np.float32(1e-8) / np.float32(f)))
+            b = ses.inject()
+            x, it = ses.solve(b, tol, niter=400)
+            orc.set_red(O.RED_CHUNKED)
+            xo, ito, _ = orc.fbicgstab(b, 400, tol)
+            assert it == ito
+            assert np.array_equal(x, xo)
+            orc.set_red(O.RED_SERIAL)
+            xs, its, _ = orc.fbicgstab(b, 400, tol)
+            assert abs(it - its) <= 2
+            for ch in range(3):
+                rs = np.array([orc.extract(ch, *r[:3], xs) for r in s.acq.receivers])
+                rg = np.array([orc.extract(ch, *r[:3], x) for r in s.acq.receivers])
+                assert np.max(np.abs(rg - rs) / np.abs(rs)) <= 1e-9
+    finally:
+        ses.close()
+
+
+def test_fgmres_matches_oracle(workdir):
+    print("case fgmres", flush=True)
+    s, ses, orc = _pair(workdir, 1, seed=2)
+    try:
+        f = 1.25
+        ses.set_freq(f)
+        orc.set_freq(f)
+        tol = float(np.float32(np.float32(1e-8) / np.float32(f)))
+        b = ses.inject()
+        x, it = ses.solve(b, tol, niter=300, algopt=2, nrestart=10)
+        orc.set_red(O.RED_CHUNKED)
+        xo, ito, _ = orc.fgmres(b, 30, tol, 10)
+        assert it == ito
+        assert np.array_equal(x, xo)
+    finally:
+        ses.close()
+
+
+def test_bicgstab_unpreconditioned(workdir):
+    print("case bicgstab", flush=True)
+    s, ses, orc = _pair(workdir, 1, seed=2)
+    try:
+        f = 1.25
+        ses.set_freq(f)
+        orc.set_freq(f)
+        b = ses.inject()
+        x, it = ses.solve(b, 1e-30, niter=40, preco=0)
+        orc.set_red(O.RED_CHUNKED)
+        xo, ito, _ = orc.fbicgstab(b, 40, 1e-30, precond=0)
+        assert it == ito == 40
+        assert np.array_equal(x, xo)
+    finally:
+        ses.close()
