@@ -90,6 +90,7 @@ DEV_SYMBOLS = [
     "csem_model_full_len", "csem_model_bytes", "csem_apply", "csem_lusgs", "csem_curl_h", "csem_airbc",
     "csem_fft2", "csem_dot", "csem_bicgstab", "csem_fgmres", "csem_gmres", "csem_scatter_add", "csem_gather",
     "csem_malloc", "csem_free", "csem_memcpy_h2d", "csem_memcpy_d2h", "csem_memset", "csem_sync",
+    "csem_kernel_timing", "csem_kernel_stats",
 ]
 
 _lib = None
@@ -127,6 +128,8 @@ def lib():
     L.csem_memcpy_d2h.argtypes = [vp, vp, lp, vp]
     L.csem_memset.argtypes = [vp, ip, lp, vp]
     L.csem_sync.argtypes = [vp]
+    L.csem_kernel_timing.argtypes = [ip]
+    L.csem_kernel_stats.argtypes = [ip, C.POINTER(C.c_char_p), C.POINTER(dp), C.POINTER(lp), C.POINTER(dp)]
     L.csem_active_model.restype = vp
     L.csem_fdfd_prepare.argtypes = [C.POINTER(EmfT)]
     L.csem_model_from_emf.argtypes = [C.POINTER(EmfT), C.POINTER(vp)]
@@ -173,6 +176,22 @@ def check(rc, what=""):
     if rc < 0:
         raise RuntimeError("%s failed (%d): %s" % (what, rc, last_error()))
     return rc
+
+
+def kernel_stats():
+    """{name: (total_ms, launches, algorithmic_bytes)} since csem_kernel_timing(1)."""
+    L = lib()
+    out = {}
+    i, n = 0, 1
+    while i < n:
+        name, ms, cnt, by = C.c_char_p(), C.c_double(), C.c_long(), C.c_double()
+        n = L.csem_kernel_stats(i, C.byref(name), C.byref(ms), C.byref(cnt), C.byref(by))
+        if n < 0:
+            break
+        if cnt.value:
+            out[name.value.decode()] = (ms.value, cnt.value, by.value)
+        i += 1
+    return out
 
 
 def fn_addr(name):

@@ -52,7 +52,9 @@ def build(verbose=False):
         _run(["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lm"])
+    # -Bsymbolic: the library's own err()/warn() (the reference's names)
+    # must not be interposed by the C library's err(3)/warn(3)
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-Wl,-Bsymbolic", "-o", tmp] + objs + ["-lm"])
     os.replace(tmp, LIB)
     main_src = os.path.join(CSRC, "host", "csem_main.c")
     if os.path.exists(main_src):

@@ -26,6 +26,102 @@ extern "C" const char *csem_last_error(void) { return g_err; }
 
 #define PI_REF 3.14159265358979323846264
 
+// ------------------------------------------------------- kernel timers
+// Optional HIP-event timing of every launch of the hot kernels, on the
+// stream they run on, with the algorithmic bytes each launch moves.
+enum {
+  KT_APPLY = 0, KT_HZ0, KT_AIR_ROWS_FWD, KT_AIR_COLS, KT_AIR_ROWS_INV, KT_AIR_GHOST,
+  KT_SGS_EX_F, KT_SGS_EY_F, KT_SGS_EZ_F, KT_SGS_EZ_B, KT_SGS_EY_B, KT_SGS_EX_B,
+  KT_DOT, KT_DOT_FIN, KT_BICG_S, KT_BICG_XR, KT_BICG_P, KT_N
+};
+static const char *KT_NAMES[KT_N] = {
+  "k_apply", "k_hz0", "k_air_rows_fwd", "k_air_cols", "k_air_rows_inv", "k_air_ghost",
+  "k_sgs_ex<fwd>", "k_sgs_ey<fwd>", "k_sgs_ez<fwd>", "k_sgs_ez<bwd>", "k_sgs_ey<bwd>", "k_sgs_ex<bwd>",
+  "k_dot_partial", "k_dot_final", "k_bicg_s", "k_bicg_xr", "k_bicg_p"};
+
+struct KtPend {
+  int id;
+  hipEvent_t a, b;
+  double bytes;
+};
+static struct {
+  int on;
+  std::vector<hipEvent_t> pool;
+  std::vector<KtPend> pend;
+  double ms[KT_N], bytes[KT_N];
+  long cnt[KT_N];
+} g_kt;
+
+static hipEvent_t kt_event()
+{
+  hipEvent_t e;
+  if (!g_kt.pool.empty()) {
+    e = g_kt.pool.back();
+    g_kt.pool.pop_back();
+    return e;
+  }
+  if (hipEventCreate(&e) != hipSuccess) return NULL;
+  return e;
+}
+
+static hipEvent_t kt_begin(hipStream_t s)
+{
+  if (!g_kt.on) return NULL;
+  hipEvent_t e = kt_event();
+  if (e) (void)hipEventRecord(e, s);
+  return e;
+}
+
+static void kt_end(int id, hipEvent_t a, hipStream_t s, double bytes)
+{
+  if (!g_kt.on || !a) return;
+  hipEvent_t b = kt_event();
+  if (!b) return;
+  (void)hipEventRecord(b, s);
+  g_kt.pend.push_back({id, a, b, bytes});
+}
+
+static void kt_collect()
+{
+  for (auto &p : g_kt.pend) {
+    float ms = 0;
+    (void)hipEventSynchronize(p.b);
+    (void)hipEventElapsedTime(&ms, p.a, p.b);
+    g_kt.ms[p.id] += ms;
+    g_kt.bytes[p.id] += p.bytes;
+    g_kt.cnt[p.id]++;
+    g_kt.pool.push_back(p.a);
+    g_kt.pool.push_back(p.b);
+  }
+  g_kt.pend.clear();
+}
+
+extern "C" int csem_kernel_timing(int on)
+{
+  kt_collect();
+  g_kt.on = on;
+  for (int i = 0; i < KT_N; i++) { g_kt.ms[i] = 0; g_kt.bytes[i] = 0; g_kt.cnt[i] = 0; }
+  return 0;
+}
+
+extern "C" int csem_kernel_stats(int id, const char **name, double *total_ms, long *count, double *bytes)
+{
+  if (id < 0 || id >= KT_N) return -1;
+  kt_collect();
+  *name = KT_NAMES[id];
+  *total_ms = g_kt.ms[id];
+  *count = g_kt.cnt[id];
+  *bytes = g_kt.bytes[id];
+  return KT_N;
+}
+
+#define KT(id, s, bytes, launch)        \
+  do {                                   \
+    hipEvent_t kta_ = kt_begin(s);       \
+    launch;                              \
+    kt_end(id, kta_, s, (double)(bytes)); \
+  } while (0)
+
 extern "C" int csem_freq_consts(float freq, csem_freq *out)
 {
   // I*2.*PI*f*mu0 (main.c:95) has a zero real part and imaginary part
@@ -217,11 +313,13 @@ static int run_airbc(csem_model *m, hipStream_t s)
   AirArgs a = air_args(m);
   const int N = m->nfft;
   const size_t row_lds = 2 * (size_t)N * sizeof(z2);
-  k_air_rows_fwd<<<N, 256, row_lds, s>>>(a);
-  k_air_cols<2><<<N / 2, 256, 3 * 2 * (size_t)N * sizeof(z2), s>>>(a);
-  k_air_rows_inv<<<m->nrowx + m->nrowy, 256, row_lds, s>>>(a);
+  const double NN = (double)N * N;
+  KT(KT_AIR_ROWS_FWD, s, 16 * NN, (k_air_rows_fwd<<<N, 256, row_lds, s>>>(a)));
+  KT(KT_AIR_COLS, s, 80 * NN, (k_air_cols<2><<<N / 2, 256, 3 * 2 * (size_t)N * sizeof(z2), s>>>(a)));
+  KT(KT_AIR_ROWS_INV, s, 32.0 * N * (m->nrowx + m->nrowy),
+     (k_air_rows_inv<<<m->nrowx + m->nrowy, 256, row_lds, s>>>(a)));
   const int ng = (m->n1 + 1) * m->n2 + m->n1 * (m->n2 + 1);
-  k_air_ghost<<<(ng + 255) / 256, 256, 0, s>>>(a);
+  KT(KT_AIR_GHOST, s, 16.0 * ng, (k_air_ghost<<<(ng + 255) / 256, 256, 0, s>>>(a)));
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -231,13 +329,13 @@ static int run_apply(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   ApplyArgs a = apply_args(m, f, x, y);
   if (m->airbc) {
     const int nh = m->n1 * m->n2;
-    k_hz0<<<(nh + 255) / 256, 256, 0, s>>>(a, m->hz0);
+    KT(KT_HZ0, s, 16.0 * nh, (k_hz0<<<(nh + 255) / 256, 256, 0, s>>>(a, m->hz0)));
     int rc = run_airbc(m, s);
     if (rc) return rc;
   }
   dim3 blk(64, 4);
   dim3 grd((m->n1 + 1 + 63) / 64, (m->n2 + 1 + 3) / 4, (m->n3 + 1 + m->kc - 1) / m->kc);
-  k_apply<64, 4><<<grd, blk, 0, s>>>(a);
+  KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<64, 4><<<grd, blk, 0, s>>>(a)));
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -255,12 +353,18 @@ static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   const int tk = ((m->n3f - 1 + 63) / 64) * 64;  // rows k = 1..n3f-1
   const int tj = ((m->n2 - 1 + 63) / 64) * 64;   // rows j = 1..n2-1
   const size_t lk = 2 * tk * sizeof(z2), lj = 2 * tj * sizeof(z2);
-  k_sgs_ex<true><<<m->n1, tk, lk, s>>>(a);
-  k_sgs_ey<true><<<m->n2, tk, lk, s>>>(a);
-  k_sgs_ez<true><<<m->n3f, tj, lj, s>>>(a);
-  k_sgs_ez<false><<<m->n3f, tj, lj, s>>>(a);
-  k_sgs_ey<false><<<m->n2, tk, lk, s>>>(a);
-  k_sgs_ex<false><<<m->n1, tk, lk, s>>>(a);
+  // algorithmic bytes per swept cell: forward Ex 40 (b, sigma, write),
+  // Ey 56 (+Ex), Ez 72 (+Ex, Ey); backward 88 (b, sigma, old value, the
+  // two other components, write)
+  const double cx = (double)m->n1 * (m->n2 - 1) * (m->n3f - 1);
+  const double cy = (double)(m->n1 - 1) * m->n2 * (m->n3f - 1);
+  const double cz = (double)(m->n1 - 1) * (m->n2 - 1) * m->n3f;
+  KT(KT_SGS_EX_F, s, 40 * cx, (k_sgs_ex<true><<<m->n1, tk, lk, s>>>(a)));
+  KT(KT_SGS_EY_F, s, 56 * cy, (k_sgs_ey<true><<<m->n2, tk, lk, s>>>(a)));
+  KT(KT_SGS_EZ_F, s, 72 * cz, (k_sgs_ez<true><<<m->n3f, tj, lj, s>>>(a)));
+  KT(KT_SGS_EZ_B, s, 88 * cz, (k_sgs_ez<false><<<m->n3f, tj, lj, s>>>(a)));
+  KT(KT_SGS_EY_B, s, 88 * cy, (k_sgs_ey<false><<<m->n2, tk, lk, s>>>(a)));
+  KT(KT_SGS_EX_B, s, 88 * cx, (k_sgs_ex<false><<<m->n1, tk, lk, s>>>(a)));
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -339,8 +443,8 @@ static int dot_partial_launch(long n, const z2 *a0, const z2 *b0, const z2 *a1, 
   DotArgs g;
   g.n = n; g.a[0] = a0; g.b[0] = b0; g.a[1] = a1; g.b[1] = b1; g.partials = partials;
   long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
-  if (a1) k_dot_partial<2><<<(unsigned)nch, RED_T, 0, s>>>(g);
-  else k_dot_partial<1><<<(unsigned)nch, RED_T, 0, s>>>(g);
+  if (a1) KT(KT_DOT, s, 64.0 * n, (k_dot_partial<2><<<(unsigned)nch, RED_T, 0, s>>>(g)));
+  else KT(KT_DOT, s, 32.0 * n, (k_dot_partial<1><<<(unsigned)nch, RED_T, 0, s>>>(g)));
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -550,7 +654,7 @@ extern "C" int csem_bicgstab(csem_model *m, const csem_freq *f, void *xv, const 
     if ((rc = run_apply(m, f, pin, v, s))) return rc;
     if ((rc = dot_partial_launch(n, v, r0, NULL, NULL, pa, s))) return rc;
     k_bicg_fin_alpha<<<1, RED_T, 0, s>>>(pa, nch, sc);
-    k_bicg_s<<<ge, 256, 0, s>>>(n, r, v, sv, sc);
+    KT(KT_BICG_S, s, 48.0 * n, (k_bicg_s<<<ge, 256, 0, s>>>(n, r, v, sv, sc)));
     const z2 *xb;
     if (precond) {
       if ((rc = run_lusgs(m, f, sv, r, s))) return rc;
@@ -563,17 +667,18 @@ extern "C" int csem_bicgstab(csem_model *m, const csem_freq *f, void *xv, const 
       xb = sv;
     }
     k_bicg_fin_omega<<<1, RED_T, 0, s>>>(pa, nch, sc);
-    k_bicg_xr<<<(unsigned)nch, RED_T, 0, s>>>(n, x, pin, xb, sv, t, r, r0, sc, pa);
+    KT(KT_BICG_XR, s, 128.0 * n, (k_bicg_xr<<<(unsigned)nch, RED_T, 0, s>>>(n, x, pin, xb, sv, t, r, r0, sc, pa)));
     k_bicg_fin_conv<<<1, RED_T, 0, s>>>(pa, nch, sc);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(hs, sc, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    kt_collect();
     relres = hs[SC_RELRES];
     if (hs[SC_CONV] != 0) {
       if (hist) hist[iter + 1] = relres;
       break;
     }
-    k_bicg_p<<<ge, 256, 0, s>>>(n, p, r, v, sc);
+    KT(KT_BICG_P, s, 64.0 * n, (k_bicg_p<<<ge, 256, 0, s>>>(n, p, r, v, sc)));
   }
   HIP_OK(hipGetLastError());
   return iter;

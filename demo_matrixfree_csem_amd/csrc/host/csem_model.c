@@ -15,6 +15,8 @@
 #include <string.h>
 #include "../../../include/csem.h"
 
+void csem_set_status(int st);
+
 #define CSEM_MAX(a, b) ((a) > (b) ? (a) : (b))
 #define CSEM_MIN(a, b) ((a) < (b) ? (a) : (b))
 
@@ -83,7 +85,11 @@ void emf_init(emf_t *emf)
   char *fx[3], *frho[3];
   int i;
   long n;
-  if (!(emf->nfreq = countparval("freqs"))) err("Need freqs= vector");
+  csem_set_status(0);
+  if (!(emf->nfreq = countparval("freqs"))) {
+    err("Need freqs= vector");
+    return;
+  }
   emf->freqs = alloc1float(emf->nfreq);
   getparfloat("freqs", emf->freqs);
   /* ascending; the comparator compares the float bit patterns as int,
@@ -111,36 +117,63 @@ void emf_init(emf_t *emf)
   if (!getparfloat("rho_air", &emf->rho_air)) emf->rho_air = 1e6;
   if (!getparint("istretch", &emf->istretch)) emf->istretch = 1;
   if (!getparint("airbc", &emf->airbc)) emf->airbc = 0;
-  if (emf->airbc && emf->nb == 0) err("airbc=1 assumes input model without air, you must ensure nb>0");
+  if (emf->airbc && emf->nb == 0) {
+    err("airbc=1 assumes input model without air, you must ensure nb>0");
+    return;
+  }
   if (emf->verb) {
     printf("freqs=");
     for (i = 0; i < emf->nfreq; i++) printf("%g,", emf->freqs[i]);
     printf("\nfvm=%d airbc=%d istretch=%d rho_air=%g nb=%d\n", emf->fvm, emf->airbc, emf->istretch,
            emf->rho_air, emf->nb);
   }
-  if (!getparstring("fx1", &fx[0])) err("Need fx1= ");
-  if (!getparstring("fx2", &fx[1])) err("Need fx2= ");
-  if (!getparstring("fx3", &fx[2])) err("Need fx3= ");
+  if (!getparstring("fx1", &fx[0])) {
+    err("Need fx1= ");
+    return;
+  }
+  if (!getparstring("fx2", &fx[1])) {
+    err("Need fx2= ");
+    return;
+  }
+  if (!getparstring("fx3", &fx[2])) {
+    err("Need fx3= ");
+    return;
+  }
   emf->nx = (int)file_floats(fx[0]) - 1;
   emf->ny = (int)file_floats(fx[1]) - 1;
   emf->nz = (int)file_floats(fx[2]) - 1;
-  if (emf->nx < 1 || emf->ny < 1 || emf->nz < 1) err("grid files must hold at least 2 nodes");
+  if (csem_last_status() < 0) return;
+  if (emf->nx < 1 || emf->ny < 1 || emf->nz < 1) {
+    err("grid files must hold at least 2 nodes");
+    return;
+  }
   emf->x1node = alloc1float(emf->nx + 1);
   emf->x2node = alloc1float(emf->ny + 1);
   emf->x3node = alloc1float(emf->nz + 1);
   read_floats(fx[0], emf->x1node, emf->nx + 1, "fx1");
   read_floats(fx[1], emf->x2node, emf->ny + 1, "fx2");
   read_floats(fx[2], emf->x3node, emf->nz + 1, "fx3");
+  if (csem_last_status() < 0) return;
   emf->rho11 = alloc3float(emf->nx, emf->ny, emf->nz);
   emf->rho22 = alloc3float(emf->nx, emf->ny, emf->nz);
   emf->rho33 = alloc3float(emf->nx, emf->ny, emf->nz);
   n = (long)emf->nx * emf->ny * emf->nz;
-  if (!getparstring("frho11", &frho[0])) err("Need frho11= ");
-  if (!getparstring("frho22", &frho[1])) err("Need frho22= ");
-  if (!getparstring("frho33", &frho[2])) err("Need frho33= ");
+  if (!getparstring("frho11", &frho[0])) {
+    err("Need frho11= ");
+    return;
+  }
+  if (!getparstring("frho22", &frho[1])) {
+    err("Need frho22= ");
+    return;
+  }
+  if (!getparstring("frho33", &frho[2])) {
+    err("Need frho33= ");
+    return;
+  }
   read_floats(frho[0], &emf->rho11[0][0][0], n, "frho11");
   read_floats(frho[1], &emf->rho22[0][0][0], n, "frho22");
   read_floats(frho[2], &emf->rho33[0][0][0], n, "frho33");
+  if (csem_last_status() < 0) return;
   {
     /* rho_water is the smallest resistivity of the input model */
     const float *a = &emf->rho11[0][0][0], *b = &emf->rho22[0][0][0], *c = &emf->rho33[0][0][0];
@@ -202,15 +235,31 @@ void acq_init(acq_t *acq, emf_t *emf)
   if (!getparfloat("lenrec", &acq->lenrec)) acq->lenrec = 8;
   if (!getparint("nsubsrc", &acq->nsubsrc)) acq->nsubsrc = 1;
   if (!getparint("nsubrec", &acq->nsubrec)) acq->nsubrec = 1;
-  if (!getparstring("fsrc", &fsrc)) err("Need fsrc= ");
-  if (!getparstring("frec", &frec)) err("Need frec= ");
+  if (!getparstring("fsrc", &fsrc)) {
+    err("Need fsrc= ");
+    return;
+  }
+  if (!getparstring("frec", &frec)) {
+    err("Need frec= ");
+    return;
+  }
   for (i = 0; i < 5; i++) {
     sc[i] = (float *)malloc(16 * sizeof(float));
     rc[i] = (float *)malloc(16 * sizeof(float));
   }
   ns = read_table(fsrc, sc, 16);
   nr = read_table(frec, rc, 16);
-  if (ns < 1) err("no source in %s", fsrc);
+  if (csem_last_status() < 0) {
+    for (i = 0; i < 5; i++) {
+      free(sc[i]);
+      free(rc[i]);
+    }
+    return;
+  }
+  if (ns < 1) {
+    err("no source in %s", fsrc);
+    return;
+  }
   acq->nsrc = 1;
   acq->nrec = nr;
   acq->src_x1 = alloc1float(1); acq->src_x2 = alloc1float(1); acq->src_x3 = alloc1float(1);
@@ -233,12 +282,30 @@ void acq_init(acq_t *acq, emf_t *emf)
     free(sc[i]);
     free(rc[i]);
   }
-  if (emf->x1node[0] > acq->x1min) err("x1 origin > acq->x1min");
-  if (emf->x1node[emf->nx] < acq->x1max) err("x1 end < acq->x1min");
-  if (emf->x2node[0] > acq->x2min) err("x2 origin > acq->x2min");
-  if (emf->x2node[emf->ny] < acq->x2max) err("x2 end < acq->x2min");
-  if (emf->x3node[0] > acq->x3min) err("x3 origin > acq->x3min");
-  if (emf->x3node[emf->nz] < acq->x3max) err("x3 end < acq->x3min");
+  if (emf->x1node[0] > acq->x1min) {
+    err("x1 origin > acq->x1min");
+    return;
+  }
+  if (emf->x1node[emf->nx] < acq->x1max) {
+    err("x1 end < acq->x1min");
+    return;
+  }
+  if (emf->x2node[0] > acq->x2min) {
+    err("x2 origin > acq->x2min");
+    return;
+  }
+  if (emf->x2node[emf->ny] < acq->x2max) {
+    err("x2 end < acq->x2min");
+    return;
+  }
+  if (emf->x3node[0] > acq->x3min) {
+    err("x3 origin > acq->x3min");
+    return;
+  }
+  if (emf->x3node[emf->nz] < acq->x3max) {
+    err("x3 end < acq->x3min");
+    return;
+  }
 }
 
 void acq_free(acq_t *acq)

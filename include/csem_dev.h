@@ -96,6 +96,14 @@ int csem_scatter_add(long nnz, const long *idx, const double *val, void *b, void
 int csem_gather(int nrec, const long *idx, const double *wts, const double *sig,
                 const double *scale, const void *x, double *out, void *stream);
 
+/* Kernel timers: csem_kernel_timing(1) resets and starts HIP-event timing
+ * of every hot-kernel launch (events recorded on the launch stream);
+ * csem_kernel_stats(id, ...) returns the kernel's name, summed duration,
+ * launch count and summed algorithmic bytes, and the number of kernel ids
+ * (ids run 0 .. n-1), or -1 for an id out of range. */
+int csem_kernel_timing(int on);
+int csem_kernel_stats(int id, const char **name, double *total_ms, long *count, double *bytes);
+
 /* Device buffer helpers for hosts without their own allocator. */
 int csem_malloc(void **p, long bytes);
 int csem_free(void *p);

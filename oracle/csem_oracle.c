@@ -33,6 +33,7 @@ struct orc {
   float lextend, rho_air, rho_water, rstretch[6];
   double *x1, *x2, *x3, *x1s, *x2s, *x3s, *d1, *d2, *d3, *d1s, *d2s, *d3s;
   double *s11, *s22, *s33; /* edge conductivities on the full (air-extended) grid */
+  double *c11, *c22, *c33; /* extended cell model as extend_model.c:256-275 dumps it */
   long lEx, lEy, lEz, lE;  /* operator vector (truncated grid when airbc) */
   long fEx, fEy, fEz, fE;  /* full grid, used by LU-SGS under airbc */
   long lHx, lHy, lHz, lH, hz_off;
@@ -302,6 +303,12 @@ orc_t *orc_build(int nx, int ny, int nz,
     }
 #undef C3
 #undef R3
+  m->c11 = dalloc(nc); m->c22 = dalloc(nc); m->c33 = dalloc(nc);
+  for (c = 0; c < nc; c++) {
+    m->c11[c] = 1. / sg11[c];
+    m->c22[c] = 1. / sg22[c];
+    m->c33[c] = rh33[c];
+  }
   free(sg11); free(sg22); free(rh33);
 
   /* fdfd_init (fdfd.c:40-62) */
@@ -346,6 +353,7 @@ void orc_free(orc_t *m)
   free(m->x1); free(m->x2); free(m->x3); free(m->x1s); free(m->x2s); free(m->x3s);
   free(m->d1); free(m->d2); free(m->d3); free(m->d1s); free(m->d2s); free(m->d3s);
   free(m->s11); free(m->s22); free(m->s33);
+  free(m->c11); free(m->c22); free(m->c33);
   free(m->xh); free(m->xx); free(m->yy); free(m->Hxg); free(m->Hyg);
   free(m->sH1); free(m->sH2); free(m->buf); free(m->buf0); free(m->tw); free(m->tmp);
   free(m->jx); free(m->jy); free(m->wx); free(m->wy);
@@ -405,6 +413,9 @@ void *orc_array(const orc_t *m, const char *s)
   if (!strcmp(s, "sigma11")) return m->s11;
   if (!strcmp(s, "sigma22")) return m->s22;
   if (!strcmp(s, "sigma33")) return m->s33;
+  if (!strcmp(s, "cell11")) return m->c11;
+  if (!strcmp(s, "cell22")) return m->c22;
+  if (!strcmp(s, "cell33")) return m->c33;
   if (!strcmp(s, "sH1")) return m->sH1;
   if (!strcmp(s, "sH2")) return m->sH2;
   if (!strcmp(s, "xh")) return m->xh;

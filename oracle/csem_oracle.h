@@ -43,7 +43,8 @@ long orc_info(const orc_t *m, int what);
  * rear,top,bottom), 8 d1fft, 9 d2fft, 10 d3fft */
 double orc_finfo(const orc_t *m, int what);
 /* named arrays: "x1","x2","x3","x1s","x2s","x3s","d1","d2","d3","d1s",
- * "d2s","d3s","sigma11","sigma22","sigma33" (double); "sH1","sH2","xh",
+ * "d2s","d3s","sigma11","sigma22","sigma33","cell11","cell22","cell33"
+ * (double; cells as extend_model.c:256-275 dumps them); "sH1","sH2","xh",
  * "Hxg","Hyg" (complex); "wx","wy" (double); "jx","jy" (int) */
 void *orc_array(const orc_t *m, const char *name);
 

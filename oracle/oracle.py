@@ -102,7 +102,8 @@ class Oracle:
     def array(self, name):
         n1, n2, n3 = self.n1, self.n2, self.n3f
         sizes = {"x1": n1 + 1, "x2": n2 + 1, "x3": n3 + 1, "sigma11": n1 * (n2 + 1) * (n3 + 1),
-                 "sigma22": (n1 + 1) * n2 * (n3 + 1), "sigma33": (n1 + 1) * (n2 + 1) * n3}
+                 "sigma22": (n1 + 1) * n2 * (n3 + 1), "sigma33": (n1 + 1) * (n2 + 1) * n3,
+                "cell11": n1 * n2 * n3, "cell22": n1 * n2 * n3, "cell33": n1 * n2 * n3}
         for a in ("x", "d"):
             for ax, n in ((1, n1), (2, n2), (3, n3)):
                 sizes["%s%d" % (a, ax)] = n + 1
