@@ -90,7 +90,7 @@ DEV_SYMBOLS = [
     "csem_model_full_len", "csem_model_bytes", "csem_apply", "csem_lusgs", "csem_curl_h", "csem_airbc",
     "csem_fft2", "csem_dot", "csem_bicgstab", "csem_fgmres", "csem_gmres", "csem_scatter_add", "csem_gather",
     "csem_malloc", "csem_free", "csem_memcpy_h2d", "csem_memcpy_d2h", "csem_memset", "csem_sync",
-    "csem_kernel_timing", "csem_kernel_stats",
+    "csem_kernel_timing", "csem_kernel_stats", "csem_set_reduction",
 ]
 
 _lib = None
@@ -129,6 +129,7 @@ def lib():
     L.csem_memset.argtypes = [vp, ip, lp, vp]
     L.csem_sync.argtypes = [vp]
     L.csem_kernel_timing.argtypes = [ip]
+    L.csem_set_reduction.argtypes = [ip]
     L.csem_kernel_stats.argtypes = [ip, C.POINTER(C.c_char_p), C.POINTER(dp), C.POINTER(lp), C.POINTER(dp)]
     L.csem_active_model.restype = vp
     L.csem_fdfd_prepare.argtypes = [C.POINTER(EmfT)]

@@ -449,11 +449,64 @@ static int dot_partial_launch(long n, const z2 *a0, const z2 *b0, const z2 *a1, 
   return 0;
 }
 
+// Reference summation order (cdotprod.c:18): one wavefront per dot product
+// adds the terms strictly left to right; every lane runs the same dependent
+// chain, fed by the 64 products of the current group through __shfl.
+// Block b of the launch computes dot product b.
+__global__ __launch_bounds__(64) void k_dot_serial(DotArgs g, double *out)
+{
+  const z2 *A = g.a[blockIdx.x], *B = g.b[blockIdx.x];
+  const int l = threadIdx.x;
+  const long n = g.n;
+  double sr = 0.0, si = 0.0;
+  for (long base = 0; base < n; base += 64) {
+    const long e = base + l;
+    double tr = 0, ti = 0;
+    if (e < n) {
+      z2 u = A[e], w = B[e];
+      double c = w.x, d = -w.y;
+      tr = u.x * c - u.y * d;
+      ti = u.x * d + u.y * c;
+    }
+    if (n - base >= 64) {
+#pragma unroll
+      for (int q = 0; q < 64; q++) {
+        sr = sr + __shfl(tr, q, 64);
+        si = si + __shfl(ti, q, 64);
+      }
+    } else {
+      for (int q = 0; q < (int)(n - base); q++) {
+        sr = sr + __shfl(tr, q, 64);
+        si = si + __shfl(ti, q, 64);
+      }
+    }
+  }
+  if (l == 0) {
+    out[2 * blockIdx.x] = sr;
+    out[2 * blockIdx.x + 1] = si;
+  }
+}
+
+static int g_red_serial = 0;
+
+extern "C" int csem_set_reduction(int serial)
+{
+  g_red_serial = serial ? 1 : 0;
+  return 0;
+}
+
 static double *g_dot_partials = NULL;
 static long g_dot_cap = 0;
 
 extern "C" int csem_dot(long n, const void *a, const void *b, void *out, void *stream)
 {
+  if (g_red_serial) {
+    DotArgs g;
+    g.n = n; g.a[0] = (const z2 *)a; g.b[0] = (const z2 *)b; g.a[1] = NULL; g.b[1] = NULL; g.partials = NULL;
+    k_dot_serial<<<1, 64, 0, S(stream)>>>(g, (double *)out);
+    HIP_OK(hipGetLastError());
+    return 0;
+  }
   long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
   if (nch > g_dot_cap) {
     if (g_dot_partials) hipFree(g_dot_partials);
@@ -495,7 +548,7 @@ __global__ __launch_bounds__(RED_T) void k_bicg_init(long n, const z2 *b, const 
 __global__ __launch_bounds__(RED_T) void k_bicg_fin_init(const double *partials, long nch, double *sc)
 {
   __shared__ double sh[RED_T];
-  final_fold<2>(partials, nch, sc + SC_DOT, sh);
+  if (nch >= 0) final_fold<2>(partials, nch, sc + SC_DOT, sh);  // nch < 0: dots already in sc
   if (threadIdx.x == 0) {
     sc[SC_RHO] = sc[SC_DOT];
     sc[SC_RHO + 1] = sc[SC_DOT + 1];
@@ -511,7 +564,7 @@ __global__ __launch_bounds__(RED_T) void k_bicg_fin_init(const double *partials,
 __global__ __launch_bounds__(RED_T) void k_bicg_fin_alpha(const double *partials, long nch, double *sc)
 {
   __shared__ double sh[RED_T];
-  final_fold<1>(partials, nch, sc + SC_DOT, sh);
+  if (nch >= 0) final_fold<1>(partials, nch, sc + SC_DOT, sh);  // nch < 0: dots already in sc
   if (threadIdx.x == 0) {
     z2 al = zdiv(zmk(sc[SC_RHO], sc[SC_RHO + 1]), zmk(sc[SC_DOT], sc[SC_DOT + 1]));
     sc[SC_ALPHA] = al.x;
@@ -523,7 +576,7 @@ __global__ __launch_bounds__(RED_T) void k_bicg_fin_alpha(const double *partials
 __global__ __launch_bounds__(RED_T) void k_bicg_fin_omega(const double *partials, long nch, double *sc)
 {
   __shared__ double sh[RED_T];
-  final_fold<2>(partials, nch, sc + SC_DOT, sh);
+  if (nch >= 0) final_fold<2>(partials, nch, sc + SC_DOT, sh);  // nch < 0: dots already in sc
   if (threadIdx.x == 0) {
     double tt = sc[SC_DOT + 2];
     sc[SC_OMEGA] = sc[SC_DOT] / tt;
@@ -536,7 +589,7 @@ __global__ __launch_bounds__(RED_T) void k_bicg_fin_omega(const double *partials
 __global__ __launch_bounds__(RED_T) void k_bicg_fin_conv(const double *partials, long nch, double *sc)
 {
   __shared__ double sh[RED_T];
-  final_fold<2>(partials, nch, sc + SC_DOT, sh);
+  if (nch >= 0) final_fold<2>(partials, nch, sc + SC_DOT, sh);  // nch < 0: dots already in sc
   if (threadIdx.x == 0) {
     double rs = sc[SC_DOT];
     double relres = sqrt(rs / sc[SC_RS0]);
@@ -603,6 +656,18 @@ __global__ void k_bicg_p(long n, z2 *p, const z2 *r, const z2 *v, const double *
   p[e] = zadd(r[e], zmul(be, zsub(p[e], zmul(om, v[e]))));
 }
 
+// In the reference summation order the dot products go straight into the
+// scalar block and the final kernels skip their fold (chunk count -1).
+static long serial_dots(long n, const z2 *a0, const z2 *b0, const z2 *a1, const z2 *b1, double *sc, hipStream_t s,
+                        long nch)
+{
+  if (!g_red_serial) return nch;
+  DotArgs g;
+  g.n = n; g.a[0] = a0; g.b[0] = b0; g.a[1] = a1; g.b[1] = b1; g.partials = NULL;
+  k_dot_serial<<<a1 ? 2 : 1, 64, 0, s>>>(g, sc + SC_DOT);
+  return -1;
+}
+
 static int ensure_kv(csem_model *m, int nv)
 {
   if (m->nkv >= nv) return 0;
@@ -637,7 +702,7 @@ extern "C" int csem_bicgstab(csem_model *m, const csem_freq *f, void *xv, const 
   HIP_OK(hipMemcpyAsync(sc + SC_TOL, &tol, sizeof(double), hipMemcpyHostToDevice, s));
   if ((rc = run_apply(m, f, x, v, s))) return rc;
   k_bicg_init<<<(unsigned)nch, RED_T, 0, s>>>(n, b, v, r, p, r0, pa);
-  k_bicg_fin_init<<<1, RED_T, 0, s>>>(pa, nch, sc);
+  k_bicg_fin_init<<<1, RED_T, 0, s>>>(pa, serial_dots(n, r, r0, r, r, sc, s, nch), sc);
   HIP_OK(hipMemcpyAsync(hs, sc, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   if (hs[SC_RS] == 0.) return 0;
@@ -652,23 +717,24 @@ extern "C" int csem_bicgstab(csem_model *m, const csem_freq *f, void *xv, const 
       pin = pp;
     }
     if ((rc = run_apply(m, f, pin, v, s))) return rc;
-    if ((rc = dot_partial_launch(n, v, r0, NULL, NULL, pa, s))) return rc;
-    k_bicg_fin_alpha<<<1, RED_T, 0, s>>>(pa, nch, sc);
+    if (!g_red_serial && (rc = dot_partial_launch(n, v, r0, NULL, NULL, pa, s))) return rc;
+    k_bicg_fin_alpha<<<1, RED_T, 0, s>>>(pa, serial_dots(n, v, r0, NULL, NULL, sc, s, nch), sc);
     KT(KT_BICG_S, s, 48.0 * n, (k_bicg_s<<<ge, 256, 0, s>>>(n, r, v, sv, sc)));
     const z2 *xb;
     if (precond) {
       if ((rc = run_lusgs(m, f, sv, r, s))) return rc;
       if ((rc = run_apply(m, f, r, t, s))) return rc;
-      if ((rc = dot_partial_launch(n, sv, t, t, t, pa, s))) return rc;
+      if (!g_red_serial && (rc = dot_partial_launch(n, sv, t, t, t, pa, s))) return rc;
       xb = r;
     } else {
       if ((rc = run_apply(m, f, sv, t, s))) return rc;
-      if ((rc = dot_partial_launch(n, t, sv, t, t, pa, s))) return rc;
+      if (!g_red_serial && (rc = dot_partial_launch(n, t, sv, t, t, pa, s))) return rc;
       xb = sv;
     }
-    k_bicg_fin_omega<<<1, RED_T, 0, s>>>(pa, nch, sc);
+    const long nom = precond ? serial_dots(n, sv, t, t, t, sc, s, nch) : serial_dots(n, t, sv, t, t, sc, s, nch);
+    k_bicg_fin_omega<<<1, RED_T, 0, s>>>(pa, nom, sc);
     KT(KT_BICG_XR, s, 128.0 * n, (k_bicg_xr<<<(unsigned)nch, RED_T, 0, s>>>(n, x, pin, xb, sv, t, r, r0, sc, pa)));
-    k_bicg_fin_conv<<<1, RED_T, 0, s>>>(pa, nch, sc);
+    k_bicg_fin_conv<<<1, RED_T, 0, s>>>(pa, serial_dots(n, r, r, r, r0, sc, s, nch), sc);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(hs, sc, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));

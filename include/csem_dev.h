@@ -75,6 +75,13 @@ int csem_fft2(csem_model *m, void *data, int dir, void *stream);
  * device pointer to two doubles. */
 int csem_dot(long n, const void *a, const void *b, void *out, void *stream);
 
+/* Summation order of every dot product (csem_dot and the solvers):
+ * 0 (default) the fixed chunked parallel order; 1 the reference's serial
+ * left-to-right order (cdotprod.c:18), one dependent add per element on a
+ * single wavefront -- slow, for end-to-end bitwise comparison with the
+ * reference. */
+int csem_set_reduction(int serial);
+
 /* Device-resident solves of A x = b.  x holds the initial guess on entry.
  * precond: 0 none, 1 LU-SGS.  Returns the iteration index at convergence
  * (as printed by "converged at iter=", solver.c:160) or niter, or < 0 on

@@ -143,7 +143,7 @@ class Oracle:
         lib().orc_lusgs_apply(self.h, _p(x), _p(y))
         return y
 
-    def airbc(self, hz):
+    def airbc_apply(self, hz):
         hz = np.ascontiguousarray(hz, np.complex128)
         hx = np.zeros((self.n1 + 1) * self.n2, np.complex128)
         hy = np.zeros(self.n1 * (self.n2 + 1), np.complex128)
