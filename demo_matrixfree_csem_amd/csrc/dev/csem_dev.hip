@@ -190,7 +190,7 @@ static int set_lds_attrs()
 extern "C" void csem_model_destroy(csem_model *m)
 {
   if (!m) return;
-  for (void *p : m->allocs) hipFree(p);
+  for (void *p : m->allocs) (void)hipFree(p);
   delete m;
 }
 
@@ -199,7 +199,8 @@ extern "C" int csem_model_create(const csem_model_desc *d, csem_model **out)
   *out = NULL;
   if (d->n1 < 2 || d->n2 < 2 || d->n3f < 2) return csem_fail(-3, "grid too small: %d %d %d", d->n1, d->n2, d->n3f);
   if (d->airbc && d->nb < 1) return csem_fail(-3, "airbc=1 needs nb>0");
-  if (d->n3f - 1 > 1024 || d->n2 - 1 > 1024) return csem_fail(-3, "sweep rows exceed 1024 threads");
+  if (d->n3f - 1 > SGS_MAXT || d->n2 - 1 > SGS_MAXT)
+    return csem_fail(-3, "LU-SGS slices hold at most %d rows (n3f-1=%d, n2-1=%d)", SGS_MAXT, d->n3f - 1, d->n2 - 1);
   csem_model *m = new csem_model();
   int rc;
   m->n1 = d->n1; m->n2 = d->n2; m->n3f = d->n3f; m->nb = d->nb; m->airbc = d->airbc; m->fvm = d->fvm;
@@ -352,7 +353,10 @@ static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   a.fEx = m->fEx; a.fEy = m->fEy; a.lEx = m->lEx; a.lEy = m->lEy;
   const int tk = ((m->n3f - 1 + 63) / 64) * 64;  // rows k = 1..n3f-1
   const int tj = ((m->n2 - 1 + 63) / 64) * 64;   // rows j = 1..n2-1
-  const size_t lk = 2 * tk * sizeof(z2), lj = 2 * tj * sizeof(z2);
+  // exchange slots + the fast axis' spacings (d2s, d2 for Ex; d1s, d1 for Ey, Ez)
+  const size_t lk = 2 * tk * sizeof(z2) + 2 * (size_t)(m->n2 + 1) * sizeof(double);
+  const size_t lky = 2 * tk * sizeof(z2) + 2 * (size_t)(m->n1 + 1) * sizeof(double);
+  const size_t lj = 2 * tj * sizeof(z2) + 2 * (size_t)(m->n1 + 1) * sizeof(double);
   // algorithmic bytes per swept cell: forward Ex 40 (b, sigma, write),
   // Ey 56 (+Ex), Ez 72 (+Ex, Ey); backward 88 (b, sigma, old value, the
   // two other components, write)
@@ -360,10 +364,10 @@ static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   const double cy = (double)(m->n1 - 1) * m->n2 * (m->n3f - 1);
   const double cz = (double)(m->n1 - 1) * (m->n2 - 1) * m->n3f;
   KT(KT_SGS_EX_F, s, 40 * cx, (k_sgs_ex<true><<<m->n1, tk, lk, s>>>(a)));
-  KT(KT_SGS_EY_F, s, 56 * cy, (k_sgs_ey<true><<<m->n2, tk, lk, s>>>(a)));
+  KT(KT_SGS_EY_F, s, 56 * cy, (k_sgs_ey<true><<<m->n2, tk, lky, s>>>(a)));
   KT(KT_SGS_EZ_F, s, 72 * cz, (k_sgs_ez<true><<<m->n3f, tj, lj, s>>>(a)));
   KT(KT_SGS_EZ_B, s, 88 * cz, (k_sgs_ez<false><<<m->n3f, tj, lj, s>>>(a)));
-  KT(KT_SGS_EY_B, s, 88 * cy, (k_sgs_ey<false><<<m->n2, tk, lk, s>>>(a)));
+  KT(KT_SGS_EY_B, s, 88 * cy, (k_sgs_ey<false><<<m->n2, tk, lky, s>>>(a)));
   KT(KT_SGS_EX_B, s, 88 * cx, (k_sgs_ex<false><<<m->n1, tk, lk, s>>>(a)));
   HIP_OK(hipGetLastError());
   return 0;
@@ -509,7 +513,7 @@ extern "C" int csem_dot(long n, const void *a, const void *b, void *out, void *s
   }
   long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
   if (nch > g_dot_cap) {
-    if (g_dot_partials) hipFree(g_dot_partials);
+    if (g_dot_partials) (void)hipFree(g_dot_partials);
     g_dot_cap = 0;
     HIP_OK(hipMalloc((void **)&g_dot_partials, nch * 4 * sizeof(double)));
     g_dot_cap = nch;
@@ -672,7 +676,7 @@ static int ensure_kv(csem_model *m, int nv)
 {
   if (m->nkv >= nv) return 0;
   if (m->kv) {
-    hipFree(m->kv);
+    (void)hipFree(m->kv);
     m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void *)m->kv), m->allocs.end());
     m->bytes -= m->nkv * m->lE * (long)sizeof(z2);
     m->kv = NULL;
@@ -847,8 +851,8 @@ extern "C" int csem_scatter_add(long nnz, const long *idx, const double *val, vo
   k_scatter_add<<<1, 64, 0, S(stream)>>>(nnz, di, dv, (z2 *)b);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(S(stream)));
-  hipFree(di);
-  hipFree(dv);
+  (void)hipFree(di);
+  (void)hipFree(dv);
   return 0;
 }
 
@@ -871,7 +875,7 @@ extern "C" int csem_gather(int nrec, const long *idx, const double *wts, const d
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(out, blob + bi + bw + bs + bc, bo, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
-  hipFree(blob);
+  (void)hipFree(blob);
   return 0;
 }
 

@@ -248,11 +248,21 @@ __device__ __forceinline__ z2 cell_ez(z2 b, double sig, double d1s_i, double d1s
 // -- zero in a forward sweep, the forward result in a backward one (then
 // the roles of +1 and -1 swap).
 
+// Every sweep kernel software-pipelines its global loads: the inputs of a
+// cell that do not depend on the sweep's own progress (right-hand side,
+// conductivity, the other components, old values) are loaded P diagonals
+// ahead into a register ring, so the dependent chain of a diagonal waits
+// only for LDS and arithmetic, not for memory.  The 1-D spacings of the fast
+// axis are staged in LDS once.  Dynamic LDS: 2*blockDim complex exchange
+// slots, then two arrays of (nfast+1) doubles.
+#define SGS_P 4
+#define SGS_MAXT 256  // rows per slice: n3f-1 and n2-1 must not exceed this
+
 // Ex sweep, slice i = blockIdx.x, s = k, f = j.
 template <bool FWD>
-__global__ void k_sgs_ex(LuArgs a)
+__global__ __launch_bounds__(SGS_MAXT) void k_sgs_ex(LuArgs a)
 {
-  extern __shared__ z2 xch[];  // 2 * blockDim.x
+  extern __shared__ z2 xch[];
   const int n1 = a.n1, n2 = a.n2, n3 = a.n3f;
   const int i = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
   const int k = 1 + t;
@@ -260,61 +270,82 @@ __global__ void k_sgs_ex(LuArgs a)
   const z2 *Wx = a.W, *Wy = a.W + a.fEx, *Wz = a.W + a.fEx + a.fEy;
   z2 *Wxw = a.W;
   const z2 *bx = a.x;
+  double *cd2s = (double *)(xch + 2 * nt), *cd2 = cd2s + (n2 + 1);
+  for (int q = t; q <= n2; q += nt) { cd2s[q] = a.d2s[q]; cd2[q] = a.d2[q]; }
   const double d1s_i = a.d1s[i];
   double d3s_k = 0, d3s_km = 0, d3_k = 0;
   if (krow) { d3s_k = a.d3s[k]; d3s_km = a.d3s[k - 1]; d3_k = a.d3[k]; }
-  const int dlo = 2, dhi = (n3 - 1) + (n2 - 1);
-  z2 prev = zmk(0, 0);
-  for (int st = 0; st <= dhi - dlo; st++) {
-    const int d = FWD ? dlo + st : dhi - st;
-    const int j = d - k;
-    z2 nbv = zmk(0, 0);
-    if (st > 0) {
-      if (FWD) { if (t > 0) nbv = xch[((st - 1) & 1) * nt + t - 1]; }
-      else { if (k + 1 <= n3 - 1) nbv = xch[((st - 1) & 1) * nt + t + 1]; }
+  const int dlo = 2, dhi = (n3 - 1) + (n2 - 1), nst = dhi - dlo + 1;
+  const int kop = k - a.kofs;
+  struct St { z2 b, ey_ip, ey_i, ey_jm_ip, ey_jm_i, ez_ip, ez_i, ez_km_ip, ez_km_i, ex_jm, ex_km; double sig; };
+  auto load = [&](St &c, int st) {
+    const int j = (FWD ? dlo + st : dhi - st) - k;
+    if (!(st < nst && krow && j >= 1 && j <= n2 - 1)) return;
+    c.b = (kop >= 0) ? bx[id1(n1, n2, kop, j, i)] : zmk(0, 0);
+    c.sig = a.s11[id1(n1, n2, k, j, i)];
+    if (!FWD) {
+      c.ex_jm = Wx[id1(n1, n2, k, j - 1, i)];
+      c.ex_km = Wx[id1(n1, n2, k - 1, j, i)];
+      c.ey_ip = Wy[id2(n1, n2, k, j, i + 1)]; c.ey_i = Wy[id2(n1, n2, k, j, i)];
+      c.ey_jm_ip = Wy[id2(n1, n2, k, j - 1, i + 1)]; c.ey_jm_i = Wy[id2(n1, n2, k, j - 1, i)];
+      c.ez_ip = Wz[id3(n1, n2, k, j, i + 1)]; c.ez_i = Wz[id3(n1, n2, k, j, i)];
+      c.ez_km_ip = Wz[id3(n1, n2, k - 1, j, i + 1)]; c.ez_km_i = Wz[id3(n1, n2, k - 1, j, i)];
     }
-    if (krow && j >= 1 && j <= n2 - 1) {
-      const int kop = k - a.kofs;
-      z2 b = (kop >= 0) ? bx[id1(n1, n2, kop, j, i)] : zmk(0, 0);
-      double sig = a.s11[id1(n1, n2, k, j, i)];
-      z2 ey_ip = zmk(0, 0), ey_i = zmk(0, 0), ey_jm_ip = zmk(0, 0), ey_jm_i = zmk(0, 0);
-      z2 ez_ip = zmk(0, 0), ez_i = zmk(0, 0), ez_km_ip = zmk(0, 0), ez_km_i = zmk(0, 0);
-      z2 ex_jp, ex_jm, ex_kp, ex_km;
-      if (FWD) {
-        ex_jm = prev; ex_km = nbv; ex_jp = zmk(0, 0); ex_kp = zmk(0, 0);
-      } else {
-        ex_jp = prev; ex_kp = nbv;
-        ex_jm = Wx[id1(n1, n2, k, j - 1, i)];
-        ex_km = Wx[id1(n1, n2, k - 1, j, i)];
-        ey_ip = Wy[id2(n1, n2, k, j, i + 1)]; ey_i = Wy[id2(n1, n2, k, j, i)];
-        ey_jm_ip = Wy[id2(n1, n2, k, j - 1, i + 1)]; ey_jm_i = Wy[id2(n1, n2, k, j - 1, i)];
-        ez_ip = Wz[id3(n1, n2, k, j, i + 1)]; ez_i = Wz[id3(n1, n2, k, j, i)];
-        ez_km_ip = Wz[id3(n1, n2, k - 1, j, i + 1)]; ez_km_i = Wz[id3(n1, n2, k - 1, j, i)];
+  };
+  St ring[SGS_P];
+  const z2 z0 = zmk(0, 0);
+#pragma unroll
+  for (int q = 0; q < SGS_P; q++) {
+    ring[q].b = z0; ring[q].sig = 0;
+    ring[q].ey_ip = ring[q].ey_i = ring[q].ey_jm_ip = ring[q].ey_jm_i = z0;
+    ring[q].ez_ip = ring[q].ez_i = ring[q].ez_km_ip = ring[q].ez_km_i = z0;
+    ring[q].ex_jm = ring[q].ex_km = z0;
+    load(ring[q], q);
+  }
+  __syncthreads();
+  z2 prev = z0;
+  for (int st0 = 0; st0 < nst; st0 += SGS_P) {
+#pragma unroll
+    for (int q = 0; q < SGS_P; q++) {
+      const int st = st0 + q;
+      if (st < nst) {
+        const int j = (FWD ? dlo + st : dhi - st) - k;
+        z2 nbv = z0;
+        if (st > 0) {
+          if (FWD) { if (t > 0) nbv = xch[((st - 1) & 1) * nt + t - 1]; }
+          else { if (k + 1 <= n3 - 1) nbv = xch[((st - 1) & 1) * nt + t + 1]; }
+        }
+        if (krow && j >= 1 && j <= n2 - 1) {
+          St &c = ring[q];
+          const z2 ex_jp = FWD ? z0 : prev, ex_jm = FWD ? prev : c.ex_jm;
+          const z2 ex_kp = FWD ? z0 : nbv, ex_km = FWD ? nbv : c.ex_km;
+          const double d2s_j = cd2s[j], d2s_jm = cd2s[j - 1], d2_j = cd2[j];
+          const double vol = a.fvm ? d1s_i * d2_j * d3_k : 0.0;
+          z2 v = cell_ex(c.b, c.sig, d1s_i, d2s_j, d2s_jm, d2_j, d3s_k, d3s_km, d3_k, c.ey_ip, c.ey_i, c.ey_jm_ip,
+                         c.ey_jm_i, ex_jp, ex_jm, ex_kp, ex_km, c.ez_ip, c.ez_i, c.ez_km_ip, c.ez_km_i, a.iwmu, a.fvm,
+                         a.iomu, vol);
+          if (FWD) Wxw[id1(n1, n2, k, j, i)] = v;
+          else if (kop >= 0) a.y[id1(n1, n2, kop, j, i)] = v;
+          prev = v;
+        }
+        load(ring[q], st + SGS_P);
+        xch[(st & 1) * nt + t] = prev;
+        __syncthreads();
       }
-      const double d2s_j = a.d2s[j], d2s_jm = a.d2s[j - 1], d2_j = a.d2[j];
-      double vol = a.fvm ? d1s_i * d2_j * d3_k : 0.0;
-      z2 v = cell_ex(b, sig, d1s_i, d2s_j, d2s_jm, d2_j, d3s_k, d3s_km, d3_k, ey_ip, ey_i, ey_jm_ip,
-                     ey_jm_i, ex_jp, ex_jm, ex_kp, ex_km, ez_ip, ez_i, ez_km_ip, ez_km_i, a.iwmu, a.fvm,
-                     a.iomu, vol);
-      if (FWD) Wxw[id1(n1, n2, k, j, i)] = v;
-      else if (kop >= 0) a.y[id1(n1, n2, kop, j, i)] = v;
-      prev = v;
     }
-    xch[(st & 1) * nt + t] = prev;
-    __syncthreads();
   }
   if (!FWD) {  // cells of this slice outside the sweep range are zero rows
     const int nk = a.n3op + 1;
     for (int q = t; q < nk * (n2 + 1); q += nt) {
-      int kop = q / (n2 + 1), j = q % (n2 + 1), kf = kop + a.kofs;
-      if (!(kf >= 1 && kf <= n3 - 1 && j >= 1 && j <= n2 - 1)) a.y[id1(n1, n2, kop, j, i)] = zmk(0, 0);
+      int kq = q / (n2 + 1), j = q % (n2 + 1), kf = kq + a.kofs;
+      if (!(kf >= 1 && kf <= n3 - 1 && j >= 1 && j <= n2 - 1)) a.y[id1(n1, n2, kq, j, i)] = zmk(0, 0);
     }
   }
 }
 
 // Ey sweep, slice j = blockIdx.x, s = k, f = i.
 template <bool FWD>
-__global__ void k_sgs_ey(LuArgs a)
+__global__ __launch_bounds__(SGS_MAXT) void k_sgs_ey(LuArgs a)
 {
   extern __shared__ z2 xch[];
   const int n1 = a.n1, n2 = a.n2, n3 = a.n3f;
@@ -325,61 +356,82 @@ __global__ void k_sgs_ey(LuArgs a)
   z2 *Wyw = a.W + a.fEx;
   const z2 *by = a.x + a.lEx;
   z2 *yy = a.y + a.lEx;
-  const double d2s_j = a.d2s[j];
+  double *cd1s = (double *)(xch + 2 * nt), *cd1 = cd1s + (n1 + 1);
+  for (int q = t; q <= n1; q += nt) { cd1s[q] = a.d1s[q]; cd1[q] = a.d1[q]; }
+  const double d2s_j = a.d2s[j], d2_j = a.d2[j];
   double d3s_k = 0, d3s_km = 0, d3_k = 0;
   if (krow) { d3s_k = a.d3s[k]; d3s_km = a.d3s[k - 1]; d3_k = a.d3[k]; }
-  const double d2_j = a.d2[j];
-  const int dlo = 2, dhi = (n3 - 1) + (n1 - 1);
-  z2 prev = zmk(0, 0);
-  for (int st = 0; st <= dhi - dlo; st++) {
-    const int d = FWD ? dlo + st : dhi - st;
-    const int i = d - k;
-    z2 nbv = zmk(0, 0);
-    if (st > 0) {
-      if (FWD) { if (t > 0) nbv = xch[((st - 1) & 1) * nt + t - 1]; }
-      else { if (k + 1 <= n3 - 1) nbv = xch[((st - 1) & 1) * nt + t + 1]; }
+  const int dlo = 2, dhi = (n3 - 1) + (n1 - 1), nst = dhi - dlo + 1;
+  const int kop = k - a.kofs;
+  struct St { z2 b, ez_jp, ez_j, ez_km_jp, ez_km_j, ey_im, ey_km, ex_jp, ex_j, ex_jp_im, ex_j_im; double sig; };
+  auto load = [&](St &c, int st) {
+    const int i = (FWD ? dlo + st : dhi - st) - k;
+    if (!(st < nst && krow && i >= 1 && i <= n1 - 1)) return;
+    c.b = (kop >= 0) ? by[id2(n1, n2, kop, j, i)] : zmk(0, 0);
+    c.sig = a.s22[id2(n1, n2, k, j, i)];
+    if (!FWD) {
+      c.ey_im = Wy[id2(n1, n2, k, j, i - 1)];
+      c.ey_km = Wy[id2(n1, n2, k - 1, j, i)];
+      c.ez_jp = Wz[id3(n1, n2, k, j + 1, i)]; c.ez_j = Wz[id3(n1, n2, k, j, i)];
+      c.ez_km_jp = Wz[id3(n1, n2, k - 1, j + 1, i)]; c.ez_km_j = Wz[id3(n1, n2, k - 1, j, i)];
     }
-    if (krow && i >= 1 && i <= n1 - 1) {
-      const int kop = k - a.kofs;
-      z2 b = (kop >= 0) ? by[id2(n1, n2, kop, j, i)] : zmk(0, 0);
-      double sig = a.s22[id2(n1, n2, k, j, i)];
-      z2 ez_jp = zmk(0, 0), ez_j = zmk(0, 0), ez_km_jp = zmk(0, 0), ez_km_j = zmk(0, 0);
-      z2 ey_kp, ey_km, ey_ip, ey_im;
-      if (FWD) {
-        ey_im = prev; ey_km = nbv; ey_ip = zmk(0, 0); ey_kp = zmk(0, 0);
-      } else {
-        ey_ip = prev; ey_kp = nbv;
-        ey_im = Wy[id2(n1, n2, k, j, i - 1)];
-        ey_km = Wy[id2(n1, n2, k - 1, j, i)];
-        ez_jp = Wz[id3(n1, n2, k, j + 1, i)]; ez_j = Wz[id3(n1, n2, k, j, i)];
-        ez_km_jp = Wz[id3(n1, n2, k - 1, j + 1, i)]; ez_km_j = Wz[id3(n1, n2, k - 1, j, i)];
+    c.ex_jp = Wx[id1(n1, n2, k, j + 1, i)]; c.ex_j = Wx[id1(n1, n2, k, j, i)];
+    c.ex_jp_im = Wx[id1(n1, n2, k, j + 1, i - 1)]; c.ex_j_im = Wx[id1(n1, n2, k, j, i - 1)];
+  };
+  St ring[SGS_P];
+  const z2 z0 = zmk(0, 0);
+#pragma unroll
+  for (int q = 0; q < SGS_P; q++) {
+    ring[q].b = z0; ring[q].sig = 0;
+    ring[q].ez_jp = ring[q].ez_j = ring[q].ez_km_jp = ring[q].ez_km_j = z0;
+    ring[q].ey_im = ring[q].ey_km = z0;
+    ring[q].ex_jp = ring[q].ex_j = ring[q].ex_jp_im = ring[q].ex_j_im = z0;
+    load(ring[q], q);
+  }
+  __syncthreads();
+  z2 prev = z0;
+  for (int st0 = 0; st0 < nst; st0 += SGS_P) {
+#pragma unroll
+    for (int q = 0; q < SGS_P; q++) {
+      const int st = st0 + q;
+      if (st < nst) {
+        const int i = (FWD ? dlo + st : dhi - st) - k;
+        z2 nbv = z0;
+        if (st > 0) {
+          if (FWD) { if (t > 0) nbv = xch[((st - 1) & 1) * nt + t - 1]; }
+          else { if (k + 1 <= n3 - 1) nbv = xch[((st - 1) & 1) * nt + t + 1]; }
+        }
+        if (krow && i >= 1 && i <= n1 - 1) {
+          St &c = ring[q];
+          const z2 ey_ip = FWD ? z0 : prev, ey_im = FWD ? prev : c.ey_im;
+          const z2 ey_kp = FWD ? z0 : nbv, ey_km = FWD ? nbv : c.ey_km;
+          const double d1s_i = cd1s[i], d1s_im = cd1s[i - 1], d1_i = cd1[i];
+          const double vol = a.fvm ? d1s_i * d2_j * d3_k : 0.0;
+          z2 v = cell_ey(c.b, c.sig, d1s_i, d1s_im, d1_i, d2s_j, d3s_k, d3s_km, d3_k, c.ez_jp, c.ez_j, c.ez_km_jp,
+                         c.ez_km_j, ey_kp, ey_km, ey_ip, ey_im, c.ex_jp, c.ex_j, c.ex_jp_im, c.ex_j_im, a.iwmu,
+                         a.fvm, a.iomu, vol);
+          Wyw[id2(n1, n2, k, j, i)] = v;
+          if (!FWD && kop >= 0) yy[id2(n1, n2, kop, j, i)] = v;
+          prev = v;
+        }
+        load(ring[q], st + SGS_P);
+        xch[(st & 1) * nt + t] = prev;
+        __syncthreads();
       }
-      z2 ex_jp = Wx[id1(n1, n2, k, j + 1, i)], ex_j = Wx[id1(n1, n2, k, j, i)];
-      z2 ex_jp_im = Wx[id1(n1, n2, k, j + 1, i - 1)], ex_j_im = Wx[id1(n1, n2, k, j, i - 1)];
-      const double d1s_i = a.d1s[i], d1s_im = a.d1s[i - 1], d1_i = a.d1[i];
-      double vol = a.fvm ? d1s_i * d2_j * d3_k : 0.0;
-      z2 v = cell_ey(b, sig, d1s_i, d1s_im, d1_i, d2s_j, d3s_k, d3s_km, d3_k, ez_jp, ez_j, ez_km_jp,
-                     ez_km_j, ey_kp, ey_km, ey_ip, ey_im, ex_jp, ex_j, ex_jp_im, ex_j_im, a.iwmu, a.fvm,
-                     a.iomu, vol);
-      Wyw[id2(n1, n2, k, j, i)] = v;
-      if (!FWD && kop >= 0) yy[id2(n1, n2, kop, j, i)] = v;
-      prev = v;
     }
-    xch[(st & 1) * nt + t] = prev;
-    __syncthreads();
   }
   if (!FWD) {
     const int nk = a.n3op + 1;
     for (int q = t; q < nk * (n1 + 1); q += nt) {
-      int kop = q / (n1 + 1), i = q % (n1 + 1), kf = kop + a.kofs;
-      if (!(kf >= 1 && kf <= n3 - 1 && i >= 1 && i <= n1 - 1)) yy[id2(n1, n2, kop, j, i)] = zmk(0, 0);
+      int kq = q / (n1 + 1), i = q % (n1 + 1), kf = kq + a.kofs;
+      if (!(kf >= 1 && kf <= n3 - 1 && i >= 1 && i <= n1 - 1)) yy[id2(n1, n2, kq, j, i)] = zmk(0, 0);
     }
   }
 }
 
 // Ez sweep, slice k = blockIdx.x, s = j, f = i.
 template <bool FWD>
-__global__ void k_sgs_ez(LuArgs a)
+__global__ __launch_bounds__(SGS_MAXT) void k_sgs_ez(LuArgs a)
 {
   extern __shared__ z2 xch[];
   const int n1 = a.n1, n2 = a.n2;
@@ -391,45 +443,68 @@ __global__ void k_sgs_ez(LuArgs a)
   const int kop = k - a.kofs;
   const z2 *bz = a.x + a.lEx + a.lEy;
   z2 *yz = a.y + a.lEx + a.lEy;
+  double *cd1s = (double *)(xch + 2 * nt), *cd1 = cd1s + (n1 + 1);
+  for (int q = t; q <= n1; q += nt) { cd1s[q] = a.d1s[q]; cd1[q] = a.d1[q]; }
   const double d3s_k = a.d3s[k], d3_k = a.d3[k];
   double d2s_j = 0, d2s_jm = 0, d2_j = 0;
   if (jrow) { d2s_j = a.d2s[j]; d2s_jm = a.d2s[j - 1]; d2_j = a.d2[j]; }
-  const int dlo = 2, dhi = (n2 - 1) + (n1 - 1);
-  z2 prev = zmk(0, 0);
-  for (int st = 0; st <= dhi - dlo; st++) {
-    const int d = FWD ? dlo + st : dhi - st;
-    const int i = d - j;
-    z2 nbv = zmk(0, 0);
-    if (st > 0) {
-      if (FWD) { if (t > 0) nbv = xch[((st - 1) & 1) * nt + t - 1]; }
-      else { if (j + 1 <= n2 - 1) nbv = xch[((st - 1) & 1) * nt + t + 1]; }
+  const int dlo = 2, dhi = (n2 - 1) + (n1 - 1), nst = dhi - dlo + 1;
+  struct St { z2 b, ez_im, ez_jm, ex_kp, ex_k, ex_kp_im, ex_k_im, ey_kp, ey_k, ey_kp_jm, ey_k_jm; double sig; };
+  auto load = [&](St &c, int st) {
+    const int i = (FWD ? dlo + st : dhi - st) - j;
+    if (!(st < nst && jrow && i >= 1 && i <= n1 - 1)) return;
+    c.b = (kop >= 0) ? bz[id3(n1, n2, kop, j, i)] : zmk(0, 0);
+    c.sig = a.s33[id3(n1, n2, k, j, i)];
+    if (!FWD) {
+      c.ez_im = Wz[id3(n1, n2, k, j, i - 1)];
+      c.ez_jm = Wz[id3(n1, n2, k, j - 1, i)];
     }
-    if (jrow && i >= 1 && i <= n1 - 1) {
-      z2 b = (kop >= 0) ? bz[id3(n1, n2, kop, j, i)] : zmk(0, 0);
-      double sig = a.s33[id3(n1, n2, k, j, i)];
-      z2 ez_ip, ez_im, ez_jp, ez_jm;
-      if (FWD) {
-        ez_im = prev; ez_jm = nbv; ez_ip = zmk(0, 0); ez_jp = zmk(0, 0);
-      } else {
-        ez_ip = prev; ez_jp = nbv;
-        ez_im = Wz[id3(n1, n2, k, j, i - 1)];
-        ez_jm = Wz[id3(n1, n2, k, j - 1, i)];
+    c.ex_kp = Wx[id1(n1, n2, k + 1, j, i)]; c.ex_k = Wx[id1(n1, n2, k, j, i)];
+    c.ex_kp_im = Wx[id1(n1, n2, k + 1, j, i - 1)]; c.ex_k_im = Wx[id1(n1, n2, k, j, i - 1)];
+    c.ey_kp = Wy[id2(n1, n2, k + 1, j, i)]; c.ey_k = Wy[id2(n1, n2, k, j, i)];
+    c.ey_kp_jm = Wy[id2(n1, n2, k + 1, j - 1, i)]; c.ey_k_jm = Wy[id2(n1, n2, k, j - 1, i)];
+  };
+  St ring[SGS_P];
+  const z2 z0 = zmk(0, 0);
+#pragma unroll
+  for (int q = 0; q < SGS_P; q++) {
+    ring[q].b = z0; ring[q].sig = 0;
+    ring[q].ez_im = ring[q].ez_jm = z0;
+    ring[q].ex_kp = ring[q].ex_k = ring[q].ex_kp_im = ring[q].ex_k_im = z0;
+    ring[q].ey_kp = ring[q].ey_k = ring[q].ey_kp_jm = ring[q].ey_k_jm = z0;
+    load(ring[q], q);
+  }
+  __syncthreads();
+  z2 prev = z0;
+  for (int st0 = 0; st0 < nst; st0 += SGS_P) {
+#pragma unroll
+    for (int q = 0; q < SGS_P; q++) {
+      const int st = st0 + q;
+      if (st < nst) {
+        const int i = (FWD ? dlo + st : dhi - st) - j;
+        z2 nbv = z0;
+        if (st > 0) {
+          if (FWD) { if (t > 0) nbv = xch[((st - 1) & 1) * nt + t - 1]; }
+          else { if (j + 1 <= n2 - 1) nbv = xch[((st - 1) & 1) * nt + t + 1]; }
+        }
+        if (jrow && i >= 1 && i <= n1 - 1) {
+          St &c = ring[q];
+          const z2 ez_ip = FWD ? z0 : prev, ez_im = FWD ? prev : c.ez_im;
+          const z2 ez_jp = FWD ? z0 : nbv, ez_jm = FWD ? nbv : c.ez_jm;
+          const double d1s_i = cd1s[i], d1s_im = cd1s[i - 1], d1_i = cd1[i];
+          const double vol = a.fvm ? d1s_i * d2_j * d3_k : 0.0;
+          z2 v = cell_ez(c.b, c.sig, d1s_i, d1s_im, d1_i, d2s_j, d2s_jm, d2_j, d3s_k, c.ex_kp, c.ex_k, c.ex_kp_im,
+                         c.ex_k_im, ez_ip, ez_im, ez_jp, ez_jm, c.ey_kp, c.ey_k, c.ey_kp_jm, c.ey_k_jm, a.iwmu,
+                         a.fvm, a.iomu, vol);
+          Wzw[id3(n1, n2, k, j, i)] = v;
+          if (!FWD && kop >= 0) yz[id3(n1, n2, kop, j, i)] = v;
+          prev = v;
+        }
+        load(ring[q], st + SGS_P);
+        xch[(st & 1) * nt + t] = prev;
+        __syncthreads();
       }
-      z2 ex_kp = Wx[id1(n1, n2, k + 1, j, i)], ex_k = Wx[id1(n1, n2, k, j, i)];
-      z2 ex_kp_im = Wx[id1(n1, n2, k + 1, j, i - 1)], ex_k_im = Wx[id1(n1, n2, k, j, i - 1)];
-      z2 ey_kp = Wy[id2(n1, n2, k + 1, j, i)], ey_k = Wy[id2(n1, n2, k, j, i)];
-      z2 ey_kp_jm = Wy[id2(n1, n2, k + 1, j - 1, i)], ey_k_jm = Wy[id2(n1, n2, k, j - 1, i)];
-      const double d1s_i = a.d1s[i], d1s_im = a.d1s[i - 1], d1_i = a.d1[i];
-      double vol = a.fvm ? a.d1s[i] * d2_j * d3_k : 0.0;
-      z2 v = cell_ez(b, sig, d1s_i, d1s_im, d1_i, d2s_j, d2s_jm, d2_j, d3s_k, ex_kp, ex_k, ex_kp_im,
-                     ex_k_im, ez_ip, ez_im, ez_jp, ez_jm, ey_kp, ey_k, ey_kp_jm, ey_k_jm, a.iwmu, a.fvm,
-                     a.iomu, vol);
-      Wzw[id3(n1, n2, k, j, i)] = v;
-      if (!FWD && kop >= 0) yz[id3(n1, n2, kop, j, i)] = v;
-      prev = v;
     }
-    xch[(st & 1) * nt + t] = prev;
-    __syncthreads();
   }
   if (!FWD && kop >= 0) {
     for (int q = t; q < (n2 + 1) * (n1 + 1); q += nt) {
@@ -448,42 +523,61 @@ __global__ void k_sgs_ez(LuArgs a)
 // R-point DFT, outputs b[(j/Ns) R Ns + k + r Ns].
 // `a` and `b` are LDS buffers of `cols` interleaved transforms of length N
 // (element e of column c at e*cols + c).
-__device__ __forceinline__ void fft_lds(z2 *a, z2 *b, int N, int cols, const z2 *tw, int dir, z2 **out)
+template <int R>
+__device__ __forceinline__ void fft_pass(const z2 *a, z2 *b, int N, int Ns, int cols, const z2 *tw, int dir)
 {
   const int t = threadIdx.x, nt = blockDim.x;
+  const int nb = N / R;
+  for (int w = t; w < nb * cols; w += nt) {
+    const int c = w % cols, j = w / cols;
+    const int k = j % Ns;
+    const int o = (j / Ns) * R * Ns + k;
+    z2 u[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      u[r] = a[(j + r * nb) * cols + c];
+      if (r != 0 && k != 0) {
+        z2 tv = tw[(long)r * k * (N / (R * Ns))];
+        if (dir > 0) tv.y = -tv.y;
+        u[r] = zmul(u[r], tv);
+      }
+    }
+    if (R == 4) {
+      z2 a0 = zadd(u[0], u[2]), a1 = zsub(u[0], u[2]), a2 = zadd(u[1], u[R - 1]), a3 = zsub(u[1], u[R - 1]);
+      z2 bb = dir < 0 ? zmk(a3.y, -a3.x) : zmk(-a3.y, a3.x);
+      b[(o)*cols + c] = zadd(a0, a2);
+      b[(o + Ns) * cols + c] = zadd(a1, bb);
+      b[(o + 2 * Ns) * cols + c] = zsub(a0, a2);
+      b[(o + 3 * Ns) * cols + c] = zsub(a1, bb);
+    } else {
+      b[(o)*cols + c] = zadd(u[0], u[1]);
+      b[(o + Ns) * cols + c] = zsub(u[0], u[1]);
+    }
+  }
+}
+
+// Stockham autosort pass structure shared with the oracle: radix-4 passes
+// with span Ns = 1, 4, 16, ..., then one radix-2 pass if log2(N) is odd.
+// Pass (R, Ns): for j in [0, N/R): k = j % Ns, inputs u[r] = a[j + r N/R]
+// times tw[r k N/(R Ns)] (conjugated for the inverse; skipped when r*k==0),
+// R-point DFT, outputs b[(j/Ns) R Ns + k + r Ns].
+// `a` and `b` are LDS buffers of `cols` interleaved transforms of length N
+// (element e of column c at e*cols + c).  Returns the buffer holding the
+// result; every thread of the block must call it.
+__device__ __forceinline__ z2 *fft_lds(z2 *a, z2 *b, int N, int cols, const z2 *tw, int dir)
+{
   for (int Ns = 1; Ns < N;) {
-    const int R = ((N / Ns) % 4 == 0) ? 4 : 2;
-    const int nb = N / R;
-    for (int w = t; w < nb * cols; w += nt) {
-      const int c = w % cols, j = w / cols;
-      const int k = j % Ns;
-      const int o = (j / Ns) * R * Ns + k;
-      z2 u[4];
-      for (int r = 0; r < R; r++) {
-        u[r] = a[(j + r * nb) * cols + c];
-        if (r != 0 && k != 0) {
-          z2 tv = tw[(long)r * k * (N / (R * Ns))];
-          if (dir > 0) tv.y = -tv.y;
-          u[r] = zmul(u[r], tv);
-        }
-      }
-      if (R == 4) {
-        z2 a0 = zadd(u[0], u[2]), a1 = zsub(u[0], u[2]), a2 = zadd(u[1], u[3]), a3 = zsub(u[1], u[3]);
-        z2 bb = dir < 0 ? zmk(a3.y, -a3.x) : zmk(-a3.y, a3.x);
-        b[(o)*cols + c] = zadd(a0, a2);
-        b[(o + Ns) * cols + c] = zadd(a1, bb);
-        b[(o + 2 * Ns) * cols + c] = zsub(a0, a2);
-        b[(o + 3 * Ns) * cols + c] = zsub(a1, bb);
-      } else {
-        b[(o)*cols + c] = zadd(u[0], u[1]);
-        b[(o + Ns) * cols + c] = zsub(u[0], u[1]);
-      }
+    if ((N / Ns) % 4 == 0) {
+      fft_pass<4>(a, b, N, Ns, cols, tw, dir);
+      Ns *= 4;
+    } else {
+      fft_pass<2>(a, b, N, Ns, cols, tw, dir);
+      Ns *= 2;
     }
     __syncthreads();
     z2 *tmp = a; a = b; b = tmp;
-    Ns *= R;
   }
-  *out = a;
+  return a;
 }
 
 struct AirArgs {
@@ -523,8 +617,7 @@ __global__ void k_air_rows_fwd(AirArgs a)
     A[i1] = v;
   }
   __syncthreads();
-  z2 *res;
-  fft_lds(A, B, N, 1, a.tw, -1, &res);
+  z2 *res = fft_lds(A, B, N, 1, a.tw, -1);
   for (int i1 = threadIdx.x; i1 < N; i1 += blockDim.x) a.buf[i1 + (long)N * i2] = res[i1];
 }
 
@@ -542,8 +635,7 @@ __global__ void k_air_cols(AirArgs a)
     A[e] = a.buf[c0 + c + (long)N * i2];
   }
   __syncthreads();
-  z2 *res;
-  fft_lds(A, B, N, CW, a.tw, -1, &res);
+  z2 *res = fft_lds(A, B, N, CW, a.tw, -1);
   for (int e = threadIdx.x; e < N * CW; e += blockDim.x) S[e] = res[e];
   __syncthreads();
   for (int q = 0; q < 2; q++) {
@@ -554,7 +646,7 @@ __global__ void k_air_cols(AirArgs a)
       A[e] = zmul(S[e], mul[c0 + c + (long)N * i2]);
     }
     __syncthreads();
-    fft_lds(A, B, N, CW, a.tw, +1, &res);
+    res = fft_lds(A, B, N, CW, a.tw, +1);
     for (int e = threadIdx.x; e < N * CW; e += blockDim.x) {
       int c = e % CW, i2 = e / CW;
       dst[c0 + c + (long)N * i2] = res[e];
@@ -574,8 +666,7 @@ __global__ void k_air_rows_inv(AirArgs a)
   z2 *A = sm, *B = sm + N;
   for (int i1 = threadIdx.x; i1 < N; i1 += blockDim.x) A[i1] = buf[i1 + N * row];
   __syncthreads();
-  z2 *res;
-  fft_lds(A, B, N, 1, a.tw, +1, &res);
+  z2 *res = fft_lds(A, B, N, 1, a.tw, +1);
   for (int i1 = threadIdx.x; i1 < N; i1 += blockDim.x) buf[i1 + N * row] = res[i1];
 }
 
@@ -625,8 +716,7 @@ __global__ void k_fft_rows(z2 *data, int N, const z2 *tw, int dir)
   const long row = blockIdx.x;
   for (int i = threadIdx.x; i < N; i += blockDim.x) A[i] = data[i + N * row];
   __syncthreads();
-  z2 *res;
-  fft_lds(A, B, N, 1, tw, dir, &res);
+  z2 *res = fft_lds(A, B, N, 1, tw, dir);
   for (int i = threadIdx.x; i < N; i += blockDim.x) data[i + N * row] = res[i];
 }
 
@@ -638,8 +728,7 @@ __global__ void k_fft_cols(z2 *data, int N, const z2 *tw, int dir)
   const int c0 = blockIdx.x * CW;
   for (int e = threadIdx.x; e < N * CW; e += blockDim.x) A[e] = data[c0 + e % CW + (long)N * (e / CW)];
   __syncthreads();
-  z2 *res;
-  fft_lds(A, B, N, CW, tw, dir, &res);
+  z2 *res = fft_lds(A, B, N, CW, tw, dir);
   for (int e = threadIdx.x; e < N * CW; e += blockDim.x) data[c0 + e % CW + (long)N * (e / CW)] = res[e];
 }
 
