@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 typedef double2 z2;
+struct z2x4 { z2 a, b, c, d; };
 
 __host__ __device__ __forceinline__ z2 zmk(double x, double y) { return make_double2(x, y); }
 __host__ __device__ __forceinline__ z2 zadd(z2 a, z2 b) { return zmk(a.x + b.x, a.y + b.y); }
@@ -37,6 +38,21 @@ __host__ __device__ __forceinline__ z2 zdiv(z2 n, z2 q)
   denom = (d * ratio) + c;
   return zmk(((b * ratio) + a) / denom, (b - (a * ratio)) / denom);
 }
+
+// x / d for d > 0 given rd = RN(1/d): q = RN(x*rd) is within an ulp of
+// x/d, the residual d*q - x is exact in an FMA, and one correction step
+// RN(q - rn*rd) is the correctly rounded quotient (Markstein's theorem; it
+// is the final step of the hardware division sequence too).  With the
+// residual taken as d*q - x, a zero numerator keeps its sign (-0 -> -0,
+// +0 -> +0).  Equal to the IEEE quotient for finite operands whose residual
+// does not underflow (|x| above about 1e-290).
+__host__ __device__ __forceinline__ double ddiv(double x, double d, double rd)
+{
+  const double q = x * rd;
+  const double rn = __builtin_fma(d, q, -x);
+  return __builtin_fma(-rn, rd, q);
+}
+__host__ __device__ __forceinline__ z2 zdivq(z2 a, double d, double rd) { return zmk(ddiv(a.x, d, rd), ddiv(a.y, d, rd)); }
 
 // Single precision counterparts for the mixed-precision inner solver.
 typedef float2 c2;

@@ -32,12 +32,12 @@ extern "C" const char *csem_last_error(void) { return g_err; }
 enum {
   KT_APPLY = 0, KT_HZ0, KT_AIR_ROWS_FWD, KT_AIR_COLS, KT_AIR_ROWS_INV, KT_AIR_GHOST,
   KT_SGS_EX_F, KT_SGS_EY_F, KT_SGS_EZ_F, KT_SGS_EZ_B, KT_SGS_EY_B, KT_SGS_EX_B,
-  KT_DOT, KT_DOT_FIN, KT_BICG_S, KT_BICG_XR, KT_BICG_P, KT_N
+  KT_DOT, KT_DOT_FIN, KT_BICG_S, KT_BICG_XR, KT_BICG_P, KT_LU_PRE, KT_LU_SMITH, KT_N
 };
 static const char *KT_NAMES[KT_N] = {
   "k_apply", "k_hz0", "k_air_rows_fwd", "k_air_cols", "k_air_rows_inv", "k_air_ghost",
   "k_sgs_ex<fwd>", "k_sgs_ey<fwd>", "k_sgs_ez<fwd>", "k_sgs_ez<bwd>", "k_sgs_ey<bwd>", "k_sgs_ex<bwd>",
-  "k_dot_partial", "k_dot_final", "k_bicg_s", "k_bicg_xr", "k_bicg_p"};
+  "k_dot_partial", "k_dot_final", "k_bicg_s", "k_bicg_xr", "k_bicg_p", "k_lu_pre", "k_lu_smith"};
 
 struct KtPend {
   int id;
@@ -138,8 +138,16 @@ struct csem_model {
   int n1, n2, n3f, n3, nb, airbc, fvm, kofs, nfft, kc;
   long lEx, lEy, lEz, lE, fEx, fEy, fEz, fE;
   double *d1, *d2, *d3, *d1s, *d2s, *d3s;
+  double *r1, *r2, *r3, *r1s, *r2s, *r3s;  // RN(1/d) of the spacings
   double *s11, *s22, *s33;
-  z2 *W;
+  SkewGeo g;       // LU-SGS sweep geometry
+  long nsk;        // swept cells, all components
+  z2 *W;           // LU-SGS state on the full grid
+  LuRec *rec;      // LU-SGS pre-pass records of one component, skewed
+  z2 *trash;       // LU-SGS stores of lanes off the diagonal
+  double4 *smith;  // per-frequency diagonal, skewed Ex | Ey | Ez
+  csem_freq smith_f;
+  int smith_ok;
   z2 *sH1, *sH2, *tw, *buf, *bufx, *bufy, *hz0, *hxg, *hyg;
   int *jx, *jy, *rows, nrowx, nrowy;
   double *wx, *wy;
@@ -201,6 +209,18 @@ extern "C" int csem_model_create(const csem_model_desc *d, csem_model **out)
   if (d->airbc && d->nb < 1) return csem_fail(-3, "airbc=1 needs nb>0");
   if (d->n3f - 1 > SGS_MAXT || d->n2 - 1 > SGS_MAXT)
     return csem_fail(-3, "LU-SGS slices hold at most %d rows (n3f-1=%d, n2-1=%d)", SGS_MAXT, d->n3f - 1, d->n2 - 1);
+  {
+    // LU-SGS drops the reference's sigma*0 term, the identity for finite
+    // non-negative conductivity
+    const double *sg[3] = {d->sigma11, d->sigma22, d->sigma33};
+    const long ln[3] = {(long)d->n1 * (d->n2 + 1) * (d->n3f + 1), (long)(d->n1 + 1) * d->n2 * (d->n3f + 1),
+                        (long)(d->n1 + 1) * (d->n2 + 1) * d->n3f};
+    for (int c = 0; c < 3; c++)
+      for (long e = 0; e < ln[c]; e++)
+        if (!(sg[c][e] >= 0.0) || !std::isfinite(sg[c][e]))
+          return csem_fail(-3, "conductivity must be finite and non-negative (component %d, cell %ld: %g)", c, e,
+                           sg[c][e]);
+  }
   csem_model *m = new csem_model();
   int rc;
   m->n1 = d->n1; m->n2 = d->n2; m->n3f = d->n3f; m->nb = d->nb; m->airbc = d->airbc; m->fvm = d->fvm;
@@ -219,9 +239,36 @@ extern "C" int csem_model_create(const csem_model_desc *d, csem_model **out)
 #define UP(dst, src, cnt) if ((rc = upload(m, &m->dst, src, cnt))) goto fail
   UP(d1, d->d1, n1 + 1); UP(d2, d->d2, n2 + 1); UP(d3, d->d3, n3f + 1);
   UP(d1s, d->d1s, n1 + 1); UP(d2s, d->d2s, n2 + 1); UP(d3s, d->d3s, n3f + 1);
+  {
+    // correctly rounded reciprocals of every spacing, for ddiv
+    std::vector<double> t;
+    auto recip = [&](const double *v, int n) { t.assign(v, v + n); for (double &e : t) e = 1.0 / e; return t.data(); };
+    UP(r1, recip(d->d1, n1 + 1), n1 + 1); UP(r2, recip(d->d2, n2 + 1), n2 + 1); UP(r3, recip(d->d3, n3f + 1), n3f + 1);
+    UP(r1s, recip(d->d1s, n1 + 1), n1 + 1); UP(r2s, recip(d->d2s, n2 + 1), n2 + 1);
+    UP(r3s, recip(d->d3s, n3f + 1), n3f + 1);
+  }
   UP(s11, d->sigma11, m->fEx); UP(s22, d->sigma22, m->fEy); UP(s33, d->sigma33, m->fEz);
+  {
+    SkewGeo &g = m->g;
+    g.A[0] = n3f - 1; g.B[0] = n2 - 1; g.ns[0] = n1;
+    g.A[1] = n3f - 1; g.B[1] = n1 - 1; g.ns[1] = n2;
+    g.A[2] = n2 - 1;  g.B[2] = n1 - 1; g.ns[2] = n3f;
+    long off = 0, mx = 0;
+    for (int c = 0; c < 3; c++) {
+      const long nc = (long)g.ns[c] * g.A[c] * g.B[c];
+      g.base[c] = off;
+      off += nc;
+      mx = std::max(mx, nc);
+    }
+    m->nsk = off;
+    if ((rc = dalloc(m, &m->smith, off))) goto fail;
+    if ((rc = dalloc(m, &m->rec, mx))) goto fail;
+    if ((rc = dalloc(m, &m->trash, 2 * SGS_MAXT))) goto fail;
+  }
+  // cells outside every sweep range stay zero for the pre-passes
   if ((rc = dalloc(m, &m->W, m->fE))) goto fail;
   if (hipMemset(m->W, 0, m->fE * sizeof(z2)) != hipSuccess) { rc = csem_fail(-1, "memset W"); goto fail; }
+  m->smith_ok = 0;
   m->nch = (m->fE + RED_CHUNK - 1) / RED_CHUNK;
   if ((rc = dalloc(m, &m->partials, m->nch * 4))) goto fail;
   if ((rc = dalloc(m, &m->scal, 64))) goto fail;
@@ -287,6 +334,8 @@ static ApplyArgs apply_args(const csem_model *m, const csem_freq *f, const z2 *x
   a.s33 = m->s33 + (long)ks * (n1 + 1) * (n2 + 1);
   a.d1 = m->d1; a.d2 = m->d2; a.d3 = m->d3 + ks;
   a.d1s = m->d1s; a.d2s = m->d2s; a.d3s = m->d3s + ks;
+  a.r1 = m->r1; a.r2 = m->r2; a.r3 = m->r3 + ks;
+  a.r1s = m->r1s; a.r2s = m->r2s; a.r3s = m->r3s + ks;
   a.hxg = m->hxg; a.hyg = m->hyg;
   a.iwmu = zmk(f->iwmu_re, f->iwmu_im);
   a.iomu = zmk(f->iomu_re, f->iomu_im);
@@ -334,9 +383,9 @@ static int run_apply(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
     int rc = run_airbc(m, s);
     if (rc) return rc;
   }
-  dim3 blk(64, 4);
-  dim3 grd((m->n1 + 1 + 63) / 64, (m->n2 + 1 + 3) / 4, (m->n3 + 1 + m->kc - 1) / m->kc);
-  KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<64, 4><<<grd, blk, 0, s>>>(a)));
+  dim3 blk(64, AP_TJ + 1);
+  dim3 grd((m->n1 + 1 + AP_TI - 1) / AP_TI, (m->n2 + 1 + AP_TJ - 1) / AP_TJ, (m->n3 + 1 + m->kc - 1) / m->kc);
+  KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<AP_TJ><<<grd, blk, 0, s>>>(a)));
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -344,31 +393,62 @@ static int run_apply(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
 static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStream_t s)
 {
   LuArgs a;
-  a.x = x; a.W = m->W; a.y = y;
+  a.x = x; a.y = y; a.W = m->W; a.rec = m->rec; a.trash = m->trash; a.smith = m->smith;
   a.s11 = m->s11; a.s22 = m->s22; a.s33 = m->s33;
   a.d1 = m->d1; a.d2 = m->d2; a.d3 = m->d3; a.d1s = m->d1s; a.d2s = m->d2s; a.d3s = m->d3s;
+  a.r1 = m->r1; a.r2 = m->r2; a.r3 = m->r3; a.r1s = m->r1s; a.r2s = m->r2s; a.r3s = m->r3s;
   a.iwmu = zmk(f->iwmu_re, f->iwmu_im);
   a.iomu = zmk(f->iomu_re, f->iomu_im);
   a.fvm = m->fvm; a.n1 = m->n1; a.n2 = m->n2; a.n3f = m->n3f; a.kofs = m->kofs; a.n3op = m->n3;
   a.fEx = m->fEx; a.fEy = m->fEy; a.lEx = m->lEx; a.lEy = m->lEy;
-  const int tk = ((m->n3f - 1 + 63) / 64) * 64;  // rows k = 1..n3f-1
-  const int tj = ((m->n2 - 1 + 63) / 64) * 64;   // rows j = 1..n2-1
-  // exchange slots + the fast axis' spacings (d2s, d2 for Ex; d1s, d1 for Ey, Ez)
-  const size_t lk = 2 * tk * sizeof(z2) + 2 * (size_t)(m->n2 + 1) * sizeof(double);
-  const size_t lky = 2 * tk * sizeof(z2) + 2 * (size_t)(m->n1 + 1) * sizeof(double);
-  const size_t lj = 2 * tj * sizeof(z2) + 2 * (size_t)(m->n1 + 1) * sizeof(double);
+  a.g = m->g;
+  const SkewGeo &g = m->g;
+  dim3 gp[3], gq[3];
+  for (int c = 0; c < 3; c++) gp[c] = dim3((unsigned)(((long)g.A[c] * g.B[c] + 255) / 256), (unsigned)g.ns[c]);
+  // pre-pass: the component's reference layout, 64 x 4 threads per block
+  gq[0] = dim3((m->n1 + 63) / 64, (m->n2 + 1 + 3) / 4, m->n3f + 1);
+  gq[1] = dim3((m->n1 + 1 + 63) / 64, (m->n2 + 3) / 4, m->n3f + 1);
+  gq[2] = dim3((m->n1 + 1 + 63) / 64, (m->n2 + 1 + 3) / 4, m->n3f);
+  const dim3 bq(64, 4);
+  if (!m->smith_ok || memcmp(&m->smith_f, f, sizeof *f)) {  // diagonal of this frequency
+    KT(KT_LU_SMITH, s, 48.0 * m->nsk, (k_lu_smith<0><<<gp[0], 256, 0, s>>>(a)));
+    KT(KT_LU_SMITH, s, 0.0, (k_lu_smith<1><<<gp[1], 256, 0, s>>>(a)));
+    KT(KT_LU_SMITH, s, 0.0, (k_lu_smith<2><<<gp[2], 256, 0, s>>>(a)));
+    m->smith_f = *f;
+    m->smith_ok = 1;
+  }
+  int tb[3];
+  size_t lds[3];
+  unsigned gs[3];
+  for (int c = 0; c < 3; c++) {
+    tb[c] = ((g.A[c] + 63) / 64) * 64;
+    // two exchange rows, the fast table and the diagonal table
+    lds[c] = 2 * (size_t)(tb[c] + 2) * sizeof(z2) + (size_t)(g.B[c] + 2) * (sizeof(double4) + sizeof(double)) +
+             (size_t)(g.A[c] + g.B[c] + 1) * sizeof(int);
+    gs[c] = 8 * ((g.ns[c] + 7) / 8);
+  }
   // algorithmic bytes per swept cell: forward Ex 40 (b, sigma, write),
   // Ey 56 (+Ex), Ez 72 (+Ex, Ey); backward 88 (b, sigma, old value, the
-  // two other components, write)
-  const double cx = (double)m->n1 * (m->n2 - 1) * (m->n3f - 1);
-  const double cy = (double)(m->n1 - 1) * m->n2 * (m->n3f - 1);
-  const double cz = (double)(m->n1 - 1) * (m->n2 - 1) * m->n3f;
-  KT(KT_SGS_EX_F, s, 40 * cx, (k_sgs_ex<true><<<m->n1, tk, lk, s>>>(a)));
-  KT(KT_SGS_EY_F, s, 56 * cy, (k_sgs_ey<true><<<m->n2, tk, lky, s>>>(a)));
-  KT(KT_SGS_EZ_F, s, 72 * cz, (k_sgs_ez<true><<<m->n3f, tj, lj, s>>>(a)));
-  KT(KT_SGS_EZ_B, s, 88 * cz, (k_sgs_ez<false><<<m->n3f, tj, lj, s>>>(a)));
-  KT(KT_SGS_EY_B, s, 88 * cy, (k_sgs_ey<false><<<m->n2, tk, lky, s>>>(a)));
-  KT(KT_SGS_EX_B, s, 88 * cx, (k_sgs_ex<false><<<m->n1, tk, lk, s>>>(a)));
+  // two other components, write).  The records and diagonals this
+  // schedule reads are the price of it and are not counted.
+  double nc[3];
+  for (int c = 0; c < 3; c++) nc[c] = (double)g.ns[c] * g.A[c] * g.B[c];
+  // the fvm scaling is a template argument: a runtime branch in the sweep
+  // loop would make the compiler drain the prefetched loads every step
+#define SWEEP(C, F) (m->fvm ? k_sgs<C, F, true> : k_sgs<C, F, false>)
+  KT(KT_LU_PRE, s, 0.0, (k_lu_pre<0, true><<<gq[0], bq, 0, s>>>(a)));
+  KT(KT_SGS_EX_F, s, 40 * nc[0], (SWEEP(0, true)<<<gs[0], tb[0], lds[0], s>>>(a)));
+  KT(KT_LU_PRE, s, 0.0, (k_lu_pre<1, true><<<gq[1], bq, 0, s>>>(a)));
+  KT(KT_SGS_EY_F, s, 56 * nc[1], (SWEEP(1, true)<<<gs[1], tb[1], lds[1], s>>>(a)));
+  KT(KT_LU_PRE, s, 0.0, (k_lu_pre<2, true><<<gq[2], bq, 0, s>>>(a)));
+  KT(KT_SGS_EZ_F, s, 72 * nc[2], (SWEEP(2, true)<<<gs[2], tb[2], lds[2], s>>>(a)));
+  KT(KT_LU_PRE, s, 0.0, (k_lu_pre<2, false><<<gq[2], bq, 0, s>>>(a)));
+  KT(KT_SGS_EZ_B, s, 88 * nc[2], (SWEEP(2, false)<<<gs[2], tb[2], lds[2], s>>>(a)));
+  KT(KT_LU_PRE, s, 0.0, (k_lu_pre<1, false><<<gq[1], bq, 0, s>>>(a)));
+  KT(KT_SGS_EY_B, s, 88 * nc[1], (SWEEP(1, false)<<<gs[1], tb[1], lds[1], s>>>(a)));
+  KT(KT_LU_PRE, s, 0.0, (k_lu_pre<0, false><<<gq[0], bq, 0, s>>>(a)));
+  KT(KT_SGS_EX_B, s, 88 * nc[0], (SWEEP(0, false)<<<gs[0], tb[0], lds[0], s>>>(a)));
+#undef SWEEP
   HIP_OK(hipGetLastError());
   return 0;
 }

@@ -13,104 +13,122 @@ struct ApplyArgs {
   z2 *y;
   const double *s11, *s22, *s33;  // operator k=0 plane (full grid offset by kstart)
   const double *d1, *d2, *d3, *d1s, *d2s, *d3s;  // d3, d3s offset by kstart
+  const double *r1, *r2, *r3, *r1s, *r2s, *r3s;  // RN(1/d) of the same arrays
   const z2 *hxg, *hyg;  // ghost planes under the airwave boundary
   z2 iwmu, iomu;
   int n1, n2, n3, fvm, airbc, kc;
   long lEx, lEy;
 };
 
-// y = A x (fdfd.c:86-205), one thread per (i, j) column marching over kc
-// planes.  H on the faces around each output is recomputed from E with the
-// expression of fdfd.c:133-145, so no H array is stored; Hx and Hy of the
-// previous plane are carried in registers.
-template <int TI, int TJ>
-__global__ __launch_bounds__(TI *TJ) void k_apply(ApplyArgs a)
+// y = A x (fdfd.c:86-205).  A block is AP_TJ+1 waves: wave r covers row
+// j = j0-1+r, lane l column i = i0-1+l, so wave 0 and lane 0 are a halo
+// that computes H for its neighbours and writes no output.  The block
+// marches over kc planes; H = curl E/(i w mu0) on the faces at (k, j, i)
+// (fdfd.c:133-145) is evaluated once per face, published in LDS (double
+// buffered, one barrier per plane) for the j+1 and i+1 neighbours, and
+// carried in registers to plane k+1.  Every division is by a 1-D spacing
+// and uses its precomputed reciprocal with one exact correction (ddiv), so
+// each value rounds as the reference's division does.
+#define AP_TJ 8
+#define AP_TI 63  // output columns per block (64 lanes, one halo)
+template <int TJ>
+__global__ __launch_bounds__(64 * (TJ + 1)) void k_apply(ApplyArgs a)
 {
-  const int i = blockIdx.x * TI + threadIdx.x;
-  const int j = blockIdx.y * TJ + threadIdx.y;
+  __shared__ z2 shx[2][TJ + 1][64], shy[2][TJ + 1][64], shz[2][TJ + 1][64];
+  const int l = threadIdx.x, r = threadIdx.y;
+  const int i = blockIdx.x * AP_TI - 1 + l;
+  const int j = blockIdx.y * TJ - 1 + r;
   const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
-  if (i > n1 || j > n2) return;
   const int k0 = blockIdx.z * a.kc;
   const int k1 = min(k0 + a.kc, n3 + 1);
+  const bool live = i >= 0 && i <= n1 && j >= 0 && j <= n2;
+  const bool out = live && l > 0 && r > 0;
+  const bool hx_ok = live && j < n2, hy_ok = live && i < n1, hz_ok = hy_ok && j < n2;
   const z2 *Ex = a.x, *Ey = a.x + a.lEx, *Ez = a.x + a.lEx + a.lEy;
   z2 *bx = a.y, *by = a.y + a.lEx, *bz = a.y + a.lEx + a.lEy;
-  const double *d1 = a.d1, *d2 = a.d2, *d3 = a.d3, *d1s = a.d1s, *d2s = a.d2s, *d3s = a.d3s;
-  const z2 iwmu = a.iwmu;
+  const z2 iwmu = a.iwmu, z0 = zmk(0, 0);
+  // 1-D coefficients of this column and row (clamped reads off the grid)
+  const int ic = min(max(i, 0), n1), jc = min(max(j, 0), n2);
+  const double d1s_i = a.d1s[ic], r1s_i = a.r1s[ic], d1_i = a.d1[ic], r1_i = a.r1[ic];
+  const double d2s_j = a.d2s[jc], r2s_j = a.r2s[jc], d2_j = a.d2[jc], r2_j = a.r2[jc];
+  // plane strides of the three layouts
+  const long px = (long)n1 * (n2 + 1), py = (long)(n1 + 1) * n2, pz = (long)(n1 + 1) * (n2 + 1);
+  const long ox = i + (long)n1 * j, oy = i + (long)(n1 + 1) * j, oz = i + (long)(n1 + 1) * j;
 
-#define HX(k, j, i)                                                                              \
-  zmul(zsub(zdivr(zsub(Ez[id3(n1, n2, k, (j) + 1, i)], Ez[id3(n1, n2, k, j, i)]), d2s[j]),       \
-            zdivr(zsub(Ey[id2(n1, n2, (k) + 1, j, i)], Ey[id2(n1, n2, k, j, i)]), d3s[k])),      \
-       iwmu)
-#define HY(k, j, i)                                                                              \
-  zmul(zsub(zdivr(zsub(Ex[id1(n1, n2, (k) + 1, j, i)], Ex[id1(n1, n2, k, j, i)]), d3s[k]),       \
-            zdivr(zsub(Ez[id3(n1, n2, k, j, (i) + 1)], Ez[id3(n1, n2, k, j, i)]), d1s[i])),      \
-       iwmu)
-#define HZ(k, j, i)                                                                              \
-  zmul(zsub(zdivr(zsub(Ey[id2(n1, n2, k, j, (i) + 1)], Ey[id2(n1, n2, k, j, i)]), d1s[i]),       \
-            zdivr(zsub(Ex[id1(n1, n2, k, (j) + 1, i)], Ex[id1(n1, n2, k, j, i)]), d2s[j])),      \
-       iwmu)
+  auto HX = [&](int k) {  // (Ez(j+1)-Ez(j))/d2s[j] - (Ey(k+1)-Ey(k))/d3s[k], times iwmu
+    const z2 *ez = Ez + k * pz + oz, *ey = Ey + k * py + oy;
+    return zmul(zsub(zdivq(zsub(ez[n1 + 1], ez[0]), d2s_j, r2s_j),
+                     zdivq(zsub(ey[py], ey[0]), a.d3s[k], a.r3s[k])), iwmu);
+  };
+  auto HY = [&](int k) {  // (Ex(k+1)-Ex(k))/d3s[k] - (Ez(i+1)-Ez(i))/d1s[i], times iwmu
+    const z2 *ex = Ex + k * px + ox, *ez = Ez + k * pz + oz;
+    return zmul(zsub(zdivq(zsub(ex[px], ex[0]), a.d3s[k], a.r3s[k]),
+                     zdivq(zsub(ez[1], ez[0]), d1s_i, r1s_i)), iwmu);
+  };
+  auto HZ = [&](int k) {  // (Ey(i+1)-Ey(i))/d1s[i] - (Ex(j+1)-Ex(j))/d2s[j], times iwmu
+    const z2 *ey = Ey + k * py + oy, *ex = Ex + k * px + ox;
+    return zmul(zsub(zdivq(zsub(ey[1], ey[0]), d1s_i, r1s_i),
+                     zdivq(zsub(ex[n1], ex[0]), d2s_j, r2s_j)), iwmu);
+  };
 
-  const bool iin = (i >= 1 && i <= n1 - 1), jin = (j >= 1 && j <= n2 - 1);
-  // Hx is needed where by/bz are formed (1<=i<=n1-1, j<n2), Hy where bx/bz
-  // are (i<n1, 1<=j<=n2-1); the previous plane's values ride in registers.
-  const bool need_hx = iin && j < n2, need_hy = jin && i < n1;
-  z2 hx_prev = zmk(0, 0), hy_prev = zmk(0, 0);
-  if (k0 > 0) {
-    if (need_hx) hx_prev = HX(k0 - 1, j, i);
-    if (need_hy) hy_prev = HY(k0 - 1, j, i);
+  z2 hx_prev = z0, hy_prev = z0;
+  if (k0 > 0 && k0 - 1 < n3) {
+    if (hx_ok) hx_prev = HX(k0 - 1);
+    if (hy_ok) hy_prev = HY(k0 - 1);
   }
+  const bool iin = i >= 1 && i <= n1 - 1, jin = j >= 1 && j <= n2 - 1;
   for (int k = k0; k < k1; k++) {
-    const bool kin = (k >= 1 && k <= n3 - 1) || (a.airbc && k == 0);
-    const bool bx_on = i < n1 && jin && kin;
-    const bool by_on = j < n2 && iin && kin;
-    const bool bz_on = k < n3 && iin && jin;
-    z2 hx_c = zmk(0, 0), hy_c = zmk(0, 0), hz_c = zmk(0, 0);
+    z2 hx = z0, hy = z0, hz = z0;
     if (k < n3) {
-      if (need_hx) hx_c = HX(k, j, i);
-      if (need_hy) hy_c = HY(k, j, i);
+      if (hx_ok) hx = HX(k);
+      if (hy_ok) hy = HY(k);
     }
-    if (bx_on || by_on) hz_c = HZ(k, j, i);
-    if (i < n1) {  // bx: i < n1, j <= n2, k <= n3
-      z2 v = zmk(0, 0);
-      if (bx_on) {
-        z2 t1 = zdivr(zsub(hz_c, HZ(k, j - 1, i)), d2[j]);
-        z2 hym = (k == 0) ? a.hyg[i + (long)n1 * j] : hy_prev;
-        z2 t2 = zdivr(zsub(hy_c, hym), d3[k]);
-        long q = id1(n1, n2, k, j, i);
-        v = zsub(zsub(t1, t2), zrmul(a.s11[q], Ex[q]));
-        if (a.fvm) v = zmul(v, zrmul(d1s[i] * d2[j] * d3[k], a.iomu));
+    if (hz_ok) hz = HZ(k);
+    const int q = k & 1;
+    shx[q][r][l] = hx; shy[q][r][l] = hy; shz[q][r][l] = hz;
+    __syncthreads();
+    if (out) {
+      const bool kin = (k >= 1 && k <= n3 - 1) || (a.airbc && k == 0);
+      const double d3_k = a.d3[k], r3_k = a.r3[k];
+      if (i < n1) {  // bx: i < n1, j <= n2, k <= n3
+        z2 v = z0;
+        const long c = k * px + ox;
+        if (jin && kin) {
+          z2 t1 = zdivq(zsub(hz, shz[q][r - 1][l]), d2_j, r2_j);
+          z2 hym = (k == 0) ? a.hyg[i + (long)n1 * j] : hy_prev;
+          z2 t2 = zdivq(zsub(hy, hym), d3_k, r3_k);
+          v = zsub(zsub(t1, t2), zrmul(a.s11[c], Ex[c]));
+          if (a.fvm) v = zmul(v, zrmul(d1s_i * d2_j * d3_k, a.iomu));
+        }
+        bx[c] = v;
       }
-      bx[id1(n1, n2, k, j, i)] = v;
-    }
-    if (j < n2) {  // by: i <= n1, j < n2, k <= n3
-      z2 v = zmk(0, 0);
-      if (by_on) {
-        z2 hxm = (k == 0) ? a.hxg[i + (long)(n1 + 1) * j] : hx_prev;
-        z2 t1 = zdivr(zsub(hx_c, hxm), d3[k]);
-        z2 t2 = zdivr(zsub(hz_c, HZ(k, j, i - 1)), d1[i]);
-        long q = id2(n1, n2, k, j, i);
-        v = zsub(zsub(t1, t2), zrmul(a.s22[q], Ey[q]));
-        if (a.fvm) v = zmul(v, zrmul(d1[i] * d2s[j] * d3[k], a.iomu));
+      if (j < n2) {  // by: i <= n1, j < n2, k <= n3
+        z2 v = z0;
+        const long c = k * py + oy;
+        if (iin && kin) {
+          z2 hxm = (k == 0) ? a.hxg[i + (long)(n1 + 1) * j] : hx_prev;
+          z2 t1 = zdivq(zsub(hx, hxm), d3_k, r3_k);
+          z2 t2 = zdivq(zsub(hz, shz[q][r][l - 1]), d1_i, r1_i);
+          v = zsub(zsub(t1, t2), zrmul(a.s22[c], Ey[c]));
+          if (a.fvm) v = zmul(v, zrmul(d1_i * d2s_j * d3_k, a.iomu));
+        }
+        by[c] = v;
       }
-      by[id2(n1, n2, k, j, i)] = v;
-    }
-    if (k < n3) {  // bz: i <= n1, j <= n2, k < n3
-      z2 v = zmk(0, 0);
-      if (bz_on) {
-        z2 t1 = zdivr(zsub(hy_c, HY(k, j, i - 1)), d1[i]);
-        z2 t2 = zdivr(zsub(hx_c, HX(k, j - 1, i)), d2[j]);
-        long q = id3(n1, n2, k, j, i);
-        v = zsub(zsub(t1, t2), zrmul(a.s33[q], Ez[q]));
-        if (a.fvm) v = zmul(v, zrmul(d1[i] * d2[j] * d3s[k], a.iomu));
+      if (k < n3) {  // bz: i <= n1, j <= n2, k < n3
+        z2 v = z0;
+        const long c = k * pz + oz;
+        if (iin && jin) {
+          z2 t1 = zdivq(zsub(hy, shy[q][r][l - 1]), d1_i, r1_i);
+          z2 t2 = zdivq(zsub(hx, shx[q][r - 1][l]), d2_j, r2_j);
+          v = zsub(zsub(t1, t2), zrmul(a.s33[c], Ez[c]));
+          if (a.fvm) v = zmul(v, zrmul(d1_i * d2_j * a.d3s[k], a.iomu));
+        }
+        bz[c] = v;
       }
-      bz[id3(n1, n2, k, j, i)] = v;
     }
-    hx_prev = hx_c;
-    hy_prev = hy_c;
+    hx_prev = hx;
+    hy_prev = hy;
   }
-#undef HX
-#undef HY
-#undef HZ
 }
 
 // H = curl E / (i w mu0) on all faces (fdfd.c:125-149), Hx | Hy | Hz.
@@ -154,362 +172,386 @@ __global__ void k_hz0(ApplyArgs a, z2 *hz)
 }
 
 // ----------------------------------------------------------------- LU-SGS
+//
+// lusgs_apply (fdfd.c:287-510): forward Gauss-Seidel over Ex, Ey, Ez in
+// lexicographic order, then backward over Ez, Ey, Ex in reverse order, from
+// a zero state, on the airwave-extended grid (n3f = n3 + nb planes).
+//
+// Each component sweep decouples into independent 2-D problems (slices)
+// along the component's own axis: Ex couples to Ex only through (k, j+-1)
+// and (k+-1, j) (fdfd.c:358-361), Ey through (k+-1, i) and (k, i+-1), Ez
+// through (j+-1, i) and (j, i+-1).  One workgroup owns a slice; thread t
+// owns slow index s = 1+t and visits fast index f = d - s on diagonal d, so
+// every cell sees exactly the neighbours the lexicographic loop gives it:
+// the new value of (s, f-1) from its own register and of (s-1, f) from
+// thread t-1 one diagonal earlier, through LDS (the roles of +1 and -1
+// swap in a backward sweep).
+//
+// Every cell update has the shape (fdfd.c:357-369, 383-395, 409-421)
+//   t1 = P1 - Q1, t2 = P2 - Q2, t3 = P3 - Q3, t4 = P4 - Q4,
+//   aij = ((t1 - t2)/dA - (t3 - t4)/dB) * iwmu - sigma*0,
+//   aii = c*iwmu - sigma,   E = (b - aij)/aii,
+// where the component's own neighbours along axis A sit in Q1 (+) and Q2
+// (-), along axis B in P3 (+) and P4 (-), and the two other components in
+// P1, P2, Q3, Q4:
+//   Ex: A = j (fast), B = k (slow);  P1, P2 from Ey;  Q3, Q4 from Ez
+//   Ey: A = k (slow), B = i (fast);  P1, P2 from Ez;  Q3, Q4 from Ex
+//   Ez: A = i (fast), B = j (slow);  P1, P2 from Ex;  Q3, Q4 from Ey
+// A forward sweep sees 0 for the + neighbours and new values for the -
+// ones; a backward sweep sees forward values for the - neighbours and new
+// values for the + ones.  Everything except the new values is known when a
+// sweep starts, so a parallel pre-pass evaluates it with the same
+// expressions into a record per cell,
+//   forward:  R = (t1, P2, t3, Q4)        backward: R = (P1, t2, Q3, t4),
+// and the diagonal's half of Smith's division (ratio, denominator, its
+// reciprocal, branch) is evaluated once per frequency.  The wavefront then
+// computes only the dependent remainder, in the reference's order, so each
+// intermediate rounds as in the reference.  sigma*0 is +0 for the finite,
+// non-negative conductivities the model accepts and x - (+0) == x, so that
+// term is dropped.  Divisions by spacings use ddiv.
+//
+// Layout: a sweep's per-cell inputs (record, right-hand side, diagonal)
+// are stored diagonal-major ("skewed"): slice q, then diagonal d = s + f,
+// then row s, packed without holes, so the lanes of a wave (consecutive
+// rows of one diagonal) load one contiguous run.  The state W (forward
+// values, then the backward values of Ey and Ez) and the result y keep the
+// reference layout; the pre-passes gather their neighbours from W.
+
+// Sweep geometry of component C: slice q, slow row s in 1..A, fast f in 1..B.
+//   Ex: q = i (n1 slices), s = k (A = n3f-1), f = j (B = n2-1)
+//   Ey: q = j (n2 slices), s = k (A = n3f-1), f = i (B = n1-1)
+//   Ez: q = k (n3f slices), s = j (A = n2-1), f = i (B = n1-1)
+struct SkewGeo {
+  int A[3], B[3], ns[3];
+  long base[3];  // component offsets in the skewed diagonal cache
+};
+
+__host__ __device__ __forceinline__ int sk_tri(int m) { return m > 0 ? m * (m + 1) / 2 : 0; }
+// cells on diagonals 2..d-1 of an A x B slice
+__host__ __device__ __forceinline__ int sk_off(int d, int A, int B)
+{
+  const int n = d - 2;
+  return sk_tri(n) - sk_tri(n - A) - sk_tri(n - B);
+}
+
+struct LuRec { z2x4 R; z2 b; };  // 80 B per swept cell
 
 struct LuArgs {
   const z2 *x;  // right-hand side, operator grid
   z2 *W;        // sweep state on the full grid, Ex | Ey | Ez
   z2 *y;        // result, operator grid
+  LuRec *rec;   // pre-pass records of the current sweep, skewed
+  z2 *trash;    // 2*SGS_MAXT values that lanes off the diagonal store into
+  const double4 *smith;  // (ratio, |denom|, |1/denom|, code), skewed Ex | Ey | Ez
   const double *s11, *s22, *s33;
   const double *d1, *d2, *d3, *d1s, *d2s, *d3s;
+  const double *r1, *r2, *r3, *r1s, *r2s, *r3s;
   z2 iwmu, iomu;
   int fvm, n1, n2, n3f, kofs, n3op;
   long fEx, fEy, lEx, lEy;
+  SkewGeo g;
 };
 
-// Cell updates of fdfd.c:357-369 (Ex), 383-395 (Ey), 409-421 (Ez): the cell
-// is first set to 0, so every "- E(k,j,i)" term subtracts zero.
-__device__ __forceinline__ z2 cell_ex(z2 b, double sig, double d1s_i, double d2s_j, double d2s_jm,
-                                      double d2_j, double d3s_k, double d3s_km, double d3_k,
-                                      z2 ey_ip, z2 ey_i, z2 ey_jm_ip, z2 ey_jm_i,
-                                      z2 ex_jp, z2 ex_jm, z2 ex_kp, z2 ex_km,
-                                      z2 ez_ip, z2 ez_i, z2 ez_km_ip, z2 ez_km_i,
-                                      z2 iwmu, int fvm, z2 iomu, double vol)
+template <int C>
+__device__ __forceinline__ void lu_kji(int q, int s, int f, int &k, int &j, int &i)
 {
-  const z2 z0 = zmk(0.0, 0.0);
-  z2 t1 = zsub(zdivr(zsub(ey_ip, ey_i), d1s_i), zdivr(zsub(ex_jp, z0), d2s_j));
-  z2 t2 = zsub(zdivr(zsub(ey_jm_ip, ey_jm_i), d1s_i), zdivr(zsub(z0, ex_jm), d2s_jm));
-  z2 t3 = zsub(zdivr(zsub(ex_kp, z0), d3s_k), zdivr(zsub(ez_ip, ez_i), d1s_i));
-  z2 t4 = zsub(zdivr(zsub(z0, ex_km), d3s_km), zdivr(zsub(ez_km_ip, ez_km_i), d1s_i));
-  z2 aij = zsub(zmul(zsub(zdivr(zsub(t1, t2), d2_j), zdivr(zsub(t3, t4), d3_k)), iwmu), zrmul(sig, z0));
-  double c = (1. / d2s_j + 1. / d2s_jm) / d2_j + (1. / d3s_k + 1. / d3s_km) / d3_k;
-  z2 aii = zsubr(zrmul(c, iwmu), sig);
-  if (fvm) {
-    z2 f = zrmul(vol, iomu);
-    aij = zmul(aij, f);
-    aii = zmul(aii, f);
-  }
-  return zdiv(zsub(b, aij), aii);
+  if (C == 0) { i = q; k = s; j = f; }
+  else if (C == 1) { j = q; k = s; i = f; }
+  else { k = q; j = s; i = f; }
 }
 
-__device__ __forceinline__ z2 cell_ey(z2 b, double sig, double d1s_i, double d1s_im, double d1_i,
-                                      double d2s_j, double d3s_k, double d3s_km, double d3_k,
-                                      z2 ez_jp, z2 ez_j, z2 ez_km_jp, z2 ez_km_j,
-                                      z2 ey_kp, z2 ey_km, z2 ey_ip, z2 ey_im,
-                                      z2 ex_jp, z2 ex_j, z2 ex_jp_im, z2 ex_j_im,
-                                      z2 iwmu, int fvm, z2 iomu, double vol)
+// Thread -> (slice q, skewed position p) -> cell; false past the slice.
+template <int C>
+__device__ __forceinline__ bool lu_decode(const LuArgs &a, int &q, int &s, int &f, int &p)
 {
-  const z2 z0 = zmk(0.0, 0.0);
-  z2 t1 = zsub(zdivr(zsub(ez_jp, ez_j), d2s_j), zdivr(zsub(ey_kp, z0), d3s_k));
-  z2 t2 = zsub(zdivr(zsub(ez_km_jp, ez_km_j), d2s_j), zdivr(zsub(z0, ey_km), d3s_km));
-  z2 t3 = zsub(zdivr(zsub(ey_ip, z0), d1s_i), zdivr(zsub(ex_jp, ex_j), d2s_j));
-  z2 t4 = zsub(zdivr(zsub(z0, ey_im), d1s_im), zdivr(zsub(ex_jp_im, ex_j_im), d2s_j));
-  z2 aij = zsub(zmul(zsub(zdivr(zsub(t1, t2), d3_k), zdivr(zsub(t3, t4), d1_i)), iwmu), zrmul(sig, z0));
-  double c = (1. / d3s_k + 1. / d3s_km) / d3_k + (1. / d1s_i + 1. / d1s_im) / d1_i;
-  z2 aii = zsubr(zrmul(c, iwmu), sig);
-  if (fvm) {
-    z2 f = zrmul(vol, iomu);
-    aij = zmul(aij, f);
-    aii = zmul(aii, f);
+  const int A = a.g.A[C], B = a.g.B[C];
+  q = blockIdx.y;
+  p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= A * B) return false;
+  int lo = 2, hi = A + B;  // largest d with sk_off(d) <= p
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sk_off(mid, A, B) <= p) lo = mid; else hi = mid - 1;
   }
-  return zdiv(zsub(b, aij), aii);
+  s = max(1, lo - B) + (p - sk_off(lo, A, B));
+  f = lo - s;
+  return true;
 }
 
-__device__ __forceinline__ z2 cell_ez(z2 b, double sig, double d1s_i, double d1s_im, double d1_i,
-                                      double d2s_j, double d2s_jm, double d2_j, double d3s_k,
-                                      z2 ex_kp, z2 ex_k, z2 ex_kp_im, z2 ex_k_im,
-                                      z2 ez_ip, z2 ez_im, z2 ez_jp, z2 ez_jm,
-                                      z2 ey_kp, z2 ey_k, z2 ey_kp_jm, z2 ey_k_jm,
-                                      z2 iwmu, int fvm, z2 iomu, double vol)
+// Per-frequency diagonal of component C (the aii lines fdfd.c:363, 389,
+// 415 with the fvm scaling of 364-368) reduced to Smith's first half
+// (zdiv in csem_common.h): ratio, |denom|, |1/denom| and a code whose sign
+// is the sign of denom and whose magnitude is 2 for the |c| < |d| branch.
+template <int C>
+__global__ __launch_bounds__(256) void k_lu_smith(LuArgs a)
 {
-  const z2 z0 = zmk(0.0, 0.0);
-  z2 t1 = zsub(zdivr(zsub(ex_kp, ex_k), d3s_k), zdivr(zsub(ez_ip, z0), d1s_i));
-  z2 t2 = zsub(zdivr(zsub(ex_kp_im, ex_k_im), d3s_k), zdivr(zsub(z0, ez_im), d1s_im));
-  z2 t3 = zsub(zdivr(zsub(ez_jp, z0), d2s_j), zdivr(zsub(ey_kp, ey_k), d3s_k));
-  z2 t4 = zsub(zdivr(zsub(z0, ez_jm), d2s_jm), zdivr(zsub(ey_kp_jm, ey_k_jm), d3s_k));
-  z2 aij = zsub(zmul(zsub(zdivr(zsub(t1, t2), d1_i), zdivr(zsub(t3, t4), d2_j)), iwmu), zrmul(sig, z0));
-  double c = (1. / d1s_i + 1. / d1s_im) / d1_i + (1. / d2s_j + 1. / d2s_jm) / d2_j;
-  z2 aii = zsubr(zrmul(c, iwmu), sig);
-  if (fvm) {
-    z2 f = zrmul(vol, iomu);
-    aij = zmul(aij, f);
-    aii = zmul(aii, f);
+  int q, s, f, k, j, i, p;
+  if (!lu_decode<C>(a, q, s, f, p)) return;
+  lu_kji<C>(q, s, f, k, j, i);
+  const double *d1 = a.d1, *d2 = a.d2, *d3 = a.d3, *d1s = a.d1s, *d2s = a.d2s, *d3s = a.d3s;
+  const int n1 = a.n1, n2 = a.n2;
+  double c, sig;
+  if (C == 0) {
+    c = (1. / d2s[j] + 1. / d2s[j - 1]) / d2[j] + (1. / d3s[k] + 1. / d3s[k - 1]) / d3[k];
+    sig = a.s11[id1(n1, n2, k, j, i)];
+  } else if (C == 1) {
+    c = (1. / d3s[k] + 1. / d3s[k - 1]) / d3[k] + (1. / d1s[i] + 1. / d1s[i - 1]) / d1[i];
+    sig = a.s22[id2(n1, n2, k, j, i)];
+  } else {
+    c = (1. / d1s[i] + 1. / d1s[i - 1]) / d1[i] + (1. / d2s[j] + 1. / d2s[j - 1]) / d2[j];
+    sig = a.s33[id3(n1, n2, k, j, i)];
   }
-  return zdiv(zsub(b, aij), aii);
+  z2 aii = zsubr(zrmul(c, a.iwmu), sig);
+  if (a.fvm) aii = zmul(aii, zrmul(d1s[i] * d2[j] * d3[k], a.iomu));
+  const double cc = aii.x, dd = aii.y;
+  double ratio, denom, code;
+  if (fabs(cc) < fabs(dd)) { ratio = cc / dd; denom = (cc * ratio) + dd; code = 2; }
+  else { ratio = dd / cc; denom = (dd * ratio) + cc; code = 1; }
+  if (denom < 0) code = -code;
+  ((double4 *)a.smith)[a.g.base[C] + (long)q * a.g.A[C] * a.g.B[C] + p] =
+      make_double4(ratio, fabs(denom), fabs(1.0 / denom), code);
 }
 
-// Each component sweep decouples into independent 2-D Gauss-Seidel problems
-// along the component's own axis: Ex couples to Ex only through (k,j+-1)
-// and (k+-1,j) (fdfd.c:358-361), Ey through (k+-1,i) and (k,i+-1), Ez
-// through (j+-1,i) and (j,i+-1).  One workgroup owns a slice; thread t owns
-// slow index s = 1+t and visits fast index f = d - s on diagonal d, so
-// every cell sees exactly the neighbours the lexicographic loop gives it:
-// the new values of (s,f-1) (own register) and (s-1,f) (thread t-1, one
-// diagonal earlier, through LDS), and the old values of (s,f+1), (s+1,f)
-// -- zero in a forward sweep, the forward result in a backward one (then
-// the roles of +1 and -1 swap).
+// Second half of zdiv: n / aii from the stored diagonal.  Both branches
+// are formed and selected, so a wave never splits; x/denom is evaluated as
+// (sign*x)/|denom|, which is the same IEEE quotient.
+__device__ __forceinline__ z2 smith_finish(z2 n, double4 S)
+{
+  const bool f = fabs(S.w) > 1.5;
+  const double sg = S.w < 0 ? -1.0 : 1.0;
+  const double u = f ? n.x : n.y, w = f ? n.y : n.x;
+  const double p = w * S.x;
+  const double re = (u * S.x) + w;
+  const double im = f ? p - u : u - p;
+  return zmk(ddiv(re * sg, S.y, S.z), ddiv(im * sg, S.y, S.z));
+}
 
-// Every sweep kernel software-pipelines its global loads: the inputs of a
-// cell that do not depend on the sweep's own progress (right-hand side,
-// conductivity, the other components, old values) are loaded P diagonals
-// ahead into a register ring, so the dependent chain of a diagonal waits
-// only for LDS and arithmetic, not for memory.  The 1-D spacings of the fast
-// axis are staged in LDS once.  Dynamic LDS: 2*blockDim complex exchange
-// slots, then two arrays of (nfast+1) doubles.
-#define SGS_P 4
+// Value of component C at (k, j, i) of the full-grid state.
+#define WX(k, j, i) a.W[id1(n1, n2, k, j, i)]
+#define WY(k, j, i) a.W[a.fEx + id2(n1, n2, k, j, i)]
+#define WZ(k, j, i) a.W[a.fEx + a.fEy + id3(n1, n2, k, j, i)]
+
+// Pre-pass of sweep (C, FWD): the record R and the right-hand side of every
+// swept cell, written to its skewed position.  Threads walk the cells in
+// the reference layout, so the neighbour gathers from W coalesce.  In the
+// forward pass, components swept later are still zero (fdfd.c:338-343); W
+// holds forward values, overwritten by the backward values of Ez and Ey as
+// those sweeps finish.
+template <int C, bool FWD>
+__global__ __launch_bounds__(256) void k_lu_pre(LuArgs a)
+{
+  const int n1 = a.n1, n2 = a.n2, n3 = a.n3f;
+  const int nx = (C == 0) ? n1 : n1 + 1, ny = (C == 1) ? n2 : n2 + 1;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= nx || j >= ny) return;
+  int q, s, f;
+  if (C == 0) { q = i; s = k; f = j; if (!(k >= 1 && k <= n3 - 1 && j >= 1 && j <= n2 - 1)) return; }
+  else if (C == 1) { q = j; s = k; f = i; if (!(k >= 1 && k <= n3 - 1 && i >= 1 && i <= n1 - 1)) return; }
+  else { q = k; s = j; f = i; if (!(j >= 1 && j <= n2 - 1 && i >= 1 && i <= n1 - 1)) return; }
+  const int A = a.g.A[C], B = a.g.B[C], d = s + f;
+  const int p = sk_off(d, A, B) + (s - max(1, d - B));
+  const int kop = k - a.kofs;
+  const long xo = (C == 0) ? id1(n1, n2, kop, j, i) : (C == 1) ? a.lEx + id2(n1, n2, kop, j, i)
+                                                                 : a.lEx + a.lEy + id3(n1, n2, kop, j, i);
+  LuRec &out = a.rec[(long)q * a.g.A[C] * a.g.B[C] + p];
+  out.b = kop >= 0 ? a.x[xo] : zmk(0, 0);
+  if (C == 0 && FWD) return;  // Ex forward: Ey = Ez = 0, every record term is +0
+  const z2 z0 = zmk(0, 0);
+  z2 P1, P2, Q2, Q3, P4, Q4;
+  if (C == 0) {  // fdfd.c:483-486: Ey, Ez backward, Ex forward
+    const double d1s_i = a.d1s[i], r1s_i = a.r1s[i];
+    P1 = zdivq(zsub(WY(k, j, i + 1), WY(k, j, i)), d1s_i, r1s_i);
+    P2 = zdivq(zsub(WY(k, j - 1, i + 1), WY(k, j - 1, i)), d1s_i, r1s_i);
+    Q2 = zdivq(zsub(z0, WX(k, j - 1, i)), a.d2s[j - 1], a.r2s[j - 1]);
+    Q3 = zdivq(zsub(WZ(k, j, i + 1), WZ(k, j, i)), d1s_i, r1s_i);
+    P4 = zdivq(zsub(z0, WX(k - 1, j, i)), a.d3s[k - 1], a.r3s[k - 1]);
+    Q4 = zdivq(zsub(WZ(k - 1, j, i + 1), WZ(k - 1, j, i)), d1s_i, r1s_i);
+  } else if (C == 1) {  // fdfd.c:384-387 / 458-461: Ez (zero forward, backward after), Ex forward
+    const double d2s_j = a.d2s[j], r2s_j = a.r2s[j];
+    const z2 ez_jp = FWD ? z0 : WZ(k, j + 1, i), ez_j = FWD ? z0 : WZ(k, j, i);
+    const z2 ez_km_jp = FWD ? z0 : WZ(k - 1, j + 1, i), ez_km_j = FWD ? z0 : WZ(k - 1, j, i);
+    P1 = zdivq(zsub(ez_jp, ez_j), d2s_j, r2s_j);
+    P2 = zdivq(zsub(ez_km_jp, ez_km_j), d2s_j, r2s_j);
+    Q3 = zdivq(zsub(WX(k, j + 1, i), WX(k, j, i)), d2s_j, r2s_j);
+    Q4 = zdivq(zsub(WX(k, j + 1, i - 1), WX(k, j, i - 1)), d2s_j, r2s_j);
+    if (!FWD) {
+      Q2 = zdivq(zsub(z0, WY(k - 1, j, i)), a.d3s[k - 1], a.r3s[k - 1]);
+      P4 = zdivq(zsub(z0, WY(k, j, i - 1)), a.d1s[i - 1], a.r1s[i - 1]);
+    }
+  } else {  // fdfd.c:410-413 / 435-438: Ex, Ey forward
+    const double d3s_k = a.d3s[k], r3s_k = a.r3s[k];
+    P1 = zdivq(zsub(WX(k + 1, j, i), WX(k, j, i)), d3s_k, r3s_k);
+    P2 = zdivq(zsub(WX(k + 1, j, i - 1), WX(k, j, i - 1)), d3s_k, r3s_k);
+    Q3 = zdivq(zsub(WY(k + 1, j, i), WY(k, j, i)), d3s_k, r3s_k);
+    Q4 = zdivq(zsub(WY(k + 1, j - 1, i), WY(k, j - 1, i)), d3s_k, r3s_k);
+    if (!FWD) {
+      Q2 = zdivq(zsub(z0, WZ(k, j, i - 1)), a.d1s[i - 1], a.r1s[i - 1]);
+      P4 = zdivq(zsub(z0, WZ(k, j - 1, i)), a.d2s[j - 1], a.r2s[j - 1]);
+    }
+  }
+  if (FWD) {  // Q1 = (0 - 0)/d = +0 and P3 = +0
+    out.R = z2x4{zsub(P1, z0), P2, zsub(z0, Q3), Q4};
+  } else {
+    out.R = z2x4{P1, zsub(P2, Q2), Q3, zsub(P4, Q4)};
+  }
+}
+#undef WX
+#undef WY
+#undef WZ
+
+// The sweep kernels software-pipeline their global loads: each cell's
+// record and diagonal are loaded SGS_P diagonals ahead into a register
+// ring.  Per diagonal a wave issues few instructions: the fast axis'
+// spacings come from one LDS table entry, the skewed position of the
+// diagonal from another, the neighbour exchange is branch-free (the
+// exchange rows have zero entries at both ends), and rows off the diagonal
+// compute on a clamped cell and discard the result.
+// Dynamic LDS: two exchange rows of blockDim+2 complex values, the fast
+// table (B+2 double4), the volume table (B+2 doubles) and the diagonal
+// table (A+B+1 ints).
+#define SGS_P 4   // register ring depth (diagonals loaded ahead)
+#define SGS_U 16  // diagonals per loop iteration
 #define SGS_MAXT 256  // rows per slice: n3f-1 and n2-1 must not exceed this
 
-// Ex sweep, slice i = blockIdx.x, s = k, f = j.
-template <bool FWD>
-__global__ __launch_bounds__(SGS_MAXT) void k_sgs_ex(LuArgs a)
+template <int C, bool FWD, bool FVM>
+__global__ __launch_bounds__(SGS_MAXT) void k_sgs(LuArgs a)
 {
   extern __shared__ z2 xch[];
-  const int n1 = a.n1, n2 = a.n2, n3 = a.n3f;
-  const int i = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
-  const int k = 1 + t;
-  const bool krow = k <= n3 - 1;
-  const z2 *Wx = a.W, *Wy = a.W + a.fEx, *Wz = a.W + a.fEx + a.fEy;
-  z2 *Wxw = a.W;
-  const z2 *bx = a.x;
-  double *cd2s = (double *)(xch + 2 * nt), *cd2 = cd2s + (n2 + 1);
-  for (int q = t; q <= n2; q += nt) { cd2s[q] = a.d2s[q]; cd2[q] = a.d2[q]; }
-  const double d1s_i = a.d1s[i];
-  double d3s_k = 0, d3s_km = 0, d3_k = 0;
-  if (krow) { d3s_k = a.d3s[k]; d3s_km = a.d3s[k - 1]; d3_k = a.d3[k]; }
-  const int dlo = 2, dhi = (n3 - 1) + (n2 - 1), nst = dhi - dlo + 1;
-  const int kop = k - a.kofs;
-  struct St { z2 b, ey_ip, ey_i, ey_jm_ip, ey_jm_i, ez_ip, ez_i, ez_km_ip, ez_km_i, ex_jm, ex_km; double sig; };
-  auto load = [&](St &c, int st) {
-    const int j = (FWD ? dlo + st : dhi - st) - k;
-    if (!(st < nst && krow && j >= 1 && j <= n2 - 1)) return;
-    c.b = (kop >= 0) ? bx[id1(n1, n2, kop, j, i)] : zmk(0, 0);
-    c.sig = a.s11[id1(n1, n2, k, j, i)];
-    if (!FWD) {
-      c.ex_jm = Wx[id1(n1, n2, k, j - 1, i)];
-      c.ex_km = Wx[id1(n1, n2, k - 1, j, i)];
-      c.ey_ip = Wy[id2(n1, n2, k, j, i + 1)]; c.ey_i = Wy[id2(n1, n2, k, j, i)];
-      c.ey_jm_ip = Wy[id2(n1, n2, k, j - 1, i + 1)]; c.ey_jm_i = Wy[id2(n1, n2, k, j - 1, i)];
-      c.ez_ip = Wz[id3(n1, n2, k, j, i + 1)]; c.ez_i = Wz[id3(n1, n2, k, j, i)];
-      c.ez_km_ip = Wz[id3(n1, n2, k - 1, j, i + 1)]; c.ez_km_i = Wz[id3(n1, n2, k - 1, j, i)];
+  const int nslice = a.g.ns[C];
+  // consecutive slices go to one XCD (workgroups are dealt round-robin over
+  // the 8 XCDs), so neighbouring slices share lines in its L2
+  const int per = (nslice + 7) / 8;
+  const int q = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (q >= nslice) return;
+  const int n1 = a.n1, n2 = a.n2;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int A = a.g.A[C], B = a.g.B[C], AB = A * B;
+  const int s = 1 + t;
+  const int sc = min(s, A);
+  const int XW = nt + 2;  // exchange row width
+  double4 *ftab = (double4 *)(xch + 2 * XW);
+  double *vtab = (double *)(ftab + (B + 2));  // fvm: the cell volume's fast-axis factor
+  int *dtab = (int *)(vtab + (B + 2));
+  // fast table: (d*s at f-1 or f, its reciprocal, d at f, its reciprocal)
+  {
+    const double *g0 = (C == 0) ? a.d2s : a.d1s, *g1 = (C == 0) ? a.d2 : a.d1;
+    const double *g2 = (C == 0) ? a.r2s : a.r1s, *g3 = (C == 0) ? a.r2 : a.r1;
+    for (int u = t; u <= B + 1; u += nt) {
+      const int fs = FWD ? max(u - 1, 0) : min(u, B);
+      const int fc = min(u, B);
+      ftab[u] = make_double4(g0[fs], g2[fs], g1[fc], g3[fc]);
+      vtab[u] = (C == 0) ? a.d2[fc] : a.d1s[fc];
     }
+    // skewed position of row 0 on diagonal d: off(d) - lo(d)
+    for (int d = t; d <= A + B; d += nt) dtab[d] = d >= 2 ? sk_off(d, A, B) - max(1, d - B) : 0;
+    for (int u = t; u < 2 * XW; u += nt) xch[u] = zmk(0, 0);
+  }
+  const double *h0 = (C == 2) ? a.d2s : a.d3s, *h1 = (C == 2) ? a.d2 : a.d3;
+  const double *h2 = (C == 2) ? a.r2s : a.r3s, *h3 = (C == 2) ? a.r2 : a.r3;
+  const int ssi = FWD ? sc - 1 : sc;  // slow-axis d*s of the swept neighbour
+  const double ssx = h0[ssi], rssx = h2[ssi], sd = h1[sc], rsd = h3[sc];
+  // fvm volume d1s[i]*d2[j]*d3[k] (fdfd.c:364) = (vtab[f] * vA) * vB
+  const double vA = (C == 0) ? a.d1s[q] : (C == 1) ? a.d2[q] : a.d2[sc];
+  const double vB = (C == 2) ? a.d3[q] : a.d3[sc];
+  const LuRec *Rc = a.rec + (long)q * AB;
+  const double4 *Sm = a.smith + a.g.base[C] + (long)q * AB;
+  // natural offset of cell (q, s, f) = nb + f*FS with nb per row
+  const long PL = (C == 0) ? (long)n1 * (n2 + 1) : (C == 1) ? (long)(n1 + 1) * n2 : (long)(n1 + 1) * (n2 + 1);
+  const long SS = (C == 2) ? (long)(n1 + 1) : PL;
+  const int FS = (C == 0) ? n1 : 1;
+  const long nb = ((C == 0) ? (long)q : (C == 1) ? (long)(n1 + 1) * q : PL * q) + s * SS;
+  const long wofs = (C == 0) ? 0 : (C == 1) ? a.fEx : a.fEx + a.fEy;
+  const long xofs = (C == 0) ? 0 : (C == 1) ? a.lEx : a.lEx + a.lEy;
+  z2 *Wc = a.W + wofs + nb;
+  const int kop = ((C == 2) ? q : s) - a.kofs;
+  z2 *Yc = a.y + xofs + nb - (long)a.kofs * PL;
+  const int dlo = 2, dhi = A + B, nst = dhi - dlo + 1;
+  const z2 z0 = zmk(0, 0);
+  constexpr bool NOREC = (C == 0) && FWD;  // Ex forward: Ey = Ez = 0, so R = 0
+  __syncthreads();
+
+  struct St { z2x4 R; z2 b; double4 S; };
+  auto load = [&](St &c, int st) {
+    const int d = min(FWD ? dlo + st : dhi - st, dhi);
+    const int e = min(max(dtab[max(d, dlo)] + sc, 0), AB - 1);
+    if (NOREC) c.b = Rc[e].b;
+    else { const LuRec &r = Rc[e]; c.R = r.R; c.b = r.b; }
+    c.S = Sm[e];
   };
   St ring[SGS_P];
-  const z2 z0 = zmk(0, 0);
 #pragma unroll
-  for (int q = 0; q < SGS_P; q++) {
-    ring[q].b = z0; ring[q].sig = 0;
-    ring[q].ey_ip = ring[q].ey_i = ring[q].ey_jm_ip = ring[q].ey_jm_i = z0;
-    ring[q].ez_ip = ring[q].ez_i = ring[q].ez_km_ip = ring[q].ez_km_i = z0;
-    ring[q].ex_jm = ring[q].ex_km = z0;
-    load(ring[q], q);
+  for (int u = 0; u < SGS_P; u++) {
+    if (NOREC) ring[u].R = z2x4{z0, z0, z0, z0};
+    load(ring[u], u);
   }
-  __syncthreads();
   z2 prev = z0;
-  for (int st0 = 0; st0 < nst; st0 += SGS_P) {
+  const bool afast = (C != 1);
+  // the step count is padded to a multiple of SGS_U: padded steps lie off
+  // every row (f < 1 or f > B), load clamped cells and write only to the
+  // trash rows.  The loop body has no branch, so the compiler counts the
+  // outstanding loads instead of draining them; it still drains once per
+  // loop iteration (at the back edge), hence the long unroll.
+  const int nstp = ((nst + SGS_U - 1) / SGS_U) * SGS_U;
+  for (int st0 = 0; st0 < nstp; st0 += SGS_U) {
 #pragma unroll
-    for (int q = 0; q < SGS_P; q++) {
-      const int st = st0 + q;
-      if (st < nst) {
-        const int j = (FWD ? dlo + st : dhi - st) - k;
-        z2 nbv = z0;
-        if (st > 0) {
-          if (FWD) { if (t > 0) nbv = xch[((st - 1) & 1) * nt + t - 1]; }
-          else { if (k + 1 <= n3 - 1) nbv = xch[((st - 1) & 1) * nt + t + 1]; }
+    for (int uu = 0; uu < SGS_U; uu++) {
+      const int st = st0 + uu, u = uu % SGS_P;
+      {
+        const int d = FWD ? dlo + st : dhi - st;
+        const int f = d - s;
+        // new value of (s-1, f) (forward) or (s+1, f) (backward)
+        const z2 nbv = xch[((st + 1) & 1) * XW + (FWD ? t : t + 2)];
+        const double4 ft = ftab[min(max(f, 0), B + 1)];
+        const St &c = ring[u];
+        const double fsx = ft.x, rfsx = ft.y, fdf = ft.z, rfdf = ft.w;
+        z2 t1, t2, t3, t4;
+        if (FWD) {
+          const z2 qa = afast ? zdivq(zsub(z0, prev), fsx, rfsx) : zdivq(zsub(z0, nbv), ssx, rssx);
+          const z2 pb = afast ? zdivq(zsub(z0, nbv), ssx, rssx) : zdivq(zsub(z0, prev), fsx, rfsx);
+          t1 = c.R.a; t2 = zsub(c.R.b, qa); t3 = c.R.c; t4 = zsub(pb, c.R.d);
+        } else {
+          const z2 qa = afast ? zdivq(zsub(prev, z0), fsx, rfsx) : zdivq(zsub(nbv, z0), ssx, rssx);
+          const z2 pb = afast ? zdivq(zsub(nbv, z0), ssx, rssx) : zdivq(zsub(prev, z0), fsx, rfsx);
+          t1 = zsub(c.R.a, qa); t2 = c.R.b; t3 = zsub(pb, c.R.c); t4 = c.R.d;
         }
-        if (krow && j >= 1 && j <= n2 - 1) {
-          St &c = ring[q];
-          const z2 ex_jp = FWD ? z0 : prev, ex_jm = FWD ? prev : c.ex_jm;
-          const z2 ex_kp = FWD ? z0 : nbv, ex_km = FWD ? nbv : c.ex_km;
-          const double d2s_j = cd2s[j], d2s_jm = cd2s[j - 1], d2_j = cd2[j];
-          const double vol = a.fvm ? d1s_i * d2_j * d3_k : 0.0;
-          z2 v = cell_ex(c.b, c.sig, d1s_i, d2s_j, d2s_jm, d2_j, d3s_k, d3s_km, d3_k, c.ey_ip, c.ey_i, c.ey_jm_ip,
-                         c.ey_jm_i, ex_jp, ex_jm, ex_kp, ex_km, c.ez_ip, c.ez_i, c.ez_km_ip, c.ez_km_i, a.iwmu, a.fvm,
-                         a.iomu, vol);
-          if (FWD) Wxw[id1(n1, n2, k, j, i)] = v;
-          else if (kop >= 0) a.y[id1(n1, n2, kop, j, i)] = v;
-          prev = v;
-        }
-        load(ring[q], st + SGS_P);
-        xch[(st & 1) * nt + t] = prev;
+        const z2 tA = afast ? zdivq(zsub(t1, t2), fdf, rfdf) : zdivq(zsub(t1, t2), sd, rsd);
+        const z2 tB = afast ? zdivq(zsub(t3, t4), sd, rsd) : zdivq(zsub(t3, t4), fdf, rfdf);
+        z2 aij = zmul(zsub(tA, tB), a.iwmu);
+        if (FVM) aij = zmul(aij, zrmul((vtab[min(max(f, 0), B + 1)] * vA) * vB, a.iomu));
+        const z2 v = smith_finish(zsub(c.b, aij), c.S);
+        // branch-free: lanes off the diagonal store into the trash rows
+        const bool ok = s <= A && f >= 1 && f <= B;
+        const long o = (long)f * FS;
+        if (FWD || C != 0) *(ok ? Wc + o : a.trash + t) = v;
+        if (!FWD) *(ok && kop >= 0 ? Yc + o : a.trash + SGS_MAXT + t) = v;
+        prev.x = ok ? v.x : prev.x;  // per component: a select of the vector
+        prev.y = ok ? v.y : prev.y;  // struct itself goes through the stack
+        load(ring[u], st + SGS_P);
+        xch[(st & 1) * XW + t + 1] = prev;
         __syncthreads();
       }
     }
   }
   if (!FWD) {  // cells of this slice outside the sweep range are zero rows
-    const int nk = a.n3op + 1;
-    for (int q = t; q < nk * (n2 + 1); q += nt) {
-      int kq = q / (n2 + 1), j = q % (n2 + 1), kf = kq + a.kofs;
-      if (!(kf >= 1 && kf <= n3 - 1 && j >= 1 && j <= n2 - 1)) a.y[id1(n1, n2, kq, j, i)] = zmk(0, 0);
-    }
-  }
-}
-
-// Ey sweep, slice j = blockIdx.x, s = k, f = i.
-template <bool FWD>
-__global__ __launch_bounds__(SGS_MAXT) void k_sgs_ey(LuArgs a)
-{
-  extern __shared__ z2 xch[];
-  const int n1 = a.n1, n2 = a.n2, n3 = a.n3f;
-  const int j = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
-  const int k = 1 + t;
-  const bool krow = k <= n3 - 1;
-  const z2 *Wx = a.W, *Wy = a.W + a.fEx, *Wz = a.W + a.fEx + a.fEy;
-  z2 *Wyw = a.W + a.fEx;
-  const z2 *by = a.x + a.lEx;
-  z2 *yy = a.y + a.lEx;
-  double *cd1s = (double *)(xch + 2 * nt), *cd1 = cd1s + (n1 + 1);
-  for (int q = t; q <= n1; q += nt) { cd1s[q] = a.d1s[q]; cd1[q] = a.d1[q]; }
-  const double d2s_j = a.d2s[j], d2_j = a.d2[j];
-  double d3s_k = 0, d3s_km = 0, d3_k = 0;
-  if (krow) { d3s_k = a.d3s[k]; d3s_km = a.d3s[k - 1]; d3_k = a.d3[k]; }
-  const int dlo = 2, dhi = (n3 - 1) + (n1 - 1), nst = dhi - dlo + 1;
-  const int kop = k - a.kofs;
-  struct St { z2 b, ez_jp, ez_j, ez_km_jp, ez_km_j, ey_im, ey_km, ex_jp, ex_j, ex_jp_im, ex_j_im; double sig; };
-  auto load = [&](St &c, int st) {
-    const int i = (FWD ? dlo + st : dhi - st) - k;
-    if (!(st < nst && krow && i >= 1 && i <= n1 - 1)) return;
-    c.b = (kop >= 0) ? by[id2(n1, n2, kop, j, i)] : zmk(0, 0);
-    c.sig = a.s22[id2(n1, n2, k, j, i)];
-    if (!FWD) {
-      c.ey_im = Wy[id2(n1, n2, k, j, i - 1)];
-      c.ey_km = Wy[id2(n1, n2, k - 1, j, i)];
-      c.ez_jp = Wz[id3(n1, n2, k, j + 1, i)]; c.ez_j = Wz[id3(n1, n2, k, j, i)];
-      c.ez_km_jp = Wz[id3(n1, n2, k - 1, j + 1, i)]; c.ez_km_j = Wz[id3(n1, n2, k - 1, j, i)];
-    }
-    c.ex_jp = Wx[id1(n1, n2, k, j + 1, i)]; c.ex_j = Wx[id1(n1, n2, k, j, i)];
-    c.ex_jp_im = Wx[id1(n1, n2, k, j + 1, i - 1)]; c.ex_j_im = Wx[id1(n1, n2, k, j, i - 1)];
-  };
-  St ring[SGS_P];
-  const z2 z0 = zmk(0, 0);
-#pragma unroll
-  for (int q = 0; q < SGS_P; q++) {
-    ring[q].b = z0; ring[q].sig = 0;
-    ring[q].ez_jp = ring[q].ez_j = ring[q].ez_km_jp = ring[q].ez_km_j = z0;
-    ring[q].ey_im = ring[q].ey_km = z0;
-    ring[q].ex_jp = ring[q].ex_j = ring[q].ex_jp_im = ring[q].ex_j_im = z0;
-    load(ring[q], q);
-  }
-  __syncthreads();
-  z2 prev = z0;
-  for (int st0 = 0; st0 < nst; st0 += SGS_P) {
-#pragma unroll
-    for (int q = 0; q < SGS_P; q++) {
-      const int st = st0 + q;
-      if (st < nst) {
-        const int i = (FWD ? dlo + st : dhi - st) - k;
-        z2 nbv = z0;
-        if (st > 0) {
-          if (FWD) { if (t > 0) nbv = xch[((st - 1) & 1) * nt + t - 1]; }
-          else { if (k + 1 <= n3 - 1) nbv = xch[((st - 1) & 1) * nt + t + 1]; }
-        }
-        if (krow && i >= 1 && i <= n1 - 1) {
-          St &c = ring[q];
-          const z2 ey_ip = FWD ? z0 : prev, ey_im = FWD ? prev : c.ey_im;
-          const z2 ey_kp = FWD ? z0 : nbv, ey_km = FWD ? nbv : c.ey_km;
-          const double d1s_i = cd1s[i], d1s_im = cd1s[i - 1], d1_i = cd1[i];
-          const double vol = a.fvm ? d1s_i * d2_j * d3_k : 0.0;
-          z2 v = cell_ey(c.b, c.sig, d1s_i, d1s_im, d1_i, d2s_j, d3s_k, d3s_km, d3_k, c.ez_jp, c.ez_j, c.ez_km_jp,
-                         c.ez_km_j, ey_kp, ey_km, ey_ip, ey_im, c.ex_jp, c.ex_j, c.ex_jp_im, c.ex_j_im, a.iwmu,
-                         a.fvm, a.iomu, vol);
-          Wyw[id2(n1, n2, k, j, i)] = v;
-          if (!FWD && kop >= 0) yy[id2(n1, n2, kop, j, i)] = v;
-          prev = v;
-        }
-        load(ring[q], st + SGS_P);
-        xch[(st & 1) * nt + t] = prev;
-        __syncthreads();
+    const int n3 = a.n3f;
+    if (C == 0) {
+      const int nk = a.n3op + 1;
+      for (int w = t; w < nk * (n2 + 1); w += nt) {
+        const int kq = w / (n2 + 1), j = w % (n2 + 1), kf = kq + a.kofs;
+        if (!(kf >= 1 && kf <= n3 - 1 && j >= 1 && j <= n2 - 1)) a.y[id1(n1, n2, kq, j, q)] = z0;
       }
-    }
-  }
-  if (!FWD) {
-    const int nk = a.n3op + 1;
-    for (int q = t; q < nk * (n1 + 1); q += nt) {
-      int kq = q / (n1 + 1), i = q % (n1 + 1), kf = kq + a.kofs;
-      if (!(kf >= 1 && kf <= n3 - 1 && i >= 1 && i <= n1 - 1)) yy[id2(n1, n2, kq, j, i)] = zmk(0, 0);
-    }
-  }
-}
-
-// Ez sweep, slice k = blockIdx.x, s = j, f = i.
-template <bool FWD>
-__global__ __launch_bounds__(SGS_MAXT) void k_sgs_ez(LuArgs a)
-{
-  extern __shared__ z2 xch[];
-  const int n1 = a.n1, n2 = a.n2;
-  const int k = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
-  const int j = 1 + t;
-  const bool jrow = j <= n2 - 1;
-  const z2 *Wx = a.W, *Wy = a.W + a.fEx, *Wz = a.W + a.fEx + a.fEy;
-  z2 *Wzw = a.W + a.fEx + a.fEy;
-  const int kop = k - a.kofs;
-  const z2 *bz = a.x + a.lEx + a.lEy;
-  z2 *yz = a.y + a.lEx + a.lEy;
-  double *cd1s = (double *)(xch + 2 * nt), *cd1 = cd1s + (n1 + 1);
-  for (int q = t; q <= n1; q += nt) { cd1s[q] = a.d1s[q]; cd1[q] = a.d1[q]; }
-  const double d3s_k = a.d3s[k], d3_k = a.d3[k];
-  double d2s_j = 0, d2s_jm = 0, d2_j = 0;
-  if (jrow) { d2s_j = a.d2s[j]; d2s_jm = a.d2s[j - 1]; d2_j = a.d2[j]; }
-  const int dlo = 2, dhi = (n2 - 1) + (n1 - 1), nst = dhi - dlo + 1;
-  struct St { z2 b, ez_im, ez_jm, ex_kp, ex_k, ex_kp_im, ex_k_im, ey_kp, ey_k, ey_kp_jm, ey_k_jm; double sig; };
-  auto load = [&](St &c, int st) {
-    const int i = (FWD ? dlo + st : dhi - st) - j;
-    if (!(st < nst && jrow && i >= 1 && i <= n1 - 1)) return;
-    c.b = (kop >= 0) ? bz[id3(n1, n2, kop, j, i)] : zmk(0, 0);
-    c.sig = a.s33[id3(n1, n2, k, j, i)];
-    if (!FWD) {
-      c.ez_im = Wz[id3(n1, n2, k, j, i - 1)];
-      c.ez_jm = Wz[id3(n1, n2, k, j - 1, i)];
-    }
-    c.ex_kp = Wx[id1(n1, n2, k + 1, j, i)]; c.ex_k = Wx[id1(n1, n2, k, j, i)];
-    c.ex_kp_im = Wx[id1(n1, n2, k + 1, j, i - 1)]; c.ex_k_im = Wx[id1(n1, n2, k, j, i - 1)];
-    c.ey_kp = Wy[id2(n1, n2, k + 1, j, i)]; c.ey_k = Wy[id2(n1, n2, k, j, i)];
-    c.ey_kp_jm = Wy[id2(n1, n2, k + 1, j - 1, i)]; c.ey_k_jm = Wy[id2(n1, n2, k, j - 1, i)];
-  };
-  St ring[SGS_P];
-  const z2 z0 = zmk(0, 0);
-#pragma unroll
-  for (int q = 0; q < SGS_P; q++) {
-    ring[q].b = z0; ring[q].sig = 0;
-    ring[q].ez_im = ring[q].ez_jm = z0;
-    ring[q].ex_kp = ring[q].ex_k = ring[q].ex_kp_im = ring[q].ex_k_im = z0;
-    ring[q].ey_kp = ring[q].ey_k = ring[q].ey_kp_jm = ring[q].ey_k_jm = z0;
-    load(ring[q], q);
-  }
-  __syncthreads();
-  z2 prev = z0;
-  for (int st0 = 0; st0 < nst; st0 += SGS_P) {
-#pragma unroll
-    for (int q = 0; q < SGS_P; q++) {
-      const int st = st0 + q;
-      if (st < nst) {
-        const int i = (FWD ? dlo + st : dhi - st) - j;
-        z2 nbv = z0;
-        if (st > 0) {
-          if (FWD) { if (t > 0) nbv = xch[((st - 1) & 1) * nt + t - 1]; }
-          else { if (j + 1 <= n2 - 1) nbv = xch[((st - 1) & 1) * nt + t + 1]; }
-        }
-        if (jrow && i >= 1 && i <= n1 - 1) {
-          St &c = ring[q];
-          const z2 ez_ip = FWD ? z0 : prev, ez_im = FWD ? prev : c.ez_im;
-          const z2 ez_jp = FWD ? z0 : nbv, ez_jm = FWD ? nbv : c.ez_jm;
-          const double d1s_i = cd1s[i], d1s_im = cd1s[i - 1], d1_i = cd1[i];
-          const double vol = a.fvm ? d1s_i * d2_j * d3_k : 0.0;
-          z2 v = cell_ez(c.b, c.sig, d1s_i, d1s_im, d1_i, d2s_j, d2s_jm, d2_j, d3s_k, c.ex_kp, c.ex_k, c.ex_kp_im,
-                         c.ex_k_im, ez_ip, ez_im, ez_jp, ez_jm, c.ey_kp, c.ey_k, c.ey_kp_jm, c.ey_k_jm, a.iwmu,
-                         a.fvm, a.iomu, vol);
-          Wzw[id3(n1, n2, k, j, i)] = v;
-          if (!FWD && kop >= 0) yz[id3(n1, n2, kop, j, i)] = v;
-          prev = v;
-        }
-        load(ring[q], st + SGS_P);
-        xch[(st & 1) * nt + t] = prev;
-        __syncthreads();
+    } else if (C == 1) {
+      const int nk = a.n3op + 1;
+      for (int w = t; w < nk * (n1 + 1); w += nt) {
+        const int kq = w / (n1 + 1), i = w % (n1 + 1), kf = kq + a.kofs;
+        if (!(kf >= 1 && kf <= n3 - 1 && i >= 1 && i <= n1 - 1)) a.y[a.lEx + id2(n1, n2, kq, q, i)] = z0;
       }
-    }
-  }
-  if (!FWD && kop >= 0) {
-    for (int q = t; q < (n2 + 1) * (n1 + 1); q += nt) {
-      int jj = q / (n1 + 1), i = q % (n1 + 1);
-      if (!(jj >= 1 && jj <= n2 - 1 && i >= 1 && i <= n1 - 1)) yz[id3(n1, n2, kop, jj, i)] = zmk(0, 0);
+    } else if (q - a.kofs >= 0) {
+      for (int w = t; w < (n2 + 1) * (n1 + 1); w += nt) {
+        const int jj = w / (n1 + 1), i = w % (n1 + 1);
+        if (!(jj >= 1 && jj <= n2 - 1 && i >= 1 && i <= n1 - 1))
+          a.y[a.lEx + a.lEy + id3(n1, n2, q - a.kofs, jj, i)] = z0;
+      }
     }
   }
 }
