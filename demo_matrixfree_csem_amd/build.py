@@ -52,9 +52,15 @@ def build(verbose=False):
         _run(["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
-    # -Bsymbolic: the library's own err()/warn() (the reference's names)
-    # must not be interposed by the C library's err(3)/warn(3)
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-Wl,-Bsymbolic", "-o", tmp] + objs + ["-lm"])
+    # Linked by g++ with the static libgcc, not by hipcc: C99 complex
+    # division in the host code (the GMRES back-substitution) then comes
+    # from the same libgcc __divdc3 (Smith's algorithm) that a gcc build of
+    # the reference links; hipcc would bind compiler-rt's, which rounds
+    # differently.  -Bsymbolic: the library's own err()/warn() (the
+    # reference's names) must not be interposed by the C library's.
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    _run(["g++", "-shared", "-fPIC", "-static-libgcc", "-Wl,-Bsymbolic", "-o", tmp] + objs +
+         ["-L" + os.path.join(rocm, "lib"), "-Wl,-rpath," + os.path.join(rocm, "lib"), "-lamdhip64", "-lm"])
     os.replace(tmp, LIB)
     main_src = os.path.join(CSRC, "host", "csem_main.c")
     if os.path.exists(main_src):
