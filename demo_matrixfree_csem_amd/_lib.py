@@ -71,6 +71,19 @@ class FreqT(C.Structure):
                 ("iwmu_re", C.c_double), ("iwmu_im", C.c_double)]
 
 
+class MixedOpts(C.Structure):
+    """csem_mixed_opts of include/csem_dev.h."""
+    _fields_ = [("max_inner", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
+                ("check_every", C.c_int), ("use_graph", C.c_int)]
+
+    @classmethod
+    def default(cls, **kw):
+        o = cls(2000, 1e-2, 40, 8, 1)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
 # every symbol the headers in include/ declare
 HOST_SYMBOLS = [
     "emf_init", "emf_free", "acq_init", "acq_free", "extend_model_init", "extend_model_free",
@@ -91,6 +104,8 @@ DEV_SYMBOLS = [
     "csem_fft2", "csem_dot", "csem_bicgstab", "csem_fgmres", "csem_gmres", "csem_scatter_add", "csem_gather",
     "csem_malloc", "csem_free", "csem_memcpy_h2d", "csem_memcpy_d2h", "csem_memset", "csem_sync",
     "csem_kernel_timing", "csem_kernel_stats", "csem_set_reduction",
+    "csem_vec_sub", "csem_vec_divr", "csem_vec_divz", "csem_vec_subscaled_dev", "csem_vec_addscaled",
+    "csem_mixed_solve", "csem_apply32", "csem_lusgs32", "csem_mixed_tuning", "csem_mixed_bytes",
 ]
 
 _lib = None
@@ -129,6 +144,12 @@ def lib():
     L.csem_memset.argtypes = [vp, ip, lp, vp]
     L.csem_sync.argtypes = [vp]
     L.csem_kernel_timing.argtypes = [ip]
+    L.csem_mixed_solve.argtypes = [vp, ip, C.POINTER(FreqT), vp, vp, vp, C.POINTER(MixedOpts), vp, vp, vp]
+    L.csem_apply32.argtypes = [vp, ip, C.POINTER(FreqT), vp, vp, vp]
+    L.csem_lusgs32.argtypes = [vp, ip, C.POINTER(FreqT), vp, vp, vp]
+    L.csem_mixed_tuning.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip)]
+    L.csem_mixed_bytes.argtypes = [vp]
+    L.csem_mixed_bytes.restype = lp
     L.csem_set_reduction.argtypes = [ip]
     L.csem_kernel_stats.argtypes = [ip, C.POINTER(C.c_char_p), C.POINTER(dp), C.POINTER(lp), C.POINTER(dp)]
     L.csem_active_model.restype = vp

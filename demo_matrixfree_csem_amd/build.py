@@ -42,10 +42,12 @@ def build(verbose=False):
     objs = []
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
         o = os.path.join(OBJ, os.path.basename(src) + ".o")
-        _run([HIPCC] + HIP_FLAGS + ["-c", path, "-o", o])
+        # the FP32 inner-solve kernels may contract into FMAs; the FP64
+        # translation unit keeps the reference's unfused expressions
+        flags = HIP_FLAGS if "mixed" not in src else [f if f != "-ffp-contract=off" else "-ffp-contract=fast"
+                                                      for f in HIP_FLAGS]
+        _run([HIPCC] + flags + ["-c", path, "-o", o])
         objs.append(o)
     for src in C_SOURCES:
         o = os.path.join(OBJ, os.path.basename(src) + ".o")

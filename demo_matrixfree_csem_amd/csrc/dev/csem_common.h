@@ -54,9 +54,13 @@ __host__ __device__ __forceinline__ double ddiv(double x, double d, double rd)
 }
 __host__ __device__ __forceinline__ z2 zdivq(z2 a, double d, double rd) { return zmk(ddiv(a.x, d, rd), ddiv(a.y, d, rd)); }
 
-// Single precision counterparts for the mixed-precision inner solver.
-typedef float2 c2;
-__host__ __device__ __forceinline__ c2 cmk(float x, float y) { return make_float2(x, y); }
+// Single precision counterparts for the mixed-precision inner solver.  A
+// plain aligned struct rather than HIP's float2 (a union-based vector
+// class): selects and conditional copies of it stay in registers.
+struct __align__(8) c2 {
+  float x, y;
+};
+__host__ __device__ __forceinline__ c2 cmk(float x, float y) { return c2{x, y}; }
 __host__ __device__ __forceinline__ c2 cadd(c2 a, c2 b) { return cmk(a.x + b.x, a.y + b.y); }
 __host__ __device__ __forceinline__ c2 csub(c2 a, c2 b) { return cmk(a.x - b.x, a.y - b.y); }
 __host__ __device__ __forceinline__ c2 cmul(c2 a, c2 b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }

@@ -8,8 +8,7 @@
 #include <vector>
 #include <algorithm>
 
-#include "csem_kernels.h"
-#include "../../../include/csem_dev.h"
+#include "csem_internal.h"
 
 static char g_err[1024];
 
@@ -29,15 +28,12 @@ extern "C" const char *csem_last_error(void) { return g_err; }
 // ------------------------------------------------------- kernel timers
 // Optional HIP-event timing of every launch of the hot kernels, on the
 // stream they run on, with the algorithmic bytes each launch moves.
-enum {
-  KT_APPLY = 0, KT_HZ0, KT_AIR_ROWS_FWD, KT_AIR_COLS, KT_AIR_ROWS_INV, KT_AIR_GHOST,
-  KT_SGS_EX_F, KT_SGS_EY_F, KT_SGS_EZ_F, KT_SGS_EZ_B, KT_SGS_EY_B, KT_SGS_EX_B,
-  KT_DOT, KT_DOT_FIN, KT_BICG_S, KT_BICG_XR, KT_BICG_P, KT_LU_PRE, KT_LU_SMITH, KT_N
-};
 static const char *KT_NAMES[KT_N] = {
   "k_apply", "k_hz0", "k_air_rows_fwd", "k_air_cols", "k_air_rows_inv", "k_air_ghost",
   "k_sgs_ex<fwd>", "k_sgs_ey<fwd>", "k_sgs_ez<fwd>", "k_sgs_ez<bwd>", "k_sgs_ey<bwd>", "k_sgs_ex<bwd>",
-  "k_dot_partial", "k_dot_final", "k_bicg_s", "k_bicg_xr", "k_bicg_p", "k_lu_pre", "k_lu_smith"};
+  "k_dot_partial", "k_dot_final", "k_bicg_s", "k_bicg_xr", "k_bicg_p", "k_lu_pre", "k_lu_smith",
+  "k_mx_apply", "k_mx_hz0", "k_mx_air_rows", "k_mx_air_cols", "k_mx_air_rows_inv", "k_mx_air_ghost",
+  "k_mx_lu_pre", "k_mx_sgs", "k_mx_krylov", "k_mx_dot", "k_mx_refine"};
 
 struct KtPend {
   int id;
@@ -64,15 +60,19 @@ static hipEvent_t kt_event()
   return e;
 }
 
-static hipEvent_t kt_begin(hipStream_t s)
+static int g_kt_paused;
+// timing is paused while a stream is captured into a graph
+void kt_pause(int paused) { g_kt_paused = paused; }
+
+hipEvent_t kt_begin(hipStream_t s)
 {
-  if (!g_kt.on) return NULL;
+  if (!g_kt.on || g_kt_paused) return NULL;
   hipEvent_t e = kt_event();
   if (e) (void)hipEventRecord(e, s);
   return e;
 }
 
-static void kt_end(int id, hipEvent_t a, hipStream_t s, double bytes)
+void kt_end(int id, hipEvent_t a, hipStream_t s, double bytes)
 {
   if (!g_kt.on || !a) return;
   hipEvent_t b = kt_event();
@@ -115,13 +115,6 @@ extern "C" int csem_kernel_stats(int id, const char **name, double *total_ms, lo
   return KT_N;
 }
 
-#define KT(id, s, bytes, launch)        \
-  do {                                   \
-    hipEvent_t kta_ = kt_begin(s);       \
-    launch;                              \
-    kt_end(id, kta_, s, (double)(bytes)); \
-  } while (0)
-
 extern "C" int csem_freq_consts(float freq, csem_freq *out)
 {
   // I*2.*PI*f*mu0 (main.c:95) has a zero real part and imaginary part
@@ -131,54 +124,6 @@ extern "C" int csem_freq_consts(float freq, csem_freq *out)
   z2 iwmu = zdiv(zmk(1.0, 0.0), iomu);
   out->iomu_re = iomu.x; out->iomu_im = iomu.y;
   out->iwmu_re = iwmu.x; out->iwmu_im = iwmu.y;
-  return 0;
-}
-
-struct csem_model {
-  int n1, n2, n3f, n3, nb, airbc, fvm, kofs, nfft, kc;
-  long lEx, lEy, lEz, lE, fEx, fEy, fEz, fE;
-  double *d1, *d2, *d3, *d1s, *d2s, *d3s;
-  double *r1, *r2, *r3, *r1s, *r2s, *r3s;  // RN(1/d) of the spacings
-  double *s11, *s22, *s33;
-  SkewGeo g;       // LU-SGS sweep geometry
-  long nsk;        // swept cells, all components
-  z2 *W;           // LU-SGS state on the full grid
-  LuRec *rec;      // LU-SGS pre-pass records of one component, skewed
-  z2 *trash;       // LU-SGS stores of lanes off the diagonal
-  double4 *smith;  // per-frequency diagonal, skewed Ex | Ey | Ez
-  csem_freq smith_f;
-  int smith_ok;
-  z2 *sH1, *sH2, *tw, *buf, *bufx, *bufy, *hz0, *hxg, *hyg;
-  int *jx, *jy, *rows, nrowx, nrowy;
-  double *wx, *wy;
-  int *gxj1, *gxj2, *gyj1, *gyj2;
-  double *gxw1, *gxw2, *gyw1, *gyw2;
-  double *partials, *scal;
-  long nch;
-  z2 *kv;  // Krylov work vectors (lazily allocated)
-  int nkv;
-  long bytes;
-  std::vector<void *> allocs;
-};
-
-template <class T>
-static int dalloc(csem_model *m, T **p, long count)
-{
-  long bytes = count * (long)sizeof(T);
-  if (bytes <= 0) bytes = 256;
-  hipError_t e = hipMalloc((void **)p, bytes);
-  if (e != hipSuccess) return csem_fail(-2, "hipMalloc(%ld): %s", bytes, hipGetErrorString(e));
-  m->allocs.push_back((void *)*p);
-  m->bytes += bytes;
-  return 0;
-}
-
-template <class T>
-static int upload(csem_model *m, T **p, const void *src, long count)
-{
-  int rc = dalloc(m, p, count);
-  if (rc) return rc;
-  HIP_OK(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -198,6 +143,7 @@ static int set_lds_attrs()
 extern "C" void csem_model_destroy(csem_model *m)
 {
   if (!m) return;
+  mixed_free(m);
   for (void *p : m->allocs) (void)hipFree(p);
   delete m;
 }

@@ -1,0 +1,78 @@
+// csem_internal.h -- state shared by the FP64 and mixed-precision
+// translation units of libcsem (not part of the public C ABI).
+#pragma once
+#include <vector>
+#include "csem_kernels.h"
+#include "../../../include/csem_dev.h"
+
+// ------------------------------------------------------- kernel timers
+// Optional HIP-event timing of every launch of the hot kernels, on the
+// stream they run on, with the algorithmic bytes each launch moves.
+enum {
+  KT_APPLY = 0, KT_HZ0, KT_AIR_ROWS_FWD, KT_AIR_COLS, KT_AIR_ROWS_INV, KT_AIR_GHOST,
+  KT_SGS_EX_F, KT_SGS_EY_F, KT_SGS_EZ_F, KT_SGS_EZ_B, KT_SGS_EY_B, KT_SGS_EX_B,
+  KT_DOT, KT_DOT_FIN, KT_BICG_S, KT_BICG_XR, KT_BICG_P, KT_LU_PRE, KT_LU_SMITH,
+  KT_M_APPLY, KT_M_HZ0, KT_M_AIR_ROWS, KT_M_AIR_COLS, KT_M_AIR_RINV, KT_M_AIR_GHOST,
+  KT_M_PRE, KT_M_SGS, KT_M_VEC, KT_M_DOT, KT_M_REFINE, KT_N
+};
+hipEvent_t kt_begin(hipStream_t s);
+void kt_end(int id, hipEvent_t a, hipStream_t s, double bytes);
+#define KT(id, s, bytes, launch)        \
+  do {                                   \
+    hipEvent_t kta_ = kt_begin(s);       \
+    launch;                              \
+    kt_end(id, kta_, s, (double)(bytes)); \
+  } while (0)
+
+
+struct MixState;
+struct csem_model {
+  int n1, n2, n3f, n3, nb, airbc, fvm, kofs, nfft, kc;
+  long lEx, lEy, lEz, lE, fEx, fEy, fEz, fE;
+  double *d1, *d2, *d3, *d1s, *d2s, *d3s;
+  double *r1, *r2, *r3, *r1s, *r2s, *r3s;  // RN(1/d) of the spacings
+  double *s11, *s22, *s33;
+  SkewGeo g;       // LU-SGS sweep geometry
+  long nsk;        // swept cells, all components
+  z2 *W;           // LU-SGS state on the full grid
+  LuRec *rec;      // LU-SGS pre-pass records of one component, skewed
+  z2 *trash;       // LU-SGS stores of lanes off the diagonal
+  double4 *smith;  // per-frequency diagonal, skewed Ex | Ey | Ez
+  csem_freq smith_f;
+  int smith_ok;
+  z2 *sH1, *sH2, *tw, *buf, *bufx, *bufy, *hz0, *hxg, *hyg;
+  int *jx, *jy, *rows, nrowx, nrowy;
+  double *wx, *wy;
+  int *gxj1, *gxj2, *gyj1, *gyj2;
+  double *gxw1, *gxw2, *gyw1, *gyw2;
+  double *partials, *scal;
+  long nch;
+  z2 *kv;  // Krylov work vectors (lazily allocated)
+  int nkv;
+  long bytes;
+  std::vector<void *> allocs;
+  MixState *mx;  // mixed-precision state, created on first use
+};
+
+template <class T>
+static inline int dalloc(csem_model *m, T **p, long count)
+{
+  long bytes = count * (long)sizeof(T);
+  if (bytes <= 0) bytes = 256;
+  hipError_t e = hipMalloc((void **)p, bytes);
+  if (e != hipSuccess) return csem_fail(-2, "hipMalloc(%ld): %s", bytes, hipGetErrorString(e));
+  m->allocs.push_back((void *)*p);
+  m->bytes += bytes;
+  return 0;
+}
+
+template <class T>
+static inline int upload(csem_model *m, T **p, const void *src, long count)
+{
+  int rc = dalloc(m, p, count);
+  if (rc) return rc;
+  HIP_OK(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+void mixed_free(csem_model *m);

@@ -132,6 +132,7 @@ __global__ __launch_bounds__(64 * (TJ + 1)) void k_apply(ApplyArgs a)
 }
 
 // H = curl E / (i w mu0) on all faces (fdfd.c:125-149), Hx | Hy | Hz.
+#ifndef CSEM_TYPES_ONLY
 __global__ void k_curl_h(ApplyArgs a, z2 *H, long lHx, long lHy)
 {
   const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
@@ -157,8 +158,10 @@ __global__ void k_curl_h(ApplyArgs a, z2 *H, long lHx, long lHy)
         zmul(zsub(zdivr(zsub(Ey[id2(n1, n2, k, j, i + 1)], Ey[id2(n1, n2, k, j, i)]), d1s[i]),
                   zdivr(zsub(Ex[id1(n1, n2, k, j + 1, i)], Ex[id1(n1, n2, k, j, i)]), d2s[j])), iwmu);
 }
+#endif
 
 // Hz on the plane k = 0 only (the airwave input), idz(0,j,i) = i + n1*j.
+#ifndef CSEM_TYPES_ONLY
 __global__ void k_hz0(ApplyArgs a, z2 *hz)
 {
   const int n1 = a.n1, n2 = a.n2;
@@ -170,6 +173,7 @@ __global__ void k_hz0(ApplyArgs a, z2 *hz)
                     zdivr(zsub(Ex[id1(n1, n2, 0, j + 1, i)], Ex[id1(n1, n2, 0, j, i)]), a.d2s[j])),
                a.iwmu);
 }
+#endif
 
 // ----------------------------------------------------------------- LU-SGS
 //
@@ -638,6 +642,7 @@ struct AirArgs {
 
 // Bilinear Hz(z=0) -> uniform row i2, then the forward transform of the row
 // (airbc_init_free.c:137-153, rows first).
+#ifndef CSEM_TYPES_ONLY
 __global__ void k_air_rows_fwd(AirArgs a)
 {
   extern __shared__ z2 sm[];
@@ -662,6 +667,7 @@ __global__ void k_air_rows_fwd(AirArgs a)
   z2 *res = fft_lds(A, B, N, 1, a.tw, -1);
   for (int i1 = threadIdx.x; i1 < N; i1 += blockDim.x) a.buf[i1 + (long)N * i2] = res[i1];
 }
+#endif
 
 // Column pass: forward transform of columns c0..c0+CW-1 of buf, then for
 // each multiplier (sH1 -> bufx, sH2 -> bufy) the product and the inverse
@@ -699,6 +705,7 @@ __global__ void k_air_cols(AirArgs a)
 
 // Inverse transform of the rows the back-interpolation reads (the rest of
 // the inverse rows are never consumed).
+#ifndef CSEM_TYPES_ONLY
 __global__ void k_air_rows_inv(AirArgs a)
 {
   extern __shared__ z2 sm[];
@@ -711,9 +718,11 @@ __global__ void k_air_rows_inv(AirArgs a)
   z2 *res = fft_lds(A, B, N, 1, a.tw, +1);
   for (int i1 = threadIdx.x; i1 < N; i1 += blockDim.x) buf[i1 + N * row] = res[i1];
 }
+#endif
 
 // Normalise by N*N and interpolate back to the ghost Hx at (x1[i1], x2s[i2])
 // and ghost Hy at (x1s[i1], x2[i2]) (airbc_init_free.c:162-207).
+#ifndef CSEM_TYPES_ONLY
 __global__ void k_air_ghost(AirArgs a)
 {
   const int N = a.N, n1 = a.n1, n2 = a.n2;
@@ -749,8 +758,10 @@ __global__ void k_air_ghost(AirArgs a)
     a.hyg[t] = v;
   }
 }
+#endif
 
 // Generic 2-D transform of an N x N buffer (tests and the public fft2).
+#ifndef CSEM_TYPES_ONLY
 __global__ void k_fft_rows(z2 *data, int N, const z2 *tw, int dir)
 {
   extern __shared__ z2 sm[];
@@ -761,6 +772,7 @@ __global__ void k_fft_rows(z2 *data, int N, const z2 *tw, int dir)
   z2 *res = fft_lds(A, B, N, 1, tw, dir);
   for (int i = threadIdx.x; i < N; i += blockDim.x) data[i + N * row] = res[i];
 }
+#endif
 
 template <int CW>
 __global__ void k_fft_cols(z2 *data, int N, const z2 *tw, int dir)

@@ -1,0 +1,776 @@
+// csem_mixed_kernels.h -- FP32 kernels of the batched mixed-precision
+// solve (gfx950).
+//
+// R right-hand sides share one model; RHS r has its own frequency.  Every
+// FP32 vector is R blocks of n complex floats (block r at r*n).  These
+// kernels approximate the FP64 operator and preconditioner to FP32
+// accuracy: the outer FP64 iterative refinement (csem_mixed.hip) computes
+// the true residual with the FP64 kernels, so nothing here has to round
+// like the reference.  Divisions become multiplications by reciprocals and
+// expressions are reassociated freely.
+#pragma once
+#include "csem_common.h"
+#include "csem_kernels.h"
+
+#define MX_RMAX 4  // right-hand sides per batch
+
+__device__ __forceinline__ c2 cfma(c2 acc, c2 a, c2 b)  // acc + a*b
+{
+  return cmk(__builtin_fmaf(a.x, b.x, __builtin_fmaf(-a.y, b.y, acc.x)),
+             __builtin_fmaf(a.x, b.y, __builtin_fmaf(a.y, b.x, acc.y)));
+}
+__device__ __forceinline__ c2 crfma(c2 acc, float a, c2 b)  // acc + a*b, a real
+{
+  return cmk(__builtin_fmaf(a, b.x, acc.x), __builtin_fmaf(a, b.y, acc.y));
+}
+
+// Per right-hand side constants: iwmu = 1/(i w mu0), iomu = i w mu0.
+struct MxFreq {
+  c2 iwmu[MX_RMAX], iomu[MX_RMAX];
+};
+
+// ---------------------------------------------------------------- operator
+// FP32 model arrays: spacings and reciprocals (z arrays on the full grid),
+// conductivities on the full grid.
+struct MxGeo {
+  int n1, n2, n3, n3f, kofs, airbc, fvm;
+  long n, lEx, lEy;
+  const float *d1, *d2, *d3, *d1s, *d2s, *d3s;  // full-grid z
+  const float *r1, *r2, *r3, *r1s, *r2s, *r3s;
+  const float *s11, *s22, *s33;
+};
+
+// y_r = A_r x_r for every r: the curl-curl of fdfd.c:86-205 in FP32.  H' =
+// curl E (without 1/(i w mu0)) is formed once per face and shared through
+// LDS as in the FP64 kernel; the per-frequency factor multiplies the outer
+// curl.  Ghost planes (airwave boundary) hold H' as well.  Block: 64 lanes
+// (one halo) x TJ+1 rows (one halo); the grid marches kc planes in z.
+template <int R, int TJ>
+__global__ __launch_bounds__(64 * (TJ + 1)) void k_mx_apply(MxGeo g, MxFreq fr, const c2 *x, c2 *y, const c2 *hxg,
+                                                          const c2 *hyg, int kc)
+{
+  __shared__ c2 shx[2][R][TJ + 1][64], shy[2][R][TJ + 1][64], shz[2][R][TJ + 1][64];
+  const int l = threadIdx.x, rr = threadIdx.y;
+  const int i = blockIdx.x * 63 - 1 + l, j = blockIdx.y * TJ - 1 + rr;
+  const int n1 = g.n1, n2 = g.n2, n3 = g.n3, ko = g.kofs;
+  const int k0 = blockIdx.z * kc, k1 = min(k0 + kc, n3 + 1);
+  const bool live = i >= 0 && i <= n1 && j >= 0 && j <= n2;
+  const bool out = live && l > 0 && rr > 0;
+  const bool hx_ok = live && j < n2, hy_ok = live && i < n1, hz_ok = hy_ok && j < n2;
+  const int ic = min(max(i, 0), n1), jc = min(max(j, 0), n2);
+  const float r1s_i = g.r1s[ic], r1_i = g.r1[ic], r2s_j = g.r2s[jc], r2_j = g.r2[jc];
+  const long px = (long)n1 * (n2 + 1), py = (long)(n1 + 1) * n2, pz = (long)(n1 + 1) * (n2 + 1);
+  const long ox = i + (long)n1 * j, oy = i + (long)(n1 + 1) * j, oz = oy;
+  const long n = g.n;
+  const c2 z0 = cmk(0, 0);
+  // sigma on the operator grid: the full grid offset by kofs planes
+  const float *s11 = g.s11 + ko * px, *s22 = g.s22 + ko * py, *s33 = g.s33 + ko * pz;
+  c2 hxp[R], hyp[R];
+  auto HX = [&](int r, int k) {
+    const c2 *ez = x + r * n + g.lEx + g.lEy + k * pz + oz, *ey = x + r * n + g.lEx + k * py + oy;
+    const float r3s = g.r3s[k + ko];
+    return cmk((ez[n1 + 1].x - ez[0].x) * r2s_j - (ey[py].x - ey[0].x) * r3s,
+               (ez[n1 + 1].y - ez[0].y) * r2s_j - (ey[py].y - ey[0].y) * r3s);
+  };
+  auto HY = [&](int r, int k) {
+    const c2 *ex = x + r * n + k * px + ox, *ez = x + r * n + g.lEx + g.lEy + k * pz + oz;
+    const float r3s = g.r3s[k + ko];
+    return cmk((ex[px].x - ex[0].x) * r3s - (ez[1].x - ez[0].x) * r1s_i,
+               (ex[px].y - ex[0].y) * r3s - (ez[1].y - ez[0].y) * r1s_i);
+  };
+  auto HZ = [&](int r, int k) {
+    const c2 *ey = x + r * n + g.lEx + k * py + oy, *ex = x + r * n + k * px + ox;
+    return cmk((ey[1].x - ey[0].x) * r1s_i - (ex[n1].x - ex[0].x) * r2s_j,
+               (ey[1].y - ey[0].y) * r1s_i - (ex[n1].y - ex[0].y) * r2s_j);
+  };
+  // H' of the plane below k0; below k = 0 that is the airwave ghost plane
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    hxp[r] = z0; hyp[r] = z0;
+    if (k0 > 0 && k0 - 1 < n3) {
+      if (hx_ok) hxp[r] = HX(r, k0 - 1);
+      if (hy_ok) hyp[r] = HY(r, k0 - 1);
+    } else if (k0 == 0 && g.airbc && live) {
+      if (j < n2) hxp[r] = hxg[r * (long)(n1 + 1) * n2 + i + (long)(n1 + 1) * j];
+      if (i < n1) hyp[r] = hyg[r * (long)n1 * (n2 + 1) + i + (long)n1 * j];
+    }
+  }
+  const bool iin = i >= 1 && i <= n1 - 1, jin = j >= 1 && j <= n2 - 1;
+  for (int k = k0; k < k1; k++) {
+    const int q = k & 1;
+    c2 hx[R], hy[R], hz[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      hx[r] = z0; hy[r] = z0; hz[r] = z0;
+      if (k < n3) {
+        if (hx_ok) hx[r] = HX(r, k);
+        if (hy_ok) hy[r] = HY(r, k);
+      }
+      if (hz_ok) hz[r] = HZ(r, k);
+      shx[q][r][rr][l] = hx[r]; shy[q][r][rr][l] = hy[r]; shz[q][r][rr][l] = hz[r];
+    }
+    __syncthreads();
+    if (out) {
+      const bool kin = (k >= 1 && k <= n3 - 1) || (g.airbc && k == 0);
+      const float r3_k = g.r3[k + ko], r3s_k = g.r3s[k + ko];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const c2 iw = fr.iwmu[r];
+        c2 *yr = y + r * n;
+        const c2 *xr = x + r * n;
+        if (i < n1) {  // bx
+          const long c = k * px + ox;
+          c2 v = z0;
+          if (jin && kin) {
+            const c2 hym = hyp[r];
+            const c2 cu = cmk((hz[r].x - shz[q][r][rr - 1][l].x) * r2_j - (hy[r].x - hym.x) * r3_k,
+                              (hz[r].y - shz[q][r][rr - 1][l].y) * r2_j - (hy[r].y - hym.y) * r3_k);
+            v = csub(cmul(cu, iw), crmul(s11[c], xr[c]));
+            if (g.fvm) v = cmul(v, crmul(g.d1s[i] * g.d2[j] * g.d3[k + ko], fr.iomu[r]));
+          }
+          yr[c] = v;
+        }
+        if (j < n2) {  // by
+          const long c = g.lEx + k * py + oy;
+          c2 v = z0;
+          if (iin && kin) {
+            const c2 hxm = hxp[r];
+            const c2 cu = cmk((hx[r].x - hxm.x) * r3_k - (hz[r].x - shz[q][r][rr][l - 1].x) * r1_i,
+                              (hx[r].y - hxm.y) * r3_k - (hz[r].y - shz[q][r][rr][l - 1].y) * r1_i);
+            v = csub(cmul(cu, iw), crmul(s22[c - g.lEx], xr[c]));
+            if (g.fvm) v = cmul(v, crmul(g.d1[i] * g.d2s[j] * g.d3[k + ko], fr.iomu[r]));
+          }
+          yr[c] = v;
+        }
+        if (k < n3) {  // bz
+          const long c = g.lEx + g.lEy + k * pz + oz;
+          c2 v = z0;
+          if (iin && jin) {
+            const c2 cu = cmk((hy[r].x - shy[q][r][rr][l - 1].x) * r1_i - (hx[r].x - shx[q][r][rr - 1][l].x) * r2_j,
+                              (hy[r].y - shy[q][r][rr][l - 1].y) * r1_i - (hx[r].y - shx[q][r][rr - 1][l].y) * r2_j);
+            v = csub(cmul(cu, iw), crmul(s33[c - g.lEx - g.lEy], xr[c]));
+            if (g.fvm) v = cmul(v, crmul(g.d1[i] * g.d2[j] * g.d3s[k + ko], fr.iomu[r]));
+          }
+          yr[c] = v;
+        }
+      }
+      (void)r3s_k;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) { hxp[r] = hx[r]; hyp[r] = hy[r]; }
+  }
+}
+
+// H'z = curl E on the plane k = 0 for every r (the airwave input).
+__global__ void k_mx_hz0(MxGeo g, const c2 *x, c2 *hz)
+{
+  const int n1 = g.n1, n2 = g.n2, r = blockIdx.y;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n1 * n2) return;
+  const int i = t % n1, j = t / n1;
+  const c2 *ex = x + r * g.n, *ey = x + r * g.n + g.lEx;
+  const long a = i + (long)(n1 + 1) * j, b = i + (long)n1 * j;
+  const float r1s = g.r1s[i], r2s = g.r2s[j];
+  hz[r * (long)n1 * n2 + t] = cmk((ey[a + 1].x - ey[a].x) * r1s - (ex[b + n1].x - ex[b].x) * r2s,
+                                  (ey[a + 1].y - ey[a].y) * r1s - (ex[b + n1].y - ex[b].y) * r2s);
+}
+
+// ----------------------------------------------------------- airwave FFT
+// The FP64 path's Stockham passes (radix 4, then radix 2), in FP32 on LDS.
+template <int RD>
+__device__ __forceinline__ void mx_fft_pass(const c2 *a, c2 *b, int N, int Ns, int cols, const c2 *tw, int dir)
+{
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int nb = N / RD;
+  for (int w = t; w < nb * cols; w += nt) {
+    const int c = w % cols, j = w / cols;
+    const int k = j % Ns;
+    const int o = (j / Ns) * RD * Ns + k;
+    c2 u[RD];
+#pragma unroll
+    for (int r = 0; r < RD; r++) {
+      u[r] = a[(j + r * nb) * cols + c];
+      if (r != 0 && k != 0) {
+        c2 tv = tw[(long)r * k * (N / (RD * Ns))];
+        if (dir > 0) tv.y = -tv.y;
+        u[r] = cmul(u[r], tv);
+      }
+    }
+    if (RD == 4) {
+      const c2 a0 = cadd(u[0], u[2]), a1 = csub(u[0], u[2]), a2 = cadd(u[1], u[RD - 1]), a3 = csub(u[1], u[RD - 1]);
+      const c2 bb = dir < 0 ? cmk(a3.y, -a3.x) : cmk(-a3.y, a3.x);
+      b[o * cols + c] = cadd(a0, a2);
+      b[(o + Ns) * cols + c] = cadd(a1, bb);
+      b[(o + 2 * Ns) * cols + c] = csub(a0, a2);
+      b[(o + 3 * Ns) * cols + c] = csub(a1, bb);
+    } else {
+      b[o * cols + c] = cadd(u[0], u[1]);
+      b[(o + Ns) * cols + c] = csub(u[0], u[1]);
+    }
+  }
+}
+__device__ __forceinline__ c2 *mx_fft(c2 *a, c2 *b, int N, int cols, const c2 *tw, int dir)
+{
+  for (int Ns = 1; Ns < N;) {
+    if ((N / Ns) % 4 == 0) { mx_fft_pass<4>(a, b, N, Ns, cols, tw, dir); Ns *= 4; }
+    else { mx_fft_pass<2>(a, b, N, Ns, cols, tw, dir); Ns *= 2; }
+    __syncthreads();
+    c2 *tmp = a; a = b; b = tmp;
+  }
+  return a;
+}
+
+struct MxAir {
+  int N, n1, n2, cw;                 // cw: columns per block of the column pass
+  const c2 *hz;                      // R x n1*n2
+  c2 *buf, *bufx, *bufy;             // R x N*N
+  const c2 *sH1, *sH2, *tw;
+  const int *jx, *jy;
+  const float *wx, *wy;
+  const int *rows;
+  int nrowx, nrowy;
+  const int *gxj1, *gxj2, *gyj1, *gyj2;
+  const float *gxw1, *gxw2, *gyw1, *gyw2;
+  c2 *hxg, *hyg;                     // R x ghost plane
+};
+
+// Bilinear H'z -> uniform row, forward row transform (airbc_init_free.c:137-153).
+__global__ void k_mx_air_rows(MxAir a)
+{
+  extern __shared__ c2 smc[];
+  const int N = a.N, i2 = blockIdx.x, r = blockIdx.y, n1 = a.n1;
+  c2 *A = smc, *B = smc + N;
+  const c2 *h = a.hz + r * (long)n1 * a.n2;
+  const int j2 = a.jy[i2];
+  const float w2 = a.wy[i2];
+  for (int i1 = threadIdx.x; i1 < N; i1 += blockDim.x) {
+    const int j1 = a.jx[i1];
+    c2 v = cmk(0, 0);
+    if (j1 >= 0 && j2 >= 0) {
+      const float w1 = a.wx[i1];
+      const long b0 = j1 + (long)n1 * j2;
+      v = crmul((1.f - w1) * (1.f - w2), h[b0]);
+      v = crfma(v, w1 * (1.f - w2), h[b0 + 1]);
+      v = crfma(v, (1.f - w1) * w2, h[b0 + n1]);
+      v = crfma(v, w1 * w2, h[b0 + n1 + 1]);
+    }
+    A[i1] = v;
+  }
+  __syncthreads();
+  c2 *res = mx_fft(A, B, N, 1, a.tw, -1);
+  c2 *out = a.buf + r * (long)N * N + (long)N * i2;
+  for (int i1 = threadIdx.x; i1 < N; i1 += blockDim.x) out[i1] = res[i1];
+}
+
+// Column pass: forward transform of cw columns, then for sH1 -> bufx and
+// sH2 -> bufy the product and the inverse column transform.  The forward
+// spectrum is kept in registers (16 values per thread), so LDS holds only
+// the two Stockham buffers: N*cw must equal 16*blockDim.
+__global__ __launch_bounds__(256) void k_mx_air_cols(MxAir a)
+{
+  extern __shared__ c2 smc[];
+  const int N = a.N, cw = a.cw, c0 = blockIdx.x * cw, r = blockIdx.y;
+  const long off = r * (long)N * N;
+  c2 *A = smc, *B = smc + N * cw;
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int e = t; e < N * cw; e += nt) A[e] = a.buf[off + c0 + e % cw + (long)N * (e / cw)];
+  __syncthreads();
+  c2 *res = mx_fft(A, B, N, cw, a.tw, -1);
+  c2 S[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) S[q] = res[t + q * nt];
+  __syncthreads();
+  for (int m = 0; m < 2; m++) {
+    const c2 *mul = m == 0 ? a.sH1 : a.sH2;
+    c2 *dst = (m == 0 ? a.bufx : a.bufy) + off;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int e = t + q * nt;
+      A[e] = cmul(S[q], mul[c0 + e % cw + (long)N * (e / cw)]);
+    }
+    __syncthreads();
+    c2 *o = mx_fft(A, B, N, cw, a.tw, +1);
+    for (int e = t; e < N * cw; e += nt) dst[c0 + e % cw + (long)N * (e / cw)] = o[e];
+    __syncthreads();
+  }
+}
+
+// Inverse transform of the rows the back-interpolation reads.
+__global__ void k_mx_air_rows_inv(MxAir a)
+{
+  extern __shared__ c2 smc[];
+  const int N = a.N, w = blockIdx.x, r = blockIdx.y;
+  c2 *buf = ((w < a.nrowx) ? a.bufx : a.bufy) + r * (long)N * N;
+  const long row = a.rows[w];
+  c2 *A = smc, *B = smc + N;
+  for (int i1 = threadIdx.x; i1 < N; i1 += blockDim.x) A[i1] = buf[i1 + N * row];
+  __syncthreads();
+  c2 *res = mx_fft(A, B, N, 1, a.tw, +1);
+  for (int i1 = threadIdx.x; i1 < N; i1 += blockDim.x) buf[i1 + N * row] = res[i1];
+}
+
+// Normalise by N*N and interpolate back to the ghost planes
+// (airbc_init_free.c:162-207).
+__global__ void k_mx_air_ghost(MxAir a)
+{
+  const int N = a.N, n1 = a.n1, n2 = a.n2, r = blockIdx.y;
+  const float inn = 1.f / ((float)N * (float)N);
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = (n1 + 1) * n2, ny = n1 * (n2 + 1);
+  const long off = r * (long)N * N;
+  if (t < nx) {
+    const int i1 = t % (n1 + 1), i2 = t / (n1 + 1);
+    c2 v = cmk(0, 0);
+    if (i1 < n1) {
+      const int j1 = a.gxj1[i1], j2 = a.gxj2[i2];
+      const float w1 = a.gxw1[i1], w2 = a.gxw2[i2];
+      const c2 *b = a.bufx + off + j1 + (long)N * j2;
+      v = crmul((1.f - w1) * (1.f - w2), b[0]);
+      v = crfma(v, w1 * (1.f - w2), b[1]);
+      v = crfma(v, (1.f - w1) * w2, b[N]);
+      v = crfma(v, w1 * w2, b[N + 1]);
+      v = crmul(inn, v);
+    }
+    a.hxg[r * (long)nx + t] = v;
+  } else if (t < nx + ny) {
+    t -= nx;
+    const int i1 = t % n1, i2 = t / n1;
+    c2 v = cmk(0, 0);
+    if (i2 < n2) {
+      const int j1 = a.gyj1[i1], j2 = a.gyj2[i2];
+      const float w1 = a.gyw1[i1], w2 = a.gyw2[i2];
+      const c2 *b = a.bufy + off + j1 + (long)N * j2;
+      v = crmul((1.f - w1) * (1.f - w2), b[0]);
+      v = crfma(v, w1 * (1.f - w2), b[1]);
+      v = crfma(v, (1.f - w1) * w2, b[N]);
+      v = crfma(v, w1 * w2, b[N + 1]);
+      v = crmul(inn, v);
+    }
+    a.hyg[r * (long)ny + t] = v;
+  }
+}
+
+// ----------------------------------------------------------------- LU-SGS
+// The SGS sweeps of lusgs_apply (fdfd.c:287-510) in FP32 for R right-hand
+// sides, with the same slices, diagonal wavefront and skewed per-cell data
+// as the FP64 kernels.  Written as a linear combination, the cell update of
+// fdfd.c:357-369 (Ey, Ez alike) is
+//   v = h*(beta*b - S + cAp*E(A+) + cAm*E(A-) + cBp*E(B+) + cBm*E(B-)),
+// with h = iwmu/aii, beta = i w mu0 (1/vol when fvm), S = (P1 - P2)/dA +
+// (Q3 - Q4)/dB the other components' terms, and 1-D coefficients
+// c*p = 1/(ds[u] d[u]), c*m = 1/(ds[u-1] d[u]) of the component's own
+// axes A and B (aii = (cAp+cAm+cBp+cBm)*iwmu - sigma).  The pre-pass folds
+// everything but the new neighbours into T (forward: the + neighbours are
+// 0; backward: the - neighbours hold forward values), so the wavefront
+// computes v = T + h*(cf*prev + cs*nbv): cf the fast axis' coefficient,
+// cs the slow axis', prev the row's previous cell, nbv the neighbour row.
+
+struct MxLu {
+  MxGeo g;
+  SkewGeo sk;
+  const c2 *b;   // R x n right-hand sides
+  c2 *y;         // R x n results
+  c2 *W;         // R x fE state on the full grid (Ex | Ey | Ez)
+  long fE, fEx, fEy;
+  c2 *T;         // records of the current sweep, skewed, R interleaved
+  c2 *h;         // h = iwmu/aii per swept cell, skewed Ex | Ey | Ez, R interleaved
+  c2 *trash;     // stores of lanes off the diagonal, 2 * SGS_MAXT * MX_RMAX
+  const float *c1m, *c1p, *c2m, *c2p, *c3m, *c3p;  // own-axis coefficients (z on the full grid)
+  MxFreq fr;
+};
+
+// h of every swept cell of component C for every r.
+template <int C, int R>
+__global__ __launch_bounds__(256) void k_mx_diag(MxLu a)
+{
+  const int A = a.sk.A[C], B = a.sk.B[C], q = blockIdx.y;
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= A * B) return;
+  int lo = 2, hi = A + B;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sk_off(mid, A, B) <= p) lo = mid; else hi = mid - 1;
+  }
+  const int s = max(1, lo - B) + (p - sk_off(lo, A, B)), f = lo - s;
+  int k, j, i;
+  lu_kji<C>(q, s, f, k, j, i);
+  const int n1 = a.g.n1, n2 = a.g.n2;
+  float c, sig;
+  if (C == 0) { c = a.c2p[j] + a.c2m[j] + a.c3p[k] + a.c3m[k]; sig = a.g.s11[id1(n1, n2, k, j, i)]; }
+  else if (C == 1) { c = a.c3p[k] + a.c3m[k] + a.c1p[i] + a.c1m[i]; sig = a.g.s22[id2(n1, n2, k, j, i)]; }
+  else { c = a.c1p[i] + a.c1m[i] + a.c2p[j] + a.c2m[j]; sig = a.g.s33[id3(n1, n2, k, j, i)]; }
+  const float vol = a.g.d1s[i] * a.g.d2[j] * a.g.d3[k];
+  const long o = (a.sk.base[C] + (long)q * A * B + p) * R;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const c2 iw = a.fr.iwmu[r];
+    // h = iwmu/aii (the fvm factor scales aij and aii alike and cancels)
+    const c2 aii = cmk(c * iw.x - sig, c * iw.y);
+    const float den = 1.f / (aii.x * aii.x + aii.y * aii.y);
+    a.h[o + r] = cmk((iw.x * aii.x + iw.y * aii.y) * den, (iw.y * aii.x - iw.x * aii.y) * den);
+  }
+  (void)vol;
+}
+
+// W value of component C at (k, j, i) for right-hand side r.
+#define MW0(r, k, j, i) a.W[(r) * a.fE + id1(n1, n2, k, j, i)]
+#define MW1(r, k, j, i) a.W[(r) * a.fE + a.fEx + id2(n1, n2, k, j, i)]
+#define MW2(r, k, j, i) a.W[(r) * a.fE + a.fEx + a.fEy + id3(n1, n2, k, j, i)]
+
+// Pre-pass of sweep (C, FWD) for every r: T at the skewed position.
+template <int C, bool FWD, int R>
+__global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
+{
+  const int n1 = a.g.n1, n2 = a.g.n2, n3 = a.g.n3f;
+  const int nx = (C == 0) ? n1 : n1 + 1, ny = (C == 1) ? n2 : n2 + 1;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= nx || j >= ny) return;
+  int q, s, f;
+  if (C == 0) { q = i; s = k; f = j; if (!(k >= 1 && k <= n3 - 1 && j >= 1 && j <= n2 - 1)) return; }
+  else if (C == 1) { q = j; s = k; f = i; if (!(k >= 1 && k <= n3 - 1 && i >= 1 && i <= n1 - 1)) return; }
+  else { q = k; s = j; f = i; if (!(j >= 1 && j <= n2 - 1 && i >= 1 && i <= n1 - 1)) return; }
+  const int A = a.sk.A[C], B = a.sk.B[C], d = s + f;
+  const long p = (long)q * A * B + sk_off(d, A, B) + (s - max(1, d - B));
+  const int kop = k - a.g.kofs;
+  const long xo = (C == 0) ? id1(n1, n2, kop, j, i) : (C == 1) ? a.g.lEx + id2(n1, n2, kop, j, i)
+                                                                 : a.g.lEx + a.g.lEy + id3(n1, n2, kop, j, i);
+  const float *r1 = a.g.r1, *r2 = a.g.r2, *r3 = a.g.r3, *r1s = a.g.r1s, *r2s = a.g.r2s, *r3s = a.g.r3s;
+  const float vol = a.g.d1s[i] * a.g.d2[j] * a.g.d3[k];
+  const c2 z0 = cmk(0, 0);
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    c2 S = z0, own = z0;
+    if (C == 0) {
+      if (!FWD) {  // Ey, Ez backward
+        const c2 P1 = csub(MW1(r, k, j, i + 1), MW1(r, k, j, i)), P2 = csub(MW1(r, k, j - 1, i + 1), MW1(r, k, j - 1, i));
+        const c2 Q3 = csub(MW2(r, k, j, i + 1), MW2(r, k, j, i)), Q4 = csub(MW2(r, k - 1, j, i + 1), MW2(r, k - 1, j, i));
+        S = crmul(r1s[i], cadd(crmul(r2[j], csub(P1, P2)), crmul(r3[k], csub(Q3, Q4))));
+        own = crfma(crmul(a.c2m[j], MW0(r, k, j - 1, i)), a.c3m[k], MW0(r, k - 1, j, i));
+      }
+    } else if (C == 1) {
+      const c2 Q3 = csub(MW0(r, k, j + 1, i), MW0(r, k, j, i)), Q4 = csub(MW0(r, k, j + 1, i - 1), MW0(r, k, j, i - 1));
+      c2 PP = z0;
+      if (!FWD) PP = csub(csub(MW2(r, k, j + 1, i), MW2(r, k, j, i)), csub(MW2(r, k - 1, j + 1, i), MW2(r, k - 1, j, i)));
+      S = crmul(r2s[j], cadd(crmul(r3[k], PP), crmul(r1[i], csub(Q3, Q4))));
+      if (!FWD) own = crfma(crmul(a.c3m[k], MW1(r, k - 1, j, i)), a.c1m[i], MW1(r, k, j, i - 1));
+    } else {
+      const c2 P1 = csub(MW0(r, k + 1, j, i), MW0(r, k, j, i)), P2 = csub(MW0(r, k + 1, j, i - 1), MW0(r, k, j, i - 1));
+      const c2 Q3 = csub(MW1(r, k + 1, j, i), MW1(r, k, j, i)), Q4 = csub(MW1(r, k + 1, j - 1, i), MW1(r, k, j - 1, i));
+      S = crmul(r3s[k], cadd(crmul(r1[i], csub(P1, P2)), crmul(r2[j], csub(Q3, Q4))));
+      if (!FWD) own = crfma(crmul(a.c1m[i], MW2(r, k, j, i - 1)), a.c2m[j], MW2(r, k, j - 1, i));
+    }
+    c2 bv = z0;
+    if (kop >= 0) bv = a.b[r * a.g.n + xo];
+    const c2 bb = a.g.fvm ? crmul(1.f / vol, bv) : cmul(a.fr.iomu[r], bv);
+    const c2 hh = a.h[(a.sk.base[C] + p) * R + r];
+    a.T[p * R + r] = cmul(hh, cadd(csub(bb, S), own));
+  }
+}
+#undef MW0
+#undef MW1
+#undef MW2
+
+template <int C, bool FWD, int R>
+__global__ __launch_bounds__(SGS_MAXT) void k_mx_sgs(MxLu a)
+{
+  extern __shared__ c2 xs[];
+  const int nslice = a.sk.ns[C];
+  const int per = (nslice + 7) / 8;
+  const int q = (blockIdx.x % 8) * per + blockIdx.x / 8;  // consecutive slices on one XCD
+  if (q >= nslice) return;
+  const int n1 = a.g.n1, n2 = a.g.n2;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int A = a.sk.A[C], B = a.sk.B[C], AB = A * B;
+  const int s = 1 + t, sc = min(s, A);
+  const int XW = (nt + 2) * R;  // exchange row: R values per row, zero rows at both ends
+  float *cf = (float *)(xs + 2 * XW);
+  int *dtab = (int *)(cf + (B + 2));
+  {
+    const float *g0 = (C == 0) ? (FWD ? a.c2m : a.c2p) : (FWD ? a.c1m : a.c1p);
+    for (int u = t; u <= B + 1; u += nt) cf[u] = g0[min(u, B)];
+    for (int d = t; d <= A + B; d += nt) dtab[d] = d >= 2 ? sk_off(d, A, B) - max(1, d - B) : 0;
+    for (int u = t; u < 2 * XW; u += nt) xs[u] = cmk(0, 0);
+  }
+  const float *gs = (C == 2) ? (FWD ? a.c2m : a.c2p) : (FWD ? a.c3m : a.c3p);
+  const float cs = gs[sc];
+  const c2 *Tc = a.T + (long)q * AB * R;
+  const c2 *Hc = a.h + (a.sk.base[C] + (long)q * AB) * R;
+  const long PL = (C == 0) ? (long)n1 * (n2 + 1) : (C == 1) ? (long)(n1 + 1) * n2 : (long)(n1 + 1) * (n2 + 1);
+  const long SS = (C == 2) ? (long)(n1 + 1) : PL;
+  const int FS = (C == 0) ? n1 : 1;
+  const long nb = ((C == 0) ? (long)q : (C == 1) ? (long)(n1 + 1) * q : PL * q) + s * SS;
+  const long wofs = (C == 0) ? 0 : (C == 1) ? a.fEx : a.fEx + a.fEy;
+  const long xofs = (C == 0) ? 0 : (C == 1) ? a.g.lEx : a.g.lEx + a.g.lEy;
+  const int kop = ((C == 2) ? q : s) - a.g.kofs;
+  const long yb = xofs + nb - (long)a.g.kofs * PL;
+  const int dlo = 2, dhi = A + B, nst = dhi - dlo + 1;
+  const c2 z0 = cmk(0, 0);
+  __syncthreads();
+  struct St { c2 T[R], H[R]; };
+  auto load = [&](St &c, int st) {
+    const int d = min(FWD ? dlo + st : dhi - st, dhi);
+    const long e = (long)min(max(dtab[max(d, dlo)] + sc, 0), AB - 1) * R;
+#pragma unroll
+    for (int r = 0; r < R; r++) { c.T[r] = Tc[e + r]; c.H[r] = Hc[e + r]; }
+  };
+  St ring[SGS_P];
+#pragma unroll
+  for (int u = 0; u < SGS_P; u++) load(ring[u], u);
+  c2 prev[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) prev[r] = z0;
+  const int nstp = ((nst + SGS_U - 1) / SGS_U) * SGS_U;
+  for (int st0 = 0; st0 < nstp; st0 += SGS_U) {
+#pragma unroll
+    for (int uu = 0; uu < SGS_U; uu++) {
+      const int st = st0 + uu, u = uu % SGS_P;
+      const int d = FWD ? dlo + st : dhi - st;
+      const int f = d - s;
+      const float cfv = cf[min(max(f, 0), B + 1)];
+      const bool ok = s <= A && f >= 1 && f <= B;
+      const long o = (long)f * FS;
+      const c2 *xr = xs + ((st + 1) & 1) * XW + (FWD ? t : t + 2) * R;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const c2 nbv = xr[r];
+        const c2 w = crfma(crmul(cfv, prev[r]), cs, nbv);
+        const c2 v = cfma(ring[u].T[r], ring[u].H[r], w);
+        if (FWD || C != 0) *(ok ? a.W + r * a.fE + wofs + nb + o : a.trash + r * SGS_MAXT + t) = v;
+        if (!FWD) *(ok && kop >= 0 ? a.y + r * a.g.n + yb + o : a.trash + (MX_RMAX + r) * SGS_MAXT + t) = v;
+        prev[r].x = ok ? v.x : prev[r].x;
+        prev[r].y = ok ? v.y : prev[r].y;
+      }
+      load(ring[u], st + SGS_P);
+#pragma unroll
+      for (int r = 0; r < R; r++) xs[(st & 1) * XW + (t + 1) * R + r] = prev[r];
+      __syncthreads();
+    }
+  }
+  if (!FWD) {  // operator-grid cells of this slice outside the sweep range are zero rows
+    const int n3 = a.g.n3f;
+    for (int r = 0; r < R; r++) {
+      c2 *yr = a.y + r * a.g.n;
+      if (C == 0) {
+        const int nk = a.g.n3 + 1;
+        for (int w = t; w < nk * (n2 + 1); w += nt) {
+          const int kq = w / (n2 + 1), j = w % (n2 + 1), kf = kq + a.g.kofs;
+          if (!(kf >= 1 && kf <= n3 - 1 && j >= 1 && j <= n2 - 1)) yr[id1(n1, n2, kq, j, q)] = z0;
+        }
+      } else if (C == 1) {
+        const int nk = a.g.n3 + 1;
+        for (int w = t; w < nk * (n1 + 1); w += nt) {
+          const int kq = w / (n1 + 1), i = w % (n1 + 1), kf = kq + a.g.kofs;
+          if (!(kf >= 1 && kf <= n3 - 1 && i >= 1 && i <= n1 - 1)) yr[a.g.lEx + id2(n1, n2, kq, q, i)] = z0;
+        }
+      } else if (q - a.g.kofs >= 0) {
+        for (int w = t; w < (n2 + 1) * (n1 + 1); w += nt) {
+          const int jj = w / (n1 + 1), i = w % (n1 + 1);
+          if (!(jj >= 1 && jj <= n2 - 1 && i >= 1 && i <= n1 - 1))
+            yr[a.g.lEx + a.g.lEy + id3(n1, n2, q - a.g.kofs, jj, i)] = z0;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- Krylov
+// Batched F-BiCGStab (solver.c:93-178) on FP32 vectors.  Dot products
+// accumulate in FP64 in the deterministic chunk order of the FP64 path (per
+// right-hand side: chunk, then thread, then the block fold); the scalars
+// and the convergence state live in FP64 on the device, so an iteration
+// can be replayed from a graph without the host.  Per-RHS scalar slots:
+// MS_USED marks the right-hand sides that take part in the current inner
+// solve (their correction is added to the FP64 solution).
+enum { MS_RHO = 0, MS_ALPHA = 2, MS_OMEGA = 4, MS_BETA = 6, MS_RS0 = 8, MS_REL = 9, MS_DONE = 10, MS_TOL = 11,
+       MS_IT = 12, MS_MAXIT = 13, MS_USED = 14, MS_N = 16 };
+
+struct MxKry {
+  long n;
+  int nch;                 // reduction chunks per vector
+  c2 *r, *r0, *p, *pp, *v, *s, *ss, *t, *x;  // R x n each
+  double *part;            // R x nch x 4 partial sums
+  double *sc;              // R x MS_N scalars
+};
+
+__device__ __forceinline__ bool mx_active(const MxKry &k, int r) { return k.sc[r * MS_N + MS_DONE] == 0.0; }
+
+// a*conj(b) accumulated in double
+__device__ __forceinline__ void mx_acc(double &re, double &im, c2 a, c2 b)
+{
+  re += (double)a.x * b.x + (double)a.y * b.y;
+  im += (double)a.y * b.x - (double)a.x * b.y;
+}
+// fold 4 doubles per thread over the block into part[r][chunk][0..3]
+__device__ __forceinline__ void mx_fold4(double *acc, double *out, double *sh)
+{
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const double v = block_fold(acc[q], sh);
+    if (threadIdx.x == 0) out[q] = v;
+  }
+}
+
+// init: r0 = p = r (r holds the inner right-hand side), x = 0;
+// partials of <r, r0> and <r, r>
+__global__ __launch_bounds__(RED_T) void k_mx_k_init(MxKry k)
+{
+  __shared__ double sh[RED_T];
+  const int r = blockIdx.y;
+  if (!mx_active(k, r)) return;
+  double acc[4] = {0, 0, 0, 0};
+  const long o = r * k.n, base = (long)blockIdx.x * RED_CHUNK + threadIdx.x;
+  for (int m = 0; m < RED_E; m++) {
+    const long e = base + (long)m * RED_T;
+    if (e < k.n) {
+      const c2 rv = k.r[o + e];
+      k.r0[o + e] = rv; k.p[o + e] = rv; k.x[o + e] = cmk(0, 0);
+      mx_acc(acc[0], acc[1], rv, rv);
+      mx_acc(acc[2], acc[3], rv, rv);
+    }
+  }
+  mx_fold4(acc, k.part + ((long)r * k.nch + blockIdx.x) * 4, sh);
+}
+// partials of <v, r0> (slot 0,1) and nothing else
+__global__ __launch_bounds__(RED_T) void k_mx_k_dot_vr0(MxKry k)
+{
+  __shared__ double sh[RED_T];
+  const int r = blockIdx.y;
+  if (!mx_active(k, r)) return;
+  double acc[4] = {0, 0, 0, 0};
+  const long o = r * k.n, base = (long)blockIdx.x * RED_CHUNK + threadIdx.x;
+  for (int m = 0; m < RED_E; m++) {
+    const long e = base + (long)m * RED_T;
+    if (e < k.n) mx_acc(acc[0], acc[1], k.v[o + e], k.r0[o + e]);
+  }
+  mx_fold4(acc, k.part + ((long)r * k.nch + blockIdx.x) * 4, sh);
+}
+// s = r - alpha v
+__global__ void k_mx_k_s(MxKry k)
+{
+  const int r = blockIdx.y;
+  if (!mx_active(k, r)) return;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= k.n) return;
+  const double *sc = k.sc + r * MS_N;
+  const c2 al = cmk((float)sc[MS_ALPHA], (float)sc[MS_ALPHA + 1]);
+  const long o = r * k.n + e;
+  const c2 v = k.v[o], rv = k.r[o];
+  k.s[o] = cmk(rv.x - (al.x * v.x - al.y * v.y), rv.y - (al.x * v.y + al.y * v.x));
+}
+// partials of <s, t> (0,1) and <t, t> (2,3)
+__global__ __launch_bounds__(RED_T) void k_mx_k_dot_st(MxKry k)
+{
+  __shared__ double sh[RED_T];
+  const int r = blockIdx.y;
+  if (!mx_active(k, r)) return;
+  double acc[4] = {0, 0, 0, 0};
+  const long o = r * k.n, base = (long)blockIdx.x * RED_CHUNK + threadIdx.x;
+  for (int m = 0; m < RED_E; m++) {
+    const long e = base + (long)m * RED_T;
+    if (e < k.n) {
+      const c2 tv = k.t[o + e];
+      mx_acc(acc[0], acc[1], k.s[o + e], tv);
+      mx_acc(acc[2], acc[3], tv, tv);
+    }
+  }
+  mx_fold4(acc, k.part + ((long)r * k.nch + blockIdx.x) * 4, sh);
+}
+// x += alpha pp + omega ss; r = s - omega t; partials of <r, r0>, <r, r>
+__global__ __launch_bounds__(RED_T) void k_mx_k_xr(MxKry k)
+{
+  __shared__ double sh[RED_T];
+  const int r = blockIdx.y;
+  if (!mx_active(k, r)) return;
+  const double *sc = k.sc + r * MS_N;
+  const c2 al = cmk((float)sc[MS_ALPHA], (float)sc[MS_ALPHA + 1]);
+  const c2 om = cmk((float)sc[MS_OMEGA], (float)sc[MS_OMEGA + 1]);
+  double acc[4] = {0, 0, 0, 0};
+  const long o = r * k.n, base = (long)blockIdx.x * RED_CHUNK + threadIdx.x;
+  for (int m = 0; m < RED_E; m++) {
+    const long e = base + (long)m * RED_T;
+    if (e < k.n) {
+      const long q = o + e;
+      k.x[q] = cfma(cfma(k.x[q], al, k.pp[q]), om, k.ss[q]);
+      const c2 t = k.t[q], sv = k.s[q];
+      const c2 rv = cmk(sv.x - (om.x * t.x - om.y * t.y), sv.y - (om.x * t.y + om.y * t.x));
+      k.r[q] = rv;
+      mx_acc(acc[0], acc[1], rv, k.r0[q]);
+      mx_acc(acc[2], acc[3], rv, rv);
+    }
+  }
+  mx_fold4(acc, k.part + ((long)r * k.nch + blockIdx.x) * 4, sh);
+}
+// p = r + beta (p - omega v)
+__global__ void k_mx_k_p(MxKry k)
+{
+  const int r = blockIdx.y;
+  if (!mx_active(k, r)) return;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= k.n) return;
+  const double *sc = k.sc + r * MS_N;
+  const c2 be = cmk((float)sc[MS_BETA], (float)sc[MS_BETA + 1]);
+  const c2 om = cmk((float)sc[MS_OMEGA], (float)sc[MS_OMEGA + 1]);
+  const long q = r * k.n + e;
+  const c2 v = k.v[q], pv = k.p[q];
+  const c2 d = cmk(pv.x - (om.x * v.x - om.y * v.y), pv.y - (om.x * v.y + om.y * v.x));
+  k.p[q] = cfma(k.r[q], be, d);
+}
+
+// Sum the chunk partials of right-hand side r (fixed order) into out[0..3].
+__device__ __forceinline__ void mx_sum_parts(const MxKry &k, int r, double *out, double *sh)
+{
+  double acc[4] = {0, 0, 0, 0};
+  for (int c = threadIdx.x; c < k.nch; c += blockDim.x) {
+    const double *pp = k.part + ((long)r * k.nch + c) * 4;
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[q] += pp[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const double v = block_fold(acc[q], sh);
+    if (threadIdx.x == 0) out[q] = v;
+  }
+}
+__device__ __forceinline__ void zdiv_d(double ar, double ai, double br, double bi, double *o)
+{
+  const double den = br * br + bi * bi;
+  o[0] = (ar * br + ai * bi) / den;
+  o[1] = (ai * br - ar * bi) / den;
+}
+// Scalar steps, one block of RED_T threads per right-hand side.
+// stage 0: after init (rho, rs0); 1: alpha; 2: omega; 3: rho_new, beta,
+// relres, convergence.
+__global__ __launch_bounds__(RED_T) void k_mx_k_fin(MxKry k, int stage)
+{
+  __shared__ double sh[RED_T];
+  __shared__ double d[4];
+  const int r = blockIdx.x;
+  double *sc = k.sc + r * MS_N;
+  if (sc[MS_DONE] != 0.0) return;
+  mx_sum_parts(k, r, d, sh);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double o[2];
+  if (stage == 0) {
+    sc[MS_RHO] = d[0]; sc[MS_RHO + 1] = d[1]; sc[MS_RS0] = d[2]; sc[MS_REL] = 1.0; sc[MS_IT] = 0;
+    if (!(d[2] > 0.0)) sc[MS_DONE] = 1.0;
+  } else if (stage == 1) {  // alpha = rho / <v, r0>
+    zdiv_d(sc[MS_RHO], sc[MS_RHO + 1], d[0], d[1], o);
+    sc[MS_ALPHA] = o[0]; sc[MS_ALPHA + 1] = o[1];
+  } else if (stage == 2) {  // omega = <s, t> / Re<t, t>
+    sc[MS_OMEGA] = d[0] / d[2]; sc[MS_OMEGA + 1] = d[1] / d[2];
+  } else {
+    const double rel = sqrt(d[2] / sc[MS_RS0]);
+    sc[MS_REL] = rel;
+    sc[MS_IT] += 1.0;
+    // beta = (rho_new / rho_old) * alpha / omega
+    double q1[2], q2[2];
+    zdiv_d(d[0], d[1], sc[MS_RHO], sc[MS_RHO + 1], q1);
+    const double ar = q1[0] * sc[MS_ALPHA] - q1[1] * sc[MS_ALPHA + 1];
+    const double ai = q1[0] * sc[MS_ALPHA + 1] + q1[1] * sc[MS_ALPHA];
+    zdiv_d(ar, ai, sc[MS_OMEGA], sc[MS_OMEGA + 1], q2);
+    sc[MS_BETA] = q2[0]; sc[MS_BETA + 1] = q2[1];
+    sc[MS_RHO] = d[0]; sc[MS_RHO + 1] = d[1];
+    if (rel < sc[MS_TOL] || sc[MS_IT] >= sc[MS_MAXIT] || !(rel == rel)) sc[MS_DONE] = 1.0;
+  }
+}

@@ -158,6 +158,34 @@ class Session:
         self._status("solve")
         return x, L.csem_last_iterations()
 
+    def solve_mixed(self, bs, freqs, tols, opts=None):
+        """Batched mixed-precision solve (csem_mixed_solve) of the right-hand
+        sides bs (list of complex128 vectors), RHS r at frequency freqs[r]
+        to true relative residual tols[r].  Returns (xs, iters, relres,
+        converged count)."""
+        L = lib()
+        R, n = len(bs), self.n
+        fr = (_lib.FreqT * R)(*[freq_consts(f) for f in freqs])
+        db, dx = C.c_void_p(), C.c_void_p()
+        _lib.check(L.csem_malloc(C.byref(db), R * n * 16), "csem_malloc")
+        _lib.check(L.csem_malloc(C.byref(dx), R * n * 16), "csem_malloc")
+        try:
+            hb = np.ascontiguousarray(np.concatenate(bs), np.complex128)
+            _lib.check(L.csem_memcpy_h2d(db, _p(hb), R * n * 16, None), "upload")
+            tol = np.ascontiguousarray(tols, np.float64)
+            its = np.zeros(R, np.int32)
+            rel = np.zeros(R)
+            o = opts if opts is not None else _lib.MixedOpts.default()
+            nconv = L.csem_mixed_solve(self.model, R, fr, dx, db, _p(tol), C.byref(o), _p(its), _p(rel), None)
+            _lib.check(nconv, "csem_mixed_solve")
+            hx = np.zeros(R * n, np.complex128)
+            _lib.check(L.csem_memcpy_d2h(_p(hx), dx, R * n * 16, None), "download")
+            _lib.check(L.csem_sync(None), "sync")
+        finally:
+            L.csem_free(db)
+            L.csem_free(dx)
+        return [hx[r * n:(r + 1) * n].copy() for r in range(R)], its, rel, nconv
+
     def _status(self, what):
         st = lib().csem_last_status()
         if st < 0:

@@ -103,6 +103,46 @@ int csem_scatter_add(long nnz, const long *idx, const double *val, void *b, void
 int csem_gather(int nrec, const long *idx, const double *wts, const double *sig,
                 const double *scale, const void *x, double *out, void *stream);
 
+/* Vector primitives the host-side GMRES uses (solver.c:423-499):
+ * o = a - b; o = a / r (real r); o = a / (re + i im) (Smith); y -= h x
+ * with h two doubles in device memory; y += (re + i im) x. */
+int csem_vec_sub(long n, const void *a, const void *b, void *o, void *stream);
+int csem_vec_divr(long n, const void *a, double r, void *o, void *stream);
+int csem_vec_divz(long n, const void *a, double re, double im, void *o, void *stream);
+int csem_vec_subscaled_dev(long n, void *y, const void *h, const void *x, void *stream);
+int csem_vec_addscaled(long n, void *y, double re, double im, const void *x, void *stream);
+
+/* ---- Batched mixed-precision solve (no counterpart in the reference: the
+ * multi-right-hand-side, FP32-inner path for throughput callers).
+ *
+ * R (1..4) right-hand sides of one model, RHS r with frequency constants
+ * f[r].  x and b are R device vectors of csem_model_len complex doubles,
+ * back to back.  Solves A_r x_r = b_r from x = 0 to a true FP64 relative
+ * residual |b - A x| / |b| < tol[r] by FP64 iterative refinement around an
+ * FP32 LU-SGS + F-BiCGStab inner solve with FP64 dot products.  Returns
+ * the number of right-hand sides that reached tol, or < 0 on error;
+ * iters[r] receives the inner iterations spent, relres[r] the final true
+ * relative residual (host arrays).  opt may be NULL for the defaults. */
+typedef struct {
+  int max_inner;     /* inner iteration budget per right-hand side (default 2000) */
+  double inner_tol;  /* reduction asked of one inner solve (1e-2: FP32
+                        BiCGStab stagnates when asked for much more) */
+  int max_outer;     /* refinement steps (40) */
+  int check_every;   /* inner iterations between convergence checks (8) */
+  int use_graph;     /* replay a captured hipGraph of one inner iteration (1) */
+} csem_mixed_opts;
+int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *x, const void *b, const double *tol,
+                     const csem_mixed_opts *opt, int *iters, double *relres, void *stream);
+/* The FP32 operator and preconditioner on R complex-float vectors back to
+ * back (length R * csem_model_len), for tests and custom drivers. */
+int csem_apply32(csem_model *m, int R, const csem_freq *f, const void *x, void *y, void *stream);
+int csem_lusgs32(csem_model *m, int R, const csem_freq *f, const void *x, void *y, void *stream);
+/* Tile shape the autotuner picked for the FP32 operator at batch size R
+ * (rows per block, planes per block); -1 before the first solve. */
+int csem_mixed_tuning(csem_model *m, int R, int *rows, int *planes);
+/* device bytes held by the mixed-precision state of a model */
+long csem_mixed_bytes(const csem_model *m);
+
 /* Kernel timers: csem_kernel_timing(1) resets and starts HIP-event timing
  * of every hot-kernel launch (events recorded on the launch stream);
  * csem_kernel_stats(id, ...) returns the kernel's name, summed duration,

@@ -1,0 +1,190 @@
+"""Batched mixed-precision path on the GPU, checked against the FP64
+oracle (the pinned CPU restatement of the reference) and against the FP64
+device kernels.
+
+Tolerances (stated here and in the assertions):
+  * FP32 operator vs FP64 operator: norm-wise relative 2e-6 on random
+    fields (FP32 rounding of inputs and of the curl-curl differences);
+  * FP32 LU-SGS vs FP64 LU-SGS: norm-wise relative 1e-4 (a preconditioner:
+    only its action needs to be close);
+  * mixed solve vs the oracle's FP64 F-BiCGStab, both to tol = 1e-8/f:
+    every receiver value within 1e-6 of the channel's largest value, and
+    the true FP64 residual |b - A x| / |b| (oracle operator) below tol;
+  * batched vs alone: the same receivers within 1e-6 (channel-normalised);
+  * graph replay vs eager launches: bit-identical.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import oracle as O  # noqa: E402
+
+from demo_matrixfree_csem_amd import surveys, _lib  # noqa: E402
+from demo_matrixfree_csem_amd.session import Session, freq_consts  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+FREQS = [0.25, 0.75, 1.25]
+
+
+def _setup(workdir, seed=2, **kw):
+    s = surveys.tiny(seed, airbc=1, **kw)
+    s.flags.update(n1fft=64, n2fft=64)
+    s.freqs = list(FREQS)
+    surveys.write_files(workdir, s)
+    ses = Session(s.argv(workdir))
+    m = s.model
+    orc = O.Oracle(m.x1node, m.x2node, m.x3node, m.rho11, m.rho22, m.rho33, nb=s.flags["nb"], airbc=1, nfft=64)
+    return s, ses, orc
+
+
+def _dev(a):
+    d = C.c_void_p()
+    _lib.check(_lib.lib().csem_malloc(C.byref(d), a.nbytes), "malloc")
+    _lib.check(_lib.lib().csem_memcpy_h2d(d, a.ctypes.data, a.nbytes, None), "h2d")
+    return d
+
+
+def _host(d, like):
+    out = np.empty_like(like)
+    _lib.check(_lib.lib().csem_memcpy_d2h(out.ctypes.data, d, out.nbytes, None), "d2h")
+    _lib.check(_lib.lib().csem_sync(None), "sync")
+    return out
+
+
+def _tol(f):
+    return float(np.float32(np.float32(1e-8) / np.float32(f)))
+
+
+def test_apply32_close_to_fp64(workdir):
+    print("case apply32", flush=True)
+    s, ses, orc = _setup(workdir)
+    L = _lib.lib()
+    try:
+        n, R = ses.n, 3
+        rng = np.random.default_rng(11)
+        x = (rng.standard_normal((R, n)) + 1j * rng.standard_normal((R, n))).astype(np.complex64)
+        fr = (_lib.FreqT * R)(*[freq_consts(f) for f in FREQS])
+        dx, dy = _dev(x), _dev(np.zeros_like(x))
+        _lib.check(L.csem_apply32(ses.model, R, fr, dx, dy, None), "apply32")
+        y32 = _host(dy, x)
+        for r, f in enumerate(FREQS):
+            orc.set_freq(f)
+            y64 = orc.apply(x[r].astype(np.complex128))
+            err = np.linalg.norm(y32[r] - y64) / np.linalg.norm(y64)
+            assert err < 2e-6, (f, err)
+        # the same right-hand side alone gives the same values
+        dx1, dy1 = _dev(x[1:2].copy()), _dev(np.zeros_like(x[1:2]))
+        _lib.check(L.csem_apply32(ses.model, 1, (_lib.FreqT * 1)(fr[1]), dx1, dy1, None), "apply32")
+        y1 = _host(dy1, x[1:2])
+        assert np.abs(y1[0] - y32[1]).max() <= 1e-6 * np.abs(y32[1]).max()
+        for d in (dx, dy, dx1, dy1):
+            L.csem_free(d)
+    finally:
+        ses.close()
+
+
+def test_lusgs32_close_to_fp64(workdir):
+    print("case lusgs32", flush=True)
+    s, ses, orc = _setup(workdir)
+    L = _lib.lib()
+    try:
+        n, R = ses.n, 3
+        rng = np.random.default_rng(12)
+        x = (rng.standard_normal((R, n)) + 1j * rng.standard_normal((R, n))).astype(np.complex64)
+        fr = (_lib.FreqT * R)(*[freq_consts(f) for f in FREQS])
+        dx, dy = _dev(x), _dev(np.zeros_like(x))
+        _lib.check(L.csem_lusgs32(ses.model, R, fr, dx, dy, None), "lusgs32")
+        y32 = _host(dy, x)
+        for r, f in enumerate(FREQS):
+            orc.set_freq(f)
+            y64 = orc.lusgs(x[r].astype(np.complex128))
+            err = np.linalg.norm(y32[r] - y64) / np.linalg.norm(y64)
+            assert err < 1e-4, (f, err)
+        L.csem_free(dx)
+        L.csem_free(dy)
+    finally:
+        ses.close()
+
+
+def _receivers(ses, x):
+    return ses.extract(x).astype(np.complex128)
+
+
+def _rec_err(a, b):
+    """max over channels of max |a - b| / max |b| (channel-normalised)."""
+    scale = np.abs(b).max(axis=1, keepdims=True)
+    return float((np.abs(a - b) / np.where(scale > 0, scale, 1)).max())
+
+
+def test_mixed_solve_matches_fp64_oracle(workdir):
+    print("case mixed vs oracle", flush=True)
+    s, ses, orc = _setup(workdir)
+    try:
+        bs, tols = [], []
+        for f in FREQS:
+            ses.set_freq(f)
+            bs.append(ses.inject())
+            tols.append(_tol(f))
+        xs, its, rel, nconv = ses.solve_mixed(bs, FREQS, tols)
+        assert nconv == 3, (its, rel)
+        for r, f in enumerate(FREQS):
+            orc.set_freq(f)
+            orc.set_red(O.RED_CHUNKED)
+            xo, ito, _ = orc.fbicgstab(bs[r], 2000, tols[r])
+            res = np.linalg.norm(bs[r] - orc.apply(xs[r])) / np.linalg.norm(bs[r])
+            assert res < tols[r], (f, res)
+            e = _rec_err(_receivers(ses, xs[r]), _receivers(ses, xo))
+            print("f=%g inner iterations %d (fp64 %d) true relres %.2e receiver error %.2e"
+                  % (f, its[r], ito, res, e), flush=True)
+            assert e < 1e-6, (f, e)
+    finally:
+        ses.close()
+
+
+def test_mixed_batch_equals_alone(workdir):
+    print("case mixed batch vs alone", flush=True)
+    s, ses, orc = _setup(workdir, seed=3)
+    try:
+        bs, tols = [], []
+        for f in FREQS:
+            ses.set_freq(f)
+            bs.append(ses.inject())
+            tols.append(_tol(f))
+        xs, its, rel, nconv = ses.solve_mixed(bs, FREQS, tols)
+        assert nconv == 3
+        for r, f in enumerate(FREQS):
+            x1, it1, rel1, nc1 = ses.solve_mixed([bs[r]], [f], [tols[r]])
+            assert nc1 == 1
+            e = _rec_err(_receivers(ses, xs[r]), _receivers(ses, x1[0]))
+            print("f=%g batched %d alone %d iterations, receiver difference %.2e" % (f, its[r], it1[0], e),
+                  flush=True)
+            assert e < 1e-6, (f, e)
+    finally:
+        ses.close()
+
+
+def test_mixed_graph_replay_equals_eager(workdir):
+    print("case mixed graph vs eager", flush=True)
+    s, ses, orc = _setup(workdir, seed=4)
+    try:
+        bs, tols = [], []
+        for f in FREQS[:2]:
+            ses.set_freq(f)
+            bs.append(ses.inject())
+            tols.append(_tol(f))
+        xg, itg, relg, _ = ses.solve_mixed(bs, FREQS[:2], tols, _lib.MixedOpts.default(use_graph=1))
+        xe, ite, rele, _ = ses.solve_mixed(bs, FREQS[:2], tols, _lib.MixedOpts.default(use_graph=0))
+        assert list(itg) == list(ite)
+        for a, b in zip(xg, xe):
+            assert np.array_equal(a, b)
+        # replaying again after eager work gives the same result
+        xg2, itg2, _, _ = ses.solve_mixed(bs, FREQS[:2], tols, _lib.MixedOpts.default(use_graph=1))
+        assert list(itg2) == list(itg)
+        for a, b in zip(xg2, xg):
+            assert np.array_equal(a, b)
+    finally:
+        ses.close()
