@@ -7,6 +7,7 @@
 #include <string.h>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 
 #include "csem_internal.h"
 
@@ -60,13 +61,15 @@ static hipEvent_t kt_event()
   return e;
 }
 
-static int g_kt_paused;
-// timing is paused while a stream is captured into a graph
+static std::mutex g_kt_mu;  // solves on several host threads record concurrently
+static thread_local int g_kt_paused;
+// timing is paused while this thread captures a stream into a graph
 void kt_pause(int paused) { g_kt_paused = paused; }
 
 hipEvent_t kt_begin(hipStream_t s)
 {
   if (!g_kt.on || g_kt_paused) return NULL;
+  std::lock_guard<std::mutex> lk(g_kt_mu);
   hipEvent_t e = kt_event();
   if (e) (void)hipEventRecord(e, s);
   return e;
@@ -75,6 +78,7 @@ hipEvent_t kt_begin(hipStream_t s)
 void kt_end(int id, hipEvent_t a, hipStream_t s, double bytes)
 {
   if (!g_kt.on || !a) return;
+  std::lock_guard<std::mutex> lk(g_kt_mu);
   hipEvent_t b = kt_event();
   if (!b) return;
   (void)hipEventRecord(b, s);
@@ -83,6 +87,7 @@ void kt_end(int id, hipEvent_t a, hipStream_t s, double bytes)
 
 static void kt_collect()
 {
+  std::lock_guard<std::mutex> lk(g_kt_mu);
   for (auto &p : g_kt.pend) {
     float ms = 0;
     (void)hipEventSynchronize(p.b);
@@ -290,23 +295,29 @@ static ApplyArgs apply_args(const csem_model *m, const csem_freq *f, const z2 *x
   return a;
 }
 
-static AirArgs air_args(const csem_model *m)
+static AirArgs air_args(const csem_model *m, const AirBufs &b)
 {
   AirArgs a;
   a.N = m->nfft; a.n1 = m->n1; a.n2 = m->n2;
-  a.hz = m->hz0; a.buf = m->buf; a.bufx = m->bufx; a.bufy = m->bufy;
+  a.hz = b.hz0; a.buf = b.buf; a.bufx = b.bufx; a.bufy = b.bufy;
   a.sH1 = m->sH1; a.sH2 = m->sH2; a.tw = m->tw;
   a.jx = m->jx; a.jy = m->jy; a.wx = m->wx; a.wy = m->wy;
   a.rows = m->rows; a.nrowx = m->nrowx; a.nrowy = m->nrowy;
   a.gxj1 = m->gxj1; a.gxj2 = m->gxj2; a.gyj1 = m->gyj1; a.gyj2 = m->gyj2;
   a.gxw1 = m->gxw1; a.gxw2 = m->gxw2; a.gyw1 = m->gyw1; a.gyw2 = m->gyw2;
-  a.hxg = m->hxg; a.hyg = m->hyg;
+  a.hxg = b.hxg; a.hyg = b.hyg;
   return a;
 }
 
-static int run_airbc(csem_model *m, hipStream_t s)
+// the model's own scratch (the public entry points use it)
+static AirBufs model_bufs(const csem_model *m)
 {
-  AirArgs a = air_args(m);
+  return AirBufs{m->hz0, m->buf, m->bufx, m->bufy, m->hxg, m->hyg};
+}
+
+static int run_airbc(csem_model *m, hipStream_t s, const AirBufs &b)
+{
+  AirArgs a = air_args(m, b);
   const int N = m->nfft;
   const size_t row_lds = 2 * (size_t)N * sizeof(z2);
   const double NN = (double)N * N;
@@ -320,13 +331,16 @@ static int run_airbc(csem_model *m, hipStream_t s)
   return 0;
 }
 
-static int run_apply(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStream_t s)
+// y = A x with the airwave scratch b: concurrent callers on one model
+// pass scratch of their own.
+int run_apply_bufs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStream_t s, const AirBufs &b)
 {
   ApplyArgs a = apply_args(m, f, x, y);
+  a.hxg = b.hxg; a.hyg = b.hyg;
   if (m->airbc) {
     const int nh = m->n1 * m->n2;
-    KT(KT_HZ0, s, 16.0 * nh, (k_hz0<<<(nh + 255) / 256, 256, 0, s>>>(a, m->hz0)));
-    int rc = run_airbc(m, s);
+    KT(KT_HZ0, s, 16.0 * nh, (k_hz0<<<(nh + 255) / 256, 256, 0, s>>>(a, b.hz0)));
+    int rc = run_airbc(m, s, b);
     if (rc) return rc;
   }
   dim3 blk(64, AP_TJ + 1);
@@ -334,6 +348,11 @@ static int run_apply(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<AP_TJ><<<grd, blk, 0, s>>>(a)));
   HIP_OK(hipGetLastError());
   return 0;
+}
+
+static int run_apply(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStream_t s)
+{
+  return run_apply_bufs(m, f, x, y, s, model_bufs(m));
 }
 
 static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStream_t s)
@@ -424,7 +443,7 @@ extern "C" int csem_airbc(csem_model *m, const void *hz0, void *hxg, void *hyg, 
   if (!m->airbc) return csem_fail(-3, "model has no airwave boundary");
   hipStream_t s = S(stream);
   HIP_OK(hipMemcpyAsync(m->hz0, hz0, (size_t)m->n1 * m->n2 * sizeof(z2), hipMemcpyDeviceToDevice, s));
-  int rc = run_airbc(m, s);
+  int rc = run_airbc(m, s, model_bufs(m));
   if (rc) return rc;
   HIP_OK(hipMemcpyAsync(hxg, m->hxg, (size_t)(m->n1 + 1) * m->n2 * sizeof(z2), hipMemcpyDeviceToDevice, s));
   HIP_OK(hipMemcpyAsync(hyg, m->hyg, (size_t)m->n1 * (m->n2 + 1) * sizeof(z2), hipMemcpyDeviceToDevice, s));

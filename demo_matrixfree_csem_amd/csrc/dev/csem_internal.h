@@ -76,3 +76,9 @@ static inline int upload(csem_model *m, T **p, const void *src, long count)
 }
 
 void mixed_free(csem_model *m);
+
+// scratch of the airwave boundary of one FP64 operator application
+struct AirBufs {
+  z2 *hz0, *buf, *bufx, *bufy, *hxg, *hyg;
+};
+int run_apply_bufs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStream_t s, const AirBufs &b);

@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 // the FP64 kernels live in csem_dev.hip; this unit needs their types only
 #define CSEM_TYPES_ONLY
@@ -21,30 +22,37 @@
 
 void kt_pause(int paused);
 
+// FP32 copy of the model, shared by every solve on it.
 struct MixState {
-  int R;       // right-hand sides the buffers hold
-  long n, fE;
-  // FP32 model
   float *d1, *d2, *d3, *d1s, *d2s, *d3s, *r1, *r2, *r3, *r1s, *r2s, *r3s;
   float *s11, *s22, *s33, *c1m, *c1p, *c2m, *c2p, *c3m, *c3p;
   c2 *sH1, *sH2, *tw;
   float *wx, *wy, *gxw1, *gxw2, *gyw1, *gyw2;
   int cw;
-  c2 *buf, *bufx, *bufy, *hz0, *hxg, *hyg;
-  // LU-SGS
-  c2 *W, *T, *h, *trash;
-  float hkey[MX_RMAX][2];
+  // autotuned operator tile per batch size: rows per block, planes per block
+  int ap_tj[MX_RMAX + 1], ap_kc[MX_RMAX + 1];
+  std::mutex mu;                 // guards slots and tuning
+  std::vector<struct MixSlot *> slots;
+  std::vector<void *> allocs;
+  long bytes;
+};
+
+// Buffers of one solve in flight: solves on several host threads (each on
+// its own stream) use different slots of the same model concurrently.
+struct MixSlot {
+  int R;      // right-hand sides the buffers hold
+  int busy;
+  c2 *buf, *bufx, *bufy, *hz0, *hxg, *hyg;  // FP32 airwave scratch, R of each
+  AirBufs a64;                              // FP64 airwave scratch (true residual)
+  c2 *W, *T, *h, *trash;                    // LU-SGS
+  float hkey[MX_RMAX][2];                   // frequencies the diagonal factors h hold
   int hR;
-  // Krylov
-  c2 *kv;
+  c2 *kv;                                   // Krylov vectors
   double *part, *sc;
   int nch;
-  // refinement
-  z2 *res, *tmp64;
-  double *norm;  // device scratch for FP64 dots
-  float *scale;  // R inverse residual norms
-  // autotuned operator tile: rows per block and planes per block
-  int ap_tj[MX_RMAX + 1], ap_kc[MX_RMAX + 1];
+  z2 *res, *tmp64;                          // refinement
+  double *npart, *norm;                     // FP64 residual norms
+  float *scale;
   // captured inner iteration (a graph cannot be captured from the legacy
   // default stream: NULL callers run on this stream of default priority)
   hipStream_t own;
@@ -56,8 +64,8 @@ struct MixState {
   long bytes;
 };
 
-template <class T>
-static int mx_dalloc(MixState *x, T **p, long count)
+template <class S, class T>
+static int mx_dalloc(S *x, T **p, long count)
 {
   long bytes = count * (long)sizeof(T);
   if (bytes <= 0) bytes = 256;
@@ -68,12 +76,19 @@ static int mx_dalloc(MixState *x, T **p, long count)
   return 0;
 }
 
+static void slot_free(MixSlot *q)
+{
+  if (q->gexec) (void)hipGraphExecDestroy(q->gexec);
+  if (q->own) (void)hipStreamDestroy(q->own);
+  for (void *p : q->allocs) (void)hipFree(p);
+  delete q;
+}
+
 void mixed_free(csem_model *m)
 {
   MixState *x = m->mx;
   if (!x) return;
-  if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
-  if (x->own) (void)hipStreamDestroy(x->own);
+  for (MixSlot *q : x->slots) slot_free(q);
   for (void *p : x->allocs) (void)hipFree(p);
   delete x;
   m->mx = NULL;
@@ -133,25 +148,24 @@ static MxGeo mx_geo(const csem_model *m, const MixState *x)
   return g;
 }
 
-static int mx_state(csem_model *m, int R, MixState **out)
+#define MA(o, p, cnt) if ((rc = mx_dalloc(o, &o->p, cnt))) goto fail
+
+// The FP32 model, once per model (callers hold no lock: first use only).
+static int mx_model(csem_model *m, MixState **out)
 {
-  if (R < 1 || R > MX_RMAX) return csem_fail(-3, "batch of %d right-hand sides outside [1, %d]", R, MX_RMAX);
-  if (m->mx && m->mx->R >= R) { *out = m->mx; return 0; }
-  mixed_free(m);
+  if (m->mx) { *out = m->mx; return 0; }
   MixState *x = new MixState();
-  m->mx = x;
-  x->R = R; x->n = m->lE; x->fE = m->fE;
   const int n1 = m->n1, n2 = m->n2, n3f = m->n3f;
   const hipStream_t s = 0;
   int rc;
-#define MA(p, cnt) if ((rc = mx_dalloc(x, &x->p, cnt))) goto fail
-#define CV(dst, src, cnt, rcp) MA(dst, cnt); k_mx_cvt<<<GE(cnt)>>>(cnt, m->src, x->dst, rcp)
+#define CV(dst, src, cnt, rcp) MA(x, dst, cnt); k_mx_cvt<<<GE(cnt)>>>(cnt, m->src, x->dst, rcp)
   CV(d1, d1, n1 + 1, 0); CV(d2, d2, n2 + 1, 0); CV(d3, d3, n3f + 1, 0);
   CV(d1s, d1s, n1 + 1, 0); CV(d2s, d2s, n2 + 1, 0); CV(d3s, d3s, n3f + 1, 0);
   CV(r1, d1, n1 + 1, 1); CV(r2, d2, n2 + 1, 1); CV(r3, d3, n3f + 1, 1);
   CV(r1s, d1s, n1 + 1, 1); CV(r2s, d2s, n2 + 1, 1); CV(r3s, d3s, n3f + 1, 1);
   CV(s11, s11, m->fEx, 0); CV(s22, s22, m->fEy, 0); CV(s33, s33, m->fEz, 0);
-  MA(c1m, n1 + 1); MA(c1p, n1 + 1); MA(c2m, n2 + 1); MA(c2p, n2 + 1); MA(c3m, n3f + 1); MA(c3p, n3f + 1);
+  MA(x, c1m, n1 + 1); MA(x, c1p, n1 + 1); MA(x, c2m, n2 + 1); MA(x, c2p, n2 + 1); MA(x, c3m, n3f + 1);
+  MA(x, c3p, n3f + 1);
   k_mx_coef<<<GE(n1 + 1)>>>(n1, m->d1s, m->d1, x->c1m, x->c1p);
   k_mx_coef<<<GE(n2 + 1)>>>(n2, m->d2s, m->d2, x->c2m, x->c2p);
   k_mx_coef<<<GE(n3f + 1)>>>(n3f, m->d3s, m->d3, x->c3m, x->c3p);
@@ -164,42 +178,89 @@ static int mx_state(csem_model *m, int R, MixState **out)
       rc = csem_fail(-1, "LDS attribute");
       goto fail;
     }
-    MA(sH1, NN); MA(sH2, NN); MA(tw, N);
+    MA(x, sH1, NN); MA(x, sH2, NN); MA(x, tw, N);
     k_mx_cvtz<<<GE(NN)>>>(NN, m->sH1, x->sH1);
     k_mx_cvtz<<<GE(NN)>>>(NN, m->sH2, x->sH2);
     k_mx_cvtz<<<GE(N)>>>(N, m->tw, x->tw);
     CV(wx, wx, N, 0); CV(wy, wy, N, 0);
     CV(gxw1, gxw1, n1, 0); CV(gxw2, gxw2, n2, 0); CV(gyw1, gyw1, n1, 0); CV(gyw2, gyw2, n2, 0);
-    MA(buf, NN * R); MA(bufx, NN * R); MA(bufy, NN * R);
-    MA(hz0, (long)n1 * n2 * R); MA(hxg, (long)(n1 + 1) * n2 * R); MA(hyg, (long)n1 * (n2 + 1) * R);
-  }
-  {
-    long mx = 0;
-    for (int c = 0; c < 3; c++) mx = std::max(mx, (long)m->g.ns[c] * m->g.A[c] * m->g.B[c]);
-    MA(W, m->fE * R); MA(T, mx * R); MA(h, m->nsk * R); MA(trash, 2L * SGS_MAXT * MX_RMAX);
-    x->nch = (int)((m->lE + RED_CHUNK - 1) / RED_CHUNK);
-    MA(kv, 9L * R * m->lE); MA(part, (long)R * x->nch * 4); MA(sc, (long)R * MS_N);
-    MA(res, R * m->lE); MA(tmp64, m->lE); MA(norm, 4 * R); MA(scale, R);
   }
 #undef CV
-#undef MA
-  if (hipMemsetAsync(x->W, 0, m->fE * R * sizeof(c2), s) != hipSuccess) { rc = csem_fail(-1, "memset"); goto fail; }
-  if (hipMemsetAsync(x->sc, 0, (long)R * MS_N * sizeof(double), s) != hipSuccess) { rc = csem_fail(-1, "memset"); goto fail; }
   if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) {
     rc = csem_fail(-1, "mixed-precision setup failed");
     goto fail;
   }
-  x->hR = 0;
-  if (hipStreamCreateWithFlags(&x->own, hipStreamNonBlocking) != hipSuccess) {
-    x->own = NULL;
-    rc = csem_fail(-1, "stream");
-    goto fail;
-  }
+  m->mx = x;
   *out = x;
   return 0;
 fail:
-  mixed_free(m);
+  for (void *p : x->allocs) (void)hipFree(p);
+  delete x;
   return rc;
+}
+
+static int slot_new(csem_model *m, int R, MixSlot **out)
+{
+  MixSlot *q = new MixSlot();
+  q->R = R;
+  const int n1 = m->n1, n2 = m->n2;
+  int rc;
+  if (m->airbc) {
+    const long NN = (long)m->nfft * m->nfft;
+    MA(q, buf, NN * R); MA(q, bufx, NN * R); MA(q, bufy, NN * R);
+    MA(q, hz0, (long)n1 * n2 * R); MA(q, hxg, (long)(n1 + 1) * n2 * R); MA(q, hyg, (long)n1 * (n2 + 1) * R);
+    MA(q, a64.buf, NN); MA(q, a64.bufx, NN); MA(q, a64.bufy, NN);
+    MA(q, a64.hz0, (long)n1 * n2); MA(q, a64.hxg, (long)(n1 + 1) * n2); MA(q, a64.hyg, (long)n1 * (n2 + 1));
+  }
+  {
+    long mx = 0;
+    for (int c = 0; c < 3; c++) mx = std::max(mx, (long)m->g.ns[c] * m->g.A[c] * m->g.B[c]);
+    MA(q, W, m->fE * R); MA(q, T, mx * R); MA(q, h, m->nsk * R); MA(q, trash, 2L * SGS_MAXT * MX_RMAX);
+    q->nch = (int)((m->lE + RED_CHUNK - 1) / RED_CHUNK);
+    MA(q, kv, 9L * R * m->lE); MA(q, part, (long)R * q->nch * 4); MA(q, sc, (long)R * MS_N);
+    MA(q, res, R * m->lE); MA(q, tmp64, m->lE); MA(q, npart, (long)R * q->nch); MA(q, norm, R);
+    MA(q, scale, R);
+  }
+  if (hipMemset(q->W, 0, m->fE * R * sizeof(c2)) != hipSuccess ||
+      hipMemset(q->sc, 0, (long)R * MS_N * sizeof(double)) != hipSuccess) {
+    rc = csem_fail(-1, "memset");
+    goto fail;
+  }
+  if (hipStreamCreateWithFlags(&q->own, hipStreamNonBlocking) != hipSuccess) {
+    q->own = NULL;
+    rc = csem_fail(-1, "stream");
+    goto fail;
+  }
+  *out = q;
+  return 0;
+fail:
+  slot_free(q);
+  return rc;
+}
+#undef MA
+
+// Take a free slot holding at least R right-hand sides (a new one if none).
+static int slot_acquire(csem_model *m, MixState *x, int R, MixSlot **out)
+{
+  if (R < 1 || R > MX_RMAX) return csem_fail(-3, "batch of %d right-hand sides outside [1, %d]", R, MX_RMAX);
+  {
+    std::lock_guard<std::mutex> lk(x->mu);
+    for (MixSlot *q : x->slots)
+      if (!q->busy && q->R >= R) { q->busy = 1; *out = q; return 0; }
+  }
+  MixSlot *q;
+  int rc = slot_new(m, R, &q);
+  if (rc) return rc;
+  q->busy = 1;
+  std::lock_guard<std::mutex> lk(x->mu);
+  x->slots.push_back(q);
+  *out = q;
+  return 0;
+}
+static void slot_release(MixState *x, MixSlot *q)
+{
+  std::lock_guard<std::mutex> lk(x->mu);
+  q->busy = 0;
 }
 
 static MxFreq mx_freq(const csem_freq *f, int R)
@@ -214,53 +275,55 @@ static MxFreq mx_freq(const csem_freq *f, int R)
 }
 
 // ---------------------------------------------------------------- operator
-static MxAir mx_air(const csem_model *m, const MixState *x)
+static MxAir mx_air(const csem_model *m, const MixState *x, const MixSlot *q)
 {
   MxAir a;
   a.N = m->nfft; a.n1 = m->n1; a.n2 = m->n2; a.cw = x->cw;
-  a.hz = x->hz0; a.buf = x->buf; a.bufx = x->bufx; a.bufy = x->bufy;
+  a.hz = q->hz0; a.buf = q->buf; a.bufx = q->bufx; a.bufy = q->bufy;
   a.sH1 = x->sH1; a.sH2 = x->sH2; a.tw = x->tw;
   a.jx = m->jx; a.jy = m->jy; a.wx = x->wx; a.wy = x->wy;
   a.rows = m->rows; a.nrowx = m->nrowx; a.nrowy = m->nrowy;
   a.gxj1 = m->gxj1; a.gxj2 = m->gxj2; a.gyj1 = m->gyj1; a.gyj2 = m->gyj2;
   a.gxw1 = x->gxw1; a.gxw2 = x->gxw2; a.gyw1 = x->gyw1; a.gyw2 = x->gyw2;
-  a.hxg = x->hxg; a.hyg = x->hyg;
+  a.hxg = q->hxg; a.hyg = q->hyg;
   return a;
 }
 
 template <int R, int TJ>
-static void mx_apply_launch(const csem_model *m, const MixState *x, const MxFreq &fr, const c2 *in, c2 *out, int kc,
-                            hipStream_t s)
+static void mx_apply_launch(const csem_model *m, const MixState *x, const MixSlot *q, const MxFreq &fr, const c2 *in,
+                            c2 *out, int kc, hipStream_t s)
 {
   dim3 blk(64, TJ + 1), grd((m->n1 + 1 + 62) / 63, (m->n2 + 1 + TJ - 1) / TJ, (m->n3 + 1 + kc - 1) / kc);
-  k_mx_apply<R, TJ><<<grd, blk, 0, s>>>(mx_geo(m, x), fr, in, out, x->hxg, x->hyg, kc);
+  k_mx_apply<R, TJ><<<grd, blk, 0, s>>>(mx_geo(m, x), fr, in, out, q->hxg, q->hyg, kc);
 }
 // rows per block: 2, 4 or 8 (static LDS 3*2*R*(TJ+1)*64*8 B stays under
 // 64 KB: TJ = 8 only for R <= 2)
 template <int R>
-static void mx_apply_tile(const csem_model *m, const MixState *x, const MxFreq &fr, const c2 *in, c2 *out, int tj,
-                          int kc, hipStream_t s)
+static void mx_apply_tile(const csem_model *m, const MixState *x, const MixSlot *q, const MxFreq &fr, const c2 *in,
+                          c2 *out, int tj, int kc, hipStream_t s)
 {
-  if (tj == 2) mx_apply_launch<R, 2>(m, x, fr, in, out, kc, s);
-  else if (tj == 4 || R > 2) mx_apply_launch<R, 4>(m, x, fr, in, out, kc, s);
-  else mx_apply_launch<R, (R > 2 ? 4 : 8)>(m, x, fr, in, out, kc, s);
+  if (tj == 2) mx_apply_launch<R, 2>(m, x, q, fr, in, out, kc, s);
+  else if (tj == 4 || R > 2) mx_apply_launch<R, 4>(m, x, q, fr, in, out, kc, s);
+  else mx_apply_launch<R, (R > 2 ? 4 : 8)>(m, x, q, fr, in, out, kc, s);
 }
-static void mx_apply_kernel(const csem_model *m, const MixState *x, int R, const MxFreq &fr, const c2 *in, c2 *out,
-                            int tj, int kc, hipStream_t s)
+static void mx_apply_kernel(const csem_model *m, const MixState *x, const MixSlot *q, int R, const MxFreq &fr,
+                            const c2 *in, c2 *out, int tj, int kc, hipStream_t s)
 {
   switch (R) {
-    case 1: mx_apply_tile<1>(m, x, fr, in, out, tj, kc, s); break;
-    case 2: mx_apply_tile<2>(m, x, fr, in, out, tj, kc, s); break;
-    case 3: mx_apply_tile<3>(m, x, fr, in, out, tj, kc, s); break;
-    default: mx_apply_tile<4>(m, x, fr, in, out, tj, kc, s); break;
+    case 1: mx_apply_tile<1>(m, x, q, fr, in, out, tj, kc, s); break;
+    case 2: mx_apply_tile<2>(m, x, q, fr, in, out, tj, kc, s); break;
+    case 3: mx_apply_tile<3>(m, x, q, fr, in, out, tj, kc, s); break;
+    default: mx_apply_tile<4>(m, x, q, fr, in, out, tj, kc, s); break;
   }
 }
 
 // Autotuner: time every (rows per block, planes per block) candidate of the
 // FP32 operator on this model and batch size, keep the fastest.  Tile shape
 // changes no arithmetic: every output is formed by the same expression.
-static int mx_tune(csem_model *m, MixState *x, int R, const MxFreq &fr, const c2 *in, c2 *out, hipStream_t s)
+static int mx_tune(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, const c2 *in, c2 *out,
+                   hipStream_t s)
 {
+  std::lock_guard<std::mutex> lk(x->mu);
   if (x->ap_tj[R]) return 0;
   const int TJS[2] = {R > 2 ? 2 : 4, R > 2 ? 4 : 8}, KCS[3] = {4, 8, 16};
   hipEvent_t e0, e1;
@@ -270,9 +333,9 @@ static int mx_tune(csem_model *m, MixState *x, int R, const MxFreq &fr, const c2
   kt_pause(1);
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 3; b++) {
-      mx_apply_kernel(m, x, R, fr, in, out, TJS[a], KCS[b], s);  // warm
+      mx_apply_kernel(m, x, q, R, fr, in, out, TJS[a], KCS[b], s);  // warm
       (void)hipEventRecord(e0, s);
-      for (int q = 0; q < 3; q++) mx_apply_kernel(m, x, R, fr, in, out, TJS[a], KCS[b], s);
+      for (int w = 0; w < 3; w++) mx_apply_kernel(m, x, q, R, fr, in, out, TJS[a], KCS[b], s);
       (void)hipEventRecord(e1, s);
       (void)hipEventSynchronize(e1);
       float ms = 0;
@@ -282,19 +345,20 @@ static int mx_tune(csem_model *m, MixState *x, int R, const MxFreq &fr, const c2
   kt_pause(0);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  x->ap_tj[R] = bt;
   x->ap_kc[R] = bk;
+  x->ap_tj[R] = bt;
   HIP_OK(hipGetLastError());
   return 0;
 }
 
 // y = A x for the R right-hand sides, FP32, with the airwave boundary.
-static int mx_apply(csem_model *m, MixState *x, int R, const MxFreq &fr, const c2 *in, c2 *out, hipStream_t s)
+static int mx_apply(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, const c2 *in, c2 *out,
+                    hipStream_t s)
 {
   if (m->airbc) {
     const int nh = m->n1 * m->n2, N = m->nfft;
-    MxAir a = mx_air(m, x);
-    KT(KT_M_HZ0, s, 24.0 * nh * R, (k_mx_hz0<<<dim3((nh + 255) / 256, R), 256, 0, s>>>(mx_geo(m, x), in, x->hz0)));
+    MxAir a = mx_air(m, x, q);
+    KT(KT_M_HZ0, s, 24.0 * nh * R, (k_mx_hz0<<<dim3((nh + 255) / 256, R), 256, 0, s>>>(mx_geo(m, x), in, q->hz0)));
     KT(KT_M_AIR_ROWS, s, 8.0 * N * N * R,
        (k_mx_air_rows<<<dim3(N, R), 256, 2 * (size_t)N * sizeof(c2), s>>>(a)));
     KT(KT_M_AIR_COLS, s, 40.0 * N * N * R,
@@ -305,26 +369,26 @@ static int mx_apply(csem_model *m, MixState *x, int R, const MxFreq &fr, const c
     KT(KT_M_AIR_GHOST, s, 8.0 * ng * R, (k_mx_air_ghost<<<dim3((ng + 255) / 256, R), 256, 0, s>>>(a)));
   }
   // algorithmic bytes: read E and write y per right-hand side, sigma once
-  KT(KT_M_APPLY, s, (16.0 * R + 4.0) * m->lE,
-     mx_apply_kernel(m, x, R, fr, in, out, x->ap_tj[R] ? x->ap_tj[R] : 4, x->ap_kc[R] ? x->ap_kc[R] : 8, s));
+  const int tj = x->ap_tj[R] ? x->ap_tj[R] : 4, kc = x->ap_kc[R] ? x->ap_kc[R] : 8;
+  KT(KT_M_APPLY, s, (16.0 * R + 4.0) * m->lE, mx_apply_kernel(m, x, q, R, fr, in, out, tj, kc, s));
   HIP_OK(hipGetLastError());
   return 0;
 }
 
 // --------------------------------------------------------------- LU-SGS
-static MxLu mx_lu(const csem_model *m, MixState *x, const MxFreq &fr, const c2 *b, c2 *y)
+static MxLu mx_lu(const csem_model *m, const MixState *x, MixSlot *q, const MxFreq &fr, const c2 *b, c2 *y)
 {
   MxLu a;
   a.g = mx_geo(m, x); a.sk = m->g;
-  a.b = b; a.y = y; a.W = x->W; a.fE = m->fE; a.fEx = m->fEx; a.fEy = m->fEy;
-  a.T = x->T; a.h = x->h; a.trash = x->trash;
+  a.b = b; a.y = y; a.W = q->W; a.fE = m->fE; a.fEx = m->fEx; a.fEy = m->fEy;
+  a.T = q->T; a.h = q->h; a.trash = q->trash;
   a.c1m = x->c1m; a.c1p = x->c1p; a.c2m = x->c2m; a.c2p = x->c2p; a.c3m = x->c3m; a.c3p = x->c3p;
   a.fr = fr;
   return a;
 }
 
 template <int R>
-static void mx_lusgs_R(const csem_model *m, MixState *x, const MxLu &a, bool diag, bool sweep, hipStream_t s)
+static void mx_lusgs_R(const csem_model *m, const MxLu &a, bool diag, bool sweep, hipStream_t s)
 {
   const SkewGeo &g = m->g;
   dim3 gp[3], gq[3];
@@ -367,75 +431,64 @@ static void mx_lusgs_R(const csem_model *m, MixState *x, const MxLu &a, bool dia
   KT(KT_M_SGS, s, 40.0 * R * nc[0], (k_mx_sgs<0, false, R><<<gs[0], tb[0], lds[0], s>>>(a)));
 }
 
-// the diagonal factors h depend on the batch's frequencies: recomputed when
-// they change (never inside a captured iteration)
-static bool mx_diag_stale(MixState *x, int R, const MxFreq &fr)
+static void mx_lusgs_dispatch(const csem_model *m, int R, const MxLu &a, bool diag, bool sweep, hipStream_t s)
 {
-  bool stale = x->hR != R;
-  for (int r = 0; r < R && !stale; r++)
-    stale = x->hkey[r][0] != fr.iwmu[r].x || x->hkey[r][1] != fr.iwmu[r].y;
-  if (stale) {
-    x->hR = R;
-    for (int r = 0; r < R; r++) { x->hkey[r][0] = fr.iwmu[r].x; x->hkey[r][1] = fr.iwmu[r].y; }
+  switch (R) {
+    case 1: mx_lusgs_R<1>(m, a, diag, sweep, s); break;
+    case 2: mx_lusgs_R<2>(m, a, diag, sweep, s); break;
+    case 3: mx_lusgs_R<3>(m, a, diag, sweep, s); break;
+    default: mx_lusgs_R<4>(m, a, diag, sweep, s); break;
   }
-  return stale;
 }
 
-// Refresh the diagonal factors for this batch's frequencies (outside any
-// captured iteration), then sweep.
-static int mx_diag(csem_model *m, MixState *x, int R, const MxFreq &fr, hipStream_t s)
+// The diagonal factors h depend on the batch's frequencies: refreshed when
+// they change, outside any captured iteration.
+static int mx_diag(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, hipStream_t s)
 {
-  if (!mx_diag_stale(x, R, fr)) return 0;
-  const MxLu a = mx_lu(m, x, fr, NULL, NULL);
-  switch (R) {
-    case 1: mx_lusgs_R<1>(m, x, a, true, false, s); break;
-    case 2: mx_lusgs_R<2>(m, x, a, true, false, s); break;
-    case 3: mx_lusgs_R<3>(m, x, a, true, false, s); break;
-    default: mx_lusgs_R<4>(m, x, a, true, false, s); break;
-  }
+  bool stale = q->hR != R;
+  for (int r = 0; r < R && !stale; r++) stale = q->hkey[r][0] != fr.iwmu[r].x || q->hkey[r][1] != fr.iwmu[r].y;
+  if (!stale) return 0;
+  q->hR = R;
+  for (int r = 0; r < R; r++) { q->hkey[r][0] = fr.iwmu[r].x; q->hkey[r][1] = fr.iwmu[r].y; }
+  mx_lusgs_dispatch(m, R, mx_lu(m, x, q, fr, NULL, NULL), true, false, s);
   HIP_OK(hipGetLastError());
   return 0;
 }
 
-static int mx_lusgs(csem_model *m, MixState *x, int R, const MxFreq &fr, const c2 *b, c2 *y, hipStream_t s)
+static int mx_lusgs(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, const c2 *b, c2 *y,
+                    hipStream_t s)
 {
-  const MxLu a = mx_lu(m, x, fr, b, y);
-  switch (R) {
-    case 1: mx_lusgs_R<1>(m, x, a, false, true, s); break;
-    case 2: mx_lusgs_R<2>(m, x, a, false, true, s); break;
-    case 3: mx_lusgs_R<3>(m, x, a, false, true, s); break;
-    default: mx_lusgs_R<4>(m, x, a, false, true, s); break;
-  }
+  mx_lusgs_dispatch(m, R, mx_lu(m, x, q, fr, b, y), false, true, s);
   HIP_OK(hipGetLastError());
   return 0;
 }
 
 // ---------------------------------------------------------------- Krylov
-static MxKry mx_kry(const csem_model *m, MixState *x)
+static MxKry mx_kry(const csem_model *m, const MixSlot *q)
 {
   MxKry k;
-  const long n = m->lE, R = x->R;
-  k.n = n; k.nch = x->nch;
-  c2 *v = x->kv;
+  const long n = m->lE, R = q->R;
+  k.n = n; k.nch = q->nch;
+  c2 *v = q->kv;
   k.r = v; k.r0 = v + R * n; k.p = v + 2 * R * n; k.pp = v + 3 * R * n; k.v = v + 4 * R * n;
   k.s = v + 5 * R * n; k.ss = v + 6 * R * n; k.t = v + 7 * R * n; k.x = v + 8 * R * n;
-  k.part = x->part; k.sc = x->sc;
+  k.part = q->part; k.sc = q->sc;
   return k;
 }
 
 // one F-BiCGStab iteration for every active right-hand side (solver.c:139-177)
-static int mx_iteration(csem_model *m, MixState *x, int R, const MxFreq &fr, hipStream_t s)
+static int mx_iteration(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, hipStream_t s)
 {
-  const MxKry k = mx_kry(m, x);
-  const dim3 gr(x->nch, R), ge((unsigned)((k.n + 255) / 256), R);
+  const MxKry k = mx_kry(m, q);
+  const dim3 gr(q->nch, R), ge((unsigned)((k.n + 255) / 256), R);
   int rc;
-  if ((rc = mx_lusgs(m, x, R, fr, k.p, k.pp, s))) return rc;
-  if ((rc = mx_apply(m, x, R, fr, k.pp, k.v, s))) return rc;
+  if ((rc = mx_lusgs(m, x, q, R, fr, k.p, k.pp, s))) return rc;
+  if ((rc = mx_apply(m, x, q, R, fr, k.pp, k.v, s))) return rc;
   KT(KT_M_DOT, s, 16.0 * R * k.n, (k_mx_k_dot_vr0<<<gr, RED_T, 0, s>>>(k)));
   KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 1)));
   KT(KT_M_VEC, s, 24.0 * R * k.n, (k_mx_k_s<<<ge, 256, 0, s>>>(k)));
-  if ((rc = mx_lusgs(m, x, R, fr, k.s, k.ss, s))) return rc;
-  if ((rc = mx_apply(m, x, R, fr, k.ss, k.t, s))) return rc;
+  if ((rc = mx_lusgs(m, x, q, R, fr, k.s, k.ss, s))) return rc;
+  if ((rc = mx_apply(m, x, q, R, fr, k.ss, k.t, s))) return rc;
   KT(KT_M_DOT, s, 16.0 * R * k.n, (k_mx_k_dot_st<<<gr, RED_T, 0, s>>>(k)));
   KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 2)));
   KT(KT_M_VEC, s, 64.0 * R * k.n, (k_mx_k_xr<<<gr, RED_T, 0, s>>>(k)));
@@ -447,63 +500,90 @@ static int mx_iteration(csem_model *m, MixState *x, int R, const MxFreq &fr, hip
 
 // Capture one iteration as a graph (kernel launches only: the frequencies
 // and every scalar live in device memory or in the captured arguments).
-static int mx_graph(csem_model *m, MixState *x, int R, const MxFreq &fr, hipStream_t s)
+static int mx_graph(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, hipStream_t s)
 {
-  if (x->gexec && x->g_R == R && x->g_stream == s && !memcmp(&x->g_fr, &fr, sizeof fr)) return 0;
-  if (x->gexec) { (void)hipGraphExecDestroy(x->gexec); x->gexec = NULL; }
+  if (q->gexec && q->g_R == R && q->g_stream == s && !memcmp(&q->g_fr, &fr, sizeof fr)) return 0;
+  if (q->gexec) { (void)hipGraphExecDestroy(q->gexec); q->gexec = NULL; }
   hipGraph_t g;
   kt_pause(1);
   HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  int rc = mx_iteration(m, x, R, fr, s);
+  int rc = mx_iteration(m, x, q, R, fr, s);
   hipError_t e = hipStreamEndCapture(s, &g);
   kt_pause(0);
   if (rc) return rc;
   if (e != hipSuccess) return csem_fail(-1, "graph capture: %s", hipGetErrorString(e));
-  e = hipGraphInstantiate(&x->gexec, g, NULL, NULL, 0);
+  e = hipGraphInstantiate(&q->gexec, g, NULL, NULL, 0);
   (void)hipGraphDestroy(g);
-  if (e != hipSuccess) { x->gexec = NULL; return csem_fail(-1, "graph instantiate: %s", hipGetErrorString(e)); }
-  x->g_R = R;
-  x->g_stream = s;
-  x->g_fr = fr;
+  if (e != hipSuccess) { q->gexec = NULL; return csem_fail(-1, "graph instantiate: %s", hipGetErrorString(e)); }
+  q->g_R = R;
+  q->g_stream = s;
+  q->g_fr = fr;
   return 0;
 }
 
 // ------------------------------------------------------------ refinement
-extern "C" int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *xv, const void *bv, const double *tol,
-                                const csem_mixed_opts *opt, int *iters, double *relres, void *stream)
+// res_r = b_r - w (w = A x_r in FP64) and the chunk partials of |res_r|^2
+__global__ __launch_bounds__(RED_T) void k_mx_res64(long n, const z2 *b, const z2 *w, z2 *res, double *npart, int nch)
 {
-  MixState *x;
-  int rc = mx_state(m, R, &x);
-  if (rc) return rc;
-  // NULL: run on the model's own stream, after the caller's queued work
-  if (!stream) HIP_OK(hipDeviceSynchronize());
-  hipStream_t s = stream ? (hipStream_t)stream : x->own;
-  stream = (void *)s;
+  __shared__ double sh[RED_T];
+  double acc = 0;
+  const long base = (long)blockIdx.x * RED_CHUNK + threadIdx.x;
+  for (int m = 0; m < RED_E; m++) {
+    const long e = base + (long)m * RED_T;
+    if (e < n) {
+      const z2 v = zsub(b[e], w[e]);
+      res[e] = v;
+      acc += v.x * v.x + v.y * v.y;
+    }
+  }
+  const double v = block_fold(acc, sh);
+  if (threadIdx.x == 0) npart[blockIdx.x] = v;
+}
+// |v_r|^2 for every r (chunk partials then one block per r)
+__global__ __launch_bounds__(RED_T) void k_mx_norm_part(long n, const z2 *v, double *npart, int nch)
+{
+  __shared__ double sh[RED_T];
+  const int r = blockIdx.y;
+  double acc = 0;
+  const long base = (long)blockIdx.x * RED_CHUNK + threadIdx.x;
+  for (int m = 0; m < RED_E; m++) {
+    const long e = base + (long)m * RED_T;
+    if (e < n) { const z2 q = v[r * n + e]; acc += q.x * q.x + q.y * q.y; }
+  }
+  const double s = block_fold(acc, sh);
+  if (threadIdx.x == 0) npart[(long)r * nch + blockIdx.x] = s;
+}
+__global__ __launch_bounds__(RED_T) void k_mx_norm_fin(const double *npart, int nch, double *out)
+{
+  __shared__ double sh[RED_T];
+  const int r = blockIdx.x;
+  double acc = 0;
+  for (int c = threadIdx.x; c < nch; c += RED_T) acc += npart[(long)r * nch + c];
+  const double s = block_fold(acc, sh);
+  if (threadIdx.x == 0) out[r] = s;
+}
+
+static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const csem_freq *f, z2 *X, const z2 *B,
+                            const double *tol, const csem_mixed_opts &o, int *iters, double *relres, hipStream_t s)
+{
   const long n = m->lE;
   const MxFreq fr = mx_freq(f, R);
-  const csem_mixed_opts dflt = {2000, 1e-2, 40, 8, 1};
-  const csem_mixed_opts &o = opt ? *opt : dflt;
-  z2 *X = (z2 *)xv;
-  const z2 *B = (const z2 *)bv;
-  const MxKry k = mx_kry(m, x);
-  // scratch reuse: r32 is the Krylov r vector, the correction is k.x
-  std::vector<double> bn(R), rn(R), hsc((size_t)R * MS_N);
-  std::vector<int> used(R, 0);
+  const MxKry k = mx_kry(m, q);
+  const int nch = q->nch;
+  const unsigned gn = (unsigned)((n + 255) / 256);
+  std::vector<double> bn(R), rn(R), hsc((size_t)R * MS_N), nb(R);
   std::vector<float> scl(R);
-  for (int r = 0; r < R; r++) {
-    if ((rc = csem_dot(n, B + r * n, B + r * n, x->norm + 4 * r, stream))) return rc;
-  }
+  int rc;
+  // |b_r|, x = 0, res = b
+  k_mx_norm_part<<<dim3(nch, R), RED_T, 0, s>>>(n, B, q->npart, nch);
+  k_mx_norm_fin<<<R, RED_T, 0, s>>>(q->npart, nch, q->norm);
   HIP_OK(hipMemsetAsync(X, 0, (size_t)R * n * sizeof(z2), s));
-  HIP_OK(hipMemcpyAsync(x->res, B, (size_t)R * n * sizeof(z2), hipMemcpyDeviceToDevice, s));
-  {
-    std::vector<double> nb(4 * R);
-    HIP_OK(hipMemcpyAsync(nb.data(), x->norm, 4 * R * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    for (int r = 0; r < R; r++) bn[r] = sqrt(nb[4 * r]);
-  }
-  if ((rc = mx_tune(m, x, R, fr, k.r, k.v, s))) return rc;  // operands: any R x n FP32 buffers
-  if ((rc = mx_diag(m, x, R, fr, s))) return rc;
-  for (int r = 0; r < R; r++) { rn[r] = bn[r]; relres[r] = 1.0; iters[r] = 0; }
+  HIP_OK(hipMemcpyAsync(q->res, B, (size_t)R * n * sizeof(z2), hipMemcpyDeviceToDevice, s));
+  HIP_OK(hipMemcpyAsync(nb.data(), q->norm, R * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  for (int r = 0; r < R; r++) { bn[r] = sqrt(nb[r]); rn[r] = bn[r]; relres[r] = 1.0; iters[r] = 0; }
+  if ((rc = mx_tune(m, x, q, R, fr, k.r, k.v, s))) return rc;  // operands: any R x n FP32 buffers
+  if ((rc = mx_diag(m, x, q, R, fr, s))) return rc;
   for (int outer = 0; outer <= o.max_outer; outer++) {
     // which right-hand sides still need a correction
     bool any = false;
@@ -520,19 +600,19 @@ extern "C" int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *
       any |= act;
     }
     if (!any) break;
-    HIP_OK(hipMemcpyAsync(x->sc, hsc.data(), hsc.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(x->scale, scl.data(), R * sizeof(float), hipMemcpyHostToDevice, s));
-    KT(KT_M_REFINE, s, 24.0 * R * n, (k_mx_to32<<<dim3((unsigned)((n + 255) / 256), R), 256, 0, s>>>(n, x->res, x->scale, k.r)));
-    KT(KT_M_DOT, s, 32.0 * R * n, (k_mx_k_init<<<dim3(x->nch, R), RED_T, 0, s>>>(k)));
+    HIP_OK(hipMemcpyAsync(q->sc, hsc.data(), hsc.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(q->scale, scl.data(), R * sizeof(float), hipMemcpyHostToDevice, s));
+    KT(KT_M_REFINE, s, 24.0 * R * n, (k_mx_to32<<<dim3(gn, R), 256, 0, s>>>(n, q->res, q->scale, k.r)));
+    KT(KT_M_DOT, s, 32.0 * R * n, (k_mx_k_init<<<dim3(nch, R), RED_T, 0, s>>>(k)));
     KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 0)));
-    if (o.use_graph && (rc = mx_graph(m, x, R, fr, s))) return rc;
+    if (o.use_graph && (rc = mx_graph(m, x, q, R, fr, s))) return rc;
     // inner iterations, convergence checked every check_every
     for (;;) {
-      for (int q = 0; q < std::max(1, o.check_every); q++) {
-        if (o.use_graph) HIP_OK(hipGraphLaunch(x->gexec, s));
-        else if ((rc = mx_iteration(m, x, R, fr, s))) return rc;
+      for (int w = 0; w < std::max(1, o.check_every); w++) {
+        if (o.use_graph) HIP_OK(hipGraphLaunch(q->gexec, s));
+        else if ((rc = mx_iteration(m, x, q, R, fr, s))) return rc;
       }
-      HIP_OK(hipMemcpyAsync(hsc.data(), x->sc, hsc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIP_OK(hipMemcpyAsync(hsc.data(), q->sc, hsc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
       HIP_OK(hipStreamSynchronize(s));
       bool all = true;
       for (int r = 0; r < R; r++) all &= hsc[(size_t)r * MS_N + MS_DONE] != 0.0;
@@ -541,19 +621,19 @@ extern "C" int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *
     for (int r = 0; r < R; r++)
       if (hsc[(size_t)r * MS_N + MS_USED] != 0.0) iters[r] += (int)hsc[(size_t)r * MS_N + MS_IT];
     // x += |res| * correction (FP64), then the true residual in FP64
-    KT(KT_M_REFINE, s, 32.0 * R * n, (k_mx_acc64<<<dim3((unsigned)((n + 255) / 256), R), 256, 0, s>>>(n, X, k.x, x->sc, x->scale)));
+    KT(KT_M_REFINE, s, 32.0 * R * n, (k_mx_acc64<<<dim3(gn, R), 256, 0, s>>>(n, X, k.x, q->sc, q->scale)));
     for (int r = 0; r < R; r++) {
       if (hsc[(size_t)r * MS_N + MS_USED] == 0.0) continue;
-      if ((rc = csem_apply(m, &f[r], X + r * n, x->tmp64, stream))) return rc;
-      if ((rc = csem_vec_sub(n, B + r * n, x->tmp64, x->res + r * n, stream))) return rc;
-      if ((rc = csem_dot(n, x->res + r * n, x->res + r * n, x->norm + 4 * r, stream))) return rc;
+      if ((rc = run_apply_bufs(m, &f[r], X + r * n, q->tmp64, s, q->a64))) return rc;
+      KT(KT_M_REFINE, s, 48.0 * n,
+         (k_mx_res64<<<nch, RED_T, 0, s>>>(n, B + r * n, q->tmp64, q->res + r * n, q->npart + (long)r * nch, nch)));
     }
-    std::vector<double> nb(4 * R);
-    HIP_OK(hipMemcpyAsync(nb.data(), x->norm, 4 * R * sizeof(double), hipMemcpyDeviceToHost, s));
+    k_mx_norm_fin<<<R, RED_T, 0, s>>>(q->npart, nch, q->norm);
+    HIP_OK(hipMemcpyAsync(nb.data(), q->norm, R * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     for (int r = 0; r < R; r++) {
       if (hsc[(size_t)r * MS_N + MS_USED] == 0.0) continue;
-      rn[r] = sqrt(nb[4 * r]);
+      rn[r] = sqrt(nb[r]);
       relres[r] = bn[r] > 0 ? rn[r] / bn[r] : 0.0;
     }
   }
@@ -563,30 +643,95 @@ extern "C" int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *
   return conv;
 }
 
-extern "C" int csem_apply32(csem_model *m, int R, const csem_freq *f, const void *xv, void *yv, void *stream)
+extern "C" int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *xv, const void *bv, const double *tol,
+                                const csem_mixed_opts *opt, int *iters, double *relres, void *stream)
 {
   MixState *x;
-  int rc = mx_state(m, R, &x);
-  if (rc) return rc;
-  return mx_apply(m, x, R, mx_freq(f, R), (const c2 *)xv, (c2 *)yv, (hipStream_t)stream);
+  MixSlot *q;
+  int rc;
+  {
+    static std::mutex init_mu;  // first use of a model from several threads
+    std::lock_guard<std::mutex> lk(init_mu);
+    if ((rc = mx_model(m, &x))) return rc;
+  }
+  if ((rc = slot_acquire(m, x, R, &q))) return rc;
+  // NULL: run on the slot's own stream, after the caller's work queued on
+  // the null stream (not the whole device: other solves keep running)
+  if (!stream) {
+    hipError_t e = hipStreamSynchronize(0);
+    if (e != hipSuccess) { slot_release(x, q); return csem_fail(-1, "sync: %s", hipGetErrorString(e)); }
+  }
+  const hipStream_t s = stream ? (hipStream_t)stream : q->own;
+  const csem_mixed_opts dflt = {2000, 1e-2, 40, 8, 1};
+  rc = mixed_solve_slot(m, x, q, R, f, (z2 *)xv, (const z2 *)bv, tol, opt ? *opt : dflt, iters, relres, s);
+  slot_release(x, q);
+  return rc;
+}
+
+// The FP32 operator / preconditioner alone (tests, custom drivers).
+static int mx_single(csem_model *m, int R, const csem_freq *f, const void *xv, void *yv, void *stream, bool lu)
+{
+  MixState *x;
+  MixSlot *q;
+  int rc;
+  {
+    static std::mutex init_mu;
+    std::lock_guard<std::mutex> lk(init_mu);
+    if ((rc = mx_model(m, &x))) return rc;
+  }
+  if ((rc = slot_acquire(m, x, R, &q))) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  const MxFreq fr = mx_freq(f, R);
+  if (lu) {
+    if (!(rc = mx_diag(m, x, q, R, fr, s))) rc = mx_lusgs(m, x, q, R, fr, (const c2 *)xv, (c2 *)yv, s);
+  } else {
+    rc = mx_apply(m, x, q, R, fr, (const c2 *)xv, (c2 *)yv, s);
+  }
+  if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = csem_fail(-1, "sync");
+  slot_release(x, q);
+  return rc;
+}
+
+extern "C" int csem_apply32(csem_model *m, int R, const csem_freq *f, const void *xv, void *yv, void *stream)
+{
+  return mx_single(m, R, f, xv, yv, stream, false);
 }
 
 extern "C" int csem_lusgs32(csem_model *m, int R, const csem_freq *f, const void *xv, void *yv, void *stream)
 {
-  MixState *x;
-  int rc = mx_state(m, R, &x);
-  if (rc) return rc;
-  const MxFreq fr = mx_freq(f, R);
-  if ((rc = mx_diag(m, x, R, fr, (hipStream_t)stream))) return rc;
-  return mx_lusgs(m, x, R, fr, (const c2 *)xv, (c2 *)yv, (hipStream_t)stream);
+  return mx_single(m, R, f, xv, yv, stream, true);
 }
 
 extern "C" int csem_mixed_tuning(csem_model *m, int R, int *rows, int *planes)
 {
-  if (!m->mx || R < 1 || R > MX_RMAX) return -1;
+  if (!m->mx || R < 1 || R > MX_RMAX || !m->mx->ap_tj[R]) return -1;
   *rows = m->mx->ap_tj[R];
   *planes = m->mx->ap_kc[R];
   return 0;
 }
 
-extern "C" long csem_mixed_bytes(const csem_model *m) { return m->mx ? m->mx->bytes : 0; }
+extern "C" int csem_mixed_set_tile(csem_model *m, int R, int rows, int planes)
+{
+  MixState *x;
+  int rc;
+  if (R < 1 || R > MX_RMAX) return csem_fail(-3, "batch of %d right-hand sides outside [1, %d]", R, MX_RMAX);
+  if (!(rows == 2 || rows == 4 || (rows == 8 && R <= 2)) || !(planes >= 1 && planes <= 64))
+    return csem_fail(-3, "tile %d x %d not available for %d right-hand sides", rows, planes, R);
+  {
+    static std::mutex init_mu;
+    std::lock_guard<std::mutex> lk(init_mu);
+    if ((rc = mx_model(m, &x))) return rc;
+  }
+  std::lock_guard<std::mutex> lk(x->mu);
+  x->ap_kc[R] = planes;
+  x->ap_tj[R] = rows;
+  return 0;
+}
+
+extern "C" long csem_mixed_bytes(const csem_model *m)
+{
+  if (!m->mx) return 0;
+  long b = m->mx->bytes;
+  for (const MixSlot *q : m->mx->slots) b += q->bytes;
+  return b;
+}

@@ -140,6 +140,9 @@ int csem_lusgs32(csem_model *m, int R, const csem_freq *f, const void *x, void *
 /* Tile shape the autotuner picked for the FP32 operator at batch size R
  * (rows per block, planes per block); -1 before the first solve. */
 int csem_mixed_tuning(csem_model *m, int R, int *rows, int *planes);
+/* Fix the FP32 operator's tile at batch size R instead of timing the
+ * candidates (rows 2, 4, or 8 for R <= 2; planes 1..64). */
+int csem_mixed_set_tile(csem_model *m, int R, int rows, int planes);
 /* device bytes held by the mixed-precision state of a model */
 long csem_mixed_bytes(const csem_model *m);
 

@@ -87,6 +87,36 @@ def test_apply32_close_to_fp64(workdir):
         ses.close()
 
 
+def test_apply32_every_tile(workdir):
+    """Every tile shape the autotuner may pick gives the same operator."""
+    print("case apply32 tiles", flush=True)
+    s, ses, orc = _setup(workdir)
+    L = _lib.lib()
+    try:
+        n = ses.n
+        rng = np.random.default_rng(13)
+        for R in (1, 3):
+            x = (rng.standard_normal((R, n)) + 1j * rng.standard_normal((R, n))).astype(np.complex64)
+            fr = (_lib.FreqT * R)(*[freq_consts(f) for f in FREQS[:R]])
+            dx, dy = _dev(x), _dev(np.zeros_like(x))
+            ref = []
+            for r, f in enumerate(FREQS[:R]):
+                orc.set_freq(f)
+                ref.append(orc.apply(x[r].astype(np.complex128)))
+            for rows in ((4, 8) if R <= 2 else (2, 4)):
+                for planes in (4, 8, 16):
+                    _lib.check(L.csem_mixed_set_tile(ses.model, R, rows, planes), "set_tile")
+                    _lib.check(L.csem_apply32(ses.model, R, fr, dx, dy, None), "apply32")
+                    y = _host(dy, x)
+                    for r in range(R):
+                        err = np.linalg.norm(y[r] - ref[r]) / np.linalg.norm(ref[r])
+                        assert err < 2e-6, (R, rows, planes, r, err)
+            L.csem_free(dx)
+            L.csem_free(dy)
+    finally:
+        ses.close()
+
+
 def test_lusgs32_close_to_fp64(workdir):
     print("case lusgs32", flush=True)
     s, ses, orc = _setup(workdir)
