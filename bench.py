@@ -17,6 +17,10 @@ The iteration budget is 2000 (the reference itself needs 1073 iterations at
 0.25 Hz, beyond the 1000 of its small-model example).  Inputs (model,
 right-hand sides) are resident in HBM before the timed region.
 
+After the timed steps, a serving run (demo_matrixfree_csem_amd/serve.py)
+feeds a warm engine seeded requests on 64^3 and 100^3 models at Poisson
+arrival times and reports its solves/s and p50/p99 request latency.
+
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
 Prints one JSON line.  --gpus above 1 is refused: this path runs on one
@@ -84,6 +88,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 path (profiling runs)")
     ap.add_argument("--niter", type=int, default=2000)
+    ap.add_argument("--serve-requests", type=int, default=10, help="requests of the serving run (0: none)")
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrivals per second of the serving run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this path has no multi-GPU form; independent solves would "
@@ -266,6 +272,20 @@ def main():
         "kernel_ms_plain": ({k: round(v[0] / args.steps, 3) for k, v in sorted(stp.items(), key=lambda kv: -kv[1][0])}
                             if stp else None),
     }
+    if args.serve_requests > 0:
+        # serving: a warm engine (models resident) fed seeded requests on
+        # 64^3 and 100^3 grids at Poisson arrival times
+        from demo_matrixfree_csem_amd import serve
+        reqs = serve.quick_requests(args.serve_requests, 4)
+        eng = serve.Engine(workers=3, rhs_per_solve=1)
+        try:
+            eng.preload(reqs)
+            _, sst = eng.run(reqs, serve.poisson_arrivals(len(reqs), args.serve_rate, 4))
+        finally:
+            eng.close()
+        sst["rate_per_s"] = args.serve_rate
+        sst["stream"] = "64^3 and 100^3 grids (two models each), 1-3 frequencies, 1-2 sources per request"
+        out["serving"] = sst
     if not args.no_cpu and plain:
         out["cpu_baseline"] = cpu_baseline(survey, freqs[0], args.cpu_iters, plain["iterations"])
     print(json.dumps(out), flush=True)

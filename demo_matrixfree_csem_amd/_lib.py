@@ -105,7 +105,7 @@ DEV_SYMBOLS = [
     "csem_malloc", "csem_free", "csem_memcpy_h2d", "csem_memcpy_d2h", "csem_memset", "csem_sync",
     "csem_kernel_timing", "csem_kernel_stats", "csem_set_reduction",
     "csem_vec_sub", "csem_vec_divr", "csem_vec_divz", "csem_vec_subscaled_dev", "csem_vec_addscaled",
-    "csem_mixed_solve", "csem_apply32", "csem_lusgs32", "csem_mixed_tuning", "csem_mixed_set_tile", "csem_mixed_bytes",
+    "csem_mixed_solve", "csem_apply32", "csem_lusgs32", "csem_mixed_tuning", "csem_mixed_set_tile", "csem_mixed_bytes", "csem_mixed_slot_bytes", "csem_mem_info",
 ]
 
 _lib = None
@@ -150,6 +150,9 @@ def lib():
     L.csem_mixed_tuning.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip)]
     L.csem_mixed_set_tile.argtypes = [vp, ip, ip, ip]
     L.csem_mixed_bytes.argtypes = [vp]
+    L.csem_mixed_slot_bytes.argtypes = [vp, ip]
+    L.csem_mixed_slot_bytes.restype = lp
+    L.csem_mem_info.argtypes = [C.POINTER(lp), C.POINTER(lp)]
     L.csem_mixed_bytes.restype = lp
     L.csem_set_reduction.argtypes = [ip]
     L.csem_kernel_stats.argtypes = [ip, C.POINTER(C.c_char_p), C.POINTER(dp), C.POINTER(lp), C.POINTER(dp)]

@@ -11,6 +11,8 @@
 
 #include "csem_internal.h"
 
+std::mutex g_devwide_mu;
+
 static char g_err[1024];
 
 extern "C" int csem_fail(int code, const char *fmt, ...)
@@ -149,7 +151,7 @@ extern "C" void csem_model_destroy(csem_model *m)
 {
   if (!m) return;
   mixed_free(m);
-  for (void *p : m->allocs) (void)hipFree(p);
+  for (void *p : m->allocs) dev_free(p);
   delete m;
 }
 
@@ -218,12 +220,12 @@ extern "C" int csem_model_create(const csem_model_desc *d, csem_model **out)
   }
   // cells outside every sweep range stay zero for the pre-passes
   if ((rc = dalloc(m, &m->W, m->fE))) goto fail;
-  if (hipMemset(m->W, 0, m->fE * sizeof(z2)) != hipSuccess) { rc = csem_fail(-1, "memset W"); goto fail; }
+  if (dev_memset(m->W, 0, m->fE * sizeof(z2)) != hipSuccess) { rc = csem_fail(-1, "memset W"); goto fail; }
   m->smith_ok = 0;
   m->nch = (m->fE + RED_CHUNK - 1) / RED_CHUNK;
   if ((rc = dalloc(m, &m->partials, m->nch * 4))) goto fail;
   if ((rc = dalloc(m, &m->scal, 64))) goto fail;
-  if (hipMemset(m->scal, 0, 64 * sizeof(double)) != hipSuccess) { rc = csem_fail(-1, "memset scal"); goto fail; }
+  if (dev_memset(m->scal, 0, 64 * sizeof(double)) != hipSuccess) { rc = csem_fail(-1, "memset scal"); goto fail; }
   if (m->airbc) {
     const int N = d->nfft;
     if (N < 4 || (N & (N - 1)) || N > 2048) { rc = csem_fail(-3, "nfft=%d must be a power of two in [4, 2048]", N); goto fail; }
@@ -558,9 +560,9 @@ extern "C" int csem_dot(long n, const void *a, const void *b, void *out, void *s
   }
   long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
   if (nch > g_dot_cap) {
-    if (g_dot_partials) (void)hipFree(g_dot_partials);
+    if (g_dot_partials) dev_free(g_dot_partials);
     g_dot_cap = 0;
-    HIP_OK(hipMalloc((void **)&g_dot_partials, nch * 4 * sizeof(double)));
+    HIP_OK(dev_malloc((void **)&g_dot_partials, nch * 4 * sizeof(double)));
     g_dot_cap = nch;
   }
   return dot_launch(n, (const z2 *)a, (const z2 *)b, NULL, NULL, g_dot_partials, (double *)out, S(stream));
@@ -721,7 +723,7 @@ static int ensure_kv(csem_model *m, int nv)
 {
   if (m->nkv >= nv) return 0;
   if (m->kv) {
-    (void)hipFree(m->kv);
+    dev_free(m->kv);
     m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void *)m->kv), m->allocs.end());
     m->bytes -= m->nkv * m->lE * (long)sizeof(z2);
     m->kv = NULL;
@@ -889,15 +891,15 @@ extern "C" int csem_scatter_add(long nnz, const long *idx, const double *val, vo
   if (nnz <= 0) return 0;
   long *di;
   z2 *dv;
-  HIP_OK(hipMalloc((void **)&di, nnz * sizeof(long)));
-  HIP_OK(hipMalloc((void **)&dv, nnz * sizeof(z2)));
+  HIP_OK(dev_malloc((void **)&di, nnz * sizeof(long)));
+  HIP_OK(dev_malloc((void **)&dv, nnz * sizeof(z2)));
   HIP_OK(hipMemcpyAsync(di, idx, nnz * sizeof(long), hipMemcpyHostToDevice, S(stream)));
   HIP_OK(hipMemcpyAsync(dv, val, nnz * sizeof(z2), hipMemcpyHostToDevice, S(stream)));
   k_scatter_add<<<1, 64, 0, S(stream)>>>(nnz, di, dv, (z2 *)b);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(S(stream)));
-  (void)hipFree(di);
-  (void)hipFree(dv);
+  dev_free(di);
+  dev_free(dv);
   return 0;
 }
 
@@ -908,7 +910,7 @@ extern "C" int csem_gather(int nrec, const long *idx, const double *wts, const d
   char *blob;
   size_t bi = 8 * (size_t)nrec * sizeof(long), bw = 24 * (size_t)nrec * sizeof(double);
   size_t bs = 8 * (size_t)nrec * sizeof(double), bc = (size_t)nrec * sizeof(double), bo = (size_t)nrec * sizeof(z2);
-  HIP_OK(hipMalloc((void **)&blob, bi + bw + bs + bc + bo));
+  HIP_OK(dev_malloc((void **)&blob, bi + bw + bs + bc + bo));
   hipStream_t s = S(stream);
   HIP_OK(hipMemcpyAsync(blob, idx, bi, hipMemcpyHostToDevice, s));
   HIP_OK(hipMemcpyAsync(blob + bi, wts, bw, hipMemcpyHostToDevice, s));
@@ -920,7 +922,7 @@ extern "C" int csem_gather(int nrec, const long *idx, const double *wts, const d
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(out, blob + bi + bw + bs + bc, bo, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
-  (void)hipFree(blob);
+  dev_free(blob);
   return 0;
 }
 
@@ -928,12 +930,12 @@ extern "C" int csem_gather(int nrec, const long *idx, const double *wts, const d
 
 extern "C" int csem_malloc(void **p, long bytes)
 {
-  HIP_OK(hipMalloc(p, bytes > 0 ? bytes : 256));
+  HIP_OK(dev_malloc(p, bytes > 0 ? bytes : 256));
   return 0;
 }
 extern "C" int csem_free(void *p)
 {
-  HIP_OK(hipFree(p));
+  dev_free(p);
   return 0;
 }
 extern "C" int csem_memcpy_h2d(void *dst, const void *src, long bytes, void *stream)

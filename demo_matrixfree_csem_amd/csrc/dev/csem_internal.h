@@ -1,9 +1,39 @@
 // csem_internal.h -- state shared by the FP64 and mixed-precision
 // translation units of libcsem (not part of the public C ABI).
 #pragma once
+#include <mutex>
 #include <vector>
 #include "csem_kernels.h"
 #include "../../../include/csem_dev.h"
+
+// ------------------------------------------------ device-wide operations
+// hipFree waits for the whole device, and synchronous copies and fills run
+// on the null stream; issued while another thread captures a graph they
+// can invalidate that capture.  Every such call in the library, every
+// allocation, and every capture (mixed-precision solve slots) hold this
+// lock, so solves on several threads can capture while others allocate,
+// upload a model or free one.
+extern std::mutex g_devwide_mu;
+static inline void dev_free(void *p)
+{
+  std::lock_guard<std::mutex> lk(g_devwide_mu);
+  (void)hipFree(p);
+}
+static inline hipError_t dev_malloc(void **p, long bytes)
+{
+  std::lock_guard<std::mutex> lk(g_devwide_mu);
+  return hipMalloc(p, bytes);
+}
+static inline hipError_t dev_memset(void *p, int v, long bytes)
+{
+  std::lock_guard<std::mutex> lk(g_devwide_mu);
+  return hipMemset(p, v, bytes);
+}
+static inline hipError_t dev_upload(void *dst, const void *src, long bytes)
+{
+  std::lock_guard<std::mutex> lk(g_devwide_mu);
+  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+}
 
 // ------------------------------------------------------- kernel timers
 // Optional HIP-event timing of every launch of the hot kernels, on the
@@ -59,7 +89,7 @@ static inline int dalloc(csem_model *m, T **p, long count)
 {
   long bytes = count * (long)sizeof(T);
   if (bytes <= 0) bytes = 256;
-  hipError_t e = hipMalloc((void **)p, bytes);
+  hipError_t e = dev_malloc((void **)p, bytes);
   if (e != hipSuccess) return csem_fail(-2, "hipMalloc(%ld): %s", bytes, hipGetErrorString(e));
   m->allocs.push_back((void *)*p);
   m->bytes += bytes;
@@ -71,7 +101,7 @@ static inline int upload(csem_model *m, T **p, const void *src, long count)
 {
   int rc = dalloc(m, p, count);
   if (rc) return rc;
-  HIP_OK(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+  HIP_OK(dev_upload(*p, src, count * sizeof(T)));
   return 0;
 }
 

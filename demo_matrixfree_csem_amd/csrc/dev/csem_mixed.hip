@@ -69,7 +69,7 @@ static int mx_dalloc(S *x, T **p, long count)
 {
   long bytes = count * (long)sizeof(T);
   if (bytes <= 0) bytes = 256;
-  hipError_t e = hipMalloc((void **)p, bytes);
+  hipError_t e = dev_malloc((void **)p, bytes);
   if (e != hipSuccess) return csem_fail(-2, "hipMalloc(%ld): %s", bytes, hipGetErrorString(e));
   x->allocs.push_back((void *)*p);
   x->bytes += bytes;
@@ -80,7 +80,7 @@ static void slot_free(MixSlot *q)
 {
   if (q->gexec) (void)hipGraphExecDestroy(q->gexec);
   if (q->own) (void)hipStreamDestroy(q->own);
-  for (void *p : q->allocs) (void)hipFree(p);
+  for (void *p : q->allocs) dev_free(p);
   delete q;
 }
 
@@ -89,7 +89,7 @@ void mixed_free(csem_model *m)
   MixState *x = m->mx;
   if (!x) return;
   for (MixSlot *q : x->slots) slot_free(q);
-  for (void *p : x->allocs) (void)hipFree(p);
+  for (void *p : x->allocs) dev_free(p);
   delete x;
   m->mx = NULL;
 }
@@ -150,7 +150,11 @@ static MxGeo mx_geo(const csem_model *m, const MixState *x)
 
 #define MA(o, p, cnt) if ((rc = mx_dalloc(o, &o->p, cnt))) goto fail
 
-// The FP32 model, once per model (callers hold no lock: first use only).
+// Guards the creation of a model's mixed-precision state (first use of a
+// model from several threads) and readers of m->mx.
+static std::mutex g_mx_init_mu;
+
+// The FP32 model, once per model (callers hold g_mx_init_mu).
 static int mx_model(csem_model *m, MixState **out)
 {
   if (m->mx) { *out = m->mx; return 0; }
@@ -194,7 +198,7 @@ static int mx_model(csem_model *m, MixState **out)
   *out = x;
   return 0;
 fail:
-  for (void *p : x->allocs) (void)hipFree(p);
+  for (void *p : x->allocs) dev_free(p);
   delete x;
   return rc;
 }
@@ -221,8 +225,8 @@ static int slot_new(csem_model *m, int R, MixSlot **out)
     MA(q, res, R * m->lE); MA(q, tmp64, m->lE); MA(q, npart, (long)R * q->nch); MA(q, norm, R);
     MA(q, scale, R);
   }
-  if (hipMemset(q->W, 0, m->fE * R * sizeof(c2)) != hipSuccess ||
-      hipMemset(q->sc, 0, (long)R * MS_N * sizeof(double)) != hipSuccess) {
+  if (dev_memset(q->W, 0, m->fE * R * sizeof(c2)) != hipSuccess ||
+      dev_memset(q->sc, 0, (long)R * MS_N * sizeof(double)) != hipSuccess) {
     rc = csem_fail(-1, "memset");
     goto fail;
   }
@@ -505,11 +509,20 @@ static int mx_graph(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
   if (q->gexec && q->g_R == R && q->g_stream == s && !memcmp(&q->g_fr, &fr, sizeof fr)) return 0;
   if (q->gexec) { (void)hipGraphExecDestroy(q->gexec); q->gexec = NULL; }
   hipGraph_t g;
-  kt_pause(1);
-  HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  int rc = mx_iteration(m, x, q, R, fr, s);
-  hipError_t e = hipStreamEndCapture(s, &g);
-  kt_pause(0);
+  int rc;
+  hipError_t e;
+  {
+    std::lock_guard<std::mutex> lk(g_devwide_mu);
+    kt_pause(1);
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      kt_pause(0);
+      return csem_fail(-1, "graph capture: %s", hipGetErrorString(e));
+    }
+    rc = mx_iteration(m, x, q, R, fr, s);
+    e = hipStreamEndCapture(s, &g);
+    kt_pause(0);
+  }
   if (rc) return rc;
   if (e != hipSuccess) return csem_fail(-1, "graph capture: %s", hipGetErrorString(e));
   e = hipGraphInstantiate(&q->gexec, g, NULL, NULL, 0);
@@ -650,8 +663,7 @@ extern "C" int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *
   MixSlot *q;
   int rc;
   {
-    static std::mutex init_mu;  // first use of a model from several threads
-    std::lock_guard<std::mutex> lk(init_mu);
+    std::lock_guard<std::mutex> lk(g_mx_init_mu);
     if ((rc = mx_model(m, &x))) return rc;
   }
   if ((rc = slot_acquire(m, x, R, &q))) return rc;
@@ -675,8 +687,7 @@ static int mx_single(csem_model *m, int R, const csem_freq *f, const void *xv, v
   MixSlot *q;
   int rc;
   {
-    static std::mutex init_mu;
-    std::lock_guard<std::mutex> lk(init_mu);
+    std::lock_guard<std::mutex> lk(g_mx_init_mu);
     if ((rc = mx_model(m, &x))) return rc;
   }
   if ((rc = slot_acquire(m, x, R, &q))) return rc;
@@ -704,9 +715,16 @@ extern "C" int csem_lusgs32(csem_model *m, int R, const csem_freq *f, const void
 
 extern "C" int csem_mixed_tuning(csem_model *m, int R, int *rows, int *planes)
 {
-  if (!m->mx || R < 1 || R > MX_RMAX || !m->mx->ap_tj[R]) return -1;
-  *rows = m->mx->ap_tj[R];
-  *planes = m->mx->ap_kc[R];
+  MixState *x;
+  {
+    std::lock_guard<std::mutex> lk(g_mx_init_mu);
+    x = m->mx;
+  }
+  if (!x || R < 1 || R > MX_RMAX) return -1;
+  std::lock_guard<std::mutex> lk(x->mu);
+  if (!x->ap_tj[R]) return -1;
+  *rows = x->ap_tj[R];
+  *planes = x->ap_kc[R];
   return 0;
 }
 
@@ -718,8 +736,7 @@ extern "C" int csem_mixed_set_tile(csem_model *m, int R, int rows, int planes)
   if (!(rows == 2 || rows == 4 || (rows == 8 && R <= 2)) || !(planes >= 1 && planes <= 64))
     return csem_fail(-3, "tile %d x %d not available for %d right-hand sides", rows, planes, R);
   {
-    static std::mutex init_mu;
-    std::lock_guard<std::mutex> lk(init_mu);
+    std::lock_guard<std::mutex> lk(g_mx_init_mu);
     if ((rc = mx_model(m, &x))) return rc;
   }
   std::lock_guard<std::mutex> lk(x->mu);
@@ -728,10 +745,42 @@ extern "C" int csem_mixed_set_tile(csem_model *m, int R, int rows, int planes)
   return 0;
 }
 
+// device bytes slot_new allocates for R right-hand sides (admission control)
+extern "C" long csem_mixed_slot_bytes(const csem_model *m, int R)
+{
+  const long n = m->lE, c2b = sizeof(c2), zb = sizeof(z2);
+  long mx = 0;
+  for (int c = 0; c < 3; c++) mx = std::max(mx, (long)m->g.ns[c] * m->g.A[c] * m->g.B[c]);
+  const long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
+  long b = (m->fE * R + mx * R + m->nsk * R + 2L * SGS_MAXT * MX_RMAX + 9L * R * n) * c2b + (long)R * nch * 4 * 8 +
+           (long)R * MS_N * 8 + (R * n + n) * zb + (long)R * nch * 8 + R * 8 + R * 4;
+  if (m->airbc) {
+    const long NN = (long)m->nfft * m->nfft, p1 = (long)m->n1 * m->n2, p2 = (long)(m->n1 + 1) * m->n2,
+               p3 = (long)m->n1 * (m->n2 + 1);
+    b += (3 * NN + p1 + p2 + p3) * R * c2b + (3 * NN + p1 + p2 + p3) * zb;
+  }
+  return b;
+}
+
+extern "C" int csem_mem_info(long *free_bytes, long *total_bytes)
+{
+  size_t f = 0, t = 0;
+  HIP_OK(hipMemGetInfo(&f, &t));
+  *free_bytes = (long)f;
+  *total_bytes = (long)t;
+  return 0;
+}
+
 extern "C" long csem_mixed_bytes(const csem_model *m)
 {
-  if (!m->mx) return 0;
-  long b = m->mx->bytes;
-  for (const MixSlot *q : m->mx->slots) b += q->bytes;
+  MixState *x;
+  {
+    std::lock_guard<std::mutex> lk(g_mx_init_mu);
+    x = m->mx;
+  }
+  if (!x) return 0;
+  std::lock_guard<std::mutex> lk(x->mu);  // slots may be added by a running solve
+  long b = x->bytes;
+  for (const MixSlot *q : x->slots) b += q->bytes;
   return b;
 }

@@ -135,6 +135,44 @@ class Session:
         arr = np.ctypeslib.as_array(C.cast(base, C.POINTER(C.c_float)), (total * 2,)).copy().view(np.complex64)
         return arr.reshape(e.nchrec, e.nfreq, a.nrec)[:, ifreq, :]
 
+    def inject_at(self, sx, sy, sz):
+        """inject_source for a source at (sx, sy, sz) (the first source's
+        slot of acq_t, acq_init_free.c:65-69, is repointed for the call)."""
+        a = self.acq
+        old = (a.src_x1[0], a.src_x2[0], a.src_x3[0])
+        a.src_x1[0], a.src_x2[0], a.src_x3[0] = sx, sy, sz
+        try:
+            return self.inject()
+        finally:
+            a.src_x1[0], a.src_x2[0], a.src_x3[0] = old
+
+    def extract_at(self, x, receivers):
+        """extract_emf at the given receiver positions [(x, y, z), ...];
+        returns complex64 (nchrec, nrec)."""
+        a, e, L = self.acq, self.emf, lib()
+        nr = len(receivers)
+        r = np.asarray(receivers, np.float32).reshape(nr, -1)
+        # a pointer read from a ctypes Structure aliases the field: keep the
+        # addresses, not the pointer objects
+        names = ("rec_x1", "rec_x2", "rec_x3")
+        keep = [C.cast(getattr(a, k), C.c_void_p).value for k in names]
+        keep_fd = C.cast(e.dcal_fd, C.c_void_p).value
+        nrec = a.nrec
+        bufs = [np.ascontiguousarray(r[:, c]) for c in range(3)]
+        try:
+            a.nrec = nr
+            for k, b in zip(names, bufs):
+                setattr(a, k, C.cast(b.ctypes.data, C.POINTER(C.c_float)))
+            e.dcal_fd = L.alloc3complexf(nr, e.nfreq, e.nchrec)
+            out = self.extract(x)
+            L.free3complexf(e.dcal_fd)
+            return out
+        finally:
+            a.nrec = nrec
+            for k, v in zip(names, keep):
+                setattr(a, k, C.cast(C.c_void_p(v), C.POINTER(C.c_float)))
+            e.dcal_fd = C.cast(C.c_void_p(keep_fd), type(e.dcal_fd))
+
     def solve(self, b, tol, niter=None, algopt=None, preco=None, nrestart=None, x0=None):
         """The solver call of main.c:104-118; returns (x, iterations)."""
         e = self.emf

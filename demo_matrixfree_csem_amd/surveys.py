@@ -10,6 +10,8 @@ driver both read them.
 Generators cover the measured workloads:
   layered_64      64x64x64 uniform 100 m cells, water + 1 ohm-m + 100 ohm-m slab
   layered_100     100^3 uniform 100 m cells, 8 seeded log-uniform layers
+  marine          any grid, layered or Gaussian-random-field resistivity under water
+  serve_request   a seeded serving request: model, frequencies, sources, receivers
   tiny            small seeded random models for parity tests
 """
 from dataclasses import dataclass, field
@@ -165,3 +167,69 @@ def tiny(seed=0, nx=10, ny=9, nz=7, airbc=1, aniso=True, dz=100.0):
 
 
 GENERATORS = {"layered_64": layered_64, "layered_100": layered_100, "tiny": tiny}
+
+
+def marine(seed, nx, ny, nz, kind="layered", median=2.0, dx=100.0, water=1000.0, nlayers=8):
+    """Uniform dx cells centred in x and y, z from 0 down; seawater above
+    `water` m, below it either `nlayers` seeded log-uniform layers around
+    `median` (kind "layered") or a log-normal Gaussian random field with
+    median `median` (FFT-filtered white noise, correlation length ~1 km,
+    log-standard deviation 0.5; kind "grf")."""
+    rng = np.random.default_rng(seed)
+    x = (dx * (np.arange(nx + 1) - nx / 2)).astype(np.float32)
+    y = (dx * (np.arange(ny + 1) - ny / 2)).astype(np.float32)
+    z = (dx * np.arange(nz + 1)).astype(np.float32)
+    zc = _centers(z)
+    if kind == "grf":
+        w = rng.standard_normal((nz, ny, nx))
+        k = [np.fft.fftfreq(nn, d=dx) for nn in (nz, ny, nx)]
+        kk = np.sqrt(k[0][:, None, None] ** 2 + k[1][None, :, None] ** 2 + k[2][None, None, :] ** 2)
+        g = np.real(np.fft.ifftn(np.fft.fftn(w) * np.exp(-(kk * 1000.0) ** 2)))
+        g = (g - g.mean()) / (g.std() + 1e-30)
+        rho = median * np.exp(0.5 * g)
+    else:
+        depth = max(nz * dx - water, dx)
+        rl = median * 10 ** rng.uniform(-1, 1, nlayers)
+        layer = np.clip(((zc - water) // (depth / nlayers)).astype(int), 0, nlayers - 1)
+        rho = rl[layer][:, None, None] * np.ones((nz, ny, nx))
+    rho = np.where((zc < water)[:, None, None], RHO_WATER, rho).astype(np.float32)
+    return Model(x, y, z, rho, rho.copy(), rho.copy())
+
+
+# grids of the serving stream (cells): 64^3, 100^3, 128x128x96, 180x120x120
+SERVE_GRIDS = [(64, 64, 64), (100, 100, 100), (128, 128, 96), (180, 120, 120)]
+
+
+def serve_request(seed, grids=None, max_freqs=6, max_sources=4, nrec=(33, 121), n_models=4, nfft=None):
+    """A seeded serving request: one of `n_models` seeded models per grid
+    (layered or random-field, water median 1-5 ohm-m; requests share
+    models, so their coefficients can stay on the device), 1..max_freqs
+    frequencies in [0.125, 1.25] Hz, 1..max_sources Ex sources 50 m above
+    the seafloor and 33..121 receivers on the seafloor along y = 0;
+    nfft sets the airwave transform size (default the reference's 1024).
+    Returns (model_key, model builder, freqs, sources, receivers)."""
+    rng = np.random.default_rng(seed)
+    grids = grids or SERVE_GRIDS
+    g = grids[rng.integers(len(grids))]
+    mseed = int(rng.integers(n_models))
+    kind = "grf" if mseed % 2 else "layered"
+    median = 1.0 + mseed  # 1..n_models ohm-m
+    key = (g, kind, mseed)
+    nf = int(rng.integers(1, max_freqs + 1))
+    freqs = sorted(float(np.float32(v)) for v in rng.choice(
+        [0.125, 0.25, 0.375, 0.5, 0.625, 0.75, 0.875, 1.0, 1.125, 1.25], nf, replace=False))
+    half = 0.35 * 100.0 * g[0]
+    ns = int(rng.integers(1, max_sources + 1))
+    srcs = [(float(v), 0.0, 950.0) for v in np.linspace(-half / 2, half / 2, ns)]
+    nr = int(rng.integers(nrec[0], nrec[1] + 1))
+    recs = [(float(v), 0.0, 1000.0) for v in np.linspace(-half, half, nr)]
+
+    flags = dict(fvm=0, airbc=1, preco=1, algopt=1, tol=1e-8, niter=2000, nb=10, chsrc="Ex", chrec="Ex,Ey,Ez")
+    if nfft:
+        flags.update(n1fft=nfft, n2fft=nfft)
+
+    def build():
+        return Survey("serve_%dx%dx%d_%s%d" % (g + (kind, mseed)),
+                      marine(1000 + mseed, g[0], g[1], g[2], kind=kind, median=median),
+                      Acquisition(_acq_table(srcs[:1]), _acq_table(recs)), freqs, dict(flags))
+    return key, build, freqs, srcs, recs

@@ -143,8 +143,13 @@ int csem_mixed_tuning(csem_model *m, int R, int *rows, int *planes);
 /* Fix the FP32 operator's tile at batch size R instead of timing the
  * candidates (rows 2, 4, or 8 for R <= 2; planes 1..64). */
 int csem_mixed_set_tile(csem_model *m, int R, int rows, int planes);
-/* device bytes held by the mixed-precision state of a model */
+/* device bytes held by the mixed-precision state of a model (its FP32
+ * copy and every solve slot), and the bytes one more slot for R
+ * right-hand sides takes: what a scheduler admits solves against */
 long csem_mixed_bytes(const csem_model *m);
+long csem_mixed_slot_bytes(const csem_model *m, int R);
+/* free and total device memory (hipMemGetInfo) */
+int csem_mem_info(long *free_bytes, long *total_bytes);
 
 /* Kernel timers: csem_kernel_timing(1) resets and starts HIP-event timing
  * of every hot-kernel launch (events recorded on the launch stream);

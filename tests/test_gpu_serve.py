@@ -1,0 +1,86 @@
+"""Serving loop on the GPU: every request's receivers are the same served
+alone and served in a mix of overlapping requests on several models, and
+admission keeps the charged device footprint within the budget, evicting
+idle models when it must.
+
+Tolerance: receivers within 1e-6 of the channel's largest value (the mixed
+solve's receiver tolerance against the FP64 oracle); each solve's true
+FP64 relative residual below its tol = 1e-8/f.
+"""
+import numpy as np
+import pytest
+
+from demo_matrixfree_csem_amd import surveys
+from demo_matrixfree_csem_amd.serve import Engine, solve_tol
+
+pytestmark = pytest.mark.gpu
+SMALL = [(24, 24, 16), (32, 28, 20)]
+
+
+def _requests(n, seed, n_models=2):
+    return [surveys.serve_request(seed * 100 + i, grids=SMALL, max_freqs=2, max_sources=2, nrec=(9, 17),
+                                  n_models=n_models, nfft=64) for i in range(n)]
+
+
+def _rec_err(a, b):
+    a, b = a.astype(np.complex128), b.astype(np.complex128)
+    scale = np.abs(b).max(axis=1, keepdims=True)
+    return float((np.abs(a - b) / np.where(scale > 0, scale, 1)).max())
+
+
+def _check_converged(res):
+    for (s, f), (rec, it, rr) in res["receivers"].items():
+        assert rr < solve_tol(f), (s, f, it, rr)
+
+
+def test_each_request_same_alone_and_in_the_mix():
+    reqs = _requests(6, 1)
+    print("case serve mix: %d requests on %d models" % (len(reqs), len({r[0] for r in reqs})), flush=True)
+    mix = Engine(workers=3, rhs_per_solve=2)
+    try:
+        res_mix, st = mix.run(reqs, [0.01 * i for i in range(len(reqs))])
+    finally:
+        mix.close()
+    print(st, flush=True)
+    assert st["solves"] == sum(len(r[2]) * len(r[3]) for r in reqs)
+    assert st["peak_admitted_bytes"] <= st["budget_bytes"]
+    alone = Engine(workers=1, rhs_per_solve=1)
+    try:
+        for i, req in enumerate(reqs):
+            print("case serve alone: request %d" % i, flush=True)
+            res, _ = alone.run([req], [0.0])
+            _check_converged(res_mix[i])
+            _check_converged(res[0])
+            assert set(res[0]["receivers"]) == set(res_mix[i]["receivers"])
+            for k, (rec, it, rr) in res[0]["receivers"].items():
+                e = _rec_err(res_mix[i]["receivers"][k][0], rec)
+                assert e < 1e-6, (i, k, e)
+    finally:
+        alone.close()
+
+
+def test_admission_evicts_idle_models_within_budget():
+    reqs = _requests(6, 2, n_models=4)
+    keys = {r[0] for r in reqs}
+    probe = Engine(workers=1)
+    try:
+        probe.preload(reqs)
+        # one model resident with two solves of two right-hand sides running
+        # (two pooled slots, two more charged) and both workers' buffers
+        one = max(m.bytes() + 4 * m.slot_bytes(2) + 2 * 2 * (2 * m.n * 16) for m in probe.models.values())
+    finally:
+        probe.close()
+    budget = int(1.1 * one)
+    print("case serve eviction: %d models, budget %d bytes" % (len(keys), budget), flush=True)
+    eng = Engine(budget_bytes=budget, workers=2, rhs_per_solve=2)
+    try:
+        res, st = eng.run(reqs, [0.0] * len(reqs))
+        print(st, flush=True)
+        assert st["peak_admitted_bytes"] <= budget
+        assert all(r is not None for r in res)
+        for r in res:
+            _check_converged(r)
+        if len(keys) > 2:
+            assert st["evictions"] >= 1
+    finally:
+        eng.close()
