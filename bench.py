@@ -55,6 +55,34 @@ def _roofline(stats, name):
             "bytes_per_launch": int(per_launch), "avg_us": round(avg_s * 1e6, 3), "launches": cnt}
 
 
+def _lusgs_roofline(stats, dims, R):
+    """One LU-SGS application (its six pre-passes and six sweeps) against
+    the bytes the preconditioner needs per unknown: forward Ex 40, Ey 56,
+    Ez 72, backward 88 each (b, sigma, the values read, the value written)
+    in complex FP64, half that in complex FP32 per right-hand side (R > 0:
+    the batched FP32 path, R = 0: the FP64 path)."""
+    n1, n2, n3f = dims
+    nc = [n1 * (n3f - 1) * (n2 - 1), n2 * (n3f - 1) * (n1 - 1), n3f * (n2 - 1) * (n1 - 1)]
+    per = float(sum(c * (f + 88) for c, f in zip(nc, (40, 56, 72))))
+    if R:
+        pre, sweeps = "k_mx_lu_pre", ["k_mx_sgs"]
+        per *= R / 2.0
+        if "k_mx_sgs" not in stats:
+            return None
+        napp = stats["k_mx_sgs"][1] / 6
+    else:
+        pre, sweeps = "k_lu_pre", [k for k in stats if k.startswith("k_sgs")]
+        if "k_sgs_ex<fwd>" not in stats:
+            return None
+        napp = stats["k_sgs_ex<fwd>"][1]
+    ms = stats.get(pre, (0.0, 0, 0.0))[0] + sum(stats[k][0] for k in sweeps)
+    avg_s = ms / napp / 1e3
+    gbs = per / avg_s / 1e9
+    return {"what": "one LU-SGS application: 6 pre-passes + 6 sweeps", "bound": "hbm", "achieved": round(gbs, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "bytes_per_application": int(per), "avg_us": round(avg_s * 1e6, 3), "applications": int(napp)}
+
+
 def cpu_baseline(survey, freq, iters, gpu_iters):
     """The oracle (serial C restatement of the reference path) on this
     host: `iters` LU-SGS F-BiCGStab iterations of one solve, scaled to
@@ -88,8 +116,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 path (profiling runs)")
     ap.add_argument("--niter", type=int, default=2000)
-    ap.add_argument("--serve-requests", type=int, default=10, help="requests of the serving run (0: none)")
-    ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrivals per second of the serving run")
+    ap.add_argument("--serve-requests", type=int, default=12, help="requests of the serving run (0: none)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second of the serving run")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the mixed inner iterations eagerly instead of replaying a captured hipGraph")
+    ap.add_argument("--kernels-only", type=int, default=0, metavar="ITERS",
+                    help="counter runs: one eager batched mixed solve of ITERS inner iterations and a 1 GiB device "
+                         "copy of known size (for scaling FETCH_SIZE / WRITE_SIZE), then exit; prints no bench line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this path has no multi-GPU form; independent solves would "
@@ -125,7 +158,23 @@ def main():
     tol_arr = np.array(tols)
     its = np.zeros(R, np.int32)
     rel = np.zeros(R)
-    opts = _lib.MixedOpts.default(max_inner=args.niter)
+    opts = _lib.MixedOpts.default(max_inner=args.niter, use_graph=0 if args.no_graph else 1)
+
+    if args.kernels_only:
+        ko = _lib.MixedOpts.default(max_inner=args.kernels_only, max_outer=1, check_every=args.kernels_only,
+                                    use_graph=0)
+        L.csem_mixed_solve(model, R, fr, dxm, db, tol_arr.ctypes.data, C.byref(ko), its.ctypes.data,
+                           rel.ctypes.data, None)
+        L.csem_sync(None)
+        ca, cb = C.c_void_p(), C.c_void_p()
+        for d in (ca, cb):
+            _lib.check(L.csem_malloc(C.byref(d), 2 ** 30), "csem_malloc")
+        for _ in range(3):
+            _lib.check(L.csem_copy(cb, ca, 2 ** 30, None), "csem_copy")
+        L.csem_sync(None)
+        print(json.dumps({"kernels_only": args.kernels_only, "copy_bytes_each_way": 2 ** 30, "n_op": n}),
+              flush=True)
+        return
 
     def plain_step():
         out = []
@@ -245,6 +294,8 @@ def main():
     L.csem_kernel_timing(0)
 
     dominant = max(stm, key=lambda k: stm[k][0])
+    dd = ses.dims
+    lu_dims = (dd["n1"], dd["n2"], dd["nz"] + 2 * dd["nb"])
     out = {
         "metric": METRIC,
         "value": mixed["solves_per_s"],
@@ -268,6 +319,8 @@ def main():
         "roofline": _roofline(stm, dominant),
         "apply_roofline": _roofline(stm, "k_mx_apply"),
         "apply_fp64_roofline": _roofline(stp, "k_apply") if stp else None,
+        "lusgs_roofline_mixed": _lusgs_roofline(stm, lu_dims, R),
+        "lusgs_roofline_fp64": _lusgs_roofline(stp, lu_dims, 0) if stp else None,
         "kernel_ms_mixed": {k: round(v[0] / args.steps, 3) for k, v in sorted(stm.items(), key=lambda kv: -kv[1][0])},
         "kernel_ms_plain": ({k: round(v[0] / args.steps, 3) for k, v in sorted(stp.items(), key=lambda kv: -kv[1][0])}
                             if stp else None),
@@ -277,7 +330,8 @@ def main():
         # 64^3 and 100^3 grids at Poisson arrival times
         from demo_matrixfree_csem_amd import serve
         reqs = serve.quick_requests(args.serve_requests, 4)
-        eng = serve.Engine(workers=3, rhs_per_solve=1)
+        eng = serve.Engine(workers=3, rhs_per_solve=1,
+                           opts=_lib.MixedOpts.default(use_graph=0) if args.no_graph else None)
         try:
             eng.preload(reqs)
             _, sst = eng.run(reqs, serve.poisson_arrivals(len(reqs), args.serve_rate, 4))

@@ -102,7 +102,7 @@ DEV_SYMBOLS = [
     "csem_last_error", "csem_freq_consts", "csem_model_create", "csem_model_destroy", "csem_model_len",
     "csem_model_full_len", "csem_model_bytes", "csem_apply", "csem_lusgs", "csem_curl_h", "csem_airbc",
     "csem_fft2", "csem_dot", "csem_bicgstab", "csem_fgmres", "csem_gmres", "csem_scatter_add", "csem_gather",
-    "csem_malloc", "csem_free", "csem_memcpy_h2d", "csem_memcpy_d2h", "csem_memset", "csem_sync",
+    "csem_malloc", "csem_free", "csem_memcpy_h2d", "csem_memcpy_d2h", "csem_memset", "csem_copy", "csem_sync",
     "csem_kernel_timing", "csem_kernel_stats", "csem_set_reduction",
     "csem_vec_sub", "csem_vec_divr", "csem_vec_divz", "csem_vec_subscaled_dev", "csem_vec_addscaled",
     "csem_mixed_solve", "csem_apply32", "csem_lusgs32", "csem_mixed_tuning", "csem_mixed_set_tile", "csem_mixed_bytes", "csem_mixed_slot_bytes", "csem_mem_info",
@@ -142,6 +142,7 @@ def lib():
     L.csem_memcpy_h2d.argtypes = [vp, vp, lp, vp]
     L.csem_memcpy_d2h.argtypes = [vp, vp, lp, vp]
     L.csem_memset.argtypes = [vp, ip, lp, vp]
+    L.csem_copy.argtypes = [vp, vp, lp, vp]
     L.csem_sync.argtypes = [vp]
     L.csem_kernel_timing.argtypes = [ip]
     L.csem_mixed_solve.argtypes = [vp, ip, C.POINTER(FreqT), vp, vp, vp, C.POINTER(MixedOpts), vp, vp, vp]

@@ -394,27 +394,30 @@ static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
              (size_t)(g.A[c] + g.B[c] + 1) * sizeof(int);
     gs[c] = 8 * ((g.ns[c] + 7) / 8);
   }
-  // algorithmic bytes per swept cell: forward Ex 40 (b, sigma, write),
-  // Ey 56 (+Ex), Ez 72 (+Ex, Ey); backward 88 (b, sigma, old value, the
-  // two other components, write).  The records and diagonals this
-  // schedule reads are the price of it and are not counted.
   double nc[3];
   for (int c = 0; c < 3; c++) nc[c] = (double)g.ns[c] * g.A[c] * g.B[c];
   // the fvm scaling is a template argument: a runtime branch in the sweep
   // loop would make the compiler drain the prefetched loads every step
 #define SWEEP(C, F) (m->fvm ? k_sgs<C, F, true> : k_sgs<C, F, false>)
-  KT(KT_LU_PRE, s, 0.0, (k_lu_pre<0, true><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_SGS_EX_F, s, 40 * nc[0], (SWEEP(0, true)<<<gs[0], tb[0], lds[0], s>>>(a)));
-  KT(KT_LU_PRE, s, 0.0, (k_lu_pre<1, true><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_SGS_EY_F, s, 56 * nc[1], (SWEEP(1, true)<<<gs[1], tb[1], lds[1], s>>>(a)));
-  KT(KT_LU_PRE, s, 0.0, (k_lu_pre<2, true><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_SGS_EZ_F, s, 72 * nc[2], (SWEEP(2, true)<<<gs[2], tb[2], lds[2], s>>>(a)));
-  KT(KT_LU_PRE, s, 0.0, (k_lu_pre<2, false><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_SGS_EZ_B, s, 88 * nc[2], (SWEEP(2, false)<<<gs[2], tb[2], lds[2], s>>>(a)));
-  KT(KT_LU_PRE, s, 0.0, (k_lu_pre<1, false><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_SGS_EY_B, s, 88 * nc[1], (SWEEP(1, false)<<<gs[1], tb[1], lds[1], s>>>(a)));
-  KT(KT_LU_PRE, s, 0.0, (k_lu_pre<0, false><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_SGS_EX_B, s, 88 * nc[0], (SWEEP(0, false)<<<gs[0], tb[0], lds[0], s>>>(a)));
+  // bytes per swept cell each kernel must move: the pre-pass reads b and
+  // the other components' W values it differences and writes the record
+  // (only b for Ex forward); the sweep reads the record and the diagonal
+  // and writes W (not in the last sweep) and y (backward sweeps).  The
+  // records' round trip is the price of the parallel pre-pass: the whole
+  // application needs 144 bytes per unknown (forward Ex 40, Ey 56, Ez 72,
+  // backward 88 each: b, sigma, the values read, the value written).
+  KT(KT_LU_PRE, s, 32.0 * nc[0], (k_lu_pre<0, true><<<gq[0], bq, 0, s>>>(a)));
+  KT(KT_SGS_EX_F, s, 64.0 * nc[0], (SWEEP(0, true)<<<gs[0], tb[0], lds[0], s>>>(a)));
+  KT(KT_LU_PRE, s, 112.0 * nc[1], (k_lu_pre<1, true><<<gq[1], bq, 0, s>>>(a)));
+  KT(KT_SGS_EY_F, s, 128.0 * nc[1], (SWEEP(1, true)<<<gs[1], tb[1], lds[1], s>>>(a)));
+  KT(KT_LU_PRE, s, 128.0 * nc[2], (k_lu_pre<2, true><<<gq[2], bq, 0, s>>>(a)));
+  KT(KT_SGS_EZ_F, s, 128.0 * nc[2], (SWEEP(2, true)<<<gs[2], tb[2], lds[2], s>>>(a)));
+  KT(KT_LU_PRE, s, 144.0 * nc[2], (k_lu_pre<2, false><<<gq[2], bq, 0, s>>>(a)));
+  KT(KT_SGS_EZ_B, s, 144.0 * nc[2], (SWEEP(2, false)<<<gs[2], tb[2], lds[2], s>>>(a)));
+  KT(KT_LU_PRE, s, 144.0 * nc[1], (k_lu_pre<1, false><<<gq[1], bq, 0, s>>>(a)));
+  KT(KT_SGS_EY_B, s, 144.0 * nc[1], (SWEEP(1, false)<<<gs[1], tb[1], lds[1], s>>>(a)));
+  KT(KT_LU_PRE, s, 144.0 * nc[0], (k_lu_pre<0, false><<<gq[0], bq, 0, s>>>(a)));
+  KT(KT_SGS_EX_B, s, 128.0 * nc[0], (SWEEP(0, false)<<<gs[0], tb[0], lds[0], s>>>(a)));
 #undef SWEEP
   HIP_OK(hipGetLastError());
   return 0;
@@ -948,6 +951,23 @@ extern "C" int csem_memcpy_d2h(void *dst, const void *src, long bytes, void *str
 {
   HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(stream)));
   HIP_OK(hipStreamSynchronize(S(stream)));
+  return 0;
+}
+// A plain streaming copy (16-byte loads and stores, grid-stride): the
+// reference kernel against which hardware byte counters are scaled.
+__global__ __launch_bounds__(256) void k_copy16(long n16, const float4 *src, float4 *dst)
+{
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < n16; e += (long)gridDim.x * 256) dst[e] = src[e];
+}
+extern "C" int csem_copy(void *dst, const void *src, long bytes, void *stream)
+{
+  if (bytes <= 0) return 0;
+  if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+    return csem_fail(-3, "csem_copy needs 16-byte aligned buffers and a multiple of 16 bytes");
+  const long n16 = bytes / 16;
+  const long blocks = std::min<long>((n16 + 255) / 256, 256L * 32);
+  k_copy16<<<(unsigned)blocks, 256, 0, S(stream)>>>(n16, (const float4 *)src, (float4 *)dst);
+  HIP_OK(hipGetLastError());
   return 0;
 }
 extern "C" int csem_memset(void *p, int v, long bytes, void *stream)

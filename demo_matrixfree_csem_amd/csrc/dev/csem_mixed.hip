@@ -418,21 +418,22 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, bool diag, bool sweep
     gs[c] = 8 * ((g.ns[c] + 7) / 8);
     nc[c] = (double)g.ns[c] * g.A[c] * g.B[c];
   }
-  // algorithmic bytes per swept cell and right-hand side: read b, write
-  // the value, plus the other components read (as the FP64 accounting, in
-  // FP32: forward Ex 16, Ey 24, Ez 32; backward 40)
-  KT(KT_M_PRE, s, 0.0, (k_mx_pre<0, true, R><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 16.0 * R * nc[0], (k_mx_sgs<0, true, R><<<gs[0], tb[0], lds[0], s>>>(a)));
-  KT(KT_M_PRE, s, 0.0, (k_mx_pre<1, true, R><<<gq[1], bq, 0, s>>>(a)));
+  // bytes per swept cell and right-hand side each kernel must move: the
+  // pre-pass reads b, h and the other components' W values it differences
+  // and writes T; the sweep reads T and h and writes W (not in the last
+  // sweep) and y (backward sweeps)
+  KT(KT_M_PRE, s, 24.0 * R * nc[0], (k_mx_pre<0, true, R><<<gq[0], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[0], (k_mx_sgs<0, true, R><<<gs[0], tb[0], lds[0], s>>>(a)));
+  KT(KT_M_PRE, s, 32.0 * R * nc[1], (k_mx_pre<1, true, R><<<gq[1], bq, 0, s>>>(a)));
   KT(KT_M_SGS, s, 24.0 * R * nc[1], (k_mx_sgs<1, true, R><<<gs[1], tb[1], lds[1], s>>>(a)));
-  KT(KT_M_PRE, s, 0.0, (k_mx_pre<2, true, R><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 32.0 * R * nc[2], (k_mx_sgs<2, true, R><<<gs[2], tb[2], lds[2], s>>>(a)));
-  KT(KT_M_PRE, s, 0.0, (k_mx_pre<2, false, R><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 40.0 * R * nc[2], (k_mx_sgs<2, false, R><<<gs[2], tb[2], lds[2], s>>>(a)));
-  KT(KT_M_PRE, s, 0.0, (k_mx_pre<1, false, R><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 40.0 * R * nc[1], (k_mx_sgs<1, false, R><<<gs[1], tb[1], lds[1], s>>>(a)));
-  KT(KT_M_PRE, s, 0.0, (k_mx_pre<0, false, R><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 40.0 * R * nc[0], (k_mx_sgs<0, false, R><<<gs[0], tb[0], lds[0], s>>>(a)));
+  KT(KT_M_PRE, s, 40.0 * R * nc[2], (k_mx_pre<2, true, R><<<gq[2], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[2], (k_mx_sgs<2, true, R><<<gs[2], tb[2], lds[2], s>>>(a)));
+  KT(KT_M_PRE, s, 48.0 * R * nc[2], (k_mx_pre<2, false, R><<<gq[2], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 32.0 * R * nc[2], (k_mx_sgs<2, false, R><<<gs[2], tb[2], lds[2], s>>>(a)));
+  KT(KT_M_PRE, s, 48.0 * R * nc[1], (k_mx_pre<1, false, R><<<gq[1], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 32.0 * R * nc[1], (k_mx_sgs<1, false, R><<<gs[1], tb[1], lds[1], s>>>(a)));
+  KT(KT_M_PRE, s, 48.0 * R * nc[0], (k_mx_pre<0, false, R><<<gq[0], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[0], (k_mx_sgs<0, false, R><<<gs[0], tb[0], lds[0], s>>>(a)));
 }
 
 static void mx_lusgs_dispatch(const csem_model *m, int R, const MxLu &a, bool diag, bool sweep, hipStream_t s)

@@ -165,6 +165,10 @@ int csem_free(void *p);
 int csem_memcpy_h2d(void *dst, const void *src, long bytes, void *stream);
 int csem_memcpy_d2h(void *dst, const void *src, long bytes, void *stream);
 int csem_memset(void *p, int v, long bytes, void *stream);
+/* dst = src on the device with a plain streaming kernel (16-byte aligned
+ * buffers, bytes a multiple of 16): a copy of known size for scaling
+ * hardware byte counters. */
+int csem_copy(void *dst, const void *src, long bytes, void *stream);
 int csem_sync(void *stream);
 
 #ifdef __cplusplus
