@@ -44,7 +44,8 @@ struct MixSlot {
   int busy;
   c2 *buf, *bufx, *bufy, *hz0, *hxg, *hyg;  // FP32 airwave scratch, R of each
   AirBufs a64;                              // FP64 airwave scratch (true residual)
-  c2 *W, *T, *h, *trash;                    // LU-SGS
+  c2 *W, *trash;                            // LU-SGS state and discarded stores
+  float4 *th;                               // LU-SGS records (T, h) per swept cell and r
   float hkey[MX_RMAX][2];                   // frequencies the diagonal factors h hold
   int hR;
   c2 *kv;                                   // Krylov vectors
@@ -217,9 +218,7 @@ static int slot_new(csem_model *m, int R, MixSlot **out)
     MA(q, a64.hz0, (long)n1 * n2); MA(q, a64.hxg, (long)(n1 + 1) * n2); MA(q, a64.hyg, (long)n1 * (n2 + 1));
   }
   {
-    long mx = 0;
-    for (int c = 0; c < 3; c++) mx = std::max(mx, (long)m->g.ns[c] * m->g.A[c] * m->g.B[c]);
-    MA(q, W, m->fE * R); MA(q, T, mx * R); MA(q, h, m->nsk * R); MA(q, trash, 2L * SGS_MAXT * MX_RMAX);
+    MA(q, W, m->fE * R); MA(q, th, m->nsk * R); MA(q, trash, 2L * SGS_MAXT * MX_RMAX);
     q->nch = (int)((m->lE + RED_CHUNK - 1) / RED_CHUNK);
     MA(q, kv, 9L * R * m->lE); MA(q, part, (long)R * q->nch * 4); MA(q, sc, (long)R * MS_N);
     MA(q, res, R * m->lE); MA(q, tmp64, m->lE); MA(q, npart, (long)R * q->nch); MA(q, norm, R);
@@ -385,7 +384,7 @@ static MxLu mx_lu(const csem_model *m, const MixState *x, MixSlot *q, const MxFr
   MxLu a;
   a.g = mx_geo(m, x); a.sk = m->g;
   a.b = b; a.y = y; a.W = q->W; a.fE = m->fE; a.fEx = m->fEx; a.fEy = m->fEy;
-  a.T = q->T; a.h = q->h; a.trash = q->trash;
+  a.th = q->th; a.nsk = m->nsk; a.trash = q->trash;
   a.c1m = x->c1m; a.c1p = x->c1p; a.c2m = x->c2m; a.c2p = x->c2p; a.c3m = x->c3m; a.c3p = x->c3p;
   a.fr = fr;
   return a;
@@ -403,37 +402,37 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, bool diag, bool sweep
     k_mx_diag<2, R><<<gp[2], 256, 0, s>>>(a);
   }
   if (!sweep) return;
-  gq[0] = dim3((m->n1 + 63) / 64, (m->n2 + 1 + 3) / 4, m->n3f + 1);
-  gq[1] = dim3((m->n1 + 1 + 63) / 64, (m->n2 + 3) / 4, m->n3f + 1);
-  gq[2] = dim3((m->n1 + 1 + 63) / 64, (m->n2 + 1 + 3) / 4, m->n3f);
-  const dim3 bq(64, 4);
+  // pre-pass tiles (MxPreTile): Ex 16 slices x 8 x 8, Ey and Ez 16 x 64
+  gq[0] = dim3((g.ns[0] + 15) / 16, (g.B[0] + 7) / 8, (g.A[0] + 7) / 8);
+  gq[1] = dim3((g.B[1] + 63) / 64, (g.A[1] + 15) / 16, g.ns[1]);
+  gq[2] = dim3((g.B[2] + 63) / 64, (g.A[2] + 15) / 16, g.ns[2]);
   int tb[3];
   size_t lds[3];
-  unsigned gs[3];
+  dim3 gs[3];
   double nc[3];
   for (int c = 0; c < 3; c++) {
     tb[c] = ((g.A[c] + 63) / 64) * 64;
-    lds[c] = 2 * (size_t)(tb[c] + 2) * R * sizeof(c2) + (size_t)(g.B[c] + 2) * sizeof(float) +
+    lds[c] = 2 * (size_t)(tb[c] + 2) * sizeof(c2) + (size_t)(g.B[c] + 2) * sizeof(float) +
              (size_t)(g.A[c] + g.B[c] + 1) * sizeof(int);
-    gs[c] = 8 * ((g.ns[c] + 7) / 8);
+    gs[c] = dim3(8 * ((g.ns[c] + 7) / 8), R);
     nc[c] = (double)g.ns[c] * g.A[c] * g.B[c];
   }
   // bytes per swept cell and right-hand side each kernel must move: the
-  // pre-pass reads b, h and the other components' W values it differences
+  // pre-pass reads b and the other components' W values it differences
   // and writes T; the sweep reads T and h and writes W (not in the last
   // sweep) and y (backward sweeps)
-  KT(KT_M_PRE, s, 24.0 * R * nc[0], (k_mx_pre<0, true, R><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[0], (k_mx_sgs<0, true, R><<<gs[0], tb[0], lds[0], s>>>(a)));
-  KT(KT_M_PRE, s, 32.0 * R * nc[1], (k_mx_pre<1, true, R><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[1], (k_mx_sgs<1, true, R><<<gs[1], tb[1], lds[1], s>>>(a)));
-  KT(KT_M_PRE, s, 40.0 * R * nc[2], (k_mx_pre<2, true, R><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[2], (k_mx_sgs<2, true, R><<<gs[2], tb[2], lds[2], s>>>(a)));
-  KT(KT_M_PRE, s, 48.0 * R * nc[2], (k_mx_pre<2, false, R><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 32.0 * R * nc[2], (k_mx_sgs<2, false, R><<<gs[2], tb[2], lds[2], s>>>(a)));
-  KT(KT_M_PRE, s, 48.0 * R * nc[1], (k_mx_pre<1, false, R><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 32.0 * R * nc[1], (k_mx_sgs<1, false, R><<<gs[1], tb[1], lds[1], s>>>(a)));
-  KT(KT_M_PRE, s, 48.0 * R * nc[0], (k_mx_pre<0, false, R><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[0], (k_mx_sgs<0, false, R><<<gs[0], tb[0], lds[0], s>>>(a)));
+  KT(KT_M_PRE, s, 16.0 * R * nc[0], (k_mx_pre<0, true, R><<<gq[0], 256, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[0], (k_mx_sgs<0, true><<<gs[0], tb[0], lds[0], s>>>(a)));
+  KT(KT_M_PRE, s, 24.0 * R * nc[1], (k_mx_pre<1, true, R><<<gq[1], 256, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[1], (k_mx_sgs<1, true><<<gs[1], tb[1], lds[1], s>>>(a)));
+  KT(KT_M_PRE, s, 32.0 * R * nc[2], (k_mx_pre<2, true, R><<<gq[2], 256, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[2], (k_mx_sgs<2, true><<<gs[2], tb[2], lds[2], s>>>(a)));
+  KT(KT_M_PRE, s, 40.0 * R * nc[2], (k_mx_pre<2, false, R><<<gq[2], 256, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 32.0 * R * nc[2], (k_mx_sgs<2, false><<<gs[2], tb[2], lds[2], s>>>(a)));
+  KT(KT_M_PRE, s, 40.0 * R * nc[1], (k_mx_pre<1, false, R><<<gq[1], 256, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 32.0 * R * nc[1], (k_mx_sgs<1, false><<<gs[1], tb[1], lds[1], s>>>(a)));
+  KT(KT_M_PRE, s, 40.0 * R * nc[0], (k_mx_pre<0, false, R><<<gq[0], 256, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[0], (k_mx_sgs<0, false><<<gs[0], tb[0], lds[0], s>>>(a)));
 }
 
 static void mx_lusgs_dispatch(const csem_model *m, int R, const MxLu &a, bool diag, bool sweep, hipStream_t s)
@@ -750,10 +749,8 @@ extern "C" int csem_mixed_set_tile(csem_model *m, int R, int rows, int planes)
 extern "C" long csem_mixed_slot_bytes(const csem_model *m, int R)
 {
   const long n = m->lE, c2b = sizeof(c2), zb = sizeof(z2);
-  long mx = 0;
-  for (int c = 0; c < 3; c++) mx = std::max(mx, (long)m->g.ns[c] * m->g.A[c] * m->g.B[c]);
   const long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
-  long b = (m->fE * R + mx * R + m->nsk * R + 2L * SGS_MAXT * MX_RMAX + 9L * R * n) * c2b + (long)R * nch * 4 * 8 +
+  long b = (m->fE * R + 2L * m->nsk * R + 2L * SGS_MAXT * MX_RMAX + 9L * R * n) * c2b + (long)R * nch * 4 * 8 +
            (long)R * MS_N * 8 + (R * n + n) * zb + (long)R * nch * 8 + R * 8 + R * 4;
   if (m->airbc) {
     const long NN = (long)m->nfft * m->nfft, p1 = (long)m->n1 * m->n2, p2 = (long)(m->n1 + 1) * m->n2,

@@ -352,18 +352,24 @@ __global__ void k_mx_air_ghost(MxAir a)
 
 // ----------------------------------------------------------------- LU-SGS
 // The SGS sweeps of lusgs_apply (fdfd.c:287-510) in FP32 for R right-hand
-// sides, with the same slices, diagonal wavefront and skewed per-cell data
-// as the FP64 kernels.  Written as a linear combination, the cell update of
-// fdfd.c:357-369 (Ey, Ez alike) is
+// sides, with the same slices, diagonal wavefront and skewed per-cell
+// layout as the FP64 kernels.  Written as a linear combination, the cell
+// update of fdfd.c:357-369 (Ey, Ez alike) is
 //   v = h*(beta*b - S + cAp*E(A+) + cAm*E(A-) + cBp*E(B+) + cBm*E(B-)),
 // with h = iwmu/aii, beta = i w mu0 (1/vol when fvm), S = (P1 - P2)/dA +
 // (Q3 - Q4)/dB the other components' terms, and 1-D coefficients
 // c*p = 1/(ds[u] d[u]), c*m = 1/(ds[u-1] d[u]) of the component's own
 // axes A and B (aii = (cAp+cAm+cBp+cBm)*iwmu - sigma).  The pre-pass folds
-// everything but the new neighbours into T (forward: the + neighbours are
-// 0; backward: the - neighbours hold forward values), so the wavefront
-// computes v = T + h*(cf*prev + cs*nbv): cf the fast axis' coefficient,
-// cs the slow axis', prev the row's previous cell, nbv the neighbour row.
+// everything but the new neighbours into T = beta*b - S + own (forward: the
+// + neighbours are 0; backward: the - neighbours hold forward values), so
+// the wavefront computes v = h*(T + cf*prev + cs*nbv): cf the fast axis'
+// coefficient, cs the slow axis', prev the row's previous cell, nbv the
+// neighbour row.
+//
+// Per-cell records (T, h) are stored per right-hand side (r-major) in the
+// skewed order, one float4 at r*nsk + base[C] + q*A*B + p: h (z, w) is
+// refreshed when the frequencies change, T (x, y) by each pre-pass, and a
+// sweep step loads both with one 16-byte load.
 
 struct MxLu {
   MxGeo g;
@@ -372,8 +378,8 @@ struct MxLu {
   c2 *y;         // R x n results
   c2 *W;         // R x fE state on the full grid (Ex | Ey | Ez)
   long fE, fEx, fEy;
-  c2 *T;         // records of the current sweep, skewed, R interleaved
-  c2 *h;         // h = iwmu/aii per swept cell, skewed Ex | Ey | Ez, R interleaved
+  float4 *th;    // (T, h = iwmu/aii) per swept cell, skewed Ex | Ey | Ez, per r
+  long nsk;      // swept cells of all components (records per r)
   c2 *trash;     // stores of lanes off the diagonal, 2 * SGS_MAXT * MX_RMAX
   const float *c1m, *c1p, *c2m, *c2p, *c3m, *c3p;  // own-axis coefficients (z on the full grid)
   MxFreq fr;
@@ -399,90 +405,143 @@ __global__ __launch_bounds__(256) void k_mx_diag(MxLu a)
   if (C == 0) { c = a.c2p[j] + a.c2m[j] + a.c3p[k] + a.c3m[k]; sig = a.g.s11[id1(n1, n2, k, j, i)]; }
   else if (C == 1) { c = a.c3p[k] + a.c3m[k] + a.c1p[i] + a.c1m[i]; sig = a.g.s22[id2(n1, n2, k, j, i)]; }
   else { c = a.c1p[i] + a.c1m[i] + a.c2p[j] + a.c2m[j]; sig = a.g.s33[id3(n1, n2, k, j, i)]; }
-  const float vol = a.g.d1s[i] * a.g.d2[j] * a.g.d3[k];
-  const long o = (a.sk.base[C] + (long)q * A * B + p) * R;
+  const long o = a.sk.base[C] + (long)q * A * B + p;
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const c2 iw = a.fr.iwmu[r];
     // h = iwmu/aii (the fvm factor scales aij and aii alike and cancels)
     const c2 aii = cmk(c * iw.x - sig, c * iw.y);
     const float den = 1.f / (aii.x * aii.x + aii.y * aii.y);
-    a.h[o + r] = cmk((iw.x * aii.x + iw.y * aii.y) * den, (iw.y * aii.x - iw.x * aii.y) * den);
+    ((c2 *)a.th)[2 * (r * a.nsk + o) + 1] =
+        cmk((iw.x * aii.x + iw.y * aii.y) * den, (iw.y * aii.x - iw.x * aii.y) * den);
   }
-  (void)vol;
 }
+
+// Pre-pass tiles: TQ slices x TS rows x TF columns of a component's slices,
+// read in the natural order (x fastest, so the W and b reads are whole
+// rows) and written in the skewed order through LDS, so each diagonal's
+// run of the tile is one contiguous store.  Ex: slices run along x, so a
+// tile spans 16 slices; Ey, Ez: a tile is 16 rows x 64 columns of one slice.
+template <int C> struct MxPreTile { static constexpr int TQ = 1, TS = 16, TF = 64; };
+template <> struct MxPreTile<0> { static constexpr int TQ = 16, TS = 8, TF = 8; };
 
 // W value of component C at (k, j, i) for right-hand side r.
 #define MW0(r, k, j, i) a.W[(r) * a.fE + id1(n1, n2, k, j, i)]
 #define MW1(r, k, j, i) a.W[(r) * a.fE + a.fEx + id2(n1, n2, k, j, i)]
 #define MW2(r, k, j, i) a.W[(r) * a.fE + a.fEx + a.fEy + id3(n1, n2, k, j, i)]
 
-// Pre-pass of sweep (C, FWD) for every r: T at the skewed position.
+// Pre-pass of sweep (C, FWD) for every r: T of a tile of cells.
 template <int C, bool FWD, int R>
 __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
 {
+  using TL = MxPreTile<C>;
+  constexpr int TQ = TL::TQ, TS = TL::TS, TF = TL::TF, TFP = TF + 1, NC = TQ * TS * TF;
+  __shared__ c2 tile[R][TQ * TS * TFP];
   const int n1 = a.g.n1, n2 = a.g.n2, n3 = a.g.n3f;
-  const int nx = (C == 0) ? n1 : n1 + 1, ny = (C == 1) ? n2 : n2 + 1;
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= nx || j >= ny) return;
-  int q, s, f;
-  if (C == 0) { q = i; s = k; f = j; if (!(k >= 1 && k <= n3 - 1 && j >= 1 && j <= n2 - 1)) return; }
-  else if (C == 1) { q = j; s = k; f = i; if (!(k >= 1 && k <= n3 - 1 && i >= 1 && i <= n1 - 1)) return; }
-  else { q = k; s = j; f = i; if (!(j >= 1 && j <= n2 - 1 && i >= 1 && i <= n1 - 1)) return; }
-  const int A = a.sk.A[C], B = a.sk.B[C], d = s + f;
-  const long p = (long)q * A * B + sk_off(d, A, B) + (s - max(1, d - B));
-  const int kop = k - a.g.kofs;
-  const long xo = (C == 0) ? id1(n1, n2, kop, j, i) : (C == 1) ? a.g.lEx + id2(n1, n2, kop, j, i)
-                                                                 : a.g.lEx + a.g.lEy + id3(n1, n2, kop, j, i);
+  const int A = a.sk.A[C], B = a.sk.B[C], ns = a.sk.ns[C];
+  const int t = threadIdx.x;
+  int q0, s0, f0;
+  if (C == 0) { q0 = blockIdx.x * TQ; f0 = 1 + blockIdx.y * TF; s0 = 1 + blockIdx.z * TS; }
+  else { f0 = 1 + blockIdx.x * TF; s0 = 1 + blockIdx.y * TS; q0 = blockIdx.z; }
   const float *r1 = a.g.r1, *r2 = a.g.r2, *r3 = a.g.r3, *r1s = a.g.r1s, *r2s = a.g.r2s, *r3s = a.g.r3s;
-  const float vol = a.g.d1s[i] * a.g.d2[j] * a.g.d3[k];
   const c2 z0 = cmk(0, 0);
+  // compute: lanes along x
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    c2 S = z0, own = z0;
-    if (C == 0) {
-      if (!FWD) {  // Ey, Ez backward
-        const c2 P1 = csub(MW1(r, k, j, i + 1), MW1(r, k, j, i)), P2 = csub(MW1(r, k, j - 1, i + 1), MW1(r, k, j - 1, i));
-        const c2 Q3 = csub(MW2(r, k, j, i + 1), MW2(r, k, j, i)), Q4 = csub(MW2(r, k - 1, j, i + 1), MW2(r, k - 1, j, i));
-        S = crmul(r1s[i], cadd(crmul(r2[j], csub(P1, P2)), crmul(r3[k], csub(Q3, Q4))));
-        own = crfma(crmul(a.c2m[j], MW0(r, k, j - 1, i)), a.c3m[k], MW0(r, k - 1, j, i));
+  for (int it = 0; it < NC / 256; it++) {
+    int lq, ls, lf;
+    if (C == 0) { lq = t % TQ; lf = (t / TQ) % TF; ls = t / (TQ * TF) + it * (256 / (TQ * TF)); }
+    else { lq = 0; lf = t % TF; ls = t / TF + it * (256 / TF); }
+    const int q = q0 + lq, s = s0 + ls, f = f0 + lf;
+    if (q >= ns || s > A || f > B) continue;
+    int k, j, i;
+    lu_kji<C>(q, s, f, k, j, i);
+    const int kop = k - a.g.kofs;
+    const long xo = (C == 0) ? id1(n1, n2, kop, j, i) : (C == 1) ? a.g.lEx + id2(n1, n2, kop, j, i)
+                                                                   : a.g.lEx + a.g.lEy + id3(n1, n2, kop, j, i);
+    const float vol = a.g.d1s[i] * a.g.d2[j] * a.g.d3[k];
+    const int cell = (lq * TS + ls) * TFP + lf;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      c2 S = z0, own = z0;
+      if (C == 0) {
+        if (!FWD) {  // Ey, Ez backward
+          const c2 P1 = csub(MW1(r, k, j, i + 1), MW1(r, k, j, i)), P2 = csub(MW1(r, k, j - 1, i + 1), MW1(r, k, j - 1, i));
+          const c2 Q3 = csub(MW2(r, k, j, i + 1), MW2(r, k, j, i)), Q4 = csub(MW2(r, k - 1, j, i + 1), MW2(r, k - 1, j, i));
+          S = crmul(r1s[i], cadd(crmul(r2[j], csub(P1, P2)), crmul(r3[k], csub(Q3, Q4))));
+          own = crfma(crmul(a.c2m[j], MW0(r, k, j - 1, i)), a.c3m[k], MW0(r, k - 1, j, i));
+        }
+      } else if (C == 1) {
+        const c2 Q3 = csub(MW0(r, k, j + 1, i), MW0(r, k, j, i)), Q4 = csub(MW0(r, k, j + 1, i - 1), MW0(r, k, j, i - 1));
+        c2 PP = z0;
+        if (!FWD) PP = csub(csub(MW2(r, k, j + 1, i), MW2(r, k, j, i)), csub(MW2(r, k - 1, j + 1, i), MW2(r, k - 1, j, i)));
+        S = crmul(r2s[j], cadd(crmul(r3[k], PP), crmul(r1[i], csub(Q3, Q4))));
+        if (!FWD) own = crfma(crmul(a.c3m[k], MW1(r, k - 1, j, i)), a.c1m[i], MW1(r, k, j, i - 1));
+      } else {
+        const c2 P1 = csub(MW0(r, k + 1, j, i), MW0(r, k, j, i)), P2 = csub(MW0(r, k + 1, j, i - 1), MW0(r, k, j, i - 1));
+        const c2 Q3 = csub(MW1(r, k + 1, j, i), MW1(r, k, j, i)), Q4 = csub(MW1(r, k + 1, j - 1, i), MW1(r, k, j - 1, i));
+        S = crmul(r3s[k], cadd(crmul(r1[i], csub(P1, P2)), crmul(r2[j], csub(Q3, Q4))));
+        if (!FWD) own = crfma(crmul(a.c1m[i], MW2(r, k, j, i - 1)), a.c2m[j], MW2(r, k, j - 1, i));
       }
-    } else if (C == 1) {
-      const c2 Q3 = csub(MW0(r, k, j + 1, i), MW0(r, k, j, i)), Q4 = csub(MW0(r, k, j + 1, i - 1), MW0(r, k, j, i - 1));
-      c2 PP = z0;
-      if (!FWD) PP = csub(csub(MW2(r, k, j + 1, i), MW2(r, k, j, i)), csub(MW2(r, k - 1, j + 1, i), MW2(r, k - 1, j, i)));
-      S = crmul(r2s[j], cadd(crmul(r3[k], PP), crmul(r1[i], csub(Q3, Q4))));
-      if (!FWD) own = crfma(crmul(a.c3m[k], MW1(r, k - 1, j, i)), a.c1m[i], MW1(r, k, j, i - 1));
-    } else {
-      const c2 P1 = csub(MW0(r, k + 1, j, i), MW0(r, k, j, i)), P2 = csub(MW0(r, k + 1, j, i - 1), MW0(r, k, j, i - 1));
-      const c2 Q3 = csub(MW1(r, k + 1, j, i), MW1(r, k, j, i)), Q4 = csub(MW1(r, k + 1, j - 1, i), MW1(r, k, j - 1, i));
-      S = crmul(r3s[k], cadd(crmul(r1[i], csub(P1, P2)), crmul(r2[j], csub(Q3, Q4))));
-      if (!FWD) own = crfma(crmul(a.c1m[i], MW2(r, k, j, i - 1)), a.c2m[j], MW2(r, k, j - 1, i));
+      c2 bv = z0;
+      if (kop >= 0) bv = a.b[r * a.g.n + xo];
+      const c2 bb = a.g.fvm ? crmul(1.f / vol, bv) : cmul(a.fr.iomu[r], bv);
+      tile[r][cell] = cadd(csub(bb, S), own);
     }
-    c2 bv = z0;
-    if (kop >= 0) bv = a.b[r * a.g.n + xo];
-    const c2 bb = a.g.fvm ? crmul(1.f / vol, bv) : cmul(a.fr.iomu[r], bv);
-    const c2 hh = a.h[(a.sk.base[C] + p) * R + r];
-    a.T[p * R + r] = cmul(hh, cadd(csub(bb, S), own));
+  }
+  __syncthreads();
+  // store: the tile's cells in the skewed order (per slice, by diagonal,
+  // rows contiguous within a diagonal)
+  const long AB = (long)A * B;
+#pragma unroll
+  for (int it = 0; it < NC / 256; it++) {
+    const int e = t + it * 256, lq = e / (TS * TF), el = e % (TS * TF);
+    int lo = 2, hi = TS + TF;  // local diagonal (1-based rows and columns)
+#pragma unroll
+    for (int u = 0; u < 7; u++) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (lo < hi) { if (sk_off(mid, TS, TF) <= el) lo = mid; else hi = mid - 1; }
+    }
+    const int ls = max(1, lo - TF) + (el - sk_off(lo, TS, TF)) - 1, lf = lo - 2 - ls;
+    const int q = q0 + lq, s = s0 + ls, f = f0 + lf;
+    if (q >= ns || s > A || f > B) continue;
+    const int d = s + f;
+    const long p = a.sk.base[C] + q * AB + sk_off(d, A, B) + (s - max(1, d - B));
+    const int cell = (lq * TS + ls) * TFP + lf;
+#pragma unroll
+    for (int r = 0; r < R; r++) ((c2 *)a.th)[2 * (r * a.nsk + p)] = tile[r][cell];
   }
 }
 #undef MW0
 #undef MW1
 #undef MW2
 
-template <int C, bool FWD, int R>
+// Sweep (C, FWD) of one slice for one right-hand side: block (slice,
+// r).  The records of the diagonals ahead stream into a register ring
+// MXS_P deep, so MXS_P diagonals of loads are in flight while the wavefront
+// advances (one load and at most two stores per diagonal keep that within
+// the 63 memory operations a wave can have outstanding).  The compiler
+// waits for every outstanding load at the loop's back edge, so the loop
+// runs MXS_U diagonals per iteration.
+#ifndef MXS_P
+#define MXS_P 16
+#endif
+#ifndef MXS_U
+#define MXS_U 64
+#endif
+template <int C, bool FWD>
 __global__ __launch_bounds__(SGS_MAXT) void k_mx_sgs(MxLu a)
 {
   extern __shared__ c2 xs[];
   const int nslice = a.sk.ns[C];
   const int per = (nslice + 7) / 8;
   const int q = (blockIdx.x % 8) * per + blockIdx.x / 8;  // consecutive slices on one XCD
+  const int r = blockIdx.y;
   if (q >= nslice) return;
   const int n1 = a.g.n1, n2 = a.g.n2;
   const int t = threadIdx.x, nt = blockDim.x;
   const int A = a.sk.A[C], B = a.sk.B[C], AB = A * B;
   const int s = 1 + t, sc = min(s, A);
-  const int XW = (nt + 2) * R;  // exchange row: R values per row, zero rows at both ends
+  const int XW = nt + 2;  // exchange row: zero entries at both ends
   float *cf = (float *)(xs + 2 * XW);
   int *dtab = (int *)(cf + (B + 2));
   {
@@ -493,8 +552,7 @@ __global__ __launch_bounds__(SGS_MAXT) void k_mx_sgs(MxLu a)
   }
   const float *gs = (C == 2) ? (FWD ? a.c2m : a.c2p) : (FWD ? a.c3m : a.c3p);
   const float cs = gs[sc];
-  const c2 *Tc = a.T + (long)q * AB * R;
-  const c2 *Hc = a.h + (a.sk.base[C] + (long)q * AB) * R;
+  const float4 *Rc = a.th + r * a.nsk + a.sk.base[C] + (long)q * AB;
   const long PL = (C == 0) ? (long)n1 * (n2 + 1) : (C == 1) ? (long)(n1 + 1) * n2 : (long)(n1 + 1) * (n2 + 1);
   const long SS = (C == 2) ? (long)(n1 + 1) : PL;
   const int FS = (C == 0) ? n1 : 1;
@@ -503,71 +561,63 @@ __global__ __launch_bounds__(SGS_MAXT) void k_mx_sgs(MxLu a)
   const long xofs = (C == 0) ? 0 : (C == 1) ? a.g.lEx : a.g.lEx + a.g.lEy;
   const int kop = ((C == 2) ? q : s) - a.g.kofs;
   const long yb = xofs + nb - (long)a.g.kofs * PL;
+  c2 *Wr = a.W + r * a.fE + wofs + nb;
+  c2 *yr = a.y + r * a.g.n + yb;
+  c2 *tw = a.trash + r * SGS_MAXT + t, *ty = a.trash + (MX_RMAX + r) * SGS_MAXT + t;
   const int dlo = 2, dhi = A + B, nst = dhi - dlo + 1;
-  const c2 z0 = cmk(0, 0);
   __syncthreads();
-  struct St { c2 T[R], H[R]; };
-  auto load = [&](St &c, int st) {
+  auto load = [&](float4 &c, int st) {
     const int d = min(FWD ? dlo + st : dhi - st, dhi);
-    const long e = (long)min(max(dtab[max(d, dlo)] + sc, 0), AB - 1) * R;
-#pragma unroll
-    for (int r = 0; r < R; r++) { c.T[r] = Tc[e + r]; c.H[r] = Hc[e + r]; }
+    const long e = (long)min(max(dtab[max(d, dlo)] + sc, 0), AB - 1);
+    c = Rc[e];
   };
-  St ring[SGS_P];
+  float4 ring[MXS_P];
 #pragma unroll
-  for (int u = 0; u < SGS_P; u++) load(ring[u], u);
-  c2 prev[R];
+  for (int u = 0; u < MXS_P; u++) load(ring[u], u);
+  c2 prev = cmk(0, 0);
+  const int nstp = ((nst + MXS_U - 1) / MXS_U) * MXS_U;
+  for (int st0 = 0; st0 < nstp; st0 += MXS_U) {
 #pragma unroll
-  for (int r = 0; r < R; r++) prev[r] = z0;
-  const int nstp = ((nst + SGS_U - 1) / SGS_U) * SGS_U;
-  for (int st0 = 0; st0 < nstp; st0 += SGS_U) {
-#pragma unroll
-    for (int uu = 0; uu < SGS_U; uu++) {
-      const int st = st0 + uu, u = uu % SGS_P;
+    for (int uu = 0; uu < MXS_U; uu++) {
+      const int st = st0 + uu, u = uu % MXS_P;
       const int d = FWD ? dlo + st : dhi - st;
       const int f = d - s;
       const float cfv = cf[min(max(f, 0), B + 1)];
       const bool ok = s <= A && f >= 1 && f <= B;
       const long o = (long)f * FS;
-      const c2 *xr = xs + ((st + 1) & 1) * XW + (FWD ? t : t + 2) * R;
-#pragma unroll
-      for (int r = 0; r < R; r++) {
-        const c2 nbv = xr[r];
-        const c2 w = crfma(crmul(cfv, prev[r]), cs, nbv);
-        const c2 v = cfma(ring[u].T[r], ring[u].H[r], w);
-        if (FWD || C != 0) *(ok ? a.W + r * a.fE + wofs + nb + o : a.trash + r * SGS_MAXT + t) = v;
-        if (!FWD) *(ok && kop >= 0 ? a.y + r * a.g.n + yb + o : a.trash + (MX_RMAX + r) * SGS_MAXT + t) = v;
-        prev[r].x = ok ? v.x : prev[r].x;
-        prev[r].y = ok ? v.y : prev[r].y;
-      }
-      load(ring[u], st + SGS_P);
-#pragma unroll
-      for (int r = 0; r < R; r++) xs[(st & 1) * XW + (t + 1) * R + r] = prev[r];
+      const c2 nbv = xs[((st + 1) & 1) * XW + (FWD ? t : t + 2)];
+      const c2 w = cadd(cmk(ring[u].x, ring[u].y), crfma(crmul(cfv, prev), cs, nbv));
+      const c2 v = cmul(cmk(ring[u].z, ring[u].w), w);
+      if (FWD || C != 0) *(ok ? Wr + o : tw) = v;
+      if (!FWD) *(ok && kop >= 0 ? yr + o : ty) = v;
+      prev.x = ok ? v.x : prev.x;
+      prev.y = ok ? v.y : prev.y;
+      load(ring[u], st + MXS_P);
+      xs[(st & 1) * XW + t + 1] = prev;
       __syncthreads();
     }
   }
   if (!FWD) {  // operator-grid cells of this slice outside the sweep range are zero rows
     const int n3 = a.g.n3f;
-    for (int r = 0; r < R; r++) {
-      c2 *yr = a.y + r * a.g.n;
-      if (C == 0) {
-        const int nk = a.g.n3 + 1;
-        for (int w = t; w < nk * (n2 + 1); w += nt) {
-          const int kq = w / (n2 + 1), j = w % (n2 + 1), kf = kq + a.g.kofs;
-          if (!(kf >= 1 && kf <= n3 - 1 && j >= 1 && j <= n2 - 1)) yr[id1(n1, n2, kq, j, q)] = z0;
-        }
-      } else if (C == 1) {
-        const int nk = a.g.n3 + 1;
-        for (int w = t; w < nk * (n1 + 1); w += nt) {
-          const int kq = w / (n1 + 1), i = w % (n1 + 1), kf = kq + a.g.kofs;
-          if (!(kf >= 1 && kf <= n3 - 1 && i >= 1 && i <= n1 - 1)) yr[a.g.lEx + id2(n1, n2, kq, q, i)] = z0;
-        }
-      } else if (q - a.g.kofs >= 0) {
-        for (int w = t; w < (n2 + 1) * (n1 + 1); w += nt) {
-          const int jj = w / (n1 + 1), i = w % (n1 + 1);
-          if (!(jj >= 1 && jj <= n2 - 1 && i >= 1 && i <= n1 - 1))
-            yr[a.g.lEx + a.g.lEy + id3(n1, n2, q - a.g.kofs, jj, i)] = z0;
-        }
+    const c2 z0 = cmk(0, 0);
+    c2 *y0 = a.y + r * a.g.n;
+    if (C == 0) {
+      const int nk = a.g.n3 + 1;
+      for (int w = t; w < nk * (n2 + 1); w += nt) {
+        const int kq = w / (n2 + 1), j = w % (n2 + 1), kf = kq + a.g.kofs;
+        if (!(kf >= 1 && kf <= n3 - 1 && j >= 1 && j <= n2 - 1)) y0[id1(n1, n2, kq, j, q)] = z0;
+      }
+    } else if (C == 1) {
+      const int nk = a.g.n3 + 1;
+      for (int w = t; w < nk * (n1 + 1); w += nt) {
+        const int kq = w / (n1 + 1), i = w % (n1 + 1), kf = kq + a.g.kofs;
+        if (!(kf >= 1 && kf <= n3 - 1 && i >= 1 && i <= n1 - 1)) y0[a.g.lEx + id2(n1, n2, kq, q, i)] = z0;
+      }
+    } else if (q - a.g.kofs >= 0) {
+      for (int w = t; w < (n2 + 1) * (n1 + 1); w += nt) {
+        const int jj = w / (n1 + 1), i = w % (n1 + 1);
+        if (!(jj >= 1 && jj <= n2 - 1 && i >= 1 && i <= n1 - 1))
+          y0[a.g.lEx + a.g.lEy + id3(n1, n2, q - a.g.kofs, jj, i)] = z0;
       }
     }
   }
