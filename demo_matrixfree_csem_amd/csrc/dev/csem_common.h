@@ -54,6 +54,26 @@ __host__ __device__ __forceinline__ double ddiv(double x, double d, double rd)
 }
 __host__ __device__ __forceinline__ z2 zdivq(z2 a, double d, double rd) { return zmk(ddiv(a.x, d, rd), ddiv(a.y, d, rd)); }
 
+// Whole-wavefront lane shifts (DPP wave_shl:1 / wave_shr:1; a lane with no
+// source lane reads 0).  Every lane of the wave must be active.
+#define DPP_WAVE_SHL1 0x130  // lane l reads lane l+1
+#define DPP_WAVE_SHR1 0x138  // lane l reads lane l-1
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, true);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, true);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int CTRL>
+__device__ __forceinline__ z2 zdpp(z2 v) { return zmk(dpp_d<CTRL>(v.x), dpp_d<CTRL>(v.y)); }
+
 // Single precision counterparts for the mixed-precision inner solver.  A
 // plain aligned struct rather than HIP's float2 (a union-based vector
 // class): selects and conditional copies of it stay in registers.
@@ -61,6 +81,8 @@ struct __align__(8) c2 {
   float x, y;
 };
 __host__ __device__ __forceinline__ c2 cmk(float x, float y) { return c2{x, y}; }
+template <int CTRL>
+__device__ __forceinline__ c2 cdpp(c2 v) { return c2{dpp_f<CTRL>(v.x), dpp_f<CTRL>(v.y)}; }
 __host__ __device__ __forceinline__ c2 cadd(c2 a, c2 b) { return cmk(a.x + b.x, a.y + b.y); }
 __host__ __device__ __forceinline__ c2 csub(c2 a, c2 b) { return cmk(a.x - b.x, a.y - b.y); }
 __host__ __device__ __forceinline__ c2 cmul(c2 a, c2 b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }

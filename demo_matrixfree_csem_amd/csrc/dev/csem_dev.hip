@@ -292,7 +292,8 @@ static ApplyArgs apply_args(const csem_model *m, const csem_freq *f, const z2 *x
   a.hxg = m->hxg; a.hyg = m->hyg;
   a.iwmu = zmk(f->iwmu_re, f->iwmu_im);
   a.iomu = zmk(f->iomu_re, f->iomu_im);
-  a.n1 = n1; a.n2 = n2; a.n3 = m->n3; a.fvm = m->fvm; a.airbc = m->airbc; a.kc = m->kc;
+  a.n1 = n1; a.n2 = n2; a.n3 = m->n3; a.fvm = m->fvm; a.airbc = m->airbc;
+  a.nti = (n1 + 1 + AP_OI - 1) / AP_OI; a.ntj = (n2 + 1 + AP_OJ - 1) / AP_OJ; a.ntk = (m->n3 + 1 + AP_KC - 1) / AP_KC;
   a.lEx = m->lEx; a.lEy = m->lEy;
   return a;
 }
@@ -345,9 +346,9 @@ int run_apply_bufs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStr
     int rc = run_airbc(m, s, b);
     if (rc) return rc;
   }
-  dim3 blk(64, AP_TJ + 1);
-  dim3 grd((m->n1 + 1 + AP_TI - 1) / AP_TI, (m->n2 + 1 + AP_TJ - 1) / AP_TJ, (m->n3 + 1 + m->kc - 1) / m->kc);
-  KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<AP_TJ><<<grd, blk, 0, s>>>(a)));
+  const int ntiles = a.nti * a.ntj * a.ntk;
+  dim3 blk(64, AP_NW);
+  KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<<<8 * ((ntiles + 7) / 8), blk, 0, s>>>(a)));
   HIP_OK(hipGetLastError());
   return 0;
 }

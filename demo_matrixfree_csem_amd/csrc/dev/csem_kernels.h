@@ -16,34 +16,48 @@ struct ApplyArgs {
   const double *r1, *r2, *r3, *r1s, *r2s, *r3s;  // RN(1/d) of the same arrays
   const z2 *hxg, *hyg;  // ghost planes under the airwave boundary
   z2 iwmu, iomu;
-  int n1, n2, n3, fvm, airbc, kc;
+  int n1, n2, n3, fvm, airbc;
+  int nti, ntj, ntk;  // operator tiles along i, j, k
   long lEx, lEy;
 };
 
-// y = A x (fdfd.c:86-205).  A block is AP_TJ+1 waves: wave r covers row
-// j = j0-1+r, lane l column i = i0-1+l, so wave 0 and lane 0 are a halo
-// that computes H for its neighbours and writes no output.  The block
-// marches over kc planes; H = curl E/(i w mu0) on the faces at (k, j, i)
-// (fdfd.c:133-145) is evaluated once per face, published in LDS (double
-// buffered, one barrier per plane) for the j+1 and i+1 neighbours, and
-// carried in registers to plane k+1.  Every division is by a 1-D spacing
-// and uses its precomputed reciprocal with one exact correction (ddiv), so
-// each value rounds as the reference's division does.
-#define AP_TJ 8
-#define AP_TI 63  // output columns per block (64 lanes, one halo)
-template <int TJ>
-__global__ __launch_bounds__(64 * (TJ + 1)) void k_apply(ApplyArgs a)
+// y = A x (fdfd.c:86-205).  A block is AP_NW waves over a tile of rows
+// and columns and marches over AP_KC planes: wave w covers row j = j0-1+w,
+// lane l column i = i0-1+l.  Waves 0 and AP_NW-1 and lanes 0 and 63 are
+// halos: they load E and compute H for their neighbours but write no
+// output.  Per plane each lane loads its own E two planes ahead (and sigma
+// one plane ahead), so every E value is read once per tile; the j+1
+// neighbours of E and the j-1 neighbours of H come through LDS, the i+1
+// and i-1 neighbours through whole-wave lane shifts.  H = curl E/(i w mu0)
+// on the faces at (k, j, i) (fdfd.c:133-145) is evaluated once per face and
+// carried to plane k+1 in registers.  The k-march is fully unrolled (a loop
+// back edge would make the compiler wait for the prefetched loads).  Tiles
+// are numbered columns fastest, then rows, then planes, and consecutive
+// tiles run on one XCD, so halo rows and planes shared by neighbouring
+// tiles are L2 hits.  Every division is by a 1-D spacing and uses its
+// precomputed reciprocal with one exact correction (ddiv), so each value
+// rounds as the reference's division does.
+#define AP_NW 8   // waves per block
+#define AP_KC 8   // planes per block
+#define AP_OI 62  // output columns per block
+#define AP_OJ (AP_NW - 2)  // output rows per block
+#ifndef CSEM_TYPES_ONLY
+__global__ __launch_bounds__(64 * AP_NW) void k_apply(ApplyArgs a)
 {
-  __shared__ z2 shx[2][TJ + 1][64], shy[2][TJ + 1][64], shz[2][TJ + 1][64];
-  const int l = threadIdx.x, r = threadIdx.y;
-  const int i = blockIdx.x * AP_TI - 1 + l;
-  const int j = blockIdx.y * TJ - 1 + r;
+  __shared__ z2 sEx[AP_NW][64], sEz[AP_NW][64], sHx[AP_NW][64], sHz[AP_NW][64];
+  __shared__ double sD[AP_KC][4];  // d3s, 1/d3s, d3, 1/d3 of the tile's planes
+  const int ntiles = a.nti * a.ntj * a.ntk, per = (ntiles + 7) / 8;
+  const int T = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (T >= ntiles) return;
+  const int ti = T % a.nti, tj = (T / a.nti) % a.ntj, tk = T / (a.nti * a.ntj);
+  const int l = threadIdx.x, w = threadIdx.y;
+  const int i = ti * AP_OI - 1 + l, j = tj * AP_OJ - 1 + w;
   const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
-  const int k0 = blockIdx.z * a.kc;
-  const int k1 = min(k0 + a.kc, n3 + 1);
+  const int k0 = tk * AP_KC, k1 = min(k0 + AP_KC, n3 + 1);
   const bool live = i >= 0 && i <= n1 && j >= 0 && j <= n2;
-  const bool out = live && l > 0 && r > 0;
+  const bool out = live && l >= 1 && l <= AP_OI && w >= 1 && w <= AP_OJ;
   const bool hx_ok = live && j < n2, hy_ok = live && i < n1, hz_ok = hy_ok && j < n2;
+  const bool iin = i >= 1 && i <= n1 - 1, jin = j >= 1 && j <= n2 - 1;
   const z2 *Ex = a.x, *Ey = a.x + a.lEx, *Ez = a.x + a.lEx + a.lEy;
   z2 *bx = a.y, *by = a.y + a.lEx, *bz = a.y + a.lEx + a.lEy;
   const z2 iwmu = a.iwmu, z0 = zmk(0, 0);
@@ -51,85 +65,128 @@ __global__ __launch_bounds__(64 * (TJ + 1)) void k_apply(ApplyArgs a)
   const int ic = min(max(i, 0), n1), jc = min(max(j, 0), n2);
   const double d1s_i = a.d1s[ic], r1s_i = a.r1s[ic], d1_i = a.d1[ic], r1_i = a.r1[ic];
   const double d2s_j = a.d2s[jc], r2s_j = a.r2s[jc], d2_j = a.d2[jc], r2_j = a.r2[jc];
-  // plane strides of the three layouts
+  // plane strides and (clamped) in-plane offsets of the three layouts; for
+  // every lane that stores, the clamped offset is its own
   const long px = (long)n1 * (n2 + 1), py = (long)(n1 + 1) * n2, pz = (long)(n1 + 1) * (n2 + 1);
-  const long ox = i + (long)n1 * j, oy = i + (long)(n1 + 1) * j, oz = i + (long)(n1 + 1) * j;
-
-  auto HX = [&](int k) {  // (Ez(j+1)-Ez(j))/d2s[j] - (Ey(k+1)-Ey(k))/d3s[k], times iwmu
-    const z2 *ez = Ez + k * pz + oz, *ey = Ey + k * py + oy;
-    return zmul(zsub(zdivq(zsub(ez[n1 + 1], ez[0]), d2s_j, r2s_j),
-                     zdivq(zsub(ey[py], ey[0]), a.d3s[k], a.r3s[k])), iwmu);
+  const long ox = min(ic, n1 - 1) + (long)n1 * jc, oy = ic + (long)(n1 + 1) * min(jc, n2 - 1),
+             oz = ic + (long)(n1 + 1) * jc;
+  struct EV { z2 x, y, z; };
+  struct SV { double x, y, z; };
+  auto ldE = [&](int k) {
+    const long kc = max(k, 0);
+    EV e;
+    e.x = Ex[min(kc, (long)n3) * px + ox];
+    e.y = Ey[min(kc, (long)n3) * py + oy];
+    e.z = Ez[min(kc, (long)n3 - 1) * pz + oz];
+    return e;
   };
-  auto HY = [&](int k) {  // (Ex(k+1)-Ex(k))/d3s[k] - (Ez(i+1)-Ez(i))/d1s[i], times iwmu
-    const z2 *ex = Ex + k * px + ox, *ez = Ez + k * pz + oz;
-    return zmul(zsub(zdivq(zsub(ex[px], ex[0]), a.d3s[k], a.r3s[k]),
-                     zdivq(zsub(ez[1], ez[0]), d1s_i, r1s_i)), iwmu);
+  auto ldS = [&](int k) {
+    SV v;
+    v.x = a.s11[min(k, n3) * px + ox];
+    v.y = a.s22[min(k, n3) * py + oy];
+    v.z = a.s33[min(k, n3 - 1) * pz + oz];
+    return v;
   };
-  auto HZ = [&](int k) {  // (Ey(i+1)-Ey(i))/d1s[i] - (Ex(j+1)-Ex(j))/d2s[j], times iwmu
-    const z2 *ey = Ey + k * py + oy, *ex = Ex + k * px + ox;
-    return zmul(zsub(zdivq(zsub(ey[1], ey[0]), d1s_i, r1s_i),
-                     zdivq(zsub(ex[n1], ex[0]), d2s_j, r2s_j)), iwmu);
-  };
-
-  z2 hx_prev = z0, hy_prev = z0;
-  if (k0 > 0 && k0 - 1 < n3) {
-    if (hx_ok) hx_prev = HX(k0 - 1);
-    if (hy_ok) hy_prev = HY(k0 - 1);
+  const int wu = min(w + 1, AP_NW - 1), wd = max(w - 1, 0);
+  // the planes' spacings through LDS: read per plane as uniform values
+  // they would be vector loads that the compiler waits for together with
+  // the prefetched planes
+  if (w == 0 && l < 4 * AP_KC) {
+    const int kq = min(k0 + l / 4, n3), c = l % 4;
+    const double *src = c == 0 ? a.d3s : c == 1 ? a.r3s : c == 2 ? a.d3 : a.r3;
+    sD[l / 4][c] = src[kq];
   }
-  const bool iin = i >= 1 && i <= n1 - 1, jin = j >= 1 && j <= n2 - 1;
-  for (int k = k0; k < k1; k++) {
-    z2 hx = z0, hy = z0, hz = z0;
-    if (k < n3) {
-      if (hx_ok) hx = HX(k);
-      if (hy_ok) hy = HY(k);
+  EV e[3];
+  SV sg[2];
+  e[2] = ldE(k0 - 1);
+  e[0] = ldE(k0);
+  e[1] = ldE(k0 + 1);
+  sg[0] = ldS(k0);
+  // H of the plane below k0; below k = 0 that is the airwave ghost plane
+  z2 hx_prev = z0, hy_prev = z0;
+  if (k0 == 0) {
+    if (a.airbc) {
+      if (hx_ok) hx_prev = a.hxg[ic + (long)(n1 + 1) * jc];
+      if (hy_ok) hy_prev = a.hyg[ic + (long)n1 * jc];
     }
-    if (hz_ok) hz = HZ(k);
-    const int q = k & 1;
-    shx[q][r][l] = hx; shy[q][r][l] = hy; shz[q][r][l] = hz;
+  } else {
+    const int k = k0 - 1;
+    sEz[w][l] = e[2].z;
     __syncthreads();
-    if (out) {
+    const z2 ez_up = sEz[wu][l], ez_r = zdpp<DPP_WAVE_SHL1>(e[2].z);
+    const double d3s_k = a.d3s[k], r3s_k = a.r3s[k];
+    const z2 hx = zmul(zsub(zdivq(zsub(ez_up, e[2].z), d2s_j, r2s_j), zdivq(zsub(e[0].y, e[2].y), d3s_k, r3s_k)), iwmu);
+    const z2 hy = zmul(zsub(zdivq(zsub(e[0].x, e[2].x), d3s_k, r3s_k), zdivq(zsub(ez_r, e[2].z), d1s_i, r1s_i)), iwmu);
+    hx_prev = hx_ok ? hx : z0;
+    hy_prev = hy_ok ? hy : z0;
+    __syncthreads();
+  }
+  sEx[w][l] = e[0].x;
+  sEz[w][l] = e[0].z;
+  __syncthreads();
+#pragma unroll
+  for (int kk = 0; kk < AP_KC; kk++) {
+    const int k = k0 + kk;
+    const EV cur = e[kk % 3], nxt = e[(kk + 1) % 3];
+    e[(kk + 2) % 3] = ldE(k + 2);
+    sg[(kk + 1) % 2] = ldS(k + 1);
+    // H on the faces of plane k
+    const double d3s_k = sD[kk][0], r3s_k = sD[kk][1], d3_k = sD[kk][2], r3_k = sD[kk][3];
+    const z2 ez_up = sEz[wu][l], ex_up = sEx[wu][l];
+    const z2 ey_r = zdpp<DPP_WAVE_SHL1>(cur.y), ez_r = zdpp<DPP_WAVE_SHL1>(cur.z);
+    const z2 hx0 = zmul(zsub(zdivq(zsub(ez_up, cur.z), d2s_j, r2s_j), zdivq(zsub(nxt.y, cur.y), d3s_k, r3s_k)), iwmu);
+    const z2 hy0 = zmul(zsub(zdivq(zsub(nxt.x, cur.x), d3s_k, r3s_k), zdivq(zsub(ez_r, cur.z), d1s_i, r1s_i)), iwmu);
+    const z2 hz0 = zmul(zsub(zdivq(zsub(ey_r, cur.y), d1s_i, r1s_i), zdivq(zsub(ex_up, cur.x), d2s_j, r2s_j)), iwmu);
+    const z2 hx = (hx_ok && k < n3) ? hx0 : z0;
+    const z2 hy = (hy_ok && k < n3) ? hy0 : z0;
+    const z2 hz = hz_ok ? hz0 : z0;
+    __syncthreads();
+    sHx[w][l] = hx;
+    sHz[w][l] = hz;
+    sEx[w][l] = nxt.x;
+    sEz[w][l] = nxt.z;
+    __syncthreads();
+    const z2 hz_dn = sHz[wd][l], hx_dn = sHx[wd][l];
+    const z2 hy_l = zdpp<DPP_WAVE_SHR1>(hy), hz_l = zdpp<DPP_WAVE_SHR1>(hz);
+    if (out && k < k1) {
+      const SV sv = sg[kk % 2];
       const bool kin = (k >= 1 && k <= n3 - 1) || (a.airbc && k == 0);
-      const double d3_k = a.d3[k], r3_k = a.r3[k];
       if (i < n1) {  // bx: i < n1, j <= n2, k <= n3
         z2 v = z0;
-        const long c = k * px + ox;
         if (jin && kin) {
-          z2 t1 = zdivq(zsub(hz, shz[q][r - 1][l]), d2_j, r2_j);
-          z2 hym = (k == 0) ? a.hyg[i + (long)n1 * j] : hy_prev;
-          z2 t2 = zdivq(zsub(hy, hym), d3_k, r3_k);
-          v = zsub(zsub(t1, t2), zrmul(a.s11[c], Ex[c]));
+          const z2 t1 = zdivq(zsub(hz, hz_dn), d2_j, r2_j);
+          const z2 t2 = zdivq(zsub(hy, hy_prev), d3_k, r3_k);
+          v = zsub(zsub(t1, t2), zrmul(sv.x, cur.x));
           if (a.fvm) v = zmul(v, zrmul(d1s_i * d2_j * d3_k, a.iomu));
         }
-        bx[c] = v;
+        bx[k * px + ox] = v;
       }
       if (j < n2) {  // by: i <= n1, j < n2, k <= n3
         z2 v = z0;
-        const long c = k * py + oy;
         if (iin && kin) {
-          z2 hxm = (k == 0) ? a.hxg[i + (long)(n1 + 1) * j] : hx_prev;
-          z2 t1 = zdivq(zsub(hx, hxm), d3_k, r3_k);
-          z2 t2 = zdivq(zsub(hz, shz[q][r][l - 1]), d1_i, r1_i);
-          v = zsub(zsub(t1, t2), zrmul(a.s22[c], Ey[c]));
+          const z2 t1 = zdivq(zsub(hx, hx_prev), d3_k, r3_k);
+          const z2 t2 = zdivq(zsub(hz, hz_l), d1_i, r1_i);
+          v = zsub(zsub(t1, t2), zrmul(sv.y, cur.y));
           if (a.fvm) v = zmul(v, zrmul(d1_i * d2s_j * d3_k, a.iomu));
         }
-        by[c] = v;
+        by[k * py + oy] = v;
       }
       if (k < n3) {  // bz: i <= n1, j <= n2, k < n3
         z2 v = z0;
-        const long c = k * pz + oz;
         if (iin && jin) {
-          z2 t1 = zdivq(zsub(hy, shy[q][r][l - 1]), d1_i, r1_i);
-          z2 t2 = zdivq(zsub(hx, shx[q][r - 1][l]), d2_j, r2_j);
-          v = zsub(zsub(t1, t2), zrmul(a.s33[c], Ez[c]));
-          if (a.fvm) v = zmul(v, zrmul(d1_i * d2_j * a.d3s[k], a.iomu));
+          const z2 t1 = zdivq(zsub(hy, hy_l), d1_i, r1_i);
+          const z2 t2 = zdivq(zsub(hx, hx_dn), d2_j, r2_j);
+          v = zsub(zsub(t1, t2), zrmul(sv.z, cur.z));
+          if (a.fvm) v = zmul(v, zrmul(d1_i * d2_j * d3s_k, a.iomu));
         }
-        bz[c] = v;
+        bz[k * pz + oz] = v;
       }
     }
     hx_prev = hx;
     hy_prev = hy;
   }
 }
+#endif
 
 // H = curl E / (i w mu0) on all faces (fdfd.c:125-149), Hx | Hy | Hz.
 #ifndef CSEM_TYPES_ONLY

@@ -292,33 +292,43 @@ static MxAir mx_air(const csem_model *m, const MixState *x, const MixSlot *q)
   return a;
 }
 
-template <int R, int TJ>
+// Operator tiles: NW waves (NW - 2 output rows) x KC planes.  LDS holds
+// 4 * NW * R * 64 complex floats: NW = 8 for R <= 3, at most 6 for R = 4.
+template <int R, int NW, int KC>
 static void mx_apply_launch(const csem_model *m, const MixState *x, const MixSlot *q, const MxFreq &fr, const c2 *in,
-                            c2 *out, int kc, hipStream_t s)
+                            c2 *out, hipStream_t s)
 {
-  dim3 blk(64, TJ + 1), grd((m->n1 + 1 + 62) / 63, (m->n2 + 1 + TJ - 1) / TJ, (m->n3 + 1 + kc - 1) / kc);
-  k_mx_apply<R, TJ><<<grd, blk, 0, s>>>(mx_geo(m, x), fr, in, out, q->hxg, q->hyg, kc);
+  const int nti = (m->n1 + 1 + 61) / 62, ntj = (m->n2 + 1 + NW - 3) / (NW - 2), ntk = (m->n3 + 1 + KC - 1) / KC;
+  const int ntiles = nti * ntj * ntk;
+  k_mx_apply<R, NW, KC><<<8 * ((ntiles + 7) / 8), dim3(64, NW), 0, s>>>(mx_geo(m, x), fr, in, out, q->hxg, q->hyg,
+                                                                       nti, ntj, ntk);
 }
-// rows per block: 2, 4 or 8 (static LDS 3*2*R*(TJ+1)*64*8 B stays under
-// 64 KB: TJ = 8 only for R <= 2)
 template <int R>
 static void mx_apply_tile(const csem_model *m, const MixState *x, const MixSlot *q, const MxFreq &fr, const c2 *in,
-                          c2 *out, int tj, int kc, hipStream_t s)
+                          c2 *out, int rows, int kc, hipStream_t s)
 {
-  if (tj == 2) mx_apply_launch<R, 2>(m, x, q, fr, in, out, kc, s);
-  else if (tj == 4 || R > 2) mx_apply_launch<R, 4>(m, x, q, fr, in, out, kc, s);
-  else mx_apply_launch<R, (R > 2 ? 4 : 8)>(m, x, q, fr, in, out, kc, s);
-}
-static void mx_apply_kernel(const csem_model *m, const MixState *x, const MixSlot *q, int R, const MxFreq &fr,
-                            const c2 *in, c2 *out, int tj, int kc, hipStream_t s)
-{
-  switch (R) {
-    case 1: mx_apply_tile<1>(m, x, q, fr, in, out, tj, kc, s); break;
-    case 2: mx_apply_tile<2>(m, x, q, fr, in, out, tj, kc, s); break;
-    case 3: mx_apply_tile<3>(m, x, q, fr, in, out, tj, kc, s); break;
-    default: mx_apply_tile<4>(m, x, q, fr, in, out, tj, kc, s); break;
+  constexpr int NB = R > 3 ? 6 : 8;  // the larger block
+  const bool big = rows + 2 >= NB;
+  if (kc >= 16) {
+    if (big) mx_apply_launch<R, NB, 16>(m, x, q, fr, in, out, s);
+    else mx_apply_launch<R, NB - 2, 16>(m, x, q, fr, in, out, s);
+  } else {
+    if (big) mx_apply_launch<R, NB, 8>(m, x, q, fr, in, out, s);
+    else mx_apply_launch<R, NB - 2, 8>(m, x, q, fr, in, out, s);
   }
 }
+static void mx_apply_kernel(const csem_model *m, const MixState *x, const MixSlot *q, int R, const MxFreq &fr,
+                            const c2 *in, c2 *out, int rows, int kc, hipStream_t s)
+{
+  switch (R) {
+    case 1: mx_apply_tile<1>(m, x, q, fr, in, out, rows, kc, s); break;
+    case 2: mx_apply_tile<2>(m, x, q, fr, in, out, rows, kc, s); break;
+    case 3: mx_apply_tile<3>(m, x, q, fr, in, out, rows, kc, s); break;
+    default: mx_apply_tile<4>(m, x, q, fr, in, out, rows, kc, s); break;
+  }
+}
+// output rows per block the operator offers at batch size R
+static void mx_tile_rows(int R, int rows[2]) { rows[0] = R > 3 ? 2 : 4; rows[1] = R > 3 ? 4 : 6; }
 
 // Autotuner: time every (rows per block, planes per block) candidate of the
 // FP32 operator on this model and batch size, keep the fastest.  Tile shape
@@ -328,14 +338,16 @@ static int mx_tune(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &
 {
   std::lock_guard<std::mutex> lk(x->mu);
   if (x->ap_tj[R]) return 0;
-  const int TJS[2] = {R > 2 ? 2 : 4, R > 2 ? 4 : 8}, KCS[3] = {4, 8, 16};
+  int TJS[2];
+  mx_tile_rows(R, TJS);
+  const int KCS[2] = {8, 16};
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return csem_fail(-1, "event");
   float best = 1e30f;
-  int bt = 4, bk = 8;
+  int bt = TJS[1], bk = 8;
   kt_pause(1);
   for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 3; b++) {
+    for (int b = 0; b < 2; b++) {
       mx_apply_kernel(m, x, q, R, fr, in, out, TJS[a], KCS[b], s);  // warm
       (void)hipEventRecord(e0, s);
       for (int w = 0; w < 3; w++) mx_apply_kernel(m, x, q, R, fr, in, out, TJS[a], KCS[b], s);
@@ -372,7 +384,7 @@ static int mx_apply(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
     KT(KT_M_AIR_GHOST, s, 8.0 * ng * R, (k_mx_air_ghost<<<dim3((ng + 255) / 256, R), 256, 0, s>>>(a)));
   }
   // algorithmic bytes: read E and write y per right-hand side, sigma once
-  const int tj = x->ap_tj[R] ? x->ap_tj[R] : 4, kc = x->ap_kc[R] ? x->ap_kc[R] : 8;
+  const int tj = x->ap_tj[R] ? x->ap_tj[R] : 6, kc = x->ap_kc[R] ? x->ap_kc[R] : 8;
   KT(KT_M_APPLY, s, (16.0 * R + 4.0) * m->lE, mx_apply_kernel(m, x, q, R, fr, in, out, tj, kc, s));
   HIP_OK(hipGetLastError());
   return 0;
@@ -733,7 +745,9 @@ extern "C" int csem_mixed_set_tile(csem_model *m, int R, int rows, int planes)
   MixState *x;
   int rc;
   if (R < 1 || R > MX_RMAX) return csem_fail(-3, "batch of %d right-hand sides outside [1, %d]", R, MX_RMAX);
-  if (!(rows == 2 || rows == 4 || (rows == 8 && R <= 2)) || !(planes >= 1 && planes <= 64))
+  int ok_rows[2];
+  mx_tile_rows(R, ok_rows);
+  if (!(rows == ok_rows[0] || rows == ok_rows[1]) || !(planes == 8 || planes == 16))
     return csem_fail(-3, "tile %d x %d not available for %d right-hand sides", rows, planes, R);
   {
     std::lock_guard<std::mutex> lk(g_mx_init_mu);

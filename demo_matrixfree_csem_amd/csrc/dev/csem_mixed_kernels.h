@@ -41,120 +41,184 @@ struct MxGeo {
 };
 
 // y_r = A_r x_r for every r: the curl-curl of fdfd.c:86-205 in FP32.  H' =
-// curl E (without 1/(i w mu0)) is formed once per face and shared through
-// LDS as in the FP64 kernel; the per-frequency factor multiplies the outer
-// curl.  Ghost planes (airwave boundary) hold H' as well.  Block: 64 lanes
-// (one halo) x TJ+1 rows (one halo); the grid marches kc planes in z.
-template <int R, int TJ>
-__global__ __launch_bounds__(64 * (TJ + 1)) void k_mx_apply(MxGeo g, MxFreq fr, const c2 *x, c2 *y, const c2 *hxg,
-                                                          const c2 *hyg, int kc)
+// curl E (without 1/(i w mu0)) is formed once per face; the per-frequency
+// factor multiplies the outer curl.  Ghost planes (airwave boundary) hold
+// H' as well.  The schedule is the FP64 kernel's: a block is NW waves over
+// a tile of rows and columns (wave w row j = j0-1+w, lane l column
+// i = i0-1+l; waves 0 and NW-1 and lanes 0 and 63 are halos) marching over
+// KC planes, fully unrolled; each lane prefetches its own E of all R
+// right-hand sides two planes ahead and sigma (shared by the R right-hand
+// sides) one plane ahead; j neighbours come through LDS, i neighbours
+// through whole-wave lane shifts; tiles are numbered columns, rows, planes
+// and consecutive tiles run on one XCD.
+template <int R, int NW, int KC>
+__global__ __launch_bounds__(64 * NW) void k_mx_apply(MxGeo g, MxFreq fr, const c2 *x, c2 *y, const c2 *hxg,
+                                                      const c2 *hyg, int nti, int ntj, int ntk)
 {
-  __shared__ c2 shx[2][R][TJ + 1][64], shy[2][R][TJ + 1][64], shz[2][R][TJ + 1][64];
-  const int l = threadIdx.x, rr = threadIdx.y;
-  const int i = blockIdx.x * 63 - 1 + l, j = blockIdx.y * TJ - 1 + rr;
+  constexpr int OI = 62, OJ = NW - 2;
+  __shared__ c2 sEx[NW][R][64], sEz[NW][R][64], sHx[NW][R][64], sHz[NW][R][64];
+  __shared__ float sD[KC][4];  // 1/d3s, 1/d3, d3, d3s of the tile's planes
+  const int ntiles = nti * ntj * ntk, per = (ntiles + 7) / 8;
+  const int T = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (T >= ntiles) return;
+  const int ti = T % nti, tj = (T / nti) % ntj, tk = T / (nti * ntj);
+  const int l = threadIdx.x, w = threadIdx.y;
+  const int i = ti * OI - 1 + l, j = tj * OJ - 1 + w;
   const int n1 = g.n1, n2 = g.n2, n3 = g.n3, ko = g.kofs;
-  const int k0 = blockIdx.z * kc, k1 = min(k0 + kc, n3 + 1);
+  const int k0 = tk * KC, k1 = min(k0 + KC, n3 + 1);
   const bool live = i >= 0 && i <= n1 && j >= 0 && j <= n2;
-  const bool out = live && l > 0 && rr > 0;
+  const bool out = live && l >= 1 && l <= OI && w >= 1 && w <= OJ;
   const bool hx_ok = live && j < n2, hy_ok = live && i < n1, hz_ok = hy_ok && j < n2;
+  const bool iin = i >= 1 && i <= n1 - 1, jin = j >= 1 && j <= n2 - 1;
   const int ic = min(max(i, 0), n1), jc = min(max(j, 0), n2);
   const float r1s_i = g.r1s[ic], r1_i = g.r1[ic], r2s_j = g.r2s[jc], r2_j = g.r2[jc];
+  // fvm cell volumes without their z factor (Ex, Ey, Ez)
+  const float vx = g.d1s[ic] * g.d2[jc], vy = g.d1[ic] * g.d2s[jc], vz = g.d1[ic] * g.d2[jc];
   const long px = (long)n1 * (n2 + 1), py = (long)(n1 + 1) * n2, pz = (long)(n1 + 1) * (n2 + 1);
-  const long ox = i + (long)n1 * j, oy = i + (long)(n1 + 1) * j, oz = oy;
+  const long ox = min(ic, n1 - 1) + (long)n1 * jc, oy = ic + (long)(n1 + 1) * min(jc, n2 - 1),
+             oz = ic + (long)(n1 + 1) * jc;
   const long n = g.n;
   const c2 z0 = cmk(0, 0);
   // sigma on the operator grid: the full grid offset by kofs planes
   const float *s11 = g.s11 + ko * px, *s22 = g.s22 + ko * py, *s33 = g.s33 + ko * pz;
-  c2 hxp[R], hyp[R];
-  auto HX = [&](int r, int k) {
-    const c2 *ez = x + r * n + g.lEx + g.lEy + k * pz + oz, *ey = x + r * n + g.lEx + k * py + oy;
-    const float r3s = g.r3s[k + ko];
-    return cmk((ez[n1 + 1].x - ez[0].x) * r2s_j - (ey[py].x - ey[0].x) * r3s,
-               (ez[n1 + 1].y - ez[0].y) * r2s_j - (ey[py].y - ey[0].y) * r3s);
-  };
-  auto HY = [&](int r, int k) {
-    const c2 *ex = x + r * n + k * px + ox, *ez = x + r * n + g.lEx + g.lEy + k * pz + oz;
-    const float r3s = g.r3s[k + ko];
-    return cmk((ex[px].x - ex[0].x) * r3s - (ez[1].x - ez[0].x) * r1s_i,
-               (ex[px].y - ex[0].y) * r3s - (ez[1].y - ez[0].y) * r1s_i);
-  };
-  auto HZ = [&](int r, int k) {
-    const c2 *ey = x + r * n + g.lEx + k * py + oy, *ex = x + r * n + k * px + ox;
-    return cmk((ey[1].x - ey[0].x) * r1s_i - (ex[n1].x - ex[0].x) * r2s_j,
-               (ey[1].y - ey[0].y) * r1s_i - (ex[n1].y - ex[0].y) * r2s_j);
-  };
-  // H' of the plane below k0; below k = 0 that is the airwave ghost plane
+  struct EV { c2 x[R], y[R], z[R]; };
+  struct SV { float x, y, z; };
+  auto ldE = [&](int k) {
+    const long kc = max(k, 0), kxy = min(kc, (long)n3), kz = min(kc, (long)n3 - 1);
+    EV e;
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    hxp[r] = z0; hyp[r] = z0;
-    if (k0 > 0 && k0 - 1 < n3) {
-      if (hx_ok) hxp[r] = HX(r, k0 - 1);
-      if (hy_ok) hyp[r] = HY(r, k0 - 1);
-    } else if (k0 == 0 && g.airbc && live) {
-      if (j < n2) hxp[r] = hxg[r * (long)(n1 + 1) * n2 + i + (long)(n1 + 1) * j];
-      if (i < n1) hyp[r] = hyg[r * (long)n1 * (n2 + 1) + i + (long)n1 * j];
+    for (int r = 0; r < R; r++) {
+      e.x[r] = x[r * n + kxy * px + ox];
+      e.y[r] = x[r * n + g.lEx + kxy * py + oy];
+      e.z[r] = x[r * n + g.lEx + g.lEy + kz * pz + oz];
     }
+    return e;
+  };
+  auto ldS = [&](int k) {
+    SV v;
+    v.x = s11[min(k, n3) * px + ox];
+    v.y = s22[min(k, n3) * py + oy];
+    v.z = s33[min(k, n3 - 1) * pz + oz];
+    return v;
+  };
+  const int wu = min(w + 1, NW - 1), wd = max(w - 1, 0);
+  if (w == 0 && l < 4 * KC) {
+    const int kq = min(k0 + l / 4, n3) + ko, c = l % 4;
+    const float *src = c == 0 ? g.r3s : c == 1 ? g.r3 : c == 2 ? g.d3 : g.d3s;
+    sD[l / 4][c] = src[kq];
   }
-  const bool iin = i >= 1 && i <= n1 - 1, jin = j >= 1 && j <= n2 - 1;
-  for (int k = k0; k < k1; k++) {
-    const int q = k & 1;
+  EV e[3];
+  SV sg[2];
+  e[2] = ldE(k0 - 1);
+  e[0] = ldE(k0);
+  e[1] = ldE(k0 + 1);
+  sg[0] = ldS(k0);
+  c2 hxp[R], hyp[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) { hxp[r] = z0; hyp[r] = z0; }
+  if (k0 == 0) {
+    if (g.airbc) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (hx_ok) hxp[r] = hxg[r * (long)(n1 + 1) * n2 + ic + (long)(n1 + 1) * jc];
+        if (hy_ok) hyp[r] = hyg[r * (long)n1 * (n2 + 1) + ic + (long)n1 * jc];
+      }
+    }
+  } else {
+    const float r3s = g.r3s[k0 - 1 + ko];
+#pragma unroll
+    for (int r = 0; r < R; r++) sEz[w][r][l] = e[2].z[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const c2 ez_up = sEz[wu][r][l], ez = e[2].z[r], ez_r = cdpp<DPP_WAVE_SHL1>(ez);
+      const c2 hx = cmk((ez_up.x - ez.x) * r2s_j - (e[0].y[r].x - e[2].y[r].x) * r3s,
+                        (ez_up.y - ez.y) * r2s_j - (e[0].y[r].y - e[2].y[r].y) * r3s);
+      const c2 hy = cmk((e[0].x[r].x - e[2].x[r].x) * r3s - (ez_r.x - ez.x) * r1s_i,
+                        (e[0].x[r].y - e[2].x[r].y) * r3s - (ez_r.y - ez.y) * r1s_i);
+      hxp[r] = hx_ok ? hx : z0;
+      hyp[r] = hy_ok ? hy : z0;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) { sEx[w][r][l] = e[0].x[r]; sEz[w][r][l] = e[0].z[r]; }
+  __syncthreads();
+#pragma unroll
+  for (int kk = 0; kk < KC; kk++) {
+    const int k = k0 + kk;
+    const EV cur = e[kk % 3], nxt = e[(kk + 1) % 3];
+    e[(kk + 2) % 3] = ldE(k + 2);
+    sg[(kk + 1) % 2] = ldS(k + 1);
+    const float r3s_k = sD[kk][0], r3_k = sD[kk][1];
     c2 hx[R], hy[R], hz[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      hx[r] = z0; hy[r] = z0; hz[r] = z0;
-      if (k < n3) {
-        if (hx_ok) hx[r] = HX(r, k);
-        if (hy_ok) hy[r] = HY(r, k);
-      }
-      if (hz_ok) hz[r] = HZ(r, k);
-      shx[q][r][rr][l] = hx[r]; shy[q][r][rr][l] = hy[r]; shz[q][r][rr][l] = hz[r];
+      const c2 ez_up = sEz[wu][r][l], ex_up = sEx[wu][r][l];
+      const c2 ey_r = cdpp<DPP_WAVE_SHL1>(cur.y[r]), ez_r = cdpp<DPP_WAVE_SHL1>(cur.z[r]);
+      const c2 ex = cur.x[r], ey = cur.y[r], ez = cur.z[r];
+      const c2 hx0 = cmk((ez_up.x - ez.x) * r2s_j - (nxt.y[r].x - ey.x) * r3s_k,
+                         (ez_up.y - ez.y) * r2s_j - (nxt.y[r].y - ey.y) * r3s_k);
+      const c2 hy0 = cmk((nxt.x[r].x - ex.x) * r3s_k - (ez_r.x - ez.x) * r1s_i,
+                         (nxt.x[r].y - ex.y) * r3s_k - (ez_r.y - ez.y) * r1s_i);
+      const c2 hz0 = cmk((ey_r.x - ey.x) * r1s_i - (ex_up.x - ex.x) * r2s_j,
+                         (ey_r.y - ey.y) * r1s_i - (ex_up.y - ex.y) * r2s_j);
+      hx[r] = (hx_ok && k < n3) ? hx0 : z0;
+      hy[r] = (hy_ok && k < n3) ? hy0 : z0;
+      hz[r] = hz_ok ? hz0 : z0;
     }
     __syncthreads();
-    if (out) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      sHx[w][r][l] = hx[r]; sHz[w][r][l] = hz[r];
+      sEx[w][r][l] = nxt.x[r]; sEz[w][r][l] = nxt.z[r];
+    }
+    __syncthreads();
+    c2 hz_dn[R], hx_dn[R], hy_l[R], hz_l[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      hz_dn[r] = sHz[wd][r][l]; hx_dn[r] = sHx[wd][r][l];
+      hy_l[r] = cdpp<DPP_WAVE_SHR1>(hy[r]); hz_l[r] = cdpp<DPP_WAVE_SHR1>(hz[r]);
+    }
+    if (out && k < k1) {
+      const SV sv = sg[kk % 2];
       const bool kin = (k >= 1 && k <= n3 - 1) || (g.airbc && k == 0);
-      const float r3_k = g.r3[k + ko], r3s_k = g.r3s[k + ko];
+      const float d3_k = sD[kk][2], d3s_k = sD[kk][3];
 #pragma unroll
       for (int r = 0; r < R; r++) {
         const c2 iw = fr.iwmu[r];
         c2 *yr = y + r * n;
-        const c2 *xr = x + r * n;
         if (i < n1) {  // bx
-          const long c = k * px + ox;
           c2 v = z0;
           if (jin && kin) {
-            const c2 hym = hyp[r];
-            const c2 cu = cmk((hz[r].x - shz[q][r][rr - 1][l].x) * r2_j - (hy[r].x - hym.x) * r3_k,
-                              (hz[r].y - shz[q][r][rr - 1][l].y) * r2_j - (hy[r].y - hym.y) * r3_k);
-            v = csub(cmul(cu, iw), crmul(s11[c], xr[c]));
-            if (g.fvm) v = cmul(v, crmul(g.d1s[i] * g.d2[j] * g.d3[k + ko], fr.iomu[r]));
+            const c2 cu = cmk((hz[r].x - hz_dn[r].x) * r2_j - (hy[r].x - hyp[r].x) * r3_k,
+                              (hz[r].y - hz_dn[r].y) * r2_j - (hy[r].y - hyp[r].y) * r3_k);
+            v = csub(cmul(cu, iw), crmul(sv.x, cur.x[r]));
+            if (g.fvm) v = cmul(v, crmul(vx * d3_k, fr.iomu[r]));
           }
-          yr[c] = v;
+          yr[k * px + ox] = v;
         }
         if (j < n2) {  // by
-          const long c = g.lEx + k * py + oy;
           c2 v = z0;
           if (iin && kin) {
-            const c2 hxm = hxp[r];
-            const c2 cu = cmk((hx[r].x - hxm.x) * r3_k - (hz[r].x - shz[q][r][rr][l - 1].x) * r1_i,
-                              (hx[r].y - hxm.y) * r3_k - (hz[r].y - shz[q][r][rr][l - 1].y) * r1_i);
-            v = csub(cmul(cu, iw), crmul(s22[c - g.lEx], xr[c]));
-            if (g.fvm) v = cmul(v, crmul(g.d1[i] * g.d2s[j] * g.d3[k + ko], fr.iomu[r]));
+            const c2 cu = cmk((hx[r].x - hxp[r].x) * r3_k - (hz[r].x - hz_l[r].x) * r1_i,
+                              (hx[r].y - hxp[r].y) * r3_k - (hz[r].y - hz_l[r].y) * r1_i);
+            v = csub(cmul(cu, iw), crmul(sv.y, cur.y[r]));
+            if (g.fvm) v = cmul(v, crmul(vy * d3_k, fr.iomu[r]));
           }
-          yr[c] = v;
+          yr[g.lEx + k * py + oy] = v;
         }
         if (k < n3) {  // bz
-          const long c = g.lEx + g.lEy + k * pz + oz;
           c2 v = z0;
           if (iin && jin) {
-            const c2 cu = cmk((hy[r].x - shy[q][r][rr][l - 1].x) * r1_i - (hx[r].x - shx[q][r][rr - 1][l].x) * r2_j,
-                              (hy[r].y - shy[q][r][rr][l - 1].y) * r1_i - (hx[r].y - shx[q][r][rr - 1][l].y) * r2_j);
-            v = csub(cmul(cu, iw), crmul(s33[c - g.lEx - g.lEy], xr[c]));
-            if (g.fvm) v = cmul(v, crmul(g.d1[i] * g.d2[j] * g.d3s[k + ko], fr.iomu[r]));
+            const c2 cu = cmk((hy[r].x - hy_l[r].x) * r1_i - (hx[r].x - hx_dn[r].x) * r2_j,
+                              (hy[r].y - hy_l[r].y) * r1_i - (hx[r].y - hx_dn[r].y) * r2_j);
+            v = csub(cmul(cu, iw), crmul(sv.z, cur.z[r]));
+            if (g.fvm) v = cmul(v, crmul(vz * d3s_k, fr.iomu[r]));
           }
-          yr[c] = v;
+          yr[g.lEx + g.lEy + k * pz + oz] = v;
         }
       }
-      (void)r3s_k;
     }
 #pragma unroll
     for (int r = 0; r < R; r++) { hxp[r] = hx[r]; hyp[r] = hy[r]; }

@@ -141,7 +141,8 @@ int csem_lusgs32(csem_model *m, int R, const csem_freq *f, const void *x, void *
  * (rows per block, planes per block); -1 before the first solve. */
 int csem_mixed_tuning(csem_model *m, int R, int *rows, int *planes);
 /* Fix the FP32 operator's tile at batch size R instead of timing the
- * candidates (rows 2, 4, or 8 for R <= 2; planes 1..64). */
+ * candidates: output rows per block 4 or 6 (R <= 3), 2 or 4 (R = 4);
+ * planes per block 8 or 16. */
 int csem_mixed_set_tile(csem_model *m, int R, int rows, int planes);
 /* device bytes held by the mixed-precision state of a model (its FP32
  * copy and every solve slot), and the bytes one more slot for R

@@ -95,16 +95,17 @@ def test_apply32_every_tile(workdir):
     try:
         n = ses.n
         rng = np.random.default_rng(13)
-        for R in (1, 3):
+        for R in (1, 3, 4):
             x = (rng.standard_normal((R, n)) + 1j * rng.standard_normal((R, n))).astype(np.complex64)
-            fr = (_lib.FreqT * R)(*[freq_consts(f) for f in FREQS[:R]])
+            fq = (FREQS * 2)[:R]
+            fr = (_lib.FreqT * R)(*[freq_consts(f) for f in fq])
             dx, dy = _dev(x), _dev(np.zeros_like(x))
             ref = []
-            for r, f in enumerate(FREQS[:R]):
+            for r, f in enumerate(fq):
                 orc.set_freq(f)
                 ref.append(orc.apply(x[r].astype(np.complex128)))
-            for rows in ((4, 8) if R <= 2 else (2, 4)):
-                for planes in (4, 8, 16):
+            for rows in ((4, 6) if R <= 3 else (2, 4)):
+                for planes in (8, 16):
                     _lib.check(L.csem_mixed_set_tile(ses.model, R, rows, planes), "set_tile")
                     _lib.check(L.csem_apply32(ses.model, R, fr, dx, dy, None), "apply32")
                     y = _host(dy, x)
