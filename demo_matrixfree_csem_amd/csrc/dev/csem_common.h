@@ -58,6 +58,8 @@ __host__ __device__ __forceinline__ z2 zdivq(z2 a, double d, double rd) { return
 // source lane reads 0).  Every lane of the wave must be active.
 #define DPP_WAVE_SHL1 0x130  // lane l reads lane l+1
 #define DPP_WAVE_SHR1 0x138  // lane l reads lane l-1
+#define DPP_WAVE_ROR1 0x13C  // lane l reads lane l-1, lane 0 reads lane 63
+#define DPP_WAVE_ROL1 0x134  // lane l reads lane l+1, lane 63 reads lane 0
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v)
 {

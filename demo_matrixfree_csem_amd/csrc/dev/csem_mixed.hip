@@ -47,6 +47,7 @@ struct MixSlot {
   c2 *W, *trash;                            // LU-SGS state and discarded stores
   float4 *th;                               // LU-SGS records (T, h) per swept cell and r
   float hkey[MX_RMAX][2];                   // frequencies the diagonal factors h hold
+  int pc;                                   // preconditioner of the solve in flight (csem_mixed_opts)
   int hR;
   c2 *kv;                                   // Krylov vectors
   double *part, *sc;
@@ -61,6 +62,7 @@ struct MixSlot {
   int g_R;
   hipStream_t g_stream;
   MxFreq g_fr;  // the frequencies baked into the captured launches
+  int g_pc;     // and the preconditioner
   std::vector<void *> allocs;
   long bytes;
 };
@@ -424,7 +426,7 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, bool diag, bool sweep
   double nc[3];
   for (int c = 0; c < 3; c++) {
     tb[c] = ((g.A[c] + 63) / 64) * 64;
-    lds[c] = 2 * (size_t)(tb[c] + 2) * sizeof(c2) + (size_t)(g.B[c] + 2) * sizeof(float) +
+    lds[c] = (size_t)(tb[c] / 64) * MXS_RING * sizeof(c2) + (size_t)(g.B[c] + 2) * sizeof(float) +
              (size_t)(g.A[c] + g.B[c] + 1) * sizeof(int);
     gs[c] = dim3(8 * ((g.ns[c] + 7) / 8), R);
     nc[c] = (double)g.ns[c] * g.A[c] * g.B[c];
@@ -445,6 +447,46 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, bool diag, bool sweep
   KT(KT_M_SGS, s, 32.0 * R * nc[1], (k_mx_sgs<1, false><<<gs[1], tb[1], lds[1], s>>>(a)));
   KT(KT_M_PRE, s, 40.0 * R * nc[0], (k_mx_pre<0, false, R><<<gq[0], 256, 0, s>>>(a)));
   KT(KT_M_SGS, s, 24.0 * R * nc[0], (k_mx_sgs<0, false><<<gs[0], tb[0], lds[0], s>>>(a)));
+}
+
+// The red-black ordered sweeps: forward red, black per component (Ex, Ey,
+// Ez), backward black, red (Ez, Ey, Ex).  Bytes per swept cell and
+// right-hand side each pass must move (half the cells per pass): b, the
+// other components' W values, the own component's neighbours (except the
+// forward red pass), the value written (and y backward).
+template <int R>
+static void mx_rbsgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
+{
+  const SkewGeo &g = m->g;
+  dim3 gq[3];
+  gq[0] = dim3((m->n1 + 63) / 64, (m->n2 + 1 + 3) / 4, m->n3f + 1);
+  gq[1] = dim3((m->n1 + 1 + 63) / 64, (m->n2 + 3) / 4, m->n3f + 1);
+  gq[2] = dim3((m->n1 + 1 + 63) / 64, (m->n2 + 1 + 3) / 4, m->n3f);
+  const dim3 bq(64, 4);
+  double h[3];
+  for (int c = 0; c < 3; c++) h[c] = 0.5 * g.ns[c] * g.A[c] * g.B[c] * R;
+  KT(KT_M_SGS, s, 16.0 * h[0], (k_mx_rb<0, true, 0, R><<<gq[0], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * h[0], (k_mx_rb<0, true, 1, R><<<gq[0], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * h[1], (k_mx_rb<1, true, 0, R><<<gq[1], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 32.0 * h[1], (k_mx_rb<1, true, 1, R><<<gq[1], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 32.0 * h[2], (k_mx_rb<2, true, 0, R><<<gq[2], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 40.0 * h[2], (k_mx_rb<2, true, 1, R><<<gq[2], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 48.0 * h[2], (k_mx_rb<2, false, 1, R><<<gq[2], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 48.0 * h[2], (k_mx_rb<2, false, 0, R><<<gq[2], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 48.0 * h[1], (k_mx_rb<1, false, 1, R><<<gq[1], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 48.0 * h[1], (k_mx_rb<1, false, 0, R><<<gq[1], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 48.0 * h[0], (k_mx_rb<0, false, 1, R><<<gq[0], bq, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 48.0 * h[0], (k_mx_rb<0, false, 0, R><<<gq[0], bq, 0, s>>>(a)));
+}
+
+static void mx_rbsgs_dispatch(const csem_model *m, int R, const MxLu &a, hipStream_t s)
+{
+  switch (R) {
+    case 1: mx_rbsgs_R<1>(m, a, s); break;
+    case 2: mx_rbsgs_R<2>(m, a, s); break;
+    case 3: mx_rbsgs_R<3>(m, a, s); break;
+    default: mx_rbsgs_R<4>(m, a, s); break;
+  }
 }
 
 static void mx_lusgs_dispatch(const csem_model *m, int R, const MxLu &a, bool diag, bool sweep, hipStream_t s)
@@ -474,7 +516,8 @@ static int mx_diag(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &
 static int mx_lusgs(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, const c2 *b, c2 *y,
                     hipStream_t s)
 {
-  mx_lusgs_dispatch(m, R, mx_lu(m, x, q, fr, b, y), false, true, s);
+  if (q->pc == CSEM_PRECOND_RBSGS) mx_rbsgs_dispatch(m, R, mx_lu(m, x, q, fr, b, y), s);
+  else mx_lusgs_dispatch(m, R, mx_lu(m, x, q, fr, b, y), false, true, s);
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -518,7 +561,7 @@ static int mx_iteration(csem_model *m, MixState *x, MixSlot *q, int R, const MxF
 // and every scalar live in device memory or in the captured arguments).
 static int mx_graph(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, hipStream_t s)
 {
-  if (q->gexec && q->g_R == R && q->g_stream == s && !memcmp(&q->g_fr, &fr, sizeof fr)) return 0;
+  if (q->gexec && q->g_R == R && q->g_stream == s && q->g_pc == q->pc && !memcmp(&q->g_fr, &fr, sizeof fr)) return 0;
   if (q->gexec) { (void)hipGraphExecDestroy(q->gexec); q->gexec = NULL; }
   hipGraph_t g;
   int rc;
@@ -543,6 +586,7 @@ static int mx_graph(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
   q->g_R = R;
   q->g_stream = s;
   q->g_fr = fr;
+  q->g_pc = q->pc;
   return 0;
 }
 
@@ -599,6 +643,9 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
   std::vector<double> bn(R), rn(R), hsc((size_t)R * MS_N), nb(R);
   std::vector<float> scl(R);
   int rc;
+  if (o.precond != CSEM_PRECOND_LUSGS && o.precond != CSEM_PRECOND_RBSGS)
+    return csem_fail(-3, "unknown preconditioner %d", o.precond);
+  q->pc = o.precond;
   // |b_r|, x = 0, res = b
   k_mx_norm_part<<<dim3(nch, R), RED_T, 0, s>>>(n, B, q->npart, nch);
   k_mx_norm_fin<<<R, RED_T, 0, s>>>(q->npart, nch, q->norm);
@@ -686,7 +733,7 @@ extern "C" int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *
     if (e != hipSuccess) { slot_release(x, q); return csem_fail(-1, "sync: %s", hipGetErrorString(e)); }
   }
   const hipStream_t s = stream ? (hipStream_t)stream : q->own;
-  const csem_mixed_opts dflt = {2000, 1e-2, 40, 8, 1};
+  const csem_mixed_opts dflt = {2000, 1e-2, 40, 8, 1, CSEM_PRECOND_LUSGS};
   rc = mixed_solve_slot(m, x, q, R, f, (z2 *)xv, (const z2 *)bv, tol, opt ? *opt : dflt, iters, relres, s);
   slot_release(x, q);
   return rc;
@@ -705,6 +752,7 @@ static int mx_single(csem_model *m, int R, const csem_freq *f, const void *xv, v
   if ((rc = slot_acquire(m, x, R, &q))) return rc;
   const hipStream_t s = (hipStream_t)stream;
   const MxFreq fr = mx_freq(f, R);
+  q->pc = CSEM_PRECOND_LUSGS;
   if (lu) {
     if (!(rc = mx_diag(m, x, q, R, fr, s))) rc = mx_lusgs(m, x, q, R, fr, (const c2 *)xv, (c2 *)yv, s);
   } else {

@@ -494,15 +494,19 @@ template <> struct MxPreTile<0> { static constexpr int TQ = 16, TS = 8, TF = 8; 
 #define MW1(r, k, j, i) a.W[(r) * a.fE + a.fEx + id2(n1, n2, k, j, i)]
 #define MW2(r, k, j, i) a.W[(r) * a.fE + a.fEx + a.fEy + id3(n1, n2, k, j, i)]
 
-// Pre-pass of sweep (C, FWD) for every r: T of a tile of cells.
+// Pre-pass of sweep (C, FWD) for every r: T of a tile of cells.  Each cell
+// goes to LDS at its position in the tile's skewed order, with its global
+// record index, so the store phase is a straight copy.
 template <int C, bool FWD, int R>
 __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
 {
   using TL = MxPreTile<C>;
-  constexpr int TQ = TL::TQ, TS = TL::TS, TF = TL::TF, TFP = TF + 1, NC = TQ * TS * TF;
-  __shared__ c2 tile[R][TQ * TS * TFP];
+  constexpr int TQ = TL::TQ, TS = TL::TS, TF = TL::TF, NC = TQ * TS * TF;
+  __shared__ c2 tile[R][NC];
+  __shared__ int pidx[NC];  // global record index of each slot, -1 off the slices
   const int n1 = a.g.n1, n2 = a.g.n2, n3 = a.g.n3f;
   const int A = a.sk.A[C], B = a.sk.B[C], ns = a.sk.ns[C];
+  const long AB = (long)A * B;
   const int t = threadIdx.x;
   int q0, s0, f0;
   if (C == 0) { q0 = blockIdx.x * TQ; f0 = 1 + blockIdx.y * TF; s0 = 1 + blockIdx.z * TS; }
@@ -516,14 +520,23 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
     if (C == 0) { lq = t % TQ; lf = (t / TQ) % TF; ls = t / (TQ * TF) + it * (256 / (TQ * TF)); }
     else { lq = 0; lf = t % TF; ls = t / TF + it * (256 / TF); }
     const int q = q0 + lq, s = s0 + ls, f = f0 + lf;
-    if (q >= ns || s > A || f > B) continue;
+    // the cell's slot: its tile's skewed order (1-based local diagonal dl)
+    const int dl = ls + lf + 2;
+    const int cell = lq * TS * TF + sk_off(dl, TS, TF) + (ls + 1 - max(1, dl - TF));
+    if (q >= ns || s > A || f > B) {
+      pidx[cell] = -1;
+      continue;
+    }
+    {
+      const int d = s + f;
+      pidx[cell] = (int)(a.sk.base[C] + q * AB + sk_off(d, A, B) + (s - max(1, d - B)));
+    }
     int k, j, i;
     lu_kji<C>(q, s, f, k, j, i);
     const int kop = k - a.g.kofs;
     const long xo = (C == 0) ? id1(n1, n2, kop, j, i) : (C == 1) ? a.g.lEx + id2(n1, n2, kop, j, i)
                                                                    : a.g.lEx + a.g.lEy + id3(n1, n2, kop, j, i);
     const float vol = a.g.d1s[i] * a.g.d2[j] * a.g.d3[k];
-    const int cell = (lq * TS + ls) * TFP + lf;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       c2 S = z0, own = z0;
@@ -555,64 +568,159 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
   __syncthreads();
   // store: the tile's cells in the skewed order (per slice, by diagonal,
   // rows contiguous within a diagonal)
-  const long AB = (long)A * B;
 #pragma unroll
   for (int it = 0; it < NC / 256; it++) {
-    const int e = t + it * 256, lq = e / (TS * TF), el = e % (TS * TF);
-    int lo = 2, hi = TS + TF;  // local diagonal (1-based rows and columns)
+    const int e = t + it * 256;
+    const int p = pidx[e];
+    if (p < 0) continue;
 #pragma unroll
-    for (int u = 0; u < 7; u++) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (lo < hi) { if (sk_off(mid, TS, TF) <= el) lo = mid; else hi = mid - 1; }
+    for (int r = 0; r < R; r++) ((c2 *)a.th)[2 * (r * a.nsk + p)] = tile[r][e];
+  }
+}
+
+// One colour of a red-black ordered symmetric Gauss-Seidel sweep of
+// component C for every r: the splitting and cell update of the LU-SGS
+// above (same h, same other-component terms S for the direction, same
+// in-plane coefficients), with the cells of each 2-D slice visited red
+// ((s + f) even) before black in the forward sweep and black before red in
+// the backward one instead of lexicographically.  A cell of one colour
+// couples in-plane only to the other colour, so each colour is one fully
+// parallel pass: forward red x = h (beta b - S) (its neighbours come later),
+// forward black and both backward colours x = h (beta b - S + sum of the
+// four in-plane neighbours' current values).  Cells outside the swept
+// range keep W = 0; the last pass (backward red) writes the zero rows of
+// the operator-grid output.
+template <int C, bool FWD, int COLOR, int R>
+__global__ __launch_bounds__(256) void k_mx_rb(MxLu a)
+{
+  const int n1 = a.g.n1, n2 = a.g.n2, n3 = a.g.n3f;
+  const int nx = (C == 0) ? n1 : n1 + 1, ny = (C == 1) ? n2 : n2 + 1, nz = (C == 2) ? n3 : n3 + 1;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= nx || j >= ny || k >= nz) return;
+  int s, f;
+  bool sw;
+  if (C == 0) { s = k; f = j; sw = k >= 1 && k <= n3 - 1 && j >= 1 && j <= n2 - 1; }
+  else if (C == 1) { s = k; f = i; sw = k >= 1 && k <= n3 - 1 && i >= 1 && i <= n1 - 1; }
+  else { s = j; f = i; sw = j >= 1 && j <= n2 - 1 && i >= 1 && i <= n1 - 1; }
+  const int kop = k - a.g.kofs;
+  const long xo = (C == 0) ? id1(n1, n2, kop, j, i) : (C == 1) ? a.g.lEx + id2(n1, n2, kop, j, i)
+                                                                 : a.g.lEx + a.g.lEy + id3(n1, n2, kop, j, i);
+  const c2 z0 = cmk(0, 0);
+  if (!sw) {
+    if (!FWD && COLOR == 0 && kop >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; r++) a.y[r * a.g.n + xo] = z0;
     }
-    const int ls = max(1, lo - TF) + (el - sk_off(lo, TS, TF)) - 1, lf = lo - 2 - ls;
-    const int q = q0 + lq, s = s0 + ls, f = f0 + lf;
-    if (q >= ns || s > A || f > B) continue;
-    const int d = s + f;
-    const long p = a.sk.base[C] + q * AB + sk_off(d, A, B) + (s - max(1, d - B));
-    const int cell = (lq * TS + ls) * TFP + lf;
+    return;
+  }
+  if (((s + f) & 1) != COLOR) return;
+  const float *r1 = a.g.r1, *r2 = a.g.r2, *r3 = a.g.r3, *r1s = a.g.r1s, *r2s = a.g.r2s, *r3s = a.g.r3s;
+  const float vol = a.g.d1s[i] * a.g.d2[j] * a.g.d3[k];
+  float c, sig, cam, cap, cbm, cbp;  // in-plane coefficients: fast axis -, +; slow axis -, +
+  long wc, fa, sa;                   // own W index, fast- and slow-axis strides
+  if (C == 0) {
+    cam = a.c2m[j]; cap = a.c2p[j]; cbm = a.c3m[k]; cbp = a.c3p[k];
+    sig = a.g.s11[id1(n1, n2, k, j, i)];
+    wc = id1(n1, n2, k, j, i); fa = n1; sa = (long)n1 * (n2 + 1);
+  } else if (C == 1) {
+    cam = a.c1m[i]; cap = a.c1p[i]; cbm = a.c3m[k]; cbp = a.c3p[k];
+    sig = a.g.s22[id2(n1, n2, k, j, i)];
+    wc = a.fEx + id2(n1, n2, k, j, i); fa = 1; sa = (long)(n1 + 1) * n2;
+  } else {
+    cam = a.c1m[i]; cap = a.c1p[i]; cbm = a.c2m[j]; cbp = a.c2p[j];
+    sig = a.g.s33[id3(n1, n2, k, j, i)];
+    wc = a.fEx + a.fEy + id3(n1, n2, k, j, i); fa = 1; sa = n1 + 1;
+  }
+  c = cam + cap + cbm + cbp;
 #pragma unroll
-    for (int r = 0; r < R; r++) ((c2 *)a.th)[2 * (r * a.nsk + p)] = tile[r][cell];
+  for (int r = 0; r < R; r++) {
+    const c2 *Wr = a.W + r * a.fE;
+    c2 S = z0;
+    if (C == 0) {
+      if (!FWD) {  // Ey, Ez (backward values)
+        const c2 P1 = csub(MW1(r, k, j, i + 1), MW1(r, k, j, i)), P2 = csub(MW1(r, k, j - 1, i + 1), MW1(r, k, j - 1, i));
+        const c2 Q3 = csub(MW2(r, k, j, i + 1), MW2(r, k, j, i)), Q4 = csub(MW2(r, k - 1, j, i + 1), MW2(r, k - 1, j, i));
+        S = crmul(r1s[i], cadd(crmul(r2[j], csub(P1, P2)), crmul(r3[k], csub(Q3, Q4))));
+      }
+    } else if (C == 1) {
+      const c2 Q3 = csub(MW0(r, k, j + 1, i), MW0(r, k, j, i)), Q4 = csub(MW0(r, k, j + 1, i - 1), MW0(r, k, j, i - 1));
+      c2 PP = z0;
+      if (!FWD) PP = csub(csub(MW2(r, k, j + 1, i), MW2(r, k, j, i)), csub(MW2(r, k - 1, j + 1, i), MW2(r, k - 1, j, i)));
+      S = crmul(r2s[j], cadd(crmul(r3[k], PP), crmul(r1[i], csub(Q3, Q4))));
+    } else {
+      const c2 P1 = csub(MW0(r, k + 1, j, i), MW0(r, k, j, i)), P2 = csub(MW0(r, k + 1, j, i - 1), MW0(r, k, j, i - 1));
+      const c2 Q3 = csub(MW1(r, k + 1, j, i), MW1(r, k, j, i)), Q4 = csub(MW1(r, k + 1, j - 1, i), MW1(r, k, j - 1, i));
+      S = crmul(r3s[k], cadd(crmul(r1[i], csub(P1, P2)), crmul(r2[j], csub(Q3, Q4))));
+    }
+    c2 bv = z0;
+    if (kop >= 0) bv = a.b[r * a.g.n + xo];
+    const c2 bb = a.g.fvm ? crmul(1.f / vol, bv) : cmul(a.fr.iomu[r], bv);
+    c2 u = csub(bb, S);
+    if (!(FWD && COLOR == 0)) {
+      const c2 nf = crfma(crmul(cam, Wr[wc - fa]), cap, Wr[wc + fa]);
+      const c2 ns = crfma(crmul(cbm, Wr[wc - sa]), cbp, Wr[wc + sa]);
+      u = cadd(u, cadd(nf, ns));
+    }
+    // h = iwmu/aii, aii = c iwmu - sigma (the fvm factor cancels)
+    const c2 iw = a.fr.iwmu[r];
+    const c2 aii = cmk(c * iw.x - sig, c * iw.y);
+    const float den = 1.f / (aii.x * aii.x + aii.y * aii.y);
+    const c2 h = cmk((iw.x * aii.x + iw.y * aii.y) * den, (iw.y * aii.x - iw.x * aii.y) * den);
+    const c2 v = cmul(h, u);
+    a.W[r * a.fE + wc] = v;
+    if (!FWD && kop >= 0) a.y[r * a.g.n + xo] = v;
   }
 }
 #undef MW0
 #undef MW1
 #undef MW2
 
-// Sweep (C, FWD) of one slice for one right-hand side: block (slice,
-// r).  The records of the diagonals ahead stream into a register ring
-// MXS_P deep, so MXS_P diagonals of loads are in flight while the wavefront
-// advances (one load and at most two stores per diagonal keep that within
-// the 63 memory operations a wave can have outstanding).  The compiler
-// waits for every outstanding load at the loop's back edge, so the loop
-// runs MXS_U diagonals per iteration.
+// Sweep (C, FWD) of one slice for one right-hand side: block (slice, r),
+// one thread per row s = 1 + t.  Within a wave the new value of the
+// neighbour row from the previous diagonal comes from the neighbouring
+// lane by a whole-wave lane shift, so a diagonal costs no LDS round trip
+// and no barrier.  Consecutive waves run MXS_L diagonals apart (in the
+// sweep's direction the leading wave is the one whose rows come first):
+// the row at a wave's edge is passed through a small LDS ring, written one
+// diagonal after it is computed and read MXS_L diagonals later, and a
+// barrier every MXS_L steps orders every such write before its read (a
+// ring slot is reused only after 2 MXS_L steps).  All waves run the same
+// number of steps (the sweep's diagonals plus the lags); a row off its
+// diagonal computes on a clamped cell and stores to a trash row.  The
+// records of the diagonals ahead stream into a register ring MXS_P deep
+// (one load and at most two stores per diagonal keep that within the 63
+// memory operations a wave can have outstanding); the compiler waits for
+// every outstanding load at the loop's back edge, so the loop runs MXS_U
+// steps per iteration.
 #ifndef MXS_P
 #define MXS_P 16
 #endif
 #ifndef MXS_U
-#define MXS_U 64
+#define MXS_U 32
 #endif
+#define MXS_L 8      // steps between consecutive waves; a barrier every MXS_L steps
+#define MXS_RING 32  // ring slots per wave edge (> 2 MXS_L, a power of two)
 template <int C, bool FWD>
 __global__ __launch_bounds__(SGS_MAXT) void k_mx_sgs(MxLu a)
 {
-  extern __shared__ c2 xs[];
+  extern __shared__ c2 lds[];
   const int nslice = a.sk.ns[C];
   const int per = (nslice + 7) / 8;
   const int q = (blockIdx.x % 8) * per + blockIdx.x / 8;  // consecutive slices on one XCD
   const int r = blockIdx.y;
   if (q >= nslice) return;
   const int n1 = a.g.n1, n2 = a.g.n2;
-  const int t = threadIdx.x, nt = blockDim.x;
+  const int t = threadIdx.x, nt = blockDim.x, nw = nt / 64, wv = t / 64, lane = t % 64;
   const int A = a.sk.A[C], B = a.sk.B[C], AB = A * B;
   const int s = 1 + t, sc = min(s, A);
-  const int XW = nt + 2;  // exchange row: zero entries at both ends
-  float *cf = (float *)(xs + 2 * XW);
+  c2 *ring = lds;  // [nw][MXS_RING]: the edge row a wave receives
+  float *cf = (float *)(lds + nw * MXS_RING);
   int *dtab = (int *)(cf + (B + 2));
   {
     const float *g0 = (C == 0) ? (FWD ? a.c2m : a.c2p) : (FWD ? a.c1m : a.c1p);
     for (int u = t; u <= B + 1; u += nt) cf[u] = g0[min(u, B)];
     for (int d = t; d <= A + B; d += nt) dtab[d] = d >= 2 ? sk_off(d, A, B) - max(1, d - B) : 0;
-    for (int u = t; u < 2 * XW; u += nt) xs[u] = cmk(0, 0);
+    for (int u = t; u < nw * MXS_RING; u += nt) ring[u] = cmk(0, 0);
   }
   const float *gs = (C == 2) ? (FWD ? a.c2m : a.c2p) : (FWD ? a.c3m : a.c3p);
   const float cs = gs[sc];
@@ -628,37 +736,47 @@ __global__ __launch_bounds__(SGS_MAXT) void k_mx_sgs(MxLu a)
   c2 *Wr = a.W + r * a.fE + wofs + nb;
   c2 *yr = a.y + r * a.g.n + yb;
   c2 *tw = a.trash + r * SGS_MAXT + t, *ty = a.trash + (MX_RMAX + r) * SGS_MAXT + t;
+  // lag of this wave and the edges: forward, wave w receives row 64w from
+  // wave w-1's lane 63; backward, row 64w+65 from wave w+1's lane 0
+  const int lag = (FWD ? wv : nw - 1 - wv) * MXS_L;
+  const bool recv = FWD ? (lane == 0 && wv > 0) : (lane == 63 && wv < nw - 1);
+  const bool send = FWD ? (lane == 63 && wv < nw - 1) : (lane == 0 && wv > 0);
+  c2 *ring_in = ring + wv * MXS_RING, *ring_out = ring + (FWD ? wv + 1 : wv - 1) * MXS_RING;
   const int dlo = 2, dhi = A + B, nst = dhi - dlo + 1;
   __syncthreads();
   auto load = [&](float4 &c, int st) {
-    const int d = min(FWD ? dlo + st : dhi - st, dhi);
-    const long e = (long)min(max(dtab[max(d, dlo)] + sc, 0), AB - 1);
-    c = Rc[e];
+    const int d = min(max(FWD ? dlo + st : dhi - st, dlo), dhi);
+    c = Rc[min(max(dtab[d] + sc, 0), AB - 1)];
   };
-  float4 ring[MXS_P];
+  float4 rec[MXS_P];  // rec[tau % MXS_P] holds step tau - lag
 #pragma unroll
-  for (int u = 0; u < MXS_P; u++) load(ring[u], u);
+  for (int u = 0; u < MXS_P; u++) load(rec[u], u - lag);
   c2 prev = cmk(0, 0);
-  const int nstp = ((nst + MXS_U - 1) / MXS_U) * MXS_U;
-  for (int st0 = 0; st0 < nstp; st0 += MXS_U) {
+  c2 edge = ring_in[(-MXS_L - 1) & (MXS_RING - 1)];  // the edge value for step 0
+  const int ntk = ((nst + (nw - 1) * MXS_L + MXS_U - 1) / MXS_U) * MXS_U;
+  for (int tk0 = 0; tk0 < ntk; tk0 += MXS_U) {
 #pragma unroll
     for (int uu = 0; uu < MXS_U; uu++) {
-      const int st = st0 + uu, u = uu % MXS_P;
+      const int tau = tk0 + uu, u = uu % MXS_P, st = tau - lag;
       const int d = FWD ? dlo + st : dhi - st;
       const int f = d - s;
       const float cfv = cf[min(max(f, 0), B + 1)];
       const bool ok = s <= A && f >= 1 && f <= B;
       const long o = (long)f * FS;
-      const c2 nbv = xs[((st + 1) & 1) * XW + (FWD ? t : t + 2)];
-      const c2 w = cadd(cmk(ring[u].x, ring[u].y), crfma(crmul(cfv, prev), cs, nbv));
-      const c2 v = cmul(cmk(ring[u].z, ring[u].w), w);
+      // the neighbour row on the previous diagonal
+      c2 nbv = FWD ? cdpp<DPP_WAVE_SHR1>(prev) : cdpp<DPP_WAVE_SHL1>(prev);
+      nbv.x = recv ? edge.x : nbv.x;
+      nbv.y = recv ? edge.y : nbv.y;
+      edge = ring_in[(tau - MXS_L) & (MXS_RING - 1)];  // for the next step
+      const c2 w = cadd(cmk(rec[u].x, rec[u].y), crfma(crmul(cfv, prev), cs, nbv));
+      const c2 v = cmul(cmk(rec[u].z, rec[u].w), w);
       if (FWD || C != 0) *(ok ? Wr + o : tw) = v;
       if (!FWD) *(ok && kop >= 0 ? yr + o : ty) = v;
       prev.x = ok ? v.x : prev.x;
       prev.y = ok ? v.y : prev.y;
-      load(ring[u], st + MXS_P);
-      xs[(st & 1) * XW + t + 1] = prev;
-      __syncthreads();
+      if (send) ring_out[tau & (MXS_RING - 1)] = prev;
+      load(rec[u], st + MXS_P);
+      if (uu % MXS_L == MXS_L - 1) __syncthreads();
     }
   }
   if (!FWD) {  // operator-grid cells of this slice outside the sweep range are zero rows

@@ -130,7 +130,14 @@ typedef struct {
   int max_outer;     /* refinement steps (40) */
   int check_every;   /* inner iterations between convergence checks (8) */
   int use_graph;     /* replay a captured hipGraph of one inner iteration (1) */
+  int precond;       /* FP32 preconditioner: CSEM_PRECOND_LUSGS, the reference's
+                        lexicographic LU-SGS (fdfd.c:287-510), or
+                        CSEM_PRECOND_RBSGS, the same symmetric Gauss-Seidel
+                        splitting with the cells of each slice visited in
+                        red-black order (fully parallel passes) */
 } csem_mixed_opts;
+#define CSEM_PRECOND_LUSGS 0
+#define CSEM_PRECOND_RBSGS 1
 int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *x, const void *b, const double *tol,
                      const csem_mixed_opts *opt, int *iters, double *relres, void *stream);
 /* The FP32 operator and preconditioner on R complex-float vectors back to

@@ -7,7 +7,9 @@ Tolerances (stated here and in the assertions):
     fields (FP32 rounding of inputs and of the curl-curl differences);
   * FP32 LU-SGS vs FP64 LU-SGS: norm-wise relative 1e-4 (a preconditioner:
     only its action needs to be close);
-  * mixed solve vs the oracle's FP64 F-BiCGStab, both to tol = 1e-8/f:
+  * mixed solve (with either FP32 preconditioner: the lexicographic LU-SGS
+    or its red-black ordering) vs the oracle's FP64 F-BiCGStab, both to
+    tol = 1e-8/f:
     every receiver value within 1e-6 of the channel's largest value, and
     the true FP64 residual |b - A x| / |b| (oracle operator) below tol;
   * batched vs alone: the same receivers within 1e-6 (channel-normalised);
@@ -151,8 +153,9 @@ def _rec_err(a, b):
     return float((np.abs(a - b) / np.where(scale > 0, scale, 1)).max())
 
 
-def test_mixed_solve_matches_fp64_oracle(workdir):
-    print("case mixed vs oracle", flush=True)
+@pytest.mark.parametrize("precond", [_lib.PRECOND_LUSGS, _lib.PRECOND_RBSGS])
+def test_mixed_solve_matches_fp64_oracle(workdir, precond):
+    print("case mixed vs oracle, preconditioner %d" % precond, flush=True)
     s, ses, orc = _setup(workdir)
     try:
         bs, tols = [], []
@@ -160,7 +163,7 @@ def test_mixed_solve_matches_fp64_oracle(workdir):
             ses.set_freq(f)
             bs.append(ses.inject())
             tols.append(_tol(f))
-        xs, its, rel, nconv = ses.solve_mixed(bs, FREQS, tols)
+        xs, its, rel, nconv = ses.solve_mixed(bs, FREQS, tols, _lib.MixedOpts.default(precond=precond, max_inner=4000))
         assert nconv == 3, (its, rel)
         for r, f in enumerate(FREQS):
             orc.set_freq(f)
