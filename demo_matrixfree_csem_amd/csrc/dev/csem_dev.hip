@@ -147,6 +147,40 @@ static int set_lds_attrs()
   return 0;
 }
 
+// Planes per operator tile.  The tiles of one launch run in rounds of
+// (CUs x resident blocks per CU); a tile costs its planes plus about two
+// planes of prologue (the plane below it and the LDS set-up), so pick the
+// depth with the fewest planes-times-rounds.  The depth changes no value.
+template <int KC>
+static int apply_cost(const csem_model *m, int ncu, long *cost)
+{
+  int bpc = 0;
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_apply<KC>, 64 * AP_NW, 0));
+  const long tiles = (long)((m->n1 + 1 + AP_OI - 1) / AP_OI) * ((m->n2 + 1 + AP_OJ - 1) / AP_OJ) *
+                     ((m->n3 + 1 + KC - 1) / KC);
+  const long slots = (long)ncu * (bpc > 0 ? bpc : 1);
+  *cost = ((tiles + slots - 1) / slots) * (KC + 2);
+  return 0;
+}
+static int pick_apply_planes(csem_model *m)
+{
+  int dev = 0, ncu = 0;
+  HIP_OK(hipGetDevice(&dev));
+  HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int kcs[5] = {6, 8, 10, 12, 16};
+  long cost[5];
+  int rc;
+  if ((rc = apply_cost<6>(m, ncu, &cost[0])) || (rc = apply_cost<8>(m, ncu, &cost[1])) ||
+      (rc = apply_cost<10>(m, ncu, &cost[2])) || (rc = apply_cost<12>(m, ncu, &cost[3])) ||
+      (rc = apply_cost<16>(m, ncu, &cost[4])))
+    return rc;
+  int best = 1;
+  for (int c = 0; c < 5; c++)
+    if (cost[c] < cost[best]) best = c;
+  m->kc = kcs[best];
+  return 0;
+}
+
 extern "C" void csem_model_destroy(csem_model *m)
 {
   if (!m) return;
@@ -179,7 +213,7 @@ extern "C" int csem_model_create(const csem_model_desc *d, csem_model **out)
   m->n1 = d->n1; m->n2 = d->n2; m->n3f = d->n3f; m->nb = d->nb; m->airbc = d->airbc; m->fvm = d->fvm;
   m->kofs = d->airbc ? d->nb : 0;
   m->n3 = d->n3f - m->kofs;
-  m->kc = 8;
+  m->kc = 0;  // planes per operator tile: chosen below
   const int n1 = m->n1, n2 = m->n2, n3 = m->n3, n3f = m->n3f;
   m->lEx = (long)n1 * (n2 + 1) * (n3 + 1);
   m->lEy = (long)(n1 + 1) * n2 * (n3 + 1);
@@ -264,6 +298,7 @@ extern "C" int csem_model_create(const csem_model_desc *d, csem_model **out)
   }
 #undef UP
   if ((rc = set_lds_attrs())) goto fail;
+  if ((rc = pick_apply_planes(m))) goto fail;
   *out = m;
   return 0;
 fail:
@@ -293,7 +328,7 @@ static ApplyArgs apply_args(const csem_model *m, const csem_freq *f, const z2 *x
   a.iwmu = zmk(f->iwmu_re, f->iwmu_im);
   a.iomu = zmk(f->iomu_re, f->iomu_im);
   a.n1 = n1; a.n2 = n2; a.n3 = m->n3; a.fvm = m->fvm; a.airbc = m->airbc;
-  a.nti = (n1 + 1 + AP_OI - 1) / AP_OI; a.ntj = (n2 + 1 + AP_OJ - 1) / AP_OJ; a.ntk = (m->n3 + 1 + AP_KC - 1) / AP_KC;
+  a.nti = (n1 + 1 + AP_OI - 1) / AP_OI; a.ntj = (n2 + 1 + AP_OJ - 1) / AP_OJ; a.ntk = (m->n3 + 1 + m->kc - 1) / m->kc;
   a.lEx = m->lEx; a.lEy = m->lEy;
   return a;
 }
@@ -347,8 +382,14 @@ int run_apply_bufs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStr
     if (rc) return rc;
   }
   const int ntiles = a.nti * a.ntj * a.ntk;
-  dim3 blk(64, AP_NW);
-  KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<<<8 * ((ntiles + 7) / 8), blk, 0, s>>>(a)));
+  const dim3 blk(64, AP_NW), grd(8 * ((ntiles + 7) / 8));
+  switch (m->kc) {
+    case 6: KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<6><<<grd, blk, 0, s>>>(a))); break;
+    case 10: KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<10><<<grd, blk, 0, s>>>(a))); break;
+    case 12: KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<12><<<grd, blk, 0, s>>>(a))); break;
+    case 16: KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<16><<<grd, blk, 0, s>>>(a))); break;
+    default: KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<8><<<grd, blk, 0, s>>>(a))); break;
+  }
   HIP_OK(hipGetLastError());
   return 0;
 }

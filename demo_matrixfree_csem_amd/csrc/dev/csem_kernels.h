@@ -22,7 +22,7 @@ struct ApplyArgs {
 };
 
 // y = A x (fdfd.c:86-205).  A block is AP_NW waves over a tile of rows
-// and columns and marches over AP_KC planes: wave w covers row j = j0-1+w,
+// and columns and marches over KC planes: wave w covers row j = j0-1+w,
 // lane l column i = i0-1+l.  Waves 0 and AP_NW-1 and lanes 0 and 63 are
 // halos: they load E and compute H for their neighbours but write no
 // output.  Per plane each lane loads its own E two planes ahead (and sigma
@@ -38,14 +38,13 @@ struct ApplyArgs {
 // precomputed reciprocal with one exact correction (ddiv), so each value
 // rounds as the reference's division does.
 #define AP_NW 8   // waves per block
-#define AP_KC 8   // planes per block
 #define AP_OI 62  // output columns per block
 #define AP_OJ (AP_NW - 2)  // output rows per block
-#ifndef CSEM_TYPES_ONLY
+template <int KC>
 __global__ __launch_bounds__(64 * AP_NW) void k_apply(ApplyArgs a)
 {
   __shared__ z2 sEx[AP_NW][64], sEz[AP_NW][64], sHx[AP_NW][64], sHz[AP_NW][64];
-  __shared__ double sD[AP_KC][4];  // d3s, 1/d3s, d3, 1/d3 of the tile's planes
+  __shared__ double sD[KC][4];  // d3s, 1/d3s, d3, 1/d3 of the tile's planes
   const int ntiles = a.nti * a.ntj * a.ntk, per = (ntiles + 7) / 8;
   const int T = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
   if (T >= ntiles) return;
@@ -53,7 +52,7 @@ __global__ __launch_bounds__(64 * AP_NW) void k_apply(ApplyArgs a)
   const int l = threadIdx.x, w = threadIdx.y;
   const int i = ti * AP_OI - 1 + l, j = tj * AP_OJ - 1 + w;
   const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
-  const int k0 = tk * AP_KC, k1 = min(k0 + AP_KC, n3 + 1);
+  const int k0 = tk * KC, k1 = min(k0 + KC, n3 + 1);
   const bool live = i >= 0 && i <= n1 && j >= 0 && j <= n2;
   const bool out = live && l >= 1 && l <= AP_OI && w >= 1 && w <= AP_OJ;
   const bool hx_ok = live && j < n2, hy_ok = live && i < n1, hz_ok = hy_ok && j < n2;
@@ -91,7 +90,7 @@ __global__ __launch_bounds__(64 * AP_NW) void k_apply(ApplyArgs a)
   // the planes' spacings through LDS: read per plane as uniform values
   // they would be vector loads that the compiler waits for together with
   // the prefetched planes
-  if (w == 0 && l < 4 * AP_KC) {
+  if (w == 0 && l < 4 * KC) {
     const int kq = min(k0 + l / 4, n3), c = l % 4;
     const double *src = c == 0 ? a.d3s : c == 1 ? a.r3s : c == 2 ? a.d3 : a.r3;
     sD[l / 4][c] = src[kq];
@@ -125,7 +124,7 @@ __global__ __launch_bounds__(64 * AP_NW) void k_apply(ApplyArgs a)
   sEz[w][l] = e[0].z;
   __syncthreads();
 #pragma unroll
-  for (int kk = 0; kk < AP_KC; kk++) {
+  for (int kk = 0; kk < KC; kk++) {
     const int k = k0 + kk;
     const EV cur = e[kk % 3], nxt = e[(kk + 1) % 3];
     e[(kk + 2) % 3] = ldE(k + 2);
@@ -186,7 +185,6 @@ __global__ __launch_bounds__(64 * AP_NW) void k_apply(ApplyArgs a)
     hy_prev = hy;
   }
 }
-#endif
 
 // H = curl E / (i w mu0) on all faces (fdfd.c:125-149), Hx | Hy | Hz.
 #ifndef CSEM_TYPES_ONLY

@@ -310,14 +310,18 @@ static void mx_apply_tile(const csem_model *m, const MixState *x, const MixSlot 
                           c2 *out, int rows, int kc, hipStream_t s)
 {
   constexpr int NB = R > 3 ? 6 : 8;  // the larger block
-  const bool big = rows + 2 >= NB;
-  if (kc >= 16) {
-    if (big) mx_apply_launch<R, NB, 16>(m, x, q, fr, in, out, s);
-    else mx_apply_launch<R, NB - 2, 16>(m, x, q, fr, in, out, s);
-  } else {
-    if (big) mx_apply_launch<R, NB, 8>(m, x, q, fr, in, out, s);
-    else mx_apply_launch<R, NB - 2, 8>(m, x, q, fr, in, out, s);
+#define MX_AP(KC)                                                   \
+  do {                                                              \
+    if (rows + 2 >= NB) mx_apply_launch<R, NB, KC>(m, x, q, fr, in, out, s);     \
+    else mx_apply_launch<R, NB - 2, KC>(m, x, q, fr, in, out, s);   \
+  } while (0)
+  switch (kc) {
+    case 10: MX_AP(10); break;
+    case 12: MX_AP(12); break;
+    case 16: MX_AP(16); break;
+    default: MX_AP(8); break;
   }
+#undef MX_AP
 }
 static void mx_apply_kernel(const csem_model *m, const MixState *x, const MixSlot *q, int R, const MxFreq &fr,
                             const c2 *in, c2 *out, int rows, int kc, hipStream_t s)
@@ -342,14 +346,14 @@ static int mx_tune(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &
   if (x->ap_tj[R]) return 0;
   int TJS[2];
   mx_tile_rows(R, TJS);
-  const int KCS[2] = {8, 16};
+  const int KCS[4] = {8, 10, 12, 16};
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return csem_fail(-1, "event");
   float best = 1e30f;
   int bt = TJS[1], bk = 8;
   kt_pause(1);
   for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < 4; b++) {
       mx_apply_kernel(m, x, q, R, fr, in, out, TJS[a], KCS[b], s);  // warm
       (void)hipEventRecord(e0, s);
       for (int w = 0; w < 3; w++) mx_apply_kernel(m, x, q, R, fr, in, out, TJS[a], KCS[b], s);
@@ -795,7 +799,7 @@ extern "C" int csem_mixed_set_tile(csem_model *m, int R, int rows, int planes)
   if (R < 1 || R > MX_RMAX) return csem_fail(-3, "batch of %d right-hand sides outside [1, %d]", R, MX_RMAX);
   int ok_rows[2];
   mx_tile_rows(R, ok_rows);
-  if (!(rows == ok_rows[0] || rows == ok_rows[1]) || !(planes == 8 || planes == 16))
+  if (!(rows == ok_rows[0] || rows == ok_rows[1]) || !(planes == 8 || planes == 10 || planes == 12 || planes == 16))
     return csem_fail(-3, "tile %d x %d not available for %d right-hand sides", rows, planes, R);
   {
     std::lock_guard<std::mutex> lk(g_mx_init_mu);

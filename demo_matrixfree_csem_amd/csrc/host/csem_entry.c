@@ -21,6 +21,8 @@
 #define CSEM_MU0 (4. * CSEM_PI * 1e-7)    /* cstd.h:38-40 */
 
 void csem_set_status(int st);
+/* records an error message for csem_last_error() and returns code (device library) */
+int csem_fail(int code, const char *fmt, ...);
 
 /* vector primitives exported by the device library */
 int csem_vec_sub(long n, const void *a, const void *b, void *o, void *stream);

@@ -149,7 +149,7 @@ int csem_lusgs32(csem_model *m, int R, const csem_freq *f, const void *x, void *
 int csem_mixed_tuning(csem_model *m, int R, int *rows, int *planes);
 /* Fix the FP32 operator's tile at batch size R instead of timing the
  * candidates: output rows per block 4 or 6 (R <= 3), 2 or 4 (R = 4);
- * planes per block 8 or 16. */
+ * planes per block 8, 10, 12 or 16. */
 int csem_mixed_set_tile(csem_model *m, int R, int rows, int planes);
 /* device bytes held by the mixed-precision state of a model (its FP32
  * copy and every solve slot), and the bytes one more slot for R

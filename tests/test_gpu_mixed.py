@@ -107,7 +107,7 @@ def test_apply32_every_tile(workdir):
                 orc.set_freq(f)
                 ref.append(orc.apply(x[r].astype(np.complex128)))
             for rows in ((4, 6) if R <= 3 else (2, 4)):
-                for planes in (8, 16):
+                for planes in (8, 10, 12, 16):
                     _lib.check(L.csem_mixed_set_tile(ses.model, R, rows, planes), "set_tile")
                     _lib.check(L.csem_apply32(ses.model, R, fr, dx, dy, None), "apply32")
                     y = _host(dy, x)
