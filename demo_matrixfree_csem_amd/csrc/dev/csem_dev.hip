@@ -360,7 +360,7 @@ static int run_airbc(csem_model *m, hipStream_t s, const AirBufs &b)
   const size_t row_lds = 2 * (size_t)N * sizeof(z2);
   const double NN = (double)N * N;
   KT(KT_AIR_ROWS_FWD, s, 16 * NN, (k_air_rows_fwd<<<N, 256, row_lds, s>>>(a)));
-  KT(KT_AIR_COLS, s, 80 * NN, (k_air_cols<2><<<N / 2, 256, 3 * 2 * (size_t)N * sizeof(z2), s>>>(a)));
+  KT(KT_AIR_COLS, s, 80 * NN, (k_air_cols<2><<<8 * ((N / 2 + 7) / 8), 256, 3 * 2 * (size_t)N * sizeof(z2), s>>>(a)));
   KT(KT_AIR_ROWS_INV, s, 32.0 * N * (m->nrowx + m->nrowy),
      (k_air_rows_inv<<<m->nrowx + m->nrowy, 256, row_lds, s>>>(a)));
   const int ng = (m->n1 + 1) * m->n2 + m->n1 * (m->n2 + 1);

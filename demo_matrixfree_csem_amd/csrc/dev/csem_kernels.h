@@ -731,7 +731,11 @@ template <int CW>
 __global__ void k_air_cols(AirArgs a)
 {
   extern __shared__ z2 sm[];
-  const int N = a.N, c0 = blockIdx.x * CW;
+  // column groups: consecutive groups on one XCD, so the neighbouring
+  // strips of a row's cache lines are read and written through its L2
+  const int ng = a.N / CW, per = (ng + 7) / 8, g = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (g >= ng) return;
+  const int N = a.N, c0 = g * CW;
   z2 *S = sm, *A = sm + N * CW, *B = sm + 2 * N * CW;
   for (int e = threadIdx.x; e < N * CW; e += blockDim.x) {
     int c = e % CW, i2 = e / CW;

@@ -333,7 +333,11 @@ __global__ void k_mx_air_rows(MxAir a)
 __global__ __launch_bounds__(256) void k_mx_air_cols(MxAir a)
 {
   extern __shared__ c2 smc[];
-  const int N = a.N, cw = a.cw, c0 = blockIdx.x * cw, r = blockIdx.y;
+  // column groups: consecutive groups on one XCD, so the neighbouring
+  // strips of a row's cache lines are read and written through its L2
+  const int ng = a.N / a.cw, per = (ng + 7) / 8, g = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (g >= ng) return;
+  const int N = a.N, cw = a.cw, c0 = g * cw, r = blockIdx.y;
   const long off = r * (long)N * N;
   c2 *A = smc, *B = smc + N * cw;
   const int t = threadIdx.x, nt = blockDim.x;
