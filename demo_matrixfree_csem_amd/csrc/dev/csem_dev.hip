@@ -414,11 +414,12 @@ static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   const SkewGeo &g = m->g;
   dim3 gp[3], gq[3];
   for (int c = 0; c < 3; c++) gp[c] = dim3((unsigned)(((long)g.A[c] * g.B[c] + 255) / 256), (unsigned)g.ns[c]);
-  // pre-pass: the component's reference layout, 64 x 4 threads per block
-  gq[0] = dim3((m->n1 + 63) / 64, (m->n2 + 1 + 3) / 4, m->n3f + 1);
-  gq[1] = dim3((m->n1 + 1 + 63) / 64, (m->n2 + 3) / 4, m->n3f + 1);
-  gq[2] = dim3((m->n1 + 1 + 63) / 64, (m->n2 + 1 + 3) / 4, m->n3f);
-  const dim3 bq(64, 4);
+  // pre-pass tiles (LuPreTile): Ex 16 slices x 4 rows x 8 columns, Ey and
+  // Ez 8 rows x 64 columns, 256 threads
+  gq[0] = dim3((g.ns[0] + 15) / 16, (g.B[0] + 7) / 8, (g.A[0] + 3) / 4);
+  gq[1] = dim3((g.B[1] + 63) / 64, (g.A[1] + 7) / 8, g.ns[1]);
+  gq[2] = dim3((g.B[2] + 63) / 64, (g.A[2] + 7) / 8, g.ns[2]);
+  const dim3 bq(256);
   if (!m->smith_ok || memcmp(&m->smith_f, f, sizeof *f)) {  // diagonal of this frequency
     KT(KT_LU_SMITH, s, 48.0 * m->nsk, (k_lu_smith<0><<<gp[0], 256, 0, s>>>(a)));
     KT(KT_LU_SMITH, s, 0.0, (k_lu_smith<1><<<gp[1], 256, 0, s>>>(a)));

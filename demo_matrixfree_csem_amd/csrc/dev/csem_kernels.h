@@ -391,67 +391,112 @@ __device__ __forceinline__ z2 smith_finish(z2 n, double4 S)
 #define WZ(k, j, i) a.W[a.fEx + a.fEy + id3(n1, n2, k, j, i)]
 
 // Pre-pass of sweep (C, FWD): the record R and the right-hand side of every
-// swept cell, written to its skewed position.  Threads walk the cells in
-// the reference layout, so the neighbour gathers from W coalesce.  In the
-// forward pass, components swept later are still zero (fdfd.c:338-343); W
-// holds forward values, overwritten by the backward values of Ez and Ey as
-// those sweeps finish.
+// swept cell.  A block computes a tile of cells in the reference layout
+// (x fastest, so the neighbour gathers from W coalesce), places each record
+// in LDS at its position in the tile's skewed order together with its
+// global index, and stores the tile's records as consecutive 16-byte units
+// (each diagonal's run of the tile is contiguous in the skewed layout).  In
+// the forward pass, components swept later are still zero (fdfd.c:338-343);
+// W holds forward values, overwritten by the backward values of Ez and Ey
+// as those sweeps finish.  Tiles: Ex 16 slices x 4 rows x 8 columns (its
+// slices run along x), Ey and Ez 8 rows x 64 columns of one slice.
+template <int C> struct LuPreTile { static constexpr int TQ = 1, TS = 8, TF = 64; };
+template <> struct LuPreTile<0> { static constexpr int TQ = 16, TS = 4, TF = 8; };
 template <int C, bool FWD>
 __global__ __launch_bounds__(256) void k_lu_pre(LuArgs a)
 {
+  using TL = LuPreTile<C>;
+  constexpr int TQ = TL::TQ, TS = TL::TS, TF = TL::TF, NC = TQ * TS * TF, NU = 5;  // 16-byte units per record
+  __shared__ z2 tile[NC * NU];
+  __shared__ long pidx[NC];  // global record index of each slot, -1 off the slices
   const int n1 = a.n1, n2 = a.n2, n3 = a.n3f;
-  const int nx = (C == 0) ? n1 : n1 + 1, ny = (C == 1) ? n2 : n2 + 1;
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= nx || j >= ny) return;
-  int q, s, f;
-  if (C == 0) { q = i; s = k; f = j; if (!(k >= 1 && k <= n3 - 1 && j >= 1 && j <= n2 - 1)) return; }
-  else if (C == 1) { q = j; s = k; f = i; if (!(k >= 1 && k <= n3 - 1 && i >= 1 && i <= n1 - 1)) return; }
-  else { q = k; s = j; f = i; if (!(j >= 1 && j <= n2 - 1 && i >= 1 && i <= n1 - 1)) return; }
-  const int A = a.g.A[C], B = a.g.B[C], d = s + f;
-  const int p = sk_off(d, A, B) + (s - max(1, d - B));
-  const int kop = k - a.kofs;
-  const long xo = (C == 0) ? id1(n1, n2, kop, j, i) : (C == 1) ? a.lEx + id2(n1, n2, kop, j, i)
-                                                                 : a.lEx + a.lEy + id3(n1, n2, kop, j, i);
-  LuRec &out = a.rec[(long)q * a.g.A[C] * a.g.B[C] + p];
-  out.b = kop >= 0 ? a.x[xo] : zmk(0, 0);
-  if (C == 0 && FWD) return;  // Ex forward: Ey = Ez = 0, every record term is +0
+  const int A = a.g.A[C], B = a.g.B[C], ns = a.g.ns[C];
+  const long AB = (long)A * B;
+  const int t = threadIdx.x;
+  int q0, s0, f0;
+  if (C == 0) { q0 = blockIdx.x * TQ; f0 = 1 + blockIdx.y * TF; s0 = 1 + blockIdx.z * TS; }
+  else { f0 = 1 + blockIdx.x * TF; s0 = 1 + blockIdx.y * TS; q0 = blockIdx.z; }
   const z2 z0 = zmk(0, 0);
-  z2 P1, P2, Q2, Q3, P4, Q4;
-  if (C == 0) {  // fdfd.c:483-486: Ey, Ez backward, Ex forward
-    const double d1s_i = a.d1s[i], r1s_i = a.r1s[i];
-    P1 = zdivq(zsub(WY(k, j, i + 1), WY(k, j, i)), d1s_i, r1s_i);
-    P2 = zdivq(zsub(WY(k, j - 1, i + 1), WY(k, j - 1, i)), d1s_i, r1s_i);
-    Q2 = zdivq(zsub(z0, WX(k, j - 1, i)), a.d2s[j - 1], a.r2s[j - 1]);
-    Q3 = zdivq(zsub(WZ(k, j, i + 1), WZ(k, j, i)), d1s_i, r1s_i);
-    P4 = zdivq(zsub(z0, WX(k - 1, j, i)), a.d3s[k - 1], a.r3s[k - 1]);
-    Q4 = zdivq(zsub(WZ(k - 1, j, i + 1), WZ(k - 1, j, i)), d1s_i, r1s_i);
-  } else if (C == 1) {  // fdfd.c:384-387 / 458-461: Ez (zero forward, backward after), Ex forward
-    const double d2s_j = a.d2s[j], r2s_j = a.r2s[j];
-    const z2 ez_jp = FWD ? z0 : WZ(k, j + 1, i), ez_j = FWD ? z0 : WZ(k, j, i);
-    const z2 ez_km_jp = FWD ? z0 : WZ(k - 1, j + 1, i), ez_km_j = FWD ? z0 : WZ(k - 1, j, i);
-    P1 = zdivq(zsub(ez_jp, ez_j), d2s_j, r2s_j);
-    P2 = zdivq(zsub(ez_km_jp, ez_km_j), d2s_j, r2s_j);
-    Q3 = zdivq(zsub(WX(k, j + 1, i), WX(k, j, i)), d2s_j, r2s_j);
-    Q4 = zdivq(zsub(WX(k, j + 1, i - 1), WX(k, j, i - 1)), d2s_j, r2s_j);
-    if (!FWD) {
-      Q2 = zdivq(zsub(z0, WY(k - 1, j, i)), a.d3s[k - 1], a.r3s[k - 1]);
-      P4 = zdivq(zsub(z0, WY(k, j, i - 1)), a.d1s[i - 1], a.r1s[i - 1]);
+#pragma unroll
+  for (int it = 0; it < NC / 256; it++) {
+    int lq, ls, lf;
+    if (C == 0) { lq = t % TQ; lf = (t / TQ) % TF; ls = t / (TQ * TF) + it * (256 / (TQ * TF)); }
+    else { lq = 0; lf = t % TF; ls = t / TF + it * (256 / TF); }
+    const int q = q0 + lq, s = s0 + ls, f = f0 + lf;
+    const int dl = ls + lf + 2;
+    const int cell = lq * TS * TF + sk_off(dl, TS, TF) + (ls + 1 - max(1, dl - TF));
+    if (q >= ns || s > A || f > B) {
+      pidx[cell] = -1;
+      continue;
     }
-  } else {  // fdfd.c:410-413 / 435-438: Ex, Ey forward
-    const double d3s_k = a.d3s[k], r3s_k = a.r3s[k];
-    P1 = zdivq(zsub(WX(k + 1, j, i), WX(k, j, i)), d3s_k, r3s_k);
-    P2 = zdivq(zsub(WX(k + 1, j, i - 1), WX(k, j, i - 1)), d3s_k, r3s_k);
-    Q3 = zdivq(zsub(WY(k + 1, j, i), WY(k, j, i)), d3s_k, r3s_k);
-    Q4 = zdivq(zsub(WY(k + 1, j - 1, i), WY(k, j - 1, i)), d3s_k, r3s_k);
-    if (!FWD) {
-      Q2 = zdivq(zsub(z0, WZ(k, j, i - 1)), a.d1s[i - 1], a.r1s[i - 1]);
-      P4 = zdivq(zsub(z0, WZ(k, j - 1, i)), a.d2s[j - 1], a.r2s[j - 1]);
+    {
+      const int d = s + f;
+      pidx[cell] = q * AB + sk_off(d, A, B) + (s - max(1, d - B));
+    }
+    int k, j, i;
+    lu_kji<C>(q, s, f, k, j, i);
+    const int kop = k - a.kofs;
+    const long xo = (C == 0) ? id1(n1, n2, kop, j, i) : (C == 1) ? a.lEx + id2(n1, n2, kop, j, i)
+                                                                   : a.lEx + a.lEy + id3(n1, n2, kop, j, i);
+    z2 *out = tile + cell * NU;
+    out[4] = kop >= 0 ? a.x[xo] : z0;
+    if (C == 0 && FWD) continue;  // Ex forward: Ey = Ez = 0, every record term is +0
+    z2 P1, P2, Q2, Q3, P4, Q4;
+    if (C == 0) {  // fdfd.c:483-486: Ey, Ez backward, Ex forward
+      const double d1s_i = a.d1s[i], r1s_i = a.r1s[i];
+      P1 = zdivq(zsub(WY(k, j, i + 1), WY(k, j, i)), d1s_i, r1s_i);
+      P2 = zdivq(zsub(WY(k, j - 1, i + 1), WY(k, j - 1, i)), d1s_i, r1s_i);
+      Q2 = zdivq(zsub(z0, WX(k, j - 1, i)), a.d2s[j - 1], a.r2s[j - 1]);
+      Q3 = zdivq(zsub(WZ(k, j, i + 1), WZ(k, j, i)), d1s_i, r1s_i);
+      P4 = zdivq(zsub(z0, WX(k - 1, j, i)), a.d3s[k - 1], a.r3s[k - 1]);
+      Q4 = zdivq(zsub(WZ(k - 1, j, i + 1), WZ(k - 1, j, i)), d1s_i, r1s_i);
+    } else if (C == 1) {  // fdfd.c:384-387 / 458-461: Ez (zero forward, backward after), Ex forward
+      const double d2s_j = a.d2s[j], r2s_j = a.r2s[j];
+      const z2 ez_jp = FWD ? z0 : WZ(k, j + 1, i), ez_j = FWD ? z0 : WZ(k, j, i);
+      const z2 ez_km_jp = FWD ? z0 : WZ(k - 1, j + 1, i), ez_km_j = FWD ? z0 : WZ(k - 1, j, i);
+      P1 = zdivq(zsub(ez_jp, ez_j), d2s_j, r2s_j);
+      P2 = zdivq(zsub(ez_km_jp, ez_km_j), d2s_j, r2s_j);
+      Q3 = zdivq(zsub(WX(k, j + 1, i), WX(k, j, i)), d2s_j, r2s_j);
+      Q4 = zdivq(zsub(WX(k, j + 1, i - 1), WX(k, j, i - 1)), d2s_j, r2s_j);
+      if (!FWD) {
+        Q2 = zdivq(zsub(z0, WY(k - 1, j, i)), a.d3s[k - 1], a.r3s[k - 1]);
+        P4 = zdivq(zsub(z0, WY(k, j, i - 1)), a.d1s[i - 1], a.r1s[i - 1]);
+      }
+    } else {  // fdfd.c:410-413 / 435-438: Ex, Ey forward
+      const double d3s_k = a.d3s[k], r3s_k = a.r3s[k];
+      P1 = zdivq(zsub(WX(k + 1, j, i), WX(k, j, i)), d3s_k, r3s_k);
+      P2 = zdivq(zsub(WX(k + 1, j, i - 1), WX(k, j, i - 1)), d3s_k, r3s_k);
+      Q3 = zdivq(zsub(WY(k + 1, j, i), WY(k, j, i)), d3s_k, r3s_k);
+      Q4 = zdivq(zsub(WY(k + 1, j - 1, i), WY(k, j - 1, i)), d3s_k, r3s_k);
+      if (!FWD) {
+        Q2 = zdivq(zsub(z0, WZ(k, j, i - 1)), a.d1s[i - 1], a.r1s[i - 1]);
+        P4 = zdivq(zsub(z0, WZ(k, j - 1, i)), a.d2s[j - 1], a.r2s[j - 1]);
+      }
+    }
+    if (FWD) {  // Q1 = (0 - 0)/d = +0 and P3 = +0
+      out[0] = zsub(P1, z0); out[1] = P2; out[2] = zsub(z0, Q3); out[3] = Q4;
+    } else {
+      out[0] = P1; out[1] = zsub(P2, Q2); out[2] = Q3; out[3] = zsub(P4, Q4);
     }
   }
-  if (FWD) {  // Q1 = (0 - 0)/d = +0 and P3 = +0
-    out.R = z2x4{zsub(P1, z0), P2, zsub(z0, Q3), Q4};
+  __syncthreads();
+  // store the tile's records unit by unit (Ex forward: only b, the unit its
+  // sweep reads)
+  z2 *rec = (z2 *)a.rec;
+  if (C == 0 && FWD) {
+#pragma unroll
+    for (int it = 0; it < NC / 256; it++) {
+      const int e = t + it * 256;
+      const long p = pidx[e];
+      if (p >= 0) rec[p * NU + 4] = tile[e * NU + 4];
+    }
   } else {
-    out.R = z2x4{P1, zsub(P2, Q2), Q3, zsub(P4, Q4)};
+#pragma unroll
+    for (int it = 0; it < NC * NU / 256; it++) {
+      const int u = t + it * 256, e = u / NU, c = u % NU;
+      const long p = pidx[e];
+      if (p >= 0) rec[p * NU + c] = tile[u];
+    }
   }
 }
 #undef WX
