@@ -43,6 +43,16 @@ METRIC = "solves/s @ tol 1e-8: batched mixed vs plain FP64 GPU; apply HBM TB/s; 
 HBM_PEAK_GBS = 8000.0
 
 
+def _traffic():
+    """HBM bytes per launch by kernel, from the FETCH_SIZE / WRITE_SIZE
+    counter runs summarised in profiles/hbm_traffic.json (absent: None)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch", {})
+
+
 def _roofline(stats, name):
     if name not in stats or stats[name][1] == 0:
         return None
@@ -50,8 +60,9 @@ def _roofline(stats, name):
     avg_s = ms / cnt / 1e3
     per_launch = by / cnt
     gbs = per_launch / avg_s / 1e9
+    tr = _traffic().get(name)
     return {"kernel": name, "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": int(tr) if tr else None,
             "bytes_per_launch": int(per_launch), "avg_us": round(avg_s * 1e6, 3), "launches": cnt}
 
 

@@ -513,8 +513,12 @@ __global__ __launch_bounds__(256) void k_lu_pre(LuArgs a)
 // Dynamic LDS: two exchange rows of blockDim+2 complex values, the fast
 // table (B+2 double4), the volume table (B+2 doubles) and the diagonal
 // table (A+B+1 ints).
+#ifndef SGS_P
 #define SGS_P 4   // register ring depth (diagonals loaded ahead)
+#endif
+#ifndef SGS_U
 #define SGS_U 16  // diagonals per loop iteration
+#endif
 #define SGS_MAXT 256  // rows per slice: n3f-1 and n2-1 must not exceed this
 
 template <int C, bool FWD, bool FVM>
