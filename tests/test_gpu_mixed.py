@@ -143,6 +143,44 @@ def test_lusgs32_close_to_fp64(workdir):
         ses.close()
 
 
+def test_fp32_kernels_multi_tile(workdir):
+    """Several operator tiles along x and y, LU-SGS slices of more than 64
+    rows (several waves per slice, lagged edge exchange): the FP32 operator
+    and LU-SGS stay within their tolerances of the FP64 oracle."""
+    print("case fp32 multi-tile", flush=True)
+    s = surveys.tiny(5, airbc=1, nx=130, ny=75, nz=70)
+    s.flags.update(n1fft=128, n2fft=128)
+    s.freqs = list(FREQS)
+    surveys.write_files(workdir, s)
+    ses = Session(s.argv(workdir))
+    m = s.model
+    orc = O.Oracle(m.x1node, m.x2node, m.x3node, m.rho11, m.rho22, m.rho33, nb=s.flags["nb"], airbc=1, nfft=128)
+    L = _lib.lib()
+    try:
+        n, R = ses.n, 3
+        rng = np.random.default_rng(17)
+        x = (rng.standard_normal((R, n)) + 1j * rng.standard_normal((R, n))).astype(np.complex64)
+        fr = (_lib.FreqT * R)(*[freq_consts(f) for f in FREQS])
+        dx, dy = _dev(x), _dev(np.zeros_like(x))
+        _lib.check(L.csem_apply32(ses.model, R, fr, dx, dy, None), "apply32")
+        ya = _host(dy, x)
+        _lib.check(L.csem_lusgs32(ses.model, R, fr, dx, dy, None), "lusgs32")
+        yl = _host(dy, x)
+        for r, f in enumerate(FREQS):
+            orc.set_freq(f)
+            xr = x[r].astype(np.complex128)
+            ya64, yl64 = orc.apply(xr), orc.lusgs(xr)
+            ea = np.linalg.norm(ya[r] - ya64) / np.linalg.norm(ya64)
+            el = np.linalg.norm(yl[r] - yl64) / np.linalg.norm(yl64)
+            print("f=%g apply32 %.2e lusgs32 %.2e" % (f, ea, el), flush=True)
+            assert ea < 2e-6, (f, ea)
+            assert el < 1e-4, (f, el)
+        L.csem_free(dx)
+        L.csem_free(dy)
+    finally:
+        ses.close()
+
+
 def _receivers(ses, x):
     return ses.extract(x).astype(np.complex128)
 

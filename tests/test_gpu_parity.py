@@ -67,6 +67,32 @@ def test_lusgs_bitwise(workdir, airbc, fvm):
         ses.close()
 
 
+# A model with several operator tiles along x and y and LU-SGS slices of
+# more than 64 rows (several waves per slice): the tile halos and the
+# wave-to-wave exchange of the sweeps are exercised (nz <= ny, see tiny()).
+MEDIUM = dict(nx=130, ny=75, nz=70)
+
+
+@pytest.mark.parametrize("airbc", [0, 1])
+def test_apply_lusgs_bitwise_multi_tile(workdir, airbc):
+    print("case apply+lusgs multi-tile airbc=%d" % airbc, flush=True)
+    s, ses, orc = _pair(workdir, airbc, seed=5, nfft=128, **MEDIUM)
+    try:
+        e = ses.emf
+        assert e.n1 + 1 > 2 * 62 and min(e.nz + 2 * e.nb, e.n2) - 1 > 64, (e.n1, e.n2, e.nz)
+        ses.set_freq(0.75)
+        orc.set_freq(0.75)
+        x = _rand(ses.n, 21)
+        y = ses.fdfd_apply(x)
+        yo = orc.apply(x)
+        assert np.array_equal(y, yo), np.abs(y - yo).max() / np.abs(yo).max()
+        y = ses.lusgs_apply(x)
+        yo = orc.lusgs(x)
+        assert np.array_equal(y, yo), np.abs(y - yo).max() / np.abs(yo).max()
+    finally:
+        ses.close()
+
+
 def test_airbc_bitwise(workdir):
     print("case airbc", flush=True)
     s, ses, orc = _pair(workdir, 1)
