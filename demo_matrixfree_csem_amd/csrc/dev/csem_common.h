@@ -85,6 +85,15 @@ struct __align__(8) c2 {
 __host__ __device__ __forceinline__ c2 cmk(float x, float y) { return c2{x, y}; }
 template <int CTRL>
 __device__ __forceinline__ c2 cdpp(c2 v) { return c2{dpp_f<CTRL>(v.x), dpp_f<CTRL>(v.y)}; }
+// The same shift, but a lane with no source lane keeps `old` (its lane of
+// the first argument): one instruction per component shifts a wave's values
+// and feeds its edge lane from elsewhere.
+template <int CTRL>
+__device__ __forceinline__ c2 cdpp_keep(c2 old, c2 v)
+{
+  return c2{__int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old.x), __float_as_int(v.x), CTRL, 0xf, 0xf, false)),
+            __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old.y), __float_as_int(v.y), CTRL, 0xf, 0xf, false))};
+}
 __host__ __device__ __forceinline__ c2 cadd(c2 a, c2 b) { return cmk(a.x + b.x, a.y + b.y); }
 __host__ __device__ __forceinline__ c2 csub(c2 a, c2 b) { return cmk(a.x - b.x, a.y - b.y); }
 __host__ __device__ __forceinline__ c2 cmul(c2 a, c2 b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }

@@ -44,8 +44,8 @@ struct MixSlot {
   int busy;
   c2 *buf, *bufx, *bufy, *hz0, *hxg, *hyg;  // FP32 airwave scratch, R of each
   AirBufs a64;                              // FP64 airwave scratch (true residual)
-  c2 *W, *trash;                            // LU-SGS state and discarded stores
-  float4 *th;                               // LU-SGS records (T, h) per swept cell and r
+  c2 *W, *V;                                // LU-SGS state; a sweep's values in record-slot order
+  float4 *th;                               // LU-SGS records (T, h) per record slot and r
   float hkey[MX_RMAX][2];                   // frequencies the diagonal factors h hold
   int pc;                                   // preconditioner of the solve in flight (csem_mixed_opts)
   int hR;
@@ -151,6 +151,22 @@ static MxGeo mx_geo(const csem_model *m, const MixState *x)
   return g;
 }
 
+// Record slots of the FP32 LU-SGS (MxPad): per slice A + B + 1 diagonals of
+// A rounded up to whole waves rows.
+static MxPad mx_pad(const csem_model *m)
+{
+  MxPad p;
+  long off = 0;
+  for (int c = 0; c < 3; c++) {
+    p.AP[c] = ((m->g.A[c] + 63) / 64) * 64;
+    p.SL[c] = (long)(m->g.A[c] + m->g.B[c] + 1) * p.AP[c];
+    p.base[c] = off;
+    off += p.SL[c] * m->g.ns[c];
+  }
+  p.n = off;
+  return p;
+}
+
 #define MA(o, p, cnt) if ((rc = mx_dalloc(o, &o->p, cnt))) goto fail
 
 // Guards the creation of a model's mixed-precision state (first use of a
@@ -208,6 +224,16 @@ fail:
 
 static int slot_new(csem_model *m, int R, MixSlot **out)
 {
+  {
+    // the FP32 sweeps address W, y and a slice's records as raw buffers of
+    // under 1 GiB each (32-bit offsets, and a masked lane's offset past them)
+    const MxPad pk = mx_pad(m);
+    long big = std::max(std::max(m->fEx, m->fEy), m->fE - m->fEx - m->fEy) * (long)sizeof(c2);
+    big = std::max(big, std::max(std::max(m->lEx, m->lEy), m->lE - m->lEx - m->lEy) * (long)sizeof(c2));
+    for (int c = 0; c < 3; c++) big = std::max(big, pk.SL[c] * (long)sizeof(float4));
+    if (pk.n >= (1L << 31) || big >= (1L << 30) - (1L << 24))
+      return csem_fail(-3, "model too large for the FP32 LU-SGS addressing (%ld records, %ld-byte component)", pk.n, big);
+  }
   MixSlot *q = new MixSlot();
   q->R = R;
   const int n1 = m->n1, n2 = m->n2;
@@ -220,7 +246,7 @@ static int slot_new(csem_model *m, int R, MixSlot **out)
     MA(q, a64.hz0, (long)n1 * n2); MA(q, a64.hxg, (long)(n1 + 1) * n2); MA(q, a64.hyg, (long)n1 * (n2 + 1));
   }
   {
-    MA(q, W, m->fE * R); MA(q, th, m->nsk * R); MA(q, trash, 2L * SGS_MAXT * MX_RMAX);
+    MA(q, W, m->fE * R); MA(q, th, mx_pad(m).n * R); MA(q, V, mx_pad(m).n * R);
     q->nch = (int)((m->lE + RED_CHUNK - 1) / RED_CHUNK);
     MA(q, kv, 9L * R * m->lE); MA(q, part, (long)R * q->nch * 4); MA(q, sc, (long)R * MS_N);
     MA(q, res, R * m->lE); MA(q, tmp64, m->lE); MA(q, npart, (long)R * q->nch); MA(q, norm, R);
@@ -402,18 +428,31 @@ static MxLu mx_lu(const csem_model *m, const MixState *x, MixSlot *q, const MxFr
   MxLu a;
   a.g = mx_geo(m, x); a.sk = m->g;
   a.b = b; a.y = y; a.W = q->W; a.fE = m->fE; a.fEx = m->fEx; a.fEy = m->fEy;
-  a.th = q->th; a.nsk = m->nsk; a.trash = q->trash;
+  a.th = q->th; a.V = q->V; a.pk = mx_pad(m);
   a.c1m = x->c1m; a.c1p = x->c1p; a.c2m = x->c2m; a.c2p = x->c2p; a.c3m = x->c3m; a.c3p = x->c3p;
   a.fr = fr;
   return a;
+}
+
+// the sweep for slices of tb / 64 waves
+template <int C, bool FWD>
+static void mx_sgs_launch(dim3 g, int tb, size_t lds, hipStream_t s, const MxLu &a)
+{
+  switch (tb / 64) {
+    case 1: k_mx_sgs<C, FWD, 1><<<g, tb, lds, s>>>(a); break;
+    case 2: k_mx_sgs<C, FWD, 2><<<g, tb, lds, s>>>(a); break;
+    case 3: k_mx_sgs<C, FWD, 3><<<g, tb, lds, s>>>(a); break;
+    default: k_mx_sgs<C, FWD, 4><<<g, tb, lds, s>>>(a); break;
+  }
 }
 
 template <int R>
 static void mx_lusgs_R(const csem_model *m, const MxLu &a, bool diag, bool sweep, hipStream_t s)
 {
   const SkewGeo &g = m->g;
+  const MxPad &pk = a.pk;
   dim3 gp[3], gq[3];
-  for (int c = 0; c < 3; c++) gp[c] = dim3((unsigned)(((long)g.A[c] * g.B[c] + 255) / 256), (unsigned)g.ns[c]);
+  for (int c = 0; c < 3; c++) gp[c] = dim3((unsigned)((pk.SL[c] + 255) / 256), (unsigned)g.ns[c]);
   if (diag) {
     k_mx_diag<0, R><<<gp[0], 256, 0, s>>>(a);
     k_mx_diag<1, R><<<gp[1], 256, 0, s>>>(a);
@@ -429,28 +468,33 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, bool diag, bool sweep
   dim3 gs[3];
   double nc[3];
   for (int c = 0; c < 3; c++) {
-    tb[c] = ((g.A[c] + 63) / 64) * 64;
-    lds[c] = (size_t)(tb[c] / 64) * MXS_RING * sizeof(c2) + (size_t)(g.B[c] + 2) * sizeof(float) +
-             (size_t)(g.A[c] + g.B[c] + 1) * sizeof(int);
+    tb[c] = pk.AP[c];
+    lds[c] = (size_t)mxs_lds_bytes(g.A[c], g.B[c], pk.AP[c]);
     gs[c] = dim3(8 * ((g.ns[c] + 7) / 8), R);
     nc[c] = (double)g.ns[c] * g.A[c] * g.B[c];
   }
   // bytes per swept cell and right-hand side each kernel must move: the
   // pre-pass reads b and the other components' W values it differences
-  // and writes T; the sweep reads T and h and writes W (not in the last
-  // sweep) and y (backward sweeps)
+  // and writes T; the sweep reads T and h and writes the value to V; the
+  // copy reads V and writes W (not after the last sweep) and y (backward)
   KT(KT_M_PRE, s, 16.0 * R * nc[0], (k_mx_pre<0, true, R><<<gq[0], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[0], (k_mx_sgs<0, true><<<gs[0], tb[0], lds[0], s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[0], (mx_sgs_launch<0, true>(gs[0], tb[0], lds[0], s, a)));
+  KT(KT_M_UNSKEW, s, 16.0 * R * nc[0], (k_mx_unskew<0, true, R><<<gq[0], 256, 0, s>>>(a)));
   KT(KT_M_PRE, s, 24.0 * R * nc[1], (k_mx_pre<1, true, R><<<gq[1], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[1], (k_mx_sgs<1, true><<<gs[1], tb[1], lds[1], s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[1], (mx_sgs_launch<1, true>(gs[1], tb[1], lds[1], s, a)));
+  KT(KT_M_UNSKEW, s, 16.0 * R * nc[1], (k_mx_unskew<1, true, R><<<gq[1], 256, 0, s>>>(a)));
   KT(KT_M_PRE, s, 32.0 * R * nc[2], (k_mx_pre<2, true, R><<<gq[2], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[2], (k_mx_sgs<2, true><<<gs[2], tb[2], lds[2], s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[2], (mx_sgs_launch<2, true>(gs[2], tb[2], lds[2], s, a)));
+  KT(KT_M_UNSKEW, s, 16.0 * R * nc[2], (k_mx_unskew<2, true, R><<<gq[2], 256, 0, s>>>(a)));
   KT(KT_M_PRE, s, 40.0 * R * nc[2], (k_mx_pre<2, false, R><<<gq[2], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 32.0 * R * nc[2], (k_mx_sgs<2, false><<<gs[2], tb[2], lds[2], s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[2], (mx_sgs_launch<2, false>(gs[2], tb[2], lds[2], s, a)));
+  KT(KT_M_UNSKEW, s, 24.0 * R * nc[2], (k_mx_unskew<2, false, R><<<gq[2], 256, 0, s>>>(a)));
   KT(KT_M_PRE, s, 40.0 * R * nc[1], (k_mx_pre<1, false, R><<<gq[1], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 32.0 * R * nc[1], (k_mx_sgs<1, false><<<gs[1], tb[1], lds[1], s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[1], (mx_sgs_launch<1, false>(gs[1], tb[1], lds[1], s, a)));
+  KT(KT_M_UNSKEW, s, 24.0 * R * nc[1], (k_mx_unskew<1, false, R><<<gq[1], 256, 0, s>>>(a)));
   KT(KT_M_PRE, s, 40.0 * R * nc[0], (k_mx_pre<0, false, R><<<gq[0], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[0], (k_mx_sgs<0, false><<<gs[0], tb[0], lds[0], s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * R * nc[0], (mx_sgs_launch<0, false>(gs[0], tb[0], lds[0], s, a)));
+  KT(KT_M_UNSKEW, s, 16.0 * R * nc[0], (k_mx_unskew<0, false, R><<<gq[0], 256, 0, s>>>(a)));
 }
 
 // The red-black ordered sweeps: forward red, black per component (Ex, Ey,
@@ -816,7 +860,7 @@ extern "C" long csem_mixed_slot_bytes(const csem_model *m, int R)
 {
   const long n = m->lE, c2b = sizeof(c2), zb = sizeof(z2);
   const long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
-  long b = (m->fE * R + 2L * m->nsk * R + 2L * SGS_MAXT * MX_RMAX + 9L * R * n) * c2b + (long)R * nch * 4 * 8 +
+  long b = (m->fE * R + 3L * mx_pad(m).n * R + 9L * R * n) * c2b + (long)R * nch * 4 * 8 +
            (long)R * MS_N * 8 + (R * n + n) * zb + (long)R * nch * 8 + R * 8 + R * 4;
   if (m->airbc) {
     const long NN = (long)m->nfft * m->nfft, p1 = (long)m->n1 * m->n2, p2 = (long)(m->n1 + 1) * m->n2,

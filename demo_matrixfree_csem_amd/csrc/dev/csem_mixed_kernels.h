@@ -434,38 +434,51 @@ __global__ void k_mx_air_ghost(MxAir a)
 // coefficient, cs the slow axis', prev the row's previous cell, nbv the
 // neighbour row.
 //
-// Per-cell records (T, h) are stored per right-hand side (r-major) in the
-// skewed order, one float4 at r*nsk + base[C] + q*A*B + p: h (z, w) is
-// refreshed when the frequencies change, T (x, y) by each pre-pass, and a
-// sweep step loads both with one 16-byte load.
+// Per-cell records (T, h) are stored per right-hand side (r-major), one
+// float4 each: h (z, w) is refreshed when the frequencies change, T (x, y)
+// by each pre-pass, and a sweep step loads both with one 16-byte load.
+// Each slice of component C is a block of A + B + 1 diagonals of AP rows
+// (AP = A rounded up to whole waves): diagonal d = s + f (2..A+B) at
+// position d - 1, row s at s - 1.  The first and last diagonals and every
+// slot off the slice's cells hold zero records, so a sweep step reads its
+// wave's records from one wave-uniform base plus a fixed lane offset, and a
+// lane on a slot off the slice computes exactly 0.
+struct MxPad {
+  int AP[3];
+  long SL[3];    // records per slice
+  long base[3];  // component offsets
+  long n;        // records per right-hand side
+};
 
 struct MxLu {
   MxGeo g;
-  SkewGeo sk;
+  SkewGeo sk;    // A, B, ns per component
+  MxPad pk;
   const c2 *b;   // R x n right-hand sides
   c2 *y;         // R x n results
   c2 *W;         // R x fE state on the full grid (Ex | Ey | Ez)
   long fE, fEx, fEy;
-  float4 *th;    // (T, h = iwmu/aii) per swept cell, skewed Ex | Ey | Ez, per r
-  long nsk;      // swept cells of all components (records per r)
-  c2 *trash;     // stores of lanes off the diagonal, 2 * SGS_MAXT * MX_RMAX
+  float4 *th;    // (T, h = iwmu/aii) per record slot, R x pk.n
+  c2 *V;         // a sweep's values per record slot, R x pk.n
   const float *c1m, *c1p, *c2m, *c2p, *c3m, *c3p;  // own-axis coefficients (z on the full grid)
   MxFreq fr;
 };
 
-// h of every swept cell of component C for every r.
+// Records of component C for every r: h on the slice's cells, zero records
+// on every other slot (T is written by the pre-passes on the cells only).
 template <int C, int R>
 __global__ __launch_bounds__(256) void k_mx_diag(MxLu a)
 {
-  const int A = a.sk.A[C], B = a.sk.B[C], q = blockIdx.y;
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= A * B) return;
-  int lo = 2, hi = A + B;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (sk_off(mid, A, B) <= p) lo = mid; else hi = mid - 1;
+  const int A = a.sk.A[C], B = a.sk.B[C], AP = a.pk.AP[C], q = blockIdx.y;
+  const long p = blockIdx.x * 256L + threadIdx.x;
+  if (p >= a.pk.SL[C]) return;
+  const int s = (int)(p % AP) + 1, f = (int)(p / AP) + 1 - s;
+  const long o = a.pk.base[C] + q * a.pk.SL[C] + p;
+  if (s > A || f < 1 || f > B) {
+#pragma unroll
+    for (int r = 0; r < R; r++) a.th[r * a.pk.n + o] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
   }
-  const int s = max(1, lo - B) + (p - sk_off(lo, A, B)), f = lo - s;
   int k, j, i;
   lu_kji<C>(q, s, f, k, j, i);
   const int n1 = a.g.n1, n2 = a.g.n2;
@@ -473,14 +486,13 @@ __global__ __launch_bounds__(256) void k_mx_diag(MxLu a)
   if (C == 0) { c = a.c2p[j] + a.c2m[j] + a.c3p[k] + a.c3m[k]; sig = a.g.s11[id1(n1, n2, k, j, i)]; }
   else if (C == 1) { c = a.c3p[k] + a.c3m[k] + a.c1p[i] + a.c1m[i]; sig = a.g.s22[id2(n1, n2, k, j, i)]; }
   else { c = a.c1p[i] + a.c1m[i] + a.c2p[j] + a.c2m[j]; sig = a.g.s33[id3(n1, n2, k, j, i)]; }
-  const long o = a.sk.base[C] + (long)q * A * B + p;
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const c2 iw = a.fr.iwmu[r];
     // h = iwmu/aii (the fvm factor scales aij and aii alike and cancels)
     const c2 aii = cmk(c * iw.x - sig, c * iw.y);
     const float den = 1.f / (aii.x * aii.x + aii.y * aii.y);
-    ((c2 *)a.th)[2 * (r * a.nsk + o) + 1] =
+    ((c2 *)a.th)[2 * (r * a.pk.n + o) + 1] =
         cmk((iw.x * aii.x + iw.y * aii.y) * den, (iw.y * aii.x - iw.x * aii.y) * den);
   }
 }
@@ -510,7 +522,6 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
   __shared__ int pidx[NC];  // global record index of each slot, -1 off the slices
   const int n1 = a.g.n1, n2 = a.g.n2, n3 = a.g.n3f;
   const int A = a.sk.A[C], B = a.sk.B[C], ns = a.sk.ns[C];
-  const long AB = (long)A * B;
   const int t = threadIdx.x;
   int q0, s0, f0;
   if (C == 0) { q0 = blockIdx.x * TQ; f0 = 1 + blockIdx.y * TF; s0 = 1 + blockIdx.z * TS; }
@@ -531,10 +542,7 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
       pidx[cell] = -1;
       continue;
     }
-    {
-      const int d = s + f;
-      pidx[cell] = (int)(a.sk.base[C] + q * AB + sk_off(d, A, B) + (s - max(1, d - B)));
-    }
+    pidx[cell] = (int)(a.pk.base[C] + q * a.pk.SL[C] + (long)(s + f - 1) * a.pk.AP[C] + (s - 1));
     int k, j, i;
     lu_kji<C>(q, s, f, k, j, i);
     const int kop = k - a.g.kofs;
@@ -578,7 +586,71 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
     const int p = pidx[e];
     if (p < 0) continue;
 #pragma unroll
-    for (int r = 0; r < R; r++) ((c2 *)a.th)[2 * (r * a.nsk + p)] = tile[r][e];
+    for (int r = 0; r < R; r++) ((c2 *)a.th)[2 * (r * a.pk.n + p)] = tile[r][e];
+  }
+}
+
+// The values a sweep (C, FWD) left in V (one per record slot, each wave's
+// step a contiguous run) copied to W on the full grid and, backward, to y
+// on the operator grid, for every r: a tile of MxPreTile cells is read in
+// the skewed order through LDS (each diagonal's run of the tile
+// contiguous) and written in the natural order (x fastest).  The last
+// backward sweep (Ex) writes y only.
+template <int C, bool FWD, int R>
+__global__ __launch_bounds__(256) void k_mx_unskew(MxLu a)
+{
+  using TL = MxPreTile<C>;
+  constexpr int TQ = TL::TQ, TS = TL::TS, TF = TL::TF, NC = TQ * TS * TF;
+  __shared__ c2 tile[R][NC];
+  __shared__ int pidx[NC];  // record slot of each LDS slot, -1 off the slices
+  const int n1 = a.g.n1, n2 = a.g.n2;
+  const int A = a.sk.A[C], B = a.sk.B[C], ns = a.sk.ns[C], AP = a.pk.AP[C];
+  const int t = threadIdx.x;
+  int q0, s0, f0;
+  if (C == 0) { q0 = blockIdx.x * TQ; f0 = 1 + blockIdx.y * TF; s0 = 1 + blockIdx.z * TS; }
+  else { f0 = 1 + blockIdx.x * TF; s0 = 1 + blockIdx.y * TS; q0 = blockIdx.z; }
+  // this thread's cells in the natural order (lanes along x) and their LDS slots
+  int cell[NC / 256];
+#pragma unroll
+  for (int it = 0; it < NC / 256; it++) {
+    int lq, ls, lf;
+    if (C == 0) { lq = t % TQ; lf = (t / TQ) % TF; ls = t / (TQ * TF) + it * (256 / (TQ * TF)); }
+    else { lq = 0; lf = t % TF; ls = t / TF + it * (256 / TF); }
+    const int q = q0 + lq, s = s0 + ls, f = f0 + lf;
+    const int dl = ls + lf + 2;
+    cell[it] = lq * TS * TF + sk_off(dl, TS, TF) + (ls + 1 - max(1, dl - TF));
+    pidx[cell[it]] = (q >= ns || s > A || f > B) ? -1
+                     : (int)(a.pk.base[C] + q * a.pk.SL[C] + (long)(s + f - 1) * AP + (s - 1));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < NC / 256; it++) {
+    const int e = t + it * 256;
+    const int p = pidx[e];
+    if (p < 0) continue;
+#pragma unroll
+    for (int r = 0; r < R; r++) tile[r][e] = a.V[r * a.pk.n + p];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < NC / 256; it++) {
+    if (pidx[cell[it]] < 0) continue;
+    int lq, ls, lf;
+    if (C == 0) { lq = t % TQ; lf = (t / TQ) % TF; ls = t / (TQ * TF) + it * (256 / (TQ * TF)); }
+    else { lq = 0; lf = t % TF; ls = t / TF + it * (256 / TF); }
+    int k, j, i;
+    lu_kji<C>(q0 + lq, s0 + ls, f0 + lf, k, j, i);
+    const long wo = (C == 0) ? id1(n1, n2, k, j, i) : (C == 1) ? a.fEx + id2(n1, n2, k, j, i)
+                                                               : a.fEx + a.fEy + id3(n1, n2, k, j, i);
+    const int kop = k - a.g.kofs;
+    const long xo = (C == 0) ? id1(n1, n2, kop, j, i) : (C == 1) ? a.g.lEx + id2(n1, n2, kop, j, i)
+                                                                   : a.g.lEx + a.g.lEy + id3(n1, n2, kop, j, i);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const c2 v = tile[r][cell[it]];
+      if (FWD || C != 0) a.W[r * a.fE + wo] = v;
+      if (!FWD && kop >= 0) a.y[r * a.g.n + xo] = v;
+    }
   }
 }
 
@@ -679,33 +751,84 @@ __global__ __launch_bounds__(256) void k_mx_rb(MxLu a)
 #undef MW1
 #undef MW2
 
+typedef unsigned v2u __attribute__((ext_vector_type(2)));
+#define MX_OOB ((int)0x80000000)  // a raw-buffer offset past every buffer here (all shorter than 1 GiB)
+// raw buffer resource over bytes [p, p + len) (gfx9 format word)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t mx_rsrc(const void *p, long len)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)len, 0x00020000);
+}
+
+// Zero the cells (u, v), u < nu, v < nv, of a plane at base + u su + v sv
+// whose u lies outside [ulo, uhi] or v outside [1, nv - 2].
+__device__ __forceinline__ void mx_zero_frame(c2 *base, int nu, int nv, long su, long sv, int ulo, int uhi, int t,
+                                              int nt)
+{
+  const c2 z0 = cmk(0, 0);
+  const int lo = max(ulo, 0), hi = min(uhi, nu - 1);
+  for (int u = 0; u < nu; u++) {
+    if (u >= lo && u <= hi) u = hi;  // whole rows outside [lo, hi] only
+    else
+      for (int v = t; v < nv; v += nt) base[u * su + v * sv] = z0;
+  }
+  for (int w = t; w < 2 * (hi - lo + 1); w += nt) base[(lo + (w >> 1)) * su + ((w & 1) ? nv - 1 : 0) * sv] = z0;
+}
+
 // Sweep (C, FWD) of one slice for one right-hand side: block (slice, r),
-// one thread per row s = 1 + t.  Within a wave the new value of the
-// neighbour row from the previous diagonal comes from the neighbouring
-// lane by a whole-wave lane shift, so a diagonal costs no LDS round trip
-// and no barrier.  Consecutive waves run MXS_L diagonals apart (in the
-// sweep's direction the leading wave is the one whose rows come first):
-// the row at a wave's edge is passed through a small LDS ring, written one
-// diagonal after it is computed and read MXS_L diagonals later, and a
-// barrier every MXS_L steps orders every such write before its read (a
-// ring slot is reused only after 2 MXS_L steps).  All waves run the same
-// number of steps (the sweep's diagonals plus the lags); a row off its
-// diagonal computes on a clamped cell and stores to a trash row.  The
-// records of the diagonals ahead stream into a register ring MXS_P deep
-// (one load and at most two stores per diagonal keep that within the 63
-// memory operations a wave can have outstanding); the compiler waits for
-// every outstanding load at the loop's back edge, so the loop runs MXS_U
-// steps per iteration.
+// one thread per row s = 1 + t.  Step tau of a wave handles diagonal
+// d = s + f; within the wave, the neighbour row's new value from the
+// previous diagonal comes from the neighbouring lane by a whole-wave lane
+// shift, and the shift's edge lane keeps the value handed over by the
+// neighbouring wave, so a diagonal costs no LDS round trip and no barrier.
+// Consecutive waves run MXS_L diagonals apart (in the sweep's direction the
+// leading wave is the one whose rows come first): the row at a wave's edge
+// goes through a small LDS ring, written at the step it is computed and read
+// MXS_PD steps before it is used, and a barrier every MXS_BP steps orders
+// every such write before its read (MXS_L >= MXS_BP + MXS_PD - 1; a slot is
+// rewritten only after MXS_RING steps).  All waves run the same number of
+// steps; a lane on a slot off the slice (before its row starts, after it
+// ends, rows past A, the lag at either end) reads a zero record (or none:
+// a raw-buffer load past the slice's records reads zero) and so computes 0.
+// The values go to V in the records' slot order, one contiguous run per
+// wave and step (the W and y cells of a diagonal are a row apart each, so
+// stores in the grid's order would touch a cache line per lane);
+// k_mx_unskew copies them to W and y.  Every value a step needs comes from registers
+// filled steps ahead: the records (MXS_P steps), the fast-axis coefficient
+// of the lane's column (MXS_PC steps, from an LDS table padded with zeros
+// so that no index is clamped) and the edge value (MXS_PD steps).  Raw-buffer
+// accesses keep the memory-operation count exact, so the compiler waits for
+// each record just in time; the loop runs MXS_U steps per iteration.
+// records in flight (steps ahead); with more the compiler's count of the
+// loop's outstanding memory operations turns into a full wait at the back edge
 #ifndef MXS_P
-#define MXS_P 16
+#define MXS_P 12
 #endif
 #ifndef MXS_U
-#define MXS_U 32
+#define MXS_U 96   // steps per loop iteration (the loop leaves at the first barrier past the last step)
 #endif
-#define MXS_L 8      // steps between consecutive waves; a barrier every MXS_L steps
-#define MXS_RING 32  // ring slots per wave edge (> 2 MXS_L, a power of two)
-template <int C, bool FWD>
-__global__ __launch_bounds__(SGS_MAXT) void k_mx_sgs(MxLu a)
+#define MXS_BP 8                         // steps between barriers
+#define MXS_PD 4                         // steps ahead an edge value is read
+#define MXS_PC 4                         // steps ahead a coefficient is read
+#define MXS_L (MXS_BP + MXS_PD - 1)      // steps between consecutive waves
+#define MXS_RING 32                      // ring slots per wave edge
+static_assert(MXS_RING > MXS_L - MXS_PD + MXS_BP, "edge ring slot rewritten before it is read");
+static_assert(MXS_U % MXS_RING == 0 && MXS_U % MXS_BP == 0 && MXS_U % MXS_P == 0 && MXS_U % MXS_PD == 0 &&
+                  MXS_U % MXS_PC == 0,
+              "sweep unroll");
+// LDS of a sweep block of nw waves: per wave an edge ring and MXS_RING + 64
+// slots the other lanes' ring stores land in, then the coefficient table
+__host__ __device__ __forceinline__ int mxs_table_off(int AP, int nw) { return AP + (nw - 1) * MXS_L + MXS_U + MXS_PC + 4; }
+__host__ __device__ __forceinline__ int mxs_table_len(int A, int B, int AP, int nw)
+{
+  return mxs_table_off(AP, nw) + A + B + (nw - 1) * MXS_L + MXS_U + MXS_PC + 4;
+}
+__host__ __device__ __forceinline__ long mxs_lds_bytes(int A, int B, int AP)
+{
+  const int nw = AP / 64;
+  return (long)nw * (2 * MXS_RING + 64) * 8 + (long)mxs_table_len(A, B, AP, nw) * 4;
+}
+template <int C, bool FWD, int NW>
+__global__ __launch_bounds__(64 * NW) void k_mx_sgs(MxLu a)
 {
   extern __shared__ c2 lds[];
   const int nslice = a.sk.ns[C];
@@ -714,98 +837,103 @@ __global__ __launch_bounds__(SGS_MAXT) void k_mx_sgs(MxLu a)
   const int r = blockIdx.y;
   if (q >= nslice) return;
   const int n1 = a.g.n1, n2 = a.g.n2;
-  const int t = threadIdx.x, nt = blockDim.x, nw = nt / 64, wv = t / 64, lane = t % 64;
-  const int A = a.sk.A[C], B = a.sk.B[C], AB = A * B;
+  constexpr int nt = 64 * NW, AP = nt, lagmax = (NW - 1) * MXS_L;
+  constexpr int P = MXS_P;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int A = a.sk.A[C], B = a.sk.B[C];
+  const int nst = A + B - 1;
+  const int OFF = mxs_table_off(AP, NW), TL = mxs_table_len(A, B, AP, NW);
   const int s = 1 + t, sc = min(s, A);
-  c2 *ring = lds;  // [nw][MXS_RING]: the edge row a wave receives
-  float *cf = (float *)(lds + nw * MXS_RING);
-  int *dtab = (int *)(cf + (B + 2));
+  c2 *ring = lds;                                  // [NW][MXS_RING]: the edge row a wave receives
+  c2 *junk = lds + NW * MXS_RING;                  // [NW][MXS_RING + 64]
+  float *cft = (float *)(junk + NW * (MXS_RING + 64));  // fast-axis coefficient of column f at f + OFF
   {
     const float *g0 = (C == 0) ? (FWD ? a.c2m : a.c2p) : (FWD ? a.c1m : a.c1p);
-    for (int u = t; u <= B + 1; u += nt) cf[u] = g0[min(u, B)];
-    for (int d = t; d <= A + B; d += nt) dtab[d] = d >= 2 ? sk_off(d, A, B) - max(1, d - B) : 0;
-    for (int u = t; u < nw * MXS_RING; u += nt) ring[u] = cmk(0, 0);
+    for (int u = t; u < TL; u += nt) {
+      const int f = u - OFF;
+      cft[u] = (f >= 1 && f <= B) ? g0[f] : 0.f;
+    }
+    for (int u = t; u < NW * MXS_RING; u += nt) ring[u] = cmk(0, 0);
   }
   const float *gs = (C == 2) ? (FWD ? a.c2m : a.c2p) : (FWD ? a.c3m : a.c3p);
   const float cs = gs[sc];
-  const float4 *Rc = a.th + r * a.nsk + a.sk.base[C] + (long)q * AB;
-  const long PL = (C == 0) ? (long)n1 * (n2 + 1) : (C == 1) ? (long)(n1 + 1) * n2 : (long)(n1 + 1) * (n2 + 1);
-  const long SS = (C == 2) ? (long)(n1 + 1) : PL;
-  const int FS = (C == 0) ? n1 : 1;
-  const long nb = ((C == 0) ? (long)q : (C == 1) ? (long)(n1 + 1) * q : PL * q) + s * SS;
-  const long wofs = (C == 0) ? 0 : (C == 1) ? a.fEx : a.fEx + a.fEy;
-  const long xofs = (C == 0) ? 0 : (C == 1) ? a.g.lEx : a.g.lEx + a.g.lEy;
-  const int kop = ((C == 2) ? q : s) - a.g.kofs;
-  const long yb = xofs + nb - (long)a.g.kofs * PL;
-  c2 *Wr = a.W + r * a.fE + wofs + nb;
-  c2 *yr = a.y + r * a.g.n + yb;
-  c2 *tw = a.trash + r * SGS_MAXT + t, *ty = a.trash + (MX_RMAX + r) * SGS_MAXT + t;
+  // this slice's records as a raw buffer (the hardware checks the vector
+  // offset: a load past the end reads zero, a store past the end is dropped)
+  const __amdgpu_buffer_rsrc_t Rb = mx_rsrc(a.th + r * a.pk.n + a.pk.base[C] + (long)q * a.pk.SL[C],
+                                            a.pk.SL[C] * (long)sizeof(float4));
+  // the values go to V in the records' slot order: a wave's step is one
+  // contiguous run (a lane off the slice stores its exact 0 into a slot no
+  // reader uses, a lagged step's run lies outside the slice and is dropped)
+  const __amdgpu_buffer_rsrc_t Vb = mx_rsrc(a.V + r * a.pk.n + a.pk.base[C] + (long)q * a.pk.SL[C],
+                                            a.pk.SL[C] * (long)sizeof(c2));
   // lag of this wave and the edges: forward, wave w receives row 64w from
   // wave w-1's lane 63; backward, row 64w+65 from wave w+1's lane 0
-  const int lag = (FWD ? wv : nw - 1 - wv) * MXS_L;
-  const bool recv = FWD ? (lane == 0 && wv > 0) : (lane == 63 && wv < nw - 1);
-  const bool send = FWD ? (lane == 63 && wv < nw - 1) : (lane == 0 && wv > 0);
-  c2 *ring_in = ring + wv * MXS_RING, *ring_out = ring + (FWD ? wv + 1 : wv - 1) * MXS_RING;
-  const int dlo = 2, dhi = A + B, nst = dhi - dlo + 1;
-  __syncthreads();
-  auto load = [&](float4 &c, int st) {
-    const int d = min(max(FWD ? dlo + st : dhi - st, dlo), dhi);
-    c = Rc[min(max(dtab[d] + sc, 0), AB - 1)];
+  const int lag = (FWD ? wv : NW - 1 - wv) * MXS_L;
+  const bool send = FWD ? (lane == 63 && wv < NW - 1) : (lane == 0 && wv > 0);
+  // every lane stores to the ring each step, the sender into the
+  // neighbour's edge ring and the others into this wave's junk slots; the
+  // ring read address is held per lane (a lane-varying zero keeps it in a
+  // vector register), so each access is an address plus an immediate offset
+  const int z = __builtin_amdgcn_mbcnt_lo(0u, 0u);
+  c2 *ring_in = ring + wv * MXS_RING + z;
+  c2 *ring_st = send ? ring + (FWD ? wv + 1 : wv - 1) * MXS_RING : junk + wv * (MXS_RING + 64) + lane;
+  const int nex = ((nst + lagmax + MXS_BP - 1) / MXS_BP) * MXS_BP;  // steps run
+  const int ntk = ((nex + MXS_U - 1) / MXS_U) * MXS_U;
+  // table index of this lane's column at step tau: ib + tau forward, ib - tau backward
+  const int ib = (FWD ? 2 - lag : A + B + lag) - s + OFF;
+  // byte offset of this lane's record at step tau: rv0 + tau * RSTEP
+  constexpr int RSTEP = (FWD ? 1 : -1) * AP * (int)sizeof(float4);
+  const int rv0 = (s - 1) * (int)sizeof(float4) + (FWD ? 1 - lag : nst + lag) * AP * (int)sizeof(float4);
+  auto rec_at = [&](int vofs) -> float4 {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(Rb, vofs, 0, 0));
   };
-  float4 rec[MXS_P];  // rec[tau % MXS_P] holds step tau - lag
+  float4 rec[P];  // rec[tau % P] holds step tau
 #pragma unroll
-  for (int u = 0; u < MXS_P; u++) load(rec[u], u - lag);
+  for (int u = 0; u < P; u++) rec[u] = rec_at(rv0 + u * RSTEP);
+  __syncthreads();
+  float cfr[MXS_PC];  // coefficient of step tau in cfr[tau % MXS_PC]
+  c2 edg[MXS_PD];     // edge value of step tau in edg[tau % MXS_PD]
+#pragma unroll
+  for (int u = 0; u < MXS_PC; u++) cfr[u] = cft[FWD ? ib + u : ib - u];
+#pragma unroll
+  for (int u = 0; u < MXS_PD; u++) edg[u] = ring_in[(u - MXS_L - 1) & (MXS_RING - 1)];
   c2 prev = cmk(0, 0);
-  c2 edge = ring_in[(-MXS_L - 1) & (MXS_RING - 1)];  // the edge value for step 0
-  const int ntk = ((nst + (nw - 1) * MXS_L + MXS_U - 1) / MXS_U) * MXS_U;
   for (int tk0 = 0; tk0 < ntk; tk0 += MXS_U) {
+    const float *ct = cft + (FWD ? ib + tk0 : ib - tk0 - MXS_U - MXS_PC);
+    const int rv = rv0 + (tk0 + P) * RSTEP;  // record offset of step tk0 + P
+    const int vv = rv0 / 2 + tk0 * (RSTEP / 2);  // V offset of step tk0
 #pragma unroll
     for (int uu = 0; uu < MXS_U; uu++) {
-      const int tau = tk0 + uu, u = uu % MXS_P, st = tau - lag;
-      const int d = FWD ? dlo + st : dhi - st;
-      const int f = d - s;
-      const float cfv = cf[min(max(f, 0), B + 1)];
-      const bool ok = s <= A && f >= 1 && f <= B;
-      const long o = (long)f * FS;
-      // the neighbour row on the previous diagonal
-      c2 nbv = FWD ? cdpp<DPP_WAVE_SHR1>(prev) : cdpp<DPP_WAVE_SHL1>(prev);
-      nbv.x = recv ? edge.x : nbv.x;
-      nbv.y = recv ? edge.y : nbv.y;
-      edge = ring_in[(tau - MXS_L) & (MXS_RING - 1)];  // for the next step
-      const c2 w = cadd(cmk(rec[u].x, rec[u].y), crfma(crmul(cfv, prev), cs, nbv));
-      const c2 v = cmul(cmk(rec[u].z, rec[u].w), w);
-      if (FWD || C != 0) *(ok ? Wr + o : tw) = v;
-      if (!FWD) *(ok && kop >= 0 ? yr + o : ty) = v;
-      prev.x = ok ? v.x : prev.x;
-      prev.y = ok ? v.y : prev.y;
-      if (send) ring_out[tau & (MXS_RING - 1)] = prev;
-      load(rec[u], st + MXS_P);
-      if (uu % MXS_L == MXS_L - 1) __syncthreads();
+      const float4 rc = rec[uu % P];
+      const float cfv = cfr[uu % MXS_PC];
+      // the neighbour row's value on the previous diagonal
+      const c2 nbv = FWD ? cdpp_keep<DPP_WAVE_SHR1>(edg[uu % MXS_PD], prev)
+                         : cdpp_keep<DPP_WAVE_SHL1>(edg[uu % MXS_PD], prev);
+      const float ux = __builtin_fmaf(cfv, prev.x, rc.x), uy = __builtin_fmaf(cfv, prev.y, rc.y);
+      const float wx = __builtin_fmaf(cs, nbv.x, ux), wy = __builtin_fmaf(cs, nbv.y, uy);
+      prev = cmk(rc.z * wx - rc.w * wy, rc.z * wy + rc.w * wx);
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, prev), Vb, vv + uu * (RSTEP / 2), 0, 0);
+      ring_st[uu & (MXS_RING - 1)] = prev;
+      if (uu % MXS_BP == MXS_BP - 1) {
+        __syncthreads();
+        if (tk0 + uu + 1 >= nex) goto swept;  // the same step for every wave
+      }
+      edg[uu % MXS_PD] = ring_in[(uu + MXS_PD - MXS_L - 1) & (MXS_RING - 1)];
+      cfr[uu % MXS_PC] = ct[FWD ? uu + MXS_PC : MXS_U - uu];
+      rec[uu % P] = rec_at(rv + uu * RSTEP);
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
+swept:
   if (!FWD) {  // operator-grid cells of this slice outside the sweep range are zero rows
-    const int n3 = a.g.n3f;
-    const c2 z0 = cmk(0, 0);
+    const int kl = 1 - a.g.kofs, kh = a.g.n3f - 1 - a.g.kofs;  // operator planes of the sweep range
     c2 *y0 = a.y + r * a.g.n;
-    if (C == 0) {
-      const int nk = a.g.n3 + 1;
-      for (int w = t; w < nk * (n2 + 1); w += nt) {
-        const int kq = w / (n2 + 1), j = w % (n2 + 1), kf = kq + a.g.kofs;
-        if (!(kf >= 1 && kf <= n3 - 1 && j >= 1 && j <= n2 - 1)) y0[id1(n1, n2, kq, j, q)] = z0;
-      }
-    } else if (C == 1) {
-      const int nk = a.g.n3 + 1;
-      for (int w = t; w < nk * (n1 + 1); w += nt) {
-        const int kq = w / (n1 + 1), i = w % (n1 + 1), kf = kq + a.g.kofs;
-        if (!(kf >= 1 && kf <= n3 - 1 && i >= 1 && i <= n1 - 1)) y0[a.g.lEx + id2(n1, n2, kq, q, i)] = z0;
-      }
-    } else if (q - a.g.kofs >= 0) {
-      for (int w = t; w < (n2 + 1) * (n1 + 1); w += nt) {
-        const int jj = w / (n1 + 1), i = w % (n1 + 1);
-        if (!(jj >= 1 && jj <= n2 - 1 && i >= 1 && i <= n1 - 1))
-          y0[a.g.lEx + a.g.lEy + id3(n1, n2, q - a.g.kofs, jj, i)] = z0;
-      }
-    }
+    if (C == 0) mx_zero_frame(y0 + q, a.g.n3 + 1, n2 + 1, (long)n1 * (n2 + 1), n1, kl, kh, t, nt);
+    else if (C == 1) mx_zero_frame(y0 + a.g.lEx + (long)(n1 + 1) * q, a.g.n3 + 1, n1 + 1, (long)(n1 + 1) * n2, 1, kl, kh, t, nt);
+    else if (q - a.g.kofs >= 0)
+      mx_zero_frame(y0 + a.g.lEx + a.g.lEy + (long)(n1 + 1) * (n2 + 1) * (q - a.g.kofs), n2 + 1, n1 + 1, n1 + 1, 1, 1,
+                    n2 - 1, t, nt);
   }
 }
 
