@@ -46,9 +46,7 @@ struct MixSlot {
   AirBufs a64;                              // FP64 airwave scratch (true residual)
   c2 *W, *V;                                // LU-SGS state; a sweep's values in record-slot order
   float4 *th;                               // LU-SGS records (T, h) per record slot and r
-  float hkey[MX_RMAX][2];                   // frequencies the diagonal factors h hold
   int pc;                                   // preconditioner of the solve in flight (csem_mixed_opts)
-  int hR;
   c2 *kv;                                   // Krylov vectors
   double *part, *sc;
   int nch;
@@ -252,7 +250,9 @@ static int slot_new(csem_model *m, int R, MixSlot **out)
     MA(q, res, R * m->lE); MA(q, tmp64, m->lE); MA(q, npart, (long)R * q->nch); MA(q, norm, R);
     MA(q, scale, R);
   }
+  // records of slots off the slices stay zero (the pre-passes write the cells' records)
   if (dev_memset(q->W, 0, m->fE * R * sizeof(c2)) != hipSuccess ||
+      dev_memset(q->th, 0, mx_pad(m).n * R * sizeof(float4)) != hipSuccess ||
       dev_memset(q->sc, 0, (long)R * MS_N * sizeof(double)) != hipSuccess) {
     rc = csem_fail(-1, "memset");
     goto fail;
@@ -447,18 +447,11 @@ static void mx_sgs_launch(dim3 g, int tb, size_t lds, hipStream_t s, const MxLu 
 }
 
 template <int R>
-static void mx_lusgs_R(const csem_model *m, const MxLu &a, bool diag, bool sweep, hipStream_t s)
+static void mx_lusgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
 {
   const SkewGeo &g = m->g;
   const MxPad &pk = a.pk;
-  dim3 gp[3], gq[3];
-  for (int c = 0; c < 3; c++) gp[c] = dim3((unsigned)((pk.SL[c] + 255) / 256), (unsigned)g.ns[c]);
-  if (diag) {
-    k_mx_diag<0, R><<<gp[0], 256, 0, s>>>(a);
-    k_mx_diag<1, R><<<gp[1], 256, 0, s>>>(a);
-    k_mx_diag<2, R><<<gp[2], 256, 0, s>>>(a);
-  }
-  if (!sweep) return;
+  dim3 gq[3];
   // pre-pass tiles (MxPreTile): Ex 16 slices x 8 x 8, Ey and Ez 16 x 64
   gq[0] = dim3((g.ns[0] + 15) / 16, (g.B[0] + 7) / 8, (g.A[0] + 7) / 8);
   gq[1] = dim3((g.B[1] + 63) / 64, (g.A[1] + 15) / 16, g.ns[1]);
@@ -537,35 +530,21 @@ static void mx_rbsgs_dispatch(const csem_model *m, int R, const MxLu &a, hipStre
   }
 }
 
-static void mx_lusgs_dispatch(const csem_model *m, int R, const MxLu &a, bool diag, bool sweep, hipStream_t s)
+static void mx_lusgs_dispatch(const csem_model *m, int R, const MxLu &a, hipStream_t s)
 {
   switch (R) {
-    case 1: mx_lusgs_R<1>(m, a, diag, sweep, s); break;
-    case 2: mx_lusgs_R<2>(m, a, diag, sweep, s); break;
-    case 3: mx_lusgs_R<3>(m, a, diag, sweep, s); break;
-    default: mx_lusgs_R<4>(m, a, diag, sweep, s); break;
+    case 1: mx_lusgs_R<1>(m, a, s); break;
+    case 2: mx_lusgs_R<2>(m, a, s); break;
+    case 3: mx_lusgs_R<3>(m, a, s); break;
+    default: mx_lusgs_R<4>(m, a, s); break;
   }
-}
-
-// The diagonal factors h depend on the batch's frequencies: refreshed when
-// they change, outside any captured iteration.
-static int mx_diag(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, hipStream_t s)
-{
-  bool stale = q->hR != R;
-  for (int r = 0; r < R && !stale; r++) stale = q->hkey[r][0] != fr.iwmu[r].x || q->hkey[r][1] != fr.iwmu[r].y;
-  if (!stale) return 0;
-  q->hR = R;
-  for (int r = 0; r < R; r++) { q->hkey[r][0] = fr.iwmu[r].x; q->hkey[r][1] = fr.iwmu[r].y; }
-  mx_lusgs_dispatch(m, R, mx_lu(m, x, q, fr, NULL, NULL), true, false, s);
-  HIP_OK(hipGetLastError());
-  return 0;
 }
 
 static int mx_lusgs(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, const c2 *b, c2 *y,
                     hipStream_t s)
 {
   if (q->pc == CSEM_PRECOND_RBSGS) mx_rbsgs_dispatch(m, R, mx_lu(m, x, q, fr, b, y), s);
-  else mx_lusgs_dispatch(m, R, mx_lu(m, x, q, fr, b, y), false, true, s);
+  else mx_lusgs_dispatch(m, R, mx_lu(m, x, q, fr, b, y), s);
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -703,7 +682,6 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
   HIP_OK(hipStreamSynchronize(s));
   for (int r = 0; r < R; r++) { bn[r] = sqrt(nb[r]); rn[r] = bn[r]; relres[r] = 1.0; iters[r] = 0; }
   if ((rc = mx_tune(m, x, q, R, fr, k.r, k.v, s))) return rc;  // operands: any R x n FP32 buffers
-  if ((rc = mx_diag(m, x, q, R, fr, s))) return rc;
   for (int outer = 0; outer <= o.max_outer; outer++) {
     // which right-hand sides still need a correction
     bool any = false;
@@ -802,7 +780,7 @@ static int mx_single(csem_model *m, int R, const csem_freq *f, const void *xv, v
   const MxFreq fr = mx_freq(f, R);
   q->pc = CSEM_PRECOND_LUSGS;
   if (lu) {
-    if (!(rc = mx_diag(m, x, q, R, fr, s))) rc = mx_lusgs(m, x, q, R, fr, (const c2 *)xv, (c2 *)yv, s);
+    rc = mx_lusgs(m, x, q, R, fr, (const c2 *)xv, (c2 *)yv, s);
   } else {
     rc = mx_apply(m, x, q, R, fr, (const c2 *)xv, (c2 *)yv, s);
   }

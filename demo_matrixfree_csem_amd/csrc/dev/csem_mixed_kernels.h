@@ -435,8 +435,8 @@ __global__ void k_mx_air_ghost(MxAir a)
 // neighbour row.
 //
 // Per-cell records (T, h) are stored per right-hand side (r-major), one
-// float4 each: h (z, w) is refreshed when the frequencies change, T (x, y)
-// by each pre-pass, and a sweep step loads both with one 16-byte load.
+// float4 each, written whole by each pre-pass; a sweep step loads both with
+// one 16-byte load.
 // Each slice of component C is a block of A + B + 1 diagonals of AP rows
 // (AP = A rounded up to whole waves): diagonal d = s + f (2..A+B) at
 // position d - 1, row s at s - 1.  The first and last diagonals and every
@@ -464,39 +464,6 @@ struct MxLu {
   MxFreq fr;
 };
 
-// Records of component C for every r: h on the slice's cells, zero records
-// on every other slot (T is written by the pre-passes on the cells only).
-template <int C, int R>
-__global__ __launch_bounds__(256) void k_mx_diag(MxLu a)
-{
-  const int A = a.sk.A[C], B = a.sk.B[C], AP = a.pk.AP[C], q = blockIdx.y;
-  const long p = blockIdx.x * 256L + threadIdx.x;
-  if (p >= a.pk.SL[C]) return;
-  const int s = (int)(p % AP) + 1, f = (int)(p / AP) + 1 - s;
-  const long o = a.pk.base[C] + q * a.pk.SL[C] + p;
-  if (s > A || f < 1 || f > B) {
-#pragma unroll
-    for (int r = 0; r < R; r++) a.th[r * a.pk.n + o] = make_float4(0.f, 0.f, 0.f, 0.f);
-    return;
-  }
-  int k, j, i;
-  lu_kji<C>(q, s, f, k, j, i);
-  const int n1 = a.g.n1, n2 = a.g.n2;
-  float c, sig;
-  if (C == 0) { c = a.c2p[j] + a.c2m[j] + a.c3p[k] + a.c3m[k]; sig = a.g.s11[id1(n1, n2, k, j, i)]; }
-  else if (C == 1) { c = a.c3p[k] + a.c3m[k] + a.c1p[i] + a.c1m[i]; sig = a.g.s22[id2(n1, n2, k, j, i)]; }
-  else { c = a.c1p[i] + a.c1m[i] + a.c2p[j] + a.c2m[j]; sig = a.g.s33[id3(n1, n2, k, j, i)]; }
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const c2 iw = a.fr.iwmu[r];
-    // h = iwmu/aii (the fvm factor scales aij and aii alike and cancels)
-    const c2 aii = cmk(c * iw.x - sig, c * iw.y);
-    const float den = 1.f / (aii.x * aii.x + aii.y * aii.y);
-    ((c2 *)a.th)[2 * (r * a.pk.n + o) + 1] =
-        cmk((iw.x * aii.x + iw.y * aii.y) * den, (iw.y * aii.x - iw.x * aii.y) * den);
-  }
-}
-
 // Pre-pass tiles: TQ slices x TS rows x TF columns of a component's slices,
 // read in the natural order (x fastest, so the W and b reads are whole
 // rows) and written in the skewed order through LDS, so each diagonal's
@@ -510,15 +477,19 @@ template <> struct MxPreTile<0> { static constexpr int TQ = 16, TS = 8, TF = 8; 
 #define MW1(r, k, j, i) a.W[(r) * a.fE + a.fEx + id2(n1, n2, k, j, i)]
 #define MW2(r, k, j, i) a.W[(r) * a.fE + a.fEx + a.fEy + id3(n1, n2, k, j, i)]
 
-// Pre-pass of sweep (C, FWD) for every r: T of a tile of cells.  Each cell
-// goes to LDS at its position in the tile's skewed order, with its global
-// record index, so the store phase is a straight copy.
+// Pre-pass of sweep (C, FWD) for every r: the records (T, h) of a tile of
+// cells.  Each cell goes to LDS at its position in the tile's skewed order,
+// with its global record index and the two numbers its h needs, so the
+// store phase writes whole records in runs: h = iwmu/aii with aii =
+// c iwmu - sigma, c the sum of the cell's own-axis coefficients (the fvm
+// factor scales aij and aii alike and cancels).
 template <int C, bool FWD, int R>
 __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
 {
   using TL = MxPreTile<C>;
   constexpr int TQ = TL::TQ, TS = TL::TS, TF = TL::TF, NC = TQ * TS * TF;
   __shared__ c2 tile[R][NC];
+  __shared__ c2 csg[NC];    // (c, sigma) of each slot
   __shared__ int pidx[NC];  // global record index of each slot, -1 off the slices
   const int n1 = a.g.n1, n2 = a.g.n2, n3 = a.g.n3f;
   const int A = a.sk.A[C], B = a.sk.B[C], ns = a.sk.ns[C];
@@ -549,6 +520,9 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
     const long xo = (C == 0) ? id1(n1, n2, kop, j, i) : (C == 1) ? a.g.lEx + id2(n1, n2, kop, j, i)
                                                                    : a.g.lEx + a.g.lEy + id3(n1, n2, kop, j, i);
     const float vol = a.g.d1s[i] * a.g.d2[j] * a.g.d3[k];
+    if (C == 0) csg[cell] = cmk(a.c2p[j] + a.c2m[j] + a.c3p[k] + a.c3m[k], a.g.s11[id1(n1, n2, k, j, i)]);
+    else if (C == 1) csg[cell] = cmk(a.c3p[k] + a.c3m[k] + a.c1p[i] + a.c1m[i], a.g.s22[id2(n1, n2, k, j, i)]);
+    else csg[cell] = cmk(a.c1p[i] + a.c1m[i] + a.c2p[j] + a.c2m[j], a.g.s33[id3(n1, n2, k, j, i)]);
 #pragma unroll
     for (int r = 0; r < R; r++) {
       c2 S = z0, own = z0;
@@ -578,15 +552,22 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
     }
   }
   __syncthreads();
-  // store: the tile's cells in the skewed order (per slice, by diagonal,
+  // store: the tile's records in the skewed order (per slice, by diagonal,
   // rows contiguous within a diagonal)
 #pragma unroll
   for (int it = 0; it < NC / 256; it++) {
     const int e = t + it * 256;
     const int p = pidx[e];
     if (p < 0) continue;
+    const c2 cs = csg[e];
 #pragma unroll
-    for (int r = 0; r < R; r++) ((c2 *)a.th)[2 * (r * a.pk.n + p)] = tile[r][e];
+    for (int r = 0; r < R; r++) {
+      const c2 iw = a.fr.iwmu[r], T = tile[r][e];
+      const c2 aii = cmk(cs.x * iw.x - cs.y, cs.x * iw.y);
+      const float den = 1.f / (aii.x * aii.x + aii.y * aii.y);
+      a.th[r * a.pk.n + p] =
+          make_float4(T.x, T.y, (iw.x * aii.x + iw.y * aii.y) * den, (iw.y * aii.x - iw.x * aii.y) * den);
+    }
   }
 }
 
@@ -806,6 +787,9 @@ __device__ __forceinline__ void mx_zero_frame(c2 *base, int nu, int nv, long su,
 #ifndef MXS_U
 #define MXS_U 96   // steps per loop iteration (the loop leaves at the first barrier past the last step)
 #endif
+#ifndef MXS_MASK
+#define MXS_MASK 1                       // lanes off the slice skip their loads and stores
+#endif
 #define MXS_BP 8                         // steps between barriers
 #define MXS_PD 4                         // steps ahead an edge value is read
 #define MXS_PC 4                         // steps ahead a coefficient is read
@@ -888,9 +872,21 @@ __global__ __launch_bounds__(64 * NW) void k_mx_sgs(MxLu a)
   auto rec_at = [&](int vofs) -> float4 {
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(Rb, vofs, 0, 0));
   };
+  // a lane whose cell at step tau is off the slice (f - 1 = d - s - 1 outside
+  // [0, B) or s > A) neither loads nor stores: its offsets are past the
+  // buffers (the load reads 0, the store is dropped), and the V offset is
+  // the record offset halved, so the load's mask carries over to the store
+  const int fm0 = s <= A ? (FWD ? 1 - lag : A + B + lag - 1) - s : (1 << 30);  // f - 1 at step 0
+  auto rofs = [&](int tau, int ofs) {
+    return (MXS_MASK == 0 || (unsigned)(FWD ? fm0 + tau : fm0 - tau) < (unsigned)B) ? ofs : MX_OOB;
+  };
   float4 rec[P];  // rec[tau % P] holds step tau
+  int rof[P];     // and its record offset (MX_OOB when masked)
 #pragma unroll
-  for (int u = 0; u < P; u++) rec[u] = rec_at(rv0 + u * RSTEP);
+  for (int u = 0; u < P; u++) {
+    rof[u] = rofs(u, rv0 + u * RSTEP);
+    rec[u] = rec_at(rof[u]);
+  }
   __syncthreads();
   float cfr[MXS_PC];  // coefficient of step tau in cfr[tau % MXS_PC]
   c2 edg[MXS_PD];     // edge value of step tau in edg[tau % MXS_PD]
@@ -902,7 +898,6 @@ __global__ __launch_bounds__(64 * NW) void k_mx_sgs(MxLu a)
   for (int tk0 = 0; tk0 < ntk; tk0 += MXS_U) {
     const float *ct = cft + (FWD ? ib + tk0 : ib - tk0 - MXS_U - MXS_PC);
     const int rv = rv0 + (tk0 + P) * RSTEP;  // record offset of step tk0 + P
-    const int vv = rv0 / 2 + tk0 * (RSTEP / 2);  // V offset of step tk0
 #pragma unroll
     for (int uu = 0; uu < MXS_U; uu++) {
       const float4 rc = rec[uu % P];
@@ -913,7 +908,7 @@ __global__ __launch_bounds__(64 * NW) void k_mx_sgs(MxLu a)
       const float ux = __builtin_fmaf(cfv, prev.x, rc.x), uy = __builtin_fmaf(cfv, prev.y, rc.y);
       const float wx = __builtin_fmaf(cs, nbv.x, ux), wy = __builtin_fmaf(cs, nbv.y, uy);
       prev = cmk(rc.z * wx - rc.w * wy, rc.z * wy + rc.w * wx);
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, prev), Vb, vv + uu * (RSTEP / 2), 0, 0);
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, prev), Vb, (int)((unsigned)rof[uu % P] >> 1), 0, 0);
       ring_st[uu & (MXS_RING - 1)] = prev;
       if (uu % MXS_BP == MXS_BP - 1) {
         __syncthreads();
@@ -921,7 +916,8 @@ __global__ __launch_bounds__(64 * NW) void k_mx_sgs(MxLu a)
       }
       edg[uu % MXS_PD] = ring_in[(uu + MXS_PD - MXS_L - 1) & (MXS_RING - 1)];
       cfr[uu % MXS_PC] = ct[FWD ? uu + MXS_PC : MXS_U - uu];
-      rec[uu % P] = rec_at(rv + uu * RSTEP);
+      rof[uu % P] = rofs(tk0 + uu + P, rv + uu * RSTEP);
+      rec[uu % P] = rec_at(rof[uu % P]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
