@@ -76,7 +76,7 @@ def _lusgs_roofline(stats, dims, R):
     nc = [n1 * (n3f - 1) * (n2 - 1), n2 * (n3f - 1) * (n1 - 1), n3f * (n2 - 1) * (n1 - 1)]
     per = float(sum(c * (f + 88) for c, f in zip(nc, (40, 56, 72))))
     if R:
-        pre, sweeps = "k_mx_lu_pre", ["k_mx_sgs"]
+        pre, sweeps = "k_mx_lu_pre", [k for k in ("k_mx_sgs", "k_mx_unskew") if k in stats]
         per *= R / 2.0
         if "k_mx_sgs" not in stats:
             return None
@@ -89,7 +89,8 @@ def _lusgs_roofline(stats, dims, R):
     ms = stats.get(pre, (0.0, 0, 0.0))[0] + sum(stats[k][0] for k in sweeps)
     avg_s = ms / napp / 1e3
     gbs = per / avg_s / 1e9
-    return {"what": "one LU-SGS application: 6 pre-passes + 6 sweeps", "bound": "hbm", "achieved": round(gbs, 2),
+    what = "one LU-SGS application: 6 pre-passes + 6 sweeps" + (" + 6 copies to W/y" if "k_mx_unskew" in sweeps else "")
+    return {"what": what, "bound": "hbm", "achieved": round(gbs, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
             "bytes_per_application": int(per), "avg_us": round(avg_s * 1e6, 3), "applications": int(napp)}
 

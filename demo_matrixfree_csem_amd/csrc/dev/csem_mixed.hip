@@ -434,6 +434,17 @@ static MxLu mx_lu(const csem_model *m, const MixState *x, MixSlot *q, const MxFr
   return a;
 }
 
+// blocks of a tile kernel over component c: Ex tiles span slices (x),
+// Ey and Ez tiles lie in one slice
+template <class TL>
+static dim3 mx_tile_grid(const SkewGeo &g, int c)
+{
+  if (c == 0)
+    return dim3((unsigned)((g.ns[0] + TL::TQ - 1) / TL::TQ), (unsigned)((g.B[0] + TL::TF - 1) / TL::TF),
+                (unsigned)((g.A[0] + TL::TS - 1) / TL::TS));
+  return dim3((unsigned)((g.B[c] + TL::TF - 1) / TL::TF), (unsigned)((g.A[c] + TL::TS - 1) / TL::TS), (unsigned)g.ns[c]);
+}
+
 // the sweep for slices of tb / 64 waves
 template <int C, bool FWD>
 static void mx_sgs_launch(dim3 g, int tb, size_t lds, hipStream_t s, const MxLu &a)
@@ -451,11 +462,10 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
 {
   const SkewGeo &g = m->g;
   const MxPad &pk = a.pk;
-  dim3 gq[3];
-  // pre-pass tiles (MxPreTile): Ex 16 slices x 8 x 8, Ey and Ez 16 x 64
-  gq[0] = dim3((g.ns[0] + 15) / 16, (g.B[0] + 7) / 8, (g.A[0] + 7) / 8);
-  gq[1] = dim3((g.B[1] + 63) / 64, (g.A[1] + 15) / 16, g.ns[1]);
-  gq[2] = dim3((g.B[2] + 63) / 64, (g.A[2] + 15) / 16, g.ns[2]);
+  // tile grids of the pre-passes (MxPreTile) and the copies (MxCopyTile)
+  const dim3 gq[3] = {mx_tile_grid<MxPreTile<0>>(g, 0), mx_tile_grid<MxPreTile<1>>(g, 1), mx_tile_grid<MxPreTile<2>>(g, 2)};
+  const dim3 gu[3] = {mx_tile_grid<MxCopyTile<0>>(g, 0), mx_tile_grid<MxCopyTile<1>>(g, 1),
+                      mx_tile_grid<MxCopyTile<2>>(g, 2)};
   int tb[3];
   size_t lds[3];
   dim3 gs[3];
@@ -472,22 +482,22 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
   // copy reads V and writes W (not after the last sweep) and y (backward)
   KT(KT_M_PRE, s, 16.0 * R * nc[0], (k_mx_pre<0, true, R><<<gq[0], 256, 0, s>>>(a)));
   KT(KT_M_SGS, s, 24.0 * R * nc[0], (mx_sgs_launch<0, true>(gs[0], tb[0], lds[0], s, a)));
-  KT(KT_M_UNSKEW, s, 16.0 * R * nc[0], (k_mx_unskew<0, true, R><<<gq[0], 256, 0, s>>>(a)));
+  KT(KT_M_UNSKEW, s, 16.0 * R * nc[0], (k_mx_unskew<0, true, R><<<gu[0], 256, 0, s>>>(a)));
   KT(KT_M_PRE, s, 24.0 * R * nc[1], (k_mx_pre<1, true, R><<<gq[1], 256, 0, s>>>(a)));
   KT(KT_M_SGS, s, 24.0 * R * nc[1], (mx_sgs_launch<1, true>(gs[1], tb[1], lds[1], s, a)));
-  KT(KT_M_UNSKEW, s, 16.0 * R * nc[1], (k_mx_unskew<1, true, R><<<gq[1], 256, 0, s>>>(a)));
+  KT(KT_M_UNSKEW, s, 16.0 * R * nc[1], (k_mx_unskew<1, true, R><<<gu[1], 256, 0, s>>>(a)));
   KT(KT_M_PRE, s, 32.0 * R * nc[2], (k_mx_pre<2, true, R><<<gq[2], 256, 0, s>>>(a)));
   KT(KT_M_SGS, s, 24.0 * R * nc[2], (mx_sgs_launch<2, true>(gs[2], tb[2], lds[2], s, a)));
-  KT(KT_M_UNSKEW, s, 16.0 * R * nc[2], (k_mx_unskew<2, true, R><<<gq[2], 256, 0, s>>>(a)));
+  KT(KT_M_UNSKEW, s, 16.0 * R * nc[2], (k_mx_unskew<2, true, R><<<gu[2], 256, 0, s>>>(a)));
   KT(KT_M_PRE, s, 40.0 * R * nc[2], (k_mx_pre<2, false, R><<<gq[2], 256, 0, s>>>(a)));
   KT(KT_M_SGS, s, 24.0 * R * nc[2], (mx_sgs_launch<2, false>(gs[2], tb[2], lds[2], s, a)));
-  KT(KT_M_UNSKEW, s, 24.0 * R * nc[2], (k_mx_unskew<2, false, R><<<gq[2], 256, 0, s>>>(a)));
+  KT(KT_M_UNSKEW, s, 24.0 * R * nc[2], (k_mx_unskew<2, false, R><<<gu[2], 256, 0, s>>>(a)));
   KT(KT_M_PRE, s, 40.0 * R * nc[1], (k_mx_pre<1, false, R><<<gq[1], 256, 0, s>>>(a)));
   KT(KT_M_SGS, s, 24.0 * R * nc[1], (mx_sgs_launch<1, false>(gs[1], tb[1], lds[1], s, a)));
-  KT(KT_M_UNSKEW, s, 24.0 * R * nc[1], (k_mx_unskew<1, false, R><<<gq[1], 256, 0, s>>>(a)));
+  KT(KT_M_UNSKEW, s, 24.0 * R * nc[1], (k_mx_unskew<1, false, R><<<gu[1], 256, 0, s>>>(a)));
   KT(KT_M_PRE, s, 40.0 * R * nc[0], (k_mx_pre<0, false, R><<<gq[0], 256, 0, s>>>(a)));
   KT(KT_M_SGS, s, 24.0 * R * nc[0], (mx_sgs_launch<0, false>(gs[0], tb[0], lds[0], s, a)));
-  KT(KT_M_UNSKEW, s, 16.0 * R * nc[0], (k_mx_unskew<0, false, R><<<gq[0], 256, 0, s>>>(a)));
+  KT(KT_M_UNSKEW, s, 16.0 * R * nc[0], (k_mx_unskew<0, false, R><<<gu[0], 256, 0, s>>>(a)));
 }
 
 // The red-black ordered sweeps: forward red, black per component (Ex, Ey,

@@ -468,9 +468,14 @@ struct MxLu {
 // read in the natural order (x fastest, so the W and b reads are whole
 // rows) and written in the skewed order through LDS, so each diagonal's
 // run of the tile is one contiguous store.  Ex: slices run along x, so a
-// tile spans 16 slices; Ey, Ez: a tile is 16 rows x 64 columns of one slice.
+// tile spans 32 slices (rows of 32 values; a diagonal's run is at most 4
+// records); Ey, Ez: a tile is 16 rows x 64 columns of one slice.
 template <int C> struct MxPreTile { static constexpr int TQ = 1, TS = 16, TF = 64; };
-template <> struct MxPreTile<0> { static constexpr int TQ = 16, TS = 8, TF = 8; };
+template <> struct MxPreTile<0> { static constexpr int TQ = 32, TS = 8, TF = 4; };
+// The copy from V (k_mx_unskew) reads diagonal runs and writes rows: its Ex
+// tile trades row width for run length the other way.
+template <int C> struct MxCopyTile { static constexpr int TQ = 1, TS = 16, TF = 64; };
+template <> struct MxCopyTile<0> { static constexpr int TQ = 16, TS = 8, TF = 8; };
 
 // W value of component C at (k, j, i) for right-hand side r.
 #define MW0(r, k, j, i) a.W[(r) * a.fE + id1(n1, n2, k, j, i)]
@@ -573,14 +578,14 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
 
 // The values a sweep (C, FWD) left in V (one per record slot, each wave's
 // step a contiguous run) copied to W on the full grid and, backward, to y
-// on the operator grid, for every r: a tile of MxPreTile cells is read in
+// on the operator grid, for every r: a tile of MxCopyTile cells is read in
 // the skewed order through LDS (each diagonal's run of the tile
 // contiguous) and written in the natural order (x fastest).  The last
 // backward sweep (Ex) writes y only.
 template <int C, bool FWD, int R>
 __global__ __launch_bounds__(256) void k_mx_unskew(MxLu a)
 {
-  using TL = MxPreTile<C>;
+  using TL = MxCopyTile<C>;
   constexpr int TQ = TL::TQ, TS = TL::TS, TF = TL::TF, NC = TQ * TS * TF;
   __shared__ c2 tile[R][NC];
   __shared__ int pidx[NC];  // record slot of each LDS slot, -1 off the slices
