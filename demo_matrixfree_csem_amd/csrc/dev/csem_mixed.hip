@@ -434,15 +434,26 @@ static MxLu mx_lu(const csem_model *m, const MixState *x, MixSlot *q, const MxFr
   return a;
 }
 
-// blocks of a tile kernel over component c: Ex tiles span slices (x),
-// Ey and Ez tiles lie in one slice
+// blocks of a tile kernel over component c (mx_tile_xyz): Ex tiles span
+// slices, Ey and Ez tiles TQ slices; a 1-D grid, a multiple of the 8 XCDs
 template <class TL>
 static dim3 mx_tile_grid(const SkewGeo &g, int c)
+{
+  long n;
+  if (c == 0) n = (long)((g.ns[0] + TL::TQ - 1) / TL::TQ) * ((g.B[0] + TL::TF - 1) / TL::TF) * ((g.A[0] + TL::TS - 1) / TL::TS);
+  else n = (long)((g.B[c] + TL::TF - 1) / TL::TF) * ((g.A[c] + TL::TS - 1) / TL::TS) * ((g.ns[c] + TL::TQ - 1) / TL::TQ);
+  return dim3((unsigned)(8 * ((n + 7) / 8)));
+}
+
+// the same tiles as a 3-D grid (x, y, z as in mx_tile_xyz)
+template <class TL>
+static dim3 mx_tile_grid3(const SkewGeo &g, int c)
 {
   if (c == 0)
     return dim3((unsigned)((g.ns[0] + TL::TQ - 1) / TL::TQ), (unsigned)((g.B[0] + TL::TF - 1) / TL::TF),
                 (unsigned)((g.A[0] + TL::TS - 1) / TL::TS));
-  return dim3((unsigned)((g.B[c] + TL::TF - 1) / TL::TF), (unsigned)((g.A[c] + TL::TS - 1) / TL::TS), (unsigned)g.ns[c]);
+  return dim3((unsigned)((g.B[c] + TL::TF - 1) / TL::TF), (unsigned)((g.A[c] + TL::TS - 1) / TL::TS),
+              (unsigned)((g.ns[c] + TL::TQ - 1) / TL::TQ));
 }
 
 // the sweep for slices of tb / 64 waves
@@ -462,10 +473,10 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
 {
   const SkewGeo &g = m->g;
   const MxPad &pk = a.pk;
-  // tile grids of the pre-passes (MxPreTile) and the copies (MxCopyTile)
+  // tile grids of the pre-passes (MxPreTile, XCD-ordered) and the copies (MxCopyTile, 3-D)
   const dim3 gq[3] = {mx_tile_grid<MxPreTile<0>>(g, 0), mx_tile_grid<MxPreTile<1>>(g, 1), mx_tile_grid<MxPreTile<2>>(g, 2)};
-  const dim3 gu[3] = {mx_tile_grid<MxCopyTile<0>>(g, 0), mx_tile_grid<MxCopyTile<1>>(g, 1),
-                      mx_tile_grid<MxCopyTile<2>>(g, 2)};
+  const dim3 gu[3] = {mx_tile_grid3<MxCopyTile<0>>(g, 0), mx_tile_grid3<MxCopyTile<1>>(g, 1),
+                      mx_tile_grid3<MxCopyTile<2>>(g, 2)};
   int tb[3];
   size_t lds[3];
   dim3 gs[3];

@@ -464,13 +464,37 @@ struct MxLu {
   MxFreq fr;
 };
 
+// Tile (x, y, z) of a pre-pass over component C launched as a 1-D grid of
+// 8 * ceil(tiles / 8) blocks: tiles are numbered x fastest (Ex: slices,
+// Ey/Ez: columns), then y, then z, and consecutive tiles run on one XCD
+// (block b on XCD b mod 8), so tiles that share halo rows and planes share
+// that XCD's L2.  False past the last tile.  (The copy k_mx_unskew keeps a
+// 3-D grid: its tiles that share cache lines are the anti-diagonal
+// neighbours (y - 1, z + 1), a multiple of 8 blocks apart in that order.)
+template <class TL, int C>
+__device__ __forceinline__ bool mx_tile_xyz(const SkewGeo &sk, int &x, int &y, int &z)
+{
+  const int A = sk.A[C], B = sk.B[C], ns = sk.ns[C];
+  const int gx = (C == 0) ? (ns + TL::TQ - 1) / TL::TQ : (B + TL::TF - 1) / TL::TF;
+  const int gy = (C == 0) ? (B + TL::TF - 1) / TL::TF : (A + TL::TS - 1) / TL::TS;
+  const int gz = (C == 0) ? (A + TL::TS - 1) / TL::TS : (ns + TL::TQ - 1) / TL::TQ;
+  const int nt = gx * gy * gz, per = (nt + 7) / 8;
+  const int T = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (T >= nt) return false;
+  x = T % gx; y = (T / gx) % gy; z = T / (gx * gy);
+  return true;
+}
+
+#ifndef PRE_TQ
+#define PRE_TQ 2  // slices per Ey/Ez pre-pass tile (the other components' planes are shared)
+#endif
 // Pre-pass tiles: TQ slices x TS rows x TF columns of a component's slices,
 // read in the natural order (x fastest, so the W and b reads are whole
 // rows) and written in the skewed order through LDS, so each diagonal's
 // run of the tile is one contiguous store.  Ex: slices run along x, so a
 // tile spans 32 slices (rows of 32 values; a diagonal's run is at most 4
-// records); Ey, Ez: a tile is 16 rows x 64 columns of one slice.
-template <int C> struct MxPreTile { static constexpr int TQ = 1, TS = 16, TF = 64; };
+// records); Ey, Ez: a tile is 2 slices x 8 rows x 64 columns.
+template <int C> struct MxPreTile { static constexpr int TQ = PRE_TQ, TS = 16 / PRE_TQ, TF = 64; };
 template <> struct MxPreTile<0> { static constexpr int TQ = 32, TS = 8, TF = 4; };
 // The copy from V (k_mx_unskew) reads diagonal runs and writes rows: its Ex
 // tile trades row width for run length the other way.
@@ -499,9 +523,10 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
   const int n1 = a.g.n1, n2 = a.g.n2, n3 = a.g.n3f;
   const int A = a.sk.A[C], B = a.sk.B[C], ns = a.sk.ns[C];
   const int t = threadIdx.x;
-  int q0, s0, f0;
-  if (C == 0) { q0 = blockIdx.x * TQ; f0 = 1 + blockIdx.y * TF; s0 = 1 + blockIdx.z * TS; }
-  else { f0 = 1 + blockIdx.x * TF; s0 = 1 + blockIdx.y * TS; q0 = blockIdx.z; }
+  int q0, s0, f0, bx, by, bz;
+  if (!mx_tile_xyz<TL, C>(a.sk, bx, by, bz)) return;
+  if (C == 0) { q0 = bx * TQ; f0 = 1 + by * TF; s0 = 1 + bz * TS; }
+  else { f0 = 1 + bx * TF; s0 = 1 + by * TS; q0 = bz * TQ; }
   const float *r1 = a.g.r1, *r2 = a.g.r2, *r3 = a.g.r3, *r1s = a.g.r1s, *r2s = a.g.r2s, *r3s = a.g.r3s;
   const c2 z0 = cmk(0, 0);
   // compute: lanes along x
@@ -509,7 +534,7 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
   for (int it = 0; it < NC / 256; it++) {
     int lq, ls, lf;
     if (C == 0) { lq = t % TQ; lf = (t / TQ) % TF; ls = t / (TQ * TF) + it * (256 / (TQ * TF)); }
-    else { lq = 0; lf = t % TF; ls = t / TF + it * (256 / TF); }
+    else { const int u = t + it * 256; lf = u % TF; ls = (u / TF) % TS; lq = u / (TS * TF); }
     const int q = q0 + lq, s = s0 + ls, f = f0 + lf;
     // the cell's slot: its tile's skewed order (1-based local diagonal dl)
     const int dl = ls + lf + 2;
@@ -594,7 +619,7 @@ __global__ __launch_bounds__(256) void k_mx_unskew(MxLu a)
   const int t = threadIdx.x;
   int q0, s0, f0;
   if (C == 0) { q0 = blockIdx.x * TQ; f0 = 1 + blockIdx.y * TF; s0 = 1 + blockIdx.z * TS; }
-  else { f0 = 1 + blockIdx.x * TF; s0 = 1 + blockIdx.y * TS; q0 = blockIdx.z; }
+  else { f0 = 1 + blockIdx.x * TF; s0 = 1 + blockIdx.y * TS; q0 = blockIdx.z * TQ; }
   // this thread's cells in the natural order (lanes along x) and their LDS slots
   int cell[NC / 256];
 #pragma unroll
