@@ -43,9 +43,12 @@ def build(verbose=False):
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
         o = os.path.join(OBJ, os.path.basename(src) + ".o")
-        # the FP32 inner-solve kernels may contract into FMAs; the FP64
-        # translation unit keeps the reference's unfused expressions
-        flags = HIP_FLAGS if "mixed" not in src else [f if f != "-ffp-contract=off" else "-ffp-contract=fast"
+        # the FP32 inner-solve kernels contract a*b+c within each source
+        # expression (=on, not =fast: fusing across inlined helpers would
+        # let unrelated code change a kernel's rounding, and a right-hand
+        # side's result must not depend on the batch it is solved in); the
+        # FP64 translation unit keeps the reference's unfused expressions
+        flags = HIP_FLAGS if "mixed" not in src else [f if f != "-ffp-contract=off" else "-ffp-contract=on"
                                                       for f in HIP_FLAGS]
         _run([HIPCC] + flags + ["-c", path, "-o", o])
         objs.append(o)

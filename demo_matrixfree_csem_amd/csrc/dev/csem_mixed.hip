@@ -194,8 +194,11 @@ static int mx_model(csem_model *m, MixState **out)
     const int N = m->nfft;
     const long NN = (long)N * N;
     if (N < 64 || 4096 % N) { rc = csem_fail(-3, "mixed path needs nfft in [64, 4096] (got %d)", N); goto fail; }
-    x->cw = 4096 / N;  // 16 values per thread of a 256-thread column block
-    if (hipFuncSetAttribute((const void *)k_mx_air_cols, hipFuncAttributeMaxDynamicSharedMemorySize, 65536) != hipSuccess) {
+    x->cw = N >= 2048 ? 1 : 2048 / N;  // columns per column-pass block of N cw / 16 threads, 16 values each
+    const int lc = (int)((2 * (size_t)N * x->cw + N) * sizeof(c2)), lr = (int)(3 * (size_t)N * sizeof(c2));
+    if (hipFuncSetAttribute((const void *)k_mx_air_cols, hipFuncAttributeMaxDynamicSharedMemorySize, lc) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_mx_air_rows, hipFuncAttributeMaxDynamicSharedMemorySize, lr) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_mx_air_rows_inv, hipFuncAttributeMaxDynamicSharedMemorySize, lr) != hipSuccess) {
       rc = csem_fail(-1, "LDS attribute");
       goto fail;
     }
@@ -407,11 +410,11 @@ static int mx_apply(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
     MxAir a = mx_air(m, x, q);
     KT(KT_M_HZ0, s, 24.0 * nh * R, (k_mx_hz0<<<dim3((nh + 255) / 256, R), 256, 0, s>>>(mx_geo(m, x), in, q->hz0)));
     KT(KT_M_AIR_ROWS, s, 8.0 * N * N * R,
-       (k_mx_air_rows<<<dim3(N, R), 256, 2 * (size_t)N * sizeof(c2), s>>>(a)));
+       (k_mx_air_rows<<<dim3(N, R), 256, 3 * (size_t)N * sizeof(c2), s>>>(a)));
     KT(KT_M_AIR_COLS, s, 40.0 * N * N * R,
-       (k_mx_air_cols<<<dim3(8 * ((N / x->cw + 7) / 8), R), 256, 2 * (size_t)N * x->cw * sizeof(c2), s>>>(a)));
+       (k_mx_air_cols<<<dim3(8 * ((N / x->cw + 7) / 8), R), N * x->cw / 16, (2 * (size_t)N * x->cw + N) * sizeof(c2), s>>>(a)));
     KT(KT_M_AIR_RINV, s, 16.0 * N * (m->nrowx + m->nrowy) * R,
-       (k_mx_air_rows_inv<<<dim3(m->nrowx + m->nrowy, R), 256, 2 * (size_t)N * sizeof(c2), s>>>(a)));
+       (k_mx_air_rows_inv<<<dim3(m->nrowx + m->nrowy, R), 256, 3 * (size_t)N * sizeof(c2), s>>>(a)));
     const int ng = (m->n1 + 1) * m->n2 + m->n1 * (m->n2 + 1);
     KT(KT_M_AIR_GHOST, s, 8.0 * ng * R, (k_mx_air_ghost<<<dim3((ng + 255) / 256, R), 256, 0, s>>>(a)));
   }

@@ -241,6 +241,17 @@ __global__ void k_mx_hz0(MxGeo g, const c2 *x, c2 *hz)
 
 // ----------------------------------------------------------- airwave FFT
 // The FP64 path's Stockham passes (radix 4, then radix 2), in FP32 on LDS.
+// Stockham FFT passes over `cols` interleaved transforms of length N in
+// LDS (element n of transform c at n*cols + c), twiddles tw[n] = W_N^n from
+// an LDS table.  A radix-16 pass is two radix-4 stages in registers
+// (16 = 4 x 4, the inner twiddles W_16^(q m1) constants), so N = 1024
+// takes three passes (16, 16, 4) with one barrier each (radix 4: five).
+__device__ __forceinline__ void mx_r4(c2 &u0, c2 &u1, c2 &u2, c2 &u3, int dir)
+{
+  const c2 a0 = cadd(u0, u2), a1 = csub(u0, u2), a2 = cadd(u1, u3), a3 = csub(u1, u3);
+  const c2 bb = dir < 0 ? cmk(a3.y, -a3.x) : cmk(-a3.y, a3.x);  // -i a3 forward, +i a3 inverse
+  u0 = cadd(a0, a2); u1 = cadd(a1, bb); u2 = csub(a0, a2); u3 = csub(a1, bb);
+}
 template <int RD>
 __device__ __forceinline__ void mx_fft_pass(const c2 *a, c2 *b, int N, int Ns, int cols, const c2 *tw, int dir)
 {
@@ -250,38 +261,65 @@ __device__ __forceinline__ void mx_fft_pass(const c2 *a, c2 *b, int N, int Ns, i
     const int c = w % cols, j = w / cols;
     const int k = j % Ns;
     const int o = (j / Ns) * RD * Ns + k;
+    const int ts = k * (N / (RD * Ns));  // twiddle stride of this butterfly
     c2 u[RD];
 #pragma unroll
     for (int r = 0; r < RD; r++) {
       u[r] = a[(j + r * nb) * cols + c];
       if (r != 0 && k != 0) {
-        c2 tv = tw[(long)r * k * (N / (RD * Ns))];
+        c2 tv = tw[r * ts];
         if (dir > 0) tv.y = -tv.y;
         u[r] = cmul(u[r], tv);
       }
     }
-    if (RD == 4) {
-      const c2 a0 = cadd(u[0], u[2]), a1 = csub(u[0], u[2]), a2 = cadd(u[1], u[RD - 1]), a3 = csub(u[1], u[RD - 1]);
-      const c2 bb = dir < 0 ? cmk(a3.y, -a3.x) : cmk(-a3.y, a3.x);
-      b[o * cols + c] = cadd(a0, a2);
-      b[(o + Ns) * cols + c] = cadd(a1, bb);
-      b[(o + 2 * Ns) * cols + c] = csub(a0, a2);
-      b[(o + 3 * Ns) * cols + c] = csub(a1, bb);
+    if (RD == 16) {
+      // radix 4 over p (r = q + 4p), twiddle W_16^(q m1), radix 4 over q
+#pragma unroll
+      for (int q = 0; q < 4; q++) mx_r4(u[q], u[q + 4], u[q + 8], u[q + 12], dir);
+      const float c1 = 0.923879532511286756f, s1 = 0.382683432365089772f, h = 0.707106781186547524f;
+      const float sg = dir < 0 ? -1.f : 1.f;
+      // u[q + 4 m1] *= W_16^(q m1) = (cos, sg sin)(2 pi q m1 / 16)
+      const c2 W1 = cmk(c1, sg * s1), W2 = cmk(h, sg * h), W3 = cmk(s1, sg * c1), W4 = cmk(0.f, sg),
+               W6 = cmk(-h, sg * h), W9 = cmk(-c1, -sg * s1);
+      u[5] = cmul(u[5], W1); u[9] = cmul(u[9], W2); u[13] = cmul(u[13], W3);
+      u[6] = cmul(u[6], W2); u[10] = cmul(u[10], W4); u[14] = cmul(u[14], W6);
+      u[7] = cmul(u[7], W3); u[11] = cmul(u[11], W6); u[15] = cmul(u[15], W9);
+#pragma unroll
+      for (int m1 = 0; m1 < 4; m1++) mx_r4(u[4 * m1], u[4 * m1 + 1], u[4 * m1 + 2], u[4 * m1 + 3], dir);
+      // output m = m1 + 4 m2 sits in u[4 m1 + m2]
+#pragma unroll
+      for (int m1 = 0; m1 < 4; m1++)
+#pragma unroll
+        for (int m2 = 0; m2 < 4; m2++) b[(o + (m1 + 4 * m2) * Ns) * cols + c] = u[4 * m1 + m2];
+    } else if (RD == 4) {
+      mx_r4(u[0], u[1], u[2], u[3], dir);
+#pragma unroll
+      for (int m = 0; m < 4; m++) b[(o + m * Ns) * cols + c] = u[m];
     } else {
       b[o * cols + c] = cadd(u[0], u[1]);
       b[(o + Ns) * cols + c] = csub(u[0], u[1]);
     }
   }
 }
+// MAXR 16: fewest passes (the column pass: one butterfly per thread per
+// pass); MAXR 4: four times the butterflies per pass (a single row per
+// block keeps all its threads busy)
+template <int MAXR>
 __device__ __forceinline__ c2 *mx_fft(c2 *a, c2 *b, int N, int cols, const c2 *tw, int dir)
 {
   for (int Ns = 1; Ns < N;) {
-    if ((N / Ns) % 4 == 0) { mx_fft_pass<4>(a, b, N, Ns, cols, tw, dir); Ns *= 4; }
+    if (MAXR >= 16 && (N / Ns) % 16 == 0) { mx_fft_pass<16>(a, b, N, Ns, cols, tw, dir); Ns *= 16; }
+    else if ((N / Ns) % 4 == 0) { mx_fft_pass<4>(a, b, N, Ns, cols, tw, dir); Ns *= 4; }
     else { mx_fft_pass<2>(a, b, N, Ns, cols, tw, dir); Ns *= 2; }
     __syncthreads();
     c2 *tmp = a; a = b; b = tmp;
   }
   return a;
+}
+// the twiddle table into LDS (callers sync before the first pass)
+__device__ __forceinline__ void mx_tw_lds(c2 *dst, const c2 *tw, int N)
+{
+  for (int e = threadIdx.x; e < N; e += blockDim.x) dst[e] = tw[e];
 }
 
 struct MxAir {
@@ -303,7 +341,8 @@ __global__ void k_mx_air_rows(MxAir a)
 {
   extern __shared__ c2 smc[];
   const int N = a.N, i2 = blockIdx.x, r = blockIdx.y, n1 = a.n1;
-  c2 *A = smc, *B = smc + N;
+  c2 *A = smc, *B = smc + N, *T = smc + 2 * N;
+  mx_tw_lds(T, a.tw, N);
   const c2 *h = a.hz + r * (long)n1 * a.n2;
   const int j2 = a.jy[i2];
   const float w2 = a.wy[i2];
@@ -321,15 +360,15 @@ __global__ void k_mx_air_rows(MxAir a)
     A[i1] = v;
   }
   __syncthreads();
-  c2 *res = mx_fft(A, B, N, 1, a.tw, -1);
+  c2 *res = mx_fft<4>(A, B, N, 1, T, -1);
   c2 *out = a.buf + r * (long)N * N + (long)N * i2;
   for (int i1 = threadIdx.x; i1 < N; i1 += blockDim.x) out[i1] = res[i1];
 }
 
 // Column pass: forward transform of cw columns, then for sH1 -> bufx and
 // sH2 -> bufy the product and the inverse column transform.  The forward
-// spectrum is kept in registers (16 values per thread), so LDS holds only
-// the two Stockham buffers: N*cw must equal 16*blockDim.
+// spectrum is kept in registers (16 values per thread), so LDS holds the
+// two Stockham buffers and the twiddles: N*cw must equal 16*blockDim.
 __global__ __launch_bounds__(256) void k_mx_air_cols(MxAir a)
 {
   extern __shared__ c2 smc[];
@@ -339,11 +378,12 @@ __global__ __launch_bounds__(256) void k_mx_air_cols(MxAir a)
   if (g >= ng) return;
   const int N = a.N, cw = a.cw, c0 = g * cw, r = blockIdx.y;
   const long off = r * (long)N * N;
-  c2 *A = smc, *B = smc + N * cw;
+  c2 *A = smc, *B = smc + N * cw, *T = smc + 2 * N * cw;
   const int t = threadIdx.x, nt = blockDim.x;
+  mx_tw_lds(T, a.tw, N);
   for (int e = t; e < N * cw; e += nt) A[e] = a.buf[off + c0 + e % cw + (long)N * (e / cw)];
   __syncthreads();
-  c2 *res = mx_fft(A, B, N, cw, a.tw, -1);
+  c2 *res = mx_fft<16>(A, B, N, cw, T, -1);
   c2 S[16];
 #pragma unroll
   for (int q = 0; q < 16; q++) S[q] = res[t + q * nt];
@@ -357,7 +397,7 @@ __global__ __launch_bounds__(256) void k_mx_air_cols(MxAir a)
       A[e] = cmul(S[q], mul[c0 + e % cw + (long)N * (e / cw)]);
     }
     __syncthreads();
-    c2 *o = mx_fft(A, B, N, cw, a.tw, +1);
+    c2 *o = mx_fft<16>(A, B, N, cw, T, +1);
     for (int e = t; e < N * cw; e += nt) dst[c0 + e % cw + (long)N * (e / cw)] = o[e];
     __syncthreads();
   }
@@ -370,10 +410,11 @@ __global__ void k_mx_air_rows_inv(MxAir a)
   const int N = a.N, w = blockIdx.x, r = blockIdx.y;
   c2 *buf = ((w < a.nrowx) ? a.bufx : a.bufy) + r * (long)N * N;
   const long row = a.rows[w];
-  c2 *A = smc, *B = smc + N;
+  c2 *A = smc, *B = smc + N, *T = smc + 2 * N;
+  mx_tw_lds(T, a.tw, N);
   for (int i1 = threadIdx.x; i1 < N; i1 += blockDim.x) A[i1] = buf[i1 + N * row];
   __syncthreads();
-  c2 *res = mx_fft(A, B, N, 1, a.tw, +1);
+  c2 *res = mx_fft<4>(A, B, N, 1, T, +1);
   for (int i1 = threadIdx.x; i1 < N; i1 += blockDim.x) buf[i1 + N * row] = res[i1];
 }
 

@@ -181,6 +181,45 @@ def test_fp32_kernels_multi_tile(workdir):
         ses.close()
 
 
+@pytest.mark.parametrize("shape", [dict(), dict(nx=130, ny=75, nz=70)], ids=["tiny", "multi-wave"])
+def test_fp32_kernels_same_alone_and_batched(workdir, shape):
+    """Each right-hand side's FP32 operator and LU-SGS result is the same,
+    bit for bit, whether it is applied alone (R=1) or in a batch of 2, 3 or
+    4: no kernel's arithmetic depends on the batch size (or on the operator
+    tile the autotuner picks for it), so a solve's path does not either."""
+    print("case fp32 alone vs batched %s" % shape, flush=True)
+    s = surveys.tiny(4, airbc=1, **shape)
+    s.flags.update(n1fft=64, n2fft=64)
+    s.freqs = [0.375, 0.75, 1.25, 0.25]
+    surveys.write_files(workdir, s)
+    ses = Session(s.argv(workdir))
+    L = _lib.lib()
+    try:
+        n = ses.n
+        rng = np.random.default_rng(21)
+        x = (rng.standard_normal((4, n)) + 1j * rng.standard_normal((4, n))).astype(np.complex64)
+        for name, fn in (("apply32", L.csem_apply32), ("lusgs32", L.csem_lusgs32)):
+            alone = []
+            for r in range(4):
+                fr = (_lib.FreqT * 1)(freq_consts(s.freqs[r]))
+                dx, dy = _dev(x[r:r + 1].copy()), _dev(np.zeros_like(x[r:r + 1]))
+                _lib.check(fn(ses.model, 1, fr, dx, dy, None), name)
+                alone.append(_host(dy, x[r:r + 1])[0])
+                L.csem_free(dx)
+                L.csem_free(dy)
+            for R in (2, 3, 4):
+                fr = (_lib.FreqT * R)(*[freq_consts(f) for f in s.freqs[:R]])
+                dx, dy = _dev(x[:R].copy()), _dev(np.zeros_like(x[:R]))
+                _lib.check(fn(ses.model, R, fr, dx, dy, None), name)
+                y = _host(dy, x[:R])
+                L.csem_free(dx)
+                L.csem_free(dy)
+                for r in range(R):
+                    assert np.array_equal(y[r], alone[r]), (name, R, r, float(np.abs(y[r] - alone[r]).max()))
+    finally:
+        ses.close()
+
+
 def _receivers(ses, x):
     return ses.extract(x).astype(np.complex128)
 
