@@ -34,7 +34,7 @@ int csem_vec_addscaled(long n, void *y, double re, double im, const void *x, voi
 static struct {
   emf_t *emf;
   csem_model *model;
-  void *dx, *dy, *dh, *dg1, *dg2, *dred;
+  void *dx, *dy, *dh, *dg1, *dg2, *dred, *dhf;
   long nvec;
   int last_iters;
   /* airwave host tables (airbc_init) */
@@ -238,7 +238,8 @@ void fdfd_free(emf_t *emf)
   if (G.dg1) csem_free(G.dg1);
   if (G.dg2) csem_free(G.dg2);
   if (G.dred) csem_free(G.dred);
-  G.dx = G.dy = G.dh = G.dg1 = G.dg2 = G.dred = NULL;
+  if (G.dhf) csem_free(G.dhf);
+  G.dx = G.dy = G.dh = G.dg1 = G.dg2 = G.dred = G.dhf = NULL;
   G.nvec = 0;
 }
 
@@ -265,6 +266,23 @@ static int need_model(const char *who)
   return 1;
 }
 
+/* fdfd.c:117-149: fdfd_apply leaves H = curl E / (i w mu0) of its input in
+ * emf->xh, Hx at 0, Hy at lHx, Hz at lHx + lHy.  lHy is the reference's
+ * size (fdfd.c:59, k extent n2): when n3 > n2 the reference's Hy runs into
+ * Hz and past the array; here the Hy tail beyond lHy is dropped. */
+static int store_h(const csem_freq *f)
+{
+  emf_t *e = G.emf;
+  const long hx = e->lHx, hy = (long)e->n1 * (e->n2 + 1) * e->n3, hz = e->lHz;
+  const long hyc = hy < e->lHy ? hy : e->lHy;
+  int rc = G.dhf ? 0 : csem_malloc(&G.dhf, (hx + hy + hz) * 16);
+  if (!rc) rc = csem_curl_h(G.model, f, G.dx, G.dhf, NULL);
+  if (!rc) rc = csem_memcpy_d2h(e->xh, G.dhf, (hx + hyc) * 16, NULL);
+  if (!rc) rc = csem_memcpy_d2h(e->xh + hx + e->lHy, (char *)G.dhf + (hx + hy) * 16, hz * 16, NULL);
+  if (rc) die_dev("fdfd_apply: H to emf->xh", rc);
+  return rc;
+}
+
 void fdfd_apply(int n, csem_complex *x, csem_complex *y)
 {
   if (!need_model("fdfd_apply")) return;
@@ -273,6 +291,7 @@ void fdfd_apply(int n, csem_complex *x, csem_complex *y)
   DEV(csem_memcpy_h2d(G.dx, x, (long)n * 16, NULL));
   DEV(csem_apply(G.model, &f, G.dx, G.dy, NULL));
   DEV(csem_memcpy_d2h(y, G.dy, (long)n * 16, NULL));
+  if (G.emf->xh && store_h(&f)) return;
   csem_set_status(0);
 }
 
@@ -635,9 +654,60 @@ void inject_source(acq_t *acq, emf_t *emf, csem_complex *b)
   }
 }
 
+/* H = curl E / (i w mu0) on face (k, j, i) of the operator grid, component
+ * c (0 Hx, 1 Hy, 2 Hz), with the expressions of fdfd.c:131-146; the
+ * vertical spacings are those of operator plane k (fdfd.c:106-109). */
+static csem_complex h_face(const emf_t *emf, const csem_complex *x, int c, int k, int j, int i, csem_complex iwmu)
+{
+  const csem_complex *Ex = x, *Ey = x + emf->lEx, *Ez = x + emf->lEx + emf->lEy;
+  const double *d1s = emf->d1s, *d2s = emf->d2s, *d3s = emf->d3s + (emf->airbc ? emf->nb : 0);
+  csem_complex t1, t2;
+  if (c == 0) {
+    t1 = (Ez[I3(k, j + 1, i)] - Ez[I3(k, j, i)]) / d2s[j];
+    t2 = (Ey[I2(k + 1, j, i)] - Ey[I2(k, j, i)]) / d3s[k];
+  } else if (c == 1) {
+    t1 = (Ex[I1(k + 1, j, i)] - Ex[I1(k, j, i)]) / d3s[k];
+    t2 = (Ez[I3(k, j, i + 1)] - Ez[I3(k, j, i)]) / d1s[i];
+  } else {
+    t1 = (Ey[I2(k, j, i + 1)] - Ey[I2(k, j, i)]) / d1s[i];
+    t2 = (Ex[I1(k, j + 1, i)] - Ex[I1(k, j, i)]) / d2s[j];
+  }
+  return (t1 - t2) * iwmu;
+}
+
+/* Hx, Hy, Hz at a receiver (no counterpart in the reference, whose H
+ * channels stay zero, inject_extract.c:123-213): H on the faces around the
+ * receiver as fdfd_apply forms it (Hx at (x1, x2s, x3s), Hy at (x1s, x2,
+ * x3s), Hz at (x1s, x2s, x3)), interpolated trilinearly in the order the E
+ * channels use.  NaN when the eight faces are not all inside the operator
+ * grid (a receiver within half a cell of its top or bottom). */
+static csem_complex extract_h(const emf_t *emf, const csem_complex *x, int c, double rx, double ry, double rz)
+{
+  const int n1 = emf->nx + 2 * emf->nb, n2 = emf->ny + 2 * emf->nb, n3 = emf->nz + 2 * emf->nb;
+  const double *xa = c == 0 ? emf->x1 : emf->x1s, *ya = c == 1 ? emf->x2 : emf->x2s, *za = c == 2 ? emf->x3 : emf->x3s;
+  const int i = find_index(c == 0 ? n1 + 1 : n1, (double *)xa, rx);
+  const int j = find_index(c == 1 ? n2 + 1 : n2, (double *)ya, ry);
+  int k = find_index(c == 2 ? n3 + 1 : n3, (double *)za, rz);
+  const double w1 = (rx - xa[i]) / (xa[i + 1] - xa[i]), w2 = (ry - ya[j]) / (ya[j + 1] - ya[j]),
+               w3 = (rz - za[k]) / (za[k + 1] - za[k]);
+  const csem_complex iwmu = 1. / emf->I_omega_mu0;
+  csem_complex s = 0;
+  int q;
+  if (emf->airbc) k -= emf->nb;
+  /* faces k and k+1: Hx, Hy exist on planes 0..n3-1, Hz on 0..n3 */
+  if (k < 0 || k + 1 > (c == 2 ? emf->n3 : emf->n3 - 1)) return NAN;
+  for (q = 0; q < 8; q++) {
+    int di = q & 1, dj = (q >> 1) & 1, dk = q >> 2;
+    double f1 = di ? w1 : (1. - w1), f2 = dj ? w2 : (1. - w2), f3 = dk ? w3 : (1. - w3);
+    s += h_face(emf, x, c, k + dk, j + dj, i + di, iwmu) * f1 * f2 * f3;
+  }
+  return s;
+}
+
 /* inject_extract.c:123-213: trilinear Ex, Ey; for Ez the vertical current
  * sigma33*Ez is interpolated (at the unshifted plane index) and scaled by
- * rho_water.  H channels are left as they are (the reference has none). */
+ * rho_water.  Hx, Hy, Hz (beyond the reference, which leaves them zero)
+ * come from extract_h. */
 void extract_emf(acq_t *acq, emf_t *emf, csem_complex *x, int ifreq)
 {
   const int n1 = emf->nx + 2 * emf->nb, n2 = emf->ny + 2 * emf->nb, n3 = emf->nz + 2 * emf->nb;
@@ -645,8 +715,21 @@ void extract_emf(acq_t *acq, emf_t *emf, csem_complex *x, int ifreq)
   double w1, w2, w3;
   csem_complex s, *Ex = x, *Ey = x + emf->lEx, *Ez = x + emf->lEx + emf->lEy;
   for (ic = 0; ic < emf->nchrec; ic++) {
-    int ch = !strcmp(emf->chrec[ic], "Ex") ? 0 : !strcmp(emf->chrec[ic], "Ey") ? 1 : !strcmp(emf->chrec[ic], "Ez") ? 2 : -1;
+    const char *cn = emf->chrec[ic];
+    int ch = !strcmp(cn, "Ex") ? 0 : !strcmp(cn, "Ey") ? 1 : !strcmp(cn, "Ez") ? 2
+           : !strcmp(cn, "Hx") ? 3 : !strcmp(cn, "Hy") ? 4 : !strcmp(cn, "Hz") ? 5 : -1;
     if (ch < 0) continue;
+    if (ch >= 3) {
+      int outside = 0;
+      for (ir = 0; ir < acq->nrec; ir++) {
+        s = extract_h(emf, x, ch - 3, acq->rec_x1[ir], acq->rec_x2[ir], acq->rec_x3[ir]);
+        outside += isnan(creal(s));
+        emf->dcal_fd[ic][ifreq][ir] = s;
+      }
+      if (outside)
+        fprintf(stderr, "extract_emf: %d receiver(s) too close to the grid's top or bottom for %s (NaN)\n", outside, cn);
+      continue;
+    }
     for (ir = 0; ir < acq->nrec; ir++) {
       const double *xa = ch == 0 ? emf->x1s : emf->x1, *ya = ch == 1 ? emf->x2s : emf->x2,
                    *za = ch == 2 ? emf->x3s : emf->x3;

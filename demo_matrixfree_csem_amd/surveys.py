@@ -118,7 +118,8 @@ def layered_100(seed=1):
     """100^3 uniform 100 m cells on x,y in [-5000,5000], z in [0,10000]:
     water (0.3125 ohm-m) above 1000 m, then 8 layers of 1125 m with seeded
     log-uniform resistivity in [0.5, 200] ohm-m (by cell centre).  One Ex
-    source at (0,0,950), 51 receivers on y=0, z=1000; 0.25/0.75/1.25 Hz."""
+    source at (0,0,950), 51 receivers on y=0, z=1000; 0.25/0.75/1.25 Hz;
+    all six channels (the H channels are beyond the reference)."""
     rng = np.random.default_rng(seed)
     x = (-5000 + 100 * np.arange(101)).astype(np.float32)
     z = (100 * np.arange(101)).astype(np.float32)
@@ -131,7 +132,7 @@ def layered_100(seed=1):
     return Survey("layered_100", Model(x, x.copy(), z, rho, rho.copy(), rho.copy()),
                   Acquisition(_acq_table([(0.0, 0.0, 950.0)]), _acq_table(rec)), [0.25, 0.75, 1.25],
                   dict(fvm=0, airbc=1, preco=1, algopt=1, tol=1e-8, niter=1000, nb=10,
-                       chsrc="Ex", chrec="Ex,Ey,Ez"))
+                       chsrc="Ex", chrec="Ex,Ey,Ez,Hx,Hy,Hz"))
 
 
 def tiny(seed=0, nx=10, ny=9, nz=7, airbc=1, aniso=True, dz=100.0):

@@ -379,6 +379,7 @@ long orc_info(const orc_t *m, int what)
   case 13: return m->lHx;
   case 14: return m->lHy;
   case 15: return m->lHz;
+  case 16: return m->hz_off;
   }
   return -1;
 }
@@ -1222,8 +1223,60 @@ void orc_inject(orc_t *m, int ch, float sx, float sy, float sz, zc *b)
 }
 
 /* extract_emf, inject_extract.c:123-213 (one receiver, one channel) */
+/* H at face (k, j, i) of the operator grid (c: 0 Hx, 1 Hy, 2 Hz) from E,
+ * with the expressions of fdfd.c:131-146 and the vertical spacing of
+ * operator plane k (fdfd.c:106-109). */
+static zc orc_h_face(const orc_t *m, const zc *x, int c, int k, int j, int i)
+{
+  const int n1 = m->n1, n2 = m->n2;
+  const zc *Ex = x, *Ey = x + m->lEx, *Ez = x + m->lEx + m->lEy;
+  const double *d3s = &m->d3s[m->airbc ? m->nb : 0];
+  zc t1, t2;
+  if (c == 0) {
+    t1 = (Ez[id3(k, j + 1, i)] - Ez[id3(k, j, i)]) / m->d2s[j];
+    t2 = (Ey[id2(k + 1, j, i)] - Ey[id2(k, j, i)]) / d3s[k];
+  } else if (c == 1) {
+    t1 = (Ex[id1(k + 1, j, i)] - Ex[id1(k, j, i)]) / d3s[k];
+    t2 = (Ez[id3(k, j, i + 1)] - Ez[id3(k, j, i)]) / m->d1s[i];
+  } else {
+    t1 = (Ey[id2(k, j, i + 1)] - Ey[id2(k, j, i)]) / m->d1s[i];
+    t2 = (Ex[id1(k, j + 1, i)] - Ex[id1(k, j, i)]) / m->d2s[j];
+  }
+  return (t1 - t2) * m->iwmu;
+}
+
+/* extract_emf, inject_extract.c:123-213, for ch 0..2 (Ex, Ey, Ez).  ch 3..5
+ * (Hx, Hy, Hz) have no counterpart in the reference (its H channels stay
+ * zero): H is formed on the eight faces around the receiver as fdfd_apply
+ * forms it (Hx at (x1, x2s, x3s), Hy at (x1s, x2, x3s), Hz at (x1s, x2s,
+ * x3)) and interpolated trilinearly; NaN when a face lies outside the
+ * operator grid. */
 orc_c128 orc_extract(orc_t *m, int ch, float rx, float ry, float rz, const zc *x)
 {
+  if (ch >= 3) {
+    const int c = ch - 3, N1 = m->nx + 2 * m->nb, N2 = m->ny + 2 * m->nb, N3 = m->nz + 2 * m->nb;
+    const double *xa = c == 0 ? m->x1 : m->x1s, *ya = c == 1 ? m->x2 : m->x2s, *za = c == 2 ? m->x3 : m->x3s;
+    int i = orc_find_index(c == 0 ? N1 + 1 : N1, xa, rx);
+    int j = orc_find_index(c == 1 ? N2 + 1 : N2, ya, ry);
+    int k = orc_find_index(c == 2 ? N3 + 1 : N3, za, rz);
+    double w1 = (rx - xa[i]) / (xa[i + 1] - xa[i]);
+    double w2 = (ry - ya[j]) / (ya[j + 1] - ya[j]);
+    double w3 = (rz - za[k]) / (za[k + 1] - za[k]);
+    zc s = 0;
+    if (m->airbc) k -= m->nb;
+    if (k < 0 || k + 1 > (c == 2 ? m->n3 : m->n3 - 1)) {
+      return NAN;
+    }
+    s += orc_h_face(m, x, c, k, j, i) * (1. - w1) * (1. - w2) * (1. - w3);
+    s += orc_h_face(m, x, c, k, j, i + 1) * w1 * (1. - w2) * (1. - w3);
+    s += orc_h_face(m, x, c, k, j + 1, i) * (1. - w1) * w2 * (1. - w3);
+    s += orc_h_face(m, x, c, k, j + 1, i + 1) * w1 * w2 * (1. - w3);
+    s += orc_h_face(m, x, c, k + 1, j, i) * (1. - w1) * (1. - w2) * w3;
+    s += orc_h_face(m, x, c, k + 1, j, i + 1) * w1 * (1. - w2) * w3;
+    s += orc_h_face(m, x, c, k + 1, j + 1, i) * (1. - w1) * w2 * w3;
+    s += orc_h_face(m, x, c, k + 1, j + 1, i + 1) * w1 * w2 * w3;
+    return s;
+  }
   int n1 = m->nx + 2 * m->nb, n2 = m->ny + 2 * m->nb, n3 = m->nz + 2 * m->nb, i, j, k, kk;
   double w1, w2, w3;
   const zc *Ex = x, *Ey = x + m->lEx, *Ez = x + m->lEx + m->lEy;

@@ -125,6 +125,18 @@ class Oracle:
             return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_double)), (dbl[name],)).copy()
         raise KeyError(name)
 
+    def h_field(self):
+        """H = curl E / (i w mu0) of the last apply's input (fdfd.c:125-149)
+        as (Hx, Hy, Hz): Hx (n1+1)*n2*n3, Hy n1*(n2+1)*n3, Hz n1*n2*(n3+1)
+        on the operator grid."""
+        L = lib()
+        n1, n2, n3 = self.n1, self.n2, self.n3
+        lhx, lhz, off = L.orc_info(self.h, 13), L.orc_info(self.h, 15), L.orc_info(self.h, 16)
+        lhy = n1 * (n2 + 1) * n3
+        ptr = C.cast(L.orc_array(self.h, b"xh"), C.POINTER(C.c_double))
+        xh = np.ctypeslib.as_array(ptr, (2 * (off + lhz),)).copy().view(np.complex128)
+        return xh[:lhx], xh[lhx:lhx + lhy], xh[off:off + lhz]
+
     def set_red(self, red):
         lib().orc_set_red(self.h, red)
 

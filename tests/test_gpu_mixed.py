@@ -34,7 +34,7 @@ FREQS = [0.25, 0.75, 1.25]
 
 def _setup(workdir, seed=2, **kw):
     s = surveys.tiny(seed, airbc=1, **kw)
-    s.flags.update(n1fft=64, n2fft=64)
+    s.flags.update(n1fft=64, n2fft=64, chrec="Ex,Ey,Ez,Hx,Hy,Hz")
     s.freqs = list(FREQS)
     surveys.write_files(workdir, s)
     ses = Session(s.argv(workdir))

@@ -107,6 +107,52 @@ def test_airbc_bitwise(workdir):
         ses.close()
 
 
+@pytest.mark.parametrize("airbc", [0, 1])
+def test_curl_h_bitwise(workdir, airbc):
+    """csem_curl_h: H = curl E / (i w mu0) on every face (fdfd.c:125-149),
+    against the H the oracle's fdfd_apply leaves in its scratch."""
+    import ctypes as C
+    from demo_matrixfree_csem_amd import _lib
+    from demo_matrixfree_csem_amd.session import freq_consts
+    print("case curl_h airbc=%d" % airbc, flush=True)
+    s, ses, orc = _pair(workdir, airbc)
+    L = _lib.lib()
+    dx, dh = C.c_void_p(), C.c_void_p()
+    try:
+        n1, n2, n3 = orc.n1, orc.n2, orc.n3
+        nh = (n1 + 1) * n2 * n3 + n1 * (n2 + 1) * n3 + n1 * n2 * (n3 + 1)
+        x = _rand(ses.n, 13)
+        assert L.csem_malloc(C.byref(dx), x.nbytes) == 0 and L.csem_malloc(C.byref(dh), nh * 16) == 0
+        assert L.csem_memcpy_h2d(dx, x.ctypes.data, x.nbytes, None) == 0
+        for f in (0.25, 1.25):
+            orc.set_freq(f)
+            orc.apply(x)
+            want = np.concatenate(orc.h_field())
+            _lib.check(L.csem_curl_h(ses.model, C.byref(freq_consts(f)), dx, dh, None), "csem_curl_h")
+            got = np.zeros(nh, np.complex128)
+            assert L.csem_memcpy_d2h(got.ctypes.data, dh, got.nbytes, None) == 0
+            assert np.array_equal(got, want), np.abs(got - want).max() / np.abs(want).max()
+            # the drop-in fdfd_apply leaves the same H in emf->xh, at the
+            # reference's offsets (Hy at lHx, Hz at lHx + lHy; fdfd.c:117-119)
+            e = ses.emf
+            xh = np.full(e.lH, np.nan + 0j)
+            e.xh = C.c_void_p(xh.ctypes.data)
+            try:
+                ses.set_freq(f)
+                ses.fdfd_apply(x)
+            finally:
+                e.xh = None
+            hx, hy, hz = orc.h_field()
+            assert np.array_equal(xh[:e.lHx], hx)
+            assert np.array_equal(xh[e.lHx:e.lHx + hy.size], hy)
+            assert np.array_equal(xh[e.lHx + e.lHy:e.lHx + e.lHy + e.lHz], hz)
+    finally:
+        for p in (dx, dh):
+            if p.value:
+                L.csem_free(p)
+        ses.close()
+
+
 def test_cdotprod_chunked_order(workdir):
     print("case cdotprod", flush=True)
     s, ses, orc = _pair(workdir, 0)
