@@ -53,7 +53,7 @@ def _traffic():
         return json.load(f).get("bytes_per_launch", {})
 
 
-def _roofline(stats, name):
+def _roofline(stats, name, copy_gbs=None):
     if name not in stats or stats[name][1] == 0:
         return None
     ms, cnt, by = stats[name]
@@ -61,9 +61,35 @@ def _roofline(stats, name):
     per_launch = by / cnt
     gbs = per_launch / avg_s / 1e9
     tr = _traffic().get(name)
-    return {"kernel": name, "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": int(tr) if tr else None,
-            "bytes_per_launch": int(per_launch), "avg_us": round(avg_s * 1e6, 3), "launches": cnt}
+    out = {"kernel": name, "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": int(tr) if tr else None,
+           "bytes_per_launch": int(per_launch), "avg_us": round(avg_s * 1e6, 3), "launches": cnt}
+    if copy_gbs:
+        out["frac_of_copy"] = round(gbs / copy_gbs, 4)
+    return out
+
+
+def _copy_rate(L, reps=10):
+    """GB/s of a plain device copy of 1 GiB (k_copy16: 16-byte loads and
+    stores), bytes read plus written over the host-timed duration of `reps`
+    back-to-back copies: the bandwidth a streaming kernel can reach here."""
+    from demo_matrixfree_csem_amd import _lib
+    a, b = C.c_void_p(), C.c_void_p()
+    for d in (a, b):
+        _lib.check(L.csem_malloc(C.byref(d), 2 ** 30), "csem_malloc")
+    try:
+        for _ in range(2):
+            _lib.check(L.csem_copy(b, a, 2 ** 30, None), "csem_copy")
+        L.csem_sync(None)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _lib.check(L.csem_copy(b, a, 2 ** 30, None), "csem_copy")
+        L.csem_sync(None)
+        dt = time.perf_counter() - t0
+    finally:
+        L.csem_free(a)
+        L.csem_free(b)
+    return 2 * 2 ** 30 * reps / dt / 1e9
 
 
 def _lusgs_roofline(stats, dims, R):
@@ -309,6 +335,7 @@ def main():
     L.csem_kernel_timing(0)
 
     dominant = max(stm, key=lambda k: stm[k][0])
+    copy_gbs = _copy_rate(L)
     dd = ses.dims
     lu_dims = (dd["n1"], dd["n2"], dd["nz"] + 2 * dd["nb"])
     out = {
@@ -331,9 +358,11 @@ def main():
         "mixed": mixed,
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": round(mixed["solves_per_s"] / plain["solves_per_s"], 3) if plain else None,
-        "roofline": _roofline(stm, dominant),
-        "apply_roofline": _roofline(stm, "k_mx_apply"),
-        "apply_fp64_roofline": _roofline(stp, "k_apply") if stp else None,
+        "roofline": _roofline(stm, dominant, copy_gbs),
+        "apply_roofline": _roofline(stm, "k_mx_apply", copy_gbs),
+        "apply_fp64_roofline": _roofline(stp, "k_apply", copy_gbs) if stp else None,
+        "hbm_copy": {"achieved": round(copy_gbs, 1), "unit": "GB/s", "frac": round(copy_gbs / HBM_PEAK_GBS, 4),
+                     "what": "k_copy16, 1 GiB device copy, bytes read + written / time"},
         "lusgs_roofline_mixed": _lusgs_roofline(stm, lu_dims, R),
         "lusgs_roofline_fp64": _lusgs_roofline(stp, lu_dims, 0) if stp else None,
         "kernel_ms_mixed": {k: round(v[0] / args.steps, 3) for k, v in sorted(stm.items(), key=lambda kv: -kv[1][0])},

@@ -996,11 +996,15 @@ extern "C" int csem_memcpy_d2h(void *dst, const void *src, long bytes, void *str
   HIP_OK(hipStreamSynchronize(S(stream)));
   return 0;
 }
-// A plain streaming copy (16-byte loads and stores, grid-stride): the
-// reference kernel against which hardware byte counters are scaled.
+// A plain streaming copy, one 16-byte load and store per thread: the
+// reference kernel against which hardware byte counters are scaled, and
+// the bandwidth a streaming kernel reaches (measured: one element per
+// thread over many blocks beats grid-stride loops and several elements
+// per thread).
 __global__ __launch_bounds__(256) void k_copy16(long n16, const float4 *src, float4 *dst)
 {
-  for (long e = blockIdx.x * 256L + threadIdx.x; e < n16; e += (long)gridDim.x * 256) dst[e] = src[e];
+  const long e = blockIdx.x * 256L + threadIdx.x;
+  if (e < n16) dst[e] = src[e];
 }
 extern "C" int csem_copy(void *dst, const void *src, long bytes, void *stream)
 {
@@ -1008,8 +1012,8 @@ extern "C" int csem_copy(void *dst, const void *src, long bytes, void *stream)
   if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
     return csem_fail(-3, "csem_copy needs 16-byte aligned buffers and a multiple of 16 bytes");
   const long n16 = bytes / 16;
-  const long blocks = std::min<long>((n16 + 255) / 256, 256L * 32);
-  k_copy16<<<(unsigned)blocks, 256, 0, S(stream)>>>(n16, (const float4 *)src, (float4 *)dst);
+  if ((n16 + 255) / 256 > 0x7fffffffL) return csem_fail(-3, "csem_copy: %ld bytes is too large for one launch", bytes);
+  k_copy16<<<(unsigned)((n16 + 255) / 256), 256, 0, S(stream)>>>(n16, (const float4 *)src, (float4 *)dst);
   HIP_OK(hipGetLastError());
   return 0;
 }
