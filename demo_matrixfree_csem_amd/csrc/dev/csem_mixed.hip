@@ -477,9 +477,11 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
   const SkewGeo &g = m->g;
   const MxPad &pk = a.pk;
   // tile grids of the pre-passes (MxPreTile, XCD-ordered) and the copies (MxCopyTile, 3-D)
-  const dim3 gq[3] = {mx_tile_grid<MxPreTile<0>>(g, 0), mx_tile_grid<MxPreTile<1>>(g, 1), mx_tile_grid<MxPreTile<2>>(g, 2)};
-  const dim3 gu[3] = {mx_tile_grid3<MxCopyTile<0>>(g, 0), mx_tile_grid3<MxCopyTile<1>>(g, 1),
-                      mx_tile_grid3<MxCopyTile<2>>(g, 2)};
+  const dim3 gq[3] = {mx_tile_grid<MxPreTile<0, false>>(g, 0), mx_tile_grid<MxPreTile<1, false>>(g, 1),
+                      mx_tile_grid<MxPreTile<2, false>>(g, 2)};
+  const dim3 gqx = mx_tile_grid<MxPreTile<0, true>>(g, 0);  // the forward Ex tile
+  const dim3 gu[3] = {mx_tile_grid3<MxCopyTile<0, R>>(g, 0), mx_tile_grid3<MxCopyTile<1, R>>(g, 1),
+                      mx_tile_grid3<MxCopyTile<2, R>>(g, 2)};
   int tb[3];
   size_t lds[3];
   dim3 gs[3];
@@ -494,7 +496,7 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
   // pre-pass reads b and the other components' W values it differences
   // and writes T; the sweep reads T and h and writes the value to V; the
   // copy reads V and writes W (not after the last sweep) and y (backward)
-  KT(KT_M_PRE, s, 16.0 * R * nc[0], (k_mx_pre<0, true, R><<<gq[0], 256, 0, s>>>(a)));
+  KT(KT_M_PRE, s, 16.0 * R * nc[0], (k_mx_pre<0, true, R><<<gqx, 256, 0, s>>>(a)));
   KT(KT_M_SGS, s, 24.0 * R * nc[0], (mx_sgs_launch<0, true>(gs[0], tb[0], lds[0], s, a)));
   KT(KT_M_UNSKEW, s, 16.0 * R * nc[0], (k_mx_unskew<0, true, R><<<gu[0], 256, 0, s>>>(a)));
   KT(KT_M_PRE, s, 24.0 * R * nc[1], (k_mx_pre<1, true, R><<<gq[1], 256, 0, s>>>(a)));
