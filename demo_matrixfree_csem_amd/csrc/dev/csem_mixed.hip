@@ -412,7 +412,7 @@ static int mx_apply(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
     KT(KT_M_AIR_ROWS, s, 8.0 * N * N * R,
        (k_mx_air_rows<<<dim3(N, R), 256, 3 * (size_t)N * sizeof(c2), s>>>(a)));
     KT(KT_M_AIR_COLS, s, 40.0 * N * N * R,
-       (k_mx_air_cols<<<dim3(8 * ((N / x->cw + 7) / 8), R, 2), N * x->cw / 16, (2 * (size_t)N * x->cw + N) * sizeof(c2), s>>>(a)));
+       (k_mx_air_cols<<<dim3(8 * ((N / x->cw + 7) / 8), R), N * x->cw / 16, (2 * (size_t)N * x->cw + N) * sizeof(c2), s>>>(a)));
     KT(KT_M_AIR_RINV, s, 16.0 * N * (m->nrowx + m->nrowy) * R,
        (k_mx_air_rows_inv<<<dim3(m->nrowx + m->nrowy, R), 256, 3 * (size_t)N * sizeof(c2), s>>>(a)));
     const int ng = (m->n1 + 1) * m->n2 + m->n1 * (m->n2 + 1);
