@@ -1041,14 +1041,31 @@ __device__ __forceinline__ void mx_acc(double &re, double &im, c2 a, c2 b)
   re += (double)a.x * b.x + (double)a.y * b.y;
   im += (double)a.y * b.x - (double)a.x * b.y;
 }
-// fold 4 doubles per thread over the block into part[r][chunk][0..3]
-__device__ __forceinline__ void mx_fold4(double *acc, double *out, double *sh)
+// Fold 4 doubles per thread over the block (blockDim == RED_T) into
+// out[0..3], thread 0 writing: each value by block_fold's halving tree
+// (v[t] += v[t+128], v[t] += v[t+64], then v[t] += v[t+h] in wave 0 for
+// h = 32..1), the four trees sharing their barriers.
+__device__ __forceinline__ void mx_fold4(double *acc, double *out, double *)
 {
+  __shared__ double f4[4][RED_T];
+  const int t = threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const double v = block_fold(acc[q], sh);
-    if (threadIdx.x == 0) out[q] = v;
+  for (int q = 0; q < 4; q++) f4[q][t] = acc[q];
+  __syncthreads();
+  if (t < 128) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) f4[q][t] = f4[q][t] + f4[q][t + 128];
   }
+  __syncthreads();
+  if (t < 64) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      double v = f4[q][t] + f4[q][t + 64];
+      for (int h = 32; h >= 1; h >>= 1) v = v + __shfl_down(v, h, 64);
+      if (t == 0) out[q] = v;
+    }
+  }
+  __syncthreads();
 }
 
 // init: r0 = p = r (r holds the inner right-hand side), x = 0;
