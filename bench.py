@@ -297,6 +297,7 @@ def main():
         L.csem_sync(None)
         errs = []
         for r in range(R):
+            ses.set_freq(freqs[r])  # the H channels scale by 1 / (i w mu0)
             a = ses.extract(xm[r * n:(r + 1) * n]).astype(np.complex128)
             b = ses.extract(xp[r * n:(r + 1) * n]).astype(np.complex128)
             # per channel (Ex, Ey, Ez, Hx, Hy, Hz), relative to that channel's

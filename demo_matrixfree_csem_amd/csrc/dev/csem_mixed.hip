@@ -51,6 +51,7 @@ struct MixSlot {
   double *part, *sc;
   int nch;
   z2 *res, *tmp64;                          // refinement
+  z2 *xprev;                                // the solutions before the last correction
   double *npart, *norm;                     // FP64 residual norms
   float *scale;
   // captured inner iteration (a graph cannot be captured from the legacy
@@ -251,6 +252,7 @@ static int slot_new(csem_model *m, int R, MixSlot **out)
     q->nch = (int)((m->lE + RED_CHUNK - 1) / RED_CHUNK);
     MA(q, kv, 9L * R * m->lE); MA(q, part, (long)R * q->nch * 4); MA(q, sc, (long)R * MS_N);
     MA(q, res, R * m->lE); MA(q, tmp64, m->lE); MA(q, npart, (long)R * q->nch); MA(q, norm, R);
+    MA(q, xprev, R * m->lE);
     MA(q, scale, R);
   }
   // records of slots off the slices stay zero (the pre-passes write the cells' records)
@@ -694,6 +696,7 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
   const int nch = q->nch;
   const unsigned gn = (unsigned)((n + 255) / 256);
   std::vector<double> bn(R), rn(R), hsc((size_t)R * MS_N), nb(R);
+  std::vector<char> stalled(R, 0);  // a correction that did not lower the true residual was rejected
   std::vector<float> scl(R);
   int rc;
   if (o.precond != CSEM_PRECOND_LUSGS && o.precond != CSEM_PRECOND_RBSGS)
@@ -714,7 +717,7 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
     for (int r = 0; r < R; r++) {
       double *sc = &hsc[(size_t)r * MS_N];
       memset(sc, 0, MS_N * sizeof(double));
-      const bool act = bn[r] > 0 && relres[r] >= tol[r] && iters[r] < o.max_inner && outer < o.max_outer;
+      const bool act = bn[r] > 0 && relres[r] >= tol[r] && iters[r] < o.max_inner && outer < o.max_outer && !stalled[r];
       sc[MS_DONE] = act ? 0.0 : 1.0;
       sc[MS_USED] = act ? 1.0 : 0.0;
       // reduce this residual by what is still needed (with a margin), at most to inner_tol
@@ -745,6 +748,7 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
     for (int r = 0; r < R; r++)
       if (hsc[(size_t)r * MS_N + MS_USED] != 0.0) iters[r] += (int)hsc[(size_t)r * MS_N + MS_IT];
     // x += |res| * correction (FP64), then the true residual in FP64
+    HIP_OK(hipMemcpyAsync(q->xprev, X, (size_t)R * n * sizeof(z2), hipMemcpyDeviceToDevice, s));
     KT(KT_M_REFINE, s, 32.0 * R * n, (k_mx_acc64<<<dim3(gn, R), 256, 0, s>>>(n, X, k.x, q->sc, q->scale)));
     for (int r = 0; r < R; r++) {
       if (hsc[(size_t)r * MS_N + MS_USED] == 0.0) continue;
@@ -757,7 +761,19 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
     HIP_OK(hipStreamSynchronize(s));
     for (int r = 0; r < R; r++) {
       if (hsc[(size_t)r * MS_N + MS_USED] == 0.0) continue;
-      rn[r] = sqrt(nb[r]);
+      const double rnew = sqrt(nb[r]);
+      if (!(rnew < rn[r])) {
+        // the FP32 inner solve did not help (stagnation or divergence on an
+        // ill-conditioned system): keep the previous solution and residual
+        // and stop refining this right-hand side; it is reported as not
+        // converged
+        stalled[r] = 1;
+        HIP_OK(hipMemcpyAsync(X + r * n, q->xprev + r * n, n * sizeof(z2), hipMemcpyDeviceToDevice, s));
+        if ((rc = run_apply_bufs(m, &f[r], X + r * n, q->tmp64, s, q->a64))) return rc;
+        k_mx_res64<<<nch, RED_T, 0, s>>>(n, B + r * n, q->tmp64, q->res + r * n, q->npart + (long)r * nch, nch);
+        continue;
+      }
+      rn[r] = rnew;
       relres[r] = bn[r] > 0 ? rn[r] / bn[r] : 0.0;
     }
   }
@@ -865,7 +881,7 @@ extern "C" long csem_mixed_slot_bytes(const csem_model *m, int R)
   const long n = m->lE, c2b = sizeof(c2), zb = sizeof(z2);
   const long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
   long b = (m->fE * R + 3L * mx_pad(m).n * R + 9L * R * n) * c2b + (long)R * nch * 4 * 8 +
-           (long)R * MS_N * 8 + (R * n + n) * zb + (long)R * nch * 8 + R * 8 + R * 4;
+           (long)R * MS_N * 8 + (2 * R * n + n) * zb + (long)R * nch * 8 + R * 8 + R * 4;
   if (m->airbc) {
     const long NN = (long)m->nfft * m->nfft, p1 = (long)m->n1 * m->n2, p2 = (long)(m->n1 + 1) * m->n2,
                p3 = (long)m->n1 * (m->n2 + 1);

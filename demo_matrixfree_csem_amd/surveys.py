@@ -11,6 +11,7 @@ Generators cover the measured workloads:
   layered_64      64x64x64 uniform 100 m cells, water + 1 ohm-m + 100 ohm-m slab
   layered_100     100^3 uniform 100 m cells, 8 seeded log-uniform layers
   marine          any grid, layered or Gaussian-random-field resistivity under water
+  vti_marine      anisotropic random-field model with resistive ellipsoids, six channels
   serve_request   a seeded serving request: model, frequencies, sources, receivers
   tiny            small seeded random models for parity tests
 """
@@ -167,7 +168,56 @@ def tiny(seed=0, nx=10, ny=9, nz=7, airbc=1, aniso=True, dz=100.0):
                        chsrc="Ex", chrec="Ex,Ey,Ez"))
 
 
-GENERATORS = {"layered_64": layered_64, "layered_100": layered_100, "tiny": tiny}
+def vti_marine(seed=2, nx=180, ny=120, nz=120, nrec=121):
+    """Anisotropic (VTI) marine model: per-axis cell sizes drawn uniformly
+    in [50, 250] m (ascending nodes; x and y centred on 0, z from 0), sea
+    water (0.3125 ohm-m) above 1000 m, below it rho11 = rho22 a log-normal
+    Gaussian random field with median 2 ohm-m (FFT-filtered white noise,
+    correlation length ~1 km, log-standard deviation 0.5) carrying three
+    seeded resistive ellipsoids of 50-300 ohm-m, and rho33 = 2 rho11.
+    One Ex source 50 m above the seafloor at the centre, `nrec` receivers
+    on the seafloor along y = 0; nb=10, lextend=120 km; six frequencies
+    0.125-1.25 Hz; all six channels (the H channels are beyond the
+    reference).  The default size is 180 x 120 x 120 cells."""
+    rng = np.random.default_rng(seed)
+
+    def axis(n, centred):
+        d = rng.uniform(50.0, 250.0, n)
+        x = np.concatenate([[0.0], np.cumsum(d)])
+        return (x - x[-1] / 2 if centred else x).astype(np.float32)
+
+    x, y, z = axis(nx, True), axis(ny, True), axis(nz, False)
+    xc, yc, zc = _centers(x), _centers(y), _centers(z)
+    h = float(np.mean(np.diff(z)))
+    w = rng.standard_normal((nz, ny, nx))
+    k = [np.fft.fftfreq(nn, d=h) for nn in (nz, ny, nx)]
+    kk = np.sqrt(k[0][:, None, None] ** 2 + k[1][None, :, None] ** 2 + k[2][None, None, :] ** 2)
+    g = np.real(np.fft.ifftn(np.fft.fftn(w) * np.exp(-(kk * 1000.0) ** 2)))
+    g = (g - g.mean()) / (g.std() + 1e-30)
+    rho = 2.0 * np.exp(0.5 * g)
+    below = zc > 1000.0
+    zb = float(zc[below].min()), float(zc[-1])
+    for _ in range(3):
+        c = (rng.uniform(0.6 * x[0], 0.6 * x[-1]), rng.uniform(0.6 * y[0], 0.6 * y[-1]),
+             rng.uniform(zb[0] + 0.2 * (zb[1] - zb[0]), zb[0] + 0.7 * (zb[1] - zb[0])))
+        ax = (rng.uniform(0.1, 0.25) * (x[-1] - x[0]), rng.uniform(0.1, 0.25) * (y[-1] - y[0]),
+              rng.uniform(0.03, 0.08) * (zb[1] - zb[0]))
+        inside = (((zc[:, None, None] - c[2]) / ax[2]) ** 2 + ((yc[None, :, None] - c[1]) / ax[1]) ** 2 +
+                  ((xc[None, None, :] - c[0]) / ax[0]) ** 2) <= 1.0
+        rho = np.where(inside, rng.uniform(50.0, 300.0), rho)
+    water = (~below)[:, None, None]
+    r11 = np.where(water, RHO_WATER, rho).astype(np.float32)
+    r33 = np.where(water, RHO_WATER, 2.0 * rho).astype(np.float32)
+    half = 0.4 * float(x[-1] - x[0])
+    rec = [(float(v), 0.0, 1000.0) for v in np.linspace(-half, half, nrec)]
+    return Survey("vti_marine_%dx%dx%d" % (nx, ny, nz), Model(x, y, z, r11, r11.copy(), r33),
+                  Acquisition(_acq_table([(0.0, 0.0, 950.0)]), _acq_table(rec)),
+                  [0.125, 0.25, 0.5, 0.75, 1.0, 1.25],
+                  dict(fvm=0, airbc=1, preco=1, algopt=1, nrestart=10, tol=1e-8, niter=2000, nb=10,
+                       lextend=120e3, chsrc="Ex", chrec="Ex,Ey,Ez,Hx,Hy,Hz"))
+
+
+GENERATORS = {"layered_64": layered_64, "layered_100": layered_100, "tiny": tiny, "vti_marine": vti_marine}
 
 
 def marine(seed, nx, ny, nz, kind="layered", median=2.0, dx=100.0, water=1000.0, nlayers=8):

@@ -122,7 +122,14 @@ int csem_vec_addscaled(long n, void *y, double re, double im, const void *x, voi
  * FP32 LU-SGS + F-BiCGStab inner solve with FP64 dot products.  Returns
  * the number of right-hand sides that reached tol, or < 0 on error;
  * iters[r] receives the inner iterations spent, relres[r] the final true
- * relative residual (host arrays).  opt may be NULL for the defaults. */
+ * relative residual (host arrays).  opt may be NULL for the defaults.
+ * A refinement step that does not lower the true residual (the FP32 inner
+ * solve stagnating or diverging on an ill-conditioned system) is undone
+ * and refinement of that right-hand side stops: x keeps the best solution
+ * found and relres[r] its residual, never above 1.  A caller that needs
+ * tol then continues with the FP64 solver (csem_bicgstab) from that x,
+ * asking it for tol[r] / relres[r]: its tolerance is relative to the
+ * residual it starts from, as the reference's (solver.c:115, 158-159). */
 typedef struct {
   int max_inner;     /* inner iteration budget per right-hand side (default 2000) */
   double inner_tol;  /* reduction asked of one inner solve (1e-2: FP32
