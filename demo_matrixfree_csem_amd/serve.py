@@ -53,11 +53,30 @@ from .session import Session, freq_consts
 # The reference's host entry points (initargs, emf_init, inject_source,
 # extract_emf ...) and the airwave tables share process-global state.
 _HOST = threading.Lock()
+FP64_NITER = 4000  # iteration budget of an FP64 continuation
 
 
 def solve_tol(freq):
     """tol/f in single precision, as main.c:104-118 forms it (tol = 1e-8)."""
     return float(np.float32(np.float32(1e-8) / np.float32(freq)))
+
+
+def fp64_continue(model, freq, dx, db, tol, rel, niter, lock):
+    """Finish a right-hand side that the mixed solve stopped short of tol
+    on (its FP32 inner iterations stagnate on an ill-conditioned model):
+    the FP64 device F-BiCGStab, the reference's solver, from the mixed
+    solution in dx (device), asked for the remaining reduction tol / rel
+    (its tolerance is relative to the residual it starts from,
+    solver.c:115, 158-159).  The FP64 solver works in the model's own
+    buffers, so calls on one model hold `lock`.  Returns (iterations, the
+    solver's estimate of the final relative residual)."""
+    L = _lib.lib()
+    hist = np.zeros(niter + 2)
+    with lock:
+        it = L.csem_bicgstab(model, C.byref(freq_consts(freq)), dx, db, niter, min(0.5, tol / rel), 1,
+                             hist.ctypes.data, None)
+    _lib.check(it, "csem_bicgstab")
+    return it, rel * float(hist[it + 1] if it < niter else hist[niter - 1])
 
 
 class ResidentModel:
@@ -74,6 +93,7 @@ class ResidentModel:
         self.n = self.ses.n
         self.busy = 0          # requests in flight on this model
         self.last_used = time.time()
+        self.fp64_lock = threading.Lock()  # the FP64 solver's buffers are the model's
 
     def bytes(self):
         L = _lib.lib()
@@ -139,6 +159,8 @@ class Engine:
         self.reserved = 0          # estimated bytes of models being uploaded
         self.est_scale = 1.0       # largest ratio of a model's bytes to its estimate
         self.evictions = 0
+        self.fp64_continued = 0    # right-hand sides the FP64 solver finished
+        self._count_mu = threading.Lock()
 
     # ------------------------------------------------------ accounting
     def _used(self, running):
@@ -182,6 +204,14 @@ class Engine:
         nc = L.csem_mixed_solve(rm.model, R, fr, io.dx, io.db, t.ctypes.data, C.byref(self.opts),
                                 its.ctypes.data, rel.ctypes.data, None)
         _lib.check(nc, "csem_mixed_solve")
+        for r in range(R):
+            if not rel[r] < tols[r]:
+                off = r * n * 16
+                it, rel[r] = fp64_continue(rm.model, freqs[r], C.c_void_p(io.dx.value + off),
+                                           C.c_void_p(io.db.value + off), tols[r], rel[r], FP64_NITER, rm.fp64_lock)
+                its[r] += it
+                with self._count_mu:
+                    self.fp64_continued += 1
         hx = np.zeros(R * n, np.complex128)
         _lib.check(L.csem_memcpy_d2h(hx.ctypes.data, io.dx, R * n * 16, None), "download")
         return [hx[r * n:(r + 1) * n] for r in range(R)], its, rel
@@ -346,7 +376,8 @@ class Engine:
                  "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
                  "budget_bytes": self.budget, "peak_admitted_bytes": int(peak_acc),
                  "peak_device_bytes": int(peak_dev), "resident_models": len(self.models),
-                 "models_built": self.nbuilt, "evictions": self.evictions, "workers": self.workers, "rhs_per_solve": self.R}
+                 "models_built": self.nbuilt, "evictions": self.evictions, "workers": self.workers, "rhs_per_solve": self.R,
+                 "fp64_continued": self.fp64_continued}
         return results, stats
 
     def close(self):

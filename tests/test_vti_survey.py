@@ -89,10 +89,26 @@ def test_vti_mixed_matches_fp64_bicgstab_and_gmres(workdir):
                 assert max(e_b, e_g, e_bg) < 1e-6, (f, e_b, e_g, e_bg)
             else:
                 assert rel[r] <= 1.0 and np.all(np.isfinite(xs[r])), (f, rel[r])
-                # the FP64 solver's tolerance is relative to its initial residual
-                # (solver.c:115, 158-159): from x, ask for the rest of the way
-                xc, itc = ses.solve(bs[r], tols[r] / rel[r], niter=6000, algopt=1, x0=xs[r])
-                assert 0 <= itc < 6000, (f, itc)
+                # the serving engine's FP64 continuation from the mixed solution
+                import ctypes as C
+                import threading
+                from demo_matrixfree_csem_amd import serve
+                L = _lib.lib()
+                dx, db = C.c_void_p(), C.c_void_p()
+                for d, a in ((dx, xs[r]), (db, bs[r])):
+                    a = np.ascontiguousarray(a, np.complex128)
+                    _lib.check(L.csem_malloc(C.byref(d), a.nbytes), "malloc")
+                    _lib.check(L.csem_memcpy_h2d(d, a.ctypes.data, a.nbytes, None), "h2d")
+                try:
+                    itc, relc = serve.fp64_continue(ses.model, f, dx, db, tols[r], rel[r], 6000, threading.Lock())
+                    xc = np.zeros(ses.n, np.complex128)
+                    _lib.check(L.csem_memcpy_d2h(xc.ctypes.data, dx, xc.nbytes, None), "d2h")
+                finally:
+                    L.csem_free(dx)
+                    L.csem_free(db)
+                assert 0 <= itc < 6000 and relc < tols[r], (f, itc, relc)
+                res = np.linalg.norm(bs[r] - ses.fdfd_apply(xc)) / np.linalg.norm(bs[r])
+                assert res < 2 * tols[r], (f, res)
                 e = _rec_err(ses.extract(xc).astype(np.complex128), rb)
                 print("f=%g mixed stopped at relres %.2e after %d iterations; FP64 continued %d iterations "
                       "(from zero %d), receiver error %.2e" % (f, rel[r], its[r], itc, itb, e), flush=True)
