@@ -12,6 +12,7 @@ Generators cover the measured workloads:
   layered_100     100^3 uniform 100 m cells, 8 seeded log-uniform layers
   marine          any grid, layered or Gaussian-random-field resistivity under water
   vti_marine      anisotropic random-field model with resistive ellipsoids, six channels
+  topo_marine     seafloor topography over layering, eight sources along a line
   serve_request   a seeded serving request: model, frequencies, sources, receivers
   tiny            small seeded random models for parity tests
 """
@@ -217,7 +218,47 @@ def vti_marine(seed=2, nx=180, ny=120, nz=120, nrec=121):
                        lextend=120e3, chsrc="Ex", chrec="Ex,Ey,Ez,Hx,Hy,Hz"))
 
 
-GENERATORS = {"layered_64": layered_64, "layered_100": layered_100, "tiny": tiny, "vti_marine": vti_marine}
+def topo_marine(seed=3, nx=125, ny=125, nz=100, dx=100.0, nsrc=8, nrec=61):
+    """Marine model with seafloor topography: uniform `dx` cells, x and y
+    centred on 0, z from 0; the seafloor at 1000 m plus four seeded
+    Gaussian bumps of up to +-300 m; sea water (0.3125 ohm-m) above it,
+    below it seeded layers of 1-20 ohm-m by depth under the seafloor.
+    `nsrc` Ex sources from x = -3500 to 3500 m on y = 0, 50 m above the
+    local seafloor; `nrec` receivers on the seafloor along y = 0; six
+    frequencies 0.125-1.25 Hz.  The default size is 125 x 125 x 100 cells."""
+    rng = np.random.default_rng(seed)
+    x = (dx * (np.arange(nx + 1) - nx / 2)).astype(np.float32)
+    y = (dx * (np.arange(ny + 1) - ny / 2)).astype(np.float32)
+    z = (dx * np.arange(nz + 1)).astype(np.float32)
+    xc, yc, zc = _centers(x), _centers(y), _centers(z)
+    bumps = [(rng.uniform(-0.35, 0.35) * (x[-1] - x[0]), rng.uniform(-0.35, 0.35) * (y[-1] - y[0]),
+              rng.uniform(800.0, 2000.0), rng.choice([-1.0, 1.0]) * rng.uniform(100.0, 300.0)) for _ in range(4)]
+
+    def seafloor(px, py):
+        d = 1000.0 * np.ones(np.broadcast(px, py).shape)
+        for bx, by, w, a in bumps:
+            d = d + a * np.exp(-((px - bx) ** 2 + (py - by) ** 2) / (2 * w * w))
+        return np.clip(d, 700.0, 1300.0)
+
+    sf = seafloor(xc[None, :], yc[:, None])  # (ny, nx)
+    nl = 6
+    thick = rng.uniform(300.0, 1200.0, nl)
+    rl = 10 ** rng.uniform(0.0, np.log10(20.0), nl)
+    below = zc[:, None, None] - sf[None, :, :]  # depth under the seafloor
+    layer = np.searchsorted(np.cumsum(thick), np.maximum(below, 0.0), side="right").clip(0, nl - 1)
+    rho = np.where(below < 0, RHO_WATER, rl[layer]).astype(np.float32)
+    sx = np.linspace(-3500.0, 3500.0, nsrc)
+    src = [(float(v), 0.0, float(seafloor(v, 0.0)) - 50.0) for v in sx]
+    half = 0.4 * float(x[-1] - x[0])
+    rec = [(float(v), 0.0, float(seafloor(v, 0.0))) for v in np.linspace(-half, half, nrec)]
+    return Survey("topo_marine_%dx%dx%d" % (nx, ny, nz), Model(x, y, z, rho, rho.copy(), rho.copy()),
+                  Acquisition(_acq_table(src), _acq_table(rec)), [0.125, 0.25, 0.5, 0.75, 1.0, 1.25],
+                  dict(fvm=0, airbc=1, preco=1, algopt=1, tol=1e-8, niter=2000, nb=10, chsrc="Ex",
+                       chrec="Ex,Ey,Ez"))
+
+
+GENERATORS = {"layered_64": layered_64, "layered_100": layered_100, "tiny": tiny, "vti_marine": vti_marine,
+              "topo_marine": topo_marine}
 
 
 def marine(seed, nx, ny, nz, kind="layered", median=2.0, dx=100.0, water=1000.0, nlayers=8):
