@@ -161,3 +161,23 @@ def test_mirror_symmetry_of_a_solved_field(workdir):
     assert hy > 0
     # Hx and Hz are odd in y, Hy is even
     assert np.abs(h[3]).max() < 1e-9 * hy and np.abs(h[5]).max() < 1e-9 * hy
+
+
+def test_h_outside_the_grid_is_nan(workdir, capfd):
+    """A receiver whose eight H faces are not all inside the operator grid
+    gets NaN for that channel and a message.  Here it is 10 m below the sea
+    surface, where the operator grid starts under the airwave boundary: Hx
+    and Hy (at cell centres in z) would need a plane above the grid, Hz (on
+    the z nodes, the surface plane included) does not; the E channels are
+    unaffected."""
+    s, ses, orc = _pair(workdir, airbc=1)
+    ses.set_freq(0.5)
+    orc.set_freq(0.5)
+    x = np.random.default_rng(2).standard_normal(ses.n) + 0j
+    r0 = s.acq.receivers[0]
+    top = (float(r0[0]), float(r0[1]), 10.0)  # 10 m below the surface
+    got = ses.extract_at(x, [top, tuple(float(v) for v in r0[:3])])
+    assert np.all(np.isnan(got[3:5, 0])) and np.isfinite(got[5, 0]) and np.all(np.isfinite(got[3:, 1]))
+    assert np.all(np.isfinite(got[:3, 0]))
+    assert np.isnan(orc.extract(4, *top, x))
+    assert "too close to the grid's top or bottom" in capfd.readouterr().err
