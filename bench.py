@@ -32,6 +32,7 @@ import json
 import os
 import sys
 import tempfile
+import threading
 import time
 
 import numpy as np
@@ -225,16 +226,31 @@ def main():
             out.append(it)
         return out
 
+    # a right-hand side the mixed solve stops short of tol on is finished by
+    # the FP64 solver from the mixed solution, inside the timed step, as the
+    # serving engine does: every solve counted in the rate reaches tol
+    from demo_matrixfree_csem_amd import serve
+    fp64_lock = threading.Lock()
+
+    def finish(r, dx, relr):
+        if relr < tols[r]:
+            return 0, relr
+        it, relr = serve.fp64_continue(model, freqs[r], dx, C.c_void_p(db.value + r * n * 16), tols[r], relr,
+                                       args.niter, fp64_lock)
+        return 1, relr
+
     def mixed_step():
         nc = L.csem_mixed_solve(model, R, fr, dxm, db, tol_arr.ctypes.data, C.byref(opts), its.ctypes.data,
                                 rel.ctypes.data, None)
         _lib.check(nc, "csem_mixed_solve")
-        return nc
+        cont = 0
+        for r in range(R):
+            c, rel[r] = finish(r, C.c_void_p(dxm.value + r * n * 16), rel[r])
+            cont += c
+        return nc, cont
 
     # the same three solves as independent single right-hand-side solves on
     # three host threads, each on its own stream and solve slot
-    import threading
-
     def mixed_streams_step():
         out = [None] * R
 
@@ -243,7 +259,8 @@ def main():
             nc = L.csem_mixed_solve(model, 1, C.byref(fr[r]), C.c_void_p(dxs.value + r * n * 16),
                                     C.c_void_p(db.value + r * n * 16), tol_arr[r:r + 1].ctypes.data, C.byref(opts),
                                     itr.ctypes.data, rr.ctypes.data, None)
-            out[r] = (nc, int(itr[0]), float(rr[0]))
+            c, relr = finish(r, C.c_void_p(dxs.value + r * n * 16), float(rr[0])) if nc >= 0 else (0, float(rr[0]))
+            out[r] = (nc, int(itr[0]), relr, c)
 
         th = [threading.Thread(target=run, args=(r,)) for r in range(R)]
         for t in th:
@@ -269,14 +286,17 @@ def main():
         L.csem_kernel_timing(0)
         return dt, res, stats
 
-    dtm, nconv, stm = timed(mixed_step)
-    mixed = {"batch": {"solves_per_s": round(R * args.steps / dtm, 4), "ms_per_step": round(1e3 * dtm / args.steps, 3),
-                       "converged": int(nconv), "inner_iterations": [int(i) for i in its],
+    # solves/s counts the right-hand sides that end below their tol
+    dtm, (nconv, ncont), stm = timed(mixed_step)
+    okm = int(sum(rel[r] < tols[r] for r in range(R)))
+    mixed = {"batch": {"solves_per_s": round(okm * args.steps / dtm, 4), "ms_per_step": round(1e3 * dtm / args.steps, 3),
+                       "converged": int(nconv), "fp64_continued": ncont, "inner_iterations": [int(i) for i in its],
                        "true_relres": [float("%.3e" % r) for r in rel]}, "tol": tols}
     dts, sres, _ = timed(mixed_streams_step)
-    mixed["streams"] = {"solves_per_s": round(R * args.steps / dts, 4), "ms_per_step": round(1e3 * dts / args.steps, 3),
-                        "converged": sum(o[0] for o in sres), "inner_iterations": [o[1] for o in sres],
-                        "true_relres": [float("%.3e" % o[2]) for o in sres]}
+    oks = int(sum(o[2] < tols[r] for r, o in enumerate(sres)))
+    mixed["streams"] = {"solves_per_s": round(oks * args.steps / dts, 4), "ms_per_step": round(1e3 * dts / args.steps, 3),
+                        "converged": sum(o[0] for o in sres), "fp64_continued": sum(o[3] for o in sres),
+                        "inner_iterations": [o[1] for o in sres], "true_relres": [float("%.3e" % o[2]) for o in sres]}
     best = "batch" if mixed["batch"]["solves_per_s"] >= mixed["streams"]["solves_per_s"] else "streams"
     mixed["value_from"] = best
     mixed["solves_per_s"] = mixed[best]["solves_per_s"]
