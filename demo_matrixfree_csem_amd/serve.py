@@ -251,7 +251,7 @@ class Engine:
     def run(self, requests, arrivals):
         """Serve `requests` (serve_request tuples) arriving `arrivals`
         seconds after the start.  Returns (results, stats); results[i] maps
-        (source index, frequency) -> (receivers complex64 (3, nrec),
+        (source index, frequency) -> (receivers complex64 (channels, nrec),
         inner iterations, true relative residual) and holds the latency.
 
         A request's model is made resident first (its survey built and its

@@ -84,3 +84,37 @@ def test_admission_evicts_idle_models_within_budget():
             assert st["evictions"] >= 1
     finally:
         eng.close()
+
+
+def test_engine_finishes_a_stalled_solve_in_fp64(workdir):
+    """A request the FP32 inner solve cannot finish (the anisotropic model
+    at 0.25 Hz, strongly stretched padding): the engine continues it with
+    the FP64 solver and serves receivers equal to a plain FP64 solve."""
+    from demo_matrixfree_csem_amd.session import Session
+    s = surveys.vti_marine(2, 36, 24, 24, 33)
+    src = [tuple(float(v) for v in s.acq.sources[0][:3])]
+    recs = [tuple(float(v) for v in r[:3]) for r in s.acq.receivers]
+    req = (("vti_marine", 36, 24, 24), lambda: s, [0.25], src, recs)
+    print("case serve: stalled mixed solve finished in FP64", flush=True)
+    eng = Engine(workers=1, rhs_per_solve=1)
+    try:
+        res, st = eng.run([req], [0.0])
+    finally:
+        eng.close()
+    print(st, flush=True)
+    assert st["fp64_continued"] == 1
+    _check_converged(res[0])
+    rec, it, rr = res[0]["receivers"][(0, 0.25)]
+    surveys.write_files(workdir, s)
+    ses = Session(s.argv(workdir))
+    try:
+        ses.set_freq(0.25)
+        b = ses.inject_at(*src[0])
+        x, itp = ses.solve(b, solve_tol(0.25), niter=4000, algopt=1)
+        assert 0 < itp < 4000
+        want = ses.extract_at(x, recs)
+    finally:
+        ses.close()
+    e = _rec_err(rec, want)
+    print("served after %d iterations (relres %.2e); FP64 alone %d; receiver error %.2e" % (it, rr, itp, e), flush=True)
+    assert e < 1e-6, e
