@@ -329,22 +329,22 @@ static MxAir mx_air(const csem_model *m, const MixState *x, const MixSlot *q)
 // 4 * NW * R * 64 complex floats: NW = 8 for R <= 3, at most 6 for R = 4.
 template <int R, int NW, int KC>
 static void mx_apply_launch(const csem_model *m, const MixState *x, const MixSlot *q, const MxFreq &fr, const c2 *in,
-                            c2 *out, hipStream_t s)
+                            c2 *out, hipStream_t s, const double *sc)
 {
   const int nti = (m->n1 + 1 + 61) / 62, ntj = (m->n2 + 1 + NW - 3) / (NW - 2), ntk = (m->n3 + 1 + KC - 1) / KC;
   const int ntiles = nti * ntj * ntk;
   k_mx_apply<R, NW, KC><<<8 * ((ntiles + 7) / 8), dim3(64, NW), 0, s>>>(mx_geo(m, x), fr, in, out, q->hxg, q->hyg,
-                                                                       nti, ntj, ntk);
+                                                                       nti, ntj, ntk, sc);
 }
 template <int R>
 static void mx_apply_tile(const csem_model *m, const MixState *x, const MixSlot *q, const MxFreq &fr, const c2 *in,
-                          c2 *out, int rows, int kc, hipStream_t s)
+                          c2 *out, int rows, int kc, hipStream_t s, const double *sc)
 {
   constexpr int NB = R > 3 ? 6 : 8;  // the larger block
 #define MX_AP(KC)                                                   \
   do {                                                              \
-    if (rows + 2 >= NB) mx_apply_launch<R, NB, KC>(m, x, q, fr, in, out, s);     \
-    else mx_apply_launch<R, NB - 2, KC>(m, x, q, fr, in, out, s);   \
+    if (rows + 2 >= NB) mx_apply_launch<R, NB, KC>(m, x, q, fr, in, out, s, sc);     \
+    else mx_apply_launch<R, NB - 2, KC>(m, x, q, fr, in, out, s, sc);   \
   } while (0)
   switch (kc) {
     case 10: MX_AP(10); break;
@@ -355,13 +355,13 @@ static void mx_apply_tile(const csem_model *m, const MixState *x, const MixSlot 
 #undef MX_AP
 }
 static void mx_apply_kernel(const csem_model *m, const MixState *x, const MixSlot *q, int R, const MxFreq &fr,
-                            const c2 *in, c2 *out, int rows, int kc, hipStream_t s)
+                            const c2 *in, c2 *out, int rows, int kc, hipStream_t s, const double *sc = nullptr)
 {
   switch (R) {
-    case 1: mx_apply_tile<1>(m, x, q, fr, in, out, rows, kc, s); break;
-    case 2: mx_apply_tile<2>(m, x, q, fr, in, out, rows, kc, s); break;
-    case 3: mx_apply_tile<3>(m, x, q, fr, in, out, rows, kc, s); break;
-    default: mx_apply_tile<4>(m, x, q, fr, in, out, rows, kc, s); break;
+    case 1: mx_apply_tile<1>(m, x, q, fr, in, out, rows, kc, s, sc); break;
+    case 2: mx_apply_tile<2>(m, x, q, fr, in, out, rows, kc, s, sc); break;
+    case 3: mx_apply_tile<3>(m, x, q, fr, in, out, rows, kc, s, sc); break;
+    default: mx_apply_tile<4>(m, x, q, fr, in, out, rows, kc, s, sc); break;
   }
 }
 // output rows per block the operator offers at batch size R
@@ -405,7 +405,7 @@ static int mx_tune(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &
 
 // y = A x for the R right-hand sides, FP32, with the airwave boundary.
 static int mx_apply(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, const c2 *in, c2 *out,
-                    hipStream_t s)
+                    hipStream_t s, const double *sc = nullptr)
 {
   if (m->airbc) {
     const int nh = m->n1 * m->n2, N = m->nfft;
@@ -422,7 +422,7 @@ static int mx_apply(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
   }
   // algorithmic bytes: read E and write y per right-hand side, sigma once
   const int tj = x->ap_tj[R] ? x->ap_tj[R] : 6, kc = x->ap_kc[R] ? x->ap_kc[R] : 8;
-  KT(KT_M_APPLY, s, (16.0 * R + 4.0) * m->lE, mx_apply_kernel(m, x, q, R, fr, in, out, tj, kc, s));
+  KT(KT_M_APPLY, s, (16.0 * R + 4.0) * m->lE, mx_apply_kernel(m, x, q, R, fr, in, out, tj, kc, s, sc));
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -436,6 +436,7 @@ static MxLu mx_lu(const csem_model *m, const MixState *x, MixSlot *q, const MxFr
   a.th = q->th; a.V = q->V; a.pk = mx_pad(m);
   a.c1m = x->c1m; a.c1p = x->c1p; a.c2m = x->c2m; a.c2p = x->c2p; a.c3m = x->c3m; a.c3p = x->c3p;
   a.fr = fr;
+  a.sc = nullptr;
   return a;
 }
 
@@ -568,11 +569,14 @@ static void mx_lusgs_dispatch(const csem_model *m, int R, const MxLu &a, hipStre
   }
 }
 
+// sc: the inner solve's scalars (right-hand sides done are skipped), or NULL
 static int mx_lusgs(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, const c2 *b, c2 *y,
-                    hipStream_t s)
+                    hipStream_t s, const double *sc = nullptr)
 {
-  if (q->pc == CSEM_PRECOND_RBSGS) mx_rbsgs_dispatch(m, R, mx_lu(m, x, q, fr, b, y), s);
-  else mx_lusgs_dispatch(m, R, mx_lu(m, x, q, fr, b, y), s);
+  MxLu a = mx_lu(m, x, q, fr, b, y);
+  a.sc = sc;
+  if (q->pc == CSEM_PRECOND_RBSGS) mx_rbsgs_dispatch(m, R, a, s);
+  else mx_lusgs_dispatch(m, R, a, s);
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -596,13 +600,16 @@ static int mx_iteration(csem_model *m, MixState *x, MixSlot *q, int R, const MxF
   const MxKry k = mx_kry(m, q);
   const dim3 gr(q->nch, R), ge((unsigned)((k.n + 255) / 256), R);
   int rc;
-  if ((rc = mx_lusgs(m, x, q, R, fr, k.p, k.pp, s))) return rc;
-  if ((rc = mx_apply(m, x, q, R, fr, k.pp, k.v, s))) return rc;
+  // in a batch, the operator and preconditioner skip the right-hand sides
+  // whose inner solve is done (one alone has nothing to skip)
+  const double *live = R > 1 ? k.sc : nullptr;
+  if ((rc = mx_lusgs(m, x, q, R, fr, k.p, k.pp, s, live))) return rc;
+  if ((rc = mx_apply(m, x, q, R, fr, k.pp, k.v, s, live))) return rc;
   KT(KT_M_DOT, s, 16.0 * R * k.n, (k_mx_k_dot_vr0<<<gr, RED_T, 0, s>>>(k)));
   KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 1)));
   KT(KT_M_VEC, s, 24.0 * R * k.n, (k_mx_k_s<<<ge, 256, 0, s>>>(k)));
-  if ((rc = mx_lusgs(m, x, q, R, fr, k.s, k.ss, s))) return rc;
-  if ((rc = mx_apply(m, x, q, R, fr, k.ss, k.t, s))) return rc;
+  if ((rc = mx_lusgs(m, x, q, R, fr, k.s, k.ss, s, live))) return rc;
+  if ((rc = mx_apply(m, x, q, R, fr, k.ss, k.t, s, live))) return rc;
   KT(KT_M_DOT, s, 16.0 * R * k.n, (k_mx_k_dot_st<<<gr, RED_T, 0, s>>>(k)));
   KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 2)));
   KT(KT_M_VEC, s, 64.0 * R * k.n, (k_mx_k_xr<<<gr, RED_T, 0, s>>>(k)));

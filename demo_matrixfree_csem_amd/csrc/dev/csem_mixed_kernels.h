@@ -14,6 +14,15 @@
 
 #define MX_RMAX 4  // right-hand sides per batch
 
+// Per right-hand side scalar slots of the inner solve (device FP64), see
+// the Krylov section.
+enum { MS_RHO = 0, MS_ALPHA = 2, MS_OMEGA = 4, MS_BETA = 6, MS_RS0 = 8, MS_REL = 9, MS_DONE = 10, MS_TOL = 11,
+       MS_IT = 12, MS_MAXIT = 13, MS_USED = 14, MS_N = 16 };
+// A right-hand side whose inner solve is done (converged, or not taking
+// part) needs no operator or preconditioner output: sc == NULL (a single
+// application) computes every one.
+__device__ __forceinline__ bool mx_live(const double *sc, int r) { return !sc || sc[r * MS_N + MS_DONE] == 0.0; }
+
 __device__ __forceinline__ c2 cfma(c2 acc, c2 a, c2 b)  // acc + a*b
 {
   return cmk(__builtin_fmaf(a.x, b.x, __builtin_fmaf(-a.y, b.y, acc.x)),
@@ -53,7 +62,7 @@ struct MxGeo {
 // and consecutive tiles run on one XCD.
 template <int R, int NW, int KC>
 __global__ __launch_bounds__(64 * NW) void k_mx_apply(MxGeo g, MxFreq fr, const c2 *x, c2 *y, const c2 *hxg,
-                                                      const c2 *hyg, int nti, int ntj, int ntk)
+                                                      const c2 *hyg, int nti, int ntj, int ntk, const double *sc)
 {
   constexpr int OI = 62, OJ = NW - 2;
   __shared__ c2 sEx[NW][R][64], sEz[NW][R][64], sHx[NW][R][64], sHz[NW][R][64];
@@ -81,6 +90,9 @@ __global__ __launch_bounds__(64 * NW) void k_mx_apply(MxGeo g, MxFreq fr, const 
   const c2 z0 = cmk(0, 0);
   // sigma on the operator grid: the full grid offset by kofs planes
   const float *s11 = g.s11 + ko * px, *s22 = g.s22 + ko * py, *s33 = g.s33 + ko * pz;
+  bool rhs_on[R];  // right-hand sides still iterating
+#pragma unroll
+  for (int r = 0; r < R; r++) rhs_on[r] = R == 1 || mx_live(sc, r);
   struct EV { c2 x[R], y[R], z[R]; };
   struct SV { float x, y, z; };
   auto ldE = [&](int k) {
@@ -88,6 +100,7 @@ __global__ __launch_bounds__(64 * NW) void k_mx_apply(MxGeo g, MxFreq fr, const 
     EV e;
 #pragma unroll
     for (int r = 0; r < R; r++) {
+      if (!rhs_on[r]) { e.x[r] = e.y[r] = e.z[r] = z0; continue; }
       e.x[r] = x[r * n + kxy * px + ox];
       e.y[r] = x[r * n + g.lEx + kxy * py + oy];
       e.z[r] = x[r * n + g.lEx + g.lEy + kz * pz + oz];
@@ -186,6 +199,7 @@ __global__ __launch_bounds__(64 * NW) void k_mx_apply(MxGeo g, MxFreq fr, const 
       const float d3_k = sD[kk][2], d3s_k = sD[kk][3];
 #pragma unroll
       for (int r = 0; r < R; r++) {
+        if (!rhs_on[r]) continue;
         const c2 iw = fr.iwmu[r];
         c2 *yr = y + r * n;
         if (i < n1) {  // bx
@@ -506,6 +520,7 @@ struct MxLu {
   c2 *V;         // a sweep's values per record slot, R x pk.n
   const float *c1m, *c1p, *c2m, *c2p, *c3m, *c3p;  // own-axis coefficients (z on the full grid)
   MxFreq fr;
+  const double *sc;  // inner-solve scalars: right-hand sides done are skipped (NULL: none)
 };
 
 // Tile (x, y, z) of a pre-pass over component C launched as a 1-D grid of
@@ -606,6 +621,7 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
     else csg[cell] = cmk(a.c1p[i] + a.c1m[i] + a.c2p[j] + a.c2m[j], a.g.s33[id3(n1, n2, k, j, i)]);
 #pragma unroll
     for (int r = 0; r < R; r++) {
+      if (R > 1 && !mx_live(a.sc, r)) continue;
       c2 S = z0, own = z0;
       if (C == 0) {
         if (!FWD) {  // Ey, Ez backward
@@ -643,6 +659,7 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
     const c2 cs = csg[e];
 #pragma unroll
     for (int r = 0; r < R; r++) {
+      if (R > 1 && !mx_live(a.sc, r)) continue;
       const c2 iw = a.fr.iwmu[r], T = tile[r][e];
       const c2 aii = cmk(cs.x * iw.x - cs.y, cs.x * iw.y);
       const float den = 1.f / (aii.x * aii.x + aii.y * aii.y);
@@ -691,7 +708,8 @@ __global__ __launch_bounds__(256) void k_mx_unskew(MxLu a)
     const int p = pidx[e];
     if (p < 0) continue;
 #pragma unroll
-    for (int r = 0; r < R; r++) tile[r][e] = a.V[r * a.pk.n + p];
+    for (int r = 0; r < R; r++)
+      if (R == 1 || mx_live(a.sc, r)) tile[r][e] = a.V[r * a.pk.n + p];
   }
   __syncthreads();
 #pragma unroll
@@ -709,6 +727,7 @@ __global__ __launch_bounds__(256) void k_mx_unskew(MxLu a)
                                                                    : a.g.lEx + a.g.lEy + id3(n1, n2, kop, j, i);
 #pragma unroll
     for (int r = 0; r < R; r++) {
+      if (R > 1 && !mx_live(a.sc, r)) continue;
       const c2 v = tile[r][cell[it]];
       if (FWD || C != 0) a.W[r * a.fE + wo] = v;
       if (!FWD && kop >= 0) a.y[r * a.g.n + xo] = v;
@@ -900,7 +919,7 @@ __global__ __launch_bounds__(64 * NW) void k_mx_sgs(MxLu a)
   const int per = (nslice + 7) / 8;
   const int q = (blockIdx.x % 8) * per + blockIdx.x / 8;  // consecutive slices on one XCD
   const int r = blockIdx.y;
-  if (q >= nslice) return;
+  if (q >= nslice || !mx_live(a.sc, r)) return;
   const int n1 = a.g.n1, n2 = a.g.n2;
   constexpr int nt = 64 * NW, AP = nt, lagmax = (NW - 1) * MXS_L;
   constexpr int P = MXS_P;
@@ -1022,8 +1041,7 @@ swept:
 // can be replayed from a graph without the host.  Per-RHS scalar slots:
 // MS_USED marks the right-hand sides that take part in the current inner
 // solve (their correction is added to the FP64 solution).
-enum { MS_RHO = 0, MS_ALPHA = 2, MS_OMEGA = 4, MS_BETA = 6, MS_RS0 = 8, MS_REL = 9, MS_DONE = 10, MS_TOL = 11,
-       MS_IT = 12, MS_MAXIT = 13, MS_USED = 14, MS_N = 16 };
+
 
 struct MxKry {
   long n;
