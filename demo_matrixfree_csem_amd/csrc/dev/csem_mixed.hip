@@ -314,6 +314,7 @@ static MxFreq mx_freq(const csem_freq *f, int R)
 static MxAir mx_air(const csem_model *m, const MixState *x, const MixSlot *q)
 {
   MxAir a;
+  a.sc = nullptr;
   a.N = m->nfft; a.n1 = m->n1; a.n2 = m->n2; a.cw = x->cw;
   a.hz = q->hz0; a.buf = q->buf; a.bufx = q->bufx; a.bufy = q->bufy;
   a.sH1 = x->sH1; a.sH2 = x->sH2; a.tw = x->tw;
@@ -410,7 +411,8 @@ static int mx_apply(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
   if (m->airbc) {
     const int nh = m->n1 * m->n2, N = m->nfft;
     MxAir a = mx_air(m, x, q);
-    KT(KT_M_HZ0, s, 24.0 * nh * R, (k_mx_hz0<<<dim3((nh + 255) / 256, R), 256, 0, s>>>(mx_geo(m, x), in, q->hz0)));
+    a.sc = sc;
+    KT(KT_M_HZ0, s, 24.0 * nh * R, (k_mx_hz0<<<dim3((nh + 255) / 256, R), 256, 0, s>>>(mx_geo(m, x), in, q->hz0, sc)));
     KT(KT_M_AIR_ROWS, s, 8.0 * N * N * R,
        (k_mx_air_rows<<<dim3(N, R), 256, 3 * (size_t)N * sizeof(c2), s>>>(a)));
     KT(KT_M_AIR_COLS, s, 40.0 * N * N * R,

@@ -240,9 +240,10 @@ __global__ __launch_bounds__(64 * NW) void k_mx_apply(MxGeo g, MxFreq fr, const 
 }
 
 // H'z = curl E on the plane k = 0 for every r (the airwave input).
-__global__ void k_mx_hz0(MxGeo g, const c2 *x, c2 *hz)
+__global__ void k_mx_hz0(MxGeo g, const c2 *x, c2 *hz, const double *sc)
 {
   const int n1 = g.n1, n2 = g.n2, r = blockIdx.y;
+  if (!mx_live(sc, r)) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n1 * n2) return;
   const int i = t % n1, j = t / n1;
@@ -348,6 +349,7 @@ struct MxAir {
   const int *gxj1, *gxj2, *gyj1, *gyj2;
   const float *gxw1, *gxw2, *gyw1, *gyw2;
   c2 *hxg, *hyg;                     // R x ghost plane
+  const double *sc;                  // inner-solve scalars: right-hand sides done are skipped (NULL: none)
 };
 
 // Bilinear H'z -> uniform row, forward row transform (airbc_init_free.c:137-153).
@@ -355,6 +357,7 @@ __global__ void k_mx_air_rows(MxAir a)
 {
   extern __shared__ c2 smc[];
   const int N = a.N, i2 = blockIdx.x, r = blockIdx.y, n1 = a.n1;
+  if (!mx_live(a.sc, r)) return;
   c2 *A = smc, *B = smc + N, *T = smc + 2 * N;
   mx_tw_lds(T, a.tw, N);
   const c2 *h = a.hz + r * (long)n1 * a.n2;
@@ -394,6 +397,7 @@ __global__ __launch_bounds__(256) void k_mx_air_cols(MxAir a)
   const int ng = a.N / a.cw, per = (ng + 7) / 8, g = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
   if (g >= ng) return;
   const int N = a.N, cw = a.cw, c0 = g * cw, r = blockIdx.y;
+  if (!mx_live(a.sc, r)) return;
   const long off = r * (long)N * N;
   c2 *A = smc, *B = smc + N * cw, *T = smc + 2 * N * cw;
   const int t = threadIdx.x, nt = blockDim.x;
@@ -425,6 +429,7 @@ __global__ void k_mx_air_rows_inv(MxAir a)
 {
   extern __shared__ c2 smc[];
   const int N = a.N, w = blockIdx.x, r = blockIdx.y;
+  if (!mx_live(a.sc, r)) return;
   c2 *buf = ((w < a.nrowx) ? a.bufx : a.bufy) + r * (long)N * N;
   const long row = a.rows[w];
   c2 *A = smc, *B = smc + N, *T = smc + 2 * N;
@@ -440,6 +445,7 @@ __global__ void k_mx_air_rows_inv(MxAir a)
 __global__ void k_mx_air_ghost(MxAir a)
 {
   const int N = a.N, n1 = a.n1, n2 = a.n2, r = blockIdx.y;
+  if (!mx_live(a.sc, r)) return;
   const float inn = 1.f / ((float)N * (float)N);
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = (n1 + 1) * n2, ny = n1 * (n2 + 1);
