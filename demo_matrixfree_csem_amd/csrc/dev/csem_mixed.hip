@@ -696,6 +696,8 @@ __global__ __launch_bounds__(RED_T) void k_mx_norm_fin(const double *npart, int 
   if (threadIdx.x == 0) out[r] = s;
 }
 
+#define MX_STALL_ITERS 400  // inner iterations without 1% progress that end an inner solve
+
 static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const csem_freq *f, z2 *X, const z2 *B,
                             const double *tol, const csem_mixed_opts &o, int *iters, double *relres, hipStream_t s)
 {
@@ -742,7 +744,12 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
     KT(KT_M_DOT, s, 32.0 * R * n, (k_mx_k_init<<<dim3(nch, R), RED_T, 0, s>>>(k)));
     KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 0)));
     if (o.use_graph && (rc = mx_graph(m, x, q, R, fr, s))) return rc;
-    // inner iterations, convergence checked every check_every
+    // inner iterations, convergence checked every check_every; an inner
+    // solve whose recursive residual has not improved by 1% in
+    // MX_STALL_ITERS iterations (FP32 stagnation on an ill-conditioned
+    // system) is ended, and the refinement step decides what it was worth
+    std::vector<double> best(R, 1.0);
+    std::vector<double> best_it(R, 0.0);
     for (;;) {
       for (int w = 0; w < std::max(1, o.check_every); w++) {
         if (o.use_graph) HIP_OK(hipGraphLaunch(q->gexec, s));
@@ -751,7 +758,21 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
       HIP_OK(hipMemcpyAsync(hsc.data(), q->sc, hsc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
       HIP_OK(hipStreamSynchronize(s));
       bool all = true;
-      for (int r = 0; r < R; r++) all &= hsc[(size_t)r * MS_N + MS_DONE] != 0.0;
+      for (int r = 0; r < R; r++) {
+        double *sc = &hsc[(size_t)r * MS_N];
+        if (sc[MS_DONE] == 0.0) {
+          if (sc[MS_REL] < 0.99 * best[r]) {
+            best[r] = sc[MS_REL];
+            best_it[r] = sc[MS_IT];
+          } else if (sc[MS_IT] - best_it[r] >= MX_STALL_ITERS) {
+            const double one = 1.0;
+            sc[MS_DONE] = 1.0;
+            HIP_OK(hipMemcpyAsync(q->sc + (size_t)r * MS_N + MS_DONE, &one, sizeof one, hipMemcpyHostToDevice, s));
+            HIP_OK(hipStreamSynchronize(s));
+          }
+        }
+        all &= sc[MS_DONE] != 0.0;
+      }
       if (all) break;
     }
     for (int r = 0; r < R; r++)

@@ -123,7 +123,10 @@ int csem_vec_addscaled(long n, void *y, double re, double im, const void *x, voi
  * the number of right-hand sides that reached tol, or < 0 on error;
  * iters[r] receives the inner iterations spent, relres[r] the final true
  * relative residual (host arrays).  opt may be NULL for the defaults.
- * A refinement step that does not lower the true residual (the FP32 inner
+ * An inner solve whose recursive residual has made no new 1% improvement
+ * in 400 iterations is ended early (FP32 stagnation); its correction is
+ * kept if it lowers the true residual and the next refinement step starts
+ * afresh from that residual.  A refinement step that does not lower the true residual (the FP32 inner
  * solve stagnating or diverging on an ill-conditioned system) is undone
  * and refinement of that right-hand side stops: x keeps the best solution
  * found and relres[r] its residual, never above 1.  A caller that needs

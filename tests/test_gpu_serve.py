@@ -10,7 +10,7 @@ FP64 relative residual below its tol = 1e-8/f.
 import numpy as np
 import pytest
 
-from demo_matrixfree_csem_amd import surveys
+from demo_matrixfree_csem_amd import _lib, surveys
 from demo_matrixfree_csem_amd.serve import Engine, solve_tol
 
 pytestmark = pytest.mark.gpu
@@ -87,16 +87,17 @@ def test_admission_evicts_idle_models_within_budget():
 
 
 def test_engine_finishes_a_stalled_solve_in_fp64(workdir):
-    """A request the FP32 inner solve cannot finish (the anisotropic model
-    at 0.25 Hz, strongly stretched padding): the engine continues it with
-    the FP64 solver and serves receivers equal to a plain FP64 solve."""
+    """A request the mixed solve does not finish within its refinement
+    budget (the anisotropic model at 0.25 Hz, strongly stretched padding,
+    one refinement step allowed): the engine continues it with the FP64
+    solver and serves receivers equal to a plain FP64 solve."""
     from demo_matrixfree_csem_amd.session import Session
     s = surveys.vti_marine(2, 36, 24, 24, 33)
     src = [tuple(float(v) for v in s.acq.sources[0][:3])]
     recs = [tuple(float(v) for v in r[:3]) for r in s.acq.receivers]
     req = (("vti_marine", 36, 24, 24), lambda: s, [0.25], src, recs)
     print("case serve: stalled mixed solve finished in FP64", flush=True)
-    eng = Engine(workers=1, rhs_per_solve=1)
+    eng = Engine(workers=1, rhs_per_solve=1, opts=_lib.MixedOpts.default(max_outer=1))
     try:
         res, st = eng.run([req], [0.0])
     finally:
