@@ -696,7 +696,7 @@ __global__ __launch_bounds__(RED_T) void k_mx_norm_fin(const double *npart, int 
   if (threadIdx.x == 0) out[r] = s;
 }
 
-#define MX_STALL_ITERS 400  // inner iterations without 1% progress that end an inner solve
+#define MX_STALL_ITERS 100  // default of csem_mixed_opts.stall_iters
 
 static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const csem_freq *f, z2 *X, const z2 *B,
                             const double *tol, const csem_mixed_opts &o, int *iters, double *relres, hipStream_t s)
@@ -750,6 +750,7 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
     // system) is ended, and the refinement step decides what it was worth
     std::vector<double> best(R, 1.0);
     std::vector<double> best_it(R, 0.0);
+    const int stall = o.stall_iters == 0 ? MX_STALL_ITERS : o.stall_iters;
     for (;;) {
       for (int w = 0; w < std::max(1, o.check_every); w++) {
         if (o.use_graph) HIP_OK(hipGraphLaunch(q->gexec, s));
@@ -764,7 +765,7 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
           if (sc[MS_REL] < 0.99 * best[r]) {
             best[r] = sc[MS_REL];
             best_it[r] = sc[MS_IT];
-          } else if (sc[MS_IT] - best_it[r] >= MX_STALL_ITERS) {
+          } else if (stall > 0 && sc[MS_IT] - best_it[r] >= stall) {
             const double one = 1.0;
             sc[MS_DONE] = 1.0;
             HIP_OK(hipMemcpyAsync(q->sc + (size_t)r * MS_N + MS_DONE, &one, sizeof one, hipMemcpyHostToDevice, s));
@@ -831,7 +832,7 @@ extern "C" int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *
     if (e != hipSuccess) { slot_release(x, q); return csem_fail(-1, "sync: %s", hipGetErrorString(e)); }
   }
   const hipStream_t s = stream ? (hipStream_t)stream : q->own;
-  const csem_mixed_opts dflt = {2000, 1e-2, 40, 8, 1, CSEM_PRECOND_LUSGS};
+  const csem_mixed_opts dflt = {2000, 1e-2, 40, 8, 1, CSEM_PRECOND_LUSGS, 0};
   rc = mixed_solve_slot(m, x, q, R, f, (z2 *)xv, (const z2 *)bv, tol, opt ? *opt : dflt, iters, relres, s);
   slot_release(x, q);
   return rc;

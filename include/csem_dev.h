@@ -124,7 +124,7 @@ int csem_vec_addscaled(long n, void *y, double re, double im, const void *x, voi
  * iters[r] receives the inner iterations spent, relres[r] the final true
  * relative residual (host arrays).  opt may be NULL for the defaults.
  * An inner solve whose recursive residual has made no new 1% improvement
- * in 400 iterations is ended early (FP32 stagnation); its correction is
+ * in stall_iters iterations is ended early (FP32 stagnation); its correction is
  * kept if it lowers the true residual and the next refinement step starts
  * afresh from that residual.  A refinement step that does not lower the true residual (the FP32 inner
  * solve stagnating or diverging on an ill-conditioned system) is undone
@@ -145,6 +145,9 @@ typedef struct {
                         CSEM_PRECOND_RBSGS, the same symmetric Gauss-Seidel
                         splitting with the cells of each slice visited in
                         red-black order (fully parallel passes) */
+  int stall_iters;   /* inner iterations without a new 1% residual
+                        improvement that end an inner solve (0 = the
+                        default, 100; < 0 = never) */
 } csem_mixed_opts;
 #define CSEM_PRECOND_LUSGS 0
 #define CSEM_PRECOND_RBSGS 1
