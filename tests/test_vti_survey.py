@@ -127,3 +127,30 @@ def test_vti_mixed_matches_fp64_bicgstab_and_gmres(workdir):
         assert e < 1e-6, (f, e)
     finally:
         ses.close()
+
+
+@pytest.mark.gpu
+def test_vti_refinement_guard_without_the_stall_cut(workdir):
+    """With the stagnation cut off (stall_iters < 0) the FP32 inner solve
+    at 0.25 Hz on this model runs its whole budget and a refinement step
+    fails to lower the true residual: the guard must undo it and stop that
+    right-hand side with its best solution -- finite, relres below 1 and
+    equal to the true residual of the x returned -- instead of diverging."""
+    from demo_matrixfree_csem_amd import _lib
+    from demo_matrixfree_csem_amd.session import Session
+    s = surveys.vti_marine(2, 36, 24, 24, 33)
+    surveys.write_files(workdir, s)
+    f = 0.25
+    print("case vti_marine 36x24x24 refinement guard, stall cut off", flush=True)
+    ses = Session(s.argv(workdir))
+    try:
+        ses.set_freq(f)
+        b, tol = ses.inject(), _tol(f)
+        (x,), (it,), (rel,), n = ses.solve_mixed([b], [f], [tol],
+                                                 _lib.MixedOpts.default(max_inner=6000, stall_iters=-1))
+        print("mixed without the cut: %d inner iterations, relres %.2e" % (it, rel), flush=True)
+        assert n == 0 and tol < rel < 1e-3 and np.all(np.isfinite(x)), (n, it, rel)
+        res = np.linalg.norm(b - ses.fdfd_apply(x)) / np.linalg.norm(b)
+        assert abs(res - rel) <= 1e-3 * rel, (res, rel)
+    finally:
+        ses.close()
