@@ -60,7 +60,7 @@ static long file_floats(const char *fn)
 {
   FILE *fp = fopen(fn, "rb");
   long sz;
-  if (!fp) err("cannot open file %s\n", fn);
+  if (!fp) err("emf: unable to open %s", fn);
   if (!fp) return -1;
   fseek(fp, 0, SEEK_END);
   sz = ftell(fp);
@@ -72,11 +72,11 @@ static void read_floats(const char *fn, float *dst, long n, const char *what)
 {
   FILE *fp = fopen(fn, "rb");
   size_t got;
-  if (!fp) err("cannot open file %s=%s\n", what, fn);
+  if (!fp) err("emf: unable to open %s (%s=)", fn, what);
   if (!fp) return;
   got = fread(dst, sizeof(float), n, fp);
   fclose(fp);
-  if ((long)got != n) err("size not match %s: file=%d expected=%ld\n", what, (int)got, n);
+  if ((long)got != n) err("emf: %s holds %d values, the grid needs %ld", what, (int)got, n);
 }
 
 /* emf_init_free.c:14-172 */
@@ -87,7 +87,7 @@ void emf_init(emf_t *emf)
   long n;
   csem_set_status(0);
   if (!(emf->nfreq = countparval("freqs"))) {
-    err("Need freqs= vector");
+    err("emf: no frequencies given (freqs=f1,f2,...)");
     return;
   }
   emf->freqs = alloc1float(emf->nfreq);
@@ -118,7 +118,7 @@ void emf_init(emf_t *emf)
   if (!getparint("istretch", &emf->istretch)) emf->istretch = 1;
   if (!getparint("airbc", &emf->airbc)) emf->airbc = 0;
   if (emf->airbc && emf->nb == 0) {
-    err("airbc=1 assumes input model without air, you must ensure nb>0");
+    err("emf: the airwave boundary (airbc=1) needs padding cells below the model (nb >= 1)");
     return;
   }
   if (emf->verb) {
@@ -128,15 +128,15 @@ void emf_init(emf_t *emf)
            emf->rho_air, emf->nb);
   }
   if (!getparstring("fx1", &fx[0])) {
-    err("Need fx1= ");
+    err("emf: missing node file fx1=");
     return;
   }
   if (!getparstring("fx2", &fx[1])) {
-    err("Need fx2= ");
+    err("emf: missing node file fx2=");
     return;
   }
   if (!getparstring("fx3", &fx[2])) {
-    err("Need fx3= ");
+    err("emf: missing node file fx3=");
     return;
   }
   emf->nx = (int)file_floats(fx[0]) - 1;
@@ -159,15 +159,15 @@ void emf_init(emf_t *emf)
   emf->rho33 = alloc3float(emf->nx, emf->ny, emf->nz);
   n = (long)emf->nx * emf->ny * emf->nz;
   if (!getparstring("frho11", &frho[0])) {
-    err("Need frho11= ");
+    err("emf: missing resistivity file frho11=");
     return;
   }
   if (!getparstring("frho22", &frho[1])) {
-    err("Need frho22= ");
+    err("emf: missing resistivity file frho22=");
     return;
   }
   if (!getparstring("frho33", &frho[2])) {
-    err("Need frho33= ");
+    err("emf: missing resistivity file frho33=");
     return;
   }
   read_floats(frho[0], &emf->rho11[0][0][0], n, "frho11");
@@ -208,7 +208,7 @@ static int read_table(const char *fn, float **cols, int cap)
   FILE *fp = fopen(fn, "r");
   float x, y, z, hd, pit;
   int idx, n = 0, c;
-  if (!fp) err("file does not exist: %s", fn);
+  if (!fp) err("acq: unable to open %s", fn);
   if (!fp) return 0;
   while ((c = fgetc(fp)) != EOF && c != '\n') {
   }
@@ -236,11 +236,11 @@ void acq_init(acq_t *acq, emf_t *emf)
   if (!getparint("nsubsrc", &acq->nsubsrc)) acq->nsubsrc = 1;
   if (!getparint("nsubrec", &acq->nsubrec)) acq->nsubrec = 1;
   if (!getparstring("fsrc", &fsrc)) {
-    err("Need fsrc= ");
+    err("acq: missing source table fsrc=");
     return;
   }
   if (!getparstring("frec", &frec)) {
-    err("Need frec= ");
+    err("acq: missing receiver table frec=");
     return;
   }
   for (i = 0; i < 5; i++) {
@@ -283,27 +283,27 @@ void acq_init(acq_t *acq, emf_t *emf)
     free(rc[i]);
   }
   if (emf->x1node[0] > acq->x1min) {
-    err("x1 origin > acq->x1min");
+    err("acq: a source or receiver lies left of the model in x (below x1node[0])");
     return;
   }
   if (emf->x1node[emf->nx] < acq->x1max) {
-    err("x1 end < acq->x1min");
+    err("acq: a source or receiver lies right of the model in x (beyond x1node[nx])");
     return;
   }
   if (emf->x2node[0] > acq->x2min) {
-    err("x2 origin > acq->x2min");
+    err("acq: a source or receiver lies below the model range in y (below x2node[0])");
     return;
   }
   if (emf->x2node[emf->ny] < acq->x2max) {
-    err("x2 end < acq->x2min");
+    err("acq: a source or receiver lies beyond the model range in y (beyond x2node[ny])");
     return;
   }
   if (emf->x3node[0] > acq->x3min) {
-    err("x3 origin > acq->x3min");
+    err("acq: a source or receiver lies above the model top (above x3node[0])");
     return;
   }
   if (emf->x3node[emf->nz] < acq->x3max) {
-    err("x3 end < acq->x3min");
+    err("acq: a source or receiver lies below the model bottom (beyond x3node[nz])");
     return;
   }
 }
@@ -515,7 +515,7 @@ void write_data(acq_t *acq, emf_t *emf, char *fname, float _Complex ***dcal_fd)
 {
   FILE *fp = fopen(fname, "w");
   int ic, ifr, ir;
-  if (!fp) err("error opening file for writing");
+  if (!fp) err("write_data: unable to create %s", fname);
   if (!fp) return;
   fprintf(fp, "iTx \t iRx    ichrec  ifreq \t emf_real \t emf_imag\n");
   for (ic = 0; ic < emf->nchrec; ic++)

@@ -173,8 +173,10 @@ class Session:
                 setattr(a, k, C.cast(C.c_void_p(v), C.POINTER(C.c_float)))
             e.dcal_fd = C.cast(C.c_void_p(keep_fd), type(e.dcal_fd))
 
-    def solve(self, b, tol, niter=None, algopt=None, preco=None, nrestart=None, x0=None):
-        """The solver call of main.c:104-118; returns (x, iterations)."""
+    def solve(self, b, tol, niter=None, algopt=None, preco=None, nrestart=None, x0=None, verb=0):
+        """The solver call of main.c:104-118; returns (x, iterations).
+        With verb=1 the solver writes its iterate_*.txt history to the
+        current directory, as the reference's does (solver.c:127-141)."""
         e = self.emf
         L = lib()
         niter = e.niter if niter is None else niter
@@ -186,13 +188,13 @@ class Session:
         A = _lib.fn_addr("fdfd_apply")
         M = _lib.fn_addr("lusgs_apply")
         if preco == 0 and algopt == 1:
-            L.bicgstab(self.n, _p(x), _p(b), A, niter, tol, 0)
+            L.bicgstab(self.n, _p(x), _p(b), A, niter, tol, verb)
         elif preco == 0:
-            L.gmres(self.n, _p(x), _p(b), A, niter // nrestart, tol, nrestart, 0)
+            L.gmres(self.n, _p(x), _p(b), A, niter // nrestart, tol, nrestart, verb)
         elif algopt == 1:
-            L.fbicgstab(self.n, _p(x), _p(b), A, M, niter, tol, 0)
+            L.fbicgstab(self.n, _p(x), _p(b), A, M, niter, tol, verb)
         else:
-            L.fgmres(self.n, _p(x), _p(b), A, M, niter // nrestart, tol, nrestart, 0)
+            L.fgmres(self.n, _p(x), _p(b), A, M, niter // nrestart, tol, nrestart, verb)
         self._status("solve")
         return x, L.csem_last_iterations()
 

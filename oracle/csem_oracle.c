@@ -40,7 +40,7 @@ struct orc {
   zc *xh, *xx, *yy, *Hxg, *Hyg;
   int nfft;
   double d1fft, d2fft, d3fft;
-  zc *sH1, *sH2, *buf, *buf0, *tw, *tmp;
+  zc *sH1, *sH2, *buf, *buf0, *buf2, *tw, *tmp;
   int *jx, *jy;
   double *wx, *wy;
   zc iomu, iwmu;
@@ -355,7 +355,7 @@ void orc_free(orc_t *m)
   free(m->s11); free(m->s22); free(m->s33);
   free(m->c11); free(m->c22); free(m->c33);
   free(m->xh); free(m->xx); free(m->yy); free(m->Hxg); free(m->Hyg);
-  free(m->sH1); free(m->sH2); free(m->buf); free(m->buf0); free(m->tw); free(m->tmp);
+  free(m->sH1); free(m->sH2); free(m->buf); free(m->buf0); free(m->buf2); free(m->tw); free(m->tmp);
   free(m->jx); free(m->jy); free(m->wx); free(m->wy);
   free(m);
 }
@@ -678,6 +678,7 @@ static void fft_init(orc_t *m)
   m->tmp = zalloc(2 * N);
   m->buf = zalloc(NN);
   m->buf0 = zalloc(NN);
+  m->buf2 = zalloc(NN);
   m->sH1 = zalloc(NN);
   m->sH2 = zalloc(NN);
   m->jx = (int *)calloc(N, sizeof(int));
@@ -803,13 +804,18 @@ void orc_fft2(orc_t *m, zc *data, int dir)
   }
 }
 
-/* airbc_apply, airbc_init_free.c:130-209 */
-void orc_airbc_apply(orc_t *m, const zc *H3k, zc *H1km1, zc *H2km1)
+/* airbc_apply, airbc_init_free.c:130-209, in its four stages.  Each stage
+ * is exported on its own (orc_airbc_stage) so that tests can check it
+ * against what the mathematics fixes; orc_airbc_apply is their
+ * composition, in the reference's order of operations. */
+
+/* Stage 0 (airbc_init_free.c:137-152): Hz(z=0) at (x1s[j1], x2s[j2]),
+ * j1 < n1, j2 < n2, bilinearly onto the uniform N x N grid
+ * (x1s[0] + i1*d1fft, x2s[0] + i2*d2fft), zero outside the hull. */
+static void air_forward_map(const orc_t *m, const zc *H3k, zc *U)
 {
   int N = m->nfft, n1 = m->n1, n2 = m->n2, i1, i2, j1, j2;
-  double x1, x2, w1, w2, s1, s2;
-  long NN = (long)N * N;
-  zc *buf = m->buf, *buf0 = m->buf0;
+  double x1, x2, w1, w2;
   for (i2 = 0; i2 < N; i2++) {
     x2 = m->x2s[0] + i2 * m->d2fft;
     for (i1 = 0; i1 < N; i1++) {
@@ -817,20 +823,41 @@ void orc_airbc_apply(orc_t *m, const zc *H3k, zc *H1km1, zc *H2km1)
       if (x1 < m->x1s[n1 - 1] && x2 < m->x2s[n2 - 1]) {
         j1 = m->jx[i1]; j2 = m->jy[i2];
         w1 = m->wx[i1]; w2 = m->wy[i2];
-        buf[i1 + (long)N * i2] = (1. - w1) * (1. - w2) * H3k[j1 + (long)n1 * j2] +
-                                 w1 * (1. - w2) * H3k[j1 + 1 + (long)n1 * j2] +
-                                 (1. - w1) * w2 * H3k[j1 + (long)n1 * (j2 + 1)] +
-                                 w1 * w2 * H3k[j1 + 1 + (long)n1 * (j2 + 1)];
+        U[i1 + (long)N * i2] = (1. - w1) * (1. - w2) * H3k[j1 + (long)n1 * j2] +
+                               w1 * (1. - w2) * H3k[j1 + 1 + (long)n1 * j2] +
+                               (1. - w1) * w2 * H3k[j1 + (long)n1 * (j2 + 1)] +
+                               w1 * w2 * H3k[j1 + 1 + (long)n1 * (j2 + 1)];
       } else
-        buf[i1 + (long)N * i2] = 0.;
+        U[i1 + (long)N * i2] = 0.;
     }
   }
-  orc_fft2(m, buf, -1);
-  memcpy(buf0, buf, NN * sizeof(zc));
-  for (long q = 0; q < NN; q++) buf[q] = buf0[q] * m->sH1[q];
-  orc_fft2(m, buf, +1);
-  for (long q = 0; q < NN; q++) buf[q] /= (N * N);
-  /* ghost Hx at (x1[i1], x2s[i2]); the row i1 = n1 is never consumed */
+}
+
+/* Stage 1 (airbc_init_free.c:153-161, 182-188): forward 2-D DFT, times sH1
+ * (resp. sH2), inverse DFT, divided by N*N.  U is overwritten by its
+ * spectrum. */
+static void air_spectral(orc_t *m, zc *U, zc *Hxu, zc *Hyu)
+{
+  int N = m->nfft;
+  long NN = (long)N * N, q;
+  orc_fft2(m, U, -1);
+  for (q = 0; q < NN; q++) Hxu[q] = U[q] * m->sH1[q];
+  orc_fft2(m, Hxu, +1);
+  for (q = 0; q < NN; q++) Hxu[q] /= (N * N);
+  for (q = 0; q < NN; q++) Hyu[q] = U[q] * m->sH2[q];
+  orc_fft2(m, Hyu, +1);
+  for (q = 0; q < NN; q++) Hyu[q] /= (N * N);
+}
+
+/* Stage 2 (airbc_init_free.c:162-180): ghost Hx at (x1[i1], x2s[i2]) from
+ * the uniform grid, cell index (x - origin)/d truncated.  The column
+ * i1 = n1 is never consumed by the operator (its k=0 Ey rows stop at
+ * i1 = n1-1) and is set to zero: the reference reads it across the end of
+ * a uniform row. */
+static void air_back_hx(const orc_t *m, const zc *U, zc *H1km1)
+{
+  int N = m->nfft, n1 = m->n1, n2 = m->n2, i1, i2, j1, j2;
+  double w1, w2, s1, s2;
   for (i2 = 0; i2 < n2; i2++) {
     s2 = (m->x2s[i2] - m->x2s[0]);
     j2 = s2 / m->d2fft;
@@ -839,17 +866,22 @@ void orc_airbc_apply(orc_t *m, const zc *H3k, zc *H1km1, zc *H2km1)
       s1 = (m->x1[i1] - m->x1[0]);
       j1 = s1 / m->d1fft;
       w1 = s1 / m->d1fft - j1;
-      H1km1[i1 + (long)(n1 + 1) * i2] = (1. - w1) * (1. - w2) * buf[j1 + (long)N * j2] +
-                                        w1 * (1. - w2) * buf[j1 + 1 + (long)N * j2] +
-                                        (1. - w1) * w2 * buf[j1 + (long)N * (j2 + 1)] +
-                                        w1 * w2 * buf[j1 + 1 + (long)N * (j2 + 1)];
+      H1km1[i1 + (long)(n1 + 1) * i2] = (1. - w1) * (1. - w2) * U[j1 + (long)N * j2] +
+                                        w1 * (1. - w2) * U[j1 + 1 + (long)N * j2] +
+                                        (1. - w1) * w2 * U[j1 + (long)N * (j2 + 1)] +
+                                        w1 * w2 * U[j1 + 1 + (long)N * (j2 + 1)];
     }
     H1km1[n1 + (long)(n1 + 1) * i2] = 0.;
   }
-  for (long q = 0; q < NN; q++) buf[q] = buf0[q] * m->sH2[q];
-  orc_fft2(m, buf, +1);
-  for (long q = 0; q < NN; q++) buf[q] /= (N * N);
-  /* ghost Hy at (x1s[i1], x2[i2]); the column i2 = n2 is never consumed */
+}
+
+/* Stage 3 (airbc_init_free.c:189-207): ghost Hy at (x1s[i1], x2[i2]).  The
+ * row i2 = n2 is never consumed (the k=0 Ex rows stop at i2 = n2-1) and
+ * is set to zero: the reference reads it past the end of the buffer. */
+static void air_back_hy(const orc_t *m, const zc *U, zc *H2km1)
+{
+  int N = m->nfft, n1 = m->n1, n2 = m->n2, i1, i2, j1, j2;
+  double w1, w2, s1, s2;
   for (i2 = 0; i2 < n2; i2++) {
     s2 = (m->x2[i2] - m->x2[0]);
     j2 = s2 / m->d2fft;
@@ -858,13 +890,35 @@ void orc_airbc_apply(orc_t *m, const zc *H3k, zc *H1km1, zc *H2km1)
       s1 = (m->x1s[i1] - m->x1s[0]);
       j1 = s1 / m->d1fft;
       w1 = s1 / m->d1fft - j1;
-      H2km1[i1 + (long)n1 * i2] = (1. - w1) * (1. - w2) * buf[j1 + (long)N * j2] +
-                                  w1 * (1. - w2) * buf[j1 + 1 + (long)N * j2] +
-                                  (1. - w1) * w2 * buf[j1 + (long)N * (j2 + 1)] +
-                                  w1 * w2 * buf[j1 + 1 + (long)N * (j2 + 1)];
+      H2km1[i1 + (long)n1 * i2] = (1. - w1) * (1. - w2) * U[j1 + (long)N * j2] +
+                                  w1 * (1. - w2) * U[j1 + 1 + (long)N * j2] +
+                                  (1. - w1) * w2 * U[j1 + (long)N * (j2 + 1)] +
+                                  w1 * w2 * U[j1 + 1 + (long)N * (j2 + 1)];
     }
   }
   for (i1 = 0; i1 < n1; i1++) H2km1[i1 + (long)n1 * n2] = 0.;
+}
+
+void orc_airbc_stage(orc_t *m, int stage, const zc *in, zc *out, zc *out2)
+{
+  long NN = (long)m->nfft * m->nfft;
+  switch (stage) {
+  case 0: air_forward_map(m, in, out); break;
+  case 1:
+    memcpy(m->buf0, in, NN * sizeof(zc));
+    air_spectral(m, m->buf0, out, out2);
+    break;
+  case 2: air_back_hx(m, in, out); break;
+  case 3: air_back_hy(m, in, out); break;
+  }
+}
+
+void orc_airbc_apply(orc_t *m, const zc *H3k, zc *H1km1, zc *H2km1)
+{
+  air_forward_map(m, H3k, m->buf0);
+  air_spectral(m, m->buf0, m->buf, m->buf2);
+  air_back_hx(m, m->buf, H1km1);
+  air_back_hy(m, m->buf2, H2km1);
 }
 
 /* ------------------------------------------------------------ reductions */

@@ -55,6 +55,11 @@ void orc_set_freq(orc_t *m, float freq);
 void orc_fdfd_apply(orc_t *m, const orc_c128 *x, orc_c128 *y);
 void orc_lusgs_apply(orc_t *m, const orc_c128 *x, orc_c128 *y);
 void orc_airbc_apply(orc_t *m, const orc_c128 *H3k, orc_c128 *H1km1, orc_c128 *H2km1);
+/* One stage of orc_airbc_apply: 0 forward bilinear map Hz(n1*n2) ->
+ * uniform N*N; 1 uniform N*N -> (Hx, Hy) on the uniform grid (DFT,
+ * multipliers, inverse DFT / N^2; out and out2); 2 uniform -> ghost Hx
+ * (n1+1)*n2; 3 uniform -> ghost Hy n1*(n2+1). */
+void orc_airbc_stage(orc_t *m, int stage, const orc_c128 *in, orc_c128 *out, orc_c128 *out2);
 
 /* 2-D transform used by the airwave boundary: data[i1 + n*i2].
  * dir=-1 forward (rows i1 then columns i2), dir=+1 inverse unnormalised

@@ -49,6 +49,7 @@ def lib():
         for f in ("orc_fdfd_apply", "orc_lusgs_apply"):
             getattr(L, f).argtypes = [vp, vp, vp]
         L.orc_airbc_apply.argtypes = [vp, vp, vp, vp]
+        L.orc_airbc_stage.argtypes = [vp, ip, vp, vp, vp]
         L.orc_fft2.argtypes = [vp, vp, ip]
         L.orc_dot.restype = _C128
         L.orc_dot.argtypes = [vp, C.c_long, vp, vp]
@@ -161,6 +162,18 @@ class Oracle:
         hy = np.zeros(self.n1 * (self.n2 + 1), np.complex128)
         lib().orc_airbc_apply(self.h, _p(hz), _p(hx), _p(hy))
         return hx, hy
+
+    def airbc_stage(self, stage, data):
+        """One stage of airbc_apply (see orc_airbc_stage): 0 forward map
+        (returns N*N), 1 spectral continuation (returns Hx, Hy on the
+        uniform grid), 2 / 3 back maps to ghost Hx / Hy."""
+        N, n1, n2 = self.nfft, self.n1, self.n2
+        d = np.ascontiguousarray(data, np.complex128)
+        size = {0: N * N, 1: N * N, 2: (n1 + 1) * n2, 3: n1 * (n2 + 1)}[stage]
+        out = np.zeros(size, np.complex128)
+        out2 = np.zeros(size if stage == 1 else 1, np.complex128)
+        lib().orc_airbc_stage(self.h, stage, _p(d), _p(out), _p(out2))
+        return (out, out2) if stage == 1 else out
 
     def fft2(self, data, direction):
         d = np.ascontiguousarray(data, np.complex128).copy()
