@@ -100,7 +100,8 @@ HOST_SYMBOLS = [
     "free1double", "free3double", "alloc1complex", "alloc2complex", "free1complex", "free2complex",
     "alloc3complexf", "free3complexf", "err", "warn", "csem_last_status", "csem_set_exit_on_error",
     "csem_active_model", "csem_last_iterations", "csem_fdfd_prepare", "csem_model_from_emf",
-    "iproc", "nproc", "ierr",
+    "iproc", "nproc", "ierr", "aphi_apply", "gmres_init", "gmres_free", "gmres_apply",
+    "Acsr_init", "Acsr_free", "Acsr_apply", "Acsr_lusgs_apply", "Acoo_init", "Acoo_free", "Acoo_solve",
 ]
 DEV_SYMBOLS = [
     "csem_last_error", "csem_freq_consts", "csem_model_create", "csem_model_destroy", "csem_model_len",
@@ -109,6 +110,8 @@ DEV_SYMBOLS = [
     "csem_malloc", "csem_free", "csem_memcpy_h2d", "csem_memcpy_d2h", "csem_memset", "csem_copy", "csem_sync",
     "csem_kernel_timing", "csem_kernel_stats", "csem_set_reduction",
     "csem_vec_sub", "csem_vec_divr", "csem_vec_divz", "csem_vec_subscaled_dev", "csem_vec_addscaled",
+    "csem_op_apply", "csem_op_lusgs", "csem_op_aphi", "csem_op_gmres", "csem_op_host",
+    "csem_gmres_prec_release", "csem_bicgstab_op", "csem_gmres_op",
     "csem_mixed_solve", "csem_apply32", "csem_lusgs32", "csem_mixed_tuning", "csem_mixed_set_tile", "csem_mixed_bytes", "csem_mixed_slot_bytes", "csem_mem_info",
 ]
 
@@ -175,6 +178,11 @@ def lib():
     L.initargs.argtypes = [ip, C.POINTER(C.c_char_p)]
     L.fdfd_apply.argtypes = [ip, vp, vp]
     L.lusgs_apply.argtypes = [ip, vp, vp]
+    L.aphi_apply.argtypes = [ip, vp, vp]
+    L.gmres_init.argtypes = [ip, ip, vp]
+    L.gmres_free.argtypes = []
+    L.gmres_apply.argtypes = [ip, vp, vp]
+    L.Acsr_init.argtypes = [C.POINTER(EmfT), ip]
     L.airbc_apply.argtypes = [C.POINTER(EmfT), vp, vp, vp]
     op_t = C.CFUNCTYPE(None, ip, vp, vp)
     L.fbicgstab.argtypes = [ip, vp, vp, vp, vp, ip, dp, ip]

@@ -781,23 +781,340 @@ static int ensure_kv(csem_model *m, int nv)
   return 0;
 }
 
-// Flexible right-preconditioned BiCGStab (solver.c:93-178) or, with
-// precond=0, BiCGStab (solver.c:16-87), all vectors and scalars resident.
-extern "C" int csem_bicgstab(csem_model *m, const csem_freq *f, void *xv, const void *bv, int niter,
-                             double tol, int precond, double *hist, void *stream)
+extern "C" int csem_op_apply(void *, csem_model *m, const csem_freq *f, const void *x, void *y, void *stream)
 {
-  hipStream_t s = S(stream);
-  const long n = m->lE, nch = (n + RED_CHUNK - 1) / RED_CHUNK;
-  int rc = ensure_kv(m, 7);
-  if (rc) return rc;
-  z2 *x = (z2 *)xv;
-  const z2 *b = (const z2 *)bv;
-  z2 *r = m->kv, *r0 = r + n, *p = r0 + n, *pp = p + n, *v = pp + n, *sv = v + n, *t = sv + n;
-  double *sc = m->scal, *pa = m->partials;
+  if (!m) return csem_fail(-4, "csem_op_apply: no model");
+  return run_apply(m, f, (const z2 *)x, (z2 *)y, S(stream));
+}
+
+extern "C" int csem_op_lusgs(void *, csem_model *m, const csem_freq *f, const void *x, void *y, void *stream)
+{
+  if (!m) return csem_fail(-4, "csem_op_lusgs: no model");
+  return run_lusgs(m, f, (const z2 *)x, (z2 *)y, S(stream));
+}
+
+// ------------------------------------------------- A-phi preconditioner
+//
+// aphi_apply (fdfd.c:513-821): b_phi = -div x; one symmetric Gauss-Seidel
+// pass from zero over the A-phi splitting, ax, ay, az, phi forward then
+// phi, az, ay, ax backward; y = A + grad(phi).  Each row is the
+// reference's expression tree (the arithmetic helpers of csem_common.h, IEEE
+// division, Smith's complex division).  Each component's rows couple only
+// within 2-D slices (ax: fixed i, rows (k, j); ay: fixed j, (k, i); az: fixed
+// k, (j, i)) and phi's within the 3-D grid, so the lexicographic order is
+// kept by sweeping diagonals (slow + fast = d) of all slices at once, and
+// phi by planes k + j + i = d: every cell then reads its predecessors' new
+// values and its successors' old ones, as in the serial loops.  A cell's
+// own value, which the reference zeroes before forming its row, enters as
+// zero.  One launch per diagonal.
+
+struct AphiArgs {
+  int n1, n2, n3, fvm;
+  long lEx, lEy;
+  const double *d1, *d2, *d3, *d1s, *d2s, *d3s;  // d3, d3s at the operator's first plane
+  const double *s11, *s22, *s33;                 // conductivities at the operator's first plane
+  const double *S11, *S22, *S33;                 // the same arrays from the full grid's first plane
+  z2 iwmu, iomu;
+  const z2 *bx, *by, *bz;
+  z2 *ax, *ay, *az, *v, *bv;
+};
+
+__device__ __forceinline__ long a_idv(const AphiArgs &a, int k, int j, int i)
+{
+  return i + (long)(a.n1 + 1) * (j + (long)(a.n2 + 1) * k);
+}
+__device__ __forceinline__ z2 zneg(z2 x) { return zmk(-x.x, -x.y); }
+
+// fdfd.c:554-566, plus zeroing A and phi (fdfd.c:570-573)
+__global__ void k_aphi_init(AphiArgs a, long lE, long lv)
+{
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int n1 = a.n1, n2 = a.n2;
+  if (e < lE) a.ax[e] = zmk(0.0, 0.0);  // ax, ay, az are y
+  if (e >= lv) return;
+  a.v[e] = zmk(0.0, 0.0);
+  const int i = (int)(e % (n1 + 1)), j = (int)((e / (n1 + 1)) % (n2 + 1)), k = (int)(e / ((long)(n1 + 1) * (n2 + 1)));
+  z2 b = zmk(0.0, 0.0);
+  if (k >= 1 && k < a.n3 && j >= 1 && j < n2 && i >= 1 && i < n1) {
+    b = zadd(b, zdivr(zneg(zsub(a.bx[id1(n1, n2, k, j, i)], a.bx[id1(n1, n2, k, j, i - 1)])), a.d1[i]));
+    b = zadd(b, zdivr(zneg(zsub(a.by[id2(n1, n2, k, j, i)], a.by[id2(n1, n2, k, j - 1, i)])), a.d2[j]));
+    b = zadd(b, zdivr(zneg(zsub(a.bz[id3(n1, n2, k, j, i)], a.bz[id3(n1, n2, k - 1, j, i)])), a.d3[k]));
+  }
+  a.bv[e] = b;
+}
+
+__device__ __forceinline__ void aphi_fvm(const AphiArgs &a, int k, int j, int i, z2 &aij, z2 &aii)
+{
+  if (a.fvm) {
+    const double vol = a.d1s[i] * a.d2[j] * a.d3[k];
+    const z2 vq = zrmul(vol, a.iomu);
+    aij = zmul(aij, vq);
+    aii = zmul(aii, vq);
+  }
+}
+
+// fdfd.c:585-604 / 768-787
+__device__ void aphi_row_ax(const AphiArgs &a, int k, int j, int i)
+{
+  const int n1 = a.n1, n2 = a.n2, kp1 = k + 1, km1 = k - 1, jp1 = j + 1, jm1 = j - 1;
+  const int ip1 = i + 1 < n1 ? i + 1 : n1;
+  const z2 Z = zmk(0.0, 0.0);
+  const z2 *ax = a.ax, *ay = a.ay, *az = a.az, *v = a.v;
+  z2 t1 = zsub(zdivr(zsub(ay[id2(n1, n2, k, j, ip1)], ay[id2(n1, n2, k, j, i)]), a.d1s[i]),
+               zdivr(zsub(ax[id1(n1, n2, k, jp1, i)], Z), a.d2s[j]));
+  z2 t2 = zsub(zdivr(zsub(ay[id2(n1, n2, k, jm1, ip1)], ay[id2(n1, n2, k, jm1, i)]), a.d1s[i]),
+               zdivr(zsub(Z, ax[id1(n1, n2, k, jm1, i)]), a.d2s[jm1]));
+  z2 t3 = zsub(zdivr(zsub(ax[id1(n1, n2, kp1, j, i)], Z), a.d3s[k]),
+               zdivr(zsub(az[id3(n1, n2, k, j, ip1)], az[id3(n1, n2, k, j, i)]), a.d1s[i]));
+  z2 t4 = zsub(zdivr(zsub(Z, ax[id1(n1, n2, km1, j, i)]), a.d3s[km1]),
+               zdivr(zsub(az[id3(n1, n2, km1, j, ip1)], az[id3(n1, n2, km1, j, i)]), a.d1s[i]));
+  const double sg = a.s11[id1(n1, n2, k, j, i)];
+  z2 aij = zsub(zmul(zsub(zdivr(zsub(t1, t2), a.d2[j]), zdivr(zsub(t3, t4), a.d3[k])), a.iwmu),
+                zrmul(sg, zadd(Z, zdivr(zsub(v[a_idv(a, k, j, ip1)], v[a_idv(a, k, j, i)]), a.d1s[i]))));
+  const double r = (1. / a.d2s[j] + 1. / a.d2s[jm1]) / a.d2[j] + (1. / a.d3s[k] + 1. / a.d3s[km1]) / a.d3[k];
+  z2 aii = zsubr(zrmul(r, a.iwmu), sg);
+  aphi_fvm(a, k, j, i, aij, aii);
+  a.ax[id1(n1, n2, k, j, i)] = zdiv(zsub(a.bx[id1(n1, n2, k, j, i)], aij), aii);
+}
+
+// fdfd.c:606-627 / 746-766
+__device__ void aphi_row_ay(const AphiArgs &a, int k, int j, int i)
+{
+  const int n1 = a.n1, n2 = a.n2, kp1 = k + 1, km1 = k - 1, jp1 = j + 1;
+  const int ip1 = i + 1, im1 = i - 1;
+  const z2 Z = zmk(0.0, 0.0);
+  const z2 *ax = a.ax, *ay = a.ay, *az = a.az, *v = a.v;
+  z2 t1 = zsub(zdivr(zsub(az[id3(n1, n2, k, jp1, i)], az[id3(n1, n2, k, j, i)]), a.d2s[j]),
+               zdivr(zsub(ay[id2(n1, n2, kp1, j, i)], Z), a.d3s[k]));
+  z2 t2 = zsub(zdivr(zsub(az[id3(n1, n2, km1, jp1, i)], az[id3(n1, n2, km1, j, i)]), a.d2s[j]),
+               zdivr(zsub(Z, ay[id2(n1, n2, km1, j, i)]), a.d3s[km1]));
+  z2 t3 = zsub(zdivr(zsub(ay[id2(n1, n2, k, j, ip1)], Z), a.d1s[i]),
+               zdivr(zsub(ax[id1(n1, n2, k, jp1, i)], ax[id1(n1, n2, k, j, i)]), a.d2s[j]));
+  z2 t4 = zsub(zdivr(zsub(Z, ay[id2(n1, n2, k, j, im1)]), a.d1s[im1]),
+               zdivr(zsub(ax[id1(n1, n2, k, jp1, im1)], ax[id1(n1, n2, k, j, im1)]), a.d2s[j]));
+  const double sg = a.s22[id2(n1, n2, k, j, i)];
+  z2 aij = zsub(zmul(zsub(zdivr(zsub(t1, t2), a.d3[k]), zdivr(zsub(t3, t4), a.d1[i])), a.iwmu),
+                zrmul(sg, zadd(Z, zdivr(zsub(v[a_idv(a, k, jp1, i)], v[a_idv(a, k, j, i)]), a.d2s[j]))));
+  const double r = (1. / a.d3s[k] + 1. / a.d3s[km1]) / a.d3[k] + (1. / a.d1s[i] + 1. / a.d1s[im1]) / a.d1[i];
+  z2 aii = zsubr(zrmul(r, a.iwmu), sg);
+  aphi_fvm(a, k, j, i, aij, aii);
+  a.ay[id2(n1, n2, k, j, i)] = zdiv(zsub(a.by[id2(n1, n2, k, j, i)], aij), aii);
+}
+
+// fdfd.c:628-649 / 724-744
+__device__ void aphi_row_az(const AphiArgs &a, int k, int j, int i)
+{
+  const int n1 = a.n1, n2 = a.n2, kp1 = k + 1, jp1 = j + 1, jm1 = j - 1;
+  const int ip1 = i + 1, im1 = i - 1;
+  const z2 Z = zmk(0.0, 0.0);
+  const z2 *ax = a.ax, *ay = a.ay, *az = a.az, *v = a.v;
+  z2 t1 = zsub(zdivr(zsub(ax[id1(n1, n2, kp1, j, i)], ax[id1(n1, n2, k, j, i)]), a.d3s[k]),
+               zdivr(zsub(az[id3(n1, n2, k, j, ip1)], Z), a.d1s[i]));
+  z2 t2 = zsub(zdivr(zsub(ax[id1(n1, n2, kp1, j, im1)], ax[id1(n1, n2, k, j, im1)]), a.d3s[k]),
+               zdivr(zsub(Z, az[id3(n1, n2, k, j, im1)]), a.d1s[im1]));
+  z2 t3 = zsub(zdivr(zsub(az[id3(n1, n2, k, jp1, i)], Z), a.d2s[j]),
+               zdivr(zsub(ay[id2(n1, n2, kp1, j, i)], ay[id2(n1, n2, k, j, i)]), a.d3s[k]));
+  z2 t4 = zsub(zdivr(zsub(Z, az[id3(n1, n2, k, jm1, i)]), a.d2s[jm1]),
+               zdivr(zsub(ay[id2(n1, n2, kp1, jm1, i)], ay[id2(n1, n2, k, jm1, i)]), a.d3s[k]));
+  const double sg = a.s33[id3(n1, n2, k, j, i)];
+  z2 aij = zsub(zmul(zsub(zdivr(zsub(t1, t2), a.d1[i]), zdivr(zsub(t3, t4), a.d2[j])), a.iwmu),
+                zrmul(sg, zadd(Z, zdivr(zsub(v[a_idv(a, kp1, j, i)], v[a_idv(a, k, j, i)]), a.d3s[k]))));
+  const double r = (1. / a.d1s[i] + 1. / a.d1s[im1]) / a.d1[i] + (1. / a.d2s[j] + 1. / a.d2s[jm1]) / a.d2[j];
+  z2 aii = zsubr(zrmul(r, a.iwmu), sg);
+  aphi_fvm(a, k, j, i, aij, aii);
+  a.az[id3(n1, n2, k, j, i)] = zdiv(zsub(a.bz[id3(n1, n2, k, j, i)], aij), aii);
+}
+
+// fdfd.c:650-722; the conductivity behind the cell (t2, t4, t6) is read from
+// the full grid's plane k as the reference does (emf->sigma*[k][j][i])
+__device__ void aphi_row_v(const AphiArgs &a, int k, int j, int i)
+{
+  const int n1 = a.n1, n2 = a.n2, kp1 = k + 1, km1 = k - 1, jp1 = j + 1, jm1 = j - 1;
+  const int ip1 = i + 1, im1 = i - 1;
+  const z2 Z = zmk(0.0, 0.0);
+  const z2 *ax = a.ax, *ay = a.ay, *az = a.az, *v = a.v;
+  z2 t1 = zrmul(a.s11[id1(n1, n2, k, j, i)],
+                zadd(ax[id1(n1, n2, k, j, i)], zdivr(zsub(v[a_idv(a, k, j, ip1)], Z), a.d1s[i])));
+  z2 t2 = zrmul(a.S11[id1(n1, n2, k, j, im1)],
+                zadd(ax[id1(n1, n2, k, j, im1)], zdivr(zsub(Z, v[a_idv(a, k, j, im1)]), a.d1s[im1])));
+  z2 t3 = zrmul(a.s22[id2(n1, n2, k, j, i)],
+                zadd(ay[id2(n1, n2, k, j, i)], zdivr(zsub(v[a_idv(a, k, jp1, i)], Z), a.d2s[j])));
+  z2 t4 = zrmul(a.S22[id2(n1, n2, k, jm1, i)],
+                zadd(ay[id2(n1, n2, k, jm1, i)], zdivr(zsub(Z, v[a_idv(a, k, jm1, i)]), a.d2s[jm1])));
+  z2 t5 = zrmul(a.s33[id3(n1, n2, k, j, i)],
+                zadd(az[id3(n1, n2, k, j, i)], zdivr(zsub(v[a_idv(a, kp1, j, i)], Z), a.d3s[k])));
+  z2 t6 = zrmul(a.S33[id3(n1, n2, km1, j, i)],
+                zadd(az[id3(n1, n2, km1, j, i)], zdivr(zsub(Z, v[a_idv(a, km1, j, i)]), a.d3s[km1])));
+  z2 aij = zadd(zadd(zdivr(zsub(t1, t2), a.d1[i]), zdivr(zsub(t3, t4), a.d2[j])), zdivr(zsub(t5, t6), a.d3[k]));
+  const double r = -((1. / a.d1s[i] + 1. / a.d1s[im1]) / a.d1[i] + (1. / a.d2s[j] + 1. / a.d2s[jm1]) / a.d2[j] +
+                     (1. / a.d3s[k] + 1. / a.d3s[km1]) / a.d3[k]);
+  z2 aii = zmk(r, 0.0);
+  aphi_fvm(a, k, j, i, aij, aii);
+  a.v[a_idv(a, k, j, i)] = zdiv(zsub(a.bv[a_idv(a, k, j, i)], aij), aii);
+}
+
+// One diagonal slow + fast = d of every 2-D slice of component C: thread
+// (slice, slow) takes fast = d - slow when it is in range.  Slices: C=0 i,
+// C=1 j, C=2 k; (slow, fast): (k, j), (k, i), (j, i).  lo*/hi* are the
+// inclusive row ranges of fdfd.c's loops.
+template <int C>
+__global__ void k_aphi_diag(AphiArgs a, int d, int nslice, int lo_s, int hi_s, int lo_f, int hi_f)
+{
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int slice = (int)(t % nslice), slow = lo_s + (int)(t / nslice), fast = d - slow;
+  if (slow > hi_s || fast < lo_f || fast > hi_f) return;
+  if (C == 0) aphi_row_ax(a, slow, fast, slice);
+  else if (C == 1) aphi_row_ay(a, slow, slice, fast);
+  else aphi_row_az(a, slice, slow, fast);
+}
+
+// One plane k + j + i = d of phi's rows (k, j, i in [1, n-1]).
+__global__ void k_aphi_vplane(AphiArgs a, int d)
+{
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int nj = a.n2 - 1;
+  const int j = 1 + (int)(t % nj), k = 1 + (int)(t / nj), i = d - k - j;
+  if (k > a.n3 - 1 || i < 1 || i > a.n1 - 1) return;
+  aphi_row_v(a, k, j, i);
+}
+
+// y = A + grad(phi) in place (fdfd.c:793-820)
+__global__ void k_aphi_grad(AphiArgs a, long lE)
+{
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= lE) return;
+  const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
+  int i, j, k;
+  if (e < a.lEx) {
+    i = (int)(e % n1); j = (int)((e / n1) % (n2 + 1)); k = (int)(e / ((long)n1 * (n2 + 1)));
+    if (k >= 1 && k < n3 && j >= 1 && j < n2)
+      a.ax[e] = zadd(a.ax[e], zdivr(zsub(a.v[a_idv(a, k, j, i + 1)], a.v[a_idv(a, k, j, i)]), a.d1s[i]));
+  } else if (e < a.lEx + a.lEy) {
+    const long q = e - a.lEx;
+    i = (int)(q % (n1 + 1)); j = (int)((q / (n1 + 1)) % n2); k = (int)(q / ((long)(n1 + 1) * n2));
+    if (k >= 1 && k < n3 && i >= 1 && i < n1)
+      a.ax[e] = zadd(a.ax[e], zdivr(zsub(a.v[a_idv(a, k, j + 1, i)], a.v[a_idv(a, k, j, i)]), a.d2s[j]));
+  } else {
+    const long q = e - a.lEx - a.lEy;
+    i = (int)(q % (n1 + 1)); j = (int)((q / (n1 + 1)) % (n2 + 1)); k = (int)(q / ((long)(n1 + 1) * (n2 + 1)));
+    if (j >= 1 && j < n2 && i >= 1 && i < n1)
+      a.ax[e] = zadd(a.ax[e], zdivr(zsub(a.v[a_idv(a, k + 1, j, i)], a.v[a_idv(a, k, j, i)]), a.d3s[k]));
+  }
+}
+
+template <int C>
+static int aphi_sweep(const AphiArgs &a, bool fwd, hipStream_t s)
+{
+  const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
+  int nslice, lo_s, hi_s, lo_f, hi_f;
+  if (C == 0) { nslice = n1; lo_s = 1; hi_s = n3 - 1; lo_f = 1; hi_f = n2 - 1; }
+  else if (C == 1) { nslice = n2; lo_s = 1; hi_s = n3 - 1; lo_f = 1; hi_f = n1 - 1; }
+  else { nslice = n3; lo_s = 1; hi_s = n2 - 1; lo_f = 1; hi_f = n1 - 1; }
+  if (hi_s < lo_s || hi_f < lo_f || nslice <= 0) return 0;
+  const long nt = (long)nslice * (hi_s - lo_s + 1);
+  const unsigned nb = (unsigned)((nt + 255) / 256);
+  const int d0 = lo_s + lo_f, d1 = hi_s + hi_f;
+  for (int q = 0; q <= d1 - d0; q++) k_aphi_diag<C><<<nb, 256, 0, s>>>(a, fwd ? d0 + q : d1 - q, nslice, lo_s, hi_s, lo_f, hi_f);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+static int aphi_vsweep(const AphiArgs &a, bool fwd, hipStream_t s)
+{
+  if (a.n1 < 2 || a.n2 < 2 || a.n3 < 2) return 0;
+  const long nt = (long)(a.n3 - 1) * (a.n2 - 1);
+  const unsigned nb = (unsigned)((nt + 255) / 256);
+  const int d0 = 3, d1 = (a.n1 - 1) + (a.n2 - 1) + (a.n3 - 1);
+  for (int q = 0; q <= d1 - d0; q++) k_aphi_vplane<<<nb, 256, 0, s>>>(a, fwd ? d0 + q : d1 - q);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+static int run_aphi(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStream_t s)
+{
+  const int n1 = m->n1, n2 = m->n2, n3 = m->n3, ks = m->kofs;
+  const long lv = (long)(n1 + 1) * (n2 + 1) * (n3 + 1);
+  int rc;
+  if (!m->aphi_v && ((rc = dalloc(m, &m->aphi_v, lv)) || (rc = dalloc(m, &m->aphi_bv, lv)))) return rc;
+  AphiArgs a;
+  a.n1 = n1; a.n2 = n2; a.n3 = n3; a.fvm = m->fvm; a.lEx = m->lEx; a.lEy = m->lEy;
+  a.d1 = m->d1; a.d2 = m->d2; a.d3 = m->d3 + ks; a.d1s = m->d1s; a.d2s = m->d2s; a.d3s = m->d3s + ks;
+  a.s11 = m->s11 + (long)n1 * (n2 + 1) * ks;
+  a.s22 = m->s22 + (long)(n1 + 1) * n2 * ks;
+  a.s33 = m->s33 + (long)(n1 + 1) * (n2 + 1) * ks;
+  a.S11 = m->s11; a.S22 = m->s22; a.S33 = m->s33;
+  a.iwmu = zmk(f->iwmu_re, f->iwmu_im);
+  a.iomu = zmk(f->iomu_re, f->iomu_im);
+  a.bx = x; a.by = x + m->lEx; a.bz = x + m->lEx + m->lEy;
+  a.ax = y; a.ay = y + m->lEx; a.az = y + m->lEx + m->lEy;
+  a.v = m->aphi_v; a.bv = m->aphi_bv;
+  const long ne = lv > m->lE ? lv : m->lE;
+  k_aphi_init<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(a, m->lE, lv);
+  HIP_OK(hipGetLastError());
+  if ((rc = aphi_sweep<0>(a, true, s)) || (rc = aphi_sweep<1>(a, true, s)) || (rc = aphi_sweep<2>(a, true, s)) ||
+      (rc = aphi_vsweep(a, true, s)) || (rc = aphi_vsweep(a, false, s)) || (rc = aphi_sweep<2>(a, false, s)) ||
+      (rc = aphi_sweep<1>(a, false, s)) || (rc = aphi_sweep<0>(a, false, s)))
+    return rc;
+  k_aphi_grad<<<(unsigned)((m->lE + 255) / 256), 256, 0, s>>>(a, m->lE);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int csem_op_aphi(void *, csem_model *m, const csem_freq *f, const void *x, void *y, void *stream)
+{
+  if (!m) return csem_fail(-4, "csem_op_aphi: no model");
+  return run_aphi(m, f, (const z2 *)x, (z2 *)y, S(stream));
+}
+
+// Krylov work vectors, scalars and partial sums: the model's when the solve
+// has one, otherwise the call's own (a solve over host operators only).
+struct KrylovWork {
+  z2 *kv;
+  double *scal, *partials;
+  void *own[3];
+};
+
+static int kwork_get(csem_model *m, long n, int nv, KrylovWork *w)
+{
+  memset(w, 0, sizeof *w);
+  if (m) {
+    int rc = ensure_kv(m, nv);
+    if (rc) return rc;
+    w->kv = m->kv; w->scal = m->scal; w->partials = m->partials;
+    return 0;
+  }
+  const long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
+  if (dev_malloc(&w->own[0], (long)nv * n * (long)sizeof(z2)) != hipSuccess ||
+      dev_malloc(&w->own[1], 64 * sizeof(double)) != hipSuccess ||
+      dev_malloc(&w->own[2], nch * 4 * (long)sizeof(double)) != hipSuccess)
+    return csem_fail(-2, "Krylov work vectors (%ld x %d)", n, nv);
+  HIP_OK(dev_memset(w->own[1], 0, 64 * sizeof(double)));
+  w->kv = (z2 *)w->own[0]; w->scal = (double *)w->own[1]; w->partials = (double *)w->own[2];
+  return 0;
+}
+
+static void kwork_release(KrylovWork *w)
+{
+  for (int q = 0; q < 3; q++)
+    if (w->own[q]) dev_free(w->own[q]);
+}
+
+// Flexible right-preconditioned BiCGStab (solver.c:93-178) or, with M =
+// NULL, BiCGStab (solver.c:16-87), all vectors and scalars resident.  The
+// operators are called through their pointers; the library's own run as
+// device kernels, a caller's host operator through host copies.
+static int bicgstab_run(csem_model *m, const csem_freq *f, long n, z2 *x, const z2 *b, int niter, double tol,
+                        csem_op A, void *actx, csem_op M, void *mctx, double *hist, hipStream_t s,
+                        const KrylovWork &w)
+{
+  const long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
+  int rc;
+  z2 *r = w.kv, *r0 = r + n, *p = r0 + n, *pp = p + n, *v = pp + n, *sv = v + n, *t = sv + n;
+  double *sc = w.scal, *pa = w.partials;
   const unsigned ge = (unsigned)((n + 255) / 256);
   double hs[16];
   HIP_OK(hipMemcpyAsync(sc + SC_TOL, &tol, sizeof(double), hipMemcpyHostToDevice, s));
-  if ((rc = run_apply(m, f, x, v, s))) return rc;
+  if ((rc = A(actx, m, f, x, v, s))) return rc;
   k_bicg_init<<<(unsigned)nch, RED_T, 0, s>>>(n, b, v, r, p, r0, pa);
   k_bicg_fin_init<<<1, RED_T, 0, s>>>(pa, serial_dots(n, r, r0, r, r, sc, s, nch), sc);
   HIP_OK(hipMemcpyAsync(hs, sc, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -809,26 +1126,26 @@ extern "C" int csem_bicgstab(csem_model *m, const csem_freq *f, void *xv, const 
   for (iter = 0; iter < niter; ++iter) {
     if (hist) hist[iter] = relres;
     const z2 *pin = p;
-    if (precond) {
-      if ((rc = run_lusgs(m, f, p, pp, s))) return rc;
+    if (M) {
+      if ((rc = M(mctx, m, f, p, pp, s))) return rc;
       pin = pp;
     }
-    if ((rc = run_apply(m, f, pin, v, s))) return rc;
+    if ((rc = A(actx, m, f, pin, v, s))) return rc;
     if (!g_red_serial && (rc = dot_partial_launch(n, v, r0, NULL, NULL, pa, s))) return rc;
     k_bicg_fin_alpha<<<1, RED_T, 0, s>>>(pa, serial_dots(n, v, r0, NULL, NULL, sc, s, nch), sc);
     KT(KT_BICG_S, s, 48.0 * n, (k_bicg_s<<<ge, 256, 0, s>>>(n, r, v, sv, sc)));
     const z2 *xb;
-    if (precond) {
-      if ((rc = run_lusgs(m, f, sv, r, s))) return rc;
-      if ((rc = run_apply(m, f, r, t, s))) return rc;
+    if (M) {
+      if ((rc = M(mctx, m, f, sv, r, s))) return rc;
+      if ((rc = A(actx, m, f, r, t, s))) return rc;
       if (!g_red_serial && (rc = dot_partial_launch(n, sv, t, t, t, pa, s))) return rc;
       xb = r;
     } else {
-      if ((rc = run_apply(m, f, sv, t, s))) return rc;
+      if ((rc = A(actx, m, f, sv, t, s))) return rc;
       if (!g_red_serial && (rc = dot_partial_launch(n, t, sv, t, t, pa, s))) return rc;
       xb = sv;
     }
-    const long nom = precond ? serial_dots(n, sv, t, t, t, sc, s, nch) : serial_dots(n, t, sv, t, t, sc, s, nch);
+    const long nom = M ? serial_dots(n, sv, t, t, t, sc, s, nch) : serial_dots(n, t, sv, t, t, sc, s, nch);
     k_bicg_fin_omega<<<1, RED_T, 0, s>>>(pa, nom, sc);
     KT(KT_BICG_XR, s, 128.0 * n, (k_bicg_xr<<<(unsigned)nch, RED_T, 0, s>>>(n, x, pin, xb, sv, t, r, r0, sc, pa)));
     k_bicg_fin_conv<<<1, RED_T, 0, s>>>(pa, serial_dots(n, r, r, r, r0, sc, s, nch), sc);
@@ -845,6 +1162,30 @@ extern "C" int csem_bicgstab(csem_model *m, const csem_freq *f, void *xv, const 
   }
   HIP_OK(hipGetLastError());
   return iter;
+}
+
+extern "C" int csem_bicgstab_op(csem_model *m, const csem_freq *f, long n, void *xv, const void *bv, int niter,
+                                double tol, csem_op A, void *actx, csem_op M, void *mctx, double *hist,
+                                void *stream)
+{
+  if (!A) return csem_fail(-3, "csem_bicgstab_op: no operator");
+  if (m && n != m->lE) return csem_fail(-3, "csem_bicgstab_op: n=%ld, the model's vectors hold %ld", n, m->lE);
+  if (n <= 0) return csem_fail(-3, "csem_bicgstab_op: n=%ld", n);
+  KrylovWork w;
+  int rc = kwork_get(m, n, 7, &w);
+  if (!rc) rc = bicgstab_run(m, f, n, (z2 *)xv, (const z2 *)bv, niter, tol, A, actx, M, mctx, hist, S(stream), w);
+  if (!m) {
+    (void)hipStreamSynchronize(S(stream));
+    kwork_release(&w);
+  }
+  return rc;
+}
+
+extern "C" int csem_bicgstab(csem_model *m, const csem_freq *f, void *xv, const void *bv, int niter,
+                             double tol, int precond, double *hist, void *stream)
+{
+  return csem_bicgstab_op(m, f, m->lE, xv, bv, niter, tol, csem_op_apply, NULL, precond ? csem_op_lusgs : NULL,
+                          NULL, hist, stream);
 }
 
 // ------------------------------------------------- vector primitives

@@ -78,6 +78,7 @@ struct csem_model {
   double *partials, *scal;
   long nch;
   z2 *kv;  // Krylov work vectors (lazily allocated)
+  z2 *aphi_v, *aphi_bv;  // A-phi potential and its right-hand side (lazily allocated)
   int nkv;
   long bytes;
   std::vector<void *> allocs;

@@ -36,6 +36,9 @@ static struct {
   csem_model *model;
   void *dx, *dy, *dh, *dg1, *dg2, *dred, *dhf;
   long nvec;
+  void *px, *py; /* staging of gmres_apply / aphi_apply, apart from the operators' (a caller's
+                    op_t inside the inner GMRES may call fdfd_apply, which stages through dx, dy) */
+  long npvec;
   int last_iters;
   /* airwave host tables (airbc_init) */
   double *twiddle;
@@ -176,7 +179,6 @@ static void ensure_staging(long n)
 void csem_fdfd_prepare(emf_t *emf)
 {
   G.emf = emf;
-  if (emf->preco == 3) err("preco=3 (A-phi preconditioner) is not provided by this library");
   if (emf->airbc) {
     emf->xx = emf->yy = NULL; /* LU-SGS scratch lives on the device */
     emf->n3 = emf->nz + emf->nb;
@@ -232,6 +234,7 @@ void fdfd_free(emf_t *emf)
   if (emf->airbc) airbc_free(emf);
   if (G.model) csem_model_destroy(G.model);
   G.model = NULL;
+  G.emf = NULL;
   if (G.dx) csem_free(G.dx);
   if (G.dy) csem_free(G.dy);
   if (G.dh) csem_free(G.dh);
@@ -239,8 +242,10 @@ void fdfd_free(emf_t *emf)
   if (G.dg2) csem_free(G.dg2);
   if (G.dred) csem_free(G.dred);
   if (G.dhf) csem_free(G.dhf);
-  G.dx = G.dy = G.dh = G.dg1 = G.dg2 = G.dred = G.dhf = NULL;
-  G.nvec = 0;
+  if (G.px) csem_free(G.px);
+  if (G.py) csem_free(G.py);
+  G.dx = G.dy = G.dh = G.dg1 = G.dg2 = G.dred = G.dhf = G.px = G.py = NULL;
+  G.nvec = G.npvec = 0;
 }
 
 /* iomu = emf->I_omega_mu0 as the caller set it (main.c:95), iwmu = 1/iomu
@@ -353,78 +358,21 @@ static void log_history(const char *file, const char *tag, const double *hist, i
   if (conv_iter >= 0) printf("converged at iter=%d\n", conv_iter);
 }
 
-static void device_bicgstab(int n, csem_complex *x, csem_complex *b, int niter, double tol, int verb,
-                            int precond)
-{
-  void *dx = NULL, *db = NULL;
-  double *hist = (double *)calloc(niter + 2, sizeof(double));
-  csem_freq f = freq_of(G.emf);
-  int it;
-  if (csem_malloc(&dx, (long)n * 16) || csem_malloc(&db, (long)n * 16)) {
-    free(hist);
-    die_dev("solver allocation", -2);
-    return;
-  }
-  if (csem_memcpy_h2d(dx, x, (long)n * 16, NULL) || csem_memcpy_h2d(db, b, (long)n * 16, NULL)) {
-    die_dev("solver upload", -1);
-    goto out;
-  }
-  it = csem_bicgstab(G.model, &f, dx, db, niter, tol, precond, hist, NULL);
-  if (it < 0) {
-    die_dev("csem_bicgstab", it);
-    goto out;
-  }
-  if (csem_memcpy_d2h(x, dx, (long)n * 16, NULL)) {
-    die_dev("solver download", -1);
-    goto out;
-  }
-  G.last_iters = it;
-  if (verb)
-    log_history(precond ? "iterate_fbicgstab.txt" : "iterate_bicgstab.txt", precond ? "f-bicgstab" : "bicgstab",
-                hist, (it < niter ? it + 1 : niter), it < niter ? it : -1);
-  csem_set_status(0);
-out:
-  csem_free(dx);
-  csem_free(db);
-  free(hist);
-}
-
-void fbicgstab(int n, csem_complex *x, csem_complex *b, op_t Aop, op_t invMop, int niter, double tol, int verb)
-{
-  if (!need_model("fbicgstab")) return;
-  if (Aop != fdfd_apply || invMop != lusgs_apply) {
-    csem_set_status(-5);
-    err("fbicgstab: only the device operators (fdfd_apply, lusgs_apply) are provided");
-    return;
-  }
-  device_bicgstab(n, x, b, niter, tol, verb, 1);
-}
-
-void bicgstab(int n, csem_complex *x, csem_complex *b, op_t Aop, int niter, double tol, int verb)
-{
-  if (!need_model("bicgstab")) return;
-  if (Aop != fdfd_apply) {
-    csem_set_status(-5);
-    err("bicgstab: only the device operator fdfd_apply is provided");
-    return;
-  }
-  device_bicgstab(n, x, b, niter, tol, verb, 0);
-}
-
 /* Restarted (flexible) GMRES with the vectors on the device and the small
  * Hessenberg / Givens recurrence on the host, in the arithmetic of
- * solver.c:403-512 (fgmres) and solver.c:181-285 (gmres). */
-static int gmres_core(csem_model *model, const csem_freq *fq, void *x, const void *b, int nouter, double tol,
-                      int m, int precond, int flexible, double *hist)
+ * solver.c:403-512 (fgmres) and solver.c:181-285 (gmres).  A and M are
+ * called through their pointers (M = NULL: Z = V). */
+static int gmres_core(csem_model *model, const csem_freq *fq, long n, void *x, const void *b, int nouter,
+                      double tol, int m, csem_op A, void *actx, csem_op M, void *mctx, int flexible, double *hist)
 {
-  const long n = csem_model_len(model), nb = n * 16;
+  const long nb = n * 16;
   const csem_freq f = *fq;
   void *w = NULL, *r = NULL, *V = NULL, *Z = NULL, *hd = NULL;
-  csem_complex *h = (csem_complex *)calloc((size_t)(m + 1) * m, sizeof(csem_complex));
-  csem_complex *g = (csem_complex *)calloc(m + 1, sizeof(csem_complex));
-  csem_complex *y = (csem_complex *)calloc(m + 1, sizeof(csem_complex));
-  csem_complex *c = (csem_complex *)calloc(m + 1, sizeof(csem_complex));
-  double *s = (double *)calloc(m + 1, sizeof(double));
+  csem_complex *h = (csem_complex *)calloc((size_t)(m + 1) * (m > 0 ? m : 1), sizeof(csem_complex));
+  csem_complex *g = (csem_complex *)calloc(m + 2, sizeof(csem_complex));
+  csem_complex *y = (csem_complex *)calloc(m + 2, sizeof(csem_complex));
+  csem_complex *c = (csem_complex *)calloc(m + 2, sizeof(csem_complex));
+  double *s = (double *)calloc(m + 2, sizeof(double));
   double dv[2 * 64 + 2];
   double beta, tmp, r0 = 1, relres;
   csem_complex ss;
@@ -439,15 +387,15 @@ static int gmres_core(csem_model *model, const csem_freq *fq, void *x, const voi
   }
   CK(csem_malloc(&w, nb)); CK(csem_malloc(&r, nb));
   CK(csem_malloc(&V, nb * (m + 1)));
-  if (precond) CK(csem_malloc(&Z, nb * m));
+  if (M) CK(csem_malloc(&Z, nb * m));
   CK(csem_malloc(&hd, (long)(2 * 64 + 2) * sizeof(double)));
 #define VJ(k) ((char *)V + (long)(k) * nb)
-#define ZJ(k) (precond ? (char *)Z + (long)(k) * nb : VJ(k))
+#define ZJ(k) (M ? (char *)Z + (long)(k) * nb : VJ(k))
 /* storage element (column j, row i) of the Hessenberg: fgmres keeps
  * h[j][i] (solver.c:412), gmres h[i][j] (solver.c:190) -- same numbers */
 #define HH(row, col) h[(long)(col) * (m + 1) + (row)]
   for (iter = 0; iter < nouter; iter++) {
-    CK(csem_apply(model, &f, x, w, NULL));
+    CK(A(actx, model, &f, x, w, NULL));
     CK(csem_vec_sub(n, b, w, r, NULL));
     CK(csem_dot(n, r, r, hd, NULL));
     CK(csem_memcpy_d2h(dv, hd, 16, NULL));
@@ -462,8 +410,8 @@ static int gmres_core(csem_model *model, const csem_freq *fq, void *x, const voi
       if (hist) hist[0] = flexible ? 1.0 : beta / r0;
     }
     for (j = 0; j < mcur; j++) {
-      if (precond) CK(csem_lusgs(model, &f, VJ(j), ZJ(j), NULL));
-      CK(csem_apply(model, &f, ZJ(j), w, NULL));
+      if (M) CK(M(mctx, model, &f, VJ(j), ZJ(j), NULL));
+      CK(A(actx, model, &f, ZJ(j), w, NULL));
       for (i = 0; i <= j; i++) { /* modified Gram-Schmidt, solver.c:445-448 */
         CK(csem_dot(n, w, VJ(i), (char *)hd + 16 * i, NULL));
         CK(csem_vec_subscaled_dev(n, w, (char *)hd + 16 * i, VJ(i), NULL));
@@ -535,7 +483,8 @@ int csem_fgmres(csem_model *m, const csem_freq *f, void *x, const void *b, int n
                 int precond, double *hist, void *stream)
 {
   (void)stream;
-  int rc = gmres_core(m, f, x, b, nouter, tol, mrestart, precond, 1, hist);
+  int rc = gmres_core(m, f, csem_model_len(m), x, b, nouter, tol, mrestart, csem_op_apply, NULL,
+                      precond ? csem_op_lusgs : NULL, NULL, 1, hist);
   if (rc == -3) return csem_fail(-3, "restart length %d outside [1, 64]", mrestart);
   return rc;
 }
@@ -544,72 +493,367 @@ int csem_gmres(csem_model *m, const csem_freq *f, void *x, const void *b, int no
                double *hist, void *stream)
 {
   (void)stream;
-  int rc = gmres_core(m, f, x, b, nouter, tol, mrestart, 0, 0, hist);
+  int rc = gmres_core(m, f, csem_model_len(m), x, b, nouter, tol, mrestart, csem_op_apply, NULL, NULL, NULL, 0,
+                      hist);
   if (rc == -3) return csem_fail(-3, "restart length %d outside [1, 64]", mrestart);
   return rc;
 }
 
-static void host_gmres(int n, csem_complex *xh, csem_complex *bh, int nouter, double tol, int m, int verb,
-                       int flexible)
+int csem_gmres_op(csem_model *m, const csem_freq *f, long n, void *x, const void *b, int nouter, double tol,
+                  int mrestart, csem_op A, void *actx, csem_op M, void *mctx, double *hist, void *stream)
 {
+  (void)stream;
+  if (!A) return csem_fail(-3, "csem_gmres_op: no operator");
+  if (m && n != csem_model_len(m))
+    return csem_fail(-3, "csem_gmres_op: n=%ld, the model's vectors hold %ld", n, csem_model_len(m));
+  int rc = gmres_core(m, f, n, x, b, nouter, tol, mrestart, A, actx, M, mctx, M != NULL, hist);
+  if (rc == -3) return csem_fail(-3, "restart length %d outside [1, 64]", mrestart);
+  return rc;
+}
+
+/* ------------------------------------------------ operators as arguments */
+
+/* A caller's host operator on device vectors: download x, call, upload y.
+ * The host entry status is cleared first so that an error the operator
+ * reports through csem_last_status() is seen here. */
+int csem_op_host(void *ctx, csem_model *m, const csem_freq *f, const void *x, void *y, void *stream)
+{
+  csem_host_op *h = (csem_host_op *)ctx;
+  const long bytes = (long)h->n * 16;
+  int rc;
+  (void)m; (void)f;
+  if ((rc = csem_memcpy_d2h(h->hx, x, bytes, stream))) return rc;
+  csem_set_status(0);
+  h->fn(h->n, h->hx, h->hy);
+  if ((rc = csem_last_status()) < 0) return rc;
+  return csem_memcpy_h2d(y, h->hy, bytes, stream);
+}
+
+/* gmres_apply (gmres_init_free.c:58-137): y = GMRES(m) solution of A y = x
+ * from y = 0 after exactly m Arnoldi steps, modified Gram-Schmidt, complex
+ * Givens rotations (c complex, s real), back substitution, y = V_m z.  The
+ * reference applies no breakdown test and no residual test here. */
+int csem_op_gmres(void *ctx, csem_model *model, const csem_freq *f, const void *bv, void *xv, void *stream)
+{
+  csem_gmres_prec *P = (csem_gmres_prec *)ctx;
+  const int m = P->m;
+  const long n = P->n, nb = n * 16;
+  csem_complex h[65 * 64], g[66], y[66], c[66], ss;
+  double s[66], dv[2 * 64 + 2], beta, tmp;
+  int i, j, rc;
+  (void)stream;
+  if (m < 1 || m > 64) return csem_fail(-3, "gmres_apply: inner dimension %d outside [1, 64]", m);
+  if (!P->work && (rc = csem_malloc(&P->work, nb * (m + 2) + (long)(2 * 64 + 2) * sizeof(double)))) return rc;
+  char *V = (char *)P->work, *w = V + nb * (m + 1), *hd = w + nb;
+#define VJ(k) (V + (long)(k) * nb)
+#define HH(row, col) h[(long)(col) * (m + 1) + (row)]
+  if ((rc = csem_memset(xv, 0, nb, NULL))) return rc;
+  if ((rc = csem_dot(n, bv, bv, hd, NULL)) || (rc = csem_memcpy_d2h(dv, hd, 16, NULL))) return rc;
+  beta = sqrt(dv[0]); /* r = b (x = 0), gmres_init_free.c:80-82 */
+  if (beta == 0.0) return 0;
+  if ((rc = csem_vec_divr(n, bv, beta, VJ(0), NULL))) return rc;
+  memset(g, 0, sizeof g);
+  g[0] = beta;
+  memset(h, 0, sizeof h);
+  for (j = 0; j < m; j++) {
+    if ((rc = P->A(P->actx, model, f, VJ(j), w, NULL))) return rc;
+    for (i = 0; i <= j; i++) {
+      if ((rc = csem_dot(n, w, VJ(i), hd + 16 * i, NULL))) return rc;
+      if ((rc = csem_vec_subscaled_dev(n, w, hd + 16 * i, VJ(i), NULL))) return rc;
+    }
+    if ((rc = csem_dot(n, w, w, hd + 16 * (j + 1), NULL))) return rc;
+    if ((rc = csem_memcpy_d2h(dv, hd, 16 * (j + 2), NULL))) return rc;
+    for (i = 0; i <= j; i++) HH(i, j) = dv[2 * i] + I * dv[2 * i + 1];
+    HH(j + 1, j) = sqrt(dv[2 * (j + 1)]);
+    if ((rc = csem_vec_divz(n, w, creal(HH(j + 1, j)), cimag(HH(j + 1, j)), VJ(j + 1), NULL))) return rc;
+    for (i = 0; i < j; i++) {
+      ss = conj(c[i]) * HH(i, j) + s[i] * HH(i + 1, j);
+      HH(i + 1, j) = -s[i] * HH(i, j) + c[i] * HH(i + 1, j);
+      HH(i, j) = ss;
+    }
+    tmp = sqrt(creal(HH(j, j) * conj(HH(j, j)) + HH(j + 1, j) * conj(HH(j + 1, j))));
+    s[j] = creal(HH(j + 1, j)) / tmp;
+    c[j] = HH(j, j) / tmp;
+    HH(j, j) = conj(c[j]) * HH(j, j) + s[j] * HH(j + 1, j);
+    HH(j + 1, j) = 0.;
+    g[j + 1] = -s[j] * g[j];
+    g[j] = conj(c[j]) * g[j];
+  }
+  y[m - 1] = g[m - 1] / HH(m - 1, m - 1);
+  for (i = m - 2; i >= 0; i--) {
+    y[i] = g[i];
+    for (j = i + 1; j < m; j++) y[i] -= HH(i, j) * y[j];
+    y[i] /= HH(i, i);
+  }
+  for (j = 0; j < m; j++)
+    if ((rc = csem_vec_addscaled(n, xv, creal(y[j]), cimag(y[j]), VJ(j), NULL))) return rc;
+#undef VJ
+#undef HH
+  return 0;
+}
+
+void csem_gmres_prec_release(csem_gmres_prec *p)
+{
+  if (p && p->work) csem_free(p->work);
+  if (p) p->work = NULL;
+}
+
+/* An op_t of the reference's shape, as an operator on device vectors: the
+ * library's own entry points map to their device kernels, any other
+ * function is called on host copies. */
+typedef struct {
+  csem_op op;
+  void *ctx;
+  csem_host_op host;
+} opmap_t;
+
+static struct {
+  csem_gmres_prec prec;
+  opmap_t inner;
+  int on;
+} GP; /* gmres_init / gmres_free state (gmres_init_free.c:27) */
+
+static int map_op(op_t fn, int n, opmap_t *o)
+{
+  memset(o, 0, sizeof *o);
+  if (!fn) return 0;
+  if (fn == fdfd_apply) o->op = csem_op_apply;
+  else if (fn == lusgs_apply) o->op = csem_op_lusgs;
+  else if (fn == aphi_apply) o->op = csem_op_aphi;
+  else if (fn == gmres_apply) {
+    if (!GP.on) return csem_fail(-4, "gmres_apply used before gmres_init");
+    o->op = csem_op_gmres;
+    o->ctx = &GP.prec;
+  } else {
+    o->op = csem_op_host;
+    o->host.fn = (void (*)(int, void *, void *))fn;
+    o->host.n = n;
+    o->host.hx = (double *)malloc((size_t)n * 16);
+    o->host.hy = (double *)malloc((size_t)n * 16);
+    if (!o->host.hx || !o->host.hy) return csem_fail(-2, "host staging of %d values", n);
+    o->ctx = &o->host;
+  }
+  return 0;
+}
+
+static void unmap_op(opmap_t *o)
+{
+  free(o->host.hx);
+  free(o->host.hy);
+  o->host.hx = o->host.hy = NULL;
+}
+
+/* does the operator run on the active model's device kernels */
+static int op_on_model(const opmap_t *o)
+{
+  if (o->op == csem_op_apply || o->op == csem_op_lusgs || o->op == csem_op_aphi) return 1;
+  if (o->op == csem_op_gmres) return op_on_model(&GP.inner);
+  return 0;
+}
+
+enum { K_BICGSTAB, K_FBICGSTAB, K_GMRES, K_FGMRES };
+
+/* bicgstab / fbicgstab / gmres / fgmres (solver.c:16-512): x and b are
+ * host arrays; the solve runs on the device with one upload of each and
+ * one download of x.  Operators on the active model run as its kernels. */
+static void krylov(const char *who, int kind, int n, csem_complex *xh, csem_complex *bh, op_t Aop, op_t invMop,
+                   int niter, double tol, int m, int verb)
+{
+  opmap_t A, M;
   void *x = NULL, *b = NULL;
-  double *hist = (double *)calloc((size_t)nouter * m + 2, sizeof(double));
-  csem_freq f = freq_of(G.emf);
-  int steps, i;
-  if (csem_malloc(&x, (long)n * 16) || csem_malloc(&b, (long)n * 16) ||
-      csem_memcpy_h2d(x, xh, (long)n * 16, NULL) || csem_memcpy_h2d(b, bh, (long)n * 16, NULL)) {
-    die_dev("gmres staging", -2);
+  const int gm = kind == K_GMRES || kind == K_FGMRES;
+  const int nh = gm ? niter * m + 2 : niter + 2;
+  double *hist = (double *)calloc(nh > 2 ? nh : 2, sizeof(double));
+  csem_model *model = NULL;
+  csem_freq f;
+  int it, i, rc;
+  memset(&f, 0, sizeof f);
+  memset(&A, 0, sizeof A);
+  memset(&M, 0, sizeof M);
+  if (!Aop || ((kind == K_FBICGSTAB || kind == K_FGMRES) && !invMop)) {
+    csem_set_status(-3);
+    err("%s: no %s operator", who, Aop ? "preconditioner" : "system");
     goto out;
   }
-  steps = flexible ? csem_fgmres(G.model, &f, x, b, nouter, tol, m, 1, hist, NULL)
-                   : csem_gmres(G.model, &f, x, b, nouter, tol, m, hist, NULL);
-  if (steps < 0) {
-    die_dev(flexible ? "csem_fgmres" : "csem_gmres", steps);
+  if ((rc = map_op(Aop, n, &A)) || (rc = map_op(kind == K_FBICGSTAB || kind == K_FGMRES ? invMop : NULL, n, &M))) {
+    die_dev(who, rc);
     goto out;
   }
-  if (csem_memcpy_d2h(xh, x, (long)n * 16, NULL)) {
-    die_dev("gmres download", -1);
+  if (op_on_model(&A) || (M.op && op_on_model(&M))) {
+    if (!need_model(who)) goto out;
+    if (n != csem_model_len(G.model)) {
+      csem_set_status(-3);
+      err("%s: n=%d but the operator's vectors hold %ld values", who, n, csem_model_len(G.model));
+      goto out;
+    }
+    model = G.model;
+  }
+  if (G.emf) f = freq_of(G.emf);
+  if ((rc = csem_malloc(&x, (long)n * 16)) || (rc = csem_malloc(&b, (long)n * 16)) ||
+      (rc = csem_memcpy_h2d(x, xh, (long)n * 16, NULL)) || (rc = csem_memcpy_h2d(b, bh, (long)n * 16, NULL))) {
+    die_dev("solver staging", rc);
     goto out;
   }
-  G.last_iters = steps;
-  if (verb) {
-    const char *file = flexible ? "iterate_fgmres.txt" : "iterate_gmres.txt";
+  if (gm)
+    it = csem_gmres_op(model, &f, n, x, b, niter, tol, m, A.op, A.ctx, M.op, M.ctx, hist, NULL);
+  else
+    it = csem_bicgstab_op(model, &f, n, x, b, niter, tol, A.op, A.ctx, M.op, M.ctx, hist, NULL);
+  if (it < 0) {
+    die_dev(who, it);
+    goto out;
+  }
+  if ((rc = csem_memcpy_d2h(xh, x, (long)n * 16, NULL))) {
+    die_dev("solver download", rc);
+    goto out;
+  }
+  G.last_iters = it;
+  if (verb && !gm) {
+    log_history(kind == K_FBICGSTAB ? "iterate_fbicgstab.txt" : "iterate_bicgstab.txt",
+                kind == K_FBICGSTAB ? "f-bicgstab" : "bicgstab", hist, (it < niter ? it + 1 : niter),
+                it < niter ? it : -1);
+  } else if (verb) {
+    const char *file = kind == K_FGMRES ? "iterate_fgmres.txt" : "iterate_gmres.txt";
     FILE *fp = fopen(file, "w");
     if (fp) {
       fprintf(fp, "iteration \t error\n");
-      for (i = 0; i <= steps; i++) fprintf(fp, "%d \t %e\n", i, hist[i]);
+      for (i = 0; i <= it; i++) fprintf(fp, "%d \t %e\n", i, hist[i]);
       fclose(fp);
     }
-    for (i = 0; i <= steps; i++) printf("%s iter=%d relres=%e\n", flexible ? "f-gmres" : "gmres", i, hist[i]);
+    for (i = 0; i <= it; i++) printf("%s iter=%d relres=%e\n", kind == K_FGMRES ? "f-gmres" : "gmres", i, hist[i]);
   }
   csem_set_status(0);
 out:
   if (x) csem_free(x);
   if (b) csem_free(b);
+  unmap_op(&A);
+  unmap_op(&M);
   free(hist);
+}
+
+void fbicgstab(int n, csem_complex *x, csem_complex *b, op_t Aop, op_t invMop, int niter, double tol, int verb)
+{
+  krylov("fbicgstab", K_FBICGSTAB, n, x, b, Aop, invMop, niter, tol, 0, verb);
+}
+
+void bicgstab(int n, csem_complex *x, csem_complex *b, op_t Aop, int niter, double tol, int verb)
+{
+  krylov("bicgstab", K_BICGSTAB, n, x, b, Aop, NULL, niter, tol, 0, verb);
 }
 
 void fgmres(int n, csem_complex *x, csem_complex *b, op_t Aop, op_t invMop, int niter, double tol, int m, int verb)
 {
-  if (!need_model("fgmres")) return;
-  if (Aop != fdfd_apply || invMop != lusgs_apply) {
-    csem_set_status(-5);
-    err("fgmres: only the device operators (fdfd_apply, lusgs_apply) are provided");
-    return;
-  }
-  host_gmres(n, x, b, niter, tol, m, verb, 1);
+  krylov("fgmres", K_FGMRES, n, x, b, Aop, invMop, niter, tol, m, verb);
 }
 
 void gmres(int n, csem_complex *x, csem_complex *b, op_t Aop, int niter, double tol, int m, int verb)
 {
-  if (!need_model("gmres")) return;
-  if (Aop != fdfd_apply) {
-    csem_set_status(-5);
-    err("gmres: only the device operator fdfd_apply is provided");
+  krylov("gmres", K_GMRES, n, x, b, Aop, NULL, niter, tol, m, verb);
+}
+
+/* gmres_init_free.c:29-44: the inner-GMRES preconditioner (preco=2) on
+ * Aop with m Arnoldi steps per application */
+void gmres_init(int n, int m, op_t Aop)
+{
+  int rc;
+  if (GP.on) gmres_free();
+  memset(&GP, 0, sizeof GP);
+  if ((rc = map_op(Aop, n, &GP.inner))) {
+    die_dev("gmres_init", rc);
     return;
   }
-  host_gmres(n, x, b, niter, tol, m, verb, 0);
+  if (m < 1 || m > 64) {
+    unmap_op(&GP.inner);
+    csem_set_status(-3);
+    err("gmres_init: inner dimension mp=%d outside [1, 64]", m);
+    return;
+  }
+  GP.prec.m = m;
+  GP.prec.n = n;
+  GP.prec.A = GP.inner.op;
+  GP.prec.actx = GP.inner.ctx;
+  GP.on = 1;
+  csem_set_status(0);
 }
+
+void gmres_free(void)
+{
+  if (!GP.on) return;
+  csem_gmres_prec_release(&GP.prec);
+  unmap_op(&GP.inner);
+  memset(&GP, 0, sizeof GP);
+}
+
+/* op_t entry points that are preconditioners: host arrays in and out */
+static void apply_staged(const char *who, csem_op op, void *ctx, int n, csem_complex *x, csem_complex *y,
+                         int on_model)
+{
+  csem_freq f;
+  int rc;
+  memset(&f, 0, sizeof f);
+  if (on_model) {
+    if (!need_model(who)) return;
+    if (n != csem_model_len(G.model)) {
+      csem_set_status(-3);
+      err("%s: n=%d but the operator's vectors hold %ld values", who, n, csem_model_len(G.model));
+      return;
+    }
+  }
+  if (G.emf) f = freq_of(G.emf);
+  if (G.npvec < n) {
+    if (G.px) csem_free(G.px);
+    if (G.py) csem_free(G.py);
+    G.px = G.py = NULL;
+    G.npvec = 0;
+    if ((rc = csem_malloc(&G.px, (long)n * 16)) || (rc = csem_malloc(&G.py, (long)n * 16))) {
+      die_dev(who, rc);
+      return;
+    }
+    G.npvec = n;
+  }
+  if ((rc = csem_memcpy_h2d(G.px, x, (long)n * 16, NULL)) ||
+      (rc = op(ctx, on_model ? G.model : NULL, &f, G.px, G.py, NULL)) ||
+      (rc = csem_memcpy_d2h(y, G.py, (long)n * 16, NULL))) {
+    die_dev(who, rc);
+    return;
+  }
+  csem_set_status(0);
+}
+
+/* gmres_init_free.c:58: gmres_apply(n, b, x) -- b in, x out */
+void gmres_apply(int n, csem_complex *b, csem_complex *x)
+{
+  if (!GP.on) {
+    csem_set_status(-4);
+    err("gmres_apply called before gmres_init");
+    return;
+  }
+  apply_staged("gmres_apply", csem_op_gmres, &GP.prec, n, b, x, op_on_model(&GP.inner));
+}
+
+/* fdfd.c:513-821 */
+void aphi_apply(int n, csem_complex *x, csem_complex *y)
+{
+  apply_staged("aphi_apply", csem_op_aphi, NULL, n, x, y, 1);
+}
+
+/* The matrix-storage variants of main.c's main2 / main3 (Acsr.c, Acoo.c):
+ * not provided; each call reports an error. */
+static void not_provided(const char *who)
+{
+  csem_set_status(-5);
+  err("%s: the matrix-storage path is not provided by this library (use fdfd_apply / lusgs_apply)", who);
+}
+void Acsr_init(emf_t *emf, int ifreq) { (void)emf; (void)ifreq; not_provided("Acsr_init"); }
+void Acsr_free(void) { not_provided("Acsr_free"); }
+void Acsr_apply(int n, csem_complex *x, csem_complex *y) { (void)n; (void)x; (void)y; not_provided("Acsr_apply"); }
+void Acsr_lusgs_apply(int n, csem_complex *x, csem_complex *y)
+{
+  (void)n; (void)x; (void)y;
+  not_provided("Acsr_lusgs_apply");
+}
+void Acoo_init(emf_t *emf, int ifreq) { (void)emf; (void)ifreq; not_provided("Acoo_init"); }
+void Acoo_free(void) { not_provided("Acoo_free"); }
+void Acoo_solve(emf_t *emf) { (void)emf; not_provided("Acoo_solve"); }
 
 /* ---------------------------------------------------------------- sources */
 

@@ -35,7 +35,7 @@ int main(int argc, char **argv)
   if (!getparfloat("tol", &emf->tol)) emf->tol = 1e-8;
   if (!getparint("nrestart", &emf->nrestart)) emf->nrestart = 10;
   if (!getparint("mp", &emf->mp)) emf->mp = 30;
-  if (emf->preco != 0 && emf->preco != 1) err("preco=%d is not provided (0 none, 1 LU-SGS)", emf->preco);
+  if (emf->preco < 0 || emf->preco > 3) err("preco=%d: 0 none, 1 LU-SGS, 2 inner GMRES, 3 A-phi", emf->preco);
   emf_init(emf);
   acq_init(acq, emf);
   extend_model_init(acq, emf);
@@ -62,10 +62,14 @@ int main(int argc, char **argv)
       if (emf->algopt == 2)
         gmres(emf->ne, emf->xe, emf->be, fdfd_apply, emf->niter / emf->nrestart, tol, emf->nrestart, emf->verb);
     } else {
-      if (emf->algopt == 1) fbicgstab(emf->ne, emf->xe, emf->be, fdfd_apply, lusgs_apply, emf->niter, tol, emf->verb);
+      /* main.c:106-117: the preconditioner by preco */
+      op_t M = emf->preco == 1 ? lusgs_apply : emf->preco == 2 ? gmres_apply : aphi_apply;
+      if (emf->preco == 2) gmres_init(emf->ne, emf->mp, fdfd_apply);
+      if (emf->algopt == 1) fbicgstab(emf->ne, emf->xe, emf->be, fdfd_apply, M, emf->niter, tol, emf->verb);
       if (emf->algopt == 2)
-        fgmres(emf->ne, emf->xe, emf->be, fdfd_apply, lusgs_apply, emf->niter / emf->nrestart, tol,
-               emf->nrestart, emf->verb);
+        fgmres(emf->ne, emf->xe, emf->be, fdfd_apply, M, emf->niter / emf->nrestart, tol, emf->nrestart,
+               emf->verb);
+      if (emf->preco == 2) gmres_free();
     }
     printf("elapsed time: %g sec\n", now() - t0);
     extract_emf(acq, emf, emf->xe, ifreq);

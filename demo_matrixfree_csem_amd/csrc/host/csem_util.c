@@ -21,6 +21,7 @@ int iproc = 0, nproc = 1, ierr = 0;
 
 static int g_status = 0;
 static int g_exit_on_error = 1;
+int csem_fail(int code, const char *fmt, ...); /* device library: records the message */
 
 int csem_last_status(void) { return g_status; }
 void csem_set_exit_on_error(int on) { g_exit_on_error = on; }
@@ -34,7 +35,10 @@ void err(char *fmt, ...)
   vsnprintf(msg, sizeof msg, fmt, ap);
   va_end(ap);
   fprintf(stderr, "\n%s\n", msg);
-  g_status = -100;
+  /* keep a specific code the caller set just before; the message goes
+   * where csem_last_error() reads it */
+  if (g_status >= 0) g_status = -100;
+  csem_fail(g_status, "%s", msg);
   if (g_exit_on_error) exit(1);
 }
 

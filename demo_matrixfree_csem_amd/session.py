@@ -173,30 +173,61 @@ class Session:
                 setattr(a, k, C.cast(C.c_void_p(v), C.POINTER(C.c_float)))
             e.dcal_fd = C.cast(C.c_void_p(keep_fd), type(e.dcal_fd))
 
-    def solve(self, b, tol, niter=None, algopt=None, preco=None, nrestart=None, x0=None, verb=0):
+    def solve(self, b, tol, niter=None, algopt=None, preco=None, nrestart=None, x0=None, verb=0, mp=None):
         """The solver call of main.c:104-118; returns (x, iterations).
-        With verb=1 the solver writes its iterate_*.txt history to the
-        current directory, as the reference's does (solver.c:127-141)."""
+        preco 0 none, 1 LU-SGS, 2 inner GMRES(mp) on fdfd_apply, 3 A-phi;
+        algopt 1 (F-)BiCGStab, 2 (F)GMRES(nrestart).  With verb=1 the
+        solver writes its iterate_*.txt history to the current directory,
+        as the reference's does (solver.c:127-141)."""
         e = self.emf
         L = lib()
         niter = e.niter if niter is None else niter
         algopt = e.algopt if algopt is None else algopt
         preco = e.preco if preco is None else preco
         nrestart = e.nrestart if nrestart is None else nrestart
+        mp = e.mp if mp is None else mp
         x = np.zeros(self.n, np.complex128) if x0 is None else np.array(x0, np.complex128)
         b = np.ascontiguousarray(b, np.complex128)
         A = _lib.fn_addr("fdfd_apply")
-        M = _lib.fn_addr("lusgs_apply")
-        if preco == 0 and algopt == 1:
-            L.bicgstab(self.n, _p(x), _p(b), A, niter, tol, verb)
-        elif preco == 0:
-            L.gmres(self.n, _p(x), _p(b), A, niter // nrestart, tol, nrestart, verb)
-        elif algopt == 1:
-            L.fbicgstab(self.n, _p(x), _p(b), A, M, niter, tol, verb)
-        else:
-            L.fgmres(self.n, _p(x), _p(b), A, M, niter // nrestart, tol, nrestart, verb)
-        self._status("solve")
+        M = _lib.fn_addr({1: "lusgs_apply", 2: "gmres_apply", 3: "aphi_apply"}.get(preco, "lusgs_apply"))
+        if preco == 2:
+            L.gmres_init(self.n, mp, A)
+            self._status("gmres_init")
+        try:
+            if preco == 0 and algopt == 1:
+                L.bicgstab(self.n, _p(x), _p(b), A, niter, tol, verb)
+            elif preco == 0:
+                L.gmres(self.n, _p(x), _p(b), A, niter // nrestart, tol, nrestart, verb)
+            elif algopt == 1:
+                L.fbicgstab(self.n, _p(x), _p(b), A, M, niter, tol, verb)
+            else:
+                L.fgmres(self.n, _p(x), _p(b), A, M, niter // nrestart, tol, nrestart, verb)
+            self._status("solve")
+        finally:
+            if preco == 2:
+                L.gmres_free()
         return x, L.csem_last_iterations()
+
+    def aphi_apply(self, x):
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.zeros(self.n, np.complex128)
+        lib().aphi_apply(self.n, _p(x), _p(y))
+        self._status("aphi_apply")
+        return y
+
+    def gmres_apply(self, b, mp):
+        """gmres_init(n, mp, fdfd_apply); gmres_apply(b -> x); gmres_free"""
+        L = lib()
+        b = np.ascontiguousarray(b, np.complex128)
+        x = np.zeros(self.n, np.complex128)
+        L.gmres_init(self.n, mp, _lib.fn_addr("fdfd_apply"))
+        self._status("gmres_init")
+        try:
+            L.gmres_apply(self.n, _p(b), _p(x))
+            self._status("gmres_apply")
+        finally:
+            L.gmres_free()
+        return x
 
     def solve_mixed(self, bs, freqs, tols, opts=None):
         """Batched mixed-precision solve (csem_mixed_solve) of the right-hand

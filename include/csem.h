@@ -15,8 +15,12 @@
  *   fdfd_apply   (op_t)                 fdfd.c:86-205
  *   lusgs_apply  (op_t)                 fdfd.c:287-510
  *   airbc_init / airbc_free / _apply    airbc_init_free.c:24-209
+ *   aphi_apply   (op_t)                 fdfd.c:513-821 (A-phi preconditioner, preco=3)
+ *   gmres_init / gmres_free / _apply    gmres_init_free.c:29-137 (inner GMRES, preco=2)
  *   bicgstab / fbicgstab                solver.c:16-87, 93-178
  *   gmres / fgmres                      solver.c:181-285, 403-512
+ *   Acsr_* / Acoo_*                     Acsr.c:22-748, Acoo.c (main.c:148-151, 221-223;
+ *                                       not provided: they report an error)
  *   cdotprod                            cdotprod.c:13-20
  *   inject_source / extract_emf         inject_extract.c:24-213
  *   write_data                          write_data.c:15-39
@@ -107,6 +111,14 @@ void lusgs_apply(int n, csem_complex *x, csem_complex *y);
 void airbc_init(emf_t *emf);
 void airbc_free(emf_t *emf);
 void airbc_apply(emf_t *emf, csem_complex *H3k, csem_complex *H1km1, csem_complex *H2km1);
+void aphi_apply(int n, csem_complex *x, csem_complex *y);
+void gmres_init(int n, int m, op_t Aop);
+void gmres_free(void);
+void gmres_apply(int n, csem_complex *b, csem_complex *x);
+/* The Krylov drivers accept any op_t.  fdfd_apply, lusgs_apply,
+ * aphi_apply and gmres_apply (on fdfd_apply) run as device kernels and
+ * the whole solve stays device-resident; any other function is called on
+ * host copies of its input and output vectors. */
 void bicgstab(int n, csem_complex *x, csem_complex *b, op_t Aop, int niter, double tol, int verb);
 void gmres(int n, csem_complex *x, csem_complex *b, op_t Aop, int niter, double tol, int m, int verb);
 void fbicgstab(int n, csem_complex *x, csem_complex *b, op_t Aop, op_t invMop, int niter, double tol, int verb);
@@ -115,6 +127,17 @@ csem_complex cdotprod(int n, csem_complex *a, csem_complex *b);
 void inject_source(acq_t *acq, emf_t *emf, csem_complex *b);
 void extract_emf(acq_t *acq, emf_t *emf, csem_complex *x, int ifreq);
 void write_data(acq_t *acq, emf_t *emf, char *fname, float _Complex ***dcal_fd);
+/* The reference's matrix-storage variants (main.c:148-151, 221-223: CSR
+ * assembly with CSR LU-SGS, and a MUMPS direct solve, used only by its
+ * main2 / main3).  Declared so that a reference main.c links; each call
+ * reports an error through err() / csem_last_status(). */
+void Acsr_init(emf_t *emf, int ifreq);
+void Acsr_free(void);
+void Acsr_apply(int n, csem_complex *x, csem_complex *y);
+void Acsr_lusgs_apply(int n, csem_complex *x, csem_complex *y);
+void Acoo_init(emf_t *emf, int ifreq);
+void Acoo_free(void);
+void Acoo_solve(emf_t *emf);
 int find_index(int n, double *x, double val);
 double create_nugrid(int n, double len, double dx, double *x, int istretch);
 

@@ -93,6 +93,53 @@ int csem_fgmres(csem_model *m, const csem_freq *f, void *x, const void *b, int n
 int csem_gmres(csem_model *m, const csem_freq *f, void *x, const void *b, int nouter,
                double tol, int mrestart, double *hist, void *stream);
 
+/* ---- Operators as arguments.  The reference's Krylov drivers are generic
+ * over op_t (solver.h:4): any y = Op x.  Here an operator acts on device
+ * vectors (interleaved complex doubles) as op(ctx, model, freq, x, y,
+ * stream) and returns 0 or a negative error code.  The library's own:
+ *   csem_op_apply   y = A x            (fdfd_apply, fdfd.c:86-205)
+ *   csem_op_lusgs   y = M^-1 x, LU-SGS (lusgs_apply, fdfd.c:287-510)
+ *   csem_op_aphi    y = M^-1 x, A-phi  (aphi_apply, fdfd.c:513-821)
+ *   csem_op_gmres   y = GMRES(m) x     (gmres_apply, gmres_init_free.c:58-137);
+ *                   ctx is a csem_gmres_prec
+ *   csem_op_host    y = fn(x) for a caller's host function fn of the
+ *                   reference's op_t shape; ctx is a csem_host_op.  x is
+ *                   copied to host memory, fn called, y copied back.
+ * The device operators need the model; csem_op_host does not. */
+typedef int (*csem_op)(void *ctx, csem_model *m, const csem_freq *f, const void *x, void *y, void *stream);
+int csem_op_apply(void *ctx, csem_model *m, const csem_freq *f, const void *x, void *y, void *stream);
+int csem_op_lusgs(void *ctx, csem_model *m, const csem_freq *f, const void *x, void *y, void *stream);
+int csem_op_aphi(void *ctx, csem_model *m, const csem_freq *f, const void *x, void *y, void *stream);
+int csem_op_gmres(void *ctx, csem_model *m, const csem_freq *f, const void *x, void *y, void *stream);
+int csem_op_host(void *ctx, csem_model *m, const csem_freq *f, const void *x, void *y, void *stream);
+typedef struct {
+  void (*fn)(int n, void *x, void *y); /* host arrays of n interleaved complex doubles */
+  int n;
+  double *hx, *hy;                     /* host staging, 2n doubles each (allocated by the caller) */
+} csem_host_op;
+/* Inner GMRES(m) preconditioner on the operator (A, actx): a fixed m
+ * Arnoldi steps from x = 0 (no convergence test), the least-squares
+ * solution by Givens rotations and back substitution.  Device work
+ * vectors are allocated on first use for vectors of length n and released
+ * by csem_gmres_prec_release. */
+typedef struct {
+  int m;
+  long n;
+  csem_op A;
+  void *actx;
+  void *work; /* device, (m + 3) * n complex doubles + scalars; NULL until first use */
+} csem_gmres_prec;
+void csem_gmres_prec_release(csem_gmres_prec *p);
+/* Krylov drivers over operators.  n is the vector length (csem_model_len
+ * when m is given; m may be NULL when A and M are host operators).  M =
+ * NULL: no preconditioner, i.e. bicgstab / gmres (solver.c:16-87,
+ * 181-285); otherwise fbicgstab / fgmres (solver.c:93-178, 403-512).
+ * Return values and hist as csem_bicgstab / csem_fgmres. */
+int csem_bicgstab_op(csem_model *m, const csem_freq *f, long n, void *x, const void *b, int niter, double tol,
+                     csem_op A, void *actx, csem_op M, void *mctx, double *hist, void *stream);
+int csem_gmres_op(csem_model *m, const csem_freq *f, long n, void *x, const void *b, int nouter, double tol,
+                  int mrestart, csem_op A, void *actx, csem_op M, void *mctx, double *hist, void *stream);
+
 /* Sparse source term: b[idx[q]] += val[q] for q in order (val interleaved
  * complex, host arrays). */
 int csem_scatter_add(long nnz, const long *idx, const double *val, void *b, void *stream);

@@ -45,6 +45,8 @@ struct orc {
   double *wx, *wy;
   zc iomu, iwmu;
   int red;
+  int mp;           /* inner GMRES dimension of the preco=2 preconditioner */
+  zc *av, *abv;     /* A-phi potential and its right-hand side (preco=3) */
 };
 
 /* ------------------------------------------------------------------ grid */
@@ -124,6 +126,7 @@ orc_t *orc_build(int nx, int ny, int nz,
                  int airbc, int fvm, int nfft)
 {
   orc_t *m = (orc_t *)calloc(1, sizeof(orc_t));
+  m->mp = 30; /* main.c:69 default */
   int i, j, k, im1, jm1, km1;
   long c;
   double dx;
@@ -355,6 +358,7 @@ void orc_free(orc_t *m)
   free(m->s11); free(m->s22); free(m->s33);
   free(m->c11); free(m->c22); free(m->c33);
   free(m->xh); free(m->xx); free(m->yy); free(m->Hxg); free(m->Hyg);
+  free(m->av); free(m->abv);
   free(m->sH1); free(m->sH2); free(m->buf); free(m->buf0); free(m->buf2); free(m->tw); free(m->tmp);
   free(m->jx); free(m->jy); free(m->wx); free(m->wy);
   free(m);
@@ -973,6 +977,261 @@ orc_c128 orc_dot(const orc_t *m, long n, const zc *a, const zc *b)
   return mk(vr[0], vi[0]);
 }
 
+/* ----------------------------------------------- other preconditioners */
+
+#define idv(k, j, i) ((i) + (long)(n1 + 1) * ((j) + (long)(n2 + 1) * (k)))
+
+/* aphi_apply, fdfd.c:513-821: the A-phi splitting E = A + grad(phi).
+ * b_phi = -div(x); one forward and one backward Gauss-Seidel sweep on
+ * (A, phi) from zero, in the order ax, ay, az, phi forward and phi, az,
+ * ay, ax backward; then y = A + grad(phi).  A lives in y.  As written
+ * there: every cell zeroes itself before its row is formed; the phi rows
+ * read the conductivity behind the cell (t2, t4, t6) from the full grid
+ * without the airwave boundary's plane offset (fdfd.c:656-661), the
+ * others with it. */
+void orc_aphi_apply(orc_t *m, const zc *x, zc *y)
+{
+  const int n1 = m->n1, n2 = m->n2, n3 = m->n3, kstart = m->airbc ? m->nb : 0;
+  int i, j, k, ip1, jp1, kp1, im1, jm1, km1;
+  zc t1, t2, t3, t4, t5, t6, aij_xj, aii, iwmu = m->iwmu;
+  double vol;
+  const double *d1 = m->d1, *d2 = m->d2, *d3 = &m->d3[kstart];
+  const double *d1s = m->d1s, *d2s = m->d2s, *d3s = &m->d3s[kstart];
+  const double *sigma11 = &m->s11[(long)n1 * (n2 + 1) * kstart];
+  const double *sigma22 = &m->s22[(long)(n1 + 1) * n2 * kstart];
+  const double *sigma33 = &m->s33[(long)(n1 + 1) * (n2 + 1) * kstart];
+  const double *S11 = m->s11, *S22 = m->s22, *S33 = m->s33; /* emf->sigma*[k][j][i], full grid */
+  const long lv = (long)(n1 + 1) * (n2 + 1) * (n3 + 1);
+  zc *ax = y, *ay = y + m->lEx, *az = y + m->lEx + m->lEy;
+  const zc *bx = x, *by = x + m->lEx, *bz = x + m->lEx + m->lEy;
+  zc *Ex, *Ey, *Ez;
+  if (!m->av) {
+    m->av = zalloc(lv);
+    m->abv = zalloc(lv);
+  }
+  zc *v = m->av, *bv = m->abv;
+#define FVM_SCALE_A()                                \
+  if (m->fvm) {                                      \
+    vol = d1s[i] * d2[j] * d3[k];                    \
+    aij_xj *= vol * m->iomu;                         \
+    aii *= vol * m->iomu;                            \
+  }
+  /* 1. bv = -div(x) (fdfd.c:554-566) */
+  memset(bv, 0, lv * sizeof(zc));
+  for (k = 1; k < n3; k++) {
+    km1 = k - 1;
+    for (j = 1; j < n2; j++) {
+      jm1 = j - 1;
+      for (i = 1; i < n1; i++) {
+        im1 = i - 1;
+        bv[idv(k, j, i)] = 0;
+        bv[idv(k, j, i)] += -(bx[id1(k, j, i)] - bx[id1(k, j, im1)]) / d1[i];
+        bv[idv(k, j, i)] += -(by[id2(k, j, i)] - by[id2(k, jm1, i)]) / d2[j];
+        bv[idv(k, j, i)] += -(bz[id3(k, j, i)] - bz[id3(km1, j, i)]) / d3[k];
+      }
+    }
+  }
+  memset(ax, 0, m->lEx * sizeof(zc));
+  memset(ay, 0, m->lEy * sizeof(zc));
+  memset(az, 0, m->lEz * sizeof(zc));
+  memset(v, 0, lv * sizeof(zc));
+#define ROW_AX()                                                                                                 \
+  ip1 = MINI(i + 1, n1); im1 = MAXI(i - 1, 0);                                                                   \
+  ax[id1(k, j, i)] = 0;                                                                                          \
+  t1 = (ay[id2(k, j, ip1)] - ay[id2(k, j, i)]) / d1s[i] - (ax[id1(k, jp1, i)] - ax[id1(k, j, i)]) / d2s[j];      \
+  t2 = (ay[id2(k, jm1, ip1)] - ay[id2(k, jm1, i)]) / d1s[i] - (ax[id1(k, j, i)] - ax[id1(k, jm1, i)]) / d2s[jm1]; \
+  t3 = (ax[id1(kp1, j, i)] - ax[id1(k, j, i)]) / d3s[k] - (az[id3(k, j, ip1)] - az[id3(k, j, i)]) / d1s[i];      \
+  t4 = (ax[id1(k, j, i)] - ax[id1(km1, j, i)]) / d3s[km1] - (az[id3(km1, j, ip1)] - az[id3(km1, j, i)]) / d1s[i]; \
+  aij_xj = ((t1 - t2) / d2[j] - (t3 - t4) / d3[k]) * iwmu -                                                      \
+           sigma11[id1(k, j, i)] * (ax[id1(k, j, i)] + (v[idv(k, j, ip1)] - v[idv(k, j, i)]) / d1s[i]);          \
+  aii = ((1. / d2s[j] + 1. / d2s[jm1]) / d2[j] + (1. / d3s[k] + 1. / d3s[km1]) / d3[k]) * iwmu -                 \
+        sigma11[id1(k, j, i)];                                                                                   \
+  FVM_SCALE_A();                                                                                                 \
+  ax[id1(k, j, i)] = (bx[id1(k, j, i)] - aij_xj) / aii;
+#define ROW_AY()                                                                                                   \
+  ip1 = MINI(i + 1, n1); im1 = MAXI(i - 1, 0);                                                                     \
+  ay[id2(k, j, i)] = 0;                                                                                            \
+  t1 = (az[id3(k, jp1, i)] - az[id3(k, j, i)]) / d2s[j] - (ay[id2(kp1, j, i)] - ay[id2(k, j, i)]) / d3s[k];        \
+  t2 = (az[id3(km1, jp1, i)] - az[id3(km1, j, i)]) / d2s[j] - (ay[id2(k, j, i)] - ay[id2(km1, j, i)]) / d3s[km1];  \
+  t3 = (ay[id2(k, j, ip1)] - ay[id2(k, j, i)]) / d1s[i] - (ax[id1(k, jp1, i)] - ax[id1(k, j, i)]) / d2s[j];        \
+  t4 = (ay[id2(k, j, i)] - ay[id2(k, j, im1)]) / d1s[im1] - (ax[id1(k, jp1, im1)] - ax[id1(k, j, im1)]) / d2s[j];  \
+  aij_xj = ((t1 - t2) / d3[k] - (t3 - t4) / d1[i]) * iwmu -                                                        \
+           sigma22[id2(k, j, i)] * (ay[id2(k, j, i)] + (v[idv(k, jp1, i)] - v[idv(k, j, i)]) / d2s[j]);            \
+  aii = ((1. / d3s[k] + 1. / d3s[km1]) / d3[k] + (1. / d1s[i] + 1. / d1s[im1]) / d1[i]) * iwmu -                   \
+        sigma22[id2(k, j, i)];                                                                                     \
+  FVM_SCALE_A();                                                                                                   \
+  ay[id2(k, j, i)] = (by[id2(k, j, i)] - aij_xj) / aii;
+#define ROW_AZ()                                                                                                   \
+  ip1 = MINI(i + 1, n1); im1 = MAXI(i - 1, 0);                                                                     \
+  az[id3(k, j, i)] = 0;                                                                                            \
+  t1 = (ax[id1(kp1, j, i)] - ax[id1(k, j, i)]) / d3s[k] - (az[id3(k, j, ip1)] - az[id3(k, j, i)]) / d1s[i];        \
+  t2 = (ax[id1(kp1, j, im1)] - ax[id1(k, j, im1)]) / d3s[k] - (az[id3(k, j, i)] - az[id3(k, j, im1)]) / d1s[im1];  \
+  t3 = (az[id3(k, jp1, i)] - az[id3(k, j, i)]) / d2s[j] - (ay[id2(kp1, j, i)] - ay[id2(k, j, i)]) / d3s[k];        \
+  t4 = (az[id3(k, j, i)] - az[id3(k, jm1, i)]) / d2s[jm1] - (ay[id2(kp1, jm1, i)] - ay[id2(k, jm1, i)]) / d3s[k];  \
+  aij_xj = ((t1 - t2) / d1[i] - (t3 - t4) / d2[j]) * iwmu -                                                        \
+           sigma33[id3(k, j, i)] * (az[id3(k, j, i)] + (v[idv(kp1, j, i)] - v[idv(k, j, i)]) / d3s[k]);            \
+  aii = ((1. / d1s[i] + 1. / d1s[im1]) / d1[i] + (1. / d2s[j] + 1. / d2s[jm1]) / d2[j]) * iwmu -                   \
+        sigma33[id3(k, j, i)];                                                                                     \
+  FVM_SCALE_A();                                                                                                   \
+  az[id3(k, j, i)] = (bz[id3(k, j, i)] - aij_xj) / aii;
+#define ROW_V()                                                                                                    \
+  ip1 = i + 1; im1 = i - 1;                                                                                        \
+  v[idv(k, j, i)] = 0;                                                                                             \
+  t1 = sigma11[id1(k, j, i)] * (ax[id1(k, j, i)] + (v[idv(k, j, ip1)] - v[idv(k, j, i)]) / d1s[i]);                 \
+  t2 = S11[id1(k, j, im1)] * (ax[id1(k, j, im1)] + (v[idv(k, j, i)] - v[idv(k, j, im1)]) / d1s[im1]);               \
+  t3 = sigma22[id2(k, j, i)] * (ay[id2(k, j, i)] + (v[idv(k, jp1, i)] - v[idv(k, j, i)]) / d2s[j]);                 \
+  t4 = S22[id2(k, jm1, i)] * (ay[id2(k, jm1, i)] + (v[idv(k, j, i)] - v[idv(k, jm1, i)]) / d2s[jm1]);               \
+  t5 = sigma33[id3(k, j, i)] * (az[id3(k, j, i)] + (v[idv(kp1, j, i)] - v[idv(k, j, i)]) / d3s[k]);                 \
+  t6 = S33[id3(km1, j, i)] * (az[id3(km1, j, i)] + (v[idv(k, j, i)] - v[idv(km1, j, i)]) / d3s[km1]);               \
+  aij_xj = ((t1 - t2) / d1[i] + (t3 - t4) / d2[j] + (t5 - t6) / d3[k]);                                            \
+  aii = -((1. / d1s[i] + 1. / d1s[im1]) / d1[i] + (1. / d2s[j] + 1. / d2s[jm1]) / d2[j] +                          \
+          (1. / d3s[k] + 1. / d3s[km1]) / d3[k]);                                                                  \
+  FVM_SCALE_A();                                                                                                   \
+  v[idv(k, j, i)] = (bv[idv(k, j, i)] - aij_xj) / aii;
+  /* 2. forward Gauss-Seidel (fdfd.c:575-687) */
+  for (k = 1; k < n3; k++) {
+    kp1 = MINI(k + 1, n3); km1 = MAXI(k - 1, 0);
+    for (j = 1; j < n2; j++) {
+      jp1 = MINI(j + 1, n2); jm1 = MAXI(j - 1, 0);
+      for (i = 0; i < n1; i++) { ROW_AX(); }
+    }
+  }
+  for (k = 1; k < n3; k++) {
+    kp1 = MINI(k + 1, n3); km1 = MAXI(k - 1, 0);
+    for (j = 0; j < n2; j++) {
+      jp1 = MINI(j + 1, n2); jm1 = MAXI(j - 1, 0);
+      for (i = 1; i < n1; i++) { ROW_AY(); }
+    }
+  }
+  for (k = 0; k < n3; k++) {
+    kp1 = MINI(k + 1, n3); km1 = MAXI(k - 1, 0);
+    for (j = 1; j < n2; j++) {
+      jp1 = MINI(j + 1, n2); jm1 = MAXI(j - 1, 0);
+      for (i = 1; i < n1; i++) { ROW_AZ(); }
+    }
+  }
+  for (k = 1; k < n3; k++) {
+    kp1 = k + 1; km1 = k - 1;
+    for (j = 1; j < n2; j++) {
+      jp1 = j + 1; jm1 = j - 1;
+      for (i = 1; i < n1; i++) { ROW_V(); }
+    }
+  }
+  /* backward Gauss-Seidel (fdfd.c:689-790) */
+  for (k = n3 - 1; k >= 1; k--) {
+    kp1 = k + 1; km1 = k - 1;
+    for (j = n2 - 1; j >= 1; j--) {
+      jp1 = j + 1; jm1 = j - 1;
+      for (i = n1 - 1; i >= 1; i--) { ROW_V(); }
+    }
+  }
+  for (k = n3 - 1; k >= 0; k--) {
+    kp1 = MINI(k + 1, n3); km1 = MAXI(k - 1, 0);
+    for (j = n2 - 1; j > 0; j--) {
+      jp1 = MINI(j + 1, n2); jm1 = MAXI(j - 1, 0);
+      for (i = n1 - 1; i > 0; i--) { ROW_AZ(); }
+    }
+  }
+  for (k = n3 - 1; k > 0; k--) {
+    kp1 = MINI(k + 1, n3); km1 = MAXI(k - 1, 0);
+    for (j = n2 - 1; j >= 0; j--) {
+      jp1 = MINI(j + 1, n2); jm1 = MAXI(j - 1, 0);
+      for (i = n1 - 1; i > 0; i--) { ROW_AY(); }
+    }
+  }
+  for (k = n3 - 1; k > 0; k--) {
+    kp1 = MINI(k + 1, n3); km1 = MAXI(k - 1, 0);
+    for (j = n2 - 1; j > 0; j--) {
+      jp1 = MINI(j + 1, n2); jm1 = MAXI(j - 1, 0);
+      for (i = n1 - 1; i >= 0; i--) { ROW_AX(); }
+    }
+  }
+  /* 3. y = A + grad(phi) (fdfd.c:793-820), in place */
+  Ex = y; Ey = y + m->lEx; Ez = y + m->lEx + m->lEy;
+  for (k = 1; k < n3; k++)
+    for (j = 1; j < n2; j++)
+      for (i = 0; i < n1; i++) Ex[id1(k, j, i)] = ax[id1(k, j, i)] + (v[idv(k, j, i + 1)] - v[idv(k, j, i)]) / d1s[i];
+  for (k = 1; k < n3; k++)
+    for (j = 0; j < n2; j++)
+      for (i = 1; i < n1; i++) Ey[id2(k, j, i)] = ay[id2(k, j, i)] + (v[idv(k, j + 1, i)] - v[idv(k, j, i)]) / d2s[j];
+  for (k = 0; k < n3; k++)
+    for (j = 1; j < n2; j++)
+      for (i = 1; i < n1; i++) Ez[id3(k, j, i)] = az[id3(k, j, i)] + (v[idv(k + 1, j, i)] - v[idv(k, j, i)]) / d3s[k];
+#undef ROW_AX
+#undef ROW_AY
+#undef ROW_AZ
+#undef ROW_V
+#undef FVM_SCALE_A
+}
+#undef idv
+
+/* gmres_apply, gmres_init_free.c:58-137: y = m-step GMRES solution of
+ * A y = x from y = 0 (no residual test, no breakdown test). */
+void orc_gmres_prec(orc_t *m, const zc *b, zc *x)
+{
+  const long n = m->lE;
+  const int mp = m->mp;
+  long i, k;
+  int j;
+  double beta, tmp;
+  zc ss;
+  zc *w = zalloc(n), *r = zalloc(n), *v = zalloc(n * (mp + 1));
+  zc *h = zalloc((long)(mp + 1) * mp), *g = zalloc(mp + 1), *y = zalloc(mp + 1), *c = zalloc(mp + 1);
+  double *s = dalloc(mp + 1);
+#define HG(a, b2) h[(long)(a) * mp + (b2)] /* h[a][b2], alloc2complex(m, m+1) */
+  memset(x, 0, n * sizeof(zc));
+  for (i = 0; i < n; i++) r[i] = b[i];
+  beta = sqrt(creal(orc_dot(m, n, r, r)));
+  if (beta == 0.0) goto out;
+  for (i = 0; i < n; i++) v[i] = r[i] / beta;
+  memset(g, 0, (mp + 1) * sizeof(zc));
+  g[0] = beta;
+  for (j = 0; j < mp; j++) {
+    orc_fdfd_apply(m, &v[n * j], w);
+    for (i = 0; i <= j; i++) {
+      HG(i, j) = orc_dot(m, n, w, &v[n * i]);
+      for (k = 0; k < n; k++) w[k] -= HG(i, j) * v[n * i + k];
+    }
+    HG(j + 1, j) = sqrt(creal(orc_dot(m, n, w, w)));
+    for (i = 0; i < n; i++) v[n * (j + 1) + i] = w[i] / HG(j + 1, j);
+    if (j > 0) {
+      for (i = 0; i < j; i++) {
+        ss = conj(c[i]) * HG(i, j) + s[i] * HG(i + 1, j);
+        HG(i + 1, j) = -s[i] * HG(i, j) + c[i] * HG(i + 1, j);
+        HG(i, j) = ss;
+      }
+    }
+    tmp = sqrt(creal(HG(j, j) * conj(HG(j, j)) + HG(j + 1, j) * conj(HG(j + 1, j))));
+    s[j] = creal(HG(j + 1, j)) / tmp;
+    c[j] = HG(j, j) / tmp;
+    HG(j, j) = conj(c[j]) * HG(j, j) + s[j] * HG(j + 1, j);
+    HG(j + 1, j) = 0.;
+    g[j + 1] = -s[j] * g[j];
+    g[j] = conj(c[j]) * g[j];
+  }
+  y[mp - 1] = g[mp - 1] / HG(mp - 1, mp - 1);
+  for (i = mp - 2; i >= 0; i--) {
+    y[i] = g[i];
+    for (j = i + 1; j < mp; j++) y[i] -= HG(i, j) * y[j];
+    y[i] /= HG(i, i);
+  }
+  for (j = 0; j < mp; j++)
+    for (i = 0; i < n; i++) x[i] += y[j] * v[n * j + i];
+#undef HG
+out:
+  free(w); free(r); free(v); free(h); free(g); free(y); free(c); free(s);
+}
+
+void orc_set_mp(orc_t *m, int mp) { m->mp = mp; }
+
+/* the preconditioner of preco = 1 (LU-SGS), 2 (inner GMRES), 3 (A-phi) */
+static void orc_precond(orc_t *m, int precond, const zc *x, zc *y)
+{
+  if (precond == 2) orc_gmres_prec(m, x, y);
+  else if (precond == 3) orc_aphi_apply(m, x, y);
+  else orc_lusgs_apply(m, x, y);
+}
+
 /* --------------------------------------------------------------- Krylov */
 
 /* bicgstab (solver.c:16-87) and fbicgstab (solver.c:93-178); the
@@ -1004,7 +1263,7 @@ int orc_fbicgstab(orc_t *m, zc *x, const zc *b, int niter, double tol, int preco
   for (iter = 0; iter < niter; ++iter) {
     if (hist) hist[iter] = relres;
     if (precond) {
-      orc_lusgs_apply(m, p, pp);
+      orc_precond(m, precond, p, pp);
       orc_fdfd_apply(m, pp, v);
     } else {
       orc_fdfd_apply(m, p, v);
@@ -1013,7 +1272,7 @@ int orc_fbicgstab(orc_t *m, zc *x, const zc *b, int niter, double tol, int preco
     alpha = rho_old / alpha;
     for (i = 0; i < n; i++) s[i] = r[i] - alpha * v[i];
     if (precond) {
-      orc_lusgs_apply(m, s, r);
+      orc_precond(m, precond, s, r);
       orc_fdfd_apply(m, r, t);
       omega = orc_dot(m, n, s, t) / creal(orc_dot(m, n, t, t));
       for (i = 0; i < n; i++) {
@@ -1076,7 +1335,7 @@ int orc_fgmres(orc_t *m, zc *x, const zc *b, int nouter, double tol, int mr, int
     }
     for (j = 0; j < mr; j++) {
       zc *zj = precond ? &z[n * j] : &v[n * j];
-      if (precond) orc_lusgs_apply(m, &v[n * j], zj);
+      if (precond) orc_precond(m, precond, &v[n * j], zj);
       orc_fdfd_apply(m, zj, w);
       for (i = 0; i <= j; i++) {
         H(j, i) = orc_dot(m, n, w, &v[n * i]);

@@ -54,6 +54,12 @@ void orc_set_freq(orc_t *m, float freq);
 
 void orc_fdfd_apply(orc_t *m, const orc_c128 *x, orc_c128 *y);
 void orc_lusgs_apply(orc_t *m, const orc_c128 *x, orc_c128 *y);
+/* aphi_apply (fdfd.c:513-821), the A-phi preconditioner of preco=3 */
+void orc_aphi_apply(orc_t *m, const orc_c128 *x, orc_c128 *y);
+/* gmres_apply (gmres_init_free.c:58-137), the inner-GMRES preconditioner of
+ * preco=2 with orc_set_mp's dimension (default 30, main.c:69) */
+void orc_gmres_prec(orc_t *m, const orc_c128 *b, orc_c128 *x);
+void orc_set_mp(orc_t *m, int mp);
 void orc_airbc_apply(orc_t *m, const orc_c128 *H3k, orc_c128 *H1km1, orc_c128 *H2km1);
 /* One stage of orc_airbc_apply: 0 forward bilinear map Hz(n1*n2) ->
  * uniform N*N; 1 uniform N*N -> (Hx, Hy) on the uniform grid (DFT,
@@ -72,6 +78,7 @@ orc_c128 orc_dot(const orc_t *m, long n, const orc_c128 *a, const orc_c128 *b);
  * iteration index at convergence (the number printed by "converged at
  * iter="), or niter when the loop ran out; hist (may be NULL, length
  * niter+1) receives relres per iteration. */
+/* precond: 0 none, 1 LU-SGS, 2 inner GMRES, 3 A-phi (main.c:104-118) */
 int orc_fbicgstab(orc_t *m, orc_c128 *x, const orc_c128 *b, int niter, double tol, int precond, double *hist);
 int orc_fgmres(orc_t *m, orc_c128 *x, const orc_c128 *b, int nouter, double tol, int mrestart, int precond, double *hist);
 

@@ -45,8 +45,9 @@ def lib():
         L.orc_array.restype = vp
         L.orc_array.argtypes = [vp, C.c_char_p]
         L.orc_set_red.argtypes = [vp, ip]
+        L.orc_set_mp.argtypes = [vp, ip]
         L.orc_set_freq.argtypes = [vp, fp]
-        for f in ("orc_fdfd_apply", "orc_lusgs_apply"):
+        for f in ("orc_fdfd_apply", "orc_lusgs_apply", "orc_aphi_apply", "orc_gmres_prec"):
             getattr(L, f).argtypes = [vp, vp, vp]
         L.orc_airbc_apply.argtypes = [vp, vp, vp, vp]
         L.orc_airbc_stage.argtypes = [vp, ip, vp, vp, vp]
@@ -149,6 +150,23 @@ class Oracle:
         y = np.zeros(self.lE, np.complex128)
         lib().orc_fdfd_apply(self.h, _p(x), _p(y))
         return y
+
+    def aphi(self, x):
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.zeros(self.lE, np.complex128)
+        lib().orc_aphi_apply(self.h, _p(x), _p(y))
+        return y
+
+    def gmres_prec(self, b, mp=None):
+        if mp is not None:
+            lib().orc_set_mp(self.h, mp)
+        b = np.ascontiguousarray(b, np.complex128)
+        x = np.zeros(self.lE, np.complex128)
+        lib().orc_gmres_prec(self.h, _p(b), _p(x))
+        return x
+
+    def set_mp(self, mp):
+        lib().orc_set_mp(self.h, mp)
 
     def lusgs(self, x):
         x = np.ascontiguousarray(x, np.complex128)

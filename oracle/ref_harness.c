@@ -4,9 +4,9 @@
  *
  * Usage: ref_harness <reference key=value flags> hmode=<mode> [hfreq=f]
  *        [hin=file] [hout=prefix]
- * Modes: grid, apply, lusgs, inject, extract, solve.  Binary outputs are
+ * Modes: grid, apply, lusgs, aphi, gmresprec, inject, extract, solve.  Binary outputs are
  * raw little-endian doubles / complex128 / complex64 named <prefix>_<what>.
- * airbc=1 is accepted for lusgs/inject/extract (the index shift only);
+ * airbc=1 is accepted for lusgs/aphi/inject/extract (the index shift only);
  * apply and solve need airbc=0 because airbc_apply lives in the file this
  * build cannot compile (no FFTW). */
 #include "cstd.h"
@@ -22,6 +22,10 @@ void extend_model_init(acq_t *acq, emf_t *emf);
 void fdfd_init(emf_t *emf);
 void fdfd_apply(int n, complex *x, complex *y);
 void lusgs_apply(int n, complex *x, complex *y);
+void aphi_apply(int n, complex *x, complex *y);
+void gmres_init(int n, int m, op_t Aop);
+void gmres_free();
+void gmres_apply(int n, complex *b, complex *x);
 void inject_source(acq_t *acq, emf_t *emf, complex *b);
 void extract_emf(acq_t *acq, emf_t *emf, complex *x, int ifreq);
 void write_data(acq_t *acq, emf_t *emf, char *fname, float _Complex ***dcal_fd);
@@ -140,6 +144,16 @@ int main(int argc, char **argv)
   } else if (!strcmp(mode, "lusgs")) {
     lusgs_apply(emf->ne, x, y);
     dump("y", y, emf->ne * sizeof(complex));
+  } else if (!strcmp(mode, "aphi")) {
+    if (emf->preco != 3) err("aphi needs preco=3 (fdfd_init allocates its potential then)");
+    aphi_apply(emf->ne, x, y);
+    dump("y", y, emf->ne * sizeof(complex));
+  } else if (!strcmp(mode, "gmresprec")) {
+    if (airbc) err("gmresprec needs airbc=0 in this build");
+    gmres_init(emf->ne, emf->mp, fdfd_apply);
+    gmres_apply(emf->ne, x, y);
+    gmres_free();
+    dump("y", y, emf->ne * sizeof(complex));
   } else if (!strcmp(mode, "inject")) {
     inject_source(acq, emf, y);
     dump("b", y, emf->ne * sizeof(complex));
@@ -171,6 +185,14 @@ int main(int argc, char **argv)
       } else if (emf->preco == 1) {
         if (emf->algopt == 1) fbicgstab(emf->ne, emf->xe, emf->be, fdfd_apply, lusgs_apply, emf->niter, tol, emf->verb);
         if (emf->algopt == 2) fgmres(emf->ne, emf->xe, emf->be, fdfd_apply, lusgs_apply, emf->niter / emf->nrestart, tol, emf->nrestart, emf->verb);
+      } else if (emf->preco == 2) { /* main.c:110-115 */
+        gmres_init(emf->ne, emf->mp, fdfd_apply);
+        if (emf->algopt == 1) fbicgstab(emf->ne, emf->xe, emf->be, fdfd_apply, gmres_apply, emf->niter, tol, emf->verb);
+        if (emf->algopt == 2) fgmres(emf->ne, emf->xe, emf->be, fdfd_apply, gmres_apply, emf->niter / emf->nrestart, tol, emf->nrestart, emf->verb);
+        gmres_free();
+      } else if (emf->preco == 3) {
+        if (emf->algopt == 1) fbicgstab(emf->ne, emf->xe, emf->be, fdfd_apply, aphi_apply, emf->niter, tol, emf->verb);
+        if (emf->algopt == 2) fgmres(emf->ne, emf->xe, emf->be, fdfd_apply, aphi_apply, emf->niter / emf->nrestart, tol, emf->nrestart, emf->verb);
       }
       snprintf(what, sizeof what, "x%d", ifreq);
       dump(what, emf->xe, emf->ne * sizeof(complex));

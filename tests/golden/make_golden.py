@@ -93,15 +93,22 @@ def ref_tiny():
             out[name + "_apply"] = np.fromfile(os.path.join(d, "ref_y"), np.complex128)
         O.run_ref(d, argv, "lusgs", hfreq=f, hin=os.path.join(d, "xin"))
         out[name + "_lusgs"] = np.fromfile(os.path.join(d, "ref_y"), np.complex128)
+        # the other preconditioners: A-phi (preco=3) on every case, inner
+        # GMRES (preco=2, mp=8) where the operator runs (airbc=0)
+        O.run_ref(d, argv + ["preco=3"], "aphi", hfreq=f, hin=os.path.join(d, "xin"))
+        out[name + "_aphi"] = np.fromfile(os.path.join(d, "ref_y"), np.complex128)
+        if not s.flags["airbc"]:
+            O.run_ref(d, argv + ["mp=8"], "gmresprec", hfreq=f, hin=os.path.join(d, "xin"))
+            out[name + "_gmresprec"] = np.fromfile(os.path.join(d, "ref_y"), np.complex128)
         O.run_ref(d, argv, "inject", hfreq=f)
         out[name + "_inject"] = np.fromfile(os.path.join(d, "ref_b"), np.complex128)
         O.run_ref(d, argv, "extract", hfreq=f, hin=os.path.join(d, "xin"))
         out[name + "_extract"] = np.stack([np.fromfile(os.path.join(d, "ref_rec%d" % c), np.complex64)
                                            for c in range(3)])
         if not s.flags["airbc"]:
-            for algopt, preco in ((1, 1), (1, 0), (2, 1), (2, 0)):
+            for algopt, preco in ((1, 1), (1, 0), (2, 1), (2, 0), (1, 2), (2, 2), (1, 3), (2, 3)):
                 argv2 = argv + ["preco=%d" % preco, "algopt=%d" % algopt, "niter=60", "nrestart=10",
-                                "freqs=0.75", "verb=1"]
+                                "freqs=0.75", "verb=1", "mp=8"]
                 stdout = O.run_ref(d, argv2, "solve")
                 tag = "%s_solve_p%d_a%d" % (name, preco, algopt)
                 out[tag] = np.fromfile(os.path.join(d, "ref_x0"), np.complex128)

@@ -15,10 +15,19 @@ output and against the oracle's full-size solves.
 * layered_64 (64^3, 0.25 Hz) and layered_100 (100^3, 0.25/0.75/1.25 Hz):
   the device solve in its chunked summation order equals the oracle in
   the same order bit for bit (SHA-256 of the solution vector, iteration
-  count); against the oracle in the reference's serial order (what the
-  reference computes) every receiver agrees to 1e-9 of its channel's
-  largest value (of the largest E value for Ey) and the iteration count
-  to +-2.
+  count).  In the reference's serial summation order
+  (csem_set_reduction(1)) the device solve of layered_64 equals the
+  oracle in that order -- what the reference computes -- bit for bit.
+* The two summation orders themselves: BiCGStab amplifies last-bit
+  differences of its dot products, so two solves that differ only in the
+  order of their sums follow different trajectories and stop at different
+  points below the same relative residual tol = 1e-8/f.  Measured on the
+  oracle (fixture): run_layered 166 / 166 iterations, layered_64 274 / 272,
+  layered_100 at 0.75 Hz 478 / 522 and at 1.25 Hz 427 / 455 (serial /
+  chunked); receivers apart by 1e-10 .. 2.5e-8 of the channel's largest
+  value.  The test bounds them by what the tolerance allows: receivers
+  within 5 tol of the channel's largest value (Ey, zero by symmetry on the
+  line, of the largest E value), iteration counts within 10 %.
 
 The oracle's results are fixtures (tests/golden/configs_oracle.npz,
 written by tests/golden/make_configs.py): the full-size oracle solves take
@@ -113,16 +122,19 @@ def _check_case(name, s, workdir):
             ser = "%s_f%d_serial" % (name, i)
             assert it == int(_fix(chk + "_iter")), (it, int(FIX[chk + "_iter"]))
             assert hashlib.sha256(x.tobytes()).hexdigest() == str(_fix(chk + "_sha256"))
-            assert abs(it - int(_fix(ser + "_iter"))) <= 2, (it, int(FIX[ser + "_iter"]))
+            its = int(_fix(ser + "_iter"))
+            print("  iterations: device %d, reference order %d" % (it, its))
+            assert abs(it - its) <= 0.1 * its, (it, its)
             # receivers in double: the chunked-order oracle's receivers are
             # those of this exact vector, so compare the serial ones to them
             rc, rs = _fix(chk + "_rec"), _fix(ser + "_rec")
             emax = np.abs(rs[[0, 2]]).max()
+            tol = _tol(s, f)
             for ch in range(3):
                 scale = emax if ch == 1 else np.abs(rs[ch]).max()
                 d = np.abs(rc[ch] - rs[ch]).max() / scale
-                print("  channel %d: serial vs device order %.3g of the channel scale" % (ch, d))
-                assert d <= 1e-9, (f, ch, d)
+                print("  channel %d: serial vs device order %.3g of the channel scale (tol %.2g)" % (ch, d, tol))
+                assert d <= 5 * tol, (f, ch, d)
             # and the float32 receivers extract_emf stores are these, rounded
             rec = ses.extract(x)
             assert np.array_equal(rec[:3], rc[:3].astype(np.complex64))
@@ -132,6 +144,31 @@ def _check_case(name, s, workdir):
 
 def test_layered_64_plain_fp64(workdir):
     _check_case("layered_64", MC.cases()["layered_64"], workdir)
+
+
+def test_layered_64_reference_order_bitwise(workdir):
+    """In the reference's serial summation order the device solve is the
+    reference's: SHA-256 of the solution and the iteration count equal the
+    oracle's serial-order solve (the oracle's serial order equals the
+    reference build bit for bit on every golden case)."""
+    from demo_matrixfree_csem_amd import _lib
+    s = MC.cases()["layered_64"]
+    ses = _session(s, workdir)
+    L = _lib.lib()
+    try:
+        f = s.freqs[0]
+        print("case layered_64 f=%g serial order" % f, flush=True)
+        ses.set_freq(f)
+        b = ses.inject()
+        L.csem_set_reduction(1)
+        try:
+            x, it = ses.solve(b, _tol(s, f), niter=int(s.flags["niter"]))
+        finally:
+            L.csem_set_reduction(0)
+        assert it == int(_fix("layered_64_f0_serial_iter"))
+        assert hashlib.sha256(x.tobytes()).hexdigest() == str(_fix("layered_64_f0_serial_sha256"))
+    finally:
+        ses.close()
 
 
 def test_layered_100_plain_fp64(workdir):

@@ -93,6 +93,41 @@ def test_golden_solves(golden, preco, algopt):
     assert np.array_equal(x, golden["%s_solve_p%d_a%d" % (name, preco, algopt)])
 
 
+@pytest.mark.parametrize("case", [c[0] for c in _cases()])
+def test_golden_other_preconditioners(golden, case):
+    """aphi_apply (fdfd.c:513-821) on every case, airbc 0 and 1, fvm 0 and
+    1; gmres_apply (gmres_init_free.c:58-137, mp = 8) where the reference
+    build can run the operator (airbc = 0)."""
+    name, kw, fl = [c for c in _cases() if c[0] == case][0]
+    s = surveys.tiny(**kw)
+    s.flags.update(fl)
+    orc = _oracle(s, nfft=64)
+    orc.set_freq(0.75)
+    x = _rand(orc.lE, 100)
+    assert np.array_equal(orc.aphi(x), golden[name + "_aphi"])
+    if name + "_gmresprec" in golden:
+        assert np.array_equal(orc.gmres_prec(x, mp=8), golden[name + "_gmresprec"])
+
+
+@pytest.mark.parametrize("preco,algopt", [(2, 1), (2, 2), (3, 1), (3, 2)])
+def test_golden_solves_other_preconditioners(golden, preco, algopt):
+    """main.c:110-117: F-BiCGStab and FGMRES(10) with the inner-GMRES (mp=8)
+    and A-phi preconditioners, 60 iterations, bit for bit"""
+    name, kw, fl = _cases()[0]
+    s = surveys.tiny(**kw)
+    s.flags.update(fl)
+    orc = _oracle(s)
+    orc.set_freq(0.75)
+    orc.set_mp(8)
+    tol = float(np.float32(np.float32(1e-8) / np.float32(0.75)))
+    b = orc.inject(0, *s.acq.sources[0][:3])
+    if algopt == 1:
+        x, it, _ = orc.fbicgstab(b, 60, tol, precond=preco)
+    else:
+        x, it, _ = orc.fgmres(b, 6, tol, 10, precond=preco)
+    assert np.array_equal(x, golden["%s_solve_p%d_a%d" % (name, preco, algopt)])
+
+
 @pytest.mark.skipif(not os.path.exists(O.REF_HARNESS), reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("seed,airbc,fvm", [(11, 0, 0), (12, 1, 0), (13, 1, 1), (14, 0, 1)])
 def test_live_reference(workdir, seed, airbc, fvm):
