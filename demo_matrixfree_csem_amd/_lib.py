@@ -106,7 +106,7 @@ HOST_SYMBOLS = [
 DEV_SYMBOLS = [
     "csem_last_error", "csem_freq_consts", "csem_model_create", "csem_model_destroy", "csem_model_len",
     "csem_model_full_len", "csem_model_bytes", "csem_apply", "csem_lusgs", "csem_curl_h", "csem_airbc",
-    "csem_fft2", "csem_dot", "csem_bicgstab", "csem_fgmres", "csem_gmres", "csem_scatter_add", "csem_gather",
+    "csem_fft2", "csem_dot", "csem_dot_noalloc", "csem_bicgstab", "csem_fgmres", "csem_gmres", "csem_scatter_add", "csem_gather",
     "csem_malloc", "csem_free", "csem_memcpy_h2d", "csem_memcpy_d2h", "csem_memset", "csem_copy", "csem_sync",
     "csem_kernel_timing", "csem_kernel_stats", "csem_set_reduction",
     "csem_vec_sub", "csem_vec_divr", "csem_vec_divz", "csem_vec_subscaled_dev", "csem_vec_addscaled",
@@ -139,6 +139,7 @@ def lib():
     L.csem_airbc.argtypes = [vp, vp, vp, vp, vp]
     L.csem_fft2.argtypes = [vp, vp, ip, vp]
     L.csem_dot.argtypes = [lp, vp, vp, vp, vp]
+    L.csem_dot_noalloc.argtypes = [lp, vp, vp, vp, vp]
     L.csem_bicgstab.argtypes = [vp, C.POINTER(FreqT), vp, vp, ip, dp, ip, vp, vp]
     L.csem_fgmres.argtypes = [vp, C.POINTER(FreqT), vp, vp, ip, dp, ip, ip, vp, vp]
     L.csem_gmres.argtypes = [vp, C.POINTER(FreqT), vp, vp, ip, dp, ip, vp, vp]

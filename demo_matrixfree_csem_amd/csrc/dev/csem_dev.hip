@@ -523,11 +523,12 @@ static int dot_launch(long n, const z2 *a0, const z2 *b0, const z2 *a1, const z2
   DotArgs g;
   g.n = n; g.a[0] = a0; g.b[0] = b0; g.a[1] = a1; g.b[1] = b1; g.partials = partials;
   long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
+  // no chunk (n = 0): the fold alone writes the zero sum
   if (a1) {
-    k_dot_partial<2><<<(unsigned)nch, RED_T, 0, s>>>(g);
+    if (nch) k_dot_partial<2><<<(unsigned)nch, RED_T, 0, s>>>(g);
     k_dot_final<2><<<1, RED_T, 0, s>>>(partials, nch, out);
   } else {
-    k_dot_partial<1><<<(unsigned)nch, RED_T, 0, s>>>(g);
+    if (nch) k_dot_partial<1><<<(unsigned)nch, RED_T, 0, s>>>(g);
     k_dot_final<1><<<1, RED_T, 0, s>>>(partials, nch, out);
   }
   HIP_OK(hipGetLastError());
@@ -612,6 +613,48 @@ extern "C" int csem_dot(long n, const void *a, const void *b, void *out, void *s
     g_dot_cap = nch;
   }
   return dot_launch(n, (const z2 *)a, (const z2 *)b, NULL, NULL, g_dot_partials, (double *)out, S(stream));
+}
+
+// csem_dot without allocating: the chunk partial sums go to a fixed device
+// array instead of a buffer allocated on first use.  `scratch` names the
+// array in device code only; host code uses the address hipGetSymbolAddress
+// returns for the current device (each device holds its own copy).
+#define CSEM_DOT_NOALLOC_CHUNKS 16384  // 16384 chunks of RED_CHUNK elements: n <= 2^26
+static __device__ __align__(256) double scratch[CSEM_DOT_NOALLOC_CHUNKS * 4];
+static double *g_scratch_dev[64];
+static std::mutex g_scratch_mu;
+
+static int scratch_ptr(double **out)
+{
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return csem_fail(-3, "csem_dot_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = NULL;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return csem_fail(-1, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_dev[dev] = (double *)p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+extern "C" int csem_dot_noalloc(long n, const void *a, const void *b, void *out, void *stream)
+{
+  if (n < 0 || n > (long)CSEM_DOT_NOALLOC_CHUNKS * RED_CHUNK)
+    return csem_fail(-3, "csem_dot_noalloc: n=%ld outside [0, %ld]", n, (long)CSEM_DOT_NOALLOC_CHUNKS * RED_CHUNK);
+  if (g_red_serial) {
+    DotArgs g;
+    g.n = n; g.a[0] = (const z2 *)a; g.b[0] = (const z2 *)b; g.a[1] = NULL; g.b[1] = NULL; g.partials = NULL;
+    k_dot_serial<<<1, 64, 0, S(stream)>>>(g, (double *)out);
+    HIP_OK(hipGetLastError());
+    return 0;
+  }
+  double *part = NULL;
+  int rc = scratch_ptr(&part);
+  if (rc) return rc;
+  return dot_launch(n, (const z2 *)a, (const z2 *)b, NULL, NULL, part, (double *)out, S(stream));
 }
 
 // ---------------------------------------------------------- BiCGStab

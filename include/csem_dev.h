@@ -75,6 +75,16 @@ int csem_fft2(csem_model *m, void *data, int dir, void *stream);
  * device pointer to two doubles. */
 int csem_dot(long n, const void *a, const void *b, void *out, void *stream);
 
+/* csem_dot for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  No counterpart in the reference.
+ * Same kernels and result as csem_dot, but the chunk partial sums live in
+ * a fixed device array of 512 KiB per device (held until the process
+ * ends) instead of a buffer allocated on first use; n above 2^26
+ * (67,108,864) is rejected with an error.  Calls on one device share that
+ * array: calls that overlap in time on one device (several streams or
+ * host threads) race and must be ordered by the caller. */
+int csem_dot_noalloc(long n, const void *a, const void *b, void *out, void *stream);
+
 /* Summation order of every dot product (csem_dot and the solvers):
  * 0 (default) the fixed chunked parallel order; 1 the reference's serial
  * left-to-right order (cdotprod.c:18), one dependent add per element on a
