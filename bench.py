@@ -122,6 +122,59 @@ def _lusgs_roofline(stats, dims, R):
             "bytes_per_application": int(per), "avg_us": round(avg_s * 1e6, 3), "applications": int(napp)}
 
 
+def apply_large(reps=20, copy_gbs=None, counters_only=False):
+    """The operator on the 180 x 120 x 120 anisotropic model (11.1 M
+    unknowns: the FP64 apply moves 443 MB, beyond the 256 MB Infinity
+    Cache, so every launch streams from HBM): `reps` launches of the FP64
+    operator and of the FP32 one at R = 1 and R = 3, timed per launch with
+    HIP events on their stream.  This replaces the process's active model
+    (run it last)."""
+    from demo_matrixfree_csem_amd import surveys, _lib
+    from demo_matrixfree_csem_amd.session import Session, freq_consts
+    L = _lib.lib()
+    s = surveys.vti_marine()
+    work = tempfile.mkdtemp(prefix="csem_bench_vti_")
+    surveys.write_files(work, s)
+    ses = Session(s.argv(work))
+    model, n = ses.model, ses.n
+    rng = np.random.default_rng(7)
+    hx = (rng.standard_normal(3 * n) + 1j * rng.standard_normal(3 * n)).astype(np.complex128)
+    dx, dy = C.c_void_p(), C.c_void_p()
+    for d in (dx, dy):
+        _lib.check(L.csem_malloc(C.byref(d), 3 * n * 16), "csem_malloc")
+    out = {"workload": "vti_marine 180x120x120 + nb=10 (n_op %d), 0.5 Hz" % n, "n_op": n}
+    try:
+        _lib.check(L.csem_memcpy_h2d(dx, hx.ctypes.data, 3 * n * 16, None), "upload")
+        fr = freq_consts(0.5)
+        frs = (_lib.FreqT * 3)(fr, fr, fr)
+        h32 = hx.astype(np.complex64)
+        _lib.check(L.csem_memcpy_h2d(dy, h32.ctypes.data, 3 * n * 8, None), "upload")
+        dy32 = C.c_void_p(dy.value + 3 * n * 8)
+        runs = [("fp64", lambda: L.csem_apply(model, C.byref(fr), dx, C.c_void_p(dx.value + n * 16), None), "k_apply")]
+        for R in (1, 3):
+            runs.append(("fp32_r%d" % R, (lambda R=R: L.csem_apply32(model, R, frs, dy, dy32, None)), "k_mx_apply"))
+        for name, fn, kern in runs:
+            _lib.check(fn(), name)  # warm (creates the FP32 state on first use)
+            L.csem_sync(None)
+            L.csem_kernel_timing(1)
+            for _ in range(reps):
+                _lib.check(fn(), name)
+            L.csem_sync(None)
+            st = _lib.kernel_stats()
+            L.csem_kernel_timing(0)
+            if not counters_only:
+                out[name] = _roofline(st, kern, copy_gbs)
+        rows, planes = C.c_int(), C.c_int()
+        for R in (1, 3):
+            if L.csem_mixed_tuning(model, R, C.byref(rows), C.byref(planes)) == 0:
+                out["fp32_r%d_tile" % R] = {"rows": rows.value, "planes": planes.value}
+    finally:
+        L.csem_free(dx)
+        L.csem_free(dy)
+        ses.close()
+    return out
+
+
 def cpu_baseline(survey, freq, iters, gpu_iters):
     """The oracle (serial C restatement of the reference path) on this
     host: `iters` LU-SGS F-BiCGStab iterations of one solve, scaled to
@@ -160,8 +213,14 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the mixed inner iterations eagerly instead of replaying a captured hipGraph")
     ap.add_argument("--kernels-only", type=int, default=0, metavar="ITERS",
-                    help="counter runs: one eager batched mixed solve of ITERS inner iterations and a 1 GiB device "
+                    help="counter runs: eager mixed solves of exactly ITERS inner iterations with every right-hand "
+                         "side working throughout (no convergence, no stagnation cut), one batched solve of the "
+                         "three frequencies (--kernels-r 3) or three single ones (--kernels-r 1), and a 1 GiB device "
                          "copy of known size (for scaling FETCH_SIZE / WRITE_SIZE), then exit; prints no bench line")
+    ap.add_argument("--kernels-r", type=int, default=1, choices=(1, 3))
+    ap.add_argument("--apply-only", action="store_true",
+                    help="counter runs: only the operator launches on the 180x120x120 model and the copy, then exit")
+    ap.add_argument("--no-large", action="store_true", help="skip the 180x120x120 operator measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this path has no multi-GPU form; independent solves would "
@@ -199,20 +258,38 @@ def main():
     rel = np.zeros(R)
     opts = _lib.MixedOpts.default(max_inner=args.niter, use_graph=0 if args.no_graph else 1)
 
-    if args.kernels_only:
-        ko = _lib.MixedOpts.default(max_inner=args.kernels_only, max_outer=1, check_every=args.kernels_only,
-                                    use_graph=0)
-        L.csem_mixed_solve(model, R, fr, dxm, db, tol_arr.ctypes.data, C.byref(ko), its.ctypes.data,
-                           rel.ctypes.data, None)
-        L.csem_sync(None)
+    def copy_1gib():
         ca, cb = C.c_void_p(), C.c_void_p()
         for d in (ca, cb):
             _lib.check(L.csem_malloc(C.byref(d), 2 ** 30), "csem_malloc")
         for _ in range(3):
             _lib.check(L.csem_copy(cb, ca, 2 ** 30, None), "csem_copy")
         L.csem_sync(None)
-        print(json.dumps({"kernels_only": args.kernels_only, "copy_bytes_each_way": 2 ** 30, "n_op": n}),
+        L.csem_free(ca)
+        L.csem_free(cb)
+
+    if args.kernels_only:
+        ko = _lib.MixedOpts.default(max_inner=args.kernels_only, max_outer=1, check_every=args.kernels_only,
+                                    use_graph=0, inner_tol=1e-300, stall_iters=-1)
+        never = np.zeros(R)  # tol 0: every right-hand side keeps working
+        if args.kernels_r == 3:
+            L.csem_mixed_solve(model, R, fr, dxm, db, never.ctypes.data, C.byref(ko), its.ctypes.data,
+                               rel.ctypes.data, None)
+        else:
+            for r in range(R):
+                L.csem_mixed_solve(model, 1, C.byref(fr[r]), C.c_void_p(dxm.value + r * n * 16),
+                                   C.c_void_p(db.value + r * n * 16), never[r:r + 1].ctypes.data, C.byref(ko),
+                                   its[r:r + 1].ctypes.data, rel[r:r + 1].ctypes.data, None)
+        L.csem_sync(None)
+        copy_1gib()
+        print(json.dumps({"kernels_only": args.kernels_only, "kernels_r": args.kernels_r,
+                          "copy_bytes_each_way": 2 ** 30, "n_op": n, "inner_iterations": [int(i) for i in its]}),
               flush=True)
+        return
+    if args.apply_only:
+        res = apply_large(counters_only=True)
+        copy_1gib()
+        print(json.dumps({"apply_only": res["n_op"], "copy_bytes_each_way": 2 ** 30}), flush=True)
         return
 
     def plain_step():
@@ -344,13 +421,22 @@ def main():
             tr.append(float("%.3e" % (np.linalg.norm(br - w) / np.linalg.norm(br))))
         L.csem_free(dw)
         plain["true_relres"] = tr
-    # per-kernel times of the mixed path: one more solve, launched eagerly
-    # (graph replays are invisible to the launch timers), outside the timed
-    # region
+    # per-kernel times of the mixed path, in the mode the value comes from:
+    # the same solves once more, launched eagerly (graph replays are
+    # invisible to the launch timers), one after the other so that no
+    # kernel shares the GPU with another solve's, outside the timed region.
+    # Byte totals count only the right-hand sides a batched launch works on.
     popts = _lib.MixedOpts.default(max_inner=args.niter, use_graph=0)
     L.csem_kernel_timing(1)
-    L.csem_mixed_solve(model, R, fr, dxm, db, tol_arr.ctypes.data, C.byref(popts), its.ctypes.data,
-                       rel.ctypes.data, None)
+    if best == "batch":
+        L.csem_mixed_solve(model, R, fr, dxm, db, tol_arr.ctypes.data, C.byref(popts), its.ctypes.data,
+                           rel.ctypes.data, None)
+    else:
+        for r in range(R):
+            itr, rr = np.zeros(1, np.int32), np.zeros(1)
+            L.csem_mixed_solve(model, 1, C.byref(fr[r]), C.c_void_p(dxs.value + r * n * 16),
+                               C.c_void_p(db.value + r * n * 16), tol_arr[r:r + 1].ctypes.data, C.byref(popts),
+                               itr.ctypes.data, rr.ctypes.data, None)
     L.csem_sync(None)
     stm = _lib.kernel_stats()
     L.csem_kernel_timing(0)
@@ -380,13 +466,16 @@ def main():
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": round(mixed["solves_per_s"] / plain["solves_per_s"], 3) if plain else None,
         "roofline": _roofline(stm, dominant, copy_gbs),
+        "roofline_what": "dominant kernel by time of the mixed solves in the mode of value (%s), per launch" % best,
         "apply_roofline": _roofline(stm, "k_mx_apply", copy_gbs),
         "apply_fp64_roofline": _roofline(stp, "k_apply", copy_gbs) if stp else None,
         "hbm_copy": {"achieved": round(copy_gbs, 1), "unit": "GB/s", "frac": round(copy_gbs / HBM_PEAK_GBS, 4),
                      "what": "k_copy16, 1 GiB device copy, bytes read + written / time"},
         "lusgs_roofline_mixed": _lusgs_roofline(stm, lu_dims, R),
         "lusgs_roofline_fp64": _lusgs_roofline(stp, lu_dims, 0) if stp else None,
-        "kernel_ms_mixed": {k: round(v[0] / args.steps, 3) for k, v in sorted(stm.items(), key=lambda kv: -kv[1][0])},
+        "kernel_ms_mixed": {k: round(v[0], 3) for k, v in sorted(stm.items(), key=lambda kv: -kv[1][0])},
+        "kernel_ms_mixed_what": "summed kernel time of one step's %d solves (%s mode), launched eagerly one "
+                                "solve after the other" % (R, best),
         "kernel_ms_plain": ({k: round(v[0] / args.steps, 3) for k, v in sorted(stp.items(), key=lambda kv: -kv[1][0])}
                             if stp else None),
     }
@@ -407,6 +496,9 @@ def main():
         out["serving"] = sst
     if not args.no_cpu and plain:
         out["cpu_baseline"] = cpu_baseline(survey, freqs[0], args.cpu_iters, plain["iterations"])
+    if not args.no_large:
+        ses.close()
+        out["apply_large"] = apply_large(copy_gbs=copy_gbs)
     print(json.dumps(out), flush=True)
 
 

@@ -65,6 +65,7 @@ static hipEvent_t kt_event()
 
 static std::mutex g_kt_mu;  // solves on several host threads record concurrently
 static thread_local int g_kt_paused;
+thread_local int g_kt_live;
 // timing is paused while this thread captures a stream into a graph
 void kt_pause(int paused) { g_kt_paused = paused; }
 
@@ -250,7 +251,6 @@ extern "C" int csem_model_create(const csem_model_desc *d, csem_model **out)
     m->nsk = off;
     if ((rc = dalloc(m, &m->smith, off))) goto fail;
     if ((rc = dalloc(m, &m->rec, mx))) goto fail;
-    if ((rc = dalloc(m, &m->trash, 2 * SGS_MAXT))) goto fail;
   }
   // cells outside every sweep range stay zero for the pre-passes
   if ((rc = dalloc(m, &m->W, m->fE))) goto fail;
@@ -399,10 +399,22 @@ static int run_apply(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   return run_apply_bufs(m, f, x, y, s, model_bufs(m));
 }
 
+// the sweep instance for a slice of nw waves (1..4: at most SGS_MAXT rows)
+template <int C, bool FWD>
+static void (*sgs2_kernel(bool fvm, int nw))(LuArgs)
+{
+  switch (nw) {
+    case 1: return fvm ? k_sgs2<C, FWD, true, 1> : k_sgs2<C, FWD, false, 1>;
+    case 2: return fvm ? k_sgs2<C, FWD, true, 2> : k_sgs2<C, FWD, false, 2>;
+    case 3: return fvm ? k_sgs2<C, FWD, true, 3> : k_sgs2<C, FWD, false, 3>;
+    default: return fvm ? k_sgs2<C, FWD, true, 4> : k_sgs2<C, FWD, false, 4>;
+  }
+}
+
 static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStream_t s)
 {
   LuArgs a;
-  a.x = x; a.y = y; a.W = m->W; a.rec = m->rec; a.trash = m->trash; a.smith = m->smith;
+  a.x = x; a.y = y; a.W = m->W; a.rec = m->rec; a.smith = m->smith;
   a.s11 = m->s11; a.s22 = m->s22; a.s33 = m->s33;
   a.d1 = m->d1; a.d2 = m->d2; a.d3 = m->d3; a.d1s = m->d1s; a.d2s = m->d2s; a.d3s = m->d3s;
   a.r1 = m->r1; a.r2 = m->r2; a.r3 = m->r3; a.r1s = m->r1s; a.r2s = m->r2s; a.r3s = m->r3s;
@@ -432,16 +444,14 @@ static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   unsigned gs[3];
   for (int c = 0; c < 3; c++) {
     tb[c] = ((g.A[c] + 63) / 64) * 64;
-    // two exchange rows, the fast table and the diagonal table
-    lds[c] = 2 * (size_t)(tb[c] + 2) * sizeof(z2) + (size_t)(g.B[c] + 2) * (sizeof(double4) + sizeof(double)) +
-             (size_t)(g.A[c] + g.B[c] + 1) * sizeof(int);
+    lds[c] = (size_t)sg2_lds_bytes(g.A[c], g.B[c], tb[c] / 64);
     gs[c] = 8 * ((g.ns[c] + 7) / 8);
   }
   double nc[3];
   for (int c = 0; c < 3; c++) nc[c] = (double)g.ns[c] * g.A[c] * g.B[c];
   // the fvm scaling is a template argument: a runtime branch in the sweep
   // loop would make the compiler drain the prefetched loads every step
-#define SWEEP(C, F) (m->fvm ? k_sgs<C, F, true> : k_sgs<C, F, false>)
+#define SWEEP(C, F) sgs2_kernel<C, F>(m->fvm != 0, tb[C] / 64)
   // bytes per swept cell each kernel must move: the pre-pass reads b and
   // the other components' W values it differences and writes the record
   // (only b for Ex forward); the sweep reads the record and the diagonal
@@ -450,17 +460,17 @@ static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   // application needs 144 bytes per unknown (forward Ex 40, Ey 56, Ez 72,
   // backward 88 each: b, sigma, the values read, the value written).
   KT(KT_LU_PRE, s, 32.0 * nc[0], (k_lu_pre<0, true><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_SGS_EX_F, s, 64.0 * nc[0], (SWEEP(0, true)<<<gs[0], tb[0], lds[0], s>>>(a)));
+  KT(KT_SGS_EX_F, s, 64.0 * nc[0], (SWEEP(0, true)<<<gs[0], tb[0] + 64, lds[0], s>>>(a)));
   KT(KT_LU_PRE, s, 112.0 * nc[1], (k_lu_pre<1, true><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_SGS_EY_F, s, 128.0 * nc[1], (SWEEP(1, true)<<<gs[1], tb[1], lds[1], s>>>(a)));
+  KT(KT_SGS_EY_F, s, 128.0 * nc[1], (SWEEP(1, true)<<<gs[1], tb[1] + 64, lds[1], s>>>(a)));
   KT(KT_LU_PRE, s, 128.0 * nc[2], (k_lu_pre<2, true><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_SGS_EZ_F, s, 128.0 * nc[2], (SWEEP(2, true)<<<gs[2], tb[2], lds[2], s>>>(a)));
+  KT(KT_SGS_EZ_F, s, 128.0 * nc[2], (SWEEP(2, true)<<<gs[2], tb[2] + 64, lds[2], s>>>(a)));
   KT(KT_LU_PRE, s, 144.0 * nc[2], (k_lu_pre<2, false><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_SGS_EZ_B, s, 144.0 * nc[2], (SWEEP(2, false)<<<gs[2], tb[2], lds[2], s>>>(a)));
+  KT(KT_SGS_EZ_B, s, 144.0 * nc[2], (SWEEP(2, false)<<<gs[2], tb[2] + 64, lds[2], s>>>(a)));
   KT(KT_LU_PRE, s, 144.0 * nc[1], (k_lu_pre<1, false><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_SGS_EY_B, s, 144.0 * nc[1], (SWEEP(1, false)<<<gs[1], tb[1], lds[1], s>>>(a)));
+  KT(KT_SGS_EY_B, s, 144.0 * nc[1], (SWEEP(1, false)<<<gs[1], tb[1] + 64, lds[1], s>>>(a)));
   KT(KT_LU_PRE, s, 144.0 * nc[0], (k_lu_pre<0, false><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_SGS_EX_B, s, 128.0 * nc[0], (SWEEP(0, false)<<<gs[0], tb[0], lds[0], s>>>(a)));
+  KT(KT_SGS_EX_B, s, 128.0 * nc[0], (SWEEP(0, false)<<<gs[0], tb[0] + 64, lds[0], s>>>(a)));
 #undef SWEEP
   HIP_OK(hipGetLastError());
   return 0;

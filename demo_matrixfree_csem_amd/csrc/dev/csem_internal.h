@@ -47,6 +47,13 @@ enum {
 };
 hipEvent_t kt_begin(hipStream_t s);
 void kt_end(int id, hipEvent_t a, hipStream_t s, double bytes);
+// Right-hand sides a batched launch works on, for the timers' byte counts:
+// the batch kernels skip right-hand sides whose inner solve is done, and
+// the host learns which are done at every convergence check, so it sets
+// the count of the others here (0: all R).  A right-hand side that finishes
+// between two checks is counted until the next one.
+extern thread_local int g_kt_live;
+static inline double kt_rhs(int R) { return g_kt_live > 0 && g_kt_live < R ? (double)g_kt_live : (double)R; }
 #define KT(id, s, bytes, launch)        \
   do {                                   \
     hipEvent_t kta_ = kt_begin(s);       \
@@ -66,7 +73,6 @@ struct csem_model {
   long nsk;        // swept cells, all components
   z2 *W;           // LU-SGS state on the full grid
   LuRec *rec;      // LU-SGS pre-pass records of one component, skewed
-  z2 *trash;       // LU-SGS stores of lanes off the diagonal
   double4 *smith;  // per-frequency diagonal, skewed Ex | Ey | Ez
   csem_freq smith_f;
   int smith_ok;

@@ -300,7 +300,6 @@ struct LuArgs {
   z2 *W;        // sweep state on the full grid, Ex | Ey | Ez
   z2 *y;        // result, operator grid
   LuRec *rec;   // pre-pass records of the current sweep, skewed
-  z2 *trash;    // 2*SGS_MAXT values that lanes off the diagonal store into
   const double4 *smith;  // (ratio, |denom|, |1/denom|, code), skewed Ex | Ey | Ez
   const double *s11, *s22, *s33;
   const double *d1, *d2, *d3, *d1s, *d2s, *d3s;
@@ -503,44 +502,140 @@ __global__ __launch_bounds__(256) void k_lu_pre(LuArgs a)
 #undef WY
 #undef WZ
 
-// The sweep kernels software-pipeline their global loads: each cell's
-// record and diagonal are loaded SGS_P diagonals ahead into a register
-// ring.  Per diagonal a wave issues few instructions: the fast axis'
-// spacings come from one LDS table entry, the skewed position of the
-// diagonal from another, the neighbour exchange is branch-free (the
-// exchange rows have zero entries at both ends), and rows off the diagonal
-// compute on a clamped cell and discard the result.
-// Dynamic LDS: two exchange rows of blockDim+2 complex values, the fast
-// table (B+2 double4), the volume table (B+2 doubles) and the diagonal
-// table (A+B+1 ints).
-#ifndef SGS_P
-#define SGS_P 4   // register ring depth (diagonals loaded ahead)
-#endif
-#ifndef SGS_U
-#define SGS_U 16  // diagonals per loop iteration
-#endif
 #define SGS_MAXT 256  // rows per slice: n3f-1 and n2-1 must not exceed this
 
-template <int C, bool FWD, bool FVM>
-__global__ __launch_bounds__(SGS_MAXT) void k_sgs(LuArgs a)
+// The sweep of one slice.  One thread per row s = 1 + t visits the fast
+// index f = d - s on diagonal d (the roles of +1 and -1 swap in a backward
+// sweep); per step it forms the cell's row from the record, the new value
+// of (s, f-1) in its own register and the new value of (s-1, f) of the
+// neighbouring row (fdfd.c:357-369, 383-395, 409-421, in that expression
+// order).  A diagonal costs no LDS round trip and no barrier:
+//   - the neighbour row's new value (s - 1, f) (forward; (s + 1, f)
+//     backward) comes from the neighbouring lane by a whole-wave DPP lane
+//     shift of its previous value; the shift's edge lane keeps the value
+//     handed over by the neighbouring wave;
+//   - consecutive waves run SG2_L diagonals apart (in the sweep's direction
+//     the wave whose rows come first leads); the row at a wave's edge goes
+//     through a small LDS ring, written at the step it is computed and read
+//     SG2_PD steps before it is used, and a barrier every SG2_BP steps
+//     orders every write before its read (SG2_L >= SG2_BP + SG2_PD - 1; a
+//     slot is rewritten only SG2_RING steps later, while the barriers keep
+//     every wave within SG2_BP steps of the others);
+//   - each step's record and diagonal are loaded SG2_P steps ahead into
+//     registers by raw buffer loads (a lane whose cell is off the slice
+//     gets an offset past the buffer: the load reads 0 and issues no
+//     traffic, and its result is discarded); its fast-axis spacings SG2_PC
+//     steps ahead from the LDS table;
+//   - the values do not go to memory from the computing waves: a wave's
+//     vector-memory counter retires loads and stores in issue order, so
+//     waiting for a prefetched record would also wait for every earlier
+//     store to complete.  Each step's values go to an LDS staging area
+//     instead (two halves of SG2_BP steps), and one more wave of the block
+//     stores each half to W and y (raw buffer stores, dropped for lanes off
+//     the slice) while the computing waves fill the other half.
+// All waves run the same number of steps and meet at the same barriers.
+#define SG2_BP 8
+#define SG2_PD 4
+#define SG2_L (SG2_BP + SG2_PD - 1)
+#define SG2_RING 32
+#define SG2_P 4
+#define SG2_PC 4
+#define SG2_U 32
+#define SG2_OOB ((int)0x80000000)  // a buffer offset past every buffer (all shorter than 2 GiB)
+static_assert(SG2_RING > SG2_L - SG2_PD + SG2_BP, "edge ring slot rewritten before it is read");
+static_assert(SG2_U % SG2_RING == 0 && SG2_U % SG2_BP == 0 && SG2_U % SG2_P == 0 && SG2_U % SG2_PD == 0 &&
+                  SG2_U % SG2_PC == 0,
+              "sweep unroll");
+typedef unsigned sg2_v4u __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t sg2_rsrc(const void *p, long len)
 {
-  extern __shared__ z2 xch[];
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)len, 0x00020000);
+}
+__device__ __forceinline__ z2 sg2_ld(__amdgpu_buffer_rsrc_t r, int ofs)
+{
+  return __builtin_bit_cast(z2, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, 0, 0));
+}
+__device__ __forceinline__ void sg2_st(z2 v, __amdgpu_buffer_rsrc_t r, int ofs)
+{
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(sg2_v4u, v), r, ofs, 0, 0);
+}
+// lane shift of a complex double; lanes with no source lane keep `old`
+template <int CTRL>
+__device__ __forceinline__ double sg2_dpp_keep(double old, double v)
+{
+  const unsigned long long o = (unsigned long long)__double_as_longlong(old);
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(b >> 32), CTRL,
+                                                            0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int CTRL>
+__device__ __forceinline__ z2 sg2_zdpp_keep(z2 old, z2 v)
+{
+  return zmk(sg2_dpp_keep<CTRL>(old.x, v.x), sg2_dpp_keep<CTRL>(old.y, v.y));
+}
+
+// LDS of a block of nw computing waves: the edge rings and the junk slots
+// the other lanes' ring stores land in, the value staging (2 halves x nw
+// waves x SG2_BP steps x 64 lanes), the fast table (B + 2 double4 and
+// B + 2 doubles) and the diagonal table (A + B + 1 ints)
+__host__ __device__ __forceinline__ long sg2_lds_bytes(int A, int B, int nw)
+{
+  return (long)nw * (2 * SG2_RING + 64) * 16 + 2L * nw * SG2_BP * 64 * 16 + (long)(B + 2) * 40 +
+         (long)(A + B + 1) * 4;
+}
+
+// Backward sweeps: the operator-grid cells of slice q outside the sweep
+// range are zero rows of the result.
+template <int C>
+__device__ void sg2_zero_frame(const LuArgs &a, int q, int t, int nt)
+{
+  const int n1 = a.n1, n2 = a.n2, n3 = a.n3f;
+  const z2 z0 = zmk(0, 0);
+  if (C == 0) {
+    const int nk = a.n3op + 1;
+    for (int w = t; w < nk * (n2 + 1); w += nt) {
+      const int kq = w / (n2 + 1), j = w % (n2 + 1), kf = kq + a.kofs;
+      if (!(kf >= 1 && kf <= n3 - 1 && j >= 1 && j <= n2 - 1)) a.y[id1(n1, n2, kq, j, q)] = z0;
+    }
+  } else if (C == 1) {
+    const int nk = a.n3op + 1;
+    for (int w = t; w < nk * (n1 + 1); w += nt) {
+      const int kq = w / (n1 + 1), i = w % (n1 + 1), kf = kq + a.kofs;
+      if (!(kf >= 1 && kf <= n3 - 1 && i >= 1 && i <= n1 - 1)) a.y[a.lEx + id2(n1, n2, kq, q, i)] = z0;
+    }
+  } else if (q - a.kofs >= 0) {
+    for (int w = t; w < (n2 + 1) * (n1 + 1); w += nt) {
+      const int jj = w / (n1 + 1), i = w % (n1 + 1);
+      if (!(jj >= 1 && jj <= n2 - 1 && i >= 1 && i <= n1 - 1))
+        a.y[a.lEx + a.lEy + id3(n1, n2, q - a.kofs, jj, i)] = z0;
+    }
+  }
+}
+
+template <int C, bool FWD, bool FVM, int NW>
+__global__ __launch_bounds__(64 * (NW + 1)) void k_sgs2(LuArgs a)
+{
+  extern __shared__ z2 xs2[];
   const int nslice = a.g.ns[C];
-  // consecutive slices go to one XCD (workgroups are dealt round-robin over
-  // the 8 XCDs), so neighbouring slices share lines in its L2
   const int per = (nslice + 7) / 8;
-  const int q = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int q = (blockIdx.x % 8) * per + blockIdx.x / 8;  // consecutive slices on one XCD
   if (q >= nslice) return;
   const int n1 = a.n1, n2 = a.n2;
-  const int t = threadIdx.x, nt = blockDim.x;
+  constexpr int nt = 64 * (NW + 1), lagmax = (NW - 1) * SG2_L;  // wave NW stores
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const int A = a.g.A[C], B = a.g.B[C], AB = A * B;
-  const int s = 1 + t;
-  const int sc = min(s, A);
-  const int XW = nt + 2;  // exchange row width
-  double4 *ftab = (double4 *)(xch + 2 * XW);
-  double *vtab = (double *)(ftab + (B + 2));  // fvm: the cell volume's fast-axis factor
+  const int nst = A + B - 1;
+  const int s = 1 + t, sc = min(s, A);
+  z2 *ring = xs2;                               // [NW][SG2_RING]
+  z2 *junk = xs2 + NW * SG2_RING;               // [NW][SG2_RING + 64]
+  z2 *stg = junk + NW * (SG2_RING + 64);        // [2][NW][SG2_BP][64]
+  double4 *ftab = (double4 *)(stg + 2 * NW * SG2_BP * 64);
+  double *vtab = (double *)(ftab + (B + 2));
   int *dtab = (int *)(vtab + (B + 2));
-  // fast table: (d*s at f-1 or f, its reciprocal, d at f, its reciprocal)
   {
     const double *g0 = (C == 0) ? a.d2s : a.d1s, *g1 = (C == 0) ? a.d2 : a.d1;
     const double *g2 = (C == 0) ? a.r2s : a.r1s, *g3 = (C == 0) ? a.r2 : a.r1;
@@ -550,118 +645,159 @@ __global__ __launch_bounds__(SGS_MAXT) void k_sgs(LuArgs a)
       ftab[u] = make_double4(g0[fs], g2[fs], g1[fc], g3[fc]);
       vtab[u] = (C == 0) ? a.d2[fc] : a.d1s[fc];
     }
-    // skewed position of row 0 on diagonal d: off(d) - lo(d)
     for (int d = t; d <= A + B; d += nt) dtab[d] = d >= 2 ? sk_off(d, A, B) - max(1, d - B) : 0;
-    for (int u = t; u < 2 * XW; u += nt) xch[u] = zmk(0, 0);
+    for (int u = t; u < NW * SG2_RING; u += nt) ring[u] = zmk(0, 0);
   }
   const double *h0 = (C == 2) ? a.d2s : a.d3s, *h1 = (C == 2) ? a.d2 : a.d3;
   const double *h2 = (C == 2) ? a.r2s : a.r3s, *h3 = (C == 2) ? a.r2 : a.r3;
   const int ssi = FWD ? sc - 1 : sc;  // slow-axis d*s of the swept neighbour
   const double ssx = h0[ssi], rssx = h2[ssi], sd = h1[sc], rsd = h3[sc];
-  // fvm volume d1s[i]*d2[j]*d3[k] (fdfd.c:364) = (vtab[f] * vA) * vB
   const double vA = (C == 0) ? a.d1s[q] : (C == 1) ? a.d2[q] : a.d2[sc];
   const double vB = (C == 2) ? a.d3[q] : a.d3[sc];
-  const LuRec *Rc = a.rec + (long)q * AB;
-  const double4 *Sm = a.smith + a.g.base[C] + (long)q * AB;
-  // natural offset of cell (q, s, f) = nb + f*FS with nb per row
+  // this slice's records and diagonals as raw buffers
+  const __amdgpu_buffer_rsrc_t Rb = sg2_rsrc(a.rec + (long)q * AB, (long)AB * (long)sizeof(LuRec));
+  const __amdgpu_buffer_rsrc_t Sb = sg2_rsrc(a.smith + a.g.base[C] + (long)q * AB, (long)AB * (long)sizeof(double4));
+  // natural offset of cell (q, s, f) = nb + f*FS within the component
   const long PL = (C == 0) ? (long)n1 * (n2 + 1) : (C == 1) ? (long)(n1 + 1) * n2 : (long)(n1 + 1) * (n2 + 1);
   const long SS = (C == 2) ? (long)(n1 + 1) : PL;
   const int FS = (C == 0) ? n1 : 1;
-  const long nb = ((C == 0) ? (long)q : (C == 1) ? (long)(n1 + 1) * q : PL * q) + s * SS;
+  const long nb = ((C == 0) ? (long)q : (C == 1) ? (long)(n1 + 1) * q : PL * q) + sc * SS;
   const long wofs = (C == 0) ? 0 : (C == 1) ? a.fEx : a.fEx + a.fEy;
   const long xofs = (C == 0) ? 0 : (C == 1) ? a.lEx : a.lEx + a.lEy;
-  z2 *Wc = a.W + wofs + nb;
-  const int kop = ((C == 2) ? q : s) - a.kofs;
-  z2 *Yc = a.y + xofs + nb - (long)a.kofs * PL;
-  const int dlo = 2, dhi = A + B, nst = dhi - dlo + 1;
+  const long wlen = (C == 0) ? a.fEx : (C == 1) ? a.fEy : (long)(n1 + 1) * (n2 + 1) * a.n3f;
+  const long ylen = (C == 0) ? a.lEx : (C == 1) ? a.lEy : (long)(n1 + 1) * (n2 + 1) * a.n3op;
+  const __amdgpu_buffer_rsrc_t Wb = sg2_rsrc(a.W + wofs, wlen * (long)sizeof(z2));
+  const __amdgpu_buffer_rsrc_t Yb = sg2_rsrc(a.y + xofs, ylen * (long)sizeof(z2));
+  const int kop = ((C == 2) ? q : sc) - a.kofs;
+  const long ynb = nb - (long)a.kofs * PL;
+  const int lag = (FWD ? wv : NW - 1 - wv) * SG2_L;
+  const int nex = ((nst + lagmax + SG2_BP - 1) / SG2_BP) * SG2_BP;  // steps run
+  const int ntk = ((nex + SG2_U - 1) / SG2_U) * SG2_U;
   const z2 z0 = zmk(0, 0);
-  constexpr bool NOREC = (C == 0) && FWD;  // Ex forward: Ey = Ez = 0, so R = 0
-  __syncthreads();
-
-  struct St { z2x4 R; z2 b; double4 S; };
-  auto load = [&](St &c, int st) {
-    const int d = min(FWD ? dlo + st : dhi - st, dhi);
-    const int e = min(max(dtab[max(d, dlo)] + sc, 0), AB - 1);
-    if (NOREC) c.b = Rc[e].b;
-    else { const LuRec &r = Rc[e]; c.R = r.R; c.b = r.b; }
-    c.S = Sm[e];
-  };
-  St ring[SGS_P];
+  if (wv == NW) {
+    // the store wave: after the barrier that ends half h (steps
+    // SG2_BP h .. SG2_BP h + SG2_BP - 1), store that half's values while the
+    // computing waves fill the other half
+    __syncthreads();  // the tables and rings are ready
+    const int nhalf = nex / SG2_BP;
+    for (int h = 0; h < nhalf; h++) {
+      __syncthreads();
+      const z2 *src = stg + (h & 1) * NW * SG2_BP * 64 + lane;
 #pragma unroll
-  for (int u = 0; u < SGS_P; u++) {
-    if (NOREC) ring[u].R = z2x4{z0, z0, z0, z0};
-    load(ring[u], u);
+      for (int w = 0; w < NW; w++) {
+        const int sw = 1 + 64 * w + lane, scw = min(sw, A);
+        const int lw = (FWD ? w : NW - 1 - w) * SG2_L;
+        const long nbw = ((C == 0) ? (long)q : (C == 1) ? (long)(n1 + 1) * q : PL * q) + scw * SS;
+        const int kopw = ((C == 2) ? q : scw) - a.kofs;
+#pragma unroll
+        for (int u = 0; u < SG2_BP; u++) {
+          const int tau = h * SG2_BP + u;
+          const int d = FWD ? 2 + tau - lw : A + B - tau + lw, f = d - sw;
+          const bool ok = sw <= A && f >= 1 && f <= B;
+          const z2 v = src[(w * SG2_BP + u) * 64];
+          if (FWD || C != 0) sg2_st(v, Wb, ok ? (int)(nbw + (long)f * FS) * (int)sizeof(z2) : SG2_OOB);
+          if (!FWD)
+            sg2_st(v, Yb, ok && kopw >= 0 ? (int)(nbw - (long)a.kofs * PL + (long)f * FS) * (int)sizeof(z2) : SG2_OOB);
+        }
+      }
+    }
+    if (!FWD) sg2_zero_frame<C>(a, q, t, nt);
+    return;
   }
+  const bool send = FWD ? (lane == 63 && wv < NW - 1) : (lane == 0 && wv > 0);
+  const int zl = __builtin_amdgcn_mbcnt_lo(0u, 0u);  // a lane-varying zero: the ring address stays per lane
+  z2 *ring_in = ring + wv * SG2_RING + zl;
+  z2 *ring_st = send ? ring + (FWD ? wv + 1 : wv - 1) * SG2_RING : junk + wv * (SG2_RING + 64) + lane;
+  z2 *stg_w = stg + wv * SG2_BP * 64 + lane;  // this lane's staging slot of step u of half 0
+  constexpr bool NOREC = (C == 0) && FWD;  // Ex forward: Ey = Ez = 0, so R = 0 (only b is stored)
+  // diagonal of step tau and whether this lane's cell (s, d - s) is on the slice
+  auto diag = [&](int tau) { return FWD ? 2 + tau - lag : A + B - tau + lag; };
+  auto onslice = [&](int d) {
+    const int f = d - s;
+    return s <= A && f >= 1 && f <= B;
+  };
+  struct Rg { z2 r0, r1, r2, r3, b; double4 S; };
+  auto load = [&](Rg &g, int tau) {
+    const int d = diag(tau);
+    const bool ok = onslice(d);
+    const int e = dtab[min(max(d, 2), A + B)] + s;
+    const int ro = ok ? e * (int)sizeof(LuRec) : SG2_OOB, so = ok ? e * (int)sizeof(double4) : SG2_OOB;
+    if (!NOREC) {
+      g.r0 = sg2_ld(Rb, ro);
+      g.r1 = sg2_ld(Rb, ro + 16);
+      g.r2 = sg2_ld(Rb, ro + 32);
+      g.r3 = sg2_ld(Rb, ro + 48);
+    }
+    g.b = sg2_ld(Rb, ro + 64);
+    const z2 s01 = sg2_ld(Sb, so), s23 = sg2_ld(Sb, so + 16);
+    g.S = make_double4(s01.x, s01.y, s23.x, s23.y);
+  };
+  __syncthreads();  // the tables and rings are ready (load reads the diagonal table)
+  Rg rg[SG2_P];
+#pragma unroll
+  for (int u = 0; u < SG2_P; u++) {
+    if (NOREC) rg[u].r0 = rg[u].r1 = rg[u].r2 = rg[u].r3 = z0;
+    load(rg[u], u);
+  }
+  double4 ftr[SG2_PC];
+  double vtr[SG2_PC];
+  z2 edg[SG2_PD];
+#pragma unroll
+  for (int u = 0; u < SG2_PC; u++) {
+    const int fi = min(max(diag(u) - s, 0), B + 1);
+    ftr[u] = ftab[fi];
+    vtr[u] = FVM ? vtab[fi] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < SG2_PD; u++) edg[u] = ring_in[(u - SG2_L - 1) & (SG2_RING - 1)];
   z2 prev = z0;
   const bool afast = (C != 1);
-  // the step count is padded to a multiple of SGS_U: padded steps lie off
-  // every row (f < 1 or f > B), load clamped cells and write only to the
-  // trash rows.  The loop body has no branch, so the compiler counts the
-  // outstanding loads instead of draining them; it still drains once per
-  // loop iteration (at the back edge), hence the long unroll.
-  const int nstp = ((nst + SGS_U - 1) / SGS_U) * SGS_U;
-  for (int st0 = 0; st0 < nstp; st0 += SGS_U) {
+  for (int tk0 = 0; tk0 < ntk; tk0 += SG2_U) {
 #pragma unroll
-    for (int uu = 0; uu < SGS_U; uu++) {
-      const int st = st0 + uu, u = uu % SGS_P;
-      {
-        const int d = FWD ? dlo + st : dhi - st;
-        const int f = d - s;
-        // new value of (s-1, f) (forward) or (s+1, f) (backward)
-        const z2 nbv = xch[((st + 1) & 1) * XW + (FWD ? t : t + 2)];
-        const double4 ft = ftab[min(max(f, 0), B + 1)];
-        const St &c = ring[u];
-        const double fsx = ft.x, rfsx = ft.y, fdf = ft.z, rfdf = ft.w;
-        z2 t1, t2, t3, t4;
-        if (FWD) {
-          const z2 qa = afast ? zdivq(zsub(z0, prev), fsx, rfsx) : zdivq(zsub(z0, nbv), ssx, rssx);
-          const z2 pb = afast ? zdivq(zsub(z0, nbv), ssx, rssx) : zdivq(zsub(z0, prev), fsx, rfsx);
-          t1 = c.R.a; t2 = zsub(c.R.b, qa); t3 = c.R.c; t4 = zsub(pb, c.R.d);
-        } else {
-          const z2 qa = afast ? zdivq(zsub(prev, z0), fsx, rfsx) : zdivq(zsub(nbv, z0), ssx, rssx);
-          const z2 pb = afast ? zdivq(zsub(nbv, z0), ssx, rssx) : zdivq(zsub(prev, z0), fsx, rfsx);
-          t1 = zsub(c.R.a, qa); t2 = c.R.b; t3 = zsub(pb, c.R.c); t4 = c.R.d;
-        }
-        const z2 tA = afast ? zdivq(zsub(t1, t2), fdf, rfdf) : zdivq(zsub(t1, t2), sd, rsd);
-        const z2 tB = afast ? zdivq(zsub(t3, t4), sd, rsd) : zdivq(zsub(t3, t4), fdf, rfdf);
-        z2 aij = zmul(zsub(tA, tB), a.iwmu);
-        if (FVM) aij = zmul(aij, zrmul((vtab[min(max(f, 0), B + 1)] * vA) * vB, a.iomu));
-        const z2 v = smith_finish(zsub(c.b, aij), c.S);
-        // branch-free: lanes off the diagonal store into the trash rows
-        const bool ok = s <= A && f >= 1 && f <= B;
-        const long o = (long)f * FS;
-        if (FWD || C != 0) *(ok ? Wc + o : a.trash + t) = v;
-        if (!FWD) *(ok && kop >= 0 ? Yc + o : a.trash + SGS_MAXT + t) = v;
-        prev.x = ok ? v.x : prev.x;  // per component: a select of the vector
-        prev.y = ok ? v.y : prev.y;  // struct itself goes through the stack
-        load(ring[u], st + SGS_P);
-        xch[(st & 1) * XW + t + 1] = prev;
+    for (int uu = 0; uu < SG2_U; uu++) {
+      const int tau = tk0 + uu;
+      const Rg &c = rg[uu % SG2_P];
+      const double4 ft = ftr[uu % SG2_PC];
+      const int d = diag(tau), f = d - s;
+      const bool ok = onslice(d);
+      // new value of (s-1, f) (forward) or (s+1, f) (backward)
+      const z2 nbv = FWD ? sg2_zdpp_keep<DPP_WAVE_SHR1>(edg[uu % SG2_PD], prev)
+                         : sg2_zdpp_keep<DPP_WAVE_SHL1>(edg[uu % SG2_PD], prev);
+      const double fsx = ft.x, rfsx = ft.y, fdf = ft.z, rfdf = ft.w;
+      z2 t1, t2, t3, t4;
+      if (FWD) {
+        const z2 qa = afast ? zdivq(zsub(z0, prev), fsx, rfsx) : zdivq(zsub(z0, nbv), ssx, rssx);
+        const z2 pb = afast ? zdivq(zsub(z0, nbv), ssx, rssx) : zdivq(zsub(z0, prev), fsx, rfsx);
+        t1 = c.r0; t2 = zsub(c.r1, qa); t3 = c.r2; t4 = zsub(pb, c.r3);
+      } else {
+        const z2 qa = afast ? zdivq(zsub(prev, z0), fsx, rfsx) : zdivq(zsub(nbv, z0), ssx, rssx);
+        const z2 pb = afast ? zdivq(zsub(nbv, z0), ssx, rssx) : zdivq(zsub(prev, z0), fsx, rfsx);
+        t1 = zsub(c.r0, qa); t2 = c.r1; t3 = zsub(pb, c.r2); t4 = c.r3;
+      }
+      const z2 tA = afast ? zdivq(zsub(t1, t2), fdf, rfdf) : zdivq(zsub(t1, t2), sd, rsd);
+      const z2 tB = afast ? zdivq(zsub(t3, t4), sd, rsd) : zdivq(zsub(t3, t4), fdf, rfdf);
+      z2 aij = zmul(zsub(tA, tB), a.iwmu);
+      if (FVM) aij = zmul(aij, zrmul((vtr[uu % SG2_PC] * vA) * vB, a.iomu));
+      const z2 v = smith_finish(zsub(c.b, aij), c.S);
+      stg_w[((uu / SG2_BP) & 1) * NW * SG2_BP * 64 + (uu % SG2_BP) * 64] = v;
+      prev.x = ok ? v.x : prev.x;
+      prev.y = ok ? v.y : prev.y;
+      ring_st[uu & (SG2_RING - 1)] = prev;
+      if (uu % SG2_BP == SG2_BP - 1) {
         __syncthreads();
+        if (tau + 1 >= nex) goto swept;  // the same step for every wave
       }
+      edg[uu % SG2_PD] = ring_in[(uu + SG2_PD - SG2_L - 1) & (SG2_RING - 1)];
+      {
+        const int fi = min(max(diag(tau + SG2_PC) - s, 0), B + 1);
+        ftr[uu % SG2_PC] = ftab[fi];
+        if (FVM) vtr[uu % SG2_PC] = vtab[fi];
+      }
+      load(rg[uu % SG2_P], tau + SG2_P);
     }
   }
-  if (!FWD) {  // cells of this slice outside the sweep range are zero rows
-    const int n3 = a.n3f;
-    if (C == 0) {
-      const int nk = a.n3op + 1;
-      for (int w = t; w < nk * (n2 + 1); w += nt) {
-        const int kq = w / (n2 + 1), j = w % (n2 + 1), kf = kq + a.kofs;
-        if (!(kf >= 1 && kf <= n3 - 1 && j >= 1 && j <= n2 - 1)) a.y[id1(n1, n2, kq, j, q)] = z0;
-      }
-    } else if (C == 1) {
-      const int nk = a.n3op + 1;
-      for (int w = t; w < nk * (n1 + 1); w += nt) {
-        const int kq = w / (n1 + 1), i = w % (n1 + 1), kf = kq + a.kofs;
-        if (!(kf >= 1 && kf <= n3 - 1 && i >= 1 && i <= n1 - 1)) a.y[a.lEx + id2(n1, n2, kq, q, i)] = z0;
-      }
-    } else if (q - a.kofs >= 0) {
-      for (int w = t; w < (n2 + 1) * (n1 + 1); w += nt) {
-        const int jj = w / (n1 + 1), i = w % (n1 + 1);
-        if (!(jj >= 1 && jj <= n2 - 1 && i >= 1 && i <= n1 - 1))
-          a.y[a.lEx + a.lEy + id3(n1, n2, q - a.kofs, jj, i)] = z0;
-      }
-    }
-  }
+swept:
+  if (!FWD) sg2_zero_frame<C>(a, q, t, nt);
 }
 
 // ----------------------------------------------------------- airwave FFT

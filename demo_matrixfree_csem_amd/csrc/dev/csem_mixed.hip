@@ -412,19 +412,19 @@ static int mx_apply(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
     const int nh = m->n1 * m->n2, N = m->nfft;
     MxAir a = mx_air(m, x, q);
     a.sc = sc;
-    KT(KT_M_HZ0, s, 24.0 * nh * R, (k_mx_hz0<<<dim3((nh + 255) / 256, R), 256, 0, s>>>(mx_geo(m, x), in, q->hz0, sc)));
-    KT(KT_M_AIR_ROWS, s, 8.0 * N * N * R,
+    KT(KT_M_HZ0, s, 24.0 * nh * kt_rhs(R), (k_mx_hz0<<<dim3((nh + 255) / 256, R), 256, 0, s>>>(mx_geo(m, x), in, q->hz0, sc)));
+    KT(KT_M_AIR_ROWS, s, 8.0 * N * N * kt_rhs(R),
        (k_mx_air_rows<<<dim3(N, R), 256, 3 * (size_t)N * sizeof(c2), s>>>(a)));
-    KT(KT_M_AIR_COLS, s, 40.0 * N * N * R,
+    KT(KT_M_AIR_COLS, s, 40.0 * N * N * kt_rhs(R),
        (k_mx_air_cols<<<dim3(8 * ((N / x->cw + 7) / 8), R), N * x->cw / 16, (2 * (size_t)N * x->cw + N) * sizeof(c2), s>>>(a)));
-    KT(KT_M_AIR_RINV, s, 16.0 * N * (m->nrowx + m->nrowy) * R,
+    KT(KT_M_AIR_RINV, s, 16.0 * N * (m->nrowx + m->nrowy) * kt_rhs(R),
        (k_mx_air_rows_inv<<<dim3(m->nrowx + m->nrowy, R), 256, 3 * (size_t)N * sizeof(c2), s>>>(a)));
     const int ng = (m->n1 + 1) * m->n2 + m->n1 * (m->n2 + 1);
-    KT(KT_M_AIR_GHOST, s, 8.0 * ng * R, (k_mx_air_ghost<<<dim3((ng + 255) / 256, R), 256, 0, s>>>(a)));
+    KT(KT_M_AIR_GHOST, s, 8.0 * ng * kt_rhs(R), (k_mx_air_ghost<<<dim3((ng + 255) / 256, R), 256, 0, s>>>(a)));
   }
   // algorithmic bytes: read E and write y per right-hand side, sigma once
   const int tj = x->ap_tj[R] ? x->ap_tj[R] : 6, kc = x->ap_kc[R] ? x->ap_kc[R] : 8;
-  KT(KT_M_APPLY, s, (16.0 * R + 4.0) * m->lE, mx_apply_kernel(m, x, q, R, fr, in, out, tj, kc, s, sc));
+  KT(KT_M_APPLY, s, (16.0 * kt_rhs(R) + 4.0) * m->lE, mx_apply_kernel(m, x, q, R, fr, in, out, tj, kc, s, sc));
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -501,24 +501,24 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
   // pre-pass reads b and the other components' W values it differences
   // and writes T; the sweep reads T and h and writes the value to V; the
   // copy reads V and writes W (not after the last sweep) and y (backward)
-  KT(KT_M_PRE, s, 16.0 * R * nc[0], (k_mx_pre<0, true, R><<<gqx, 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[0], (mx_sgs_launch<0, true>(gs[0], tb[0], lds[0], s, a)));
-  KT(KT_M_UNSKEW, s, 16.0 * R * nc[0], (k_mx_unskew<0, true, R><<<gu[0], 256, 0, s>>>(a)));
-  KT(KT_M_PRE, s, 24.0 * R * nc[1], (k_mx_pre<1, true, R><<<gq[1], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[1], (mx_sgs_launch<1, true>(gs[1], tb[1], lds[1], s, a)));
-  KT(KT_M_UNSKEW, s, 16.0 * R * nc[1], (k_mx_unskew<1, true, R><<<gu[1], 256, 0, s>>>(a)));
-  KT(KT_M_PRE, s, 32.0 * R * nc[2], (k_mx_pre<2, true, R><<<gq[2], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[2], (mx_sgs_launch<2, true>(gs[2], tb[2], lds[2], s, a)));
-  KT(KT_M_UNSKEW, s, 16.0 * R * nc[2], (k_mx_unskew<2, true, R><<<gu[2], 256, 0, s>>>(a)));
-  KT(KT_M_PRE, s, 40.0 * R * nc[2], (k_mx_pre<2, false, R><<<gq[2], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[2], (mx_sgs_launch<2, false>(gs[2], tb[2], lds[2], s, a)));
-  KT(KT_M_UNSKEW, s, 24.0 * R * nc[2], (k_mx_unskew<2, false, R><<<gu[2], 256, 0, s>>>(a)));
-  KT(KT_M_PRE, s, 40.0 * R * nc[1], (k_mx_pre<1, false, R><<<gq[1], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[1], (mx_sgs_launch<1, false>(gs[1], tb[1], lds[1], s, a)));
-  KT(KT_M_UNSKEW, s, 24.0 * R * nc[1], (k_mx_unskew<1, false, R><<<gu[1], 256, 0, s>>>(a)));
-  KT(KT_M_PRE, s, 40.0 * R * nc[0], (k_mx_pre<0, false, R><<<gq[0], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * R * nc[0], (mx_sgs_launch<0, false>(gs[0], tb[0], lds[0], s, a)));
-  KT(KT_M_UNSKEW, s, 16.0 * R * nc[0], (k_mx_unskew<0, false, R><<<gu[0], 256, 0, s>>>(a)));
+  KT(KT_M_PRE, s, 16.0 * kt_rhs(R) * nc[0], (k_mx_pre<0, true, R><<<gqx, 256, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * kt_rhs(R) * nc[0], (mx_sgs_launch<0, true>(gs[0], tb[0], lds[0], s, a)));
+  KT(KT_M_UNSKEW, s, 16.0 * kt_rhs(R) * nc[0], (k_mx_unskew<0, true, R><<<gu[0], 256, 0, s>>>(a)));
+  KT(KT_M_PRE, s, 24.0 * kt_rhs(R) * nc[1], (k_mx_pre<1, true, R><<<gq[1], 256, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * kt_rhs(R) * nc[1], (mx_sgs_launch<1, true>(gs[1], tb[1], lds[1], s, a)));
+  KT(KT_M_UNSKEW, s, 16.0 * kt_rhs(R) * nc[1], (k_mx_unskew<1, true, R><<<gu[1], 256, 0, s>>>(a)));
+  KT(KT_M_PRE, s, 32.0 * kt_rhs(R) * nc[2], (k_mx_pre<2, true, R><<<gq[2], 256, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * kt_rhs(R) * nc[2], (mx_sgs_launch<2, true>(gs[2], tb[2], lds[2], s, a)));
+  KT(KT_M_UNSKEW, s, 16.0 * kt_rhs(R) * nc[2], (k_mx_unskew<2, true, R><<<gu[2], 256, 0, s>>>(a)));
+  KT(KT_M_PRE, s, 40.0 * kt_rhs(R) * nc[2], (k_mx_pre<2, false, R><<<gq[2], 256, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * kt_rhs(R) * nc[2], (mx_sgs_launch<2, false>(gs[2], tb[2], lds[2], s, a)));
+  KT(KT_M_UNSKEW, s, 24.0 * kt_rhs(R) * nc[2], (k_mx_unskew<2, false, R><<<gu[2], 256, 0, s>>>(a)));
+  KT(KT_M_PRE, s, 40.0 * kt_rhs(R) * nc[1], (k_mx_pre<1, false, R><<<gq[1], 256, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * kt_rhs(R) * nc[1], (mx_sgs_launch<1, false>(gs[1], tb[1], lds[1], s, a)));
+  KT(KT_M_UNSKEW, s, 24.0 * kt_rhs(R) * nc[1], (k_mx_unskew<1, false, R><<<gu[1], 256, 0, s>>>(a)));
+  KT(KT_M_PRE, s, 40.0 * kt_rhs(R) * nc[0], (k_mx_pre<0, false, R><<<gq[0], 256, 0, s>>>(a)));
+  KT(KT_M_SGS, s, 24.0 * kt_rhs(R) * nc[0], (mx_sgs_launch<0, false>(gs[0], tb[0], lds[0], s, a)));
+  KT(KT_M_UNSKEW, s, 16.0 * kt_rhs(R) * nc[0], (k_mx_unskew<0, false, R><<<gu[0], 256, 0, s>>>(a)));
 }
 
 // The red-black ordered sweeps: forward red, black per component (Ex, Ey,
@@ -536,7 +536,7 @@ static void mx_rbsgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
   gq[2] = dim3((m->n1 + 1 + 63) / 64, (m->n2 + 1 + 3) / 4, m->n3f);
   const dim3 bq(64, 4);
   double h[3];
-  for (int c = 0; c < 3; c++) h[c] = 0.5 * g.ns[c] * g.A[c] * g.B[c] * R;
+  for (int c = 0; c < 3; c++) h[c] = 0.5 * g.ns[c] * g.A[c] * g.B[c] * kt_rhs(R);
   KT(KT_M_SGS, s, 16.0 * h[0], (k_mx_rb<0, true, 0, R><<<gq[0], bq, 0, s>>>(a)));
   KT(KT_M_SGS, s, 24.0 * h[0], (k_mx_rb<0, true, 1, R><<<gq[0], bq, 0, s>>>(a)));
   KT(KT_M_SGS, s, 24.0 * h[1], (k_mx_rb<1, true, 0, R><<<gq[1], bq, 0, s>>>(a)));
@@ -607,16 +607,16 @@ static int mx_iteration(csem_model *m, MixState *x, MixSlot *q, int R, const MxF
   const double *live = R > 1 ? k.sc : nullptr;
   if ((rc = mx_lusgs(m, x, q, R, fr, k.p, k.pp, s, live))) return rc;
   if ((rc = mx_apply(m, x, q, R, fr, k.pp, k.v, s, live))) return rc;
-  KT(KT_M_DOT, s, 16.0 * R * k.n, (k_mx_k_dot_vr0<<<gr, RED_T, 0, s>>>(k)));
+  KT(KT_M_DOT, s, 16.0 * kt_rhs(R) * k.n, (k_mx_k_dot_vr0<<<gr, RED_T, 0, s>>>(k)));
   KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 1)));
-  KT(KT_M_VEC, s, 24.0 * R * k.n, (k_mx_k_s<<<ge, 256, 0, s>>>(k)));
+  KT(KT_M_VEC, s, 24.0 * kt_rhs(R) * k.n, (k_mx_k_s<<<ge, 256, 0, s>>>(k)));
   if ((rc = mx_lusgs(m, x, q, R, fr, k.s, k.ss, s, live))) return rc;
   if ((rc = mx_apply(m, x, q, R, fr, k.ss, k.t, s, live))) return rc;
-  KT(KT_M_DOT, s, 16.0 * R * k.n, (k_mx_k_dot_st<<<gr, RED_T, 0, s>>>(k)));
+  KT(KT_M_DOT, s, 16.0 * kt_rhs(R) * k.n, (k_mx_k_dot_st<<<gr, RED_T, 0, s>>>(k)));
   KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 2)));
-  KT(KT_M_VEC, s, 64.0 * R * k.n, (k_mx_k_xr<<<gr, RED_T, 0, s>>>(k)));
+  KT(KT_M_VEC, s, 64.0 * kt_rhs(R) * k.n, (k_mx_k_xr<<<gr, RED_T, 0, s>>>(k)));
   KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 3)));
-  KT(KT_M_VEC, s, 32.0 * R * k.n, (k_mx_k_p<<<ge, 256, 0, s>>>(k)));
+  KT(KT_M_VEC, s, 32.0 * kt_rhs(R) * k.n, (k_mx_k_p<<<ge, 256, 0, s>>>(k)));
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -738,10 +738,13 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
       any |= act;
     }
     if (!any) break;
+    int nact = 0;
+    for (int r = 0; r < R; r++) nact += hsc[(size_t)r * MS_N + MS_DONE] == 0.0;
+    g_kt_live = nact;
     HIP_OK(hipMemcpyAsync(q->sc, hsc.data(), hsc.size() * sizeof(double), hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(q->scale, scl.data(), R * sizeof(float), hipMemcpyHostToDevice, s));
     KT(KT_M_REFINE, s, 24.0 * R * n, (k_mx_to32<<<dim3(gn, R), 256, 0, s>>>(n, q->res, q->scale, k.r)));
-    KT(KT_M_DOT, s, 32.0 * R * n, (k_mx_k_init<<<dim3(nch, R), RED_T, 0, s>>>(k)));
+    KT(KT_M_DOT, s, 32.0 * kt_rhs(R) * n, (k_mx_k_init<<<dim3(nch, R), RED_T, 0, s>>>(k)));
     KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 0)));
     if (o.use_graph && (rc = mx_graph(m, x, q, R, fr, s))) return rc;
     // inner iterations, convergence checked every check_every; an inner
@@ -774,6 +777,9 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
         }
         all &= sc[MS_DONE] != 0.0;
       }
+      int nlive = 0;
+      for (int r = 0; r < R; r++) nlive += hsc[(size_t)r * MS_N + MS_DONE] == 0.0;
+      g_kt_live = nlive;
       if (all) break;
     }
     for (int r = 0; r < R; r++)
@@ -808,6 +814,7 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
       relres[r] = bn[r] > 0 ? rn[r] / bn[r] : 0.0;
     }
   }
+  g_kt_live = 0;
   HIP_OK(hipGetLastError());
   int conv = 0;
   for (int r = 0; r < R; r++) conv += relres[r] < tol[r];
