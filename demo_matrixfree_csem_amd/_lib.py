@@ -102,6 +102,7 @@ HOST_SYMBOLS = [
     "csem_active_model", "csem_last_iterations", "csem_fdfd_prepare", "csem_model_from_emf",
     "iproc", "nproc", "ierr", "aphi_apply", "gmres_init", "gmres_free", "gmres_apply",
     "Acsr_init", "Acsr_free", "Acsr_apply", "Acsr_lusgs_apply", "Acoo_init", "Acoo_free", "Acoo_solve",
+    "extract_emf_double", "write_data_double",
 ]
 DEV_SYMBOLS = [
     "csem_last_error", "csem_freq_consts", "csem_model_create", "csem_model_destroy", "csem_model_len",
@@ -175,6 +176,8 @@ def lib():
     L.acq_free.argtypes = [C.POINTER(AcqT)]
     L.inject_source.argtypes = [C.POINTER(AcqT), C.POINTER(EmfT), vp]
     L.extract_emf.argtypes = [C.POINTER(AcqT), C.POINTER(EmfT), vp, ip]
+    L.extract_emf_double.argtypes = [C.POINTER(AcqT), C.POINTER(EmfT), vp, vp]
+    L.write_data_double.argtypes = [C.POINTER(AcqT), C.POINTER(EmfT), C.c_char_p, vp]
     L.write_data.argtypes = [C.POINTER(AcqT), C.POINTER(EmfT), C.c_char_p, vp]
     L.initargs.argtypes = [ip, C.POINTER(C.c_char_p)]
     L.fdfd_apply.argtypes = [ip, vp, vp]

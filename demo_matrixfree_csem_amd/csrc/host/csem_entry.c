@@ -952,48 +952,74 @@ static csem_complex extract_h(const emf_t *emf, const csem_complex *x, int c, do
  * sigma33*Ez is interpolated (at the unshifted plane index) and scaled by
  * rho_water.  Hx, Hy, Hz (beyond the reference, which leaves them zero)
  * come from extract_h. */
-void extract_emf(acq_t *acq, emf_t *emf, csem_complex *x, int ifreq)
+/* Receiver value of channel ch (0..5: Ex, Ey, Ez, Hx, Hy, Hz) at receiver
+ * ir, in double precision (before the float32 store of
+ * inject_extract.c:159,182,208). */
+static csem_complex rec_value(const acq_t *acq, const emf_t *emf, const csem_complex *x, int ch, int ir)
 {
   const int n1 = emf->nx + 2 * emf->nb, n2 = emf->ny + 2 * emf->nb, n3 = emf->nz + 2 * emf->nb;
-  int ic, ir, i, j, k, kk, q;
+  const csem_complex *Ex = x, *Ey = x + emf->lEx, *Ez = x + emf->lEx + emf->lEy;
+  int i, j, k, kk, q;
   double w1, w2, w3;
-  csem_complex s, *Ex = x, *Ey = x + emf->lEx, *Ez = x + emf->lEx + emf->lEy;
-  for (ic = 0; ic < emf->nchrec; ic++) {
-    const char *cn = emf->chrec[ic];
-    int ch = !strcmp(cn, "Ex") ? 0 : !strcmp(cn, "Ey") ? 1 : !strcmp(cn, "Ez") ? 2
-           : !strcmp(cn, "Hx") ? 3 : !strcmp(cn, "Hy") ? 4 : !strcmp(cn, "Hz") ? 5 : -1;
-    if (ch < 0) continue;
-    if (ch >= 3) {
-      int outside = 0;
-      for (ir = 0; ir < acq->nrec; ir++) {
-        s = extract_h(emf, x, ch - 3, acq->rec_x1[ir], acq->rec_x2[ir], acq->rec_x3[ir]);
-        outside += isnan(creal(s));
-        emf->dcal_fd[ic][ifreq][ir] = s;
-      }
-      if (outside)
-        fprintf(stderr, "extract_emf: %d receiver(s) too close to the grid's top or bottom for %s (NaN)\n", outside, cn);
-      continue;
-    }
-    for (ir = 0; ir < acq->nrec; ir++) {
-      const double *xa = ch == 0 ? emf->x1s : emf->x1, *ya = ch == 1 ? emf->x2s : emf->x2,
-                   *za = ch == 2 ? emf->x3s : emf->x3;
-      i = find_index(ch == 0 ? n1 : n1 + 1, (double *)xa, acq->rec_x1[ir]);
-      j = find_index(ch == 1 ? n2 : n2 + 1, (double *)ya, acq->rec_x2[ir]);
-      k = find_index(ch == 2 ? n3 : n3 + 1, (double *)za, acq->rec_x3[ir]);
-      w1 = (acq->rec_x1[ir] - xa[i]) / (xa[i + 1] - xa[i]);
-      w2 = (acq->rec_x2[ir] - ya[j]) / (ya[j + 1] - ya[j]);
-      w3 = (acq->rec_x3[ir] - za[k]) / (za[k + 1] - za[k]);
-      kk = k;
-      if (emf->airbc) k -= emf->nb;
-      s = 0;
-      for (q = 0; q < 8; q++) {
-        int di = q & 1, dj = (q >> 1) & 1, dk = q >> 2;
-        double f1 = di ? w1 : (1. - w1), f2 = dj ? w2 : (1. - w2), f3 = dk ? w3 : (1. - w3);
-        if (ch == 0) s += Ex[I1(k + dk, j + dj, i + di)] * f1 * f2 * f3;
-        else if (ch == 1) s += Ey[I2(k + dk, j + dj, i + di)] * f1 * f2 * f3;
-        else s += emf->sigma33[kk + dk][j + dj][i + di] * Ez[I3(k + dk, j + dj, i + di)] * f1 * f2 * f3;
-      }
-      emf->dcal_fd[ic][ifreq][ir] = (ch == 2) ? s * emf->rho_water : s;
-    }
+  csem_complex s;
+  if (ch >= 3) return extract_h(emf, x, ch - 3, acq->rec_x1[ir], acq->rec_x2[ir], acq->rec_x3[ir]);
+  const double *xa = ch == 0 ? emf->x1s : emf->x1, *ya = ch == 1 ? emf->x2s : emf->x2,
+               *za = ch == 2 ? emf->x3s : emf->x3;
+  i = find_index(ch == 0 ? n1 : n1 + 1, (double *)xa, acq->rec_x1[ir]);
+  j = find_index(ch == 1 ? n2 : n2 + 1, (double *)ya, acq->rec_x2[ir]);
+  k = find_index(ch == 2 ? n3 : n3 + 1, (double *)za, acq->rec_x3[ir]);
+  w1 = (acq->rec_x1[ir] - xa[i]) / (xa[i + 1] - xa[i]);
+  w2 = (acq->rec_x2[ir] - ya[j]) / (ya[j + 1] - ya[j]);
+  w3 = (acq->rec_x3[ir] - za[k]) / (za[k + 1] - za[k]);
+  kk = k;
+  if (emf->airbc) k -= emf->nb;
+  s = 0;
+  for (q = 0; q < 8; q++) {
+    int di = q & 1, dj = (q >> 1) & 1, dk = q >> 2;
+    double f1 = di ? w1 : (1. - w1), f2 = dj ? w2 : (1. - w2), f3 = dk ? w3 : (1. - w3);
+    if (ch == 0) s += Ex[I1(k + dk, j + dj, i + di)] * f1 * f2 * f3;
+    else if (ch == 1) s += Ey[I2(k + dk, j + dj, i + di)] * f1 * f2 * f3;
+    else s += emf->sigma33[kk + dk][j + dj][i + di] * Ez[I3(k + dk, j + dj, i + di)] * f1 * f2 * f3;
   }
+  return (ch == 2) ? s * emf->rho_water : s;
+}
+
+static int channel_of(const char *cn)
+{
+  return !strcmp(cn, "Ex") ? 0 : !strcmp(cn, "Ey") ? 1 : !strcmp(cn, "Ez") ? 2
+       : !strcmp(cn, "Hx") ? 3 : !strcmp(cn, "Hy") ? 4 : !strcmp(cn, "Hz") ? 5 : -1;
+}
+
+void extract_emf_double(acq_t *acq, emf_t *emf, csem_complex *x, csem_complex *out)
+{
+  int ic, ir, outside;
+  for (ic = 0; ic < emf->nchrec; ic++) {
+    const int ch = channel_of(emf->chrec[ic]);
+    outside = 0;
+    for (ir = 0; ir < acq->nrec; ir++) {
+      const csem_complex s = ch < 0 ? 0 : rec_value(acq, emf, x, ch, ir);
+      outside += ch >= 3 && isnan(creal(s));
+      out[(long)ic * acq->nrec + ir] = s;
+    }
+    if (outside)
+      fprintf(stderr, "extract_emf: %d receiver(s) too close to the grid's top or bottom for %s (NaN)\n", outside,
+              emf->chrec[ic]);
+  }
+}
+
+void extract_emf(acq_t *acq, emf_t *emf, csem_complex *x, int ifreq)
+{
+  int ic, ir;
+  csem_complex *v = (csem_complex *)malloc((size_t)emf->nchrec * acq->nrec * sizeof(csem_complex));
+  if (!v) {
+    csem_set_status(-2);
+    err("extract_emf: out of host memory");
+    return;
+  }
+  extract_emf_double(acq, emf, x, v);
+  for (ic = 0; ic < emf->nchrec; ic++) {
+    if (channel_of(emf->chrec[ic]) < 0) continue;
+    for (ir = 0; ir < acq->nrec; ir++) emf->dcal_fd[ic][ifreq][ir] = v[(long)ic * acq->nrec + ir];
+  }
+  free(v);
 }

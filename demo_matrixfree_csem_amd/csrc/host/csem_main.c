@@ -48,6 +48,10 @@ int main(int argc, char **argv)
   emf->be = alloc1complex(emf->ne);
   emf->dcal_fd = alloc3complexf(acq->nrec, emf->nfreq, emf->nchrec);
   memset(&emf->dcal_fd[0][0][0], 0, (size_t)acq->nrec * emf->nfreq * emf->nchrec * sizeof(float _Complex));
+  /* the receivers in double precision too (emf_0001_f64.txt): extract_emf
+   * rounds them to float as the reference does */
+  csem_complex *dval = (csem_complex *)calloc((size_t)emf->nchrec * emf->nfreq * acq->nrec, sizeof(csem_complex));
+  csem_complex *done = (csem_complex *)calloc((size_t)emf->nchrec * acq->nrec, sizeof(csem_complex));
   for (ifreq = 0; ifreq < emf->nfreq; ifreq++) {
     tol = emf->tol / emf->freqs[ifreq];
     emf->I_omega_mu0 = I * 2. * CSEM_PI * emf->freqs[ifreq] * CSEM_MU0;
@@ -73,8 +77,15 @@ int main(int argc, char **argv)
     }
     printf("elapsed time: %g sec\n", now() - t0);
     extract_emf(acq, emf, emf->xe, ifreq);
+    extract_emf_double(acq, emf, emf->xe, done);
+    for (int ic = 0; ic < emf->nchrec; ic++)
+      memcpy(dval + ((long)ic * emf->nfreq + ifreq) * acq->nrec, done + (long)ic * acq->nrec,
+             acq->nrec * sizeof(csem_complex));
   }
   write_data(acq, emf, "emf_0001.txt", emf->dcal_fd);
+  write_data_double(acq, emf, "emf_0001_f64.txt", dval);
+  free(dval);
+  free(done);
   free1complex(emf->xe);
   free1complex(emf->xh);
   free1complex(emf->be);

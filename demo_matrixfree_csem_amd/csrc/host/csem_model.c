@@ -511,6 +511,25 @@ void extend_model_free(emf_t *emf)
 
 /* write_data.c:15-39: header, then rows ordered channel, frequency,
  * receiver, values as float printed with %e */
+/* The same table with the receivers in double precision (%.17e, enough to
+ * round-trip every double): vals[(ic * nfreq + ifr) * nrec + ir], as
+ * extract_emf_double leaves them for each frequency. */
+void write_data_double(acq_t *acq, emf_t *emf, char *fname, csem_complex *vals)
+{
+  FILE *fp = fopen(fname, "w");
+  int ic, ifr, ir;
+  if (!fp) err("write_data_double: unable to create %s", fname);
+  if (!fp) return;
+  fprintf(fp, "iTx \t iRx    ichrec  ifreq \t emf_real \t emf_imag\n");
+  for (ic = 0; ic < emf->nchrec; ic++)
+    for (ifr = 0; ifr < emf->nfreq; ifr++)
+      for (ir = 0; ir < acq->nrec; ir++) {
+        const csem_complex v = vals[((long)ic * emf->nfreq + ifr) * acq->nrec + ir];
+        fprintf(fp, "%d \t %d \t %d \t %d \t %.17e \t %.17e\n", 1, ir + 1, ic + 1, ifr + 1, creal(v), cimag(v));
+      }
+  fclose(fp);
+}
+
 void write_data(acq_t *acq, emf_t *emf, char *fname, float _Complex ***dcal_fd)
 {
   FILE *fp = fopen(fname, "w");

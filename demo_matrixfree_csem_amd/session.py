@@ -135,6 +135,15 @@ class Session:
         arr = np.ctypeslib.as_array(C.cast(base, C.POINTER(C.c_float)), (total * 2,)).copy().view(np.complex64)
         return arr.reshape(e.nchrec, e.nfreq, a.nrec)[:, ifreq, :]
 
+    def extract_double(self, x):
+        """extract_emf_double: every receiver channel in double precision
+        (the values extract_emf rounds to float); complex128 (nchrec, nrec)."""
+        e, a = self.emf, self.acq
+        x = np.ascontiguousarray(x, np.complex128)
+        out = np.zeros(e.nchrec * a.nrec, np.complex128)
+        lib().extract_emf_double(C.byref(a), C.byref(e), _p(x), _p(out))
+        return out.reshape(e.nchrec, a.nrec)
+
     def inject_at(self, sx, sy, sz):
         """inject_source for a source at (sx, sy, sz) (the first source's
         slot of acq_t, acq_init_free.c:65-69, is repointed for the call)."""

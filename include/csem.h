@@ -127,6 +127,13 @@ csem_complex cdotprod(int n, csem_complex *a, csem_complex *b);
 void inject_source(acq_t *acq, emf_t *emf, csem_complex *b);
 void extract_emf(acq_t *acq, emf_t *emf, csem_complex *x, int ifreq);
 void write_data(acq_t *acq, emf_t *emf, char *fname, float _Complex ***dcal_fd);
+/* No counterpart in the reference, which keeps receivers only as float32
+ * (inject_extract.c:159,182,208): the double-precision values extract_emf
+ * rounds, out[ic * nrec + ir] for every receiver channel ic, and a result
+ * file in write_data's layout with them printed %.17e
+ * (vals[(ic * nfreq + ifreq) * nrec + ir]). */
+void extract_emf_double(acq_t *acq, emf_t *emf, csem_complex *x, csem_complex *out);
+void write_data_double(acq_t *acq, emf_t *emf, char *fname, csem_complex *vals);
 /* The reference's matrix-storage variants (main.c:148-151, 221-223: CSR
  * assembly with CSR LU-SGS, and a MUMPS direct solve, used only by its
  * main2 / main3).  Declared so that a reference main.c links; each call

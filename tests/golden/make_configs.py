@@ -18,6 +18,12 @@ Cases:
   layered_64   64^3 water + 1 ohm-m + 100 ohm-m slab, 0.25 Hz, lextend 40 km
   layered_100  100^3 with 8 seeded layers, 0.25 / 0.75 / 1.25 Hz (budget
                2000 iterations: 0.25 Hz needs more than 1000)
+  vti_marine   180 x 120 x 120 anisotropic model (11.1 M unknowns), 1.25 and
+               0.75 Hz of its six, six channels; F-BiCGStab in both orders
+               at 1.25 Hz and the chunked order at 0.75 Hz, and FGMRES(10)
+               in the chunked order at 1.25 Hz (key suffix _fgmres)
+  topo_marine  125 x 125 x 100 model with seafloor topography, its first
+               source, 1.25 Hz, chunked order
 
     python tests/golden/make_configs.py [-j 8]
 
@@ -62,7 +68,20 @@ def run_layered_survey():
 def cases():
     l100 = surveys.layered_100()
     l100.flags["niter"] = 2000
-    return {"run_layered": run_layered_survey(), "layered_64": surveys.layered_64(), "layered_100": l100}
+    vti = surveys.vti_marine()
+    vti.freqs = [1.25, 0.75]
+    topo = surveys.topo_marine()
+    topo.freqs = [1.25]
+    return {"run_layered": run_layered_survey(), "layered_64": surveys.layered_64(), "layered_100": l100,
+            "vti_marine": vti, "topo_marine": topo}
+
+
+# (frequency index, summation order, solver) per case; the default is every
+# frequency in both orders with F-BiCGStab
+TASKS = {
+    "vti_marine": [(0, 0, "bicgstab"), (0, 1, "bicgstab"), (1, 1, "bicgstab"), (0, 1, "fgmres")],
+    "topo_marine": [(0, 1, "bicgstab")],
+}
 
 
 CHANNELS = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
@@ -76,7 +95,7 @@ def oracle_for(s, nfft=1024):
 
 
 def solve(task):
-    name, ifreq, red = task
+    name, ifreq, red, algo = task
     s = cases()[name]
     orc = oracle_for(s)
     f = s.freqs[ifreq]
@@ -85,11 +104,15 @@ def solve(task):
     tol = float(np.float32(np.float32(s.flags["tol"]) / np.float32(f)))
     src = s.acq.sources[0]
     b = orc.inject(0, float(src[0]), float(src[1]), float(src[2]))
-    x, it, hist = orc.fbicgstab(b, int(s.flags["niter"]), tol)
+    if algo == "fgmres":
+        m = int(s.flags.get("nrestart", 10))
+        x, it, hist = orc.fgmres(b, int(s.flags["niter"]) // m, tol, m)
+    else:
+        x, it, hist = orc.fbicgstab(b, int(s.flags["niter"]), tol)
     chans = [CHANNELS[c] for c in s.flags["chrec"].split(",")]
     rec = np.array([[orc.extract(ch, float(r[0]), float(r[1]), float(r[2]), x) for r in s.acq.receivers]
                     for ch in chans])
-    print("%s f=%g red=%d iter=%d" % (name, f, red, it), flush=True)
+    print("%s f=%g red=%d %s iter=%d" % (name, f, red, algo, it), flush=True)
     return task, dict(iter=it, hist=hist, rec=rec, sha=hashlib.sha256(x.tobytes()).hexdigest(), tol=tol)
 
 
@@ -100,13 +123,18 @@ def main():
     a = ap.parse_args()
     cs = cases()
     names = a.only.split(",") if a.only else list(cs)
-    tasks = [(n, i, red) for n in names for i in range(len(cs[n].freqs)) for red in (O.RED_SERIAL, O.RED_CHUNKED)]
+    tasks = []
+    for n in names:
+        tl = TASKS.get(n) or [(i, red, "bicgstab") for i in range(len(cs[n].freqs))
+                              for red in (O.RED_SERIAL, O.RED_CHUNKED)]
+        tasks += [(n, i, red, algo) for i, red, algo in tl]
     # longest first
     tasks.sort(key=lambda t: -np.prod(cs[t[0]].model.shape) / cs[t[0]].freqs[t[1]])
     out = dict(np.load(OUT)) if os.path.exists(OUT) else {}
     with Pool(a.j) as pool:
-        for (n, i, red), r in pool.imap_unordered(solve, tasks):
-            key = "%s_f%d_%s" % (n, i, "serial" if red == O.RED_SERIAL else "chunked")
+        for (n, i, red, algo), r in pool.imap_unordered(solve, tasks):
+            key = "%s_f%d_%s%s" % (n, i, "serial" if red == O.RED_SERIAL else "chunked",
+                                   "_fgmres" if algo == "fgmres" else "")
             out[key + "_iter"] = np.int64(r["iter"])
             out[key + "_hist"] = r["hist"]
             out[key + "_rec"] = r["rec"]

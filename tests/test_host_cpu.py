@@ -73,6 +73,7 @@ def test_host_setup_equals_oracle(workdir, airbc, fvm):
         x = rng.standard_normal(ses.n) + 1j * rng.standard_normal(ses.n)
         want = np.array([[orc.extract(ch, *r[:3], x) for r in s.acq.receivers] for ch in range(3)])
         assert np.array_equal(ses.extract(x), want.astype(np.complex64))
+        assert np.array_equal(ses.extract_double(x), want)  # the double values before the float store
 
 
 def test_frequency_constants_match_reference_expression():
