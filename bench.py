@@ -44,30 +44,97 @@ METRIC = "solves/s @ tol 1e-8: batched mixed vs plain FP64 GPU; apply HBM TB/s; 
 HBM_PEAK_GBS = 8000.0
 
 
-def _traffic():
-    """HBM bytes per launch by kernel, from the FETCH_SIZE / WRITE_SIZE
-    counter runs summarised in profiles/hbm_traffic.json (absent: None)."""
+def _traffic(key="bytes_per_launch"):
+    """HBM bytes per launch by kernel (or, for the 180x120x120 operator
+    runs, by run), from the FETCH_SIZE / WRITE_SIZE counter runs summarised
+    in profiles/hbm_traffic.json (absent: {})."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if not os.path.exists(path):
         return {}
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch", {})
+        return json.load(f).get(key, {})
 
 
-def _roofline(stats, name, copy_gbs=None):
-    if name not in stats or stats[name][1] == 0:
+def _roofline(kl, name, copy_gbs=None):
+    """Roofline of kernel `name` from the roofline pass's timings `kl`:
+    algorithmic bytes per launch over the average launch duration, each
+    launch of the solve between its own HIP-event pair on its stream."""
+    k = kl.get(name) if kl else None
+    if not k or not k["launches"]:
         return None
-    ms, cnt, by = stats[name]
-    avg_s = ms / cnt / 1e3
-    per_launch = by / cnt
-    gbs = per_launch / avg_s / 1e9
+    gbs = k["bytes_per_launch"] / (k["event_us"] * 1e3)
     tr = _traffic().get(name)
     out = {"kernel": name, "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": int(tr) if tr else None,
-           "bytes_per_launch": int(per_launch), "avg_us": round(avg_s * 1e6, 3), "launches": cnt}
+           "bytes_per_launch": int(k["bytes_per_launch"]), "avg_us": k["event_us"], "launches": k["launches"]}
     if copy_gbs:
         out["frac_of_copy"] = round(gbs / copy_gbs, 4)
     return out
+
+
+# kernels also timed in back-to-back repeat loops (csem_kernel_repeat): the
+# kernels of an iteration whose output does not feed their own input
+LOOP_KERNELS = ("k_mx_apply", "k_mx_hz0", "k_mx_air_rows", "k_mx_air_cols", "k_mx_air_ghost", "k_mx_lu_pre",
+                "k_mx_sgs", "k_mx_unskew", "k_apply", "k_hz0", "k_air_rows_fwd", "k_air_cols", "k_air_ghost",
+                "k_lu_pre", "k_sgs_ex<fwd>", "k_sgs_ey<fwd>", "k_sgs_ez<fwd>", "k_sgs_ez<bwd>", "k_sgs_ey<bwd>",
+                "k_sgs_ex<bwd>", "k_dot_partial")
+LOOP_REPS, LOOP_LOOPS = 10, 24  # 24 launches: four LU-SGS applications' six of each kind
+
+
+def _loop_stats(st, looped):
+    """Per kernel: its launches and event-timed duration per launch and its
+    algorithmic bytes per launch; for the kernels in `looped`, the repeat
+    loops' time per launch and the bandwidth it gives."""
+    from demo_matrixfree_csem_amd import _lib
+    out = {}
+    for k, (ms, cnt, by) in st.items():
+        e = {"launches": cnt, "event_us": round(1e3 * ms / cnt, 3), "bytes_per_launch": int(by / cnt),
+             "event_gbs": round(by / cnt / (1e6 * ms / cnt), 1) if ms > 0 else None}
+        if k in looped:
+            lms, lcnt = _lib.kernel_loop_stats(k)
+            if lcnt:
+                e["loop_us"], e["loop_launches"] = round(1e3 * lms / lcnt, 3), lcnt
+                e["loop_gbs"] = round(e["bytes_per_launch"] / (1e3 * e["loop_us"]), 1)
+        out[k] = e
+    return out
+
+
+def roofline_pass(L, model, n, fr, db, dxm, dxp, iters=40, loops=False):
+    """Every kernel of one solve, timed per launch.  An eager
+    single-right-hand-side mixed solve of exactly `iters` inner iterations
+    (tol 0 and no stagnation cut: the right-hand side works throughout),
+    then `iters` plain FP64 F-BiCGStab iterations, with the kernel timers
+    on: each launch between its own HIP-event pair on its stream.  Measured
+    without a profiler, the pair spans the kernel and under 1 us more
+    (rocprof's durations of the same launches agree to about 1 %).
+    With `loops`, each of the first LOOP_LOOPS launches of every kernel in
+    LOOP_KERNELS is also followed by LOOP_REPS launches of it with the same
+    arguments, back to back between one event pair: the kernel with its
+    inputs warm in the caches (10-25 % faster than in the solve), reported
+    beside the per-launch figure, never in place of it.
+    Returns ({kernel: stats}, dominant mixed kernel by summed time)."""
+    from demo_matrixfree_csem_amd import _lib
+    L.csem_sync(None)
+    L.csem_kernel_timing(1)
+    if loops:
+        for k in LOOP_KERNELS:
+            _lib.check(L.csem_kernel_repeat(_lib.kernel_id(k), LOOP_REPS, LOOP_LOOPS), "csem_kernel_repeat")
+    ko = _lib.MixedOpts.default(max_inner=iters, max_outer=1, check_every=iters, use_graph=0, inner_tol=1e-300,
+                                stall_iters=-1)
+    its, rel, never = np.zeros(1, np.int32), np.zeros(1), np.zeros(1)
+    L.csem_mixed_solve(model, 1, C.byref(fr), dxm, db, never.ctypes.data, C.byref(ko), its.ctypes.data,
+                       rel.ctypes.data, None)
+    L.csem_sync(None)
+    mixed = _lib.kernel_stats()
+    L.csem_memset(dxp, 0, n * 16, None)
+    hist = np.zeros(iters + 2)
+    _lib.check(L.csem_bicgstab(model, C.byref(fr), dxp, db, iters, 0.0, 1, hist.ctypes.data, None), "csem_bicgstab")
+    L.csem_sync(None)
+    out = _loop_stats(_lib.kernel_stats(), LOOP_KERNELS if loops else ())
+    L.csem_kernel_timing(0)
+    timed = [k for k in mixed if k.startswith("k_mx_")]
+    dominant = max(timed, key=lambda k: mixed[k][0]) if timed else None
+    return out, dominant
 
 
 def _copy_rate(L, reps=10):
@@ -93,41 +160,40 @@ def _copy_rate(L, reps=10):
     return 2 * 2 ** 30 * reps / dt / 1e9
 
 
-def _lusgs_roofline(stats, dims, R):
-    """One LU-SGS application (its six pre-passes and six sweeps) against
-    the bytes the preconditioner needs per unknown: forward Ex 40, Ey 56,
-    Ez 72, backward 88 each (b, sigma, the values read, the value written)
-    in complex FP64, half that in complex FP32 per right-hand side (R > 0:
-    the batched FP32 path, R = 0: the FP64 path)."""
+def _lusgs_roofline(kl, dims, fp32):
+    """One LU-SGS application against the bytes the preconditioner needs
+    per unknown: forward Ex 40, Ey 56, Ez 72, backward 88 each (b, sigma,
+    the values read, the value written) in complex FP64, half that in
+    complex FP32 for the one right-hand side of the roofline pass.  The
+    application's time is the sum of its launches' event-timed durations:
+    FP32 six pre-passes, six sweeps and six copies to W/y (six times each
+    kernel's average over its six instances); FP64 six pre-passes and the
+    six sweeps."""
     n1, n2, n3f = dims
     nc = [n1 * (n3f - 1) * (n2 - 1), n2 * (n3f - 1) * (n1 - 1), n3f * (n2 - 1) * (n1 - 1)]
     per = float(sum(c * (f + 88) for c, f in zip(nc, (40, 56, 72))))
-    if R:
-        pre, sweeps = "k_mx_lu_pre", [k for k in ("k_mx_sgs", "k_mx_unskew") if k in stats]
-        per *= R / 2.0
-        if "k_mx_sgs" not in stats:
-            return None
-        napp = stats["k_mx_sgs"][1] / 6
+    if fp32:
+        parts = [("k_mx_lu_pre", 6), ("k_mx_sgs", 6), ("k_mx_unskew", 6)]
+        per /= 2.0
+        what = "one FP32 LU-SGS application (one right-hand side): 6 pre-passes + 6 sweeps + 6 copies to W/y"
     else:
-        pre, sweeps = "k_lu_pre", [k for k in stats if k.startswith("k_sgs")]
-        if "k_sgs_ex<fwd>" not in stats:
-            return None
-        napp = stats["k_sgs_ex<fwd>"][1]
-    ms = stats.get(pre, (0.0, 0, 0.0))[0] + sum(stats[k][0] for k in sweeps)
-    avg_s = ms / napp / 1e3
-    gbs = per / avg_s / 1e9
-    what = "one LU-SGS application: 6 pre-passes + 6 sweeps" + (" + 6 copies to W/y" if "k_mx_unskew" in sweeps else "")
-    return {"what": what, "bound": "hbm", "achieved": round(gbs, 2),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-            "bytes_per_application": int(per), "avg_us": round(avg_s * 1e6, 3), "applications": int(napp)}
+        parts = [("k_lu_pre", 6)] + [("k_sgs_%s<%s>" % (c, d), 1) for d in ("fwd", "bwd") for c in ("ex", "ey", "ez")]
+        what = "one FP64 LU-SGS application: 6 pre-passes + 6 sweeps"
+    if any(k not in kl for k, _ in parts):
+        return None
+    us = sum(m * kl[k]["event_us"] for k, m in parts)
+    gbs = per / (us * 1e3)
+    return {"what": what, "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_application": int(per), "avg_us": round(us, 3),
+            "parts_us": {k: round(m * kl[k]["event_us"], 3) for k, m in parts}}
 
 
-def apply_large(reps=20, copy_gbs=None, counters_only=False):
+def apply_large(reps=20, copy_gbs=None):
     """The operator on the 180 x 120 x 120 anisotropic model (11.1 M
     unknowns: the FP64 apply moves 443 MB, beyond the 256 MB Infinity
     Cache, so every launch streams from HBM): `reps` launches of the FP64
-    operator and of the FP32 one at R = 1 and R = 3, timed per launch with
-    HIP events on their stream.  This replaces the process's active model
+    operator and of the FP32 one at R = 1 and R = 3, each between its own
+    HIP-event pair on its stream.  This replaces the process's active model
     (run it last)."""
     from demo_matrixfree_csem_amd import surveys, _lib
     from demo_matrixfree_csem_amd.session import Session, freq_consts
@@ -160,10 +226,12 @@ def apply_large(reps=20, copy_gbs=None, counters_only=False):
             for _ in range(reps):
                 _lib.check(fn(), name)
             L.csem_sync(None)
-            st = _lib.kernel_stats()
+            kl = _loop_stats(_lib.kernel_stats(), ())
             L.csem_kernel_timing(0)
-            if not counters_only:
-                out[name] = _roofline(st, kern, copy_gbs)
+            out[name] = _roofline(kl, kern, copy_gbs)
+            tr = _traffic("apply_large_bytes_per_launch").get(name)
+            if out[name]:
+                out[name]["traffic"] = int(tr) if tr else None
         rows, planes = C.c_int(), C.c_int()
         for R in (1, 3):
             if L.csem_mixed_tuning(model, R, C.byref(rows), C.byref(planes)) == 0:
@@ -212,12 +280,12 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second of the serving run")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the mixed inner iterations eagerly instead of replaying a captured hipGraph")
-    ap.add_argument("--kernels-only", type=int, default=0, metavar="ITERS",
-                    help="counter runs: eager mixed solves of exactly ITERS inner iterations with every right-hand "
-                         "side working throughout (no convergence, no stagnation cut), one batched solve of the "
-                         "three frequencies (--kernels-r 3) or three single ones (--kernels-r 1), and a 1 GiB device "
-                         "copy of known size (for scaling FETCH_SIZE / WRITE_SIZE), then exit; prints no bench line")
-    ap.add_argument("--kernels-r", type=int, default=1, choices=(1, 3))
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the roofline pass (an eager single-right-hand-side mixed solve of 40 inner "
+                         "iterations and 40 plain FP64 iterations at the lowest frequency, every launch timed by "
+                         "its own event pair) and a 1 GiB device copy of known size (for scaling FETCH_SIZE / "
+                         "WRITE_SIZE); prints the pass's JSON instead of the bench line.  The command the "
+                         "rocprofv3 summaries and counter runs are taken on")
     ap.add_argument("--apply-only", action="store_true",
                     help="counter runs: only the operator launches on the 180x120x120 model and the copy, then exit")
     ap.add_argument("--no-large", action="store_true", help="skip the 180x120x120 operator measurement")
@@ -268,28 +336,17 @@ def main():
         L.csem_free(ca)
         L.csem_free(cb)
 
-    if args.kernels_only:
-        ko = _lib.MixedOpts.default(max_inner=args.kernels_only, max_outer=1, check_every=args.kernels_only,
-                                    use_graph=0, inner_tol=1e-300, stall_iters=-1)
-        never = np.zeros(R)  # tol 0: every right-hand side keeps working
-        if args.kernels_r == 3:
-            L.csem_mixed_solve(model, R, fr, dxm, db, never.ctypes.data, C.byref(ko), its.ctypes.data,
-                               rel.ctypes.data, None)
-        else:
-            for r in range(R):
-                L.csem_mixed_solve(model, 1, C.byref(fr[r]), C.c_void_p(dxm.value + r * n * 16),
-                                   C.c_void_p(db.value + r * n * 16), never[r:r + 1].ctypes.data, C.byref(ko),
-                                   its[r:r + 1].ctypes.data, rel[r:r + 1].ctypes.data, None)
-        L.csem_sync(None)
+    if args.roofline_only:
+        kl, dom = roofline_pass(L, model, n, fr[0], db, dxm, dxp)
         copy_1gib()
-        print(json.dumps({"kernels_only": args.kernels_only, "kernels_r": args.kernels_r,
-                          "copy_bytes_each_way": 2 ** 30, "n_op": n, "inner_iterations": [int(i) for i in its]}),
+        print(json.dumps({"roofline_only": args.workload, "n_op": n, "freq": freqs[0], "dominant": dom,
+                          "roofline": _roofline(kl, dom), "copy_bytes_each_way": 2 ** 30, "kernels": kl}),
               flush=True)
         return
     if args.apply_only:
-        res = apply_large(counters_only=True)
+        res = apply_large()
         copy_1gib()
-        print(json.dumps({"apply_only": res["n_op"], "copy_bytes_each_way": 2 ** 30}), flush=True)
+        print(json.dumps({"apply_only": res, "copy_bytes_each_way": 2 ** 30}), flush=True)
         return
 
     def plain_step():
@@ -348,28 +405,27 @@ def main():
             _lib.check(o[0], "csem_mixed_solve")
         return out
 
+    # the timed steps run with the kernel timers off: an event pair per
+    # launch would add the event packets' time to every launch
     def timed(fn):
         for _ in range(args.warmup):
             fn()
         L.csem_sync(None)
-        L.csem_kernel_timing(1)
         t0 = time.perf_counter()
         res = None
         for _ in range(args.steps):
             res = fn()
         L.csem_sync(None)
         dt = time.perf_counter() - t0
-        stats = _lib.kernel_stats()
-        L.csem_kernel_timing(0)
-        return dt, res, stats
+        return dt, res
 
     # solves/s counts the right-hand sides that end below their tol
-    dtm, (nconv, ncont), stm = timed(mixed_step)
+    dtm, (nconv, ncont) = timed(mixed_step)
     okm = int(sum(rel[r] < tols[r] for r in range(R)))
     mixed = {"batch": {"solves_per_s": round(okm * args.steps / dtm, 4), "ms_per_step": round(1e3 * dtm / args.steps, 3),
                        "converged": int(nconv), "fp64_continued": ncont, "inner_iterations": [int(i) for i in its],
                        "true_relres": [float("%.3e" % r) for r in rel]}, "tol": tols}
-    dts, sres, _ = timed(mixed_streams_step)
+    dts, sres = timed(mixed_streams_step)
     oks = int(sum(o[2] < tols[r] for r, o in enumerate(sres)))
     mixed["streams"] = {"solves_per_s": round(oks * args.steps / dts, 4), "ms_per_step": round(1e3 * dts / args.steps, 3),
                         "converged": sum(o[0] for o in sres), "fp64_continued": sum(o[3] for o in sres),
@@ -381,9 +437,9 @@ def main():
     rows, planes = C.c_int(), C.c_int()
     if L.csem_mixed_tuning(model, R, C.byref(rows), C.byref(planes)) == 0:
         mixed["autotuned_apply_tile"] = {"rows": rows.value, "planes": planes.value}
-    plain, stp = None, None
+    plain = None
     if not args.no_plain:
-        dtp, pits, stp = timed(plain_step)
+        dtp, pits = timed(plain_step)
         plain = {"solves_per_s": round(R * args.steps / dtp, 4), "ms_per_step": round(1e3 * dtp / args.steps, 3),
                  "iterations": pits}
         # receivers of both solutions (extract_emf on the host, channel-normalised difference)
@@ -424,8 +480,9 @@ def main():
     # per-kernel times of the mixed path, in the mode the value comes from:
     # the same solves once more, launched eagerly (graph replays are
     # invisible to the launch timers), one after the other so that no
-    # kernel shares the GPU with another solve's, outside the timed region.
-    # Byte totals count only the right-hand sides a batched launch works on.
+    # kernel shares the GPU with another solve's, outside the timed region;
+    # each launch between its own event pair.  Byte totals count only the
+    # right-hand sides a batched launch works on.
     popts = _lib.MixedOpts.default(max_inner=args.niter, use_graph=0)
     L.csem_kernel_timing(1)
     if best == "batch":
@@ -440,8 +497,11 @@ def main():
     L.csem_sync(None)
     stm = _lib.kernel_stats()
     L.csem_kernel_timing(0)
-
-    dominant = max(stm, key=lambda k: stm[k][0])
+    # the rooflines: every kernel of one eager solve timed per launch; then
+    # the same pass with warm-cache repeat loops, reported beside it
+    kl, dominant = roofline_pass(L, model, n, fr[0], db, dxm, dxp)
+    kw, _ = roofline_pass(L, model, n, fr[0], db, dxm, dxp, loops=True)
+    warm = {k: {x: v[x] for x in ("loop_us", "loop_launches", "loop_gbs")} for k, v in kw.items() if "loop_us" in v}
     copy_gbs = _copy_rate(L)
     dd = ses.dims
     lu_dims = (dd["n1"], dd["n2"], dd["nz"] + 2 * dd["nb"])
@@ -465,19 +525,28 @@ def main():
         "mixed": mixed,
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": round(mixed["solves_per_s"] / plain["solves_per_s"], 3) if plain else None,
-        "roofline": _roofline(stm, dominant, copy_gbs),
-        "roofline_what": "dominant kernel by time of the mixed solves in the mode of value (%s), per launch" % best,
-        "apply_roofline": _roofline(stm, "k_mx_apply", copy_gbs),
-        "apply_fp64_roofline": _roofline(stp, "k_apply", copy_gbs) if stp else None,
+        "roofline": _roofline(kl, dominant, copy_gbs),
+        "roofline_what": ("dominant kernel of the mixed solve by summed time (%s), in the roofline pass: an eager "
+                          "single-right-hand-side solve of %s at %g Hz, 40 inner iterations; duration = average "
+                          "launch, each between its own HIP-event pair on the solve's stream (the command "
+                          "bench.py --roofline-only runs the same pass alone)" % (dominant, args.workload, freqs[0])),
+        "apply_roofline": _roofline(kl, "k_mx_apply", copy_gbs),
+        "apply_fp64_roofline": _roofline(kl, "k_apply", copy_gbs),
         "hbm_copy": {"achieved": round(copy_gbs, 1), "unit": "GB/s", "frac": round(copy_gbs / HBM_PEAK_GBS, 4),
                      "what": "k_copy16, 1 GiB device copy, bytes read + written / time"},
-        "lusgs_roofline_mixed": _lusgs_roofline(stm, lu_dims, R),
-        "lusgs_roofline_fp64": _lusgs_roofline(stp, lu_dims, 0) if stp else None,
+        "lusgs_roofline_mixed": _lusgs_roofline(kl, lu_dims, True),
+        "lusgs_roofline_fp64": _lusgs_roofline(kl, lu_dims, False),
+        "kernels": kl,
+        "kernels_warm_loops": warm,
+        "kernels_warm_loops_what": ("the same pass with each of the first %d launches of a kernel followed by %d "
+                                    "back-to-back launches with the same arguments between one event pair "
+                                    "(csem_kernel_repeat): the kernel with its inputs warm in the caches"
+                                    % (LOOP_LOOPS, LOOP_REPS)),
         "kernel_ms_mixed": {k: round(v[0], 3) for k, v in sorted(stm.items(), key=lambda kv: -kv[1][0])},
+        "kernel_bytes_per_launch_mixed": {k: int(v[2] / v[1]) for k, v in stm.items() if v[1]},
         "kernel_ms_mixed_what": "summed kernel time of one step's %d solves (%s mode), launched eagerly one "
-                                "solve after the other" % (R, best),
-        "kernel_ms_plain": ({k: round(v[0] / args.steps, 3) for k, v in sorted(stp.items(), key=lambda kv: -kv[1][0])}
-                            if stp else None),
+                                "solve after the other, each launch between its own event pair (which adds the "
+                                "event packets' few microseconds to every launch)" % (R, best),
     }
     if args.serve_requests > 0:
         # serving: a warm engine (models resident) fed seeded requests on

@@ -109,7 +109,7 @@ DEV_SYMBOLS = [
     "csem_model_full_len", "csem_model_bytes", "csem_apply", "csem_lusgs", "csem_curl_h", "csem_airbc",
     "csem_fft2", "csem_dot", "csem_dot_noalloc", "csem_bicgstab", "csem_fgmres", "csem_gmres", "csem_scatter_add", "csem_gather",
     "csem_malloc", "csem_free", "csem_memcpy_h2d", "csem_memcpy_d2h", "csem_memset", "csem_copy", "csem_sync",
-    "csem_kernel_timing", "csem_kernel_stats", "csem_set_reduction",
+    "csem_kernel_timing", "csem_kernel_stats", "csem_kernel_repeat", "csem_kernel_loop_stats", "csem_set_reduction",
     "csem_vec_sub", "csem_vec_divr", "csem_vec_divz", "csem_vec_subscaled_dev", "csem_vec_addscaled",
     "csem_op_apply", "csem_op_lusgs", "csem_op_aphi", "csem_op_gmres", "csem_op_host",
     "csem_gmres_prec_release", "csem_bicgstab_op", "csem_gmres_op",
@@ -166,6 +166,8 @@ def lib():
     L.csem_mixed_bytes.restype = lp
     L.csem_set_reduction.argtypes = [ip]
     L.csem_kernel_stats.argtypes = [ip, C.POINTER(C.c_char_p), C.POINTER(dp), C.POINTER(lp), C.POINTER(dp)]
+    L.csem_kernel_repeat.argtypes = [ip, ip, ip]
+    L.csem_kernel_loop_stats.argtypes = [ip, C.POINTER(dp), C.POINTER(lp)]
     L.csem_active_model.restype = vp
     L.csem_fdfd_prepare.argtypes = [C.POINTER(EmfT)]
     L.csem_model_from_emf.argtypes = [C.POINTER(EmfT), C.POINTER(vp)]
@@ -235,6 +237,28 @@ def kernel_stats():
             out[name.value.decode()] = (ms.value, cnt.value, by.value)
         i += 1
     return out
+
+
+def kernel_id(name):
+    """The kernel timers' id of a kernel name (as kernel_stats names it)."""
+    L = lib()
+    i, n = 0, 1
+    while i < n:
+        nm, ms, cnt, by = C.c_char_p(), C.c_double(), C.c_long(), C.c_double()
+        n = L.csem_kernel_stats(i, C.byref(nm), C.byref(ms), C.byref(cnt), C.byref(by))
+        if n < 0:
+            break
+        if nm.value.decode() == name:
+            return i
+        i += 1
+    raise KeyError(name)
+
+
+def kernel_loop_stats(name):
+    """(total_ms, launches) of the repeat loops of kernel `name`."""
+    ms, cnt = C.c_double(), C.c_long()
+    check(lib().csem_kernel_loop_stats(kernel_id(name), C.byref(ms), C.byref(cnt)), "csem_kernel_loop_stats")
+    return ms.value, cnt.value
 
 
 def fn_addr(name):

@@ -46,9 +46,10 @@ struct KtPend {
 static struct {
   int on;
   std::vector<hipEvent_t> pool;
-  std::vector<KtPend> pend;
-  double ms[KT_N], bytes[KT_N];
-  long cnt[KT_N];
+  std::vector<KtPend> pend, loops;  // loops: bytes holds the launch count
+  double ms[KT_N], bytes[KT_N], loop_ms[KT_N];
+  long cnt[KT_N], loop_cnt[KT_N];
+  int rep_n[KT_N], rep_left[KT_N];  // csem_kernel_repeat
 } g_kt;
 
 static hipEvent_t kt_event()
@@ -69,23 +70,63 @@ thread_local int g_kt_live;
 // timing is paused while this thread captures a stream into a graph
 void kt_pause(int paused) { g_kt_paused = paused; }
 
-hipEvent_t kt_begin(hipStream_t s)
+bool kt_events(hipEvent_t *a, hipEvent_t *b)
 {
-  if (!g_kt.on || g_kt_paused) return NULL;
+  if (!g_kt.on || g_kt_paused) return false;
   std::lock_guard<std::mutex> lk(g_kt_mu);
-  hipEvent_t e = kt_event();
-  if (e) (void)hipEventRecord(e, s);
-  return e;
+  *a = kt_event();
+  if (!*a) return false;
+  *b = kt_event();
+  if (!*b) {
+    g_kt.pool.push_back(*a);
+    return false;
+  }
+  return true;
 }
 
-void kt_end(int id, hipEvent_t a, hipStream_t s, double bytes)
+void kt_record(int id, hipEvent_t a, hipEvent_t b, double bytes)
 {
-  if (!g_kt.on || !a) return;
   std::lock_guard<std::mutex> lk(g_kt_mu);
-  hipEvent_t b = kt_event();
-  if (!b) return;
-  (void)hipEventRecord(b, s);
   g_kt.pend.push_back({id, a, b, bytes});
+}
+
+// kernels whose output does not feed their own input: repeating a launch
+// leaves every result unchanged (the inverse row transforms work in place
+// and the Krylov kernels update their vectors, so they are not here)
+static bool kt_repeatable(int id)
+{
+  switch (id) {
+    case KT_APPLY: case KT_HZ0: case KT_AIR_ROWS_FWD: case KT_AIR_COLS: case KT_AIR_GHOST:
+    case KT_LU_PRE: case KT_LU_SMITH: case KT_DOT:
+    case KT_SGS_EX_F: case KT_SGS_EY_F: case KT_SGS_EZ_F: case KT_SGS_EZ_B: case KT_SGS_EY_B: case KT_SGS_EX_B:
+    case KT_M_APPLY: case KT_M_HZ0: case KT_M_AIR_ROWS: case KT_M_AIR_COLS: case KT_M_AIR_GHOST:
+    case KT_M_PRE: case KT_M_SGS: case KT_M_UNSKEW:
+      return true;
+    default:
+      return false;
+  }
+}
+
+int kt_repeat(int id, hipEvent_t *a, hipEvent_t *b)
+{
+  if (!g_kt.on || g_kt_paused) return 0;
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  if (g_kt.rep_left[id] <= 0 || g_kt.rep_n[id] <= 0) return 0;
+  *a = kt_event();
+  if (!*a) return 0;
+  *b = kt_event();
+  if (!*b) {
+    g_kt.pool.push_back(*a);
+    return 0;
+  }
+  g_kt.rep_left[id]--;
+  return g_kt.rep_n[id];
+}
+
+void kt_record_loop(int id, hipEvent_t a, hipEvent_t b, int n)
+{
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  g_kt.loops.push_back({id, a, b, (double)n});
 }
 
 static void kt_collect()
@@ -102,13 +143,47 @@ static void kt_collect()
     g_kt.pool.push_back(p.b);
   }
   g_kt.pend.clear();
+  for (auto &p : g_kt.loops) {
+    float ms = 0;
+    (void)hipEventSynchronize(p.b);
+    (void)hipEventElapsedTime(&ms, p.a, p.b);
+    g_kt.loop_ms[p.id] += ms;
+    g_kt.loop_cnt[p.id] += (long)p.bytes;
+    g_kt.pool.push_back(p.a);
+    g_kt.pool.push_back(p.b);
+  }
+  g_kt.loops.clear();
 }
 
 extern "C" int csem_kernel_timing(int on)
 {
   kt_collect();
+  std::lock_guard<std::mutex> lk(g_kt_mu);
   g_kt.on = on;
-  for (int i = 0; i < KT_N; i++) { g_kt.ms[i] = 0; g_kt.bytes[i] = 0; g_kt.cnt[i] = 0; }
+  for (int i = 0; i < KT_N; i++) {
+    g_kt.ms[i] = 0; g_kt.bytes[i] = 0; g_kt.cnt[i] = 0;
+    g_kt.loop_ms[i] = 0; g_kt.loop_cnt[i] = 0;
+    g_kt.rep_n[i] = g_kt.rep_left[i] = 0;
+  }
+  return 0;
+}
+
+extern "C" int csem_kernel_repeat(int id, int reps, int loops)
+{
+  if (id < 0 || id >= KT_N || !kt_repeatable(id) || reps < 0 || loops < 0)
+    return csem_fail(-1, "csem_kernel_repeat: kernel %d cannot repeat (its output feeds its input) or bad count", id);
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  g_kt.rep_n[id] = reps;
+  g_kt.rep_left[id] = loops;
+  return 0;
+}
+
+extern "C" int csem_kernel_loop_stats(int id, double *total_ms, long *count)
+{
+  if (id < 0 || id >= KT_N) return -1;
+  kt_collect();
+  *total_ms = g_kt.loop_ms[id];
+  *count = g_kt.loop_cnt[id];
   return 0;
 }
 
@@ -359,12 +434,12 @@ static int run_airbc(csem_model *m, hipStream_t s, const AirBufs &b)
   const int N = m->nfft;
   const size_t row_lds = 2 * (size_t)N * sizeof(z2);
   const double NN = (double)N * N;
-  KT(KT_AIR_ROWS_FWD, s, 16 * NN, (k_air_rows_fwd<<<N, 256, row_lds, s>>>(a)));
-  KT(KT_AIR_COLS, s, 80 * NN, (k_air_cols<2><<<8 * ((N / 2 + 7) / 8), 256, 3 * 2 * (size_t)N * sizeof(z2), s>>>(a)));
-  KT(KT_AIR_ROWS_INV, s, 32.0 * N * (m->nrowx + m->nrowy),
-     (k_air_rows_inv<<<m->nrowx + m->nrowy, 256, row_lds, s>>>(a)));
+  kt_launch(KT_AIR_ROWS_FWD, s, 16 * NN, k_air_rows_fwd, N, 256, row_lds, a);
+  kt_launch(KT_AIR_COLS, s, 80 * NN, k_air_cols<2>, 8 * ((N / 2 + 7) / 8), 256, 3 * 2 * (size_t)N * sizeof(z2), a);
+  kt_launch(KT_AIR_ROWS_INV, s, 32.0 * N * (m->nrowx + m->nrowy), k_air_rows_inv, m->nrowx + m->nrowy, 256,
+            row_lds, a);
   const int ng = (m->n1 + 1) * m->n2 + m->n1 * (m->n2 + 1);
-  KT(KT_AIR_GHOST, s, 16.0 * ng, (k_air_ghost<<<(ng + 255) / 256, 256, 0, s>>>(a)));
+  kt_launch(KT_AIR_GHOST, s, 16.0 * ng, k_air_ghost, (ng + 255) / 256, 256, 0, a);
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -377,18 +452,18 @@ int run_apply_bufs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStr
   a.hxg = b.hxg; a.hyg = b.hyg;
   if (m->airbc) {
     const int nh = m->n1 * m->n2;
-    KT(KT_HZ0, s, 16.0 * nh, (k_hz0<<<(nh + 255) / 256, 256, 0, s>>>(a, b.hz0)));
+    kt_launch(KT_HZ0, s, 16.0 * nh, k_hz0, (nh + 255) / 256, 256, 0, a, b.hz0);
     int rc = run_airbc(m, s, b);
     if (rc) return rc;
   }
   const int ntiles = a.nti * a.ntj * a.ntk;
   const dim3 blk(64, AP_NW), grd(8 * ((ntiles + 7) / 8));
   switch (m->kc) {
-    case 6: KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<6><<<grd, blk, 0, s>>>(a))); break;
-    case 10: KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<10><<<grd, blk, 0, s>>>(a))); break;
-    case 12: KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<12><<<grd, blk, 0, s>>>(a))); break;
-    case 16: KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<16><<<grd, blk, 0, s>>>(a))); break;
-    default: KT(KT_APPLY, s, 40.0 * m->lE, (k_apply<8><<<grd, blk, 0, s>>>(a))); break;
+    case 6: kt_launch(KT_APPLY, s, 40.0 * m->lE, k_apply<6>, grd, blk, 0, a); break;
+    case 10: kt_launch(KT_APPLY, s, 40.0 * m->lE, k_apply<10>, grd, blk, 0, a); break;
+    case 12: kt_launch(KT_APPLY, s, 40.0 * m->lE, k_apply<12>, grd, blk, 0, a); break;
+    case 16: kt_launch(KT_APPLY, s, 40.0 * m->lE, k_apply<16>, grd, blk, 0, a); break;
+    default: kt_launch(KT_APPLY, s, 40.0 * m->lE, k_apply<8>, grd, blk, 0, a); break;
   }
   HIP_OK(hipGetLastError());
   return 0;
@@ -433,9 +508,9 @@ static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   gq[2] = dim3((g.B[2] + 63) / 64, (g.A[2] + 7) / 8, g.ns[2]);
   const dim3 bq(256);
   if (!m->smith_ok || memcmp(&m->smith_f, f, sizeof *f)) {  // diagonal of this frequency
-    KT(KT_LU_SMITH, s, 48.0 * m->nsk, (k_lu_smith<0><<<gp[0], 256, 0, s>>>(a)));
-    KT(KT_LU_SMITH, s, 0.0, (k_lu_smith<1><<<gp[1], 256, 0, s>>>(a)));
-    KT(KT_LU_SMITH, s, 0.0, (k_lu_smith<2><<<gp[2], 256, 0, s>>>(a)));
+    kt_launch(KT_LU_SMITH, s, 48.0 * m->nsk, k_lu_smith<0>, gp[0], 256, 0, a);
+    kt_launch(KT_LU_SMITH, s, 0.0, k_lu_smith<1>, gp[1], 256, 0, a);
+    kt_launch(KT_LU_SMITH, s, 0.0, k_lu_smith<2>, gp[2], 256, 0, a);
     m->smith_f = *f;
     m->smith_ok = 1;
   }
@@ -459,18 +534,18 @@ static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   // records' round trip is the price of the parallel pre-pass: the whole
   // application needs 144 bytes per unknown (forward Ex 40, Ey 56, Ez 72,
   // backward 88 each: b, sigma, the values read, the value written).
-  KT(KT_LU_PRE, s, 32.0 * nc[0], (k_lu_pre<0, true><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_SGS_EX_F, s, 64.0 * nc[0], (SWEEP(0, true)<<<gs[0], tb[0] + 64, lds[0], s>>>(a)));
-  KT(KT_LU_PRE, s, 112.0 * nc[1], (k_lu_pre<1, true><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_SGS_EY_F, s, 128.0 * nc[1], (SWEEP(1, true)<<<gs[1], tb[1] + 64, lds[1], s>>>(a)));
-  KT(KT_LU_PRE, s, 128.0 * nc[2], (k_lu_pre<2, true><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_SGS_EZ_F, s, 128.0 * nc[2], (SWEEP(2, true)<<<gs[2], tb[2] + 64, lds[2], s>>>(a)));
-  KT(KT_LU_PRE, s, 144.0 * nc[2], (k_lu_pre<2, false><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_SGS_EZ_B, s, 144.0 * nc[2], (SWEEP(2, false)<<<gs[2], tb[2] + 64, lds[2], s>>>(a)));
-  KT(KT_LU_PRE, s, 144.0 * nc[1], (k_lu_pre<1, false><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_SGS_EY_B, s, 144.0 * nc[1], (SWEEP(1, false)<<<gs[1], tb[1] + 64, lds[1], s>>>(a)));
-  KT(KT_LU_PRE, s, 144.0 * nc[0], (k_lu_pre<0, false><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_SGS_EX_B, s, 128.0 * nc[0], (SWEEP(0, false)<<<gs[0], tb[0] + 64, lds[0], s>>>(a)));
+  kt_launch(KT_LU_PRE, s, 32.0 * nc[0], k_lu_pre<0, true>, gq[0], bq, 0, a);
+  kt_launch(KT_SGS_EX_F, s, 64.0 * nc[0], SWEEP(0, true), gs[0], tb[0] + 64, lds[0], a);
+  kt_launch(KT_LU_PRE, s, 112.0 * nc[1], k_lu_pre<1, true>, gq[1], bq, 0, a);
+  kt_launch(KT_SGS_EY_F, s, 128.0 * nc[1], SWEEP(1, true), gs[1], tb[1] + 64, lds[1], a);
+  kt_launch(KT_LU_PRE, s, 128.0 * nc[2], k_lu_pre<2, true>, gq[2], bq, 0, a);
+  kt_launch(KT_SGS_EZ_F, s, 128.0 * nc[2], SWEEP(2, true), gs[2], tb[2] + 64, lds[2], a);
+  kt_launch(KT_LU_PRE, s, 144.0 * nc[2], k_lu_pre<2, false>, gq[2], bq, 0, a);
+  kt_launch(KT_SGS_EZ_B, s, 144.0 * nc[2], SWEEP(2, false), gs[2], tb[2] + 64, lds[2], a);
+  kt_launch(KT_LU_PRE, s, 144.0 * nc[1], k_lu_pre<1, false>, gq[1], bq, 0, a);
+  kt_launch(KT_SGS_EY_B, s, 144.0 * nc[1], SWEEP(1, false), gs[1], tb[1] + 64, lds[1], a);
+  kt_launch(KT_LU_PRE, s, 144.0 * nc[0], k_lu_pre<0, false>, gq[0], bq, 0, a);
+  kt_launch(KT_SGS_EX_B, s, 128.0 * nc[0], SWEEP(0, false), gs[0], tb[0] + 64, lds[0], a);
 #undef SWEEP
   HIP_OK(hipGetLastError());
   return 0;
@@ -551,8 +626,8 @@ static int dot_partial_launch(long n, const z2 *a0, const z2 *b0, const z2 *a1, 
   DotArgs g;
   g.n = n; g.a[0] = a0; g.b[0] = b0; g.a[1] = a1; g.b[1] = b1; g.partials = partials;
   long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
-  if (a1) KT(KT_DOT, s, 64.0 * n, (k_dot_partial<2><<<(unsigned)nch, RED_T, 0, s>>>(g)));
-  else KT(KT_DOT, s, 32.0 * n, (k_dot_partial<1><<<(unsigned)nch, RED_T, 0, s>>>(g)));
+  if (a1) kt_launch(KT_DOT, s, 64.0 * n, k_dot_partial<2>, (unsigned)nch, RED_T, 0, g);
+  else kt_launch(KT_DOT, s, 32.0 * n, k_dot_partial<1>, (unsigned)nch, RED_T, 0, g);
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -1186,7 +1261,7 @@ static int bicgstab_run(csem_model *m, const csem_freq *f, long n, z2 *x, const 
     if ((rc = A(actx, m, f, pin, v, s))) return rc;
     if (!g_red_serial && (rc = dot_partial_launch(n, v, r0, NULL, NULL, pa, s))) return rc;
     k_bicg_fin_alpha<<<1, RED_T, 0, s>>>(pa, serial_dots(n, v, r0, NULL, NULL, sc, s, nch), sc);
-    KT(KT_BICG_S, s, 48.0 * n, (k_bicg_s<<<ge, 256, 0, s>>>(n, r, v, sv, sc)));
+    kt_launch(KT_BICG_S, s, 48.0 * n, k_bicg_s, ge, 256, 0, n, r, v, sv, sc);
     const z2 *xb;
     if (M) {
       if ((rc = M(mctx, m, f, sv, r, s))) return rc;
@@ -1200,7 +1275,7 @@ static int bicgstab_run(csem_model *m, const csem_freq *f, long n, z2 *x, const 
     }
     const long nom = M ? serial_dots(n, sv, t, t, t, sc, s, nch) : serial_dots(n, t, sv, t, t, sc, s, nch);
     k_bicg_fin_omega<<<1, RED_T, 0, s>>>(pa, nom, sc);
-    KT(KT_BICG_XR, s, 128.0 * n, (k_bicg_xr<<<(unsigned)nch, RED_T, 0, s>>>(n, x, pin, xb, sv, t, r, r0, sc, pa)));
+    kt_launch(KT_BICG_XR, s, 128.0 * n, k_bicg_xr, (unsigned)nch, RED_T, 0, n, x, pin, xb, sv, t, r, r0, sc, pa);
     k_bicg_fin_conv<<<1, RED_T, 0, s>>>(pa, serial_dots(n, r, r, r, r0, sc, s, nch), sc);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(hs, sc, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1211,7 +1286,7 @@ static int bicgstab_run(csem_model *m, const csem_freq *f, long n, z2 *x, const 
       if (hist) hist[iter + 1] = relres;
       break;
     }
-    KT(KT_BICG_P, s, 64.0 * n, (k_bicg_p<<<ge, 256, 0, s>>>(n, p, r, v, sc)));
+    kt_launch(KT_BICG_P, s, 64.0 * n, k_bicg_p, ge, 256, 0, n, p, r, v, sc);
   }
   HIP_OK(hipGetLastError());
   return iter;

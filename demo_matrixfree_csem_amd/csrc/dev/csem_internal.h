@@ -1,8 +1,10 @@
 // csem_internal.h -- state shared by the FP64 and mixed-precision
 // translation units of libcsem (not part of the public C ABI).
 #pragma once
+#include <cstdint>
 #include <mutex>
 #include <vector>
+#include <hip/hip_ext.h>
 #include "csem_kernels.h"
 #include "../../../include/csem_dev.h"
 
@@ -45,8 +47,36 @@ enum {
   KT_M_APPLY, KT_M_HZ0, KT_M_AIR_ROWS, KT_M_AIR_COLS, KT_M_AIR_RINV, KT_M_AIR_GHOST,
   KT_M_PRE, KT_M_SGS, KT_M_VEC, KT_M_DOT, KT_M_REFINE, KT_M_UNSKEW, KT_N
 };
-hipEvent_t kt_begin(hipStream_t s);
-void kt_end(int id, hipEvent_t a, hipStream_t s, double bytes);
+// An event pair for the next launch when timing is on and not paused
+// (graph capture pauses it), and the pair's record once launched.
+bool kt_events(hipEvent_t *a, hipEvent_t *b);
+void kt_record(int id, hipEvent_t a, hipEvent_t b, double bytes);
+// Repeat loops (csem_kernel_repeat): the repeat count for this launch of
+// kernel id (0: none) with the loop's event pair, and the loop's record.
+int kt_repeat(int id, hipEvent_t *a, hipEvent_t *b);
+void kt_record_loop(int id, hipEvent_t a, hipEvent_t b, int n);
+// Every hot kernel is launched through here.  Timed launches go through
+// hipExtLaunchKernelGGL with the launch's event pair; a launch that
+// repeats is followed by its back-to-back copies between one more pair.
+template <typename... P, typename... A>
+static inline void kt_launch(int id, hipStream_t s, double bytes, void (*k)(P...), dim3 g, dim3 b, size_t shm,
+                             A... args)
+{
+  hipEvent_t e0, e1;
+  if (kt_events(&e0, &e1)) {
+    hipExtLaunchKernelGGL(k, g, b, (std::uint32_t)shm, s, e0, e1, 0, static_cast<P>(args)...);
+    kt_record(id, e0, e1, bytes);
+    const int n = kt_repeat(id, &e0, &e1);
+    if (n > 0) {
+      (void)hipEventRecord(e0, s);
+      for (int i = 0; i < n; i++) k<<<g, b, shm, s>>>(static_cast<P>(args)...);
+      (void)hipEventRecord(e1, s);
+      kt_record_loop(id, e0, e1, n);
+    }
+  } else {
+    k<<<g, b, shm, s>>>(static_cast<P>(args)...);
+  }
+}
 // Right-hand sides a batched launch works on, for the timers' byte counts:
 // the batch kernels skip right-hand sides whose inner solve is done, and
 // the host learns which are done at every convergence check, so it sets
@@ -54,12 +84,6 @@ void kt_end(int id, hipEvent_t a, hipStream_t s, double bytes);
 // between two checks is counted until the next one.
 extern thread_local int g_kt_live;
 static inline double kt_rhs(int R) { return g_kt_live > 0 && g_kt_live < R ? (double)g_kt_live : (double)R; }
-#define KT(id, s, bytes, launch)        \
-  do {                                   \
-    hipEvent_t kta_ = kt_begin(s);       \
-    launch;                              \
-    kt_end(id, kta_, s, (double)(bytes)); \
-  } while (0)
 
 
 struct MixState;

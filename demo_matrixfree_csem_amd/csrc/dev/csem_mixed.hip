@@ -330,22 +330,22 @@ static MxAir mx_air(const csem_model *m, const MixState *x, const MixSlot *q)
 // 4 * NW * R * 64 complex floats: NW = 8 for R <= 3, at most 6 for R = 4.
 template <int R, int NW, int KC>
 static void mx_apply_launch(const csem_model *m, const MixState *x, const MixSlot *q, const MxFreq &fr, const c2 *in,
-                            c2 *out, hipStream_t s, const double *sc)
+                            c2 *out, hipStream_t s, const double *sc, double bytes)
 {
   const int nti = (m->n1 + 1 + 61) / 62, ntj = (m->n2 + 1 + NW - 3) / (NW - 2), ntk = (m->n3 + 1 + KC - 1) / KC;
   const int ntiles = nti * ntj * ntk;
-  k_mx_apply<R, NW, KC><<<8 * ((ntiles + 7) / 8), dim3(64, NW), 0, s>>>(mx_geo(m, x), fr, in, out, q->hxg, q->hyg,
-                                                                       nti, ntj, ntk, sc);
+  kt_launch(KT_M_APPLY, s, bytes, k_mx_apply<R, NW, KC>, 8 * ((ntiles + 7) / 8), dim3(64, NW), 0, mx_geo(m, x), fr,
+            in, out, q->hxg, q->hyg, nti, ntj, ntk, sc);
 }
 template <int R>
 static void mx_apply_tile(const csem_model *m, const MixState *x, const MixSlot *q, const MxFreq &fr, const c2 *in,
-                          c2 *out, int rows, int kc, hipStream_t s, const double *sc)
+                          c2 *out, int rows, int kc, hipStream_t s, const double *sc, double bytes)
 {
   constexpr int NB = R > 3 ? 6 : 8;  // the larger block
-#define MX_AP(KC)                                                   \
-  do {                                                              \
-    if (rows + 2 >= NB) mx_apply_launch<R, NB, KC>(m, x, q, fr, in, out, s, sc);     \
-    else mx_apply_launch<R, NB - 2, KC>(m, x, q, fr, in, out, s, sc);   \
+#define MX_AP(KC)                                                                         \
+  do {                                                                                    \
+    if (rows + 2 >= NB) mx_apply_launch<R, NB, KC>(m, x, q, fr, in, out, s, sc, bytes);     \
+    else mx_apply_launch<R, NB - 2, KC>(m, x, q, fr, in, out, s, sc, bytes);                \
   } while (0)
   switch (kc) {
     case 10: MX_AP(10); break;
@@ -356,13 +356,14 @@ static void mx_apply_tile(const csem_model *m, const MixState *x, const MixSlot 
 #undef MX_AP
 }
 static void mx_apply_kernel(const csem_model *m, const MixState *x, const MixSlot *q, int R, const MxFreq &fr,
-                            const c2 *in, c2 *out, int rows, int kc, hipStream_t s, const double *sc = nullptr)
+                            const c2 *in, c2 *out, int rows, int kc, hipStream_t s, const double *sc = nullptr,
+                            double bytes = 0.0)
 {
   switch (R) {
-    case 1: mx_apply_tile<1>(m, x, q, fr, in, out, rows, kc, s, sc); break;
-    case 2: mx_apply_tile<2>(m, x, q, fr, in, out, rows, kc, s, sc); break;
-    case 3: mx_apply_tile<3>(m, x, q, fr, in, out, rows, kc, s, sc); break;
-    default: mx_apply_tile<4>(m, x, q, fr, in, out, rows, kc, s, sc); break;
+    case 1: mx_apply_tile<1>(m, x, q, fr, in, out, rows, kc, s, sc, bytes); break;
+    case 2: mx_apply_tile<2>(m, x, q, fr, in, out, rows, kc, s, sc, bytes); break;
+    case 3: mx_apply_tile<3>(m, x, q, fr, in, out, rows, kc, s, sc, bytes); break;
+    default: mx_apply_tile<4>(m, x, q, fr, in, out, rows, kc, s, sc, bytes); break;
   }
 }
 // output rows per block the operator offers at batch size R
@@ -412,19 +413,20 @@ static int mx_apply(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
     const int nh = m->n1 * m->n2, N = m->nfft;
     MxAir a = mx_air(m, x, q);
     a.sc = sc;
-    KT(KT_M_HZ0, s, 24.0 * nh * kt_rhs(R), (k_mx_hz0<<<dim3((nh + 255) / 256, R), 256, 0, s>>>(mx_geo(m, x), in, q->hz0, sc)));
-    KT(KT_M_AIR_ROWS, s, 8.0 * N * N * kt_rhs(R),
-       (k_mx_air_rows<<<dim3(N, R), 256, 3 * (size_t)N * sizeof(c2), s>>>(a)));
-    KT(KT_M_AIR_COLS, s, 40.0 * N * N * kt_rhs(R),
-       (k_mx_air_cols<<<dim3(8 * ((N / x->cw + 7) / 8), R), N * x->cw / 16, (2 * (size_t)N * x->cw + N) * sizeof(c2), s>>>(a)));
-    KT(KT_M_AIR_RINV, s, 16.0 * N * (m->nrowx + m->nrowy) * kt_rhs(R),
-       (k_mx_air_rows_inv<<<dim3(m->nrowx + m->nrowy, R), 256, 3 * (size_t)N * sizeof(c2), s>>>(a)));
+    kt_launch(KT_M_HZ0, s, 24.0 * nh * kt_rhs(R), k_mx_hz0, dim3((nh + 255) / 256, R), 256, 0, mx_geo(m, x), in,
+              q->hz0, sc);
+    kt_launch(KT_M_AIR_ROWS, s, 8.0 * N * N * kt_rhs(R), k_mx_air_rows, dim3(N, R), 256,
+              3 * (size_t)N * sizeof(c2), a);
+    kt_launch(KT_M_AIR_COLS, s, 40.0 * N * N * kt_rhs(R), k_mx_air_cols, dim3(8 * ((N / x->cw + 7) / 8), R),
+              N * x->cw / 16, (2 * (size_t)N * x->cw + N) * sizeof(c2), a);
+    kt_launch(KT_M_AIR_RINV, s, 16.0 * N * (m->nrowx + m->nrowy) * kt_rhs(R), k_mx_air_rows_inv,
+              dim3(m->nrowx + m->nrowy, R), 256, 3 * (size_t)N * sizeof(c2), a);
     const int ng = (m->n1 + 1) * m->n2 + m->n1 * (m->n2 + 1);
-    KT(KT_M_AIR_GHOST, s, 8.0 * ng * kt_rhs(R), (k_mx_air_ghost<<<dim3((ng + 255) / 256, R), 256, 0, s>>>(a)));
+    kt_launch(KT_M_AIR_GHOST, s, 8.0 * ng * kt_rhs(R), k_mx_air_ghost, dim3((ng + 255) / 256, R), 256, 0, a);
   }
   // algorithmic bytes: read E and write y per right-hand side, sigma once
   const int tj = x->ap_tj[R] ? x->ap_tj[R] : 6, kc = x->ap_kc[R] ? x->ap_kc[R] : 8;
-  KT(KT_M_APPLY, s, (16.0 * kt_rhs(R) + 4.0) * m->lE, mx_apply_kernel(m, x, q, R, fr, in, out, tj, kc, s, sc));
+  mx_apply_kernel(m, x, q, R, fr, in, out, tj, kc, s, sc, (16.0 * kt_rhs(R) + 4.0) * m->lE);
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -466,13 +468,13 @@ static dim3 mx_tile_grid3(const SkewGeo &g, int c)
 
 // the sweep for slices of tb / 64 waves
 template <int C, bool FWD>
-static void mx_sgs_launch(dim3 g, int tb, size_t lds, hipStream_t s, const MxLu &a)
+static void mx_sgs_launch(dim3 g, int tb, size_t lds, hipStream_t s, const MxLu &a, double bytes)
 {
   switch (tb / 64) {
-    case 1: k_mx_sgs<C, FWD, 1><<<g, tb, lds, s>>>(a); break;
-    case 2: k_mx_sgs<C, FWD, 2><<<g, tb, lds, s>>>(a); break;
-    case 3: k_mx_sgs<C, FWD, 3><<<g, tb, lds, s>>>(a); break;
-    default: k_mx_sgs<C, FWD, 4><<<g, tb, lds, s>>>(a); break;
+    case 1: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 1>, g, tb, lds, a); break;
+    case 2: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 2>, g, tb, lds, a); break;
+    case 3: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 3>, g, tb, lds, a); break;
+    default: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 4>, g, tb, lds, a); break;
   }
 }
 
@@ -501,24 +503,25 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
   // pre-pass reads b and the other components' W values it differences
   // and writes T; the sweep reads T and h and writes the value to V; the
   // copy reads V and writes W (not after the last sweep) and y (backward)
-  KT(KT_M_PRE, s, 16.0 * kt_rhs(R) * nc[0], (k_mx_pre<0, true, R><<<gqx, 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * kt_rhs(R) * nc[0], (mx_sgs_launch<0, true>(gs[0], tb[0], lds[0], s, a)));
-  KT(KT_M_UNSKEW, s, 16.0 * kt_rhs(R) * nc[0], (k_mx_unskew<0, true, R><<<gu[0], 256, 0, s>>>(a)));
-  KT(KT_M_PRE, s, 24.0 * kt_rhs(R) * nc[1], (k_mx_pre<1, true, R><<<gq[1], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * kt_rhs(R) * nc[1], (mx_sgs_launch<1, true>(gs[1], tb[1], lds[1], s, a)));
-  KT(KT_M_UNSKEW, s, 16.0 * kt_rhs(R) * nc[1], (k_mx_unskew<1, true, R><<<gu[1], 256, 0, s>>>(a)));
-  KT(KT_M_PRE, s, 32.0 * kt_rhs(R) * nc[2], (k_mx_pre<2, true, R><<<gq[2], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * kt_rhs(R) * nc[2], (mx_sgs_launch<2, true>(gs[2], tb[2], lds[2], s, a)));
-  KT(KT_M_UNSKEW, s, 16.0 * kt_rhs(R) * nc[2], (k_mx_unskew<2, true, R><<<gu[2], 256, 0, s>>>(a)));
-  KT(KT_M_PRE, s, 40.0 * kt_rhs(R) * nc[2], (k_mx_pre<2, false, R><<<gq[2], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * kt_rhs(R) * nc[2], (mx_sgs_launch<2, false>(gs[2], tb[2], lds[2], s, a)));
-  KT(KT_M_UNSKEW, s, 24.0 * kt_rhs(R) * nc[2], (k_mx_unskew<2, false, R><<<gu[2], 256, 0, s>>>(a)));
-  KT(KT_M_PRE, s, 40.0 * kt_rhs(R) * nc[1], (k_mx_pre<1, false, R><<<gq[1], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * kt_rhs(R) * nc[1], (mx_sgs_launch<1, false>(gs[1], tb[1], lds[1], s, a)));
-  KT(KT_M_UNSKEW, s, 24.0 * kt_rhs(R) * nc[1], (k_mx_unskew<1, false, R><<<gu[1], 256, 0, s>>>(a)));
-  KT(KT_M_PRE, s, 40.0 * kt_rhs(R) * nc[0], (k_mx_pre<0, false, R><<<gq[0], 256, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * kt_rhs(R) * nc[0], (mx_sgs_launch<0, false>(gs[0], tb[0], lds[0], s, a)));
-  KT(KT_M_UNSKEW, s, 16.0 * kt_rhs(R) * nc[0], (k_mx_unskew<0, false, R><<<gu[0], 256, 0, s>>>(a)));
+  const double r = kt_rhs(R);
+  kt_launch(KT_M_PRE, s, 16.0 * r * nc[0], k_mx_pre<0, true, R>, gqx, 256, 0, a);
+  mx_sgs_launch<0, true>(gs[0], tb[0], lds[0], s, a, 24.0 * r * nc[0]);
+  kt_launch(KT_M_UNSKEW, s, 16.0 * r * nc[0], k_mx_unskew<0, true, R>, gu[0], 256, 0, a);
+  kt_launch(KT_M_PRE, s, 24.0 * r * nc[1], k_mx_pre<1, true, R>, gq[1], 256, 0, a);
+  mx_sgs_launch<1, true>(gs[1], tb[1], lds[1], s, a, 24.0 * r * nc[1]);
+  kt_launch(KT_M_UNSKEW, s, 16.0 * r * nc[1], k_mx_unskew<1, true, R>, gu[1], 256, 0, a);
+  kt_launch(KT_M_PRE, s, 32.0 * r * nc[2], k_mx_pre<2, true, R>, gq[2], 256, 0, a);
+  mx_sgs_launch<2, true>(gs[2], tb[2], lds[2], s, a, 24.0 * r * nc[2]);
+  kt_launch(KT_M_UNSKEW, s, 16.0 * r * nc[2], k_mx_unskew<2, true, R>, gu[2], 256, 0, a);
+  kt_launch(KT_M_PRE, s, 40.0 * r * nc[2], k_mx_pre<2, false, R>, gq[2], 256, 0, a);
+  mx_sgs_launch<2, false>(gs[2], tb[2], lds[2], s, a, 24.0 * r * nc[2]);
+  kt_launch(KT_M_UNSKEW, s, 24.0 * r * nc[2], k_mx_unskew<2, false, R>, gu[2], 256, 0, a);
+  kt_launch(KT_M_PRE, s, 40.0 * r * nc[1], k_mx_pre<1, false, R>, gq[1], 256, 0, a);
+  mx_sgs_launch<1, false>(gs[1], tb[1], lds[1], s, a, 24.0 * r * nc[1]);
+  kt_launch(KT_M_UNSKEW, s, 24.0 * r * nc[1], k_mx_unskew<1, false, R>, gu[1], 256, 0, a);
+  kt_launch(KT_M_PRE, s, 40.0 * r * nc[0], k_mx_pre<0, false, R>, gq[0], 256, 0, a);
+  mx_sgs_launch<0, false>(gs[0], tb[0], lds[0], s, a, 24.0 * r * nc[0]);
+  kt_launch(KT_M_UNSKEW, s, 16.0 * r * nc[0], k_mx_unskew<0, false, R>, gu[0], 256, 0, a);
 }
 
 // The red-black ordered sweeps: forward red, black per component (Ex, Ey,
@@ -537,18 +540,18 @@ static void mx_rbsgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
   const dim3 bq(64, 4);
   double h[3];
   for (int c = 0; c < 3; c++) h[c] = 0.5 * g.ns[c] * g.A[c] * g.B[c] * kt_rhs(R);
-  KT(KT_M_SGS, s, 16.0 * h[0], (k_mx_rb<0, true, 0, R><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * h[0], (k_mx_rb<0, true, 1, R><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 24.0 * h[1], (k_mx_rb<1, true, 0, R><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 32.0 * h[1], (k_mx_rb<1, true, 1, R><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 32.0 * h[2], (k_mx_rb<2, true, 0, R><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 40.0 * h[2], (k_mx_rb<2, true, 1, R><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 48.0 * h[2], (k_mx_rb<2, false, 1, R><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 48.0 * h[2], (k_mx_rb<2, false, 0, R><<<gq[2], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 48.0 * h[1], (k_mx_rb<1, false, 1, R><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 48.0 * h[1], (k_mx_rb<1, false, 0, R><<<gq[1], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 48.0 * h[0], (k_mx_rb<0, false, 1, R><<<gq[0], bq, 0, s>>>(a)));
-  KT(KT_M_SGS, s, 48.0 * h[0], (k_mx_rb<0, false, 0, R><<<gq[0], bq, 0, s>>>(a)));
+  kt_launch(KT_M_SGS, s, 16.0 * h[0], k_mx_rb<0, true, 0, R>, gq[0], bq, 0, a);
+  kt_launch(KT_M_SGS, s, 24.0 * h[0], k_mx_rb<0, true, 1, R>, gq[0], bq, 0, a);
+  kt_launch(KT_M_SGS, s, 24.0 * h[1], k_mx_rb<1, true, 0, R>, gq[1], bq, 0, a);
+  kt_launch(KT_M_SGS, s, 32.0 * h[1], k_mx_rb<1, true, 1, R>, gq[1], bq, 0, a);
+  kt_launch(KT_M_SGS, s, 32.0 * h[2], k_mx_rb<2, true, 0, R>, gq[2], bq, 0, a);
+  kt_launch(KT_M_SGS, s, 40.0 * h[2], k_mx_rb<2, true, 1, R>, gq[2], bq, 0, a);
+  kt_launch(KT_M_SGS, s, 48.0 * h[2], k_mx_rb<2, false, 1, R>, gq[2], bq, 0, a);
+  kt_launch(KT_M_SGS, s, 48.0 * h[2], k_mx_rb<2, false, 0, R>, gq[2], bq, 0, a);
+  kt_launch(KT_M_SGS, s, 48.0 * h[1], k_mx_rb<1, false, 1, R>, gq[1], bq, 0, a);
+  kt_launch(KT_M_SGS, s, 48.0 * h[1], k_mx_rb<1, false, 0, R>, gq[1], bq, 0, a);
+  kt_launch(KT_M_SGS, s, 48.0 * h[0], k_mx_rb<0, false, 1, R>, gq[0], bq, 0, a);
+  kt_launch(KT_M_SGS, s, 48.0 * h[0], k_mx_rb<0, false, 0, R>, gq[0], bq, 0, a);
 }
 
 static void mx_rbsgs_dispatch(const csem_model *m, int R, const MxLu &a, hipStream_t s)
@@ -607,16 +610,17 @@ static int mx_iteration(csem_model *m, MixState *x, MixSlot *q, int R, const MxF
   const double *live = R > 1 ? k.sc : nullptr;
   if ((rc = mx_lusgs(m, x, q, R, fr, k.p, k.pp, s, live))) return rc;
   if ((rc = mx_apply(m, x, q, R, fr, k.pp, k.v, s, live))) return rc;
-  KT(KT_M_DOT, s, 16.0 * kt_rhs(R) * k.n, (k_mx_k_dot_vr0<<<gr, RED_T, 0, s>>>(k)));
-  KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 1)));
-  KT(KT_M_VEC, s, 24.0 * kt_rhs(R) * k.n, (k_mx_k_s<<<ge, 256, 0, s>>>(k)));
+  const double r = kt_rhs(R);
+  kt_launch(KT_M_DOT, s, 16.0 * r * k.n, k_mx_k_dot_vr0, gr, RED_T, 0, k);
+  kt_launch(KT_M_DOT, s, 0.0, k_mx_k_fin, R, RED_T, 0, k, 1);
+  kt_launch(KT_M_VEC, s, 24.0 * r * k.n, k_mx_k_s, ge, 256, 0, k);
   if ((rc = mx_lusgs(m, x, q, R, fr, k.s, k.ss, s, live))) return rc;
   if ((rc = mx_apply(m, x, q, R, fr, k.ss, k.t, s, live))) return rc;
-  KT(KT_M_DOT, s, 16.0 * kt_rhs(R) * k.n, (k_mx_k_dot_st<<<gr, RED_T, 0, s>>>(k)));
-  KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 2)));
-  KT(KT_M_VEC, s, 64.0 * kt_rhs(R) * k.n, (k_mx_k_xr<<<gr, RED_T, 0, s>>>(k)));
-  KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 3)));
-  KT(KT_M_VEC, s, 32.0 * kt_rhs(R) * k.n, (k_mx_k_p<<<ge, 256, 0, s>>>(k)));
+  kt_launch(KT_M_DOT, s, 16.0 * r * k.n, k_mx_k_dot_st, gr, RED_T, 0, k);
+  kt_launch(KT_M_DOT, s, 0.0, k_mx_k_fin, R, RED_T, 0, k, 2);
+  kt_launch(KT_M_VEC, s, 64.0 * r * k.n, k_mx_k_xr, gr, RED_T, 0, k);
+  kt_launch(KT_M_DOT, s, 0.0, k_mx_k_fin, R, RED_T, 0, k, 3);
+  kt_launch(KT_M_VEC, s, 32.0 * r * k.n, k_mx_k_p, ge, 256, 0, k);
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -743,9 +747,9 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
     g_kt_live = nact;
     HIP_OK(hipMemcpyAsync(q->sc, hsc.data(), hsc.size() * sizeof(double), hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(q->scale, scl.data(), R * sizeof(float), hipMemcpyHostToDevice, s));
-    KT(KT_M_REFINE, s, 24.0 * R * n, (k_mx_to32<<<dim3(gn, R), 256, 0, s>>>(n, q->res, q->scale, k.r)));
-    KT(KT_M_DOT, s, 32.0 * kt_rhs(R) * n, (k_mx_k_init<<<dim3(nch, R), RED_T, 0, s>>>(k)));
-    KT(KT_M_DOT, s, 0.0, (k_mx_k_fin<<<R, RED_T, 0, s>>>(k, 0)));
+    kt_launch(KT_M_REFINE, s, 24.0 * R * n, k_mx_to32, dim3(gn, R), 256, 0, n, q->res, q->scale, k.r);
+    kt_launch(KT_M_DOT, s, 32.0 * kt_rhs(R) * n, k_mx_k_init, dim3(nch, R), RED_T, 0, k);
+    kt_launch(KT_M_DOT, s, 0.0, k_mx_k_fin, R, RED_T, 0, k, 0);
     if (o.use_graph && (rc = mx_graph(m, x, q, R, fr, s))) return rc;
     // inner iterations, convergence checked every check_every; an inner
     // solve whose recursive residual has not improved by 1% in
@@ -786,12 +790,12 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
       if (hsc[(size_t)r * MS_N + MS_USED] != 0.0) iters[r] += (int)hsc[(size_t)r * MS_N + MS_IT];
     // x += |res| * correction (FP64), then the true residual in FP64
     HIP_OK(hipMemcpyAsync(q->xprev, X, (size_t)R * n * sizeof(z2), hipMemcpyDeviceToDevice, s));
-    KT(KT_M_REFINE, s, 32.0 * R * n, (k_mx_acc64<<<dim3(gn, R), 256, 0, s>>>(n, X, k.x, q->sc, q->scale)));
+    kt_launch(KT_M_REFINE, s, 32.0 * R * n, k_mx_acc64, dim3(gn, R), 256, 0, n, X, k.x, q->sc, q->scale);
     for (int r = 0; r < R; r++) {
       if (hsc[(size_t)r * MS_N + MS_USED] == 0.0) continue;
       if ((rc = run_apply_bufs(m, &f[r], X + r * n, q->tmp64, s, q->a64))) return rc;
-      KT(KT_M_REFINE, s, 48.0 * n,
-         (k_mx_res64<<<nch, RED_T, 0, s>>>(n, B + r * n, q->tmp64, q->res + r * n, q->npart + (long)r * nch, nch)));
+      kt_launch(KT_M_REFINE, s, 48.0 * n, k_mx_res64, nch, RED_T, 0, n, B + r * n, q->tmp64, q->res + r * n,
+                q->npart + (long)r * nch, nch);
     }
     k_mx_norm_fin<<<R, RED_T, 0, s>>>(q->npart, nch, q->norm);
     HIP_OK(hipMemcpyAsync(nb.data(), q->norm, R * sizeof(double), hipMemcpyDeviceToHost, s));

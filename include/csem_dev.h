@@ -233,9 +233,25 @@ int csem_mem_info(long *free_bytes, long *total_bytes);
  * of every hot-kernel launch (events recorded on the launch stream);
  * csem_kernel_stats(id, ...) returns the kernel's name, summed duration,
  * launch count and summed algorithmic bytes, and the number of kernel ids
- * (ids run 0 .. n-1), or -1 for an id out of range. */
+ * (ids run 0 .. n-1), or -1 for an id out of range.  An event pair around
+ * one launch also spans the dispatch and the event packets themselves
+ * (several microseconds on top of the kernel).
+ *
+ * csem_kernel_repeat(id, reps, loops): while timing is on, each of the
+ * next `loops` launches of kernel id is followed by `reps` more launches
+ * of it with the same arguments, back to back between one event pair
+ * (each id keeps its own setting until csem_kernel_timing resets them);
+ * csem_kernel_loop_stats gives their summed time and launch count, so
+ * time / count is the kernel's duration plus only the gap between two
+ * dispatches.  Only kernels whose output does not feed their own input
+ * may repeat (operators, airwave stages but the in-place inverse rows,
+ * LU-SGS pre-passes, sweeps and copies, dot partials): a repeat leaves
+ * every result unchanged.  Returns
+ * 0, or -1 for any other id. */
 int csem_kernel_timing(int on);
 int csem_kernel_stats(int id, const char **name, double *total_ms, long *count, double *bytes);
+int csem_kernel_repeat(int id, int reps, int loops);
+int csem_kernel_loop_stats(int id, double *total_ms, long *count);
 
 /* Device buffer helpers for hosts without their own allocator. */
 int csem_malloc(void **p, long bytes);

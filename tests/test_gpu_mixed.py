@@ -89,6 +89,57 @@ def test_apply32_close_to_fp64(workdir):
         ses.close()
 
 
+REPEATABLE = ["k_apply", "k_hz0", "k_air_rows_fwd", "k_air_cols", "k_air_ghost", "k_lu_pre", "k_lu_smith",
+              "k_sgs_ex<fwd>", "k_sgs_ey<fwd>", "k_sgs_ez<fwd>", "k_sgs_ez<bwd>", "k_sgs_ey<bwd>", "k_sgs_ex<bwd>",
+              "k_mx_apply", "k_mx_hz0", "k_mx_air_rows", "k_mx_air_cols", "k_mx_air_ghost", "k_mx_lu_pre",
+              "k_mx_sgs", "k_mx_unskew"]
+
+
+def test_repeat_loops_leave_results_unchanged(workdir):
+    """The timing loops of csem_kernel_repeat launch a kernel again with the
+    same arguments: with loops on every repeatable kernel (and every launch
+    timed through hipExtLaunchKernelGGL) the FP64 operator and LU-SGS still
+    equal the oracle bit for bit and the FP32 ones equal a run without
+    timers; every loop's launches are counted; a kernel whose output feeds
+    its input is refused."""
+    print("case repeat loops", flush=True)
+    s, ses, orc = _setup(workdir)
+    L = _lib.lib()
+    try:
+        n, R = ses.n, 3
+        rng = np.random.default_rng(13)
+        x = (rng.standard_normal((R, n)) + 1j * rng.standard_normal((R, n))).astype(np.complex64)
+        fr = (_lib.FreqT * R)(*[freq_consts(f) for f in FREQS])
+        dx, dy, dz = _dev(x), _dev(np.zeros_like(x)), _dev(np.zeros_like(x))
+        _lib.check(L.csem_apply32(ses.model, R, fr, dx, dy, None), "apply32")
+        _lib.check(L.csem_lusgs32(ses.model, R, fr, dx, dz, None), "lusgs32")
+        y0, z0 = _host(dy, x), _host(dz, x)
+        L.csem_kernel_timing(1)
+        try:
+            for k in REPEATABLE:
+                _lib.check(L.csem_kernel_repeat(_lib.kernel_id(k), 3, 2), "repeat " + k)
+            assert L.csem_kernel_repeat(_lib.kernel_id("k_mx_krylov"), 3, 2) == -1
+            assert L.csem_kernel_repeat(_lib.kernel_id("k_air_rows_inv"), 3, 2) == -1
+            ses.set_freq(0.75)
+            orc.set_freq(0.75)
+            x64 = x[1].astype(np.complex128)
+            assert np.array_equal(ses.fdfd_apply(x64), orc.apply(x64))
+            assert np.array_equal(ses.lusgs_apply(x64), orc.lusgs(x64))
+            _lib.check(L.csem_apply32(ses.model, R, fr, dx, dy, None), "apply32")
+            _lib.check(L.csem_lusgs32(ses.model, R, fr, dx, dz, None), "lusgs32")
+            y1, z1 = _host(dy, x), _host(dz, x)
+            for k in REPEATABLE:
+                ms, cnt = _lib.kernel_loop_stats(k)
+                assert cnt > 0 and cnt % 3 == 0 and ms > 0, (k, cnt, ms)
+        finally:
+            L.csem_kernel_timing(0)
+        assert np.array_equal(y1, y0) and np.array_equal(z1, z0)
+        for d in (dx, dy, dz):
+            L.csem_free(d)
+    finally:
+        ses.close()
+
+
 def test_apply32_every_tile(workdir):
     """Every tile shape the autotuner may pick gives the same operator."""
     print("case apply32 tiles", flush=True)
