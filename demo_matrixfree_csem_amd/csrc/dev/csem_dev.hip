@@ -439,7 +439,8 @@ static int run_airbc(csem_model *m, hipStream_t s, const AirBufs &b)
   kt_launch(KT_AIR_ROWS_INV, s, 32.0 * N * (m->nrowx + m->nrowy), k_air_rows_inv, m->nrowx + m->nrowy, 256,
             row_lds, a);
   const int ng = (m->n1 + 1) * m->n2 + m->n1 * (m->n2 + 1);
-  kt_launch(KT_AIR_GHOST, s, 16.0 * ng, k_air_ghost, (ng + 255) / 256, 256, 0, a);
+  // bytes per ghost value: the four transform points it interpolates and the store
+  kt_launch(KT_AIR_GHOST, s, 80.0 * ng, k_air_ghost, (ng + 255) / 256, 256, 0, a);
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -452,7 +453,7 @@ int run_apply_bufs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStr
   a.hxg = b.hxg; a.hyg = b.hyg;
   if (m->airbc) {
     const int nh = m->n1 * m->n2;
-    kt_launch(KT_HZ0, s, 16.0 * nh, k_hz0, (nh + 255) / 256, 256, 0, a, b.hz0);
+    kt_launch(KT_HZ0, s, 48.0 * nh, k_hz0, (nh + 255) / 256, 256, 0, a, b.hz0);  // Ex, Ey planes, Hz
     int rc = run_airbc(m, s, b);
     if (rc) return rc;
   }
@@ -626,8 +627,16 @@ static int dot_partial_launch(long n, const z2 *a0, const z2 *b0, const z2 *a1, 
   DotArgs g;
   g.n = n; g.a[0] = a0; g.b[0] = b0; g.a[1] = a1; g.b[1] = b1; g.partials = partials;
   long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
-  if (a1) kt_launch(KT_DOT, s, 64.0 * n, k_dot_partial<2>, (unsigned)nch, RED_T, 0, g);
-  else kt_launch(KT_DOT, s, 32.0 * n, k_dot_partial<1>, (unsigned)nch, RED_T, 0, g);
+  // bytes: each distinct vector read once
+  const void *v[4] = {a0, b0, a1, b1};
+  int nv = 0;
+  for (int i = 0; i < 4; i++) {
+    bool seen = !v[i];
+    for (int j = 0; j < i; j++) seen |= v[j] == v[i];
+    nv += !seen;
+  }
+  if (a1) kt_launch(KT_DOT, s, 16.0 * nv * n, k_dot_partial<2>, (unsigned)nch, RED_T, 0, g);
+  else kt_launch(KT_DOT, s, 16.0 * nv * n, k_dot_partial<1>, (unsigned)nch, RED_T, 0, g);
   HIP_OK(hipGetLastError());
   return 0;
 }

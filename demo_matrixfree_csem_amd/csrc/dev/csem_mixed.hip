@@ -422,7 +422,7 @@ static int mx_apply(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
     kt_launch(KT_M_AIR_RINV, s, 16.0 * N * (m->nrowx + m->nrowy) * kt_rhs(R), k_mx_air_rows_inv,
               dim3(m->nrowx + m->nrowy, R), 256, 3 * (size_t)N * sizeof(c2), a);
     const int ng = (m->n1 + 1) * m->n2 + m->n1 * (m->n2 + 1);
-    kt_launch(KT_M_AIR_GHOST, s, 8.0 * ng * kt_rhs(R), k_mx_air_ghost, dim3((ng + 255) / 256, R), 256, 0, a);
+    kt_launch(KT_M_AIR_GHOST, s, 40.0 * ng * kt_rhs(R), k_mx_air_ghost, dim3((ng + 255) / 256, R), 256, 0, a);
   }
   // algorithmic bytes: read E and write y per right-hand side, sigma once
   const int tj = x->ap_tj[R] ? x->ap_tj[R] : 6, kc = x->ap_kc[R] ? x->ap_kc[R] : 8;
