@@ -165,27 +165,31 @@ def _lusgs_roofline(kl, dims, fp32):
     per unknown: forward Ex 40, Ey 56, Ez 72, backward 88 each (b, sigma,
     the values read, the value written) in complex FP64, half that in
     complex FP32 for the one right-hand side of the roofline pass.  The
-    application's time is the sum of its launches' event-timed durations:
-    FP32 six pre-passes, six sweeps and six copies to W/y (six times each
-    kernel's average over its six instances); FP64 six pre-passes and the
-    six sweeps."""
+    application's time is the sum of its launches' event-timed durations,
+    each kernel counted as often per application as the pass launched it:
+    FP32 six pre-passes, six sweeps and the two copies of the Ex sweeps'
+    values to W and y; FP64 six pre-passes and the six sweeps."""
     n1, n2, n3f = dims
     nc = [n1 * (n3f - 1) * (n2 - 1), n2 * (n3f - 1) * (n1 - 1), n3f * (n2 - 1) * (n1 - 1)]
     per = float(sum(c * (f + 88) for c, f in zip(nc, (40, 56, 72))))
     if fp32:
-        parts = [("k_mx_lu_pre", 6), ("k_mx_sgs", 6), ("k_mx_unskew", 6)]
+        napp = kl["k_mx_sgs"]["launches"] / 6.0 if "k_mx_sgs" in kl else 0
+        parts = [k for k in ("k_mx_lu_pre", "k_mx_sgs", "k_mx_unskew") if k in kl]
         per /= 2.0
-        what = "one FP32 LU-SGS application (one right-hand side): 6 pre-passes + 6 sweeps + 6 copies to W/y"
+        what = "one FP32 LU-SGS application (one right-hand side): 6 pre-passes + 6 sweeps + 2 copies (Ex)"
     else:
-        parts = [("k_lu_pre", 6)] + [("k_sgs_%s<%s>" % (c, d), 1) for d in ("fwd", "bwd") for c in ("ex", "ey", "ez")]
+        napp = kl["k_sgs_ex<fwd>"]["launches"] if "k_sgs_ex<fwd>" in kl else 0
+        parts = ["k_lu_pre"] + ["k_sgs_%s<%s>" % (c, d) for d in ("fwd", "bwd") for c in ("ex", "ey", "ez")]
         what = "one FP64 LU-SGS application: 6 pre-passes + 6 sweeps"
-    if any(k not in kl for k, _ in parts):
+    if not napp or any(k not in kl for k in parts):
         return None
-    us = sum(m * kl[k]["event_us"] for k, m in parts)
-    gbs = per / (us * 1e3)
+    us = {k: kl[k]["launches"] / napp * kl[k]["event_us"] for k in parts}
+    tot = sum(us.values())
+    gbs = per / (tot * 1e3)
     return {"what": what, "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_application": int(per), "avg_us": round(us, 3),
-            "parts_us": {k: round(m * kl[k]["event_us"], 3) for k, m in parts}}
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_application": int(per), "avg_us": round(tot, 3),
+            "parts_us": {k: round(v, 3) for k, v in us.items()},
+            "launches_per_application": {k: round(kl[k]["launches"] / napp, 2) for k in parts}}
 
 
 def apply_large(reps=20, copy_gbs=None):

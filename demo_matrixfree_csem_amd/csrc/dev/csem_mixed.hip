@@ -470,11 +470,12 @@ static dim3 mx_tile_grid3(const SkewGeo &g, int c)
 template <int C, bool FWD>
 static void mx_sgs_launch(dim3 g, int tb, size_t lds, hipStream_t s, const MxLu &a, double bytes)
 {
+  const int nt = tb + (C != 0 ? 64 : 0);  // Ey, Ez: and the store wave
   switch (tb / 64) {
-    case 1: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 1>, g, tb, lds, a); break;
-    case 2: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 2>, g, tb, lds, a); break;
-    case 3: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 3>, g, tb, lds, a); break;
-    default: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 4>, g, tb, lds, a); break;
+    case 1: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 1>, g, nt, lds, a); break;
+    case 2: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 2>, g, nt, lds, a); break;
+    case 3: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 3>, g, nt, lds, a); break;
+    default: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 4>, g, nt, lds, a); break;
   }
 }
 
@@ -495,30 +496,27 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
   double nc[3];
   for (int c = 0; c < 3; c++) {
     tb[c] = pk.AP[c];
-    lds[c] = (size_t)mxs_lds_bytes(g.A[c], g.B[c], pk.AP[c]);
+    lds[c] = (size_t)mxs_lds_bytes(g.A[c], g.B[c], pk.AP[c], c);
     gs[c] = dim3(8 * ((g.ns[c] + 7) / 8), R);
     nc[c] = (double)g.ns[c] * g.A[c] * g.B[c];
   }
   // bytes per swept cell and right-hand side each kernel must move: the
   // pre-pass reads b and the other components' W values it differences
-  // and writes T; the sweep reads T and h and writes the value to V; the
-  // copy reads V and writes W (not after the last sweep) and y (backward)
+  // and writes T; the sweep reads T and h and, for Ey and Ez, writes the
+  // value to W and (backward) y; for Ex it writes the value to V, and the
+  // copy reads V and writes W (forward) or y (backward)
   const double r = kt_rhs(R);
   kt_launch(KT_M_PRE, s, 16.0 * r * nc[0], k_mx_pre<0, true, R>, gqx, 256, 0, a);
   mx_sgs_launch<0, true>(gs[0], tb[0], lds[0], s, a, 24.0 * r * nc[0]);
   kt_launch(KT_M_UNSKEW, s, 16.0 * r * nc[0], k_mx_unskew<0, true, R>, gu[0], 256, 0, a);
   kt_launch(KT_M_PRE, s, 24.0 * r * nc[1], k_mx_pre<1, true, R>, gq[1], 256, 0, a);
   mx_sgs_launch<1, true>(gs[1], tb[1], lds[1], s, a, 24.0 * r * nc[1]);
-  kt_launch(KT_M_UNSKEW, s, 16.0 * r * nc[1], k_mx_unskew<1, true, R>, gu[1], 256, 0, a);
   kt_launch(KT_M_PRE, s, 32.0 * r * nc[2], k_mx_pre<2, true, R>, gq[2], 256, 0, a);
   mx_sgs_launch<2, true>(gs[2], tb[2], lds[2], s, a, 24.0 * r * nc[2]);
-  kt_launch(KT_M_UNSKEW, s, 16.0 * r * nc[2], k_mx_unskew<2, true, R>, gu[2], 256, 0, a);
   kt_launch(KT_M_PRE, s, 40.0 * r * nc[2], k_mx_pre<2, false, R>, gq[2], 256, 0, a);
-  mx_sgs_launch<2, false>(gs[2], tb[2], lds[2], s, a, 24.0 * r * nc[2]);
-  kt_launch(KT_M_UNSKEW, s, 24.0 * r * nc[2], k_mx_unskew<2, false, R>, gu[2], 256, 0, a);
+  mx_sgs_launch<2, false>(gs[2], tb[2], lds[2], s, a, 32.0 * r * nc[2]);
   kt_launch(KT_M_PRE, s, 40.0 * r * nc[1], k_mx_pre<1, false, R>, gq[1], 256, 0, a);
-  mx_sgs_launch<1, false>(gs[1], tb[1], lds[1], s, a, 24.0 * r * nc[1]);
-  kt_launch(KT_M_UNSKEW, s, 24.0 * r * nc[1], k_mx_unskew<1, false, R>, gu[1], 256, 0, a);
+  mx_sgs_launch<1, false>(gs[1], tb[1], lds[1], s, a, 32.0 * r * nc[1]);
   kt_launch(KT_M_PRE, s, 40.0 * r * nc[0], k_mx_pre<0, false, R>, gq[0], 256, 0, a);
   mx_sgs_launch<0, false>(gs[0], tb[0], lds[0], s, a, 24.0 * r * nc[0]);
   kt_launch(KT_M_UNSKEW, s, 16.0 * r * nc[0], k_mx_unskew<0, false, R>, gu[0], 256, 0, a);

@@ -876,10 +876,18 @@ __device__ __forceinline__ void mx_zero_frame(c2 *base, int nu, int nv, long su,
 // steps; a lane on a slot off the slice (before its row starts, after it
 // ends, rows past A, the lag at either end) reads a zero record (or none:
 // a raw-buffer load past the slice's records reads zero) and so computes 0.
-// The values go to V in the records' slot order, one contiguous run per
-// wave and step (the W and y cells of a diagonal are a row apart each, so
-// stores in the grid's order would touch a cache line per lane);
-// k_mx_unskew copies them to W and y.  Every value a step needs comes from registers
+// Where the values go.  The W and y cells of a diagonal are a row apart
+// each, so a wave's step stored in the grid's order touches a cache line
+// per lane; and a wave's vector-memory counter retires loads and stores in
+// issue order, so a wave waiting for its prefetched record also waits for
+// its earlier stores.  Ey and Ez (rows along x, contiguous in the grid): the
+// computing waves put each step's values into an LDS staging area (two
+// halves of MXS_BP steps), and one more wave of the block stores each half
+// to W and y while the others fill the other half, eight lanes per row, so
+// a store instruction writes eight rows' runs of MXS_BP values.  Ex (rows a
+// plane apart, runs only across slices): the values go to V in the records'
+// slot order, one contiguous run per wave and step, and k_mx_unskew copies
+// them to W and y through tiles that span slices.  Every value a step needs comes from registers
 // filled steps ahead: the records (MXS_P steps), the fast-axis coefficient
 // of the lane's column (MXS_PC steps, from an LDS table padded with zeros
 // so that no index is clamped) and the edge value (MXS_PD steps).  Raw-buffer
@@ -903,22 +911,28 @@ __device__ __forceinline__ void mx_zero_frame(c2 *base, int nu, int nv, long su,
 #define MXS_RING 32                      // ring slots per wave edge
 static_assert(MXS_RING > MXS_L - MXS_PD + MXS_BP, "edge ring slot rewritten before it is read");
 static_assert(MXS_U % MXS_RING == 0 && MXS_U % MXS_BP == 0 && MXS_U % MXS_P == 0 && MXS_U % MXS_PD == 0 &&
-                  MXS_U % MXS_PC == 0,
-              "sweep unroll");
-// LDS of a sweep block of nw waves: per wave an edge ring and MXS_RING + 64
-// slots the other lanes' ring stores land in, then the coefficient table
+                  MXS_U % MXS_PC == 0 && (MXS_U / MXS_BP) % 2 == 0,
+              "sweep unroll (an even number of staging halves per loop iteration)");
+// LDS of a sweep block of nw computing waves: per wave an edge ring and
+// MXS_RING + 64 slots the other lanes' ring stores land in, for Ey and Ez
+// the store wave's staging (two halves of nw x MXS_BP steps x MXS_SP), then
+// the coefficient table
 __host__ __device__ __forceinline__ int mxs_table_off(int AP, int nw) { return AP + (nw - 1) * MXS_L + MXS_U + MXS_PC + 4; }
 __host__ __device__ __forceinline__ int mxs_table_len(int A, int B, int AP, int nw)
 {
   return mxs_table_off(AP, nw) + A + B + (nw - 1) * MXS_L + MXS_U + MXS_PC + 4;
 }
-__host__ __device__ __forceinline__ long mxs_lds_bytes(int A, int B, int AP)
+#define MXS_SP 72  // staging slots per step: 64 rows, padded so that the store wave's reads (eight
+                   // steps of eight rows) fall in at most two-way bank conflicts
+__host__ __device__ __forceinline__ long mxs_lds_bytes(int A, int B, int AP, int c)
 {
   const int nw = AP / 64;
-  return (long)nw * (2 * MXS_RING + 64) * 8 + (long)mxs_table_len(A, B, AP, nw) * 4;
+  return (long)nw * (2 * MXS_RING + 64) * 8 + (c != 0 ? 2L * nw * MXS_BP * MXS_SP * 8 : 0) +
+         (long)mxs_table_len(A, B, AP, nw) * 4;
 }
+// the computing waves of a block of AP rows, and for Ey, Ez the store wave
 template <int C, bool FWD, int NW>
-__global__ __launch_bounds__(64 * NW) void k_mx_sgs(MxLu a)
+__global__ __launch_bounds__(64 * (NW + 1)) void k_mx_sgs(MxLu a)
 {
   extern __shared__ c2 lds[];
   const int nslice = a.sk.ns[C];
@@ -927,7 +941,8 @@ __global__ __launch_bounds__(64 * NW) void k_mx_sgs(MxLu a)
   const int r = blockIdx.y;
   if (q >= nslice || !mx_live(a.sc, r)) return;
   const int n1 = a.g.n1, n2 = a.g.n2;
-  constexpr int nt = 64 * NW, AP = nt, lagmax = (NW - 1) * MXS_L;
+  constexpr bool SW = (C != 0);  // a store wave writes W and y (Ex: V and k_mx_unskew)
+  constexpr int nt = 64 * (NW + (SW ? 1 : 0)), AP = 64 * NW, lagmax = (NW - 1) * MXS_L;
   constexpr int P = MXS_P;
   const int t = threadIdx.x, lane = t & 63;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -937,7 +952,8 @@ __global__ __launch_bounds__(64 * NW) void k_mx_sgs(MxLu a)
   const int s = 1 + t, sc = min(s, A);
   c2 *ring = lds;                                  // [NW][MXS_RING]: the edge row a wave receives
   c2 *junk = lds + NW * MXS_RING;                  // [NW][MXS_RING + 64]
-  float *cft = (float *)(junk + NW * (MXS_RING + 64));  // fast-axis coefficient of column f at f + OFF
+  c2 *stg = junk + NW * (MXS_RING + 64);           // SW: [2][NW][MXS_BP][MXS_SP], the store wave's halves
+  float *cft = (float *)(stg + (SW ? 2 * NW * MXS_BP * MXS_SP : 0));  // fast-axis coefficient of column f at f + OFF
   {
     const float *g0 = (C == 0) ? (FWD ? a.c2m : a.c2p) : (FWD ? a.c1m : a.c1p);
     for (int u = t; u < TL; u += nt) {
@@ -968,8 +984,64 @@ __global__ __launch_bounds__(64 * NW) void k_mx_sgs(MxLu a)
   const int z = __builtin_amdgcn_mbcnt_lo(0u, 0u);
   c2 *ring_in = ring + wv * MXS_RING + z;
   c2 *ring_st = send ? ring + (FWD ? wv + 1 : wv - 1) * MXS_RING : junk + wv * (MXS_RING + 64) + lane;
+  c2 *stg_w = stg + wv * MXS_BP * MXS_SP + lane;  // SW: this lane's staging slot of step 0 of half 0
   const int nex = ((nst + lagmax + MXS_BP - 1) / MXS_BP) * MXS_BP;  // steps run
   const int ntk = ((nex + MXS_U - 1) / MXS_U) * MXS_U;
+  // backward: the operator-grid cells of this slice outside the sweep range
+  // are zero rows (every thread of the block takes a share)
+  auto zero_frame = [&]() {
+    const int kl = 1 - a.g.kofs, kh = a.g.n3f - 1 - a.g.kofs;  // operator planes of the sweep range
+    c2 *y0 = a.y + r * a.g.n;
+    if (C == 0) mx_zero_frame(y0 + q, a.g.n3 + 1, n2 + 1, (long)n1 * (n2 + 1), n1, kl, kh, t, nt);
+    else if (C == 1) mx_zero_frame(y0 + a.g.lEx + (long)(n1 + 1) * q, a.g.n3 + 1, n1 + 1, (long)(n1 + 1) * n2, 1, kl, kh, t, nt);
+    else if (q - a.g.kofs >= 0)
+      mx_zero_frame(y0 + a.g.lEx + a.g.lEy + (long)(n1 + 1) * (n2 + 1) * (q - a.g.kofs), n2 + 1, n1 + 1, n1 + 1, 1, 1,
+                    n2 - 1, t, nt);
+  };
+  if (SW && wv == NW) {
+    // the store wave: after the barrier that ends half h (steps MXS_BP h ..
+    // MXS_BP h + MXS_BP - 1), store that half's values to W and (backward)
+    // y while the computing waves fill the other half.  Lane l takes step
+    // l % 8 of row 8 k + l / 8 of a wave: eight lanes write one row's run
+    // of consecutive columns (cell (q, s, f) is at qb + s SS + f).
+    const long PL = (C == 1) ? (long)(n1 + 1) * n2 : (long)(n1 + 1) * (n2 + 1);
+    const long SS = (C == 2) ? (long)(n1 + 1) : PL;
+    const long qb = (C == 1) ? (long)(n1 + 1) * q : PL * q;
+    const long wofs = (C == 1) ? a.fEx : a.fEx + a.fEy;
+    const long xofs = (C == 1) ? a.g.lEx : a.g.lEx + a.g.lEy;
+    const long wlen = (C == 1) ? a.fEy : a.fE - a.fEx - a.fEy;
+    const long ylen = (C == 1) ? a.g.lEy : a.g.n - a.g.lEx - a.g.lEy;
+    const __amdgpu_buffer_rsrc_t Wb = mx_rsrc(a.W + r * a.fE + wofs, wlen * (long)sizeof(c2));
+    const __amdgpu_buffer_rsrc_t Yb = mx_rsrc(a.y + r * a.g.n + xofs, ylen * (long)sizeof(c2));
+    const int u = lane % MXS_BP, rl = lane / MXS_BP;
+    static_assert(MXS_BP == 8, "eight lanes per row, eight rows per store");
+    __syncthreads();  // the computing waves' first barrier
+    const int nhalf = nex / MXS_BP;
+    for (int h = 0; h < nhalf; h++) {
+      __syncthreads();
+      const c2 *src = stg + (h & 1) * NW * MXS_BP * MXS_SP + u * MXS_SP + rl;
+      const int tau = h * MXS_BP + u;
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        const int lw = (FWD ? w : NW - 1 - w) * MXS_L;
+        const int d = FWD ? 2 + tau - lw : A + B - tau + lw;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const int sw = 1 + 64 * w + 8 * k + rl, scw = min(sw, A), f = d - sw;
+          const bool ok = sw <= A && f >= 1 && f <= B;
+          const long cell = qb + scw * SS + f;
+          const int kopw = ((C == 2) ? q : scw) - a.g.kofs;
+          const v2u v = __builtin_bit_cast(v2u, src[w * MXS_BP * MXS_SP + 8 * k]);
+          __builtin_amdgcn_raw_buffer_store_b64(v, Wb, ok ? (int)cell * (int)sizeof(c2) : MX_OOB, 0, 0);
+          if (!FWD)
+            __builtin_amdgcn_raw_buffer_store_b64(
+                v, Yb, ok && kopw >= 0 ? (int)(cell - (long)a.g.kofs * PL) * (int)sizeof(c2) : MX_OOB, 0, 0);
+        }
+      }
+    }
+    if (!FWD) zero_frame();
+    return;
+  }
   // table index of this lane's column at step tau: ib + tau forward, ib - tau backward
   const int ib = (FWD ? 2 - lag : A + B + lag) - s + OFF;
   // byte offset of this lane's record at step tau: rv0 + tau * RSTEP
@@ -1014,7 +1086,10 @@ __global__ __launch_bounds__(64 * NW) void k_mx_sgs(MxLu a)
       const float ux = __builtin_fmaf(cfv, prev.x, rc.x), uy = __builtin_fmaf(cfv, prev.y, rc.y);
       const float wx = __builtin_fmaf(cs, nbv.x, ux), wy = __builtin_fmaf(cs, nbv.y, uy);
       prev = cmk(rc.z * wx - rc.w * wy, rc.z * wy + rc.w * wx);
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, prev), Vb, (int)((unsigned)rof[uu % P] >> 1), 0, 0);
+      if (SW)  // MXS_U / MXS_BP is even: the half of step tk0 + uu is that of uu
+        stg_w[((uu / MXS_BP) & 1) * NW * MXS_BP * MXS_SP + (uu % MXS_BP) * MXS_SP] = prev;
+      else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, prev), Vb, (int)((unsigned)rof[uu % P] >> 1), 0, 0);
       ring_st[uu & (MXS_RING - 1)] = prev;
       if (uu % MXS_BP == MXS_BP - 1) {
         __syncthreads();
@@ -1028,15 +1103,7 @@ __global__ __launch_bounds__(64 * NW) void k_mx_sgs(MxLu a)
     }
   }
 swept:
-  if (!FWD) {  // operator-grid cells of this slice outside the sweep range are zero rows
-    const int kl = 1 - a.g.kofs, kh = a.g.n3f - 1 - a.g.kofs;  // operator planes of the sweep range
-    c2 *y0 = a.y + r * a.g.n;
-    if (C == 0) mx_zero_frame(y0 + q, a.g.n3 + 1, n2 + 1, (long)n1 * (n2 + 1), n1, kl, kh, t, nt);
-    else if (C == 1) mx_zero_frame(y0 + a.g.lEx + (long)(n1 + 1) * q, a.g.n3 + 1, n1 + 1, (long)(n1 + 1) * n2, 1, kl, kh, t, nt);
-    else if (q - a.g.kofs >= 0)
-      mx_zero_frame(y0 + a.g.lEx + a.g.lEy + (long)(n1 + 1) * (n2 + 1) * (q - a.g.kofs), n2 + 1, n1 + 1, n1 + 1, 1, 1,
-                    n2 - 1, t, nt);
-  }
+  if (!FWD) zero_frame();
 }
 
 // ---------------------------------------------------------------- Krylov
