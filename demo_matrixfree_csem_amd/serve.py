@@ -3,10 +3,11 @@
 Requests (a model, frequencies, sources, receivers) arrive over time.  The
 engine keeps every model it has built resident on the device (FP64 and FP32
 coefficients, airwave multipliers, and a pool of solve slots holding the
-Krylov vectors), splits each request into solves of up to `rhs_per_solve`
-right-hand sides (source x frequency pairs) and runs up to `workers` solves
-at once, each on its own stream: continuous batching -- a solve starts as
-soon as a worker and enough device memory are free while the others keep
+Krylov vectors), queues each request's right-hand sides (source x
+frequency pairs) and runs up to `workers` solves at once, each on its own
+stream, each taking up to `rhs_per_solve` waiting right-hand sides of one
+model, from one request or several: continuous batching -- a solve starts
+as soon as a worker and enough device memory are free while the others keep
 running, and solves of different requests and models overlap on the device.
 Models are built on a separate thread, so a new model's host setup does not
 hold up solves on resident ones.
@@ -24,14 +25,22 @@ request in flight are evicted, least recently used first, to make room.
 
 Every solve is csem_mixed_solve (FP32 LU-SGS + F-BiCGStab inside FP64
 iterative refinement) to the true FP64 residual |b - A x| / |b| < tol,
-tol = 1e-8/f as the reference's driver sets it (main.c:104-118).  Each
-right-hand side's arithmetic depends only on its batch, not on what else
-runs on the device, so a request gives the same receivers alone and in
-the mix.
+tol = 1e-8/f as the reference's driver sets it (main.c:104-118).  A
+right-hand side's arithmetic is the same in any batch (the FP32 kernels
+give the same values alone and batched, and every scalar and convergence
+test is per right-hand side) and does not depend on what else runs on the
+device, so a request gives the same receivers alone and in the mix.
 
     python -m demo_matrixfree_csem_amd.serve [--requests N] [--rate R] [--full]
+    python -m demo_matrixfree_csem_amd.serve --requests-file reqs.jsonl --out DIR
 
 prints one JSON line: p50/p99 latency, solves/s, the budget and the peaks.
+With --requests-file the requests come from a file (REQUEST_SCHEMA below)
+and each request's receivers are written under DIR/<id>/ in the
+reference's format (emf_0001.txt, one file per source, float32 as
+write_data.c:24-34 prints them) and in double precision
+(emf_0001_f64.txt, %.17e), with a result.json of latency, iterations and
+true residuals.
 """
 import argparse
 import concurrent.futures as cf
@@ -217,20 +226,24 @@ class Engine:
         return [hx[r * n:(r + 1) * n] for r in range(R)], its, rel
 
     def _work(self, wid, job):
-        i, rm, pairs, req = job
-        key, build, freqs, srcs, recs = req
+        """One batch solve: `items` are (request, source index, frequency)
+        right-hand sides of one model, from one request or several.
+        Returns per item the receivers in double precision (what
+        extract_emf rounds to float), the iterations and the true relative
+        residual."""
+        rm, items, requests = job
         bs = []
         with _HOST:
-            for s, f in pairs:
+            for i, s, f in items:
                 rm.ses.set_freq(f)  # the finite-volume source scales with i w mu0 (inject_extract.c:51)
-                bs.append(rm.ses.inject_at(*srcs[s]))
-        fs = [f for s, f in pairs]
+                bs.append(rm.ses.inject_at(*requests[i][3][s]))
+        fs = [f for i, s, f in items]
         xs, its, rel = self._solve(self.io[wid], rm, bs, fs, [solve_tol(f) for f in fs])
         with _HOST:
             rec = []
-            for (s, f), x in zip(pairs, xs):
+            for (i, s, f), x in zip(items, xs):
                 rm.ses.set_freq(f)
-                rec.append(rm.ses.extract_at(x, recs))
+                rec.append(rm.ses.extract_at(x, requests[i][4], double=True))
         return rec, its, rel
 
     def _estimate(self, survey):
@@ -251,8 +264,12 @@ class Engine:
     def run(self, requests, arrivals):
         """Serve `requests` (serve_request tuples) arriving `arrivals`
         seconds after the start.  Returns (results, stats); results[i] maps
-        (source index, frequency) -> (receivers complex64 (channels, nrec),
+        (source index, frequency) -> (receivers in double precision,
+        complex128 (channels, nrec): the values extract_emf rounds to float,
         inner iterations, true relative residual) and holds the latency.
+        A solve takes up to rhs_per_solve right-hand sides of one model,
+        the first waiting one and the next ones on that model from any
+        request.
 
         A request's model is made resident first (its survey built and its
         host setup and upload run on the builder thread, the upload only
@@ -270,7 +287,7 @@ class Engine:
         waiting = []           # (request, model, pairs) ready to admit
         building = {}          # model key -> {"fut", "survey", "est", "units", "stage"}
         order = sorted(range(len(requests)), key=lambda i: arrivals[i])
-        nxt, nsolves = 0, 0
+        nxt, nsolves, nbatches, nshared = 0, 0, 0, 0
         peak_acc, peak_dev = 0, 0
         self.reserved = 0
         pool = cf.ThreadPoolExecutor(self.workers)
@@ -284,9 +301,16 @@ class Engine:
             key, build, freqs, srcs, _ = requests[i]
             rm.busy += 1
             pairs = [(s, f) for s in range(len(srcs)) for f in freqs]
-            chunks = [pairs[q:q + self.R] for q in range(0, len(pairs), self.R)]
-            pending[i] = len(chunks)
-            waiting.extend((i, rm, c) for c in chunks)
+            pending[i] = len(pairs)
+            waiting.extend((i, rm, s, f) for s, f in pairs)
+
+        def next_batch():
+            """The first waiting right-hand side and, up to rhs_per_solve,
+            the next ones on the same model in arrival order, from any
+            request (cross-request batching)."""
+            rm = waiting[0][1]
+            take = [w for w in range(len(waiting)) if waiting[w][1] is rm][:self.R]
+            return rm, take
 
         try:
             while nxt < len(order) or arrived or waiting or running:
@@ -334,37 +358,42 @@ class Engine:
                     break
                 # admit solves first come first served while a worker and the memory are free
                 while waiting and free_workers:
-                    i, rm, c = waiting[0]
-                    need = rm.slot_bytes(len(c)) + max(0, 2 * len(c) * rm.n * 16 - 2 * self.io[free_workers[0]].cap)
+                    rm, take = next_batch()
+                    nb = len(take)
+                    need = rm.slot_bytes(nb) + max(0, 2 * nb * rm.n * 16 - 2 * self.io[free_workers[0]].cap)
                     if not self._evict_for(need, running):
                         break
-                    waiting.pop(0)
+                    items = [(waiting[w][0], waiting[w][2], waiting[w][3]) for w in take]
+                    for w in reversed(take):
+                        waiting.pop(w)
                     w = free_workers.pop(0)
                     running[w] = need
                     peak_acc = max(peak_acc, self._used(running))
-                    fu = pool.submit(self._work, w, (i, rm, c, requests[i]))
-                    fu.add_done_callback(lambda fu, w=w, i=i, rm=rm, c=c: done.put((w, i, rm, c, fu)))
+                    nbatches += 1
+                    nshared += len({it[0] for it in items}) > 1
+                    fu = pool.submit(self._work, w, (rm, items, requests))
+                    fu.add_done_callback(lambda fu, w=w, rm=rm, items=items: done.put((w, rm, items, fu)))
                 inflight = any(b["stage"] != "ready" for b in building.values())
                 blocked = waiting or any(b["stage"] == "ready" for b in building.values())
                 if blocked and not running and not inflight and all(m.busy for m in self.models.values()):
                     # nothing runs, nothing builds and nothing can be evicted
                     raise RuntimeError("a model or a solve does not fit the budget of %d bytes" % self.budget)
                 try:
-                    w, i, rm, c, fu = done.get(timeout=0.002)
+                    w, rm, items, fu = done.get(timeout=0.002)
                 except queue.Empty:
                     continue
                 peak_dev = max(peak_dev, _device_used() - base_dev)
                 rec, its, rel = fu.result()  # re-raises a worker's error
                 del running[w]
                 free_workers.append(w)
-                nsolves += len(c)
-                for (s, f), r, it, rr in zip(c, rec, its, rel):
+                nsolves += len(items)
+                for (i, s, f), r, it, rr in zip(items, rec, its, rel):
                     recs[i][(s, f)] = (r, int(it), float(rr))
-                pending[i] -= 1
-                if pending[i] == 0:
-                    rm.busy -= 1
-                    rm.last_used = time.time()
-                    results[i] = {"receivers": recs[i], "latency": time.time() - t0 - arrivals[i]}
+                    pending[i] -= 1
+                    if pending[i] == 0:
+                        rm.busy -= 1
+                        rm.last_used = time.time()
+                        results[i] = {"receivers": recs[i], "latency": time.time() - t0 - arrivals[i]}
         finally:
             pool.shutdown(wait=True)
             builder.shutdown(wait=True)
@@ -377,7 +406,7 @@ class Engine:
                  "budget_bytes": self.budget, "peak_admitted_bytes": int(peak_acc),
                  "peak_device_bytes": int(peak_dev), "resident_models": len(self.models),
                  "models_built": self.nbuilt, "evictions": self.evictions, "workers": self.workers, "rhs_per_solve": self.R,
-                 "fp64_continued": self.fp64_continued}
+                 "batches": nbatches, "cross_request_batches": nshared, "fp64_continued": self.fp64_continued}
         return results, stats
 
     def close(self):
@@ -387,6 +416,134 @@ class Engine:
         for b in self.io:
             b.free()
         shutil.rmtree(self.work, ignore_errors=True)
+
+
+# ------------------------------------------------------------ request files
+REQUEST_SCHEMA = """One request per line of a JSON-lines file (or a JSON list):
+
+  {"id": "r0001",                      # names the output directory
+   "arrival_s": 0.0,                   # arrival time after the start
+   "model": {"files": {"fx1": "...", "fx2": "...", "fx3": "...",
+                       "frho11": "...", "frho22": "...", "frho33": "..."},
+             "flags": {"nb": 10, "airbc": 1, "fvm": 0, "lextend": 40000,
+                       "n1fft": 1024, "chrec": "Ex,Ey,Ez"}}
+   "freqs": [0.25, 0.75],              # Hz
+   "sources": [[x, y, z], ...],        # Ex dipoles
+   "receivers": [[x, y, z], ...]}
+
+"files" are the reference's model inputs (emf_init_free.c:77-150: float32
+node coordinates, n + 1 per axis; float32 cell resistivities, nx ny nz, x
+fastest), relative paths taken from the request file's directory.  Instead
+of "files" a model may be {"synthetic": {"grid": [nx, ny, nz], "kind":
+"layered" | "grf", "seed": 1001, "median": 2.0}} (surveys.marine).  Flags
+default to those of the reference's examples (nb 10, airbc 1, fvm 0, 1024^2
+transform, channels Ex,Ey,Ez); requests with the same model inputs and
+flags share one resident model."""
+
+_MODEL_FLAGS = {"nb": 10, "airbc": 1, "fvm": 0, "lextend": None, "n1fft": 1024, "chrec": "Ex,Ey,Ez"}
+
+
+def read_model_files(paths):
+    """A surveys.Model from the reference's binary model inputs."""
+    x = [np.fromfile(paths[k], np.float32) for k in ("fx1", "fx2", "fx3")]
+    nx, ny, nz = (len(a) - 1 for a in x)
+    if min(nx, ny, nz) < 1:
+        raise ValueError("node files need at least two coordinates each")
+    rho = []
+    for k in ("frho11", "frho22", "frho33"):
+        r = np.fromfile(paths[k], np.float32)
+        if r.size != nx * ny * nz:
+            raise ValueError("%s holds %d values, the grid has %d cells" % (paths[k], r.size, nx * ny * nz))
+        rho.append(r.reshape(nz, ny, nx))
+    return surveys.Model(x[0], x[1], x[2], *rho)
+
+
+def request_from_dict(d, base_dir="."):
+    """(id, arrival, serve_request tuple) of one request-file entry."""
+    rid = str(d["id"])
+    freqs = sorted(float(np.float32(f)) for f in d["freqs"])
+    srcs = [tuple(float(v) for v in p[:3]) for p in d["sources"]]
+    recs = [tuple(float(v) for v in p[:3]) for p in d["receivers"]]
+    if not freqs or not srcs or not recs:
+        raise ValueError("request %s: freqs, sources and receivers must not be empty" % rid)
+    spec = d["model"]
+    fl = dict(_MODEL_FLAGS)
+    fl.update(spec.get("flags", {}))
+    flags = dict(fvm=int(fl["fvm"]), airbc=int(fl["airbc"]), preco=1, algopt=1, tol=1e-8, niter=2000,
+                 nb=int(fl["nb"]), chsrc="Ex", chrec=str(fl["chrec"]), n1fft=int(fl["n1fft"]),
+                 n2fft=int(fl["n1fft"]))
+    if fl["lextend"] is not None:  # padding length (default 40 km, emf_init)
+        flags["lextend"] = float(fl["lextend"])
+    fkey = tuple(sorted(flags.items()))
+    if "files" in spec:
+        paths = {k: os.path.abspath(os.path.join(base_dir, spec["files"][k]))
+                 for k in ("fx1", "fx2", "fx3", "frho11", "frho22", "frho33")}
+        key = ("files", tuple(paths[k] for k in sorted(paths)), fkey)
+
+        def model():
+            return read_model_files(paths)
+        name = "files_" + os.path.basename(paths["frho11"])
+    else:
+        sy = spec["synthetic"]
+        g = tuple(int(v) for v in sy["grid"])
+        kind, seed, median = str(sy.get("kind", "layered")), int(sy.get("seed", 1000)), float(sy.get("median", 2.0))
+        key = ("synthetic", g, kind, seed, median, fkey)
+
+        def model():
+            return surveys.marine(seed, g[0], g[1], g[2], kind=kind, median=median)
+        name = "synthetic_%dx%dx%d_%s%d" % (g + (kind, seed))
+
+    def build():
+        return surveys.Survey(name, model(), surveys.Acquisition(surveys._acq_table(srcs[:1]),
+                                                                 surveys._acq_table(recs)), freqs, dict(flags))
+    return rid, float(d.get("arrival_s", 0.0)), (key, build, freqs, srcs, recs)
+
+
+def load_requests(path):
+    """[(id, arrival, request)] from a request file (REQUEST_SCHEMA)."""
+    with open(path) as f:
+        text = f.read()
+    t = text.lstrip()
+    entries = json.loads(t) if t.startswith("[") else [json.loads(l) for l in text.splitlines() if l.strip()]
+    base = os.path.dirname(os.path.abspath(path))
+    out = [request_from_dict(d, base) for d in entries]
+    ids = [r[0] for r in out]
+    if len(set(ids)) != len(ids):
+        raise ValueError("request ids must be unique")
+    return out
+
+
+def write_results(outdir, ids, requests, results):
+    """Each request's receivers under outdir/<id>/: per source s
+    emf_%04d.txt (s + 1) in the reference's format (write_data.c:24-34:
+    channel, then frequency, then receiver; float32 printed with %e) and
+    emf_%04d_f64.txt with the double values (%.17e), plus result.json."""
+    for rid, req, res in zip(ids, requests, results):
+        key, build, freqs, srcs, recs = req
+        d = os.path.join(outdir, rid)
+        os.makedirs(d, exist_ok=True)
+        info = {"id": rid, "latency_s": round(res["latency"], 4), "solves": []}
+        for s in range(len(srcs)):
+            rows32, rows64 = [], []
+            nch = res["receivers"][(s, freqs[0])][0].shape[0]
+            for ch in range(nch):
+                for fi, f in enumerate(freqs):
+                    r = res["receivers"][(s, f)][0][ch]
+                    for ir in range(len(recs)):
+                        v32 = np.complex64(r[ir])
+                        rows32.append("%d \t %d \t %d \t %d \t %e \t %e" % (s + 1, ir + 1, ch + 1, fi + 1,
+                                                                             v32.real, v32.imag))
+                        rows64.append("%d \t %d \t %d \t %d \t %.17e \t %.17e" % (s + 1, ir + 1, ch + 1, fi + 1,
+                                                                                   r[ir].real, r[ir].imag))
+            head = "iTx \t iRx    ichrec  ifreq \t emf_real \t emf_imag\n"
+            for suffix, rows in (("", rows32), ("_f64", rows64)):
+                with open(os.path.join(d, "emf_%04d%s.txt" % (s + 1, suffix)), "w") as f:
+                    f.write(head + "\n".join(rows) + "\n")
+            for f in freqs:
+                _, it, rr = res["receivers"][(s, f)]
+                info["solves"].append({"source": s + 1, "freq": f, "iterations": it, "true_relres": rr})
+        with open(os.path.join(d, "result.json"), "w") as f:
+            json.dump(info, f, indent=1)
 
 
 def poisson_arrivals(n, rate, seed):
@@ -414,20 +571,33 @@ def main():
     ap.add_argument("--full", action="store_true",
                     help="the full request distribution (grids up to 180x120x120, 1-6 frequencies, "
                          "1-4 sources)")
+    ap.add_argument("--requests-file", default=None, help="requests from a file (see REQUEST_SCHEMA); their "
+                    "arrival_s replace the Poisson arrivals")
+    ap.add_argument("--out", default=None, help="with --requests-file: write each request's receivers here")
     args = ap.parse_args()
-    if args.full:
-        reqs = [surveys.serve_request(args.seed * 1000 + i) for i in range(args.requests)]
+    ids = None
+    if args.requests_file:
+        loaded = load_requests(args.requests_file)
+        ids = [r[0] for r in loaded]
+        arrivals = [r[1] for r in loaded]
+        reqs = [r[2] for r in loaded]
     else:
-        reqs = quick_requests(args.requests, args.seed)
+        if args.full:
+            reqs = [surveys.serve_request(args.seed * 1000 + i) for i in range(args.requests)]
+        else:
+            reqs = quick_requests(args.requests, args.seed)
+        arrivals = poisson_arrivals(len(reqs), args.rate, args.seed)
     budget = None if args.budget_gib is None else int(args.budget_gib * 2 ** 30)
     eng = Engine(budget_bytes=budget, workers=args.workers, rhs_per_solve=args.rhs)
     try:
         if not args.cold:
             eng.preload(reqs)
-        _, stats = eng.run(reqs, poisson_arrivals(len(reqs), args.rate, args.seed))
+        results, stats = eng.run(reqs, arrivals)
     finally:
         eng.close()
-    stats["rate_per_s"] = args.rate
+    if ids is not None and args.out:
+        write_results(args.out, ids, reqs, results)
+    stats["rate_per_s"] = None if args.requests_file else args.rate
     stats["warm"] = not args.cold
     print(json.dumps(stats), flush=True)
     return 0

@@ -155,9 +155,10 @@ class Session:
         finally:
             a.src_x1[0], a.src_x2[0], a.src_x3[0] = old
 
-    def extract_at(self, x, receivers):
+    def extract_at(self, x, receivers, double=False):
         """extract_emf at the given receiver positions [(x, y, z), ...];
-        returns complex64 (nchrec, nrec)."""
+        returns complex64 (nchrec, nrec), or with `double` the values
+        before the float store (extract_emf_double), complex128."""
         a, e, L = self.acq, self.emf, lib()
         nr = len(receivers)
         r = np.asarray(receivers, np.float32).reshape(nr, -1)
@@ -172,6 +173,8 @@ class Session:
             a.nrec = nr
             for k, b in zip(names, bufs):
                 setattr(a, k, C.cast(b.ctypes.data, C.POINTER(C.c_float)))
+            if double:
+                return self.extract_double(x)
             e.dcal_fd = L.alloc3complexf(nr, e.nfreq, e.nchrec)
             out = self.extract(x)
             L.free3complexf(e.dcal_fd)

@@ -119,3 +119,67 @@ def test_engine_finishes_a_stalled_solve_in_fp64(workdir):
     e = _rec_err(rec, want)
     print("served after %d iterations (relres %.2e); FP64 alone %d; receiver error %.2e" % (it, rr, itp, e), flush=True)
     assert e < 1e-6, e
+
+
+def test_request_file_served_and_written(workdir):
+    """A request file (serve.REQUEST_SCHEMA) served with two right-hand
+    sides per solve on one worker: the two single-pair requests on the
+    file-given model share one solve (cross-request batching); every
+    request's emf_0001.txt is in the reference's format and its receivers
+    equal those of this package's csem_main (the reference's FP64 solver)
+    on the same inputs within 1e-6 of the channel's largest value, and the
+    double-precision file holds what the float file rounds."""
+    import json
+    import os
+    import subprocess
+    from demo_matrixfree_csem_amd.serve import load_requests, write_results
+    s = surveys.tiny(6, nx=16, ny=14, nz=10)
+    s.flags.update(n1fft=64, n2fft=64, niter=1000)
+    mdir = os.path.join(workdir, "model")
+    surveys.write_files(mdir, s)
+    files = {k: os.path.join("model", k) for k in ("fx1", "fx2", "fx3", "frho11", "frho22", "frho33")}
+    src = [float(v) for v in s.acq.sources[0][:3]]
+    recs = [[float(v) for v in r[:3]] for r in s.acq.receivers]
+    reqs = [{"id": "p%d" % k, "model": {"files": files, "flags": {"n1fft": 64, "nb": s.flags["nb"]}},
+             "freqs": [f], "sources": [src], "receivers": recs} for k, f in enumerate((0.5, 1.25))]
+    path = os.path.join(workdir, "reqs.jsonl")
+    with open(path, "w") as fh:
+        fh.write("\n".join(json.dumps(r) for r in reqs) + "\n")
+    loaded = load_requests(path)
+    print("case serve request file", flush=True)
+    eng = Engine(workers=1, rhs_per_solve=2)
+    try:
+        res, st = eng.run([r[2] for r in loaded], [r[1] for r in loaded])
+    finally:
+        eng.close()
+    print(st, flush=True)
+    assert st["batches"] == 1 and st["cross_request_batches"] == 1
+    out = os.path.join(workdir, "out")
+    write_results(out, [r[0] for r in loaded], [r[2] for r in loaded], res)
+    # the reference's solver on the same inputs, one frequency per run
+    main = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "demo_matrixfree_csem_amd",
+                        "csem_main")
+    for k, f in enumerate((0.5, 1.25)):
+        _check_converged(res[k])
+        run = os.path.join(workdir, "main%d" % k)
+        os.makedirs(run)
+        s.freqs = [f]
+        r = subprocess.run([main] + s.argv(mdir), cwd=run, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        ref = np.loadtxt(os.path.join(run, "emf_0001.txt"), skiprows=1)
+        got = np.loadtxt(os.path.join(out, "p%d" % k, "emf_0001.txt"), skiprows=1)
+        g64 = np.loadtxt(os.path.join(out, "p%d" % k, "emf_0001_f64.txt"), skiprows=1)
+        assert np.array_equal(got[:, :4], ref[:, :4])
+        v, w = got[:, 4] + 1j * got[:, 5], ref[:, 4] + 1j * ref[:, 5]
+        v64 = g64[:, 4] + 1j * g64[:, 5]
+        for ch in range(3):
+            sel = ref[:, 2] == ch + 1
+            scale = np.abs(w[sel]).max()
+            e = np.abs(v[sel] - w[sel]).max() / scale
+            print("request p%d channel %d: served vs csem_main %.2e" % (k, ch, e), flush=True)
+            assert e < 1e-6, (k, ch, e)
+        # the float table is the double one rounded, to its printed digits
+        for a, b in zip(open(os.path.join(out, "p%d" % k, "emf_0001.txt")).read().splitlines()[1:], v64):
+            t = a.split()
+            c = np.complex64(b)
+            assert t[4:] == ["%e" % c.real, "%e" % c.imag]

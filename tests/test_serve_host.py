@@ -2,13 +2,16 @@
 arrival process, and the per-request source / receiver helpers, which must
 give what the reference's own inject_source / extract_emf give for a survey
 whose acquisition files hold that source and those receivers
-(inject_extract.c:24-130, 132-200)."""
+(inject_extract.c:24-130, 132-200); the request-file schema and the
+result files (write_data.c:24-34)."""
+import json
 import os
 
 import numpy as np
+import pytest
 
 from demo_matrixfree_csem_amd import surveys
-from demo_matrixfree_csem_amd.serve import poisson_arrivals, solve_tol
+from demo_matrixfree_csem_amd.serve import load_requests, poisson_arrivals, solve_tol, write_results
 from demo_matrixfree_csem_amd.session import Session
 
 SMALL = [(24, 24, 16), (32, 28, 20)]
@@ -87,3 +90,89 @@ def test_extract_at_equals_reference_extract(workdir):
     assert np.array_equal(ses.extract_at(x, recs[3:7]), full[:, 3:7])
     # the session's receivers are restored
     assert np.array_equal(ses.extract(x), full)
+
+
+def _request_file(workdir):
+    """A request file with a model given by the reference's binary inputs
+    (written by surveys.write_files) and one synthetic model."""
+    s = surveys.tiny(5, nx=12, ny=10, nz=8)
+    mdir = os.path.join(workdir, "model")
+    surveys.write_files(mdir, s)
+    files = {k: os.path.join("model", k) for k in ("fx1", "fx2", "fx3", "frho11", "frho22", "frho33")}
+    reqs = [
+        {"id": "a", "arrival_s": 0.5, "model": {"files": files, "flags": {"n1fft": 64}},
+         "freqs": [0.75, 0.25], "sources": [[0.0, 0.0, 950.0]], "receivers": [[-200.0, 0.0, 1000.0], [200.0, 0.0, 1000.0]]},
+        {"id": "b", "model": {"files": files, "flags": {"n1fft": 64}},
+         "freqs": [0.5], "sources": [[10.0, 0.0, 950.0], [-10.0, 0.0, 950.0]], "receivers": [[0.0, 0.0, 1000.0]]},
+        {"id": "c", "model": {"synthetic": {"grid": [24, 24, 16], "kind": "grf", "seed": 1003, "median": 4.0},
+                              "flags": {"n1fft": 64, "chrec": "Ex,Ez"}},
+         "freqs": [1.25], "sources": [[0.0, 0.0, 950.0]], "receivers": [[0.0, 0.0, 1000.0]]},
+    ]
+    path = os.path.join(workdir, "reqs.jsonl")
+    with open(path, "w") as f:
+        for r in reqs:
+            f.write(json.dumps(r) + "\n")
+    return s, path
+
+
+def test_request_file_schema(workdir):
+    """load_requests: models from the reference's binary inputs are read
+    back exactly and shared between requests with the same inputs and
+    flags; synthetic models are the survey generator's; bad inputs are
+    refused."""
+    s, path = _request_file(workdir)
+    loaded = load_requests(path)
+    assert [r[0] for r in loaded] == ["a", "b", "c"] and [r[1] for r in loaded] == [0.5, 0.0, 0.0]
+    (ka, ba, fa, sa, ra), (kb, bb, *_), (kc, bc, fc, *_) = [r[2] for r in loaded]
+    assert ka == kb and ka != kc
+    assert fa == [0.25, 0.75] and sa == [(0.0, 0.0, 950.0)] and len(ra) == 2
+    sv = ba()
+    for k in ("x1node", "x2node", "x3node", "rho11", "rho22", "rho33"):
+        assert np.array_equal(getattr(sv.model, k), np.asarray(getattr(s.model, k), np.float32)), k
+    assert sv.flags["n1fft"] == 64 and "lextend" not in sv.flags
+    sc = bc()
+    want = surveys.marine(1003, 24, 24, 16, kind="grf", median=4.0)
+    assert np.array_equal(sc.model.rho11, want.rho11) and sc.flags["chrec"] == "Ex,Ez"
+    # a resistivity file of the wrong size is refused
+    bad = os.path.join(workdir, "model", "frho22")
+    np.zeros(5, np.float32).tofile(bad)
+    with pytest.raises(ValueError):
+        load_requests(path)[0][2][1]()
+    with open(path, "a") as f:
+        f.write(json.dumps({"id": "a", "model": {"synthetic": {"grid": [8, 8, 8]}}, "freqs": [1.0],
+                            "sources": [[0, 0, 0]], "receivers": [[0, 0, 0]]}) + "\n")
+    with pytest.raises(ValueError):
+        load_requests(path)
+
+
+def test_write_results_format(workdir):
+    """write_results: the reference's table (write_data.c:24-34: channel,
+    frequency, receiver; float32 printed with %e) per source, the double
+    values beside it, and a result.json."""
+    freqs, srcs, recs = [0.25, 0.75], [(0, 0, 950), (5, 0, 950)], [(0, 0, 1000), (1, 0, 1000), (2, 0, 1000)]
+    rng = np.random.default_rng(3)
+    res = {"latency": 1.25, "receivers": {}}
+    for s in range(2):
+        for f in freqs:
+            r = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+            res["receivers"][(s, f)] = (r * 1e-9, 100 + s, 1e-9)
+    write_results(workdir, ["q"], [(None, None, freqs, srcs, recs)], [res])
+    d = os.path.join(workdir, "q")
+    for s in range(2):
+        rows = open(os.path.join(d, "emf_%04d.txt" % (s + 1))).read().splitlines()
+        rows64 = open(os.path.join(d, "emf_%04d_f64.txt" % (s + 1))).read().splitlines()
+        assert rows[0].split() == ["iTx", "iRx", "ichrec", "ifreq", "emf_real", "emf_imag"]
+        assert len(rows) == 1 + 3 * 2 * 3 and len(rows64) == len(rows)
+        k = 1
+        for ch in range(3):
+            for fi, f in enumerate(freqs):
+                for ir in range(3):
+                    t, t64 = rows[k].split(), rows64[k].split()
+                    v = res["receivers"][(s, f)][0][ch, ir]
+                    assert [int(x) for x in t[:4]] == [s + 1, ir + 1, ch + 1, fi + 1]
+                    v32 = np.complex64(v)
+                    assert t[4:] == ["%e" % v32.real, "%e" % v32.imag]
+                    assert complex(float(t64[4]), float(t64[5])) == v
+                    k += 1
+    info = json.load(open(os.path.join(d, "result.json")))
+    assert info["latency_s"] == 1.25 and len(info["solves"]) == 4
