@@ -102,10 +102,11 @@ HOST_SYMBOLS = [
     "csem_active_model", "csem_last_iterations", "csem_fdfd_prepare", "csem_model_from_emf",
     "iproc", "nproc", "ierr", "aphi_apply", "gmres_init", "gmres_free", "gmres_apply",
     "Acsr_init", "Acsr_free", "Acsr_apply", "Acsr_lusgs_apply", "Acoo_init", "Acoo_free", "Acoo_solve",
-    "extract_emf_double", "write_data_double",
+    "extract_emf_double", "write_data_double", "csem_extend_grid", "csem_model_build_device",
 ]
 DEV_SYMBOLS = [
-    "csem_last_error", "csem_freq_consts", "csem_model_create", "csem_model_destroy", "csem_model_len",
+    "csem_last_error", "csem_freq_consts", "csem_model_create", "csem_model_create_raw", "csem_model_sigma",
+    "csem_model_multiplier", "csem_model_destroy", "csem_model_len",
     "csem_model_full_len", "csem_model_bytes", "csem_apply", "csem_lusgs", "csem_curl_h", "csem_airbc",
     "csem_fft2", "csem_dot", "csem_dot_noalloc", "csem_bicgstab", "csem_fgmres", "csem_gmres", "csem_scatter_add", "csem_gather",
     "csem_malloc", "csem_free", "csem_memcpy_h2d", "csem_memcpy_d2h", "csem_memset", "csem_copy", "csem_sync",
@@ -171,6 +172,9 @@ def lib():
     L.csem_active_model.restype = vp
     L.csem_fdfd_prepare.argtypes = [C.POINTER(EmfT)]
     L.csem_model_from_emf.argtypes = [C.POINTER(EmfT), C.POINTER(vp)]
+    L.csem_model_build_device.argtypes = [C.POINTER(AcqT), C.POINTER(EmfT), C.POINTER(vp)]
+    L.csem_model_sigma.argtypes = [vp, ip, vp]
+    L.csem_model_multiplier.argtypes = [vp, ip, vp]
     for f in ("emf_init", "emf_free", "fdfd_init", "fdfd_free", "extend_model_free"):
         getattr(L, f).argtypes = [C.POINTER(EmfT)]
     L.acq_init.argtypes = [C.POINTER(AcqT), C.POINTER(EmfT)]

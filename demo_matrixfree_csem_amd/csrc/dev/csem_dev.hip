@@ -265,14 +265,162 @@ extern "C" void csem_model_destroy(csem_model *m)
   delete m;
 }
 
-extern "C" int csem_model_create(const csem_model_desc *d, csem_model **out)
+// ------------------------------------------------- device-side model setup
+#define CSEM_PI 3.14159265358979323846264  // the host's pi (cstd.h:35-37)
+// extend_model.c:108-175: the extended cells, conductivity for 11 and 22
+// and resistivity for 33, from the core cells (float, x fastest): nb
+// lateral padding columns and rows repeat the edge cells; the nb planes on
+// top are air; bottom padding plane n3-1-k copies row k of the last core
+// plane (quirk Q2: extend_model.c:169-173 index [k] where [j] is meant),
+// a padding row, so the last plane's first core row.
+struct CellArgs {
+  int nx, ny, nz, nb, n1, n2, n3;
+  const float *rho[3];
+  double rho_air;
+  double *c[3];
+};
+__global__ void k_setup_cells(CellArgs a)
+{
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)a.n1 * a.n2 * a.n3) return;
+  const int i = (int)(e % a.n1), j = (int)((e / a.n1) % a.n2), k = (int)(e / ((long)a.n1 * a.n2));
+  const int nb = a.nb;
+  const bool air = nb > 0 && k < nb;
+  int kc = k - nb, jc = j - nb;
+  if (nb > 0 && k >= a.n3 - nb) {
+    kc = a.nz - 1;
+    jc = (a.n3 - 1 - k) - nb;
+  }
+  kc = min(max(kc, 0), a.nz - 1);
+  jc = min(max(jc, 0), a.ny - 1);
+  const int ic = min(max(i - nb, 0), a.nx - 1);
+  const long src = ((long)kc * a.ny + jc) * a.nx + ic;
+  for (int q = 0; q < 3; q++) {
+    const double r = air ? a.rho_air : (double)a.rho[q][src];
+    a.c[q][e] = (q == 2) ? r : 1. / r;
+  }
+}
+
+// extend_model.c:180-252: the edge averages, each edge from the four cells
+// around it weighted by their half-widths, then the boundary edges copied
+// from the nearest interior ones (the copies compose to the interior value
+// at the clamped indices).  Same expressions as the host, no contraction.
+struct SigArgs {
+  int n1, n2, n3;
+  const double *d1, *d2, *d3, *d1s, *d2s, *d3s;
+  const double *c11, *c22, *r33;
+  double *s11, *s22, *s33;
+};
+__global__ void k_setup_sigma(SigArgs a)
+{
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
+  const long l11 = (long)n1 * (n2 + 1) * (n3 + 1), l22 = (long)(n1 + 1) * n2 * (n3 + 1),
+             l33 = (long)(n1 + 1) * (n2 + 1) * n3;
+#define CC(c, k, j, i) (c)[((long)(k) * n2 + (j)) * n1 + (i)]
+  if (e < l11) {
+    const int i = (int)(e % n1), j = (int)((e / n1) % (n2 + 1)), k = (int)(e / ((long)n1 * (n2 + 1)));
+    const int kk = min(max(k, 1), n3 - 1), jj = min(max(j, 1), n2 - 1);
+    a.s11[e] = 0.25 *
+               ((a.d2s[jj] * CC(a.c11, kk, jj, i) + a.d2s[jj - 1] * CC(a.c11, kk, jj - 1, i)) * a.d3s[kk] +
+                (a.d2s[jj] * CC(a.c11, kk - 1, jj, i) + a.d2s[jj - 1] * CC(a.c11, kk - 1, jj - 1, i)) * a.d3s[kk - 1]) /
+               (a.d2[jj] * a.d3[kk]);
+  } else if (e < l11 + l22) {
+    const long f = e - l11;
+    const int i = (int)(f % (n1 + 1)), j = (int)((f / (n1 + 1)) % n2), k = (int)(f / ((long)(n1 + 1) * n2));
+    const int kk = min(max(k, 1), n3 - 1), ii = min(max(i, 1), n1 - 1);
+    a.s22[f] = 0.25 *
+               ((a.d1s[ii] * CC(a.c22, kk, j, ii) + a.d1s[ii - 1] * CC(a.c22, kk, j, ii - 1)) * a.d3s[kk] +
+                (a.d1s[ii] * CC(a.c22, kk - 1, j, ii) + a.d1s[ii - 1] * CC(a.c22, kk - 1, j, ii - 1)) * a.d3s[kk - 1]) /
+               (a.d1[ii] * a.d3[kk]);
+  } else if (e < l11 + l22 + l33) {
+    const long f = e - l11 - l22;
+    const int i = (int)(f % (n1 + 1)), j = (int)((f / (n1 + 1)) % (n2 + 1)), k = (int)(f / ((long)(n1 + 1) * (n2 + 1)));
+    const int jj = min(max(j, 1), n2 - 1), ii = min(max(i, 1), n1 - 1);
+    const double t =
+        0.25 *
+        ((a.d1s[ii] * CC(a.r33, k, jj, ii) + a.d1s[ii - 1] * CC(a.r33, k, jj, ii - 1)) * a.d2s[jj] +
+         (a.d1s[ii] * CC(a.r33, k, jj - 1, ii) + a.d1s[ii - 1] * CC(a.r33, k, jj - 1, ii - 1)) * a.d2s[jj - 1]) /
+        (a.d1[ii] * a.d2[jj]);
+    a.s33[f] = 1. / t;
+  }
+#undef CC
+}
+
+// airbc_init_free.c:69-84: the upward-continuation and derivative
+// multipliers, in the C99 complex arithmetic the host expression performs:
+// exp(-kz dz/2) * I * kx / (kz + 1e-15) scales componentwise, then the
+// product with cexp(-I kx d1s[0]/2) = (cos, sin) of ((-kx) 0.5) d1s[0].
+// exp, sin and cos are the device's (within a few ulp of the host's).
+__global__ void k_setup_mult(int N, double dkx, double dky, double d3fft, double h1, double h2, z2 *sH1, z2 *sH2)
+{
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)N * N) return;
+  const int i1 = (int)(e % N), i2 = (int)(e / N);
+  const double ky = (i2 <= N / 2) ? i2 * dky : (i2 - N) * dky;
+  const double kx = (i1 <= N / 2) ? i1 * dkx : (i1 - N) * dkx;
+  const double kz = sqrt(kx * kx + ky * ky);
+  const double ez = exp(-kz * 0.5 * d3fft), den = kz + 1.e-15;
+  for (int c = 0; c < 2; c++) {
+    const double k = c ? ky : kx, h = c ? h2 : h1;
+    const double ar = ((ez * 0.0) * k) / den, ai = ((ez * 1.0) * k) / den;
+    const double th = ((-1.0 * k) * 0.5) * h;
+    const double cr = cos(th), ci = sin(th);
+    (c ? sH2 : sH1)[e] = zmk(ar * cr - ai * ci, ar * ci + ai * cr);
+  }
+}
+
+static int model_create(const csem_model_desc *d, const csem_model_raw *raw, csem_model **out);
+
+extern "C" int csem_model_create(const csem_model_desc *d, csem_model **out) { return model_create(d, NULL, out); }
+
+extern "C" int csem_model_create_raw(const csem_model_desc *d, const csem_model_raw *raw, csem_model **out)
+{
+  *out = NULL;
+  if (!raw || raw->nx + 2 * d->nb != d->n1 || raw->ny + 2 * d->nb != d->n2 || raw->nz + 2 * d->nb != d->n3f)
+    return csem_fail(-3, "raw model %dx%dx%d does not match the extended grid %dx%dx%d (nb %d)", raw ? raw->nx : 0,
+                     raw ? raw->ny : 0, raw ? raw->nz : 0, d->n1, d->n2, d->n3f, d->nb);
+  // positive finite resistivities give finite positive conductivities
+  const float *rh[3] = {raw->rho11, raw->rho22, raw->rho33};
+  const long nc = (long)raw->nx * raw->ny * raw->nz;
+  for (int c = 0; c < 3; c++)
+    for (long e = 0; e < nc; e++)
+      if (!(rh[c][e] > 0.f) || !std::isfinite(rh[c][e]))
+        return csem_fail(-3, "resistivity must be finite and positive (component %d, cell %ld: %g)", c, e,
+                         (double)rh[c][e]);
+  if (d->nb > 0 && (!(raw->rho_air > 0.0) || !std::isfinite(raw->rho_air)))
+    return csem_fail(-3, "air resistivity must be finite and positive (%g)", raw->rho_air);
+  return model_create(d, raw, out);
+}
+
+// σ of a component copied to a host array, or the device-built ones from
+// the extended cells
+extern "C" int csem_model_sigma(const csem_model *m, int which, double *host)
+{
+  if (which < 0 || which > 2) return csem_fail(-3, "sigma component %d", which);
+  const double *s = which == 0 ? m->s11 : which == 1 ? m->s22 : m->s33;
+  const long n = which == 0 ? m->fEx : which == 1 ? m->fEy : m->fEz;
+  std::lock_guard<std::mutex> lk(g_devwide_mu);
+  HIP_OK(hipMemcpy(host, s, n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int csem_model_multiplier(const csem_model *m, int which, double *host)
+{
+  if (!m->airbc || which < 0 || which > 1) return csem_fail(-3, "no multiplier %d", which);
+  std::lock_guard<std::mutex> lk(g_devwide_mu);
+  HIP_OK(hipMemcpy(host, which ? m->sH2 : m->sH1, (long)m->nfft * m->nfft * sizeof(z2), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+static int model_create(const csem_model_desc *d, const csem_model_raw *raw, csem_model **out)
 {
   *out = NULL;
   if (d->n1 < 2 || d->n2 < 2 || d->n3f < 2) return csem_fail(-3, "grid too small: %d %d %d", d->n1, d->n2, d->n3f);
   if (d->airbc && d->nb < 1) return csem_fail(-3, "airbc=1 needs nb>0");
   if (d->n3f - 1 > SGS_MAXT || d->n2 - 1 > SGS_MAXT)
     return csem_fail(-3, "LU-SGS slices hold at most %d rows (n3f-1=%d, n2-1=%d)", SGS_MAXT, d->n3f - 1, d->n2 - 1);
-  {
+  if (!raw) {
     // LU-SGS drops the reference's sigma*0 term, the identity for finite
     // non-negative conductivity
     const double *sg[3] = {d->sigma11, d->sigma22, d->sigma33};
@@ -310,7 +458,43 @@ extern "C" int csem_model_create(const csem_model_desc *d, csem_model **out)
     UP(r1s, recip(d->d1s, n1 + 1), n1 + 1); UP(r2s, recip(d->d2s, n2 + 1), n2 + 1);
     UP(r3s, recip(d->d3s, n3f + 1), n3f + 1);
   }
-  UP(s11, d->sigma11, m->fEx); UP(s22, d->sigma22, m->fEy); UP(s33, d->sigma33, m->fEz);
+  if (!raw) {
+    UP(s11, d->sigma11, m->fEx); UP(s22, d->sigma22, m->fEy); UP(s33, d->sigma33, m->fEz);
+  } else {
+    // the cells and the edge averages on the device
+    if ((rc = dalloc(m, &m->s11, m->fEx)) || (rc = dalloc(m, &m->s22, m->fEy)) || (rc = dalloc(m, &m->s33, m->fEz)))
+      goto fail;
+    const long nc = (long)raw->nx * raw->ny * raw->nz, ne = (long)n1 * n2 * n3f;
+    float *rho = NULL;
+    double *cel = NULL;
+    hipError_t e = dev_malloc((void **)&rho, 3 * nc * (long)sizeof(float));
+    if (e == hipSuccess) e = dev_malloc((void **)&cel, 3 * ne * (long)sizeof(double));
+    if (e == hipSuccess) e = dev_upload(rho, raw->rho11, nc * sizeof(float));
+    if (e == hipSuccess) e = dev_upload(rho + nc, raw->rho22, nc * sizeof(float));
+    if (e == hipSuccess) e = dev_upload(rho + 2 * nc, raw->rho33, nc * sizeof(float));
+    if (e == hipSuccess) {
+      CellArgs ca;
+      ca.nx = raw->nx; ca.ny = raw->ny; ca.nz = raw->nz; ca.nb = d->nb; ca.n1 = n1; ca.n2 = n2; ca.n3 = n3f;
+      ca.rho[0] = rho; ca.rho[1] = rho + nc; ca.rho[2] = rho + 2 * nc;
+      ca.rho_air = raw->rho_air;
+      ca.c[0] = cel; ca.c[1] = cel + ne; ca.c[2] = cel + 2 * ne;
+      SigArgs sa;
+      sa.n1 = n1; sa.n2 = n2; sa.n3 = n3f;
+      sa.d1 = m->d1; sa.d2 = m->d2; sa.d3 = m->d3; sa.d1s = m->d1s; sa.d2s = m->d2s; sa.d3s = m->d3s;
+      sa.c11 = cel; sa.c22 = cel + ne; sa.r33 = cel + 2 * ne;
+      sa.s11 = m->s11; sa.s22 = m->s22; sa.s33 = m->s33;
+      // null-stream launches and a device-wide wait: under the lock that
+      // graph captures on other threads hold
+      std::lock_guard<std::mutex> lk(g_devwide_mu);
+      k_setup_cells<<<(unsigned)((ne + 255) / 256), 256>>>(ca);
+      k_setup_sigma<<<(unsigned)((m->fE + 255) / 256), 256>>>(sa);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    if (rho) dev_free(rho);
+    if (cel) dev_free(cel);
+    if (e != hipSuccess) { rc = csem_fail(-1, "device setup of the conductivities: %s", hipGetErrorString(e)); goto fail; }
+  }
   {
     SkewGeo &g = m->g;
     g.A[0] = n3f - 1; g.B[0] = n2 - 1; g.ns[0] = n1;
@@ -340,7 +524,22 @@ extern "C" int csem_model_create(const csem_model_desc *d, csem_model **out)
     if (N < 4 || (N & (N - 1)) || N > 2048) { rc = csem_fail(-3, "nfft=%d must be a power of two in [4, 2048]", N); goto fail; }
     m->nfft = N;
     long NN = (long)N * N;
-    UP(sH1, d->sH1, NN); UP(sH2, d->sH2, NN); UP(tw, d->twiddle, N);
+    if (!raw) {
+      UP(sH1, d->sH1, NN); UP(sH2, d->sH2, NN);
+    } else {
+      if ((rc = dalloc(m, &m->sH1, NN)) || (rc = dalloc(m, &m->sH2, NN))) goto fail;
+      const double dkx = 2.0 * CSEM_PI / (raw->d1fft * N), dky = 2.0 * CSEM_PI / (raw->d2fft * N);
+      hipError_t e;
+      {
+        std::lock_guard<std::mutex> lk(g_devwide_mu);
+        k_setup_mult<<<(unsigned)((NN + 255) / 256), 256>>>(N, dkx, dky, raw->d3fft, d->d1s[0], d->d2s[0], m->sH1,
+                                                            m->sH2);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+      }
+      if (e != hipSuccess) { rc = csem_fail(-1, "device setup of the multipliers: %s", hipGetErrorString(e)); goto fail; }
+    }
+    UP(tw, d->twiddle, N);
     UP(jx, d->jx, N); UP(jy, d->jy, N); UP(wx, d->wx, N); UP(wy, d->wy, N);
     UP(gxj1, d->gxj1, n1); UP(gxj2, d->gxj2, n2); UP(gxw1, d->gxw1, n1); UP(gxw2, d->gxw2, n2);
     UP(gyj1, d->gyj1, n1); UP(gyj2, d->gyj2, n2); UP(gyw1, d->gyw1, n1); UP(gyw2, d->gyw2, n2);

@@ -67,8 +67,9 @@ static void die_dev(const char *what, int rc)
 static int is_pow2(int n) { return n >= 4 && (n & (n - 1)) == 0; }
 
 /* airbc_init_free.c:24-109: transform size, uniform spacing, the
- * frequency-independent multipliers and the interpolation tables. */
-void airbc_init(emf_t *emf)
+ * frequency-independent multipliers (when `multipliers`; the device setup
+ * builds them itself) and the interpolation tables. */
+static void airbc_setup(emf_t *emf, int multipliers)
 {
   int N, i1, i2, j1, j2, n1 = emf->n1, n2 = emf->n2;
   double dkx, dky, kx, ky, kz, x1, x2, s1, s2;
@@ -85,11 +86,14 @@ void airbc_init(emf_t *emf)
     printf("[n1fft, n2fft]=[%d, %d]\n", emf->n1fft, emf->n2fft);
     printf("[d1fft, d2fft, d3fft]=[%g, %g, %g]\n", emf->d1fft, emf->d2fft, emf->d3fft);
   }
-  emf->sH1kxky = alloc2complex(N, N);
-  emf->sH2kxky = alloc2complex(N, N);
+  emf->sH1kxky = emf->sH2kxky = NULL;
   dkx = 2.0 * CSEM_PI / (emf->d1fft * N);
   dky = 2.0 * CSEM_PI / (emf->d2fft * N);
-  for (i2 = 0; i2 < N; i2++) {
+  if (multipliers) {
+    emf->sH1kxky = alloc2complex(N, N);
+    emf->sH2kxky = alloc2complex(N, N);
+  }
+  for (i2 = 0; multipliers && i2 < N; i2++) {
     ky = (i2 <= N / 2) ? i2 * dky : (i2 - N) * dky;
     for (i1 = 0; i1 < N; i1++) {
       kx = (i1 <= N / 2) ? i1 * dkx : (i1 - N) * dkx;
@@ -150,10 +154,13 @@ void airbc_init(emf_t *emf)
   }
 }
 
+void airbc_init(emf_t *emf) { airbc_setup(emf, 1); }
+
 void airbc_free(emf_t *emf)
 {
-  free2complex(emf->sH1kxky);
-  free2complex(emf->sH2kxky);
+  if (emf->sH1kxky) free2complex(emf->sH1kxky);
+  if (emf->sH2kxky) free2complex(emf->sH2kxky);
+  emf->sH1kxky = emf->sH2kxky = NULL;
   free(G.twiddle);
   free1int(G.jx); free1int(G.jy); free1double(G.wx); free1double(G.wy);
   free1int(G.gxj1); free1int(G.gxj2); free1int(G.gyj1); free1int(G.gyj2);
@@ -175,14 +182,15 @@ static void ensure_staging(long n)
 }
 
 /* Host half of fdfd_init (fdfd.c:30-63): operator truncation under the
- * airwave boundary, the airwave tables and the vector lengths. */
-void csem_fdfd_prepare(emf_t *emf)
+ * airwave boundary, the airwave tables (with the multipliers when asked)
+ * and the vector lengths. */
+static void fdfd_prepare(emf_t *emf, int multipliers)
 {
   G.emf = emf;
   if (emf->airbc) {
     emf->xx = emf->yy = NULL; /* LU-SGS scratch lives on the device */
     emf->n3 = emf->nz + emf->nb;
-    airbc_init(emf);
+    airbc_setup(emf, multipliers);
   }
   emf->lEx = emf->n1 * (emf->n2 + 1) * (emf->n3 + 1);
   emf->lEy = (emf->n1 + 1) * emf->n2 * (emf->n3 + 1);
@@ -195,27 +203,71 @@ void csem_fdfd_prepare(emf_t *emf)
   emf->lv = (emf->n1 + 1) * (emf->n2 + 1) * (emf->n3 + 1);
 }
 
-/* Device half: the model the device library solves on, built from the
- * extended model and the airwave tables csem_fdfd_prepare made. */
-int csem_model_from_emf(const emf_t *emf, csem_model **out)
+void csem_fdfd_prepare(emf_t *emf) { fdfd_prepare(emf, 1); }
+
+/* The device model's description from a prepared emf_t (without the
+ * conductivities and multipliers when `arrays` is 0). */
+static void model_desc(const emf_t *emf, csem_model_desc *dd, int arrays)
 {
   csem_model_desc d;
   memset(&d, 0, sizeof d);
   d.n1 = emf->n1; d.n2 = emf->n2; d.n3f = emf->nz + 2 * emf->nb; d.nb = emf->nb;
   d.airbc = emf->airbc; d.fvm = emf->fvm;
   d.d1 = emf->d1; d.d2 = emf->d2; d.d3 = emf->d3; d.d1s = emf->d1s; d.d2s = emf->d2s; d.d3s = emf->d3s;
-  d.sigma11 = &emf->sigma11[0][0][0];
-  d.sigma22 = &emf->sigma22[0][0][0];
-  d.sigma33 = &emf->sigma33[0][0][0];
+  if (arrays) {
+    d.sigma11 = &emf->sigma11[0][0][0];
+    d.sigma22 = &emf->sigma22[0][0][0];
+    d.sigma33 = &emf->sigma33[0][0][0];
+  }
   if (emf->airbc) {
     d.nfft = emf->n1fft;
-    d.sH1 = (const double *)&emf->sH1kxky[0][0];
-    d.sH2 = (const double *)&emf->sH2kxky[0][0];
+    if (arrays) {
+      d.sH1 = (const double *)&emf->sH1kxky[0][0];
+      d.sH2 = (const double *)&emf->sH2kxky[0][0];
+    }
     d.twiddle = G.twiddle;
     d.jx = G.jx; d.jy = G.jy; d.wx = G.wx; d.wy = G.wy;
     d.gxj1 = G.gxj1; d.gxj2 = G.gxj2; d.gxw1 = G.gxw1; d.gxw2 = G.gxw2;
     d.gyj1 = G.gyj1; d.gyj2 = G.gyj2; d.gyw1 = G.gyw1; d.gyw2 = G.gyw2;
   }
+  *dd = d;
+}
+
+int csem_model_build_device(acq_t *acq, emf_t *emf, csem_model **out)
+{
+  csem_model_desc d;
+  csem_model_raw r;
+  int rc, c;
+  *out = NULL;
+  csem_extend_grid(acq, emf);
+  const int n1 = emf->n1, n2 = emf->n2, n3f = emf->nz + 2 * emf->nb;
+  fdfd_prepare(emf, 0);
+  model_desc(emf, &d, 0);
+  memset(&r, 0, sizeof r);
+  r.nx = emf->nx; r.ny = emf->ny; r.nz = emf->nz;
+  r.rho11 = &emf->rho11[0][0][0]; r.rho22 = &emf->rho22[0][0][0]; r.rho33 = &emf->rho33[0][0][0];
+  r.rho_air = emf->rho_air;
+  r.d1fft = emf->d1fft; r.d2fft = emf->d2fft; r.d3fft = emf->d3fft;
+  if ((rc = csem_model_create_raw(&d, &r, out))) return rc;
+  emf->sigma11 = alloc3double(n1, n2 + 1, n3f + 1);
+  emf->sigma22 = alloc3double(n1 + 1, n2, n3f + 1);
+  emf->sigma33 = alloc3double(n1 + 1, n2 + 1, n3f);
+  double *dst[3] = {&emf->sigma11[0][0][0], &emf->sigma22[0][0][0], &emf->sigma33[0][0][0]};
+  for (c = 0; c < 3; c++)
+    if ((rc = csem_model_sigma(*out, c, dst[c]))) {
+      csem_model_destroy(*out);
+      *out = NULL;
+      return rc;
+    }
+  return 0;
+}
+
+/* Device half: the model the device library solves on, built from the
+ * extended model and the airwave tables csem_fdfd_prepare made. */
+int csem_model_from_emf(const emf_t *emf, csem_model **out)
+{
+  csem_model_desc d;
+  model_desc(emf, &d, 1);
   return csem_model_create(&d, out);
 }
 

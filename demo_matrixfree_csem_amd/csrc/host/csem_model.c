@@ -347,7 +347,9 @@ static void extend_axis(emf_t *emf, const float *node, int nn, double *x)
   free1double(scratch);
 }
 
-void extend_model_init(acq_t *acq, emf_t *emf)
+/* The 1-D half of extend_model_init: extended and staggered nodes and
+ * spacings (extend_model.c:24-106). */
+void csem_extend_grid(acq_t *acq, emf_t *emf)
 {
   const int nb = emf->nb;
   int i, j, k, n1, n2, n3;
@@ -382,6 +384,16 @@ void extend_model_init(acq_t *acq, emf_t *emf)
   stagger_axis(n1, emf->x1, emf->x1s, emf->d1, emf->d1s);
   stagger_axis(n2, emf->x2, emf->x2s, emf->d2, emf->d2s);
   stagger_axis(n3, emf->x3, emf->x3s, emf->d3, emf->d3s);
+}
+
+void extend_model_init(acq_t *acq, emf_t *emf)
+{
+  const int nb = emf->nb;
+  int i, j, k, n1, n2, n3;
+  csem_extend_grid(acq, emf);
+  n1 = emf->n1;
+  n2 = emf->n2;
+  n3 = emf->n3;
 
   /* cell conductivity (11, 22) and resistivity (33) on the extended grid */
   double ***c11 = alloc3double(n1, n2, n3), ***c22 = alloc3double(n1, n2, n3), ***r33 = alloc3double(n1, n2, n3);

@@ -9,8 +9,10 @@ stream, each taking up to `rhs_per_solve` waiting right-hand sides of one
 model, from one request or several: continuous batching -- a solve starts
 as soon as a worker and enough device memory are free while the others keep
 running, and solves of different requests and models overlap on the device.
-Models are built on a separate thread, so a new model's host setup does not
-hold up solves on resident ones.
+Models are built on a separate thread, so a new model's setup does not
+hold up solves on resident ones; the host reads the input files and makes
+the 1-D tables, and the device builds the cells, the edge-averaged
+conductivities and the airwave multipliers (csem_model_build_device).
 
 Admission is by HBM footprint.  The engine charges every resident model
 what it holds on the device (csem_model_bytes + csem_mixed_bytes, the
@@ -89,16 +91,15 @@ def fp64_continue(model, freq, dx, db, tol, rel, niter, lock):
 
 
 class ResidentModel:
-    """One survey model on the device (host setup under _HOST)."""
+    """One survey model on the device: the input files read and the 1-D
+    tables made on the host (under _HOST), the cells, conductivities and
+    airwave multipliers built on the device (csem_model_build_device)."""
 
     def __init__(self, survey, workdir):
-        L = _lib.lib()
         surveys.write_files(workdir, survey)
         with _HOST:
-            self.ses = Session(survey.argv(workdir), device=False)
-            m = C.c_void_p()
-            _lib.check(L.csem_model_from_emf(C.byref(self.ses.emf), C.byref(m)), "csem_model_from_emf")
-        self.model = m
+            self.ses = Session(survey.argv(workdir), device=False, setup="device")
+        self.model = self.ses.built
         self.n = self.ses.n
         self.busy = 0          # requests in flight on this model
         self.last_used = time.time()
@@ -288,6 +289,7 @@ class Engine:
         building = {}          # model key -> {"fut", "survey", "est", "units", "stage"}
         order = sorted(range(len(requests)), key=lambda i: arrivals[i])
         nxt, nsolves, nbatches, nshared = 0, 0, 0, 0
+        setup_s = []           # seconds of each model's setup (input files, host tables, device build)
         peak_acc, peak_dev = 0, 0
         self.reserved = 0
         pool = cf.ThreadPoolExecutor(self.workers)
@@ -295,7 +297,10 @@ class Engine:
 
         def setup(survey):
             self.nbuilt += 1
-            return ResidentModel(survey, os.path.join(self.work, "m%d" % self.nbuilt))
+            t = time.time()
+            rm = ResidentModel(survey, os.path.join(self.work, "m%d" % self.nbuilt))
+            setup_s.append(time.time() - t)
+            return rm
 
         def enqueue(i, rm):
             key, build, freqs, srcs, _ = requests[i]
@@ -406,7 +411,8 @@ class Engine:
                  "budget_bytes": self.budget, "peak_admitted_bytes": int(peak_acc),
                  "peak_device_bytes": int(peak_dev), "resident_models": len(self.models),
                  "models_built": self.nbuilt, "evictions": self.evictions, "workers": self.workers, "rhs_per_solve": self.R,
-                 "batches": nbatches, "cross_request_batches": nshared, "fp64_continued": self.fp64_continued}
+                 "batches": nbatches, "cross_request_batches": nshared, "fp64_continued": self.fp64_continued,
+                 "model_setup_s": [round(v, 3) for v in setup_s]}
         return results, stats
 
     def close(self):

@@ -27,7 +27,14 @@ def _p(a):
 class Session:
     """One extended model, as main.c builds it (one per process)."""
 
-    def __init__(self, argv, verb=0, device=True):
+    def __init__(self, argv, verb=0, device=True, setup="host"):
+        """The reference's setup sequence (main.c:46-92) on the flags `argv`.
+        device: fdfd_init builds the active device model (else only the
+        host half, csem_fdfd_prepare).  setup="device": instead of
+        extend_model_init and fdfd_init, csem_model_build_device builds a
+        model of this session's own on the device (`self.built`, the
+        caller's to destroy), and the host keeps the 1-D tables and a copy
+        of the conductivities."""
         L = lib()
         L.csem_set_exit_on_error(0)
         args = [b"csem"] + [a.encode() if isinstance(a, str) else a for a in argv]
@@ -45,11 +52,18 @@ class Session:
         e.tol = float(np.float32(float(flags.get("tol", 1e-8))))
         e.nrestart = int(flags.get("nrestart", 10))
         e.mp = int(flags.get("mp", 30))
-        for step, fn in (("emf_init", lambda: L.emf_init(C.byref(e))),
-                         ("acq_init", lambda: L.acq_init(C.byref(self.acq), C.byref(e))),
-                         ("extend_model_init", lambda: L.extend_model_init(C.byref(self.acq), C.byref(e))),
-                         ("fdfd_init", (lambda: L.fdfd_init(C.byref(e))) if device
-                          else (lambda: L.csem_fdfd_prepare(C.byref(e))))):
+        self.built = C.c_void_p()
+        steps = [("emf_init", lambda: L.emf_init(C.byref(e))),
+                 ("acq_init", lambda: L.acq_init(C.byref(self.acq), C.byref(e)))]
+        if setup == "device":
+            steps.append(("csem_model_build_device",
+                          lambda: L.csem_model_build_device(C.byref(self.acq), C.byref(e), C.byref(self.built))))
+            device = False
+        else:
+            steps += [("extend_model_init", lambda: L.extend_model_init(C.byref(self.acq), C.byref(e))),
+                      ("fdfd_init", (lambda: L.fdfd_init(C.byref(e))) if device
+                       else (lambda: L.csem_fdfd_prepare(C.byref(e))))]
+        for step, fn in steps:
             fn()
             st = L.csem_last_status()
             if st < 0:

@@ -201,6 +201,19 @@ void csem_set_exit_on_error(int on);
 void csem_fdfd_prepare(emf_t *emf);
 int csem_model_from_emf(const emf_t *emf, csem_model **out);
 
+/* extend_model_init in two halves: the 1-D half (extended and staggered
+ * nodes and spacings, extend_model.c:24-106) alone. */
+void csem_extend_grid(acq_t *acq, emf_t *emf);
+/* Device-side setup in place of extend_model_init + fdfd_init (no
+ * counterpart in the reference; for servers that build many models): after
+ * emf_init and acq_init, the 1-D halves on the host, then the cells, the
+ * edge-averaged conductivities and the airwave multipliers on the device
+ * (csem_model_create_raw).  On return emf is as after extend_model_init and
+ * csem_fdfd_prepare, with emf->sigma11..33 copied back from the device (for
+ * inject_source and extract_emf) and no host multipliers; the model is the
+ * caller's (csem_model_destroy), not the active one. */
+int csem_model_build_device(acq_t *acq, emf_t *emf, csem_model **out);
+
 /* The device model behind fdfd_init (NULL before it). */
 csem_model *csem_active_model(void);
 /* Iteration index reached by the last Krylov call ("converged at iter="). */

@@ -54,6 +54,28 @@ int csem_freq_consts(float freq, csem_freq *out);
 
 int csem_model_create(const csem_model_desc *d, csem_model **out);
 void csem_model_destroy(csem_model *m);
+
+/* Device-side model setup (no counterpart in the reference; for servers
+ * that build many models).  The raw model: core cell resistivities as
+ * emf_init reads them (emf_init_free.c:120-150, float32, x fastest), the
+ * air resistivity, and the airwave transform spacings airbc_init sets
+ * (airbc_init_free.c:44-46).  csem_model_create_raw takes everything else
+ * from d (sigma11..33 and sH1, sH2 are ignored) and builds on the device:
+ * the extended cells (extend_model.c:108-175, quirk Q2 included) and the
+ * edge-averaged conductivities (extend_model.c:180-252), bit for bit what
+ * the host computes; and the airwave multipliers (airbc_init_free.c:69-84)
+ * with the device's exp, sin and cos, within a few ulp of the host's.
+ * csem_model_sigma / csem_model_multiplier copy a component (0, 1, 2 /
+ * 0 = sH1, 1 = sH2) of any model to a host array of its full length. */
+typedef struct {
+  int nx, ny, nz;
+  const float *rho11, *rho22, *rho33;
+  double rho_air;
+  double d1fft, d2fft, d3fft;
+} csem_model_raw;
+int csem_model_create_raw(const csem_model_desc *d, const csem_model_raw *raw, csem_model **out);
+int csem_model_sigma(const csem_model *m, int which, double *host);
+int csem_model_multiplier(const csem_model *m, int which, double *host);
 /* operator vector length lE, LU-SGS (full grid) length, device bytes held */
 long csem_model_len(const csem_model *m);
 long csem_model_full_len(const csem_model *m);
