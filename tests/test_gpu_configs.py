@@ -225,6 +225,16 @@ def test_vti_full_size(workdir):
             assert hashlib.sha256(x.tobytes()).hexdigest() == str(_fix(key + "_sha256")), f
             bs.append(b)
             tols.append(_tol(s, f))
+            if i == 0:
+                # the reference's serial summation order (the oracle's solve
+                # in that order) against the device's: within the tolerance
+                ser = "vti_marine_f0_serial"
+                its = int(_fix(ser + "_iter"))
+                print("  iterations: device %d, reference order %d" % (it, its))
+                assert abs(it - its) <= 0.1 * its, (it, its)
+                gap = _rec_gap(_fix(key + "_rec"), _fix(ser + "_rec"), chans)
+                print("  receivers, device vs reference order:", ["%s %.2e" % (c, g) for c, g in zip(chans, gap)])
+                assert max(gap) <= 5 * tols[0], gap
         f = s.freqs[0]
         print("case vti_marine f=%g FP64 FGMRES(10)" % f, flush=True)
         ses.set_freq(f)
