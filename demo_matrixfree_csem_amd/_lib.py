@@ -72,17 +72,18 @@ class FreqT(C.Structure):
 
 
 PRECOND_LUSGS, PRECOND_RBSGS = 0, 1  # csem_mixed_opts.precond
+KRYLOV_FBICGSTAB, KRYLOV_FGMRES = 0, 1  # csem_mixed_opts.krylov
 
 
 class MixedOpts(C.Structure):
     """csem_mixed_opts of include/csem_dev.h."""
     _fields_ = [("max_inner", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
                 ("check_every", C.c_int), ("use_graph", C.c_int), ("precond", C.c_int),
-                ("stall_iters", C.c_int)]
+                ("stall_iters", C.c_int), ("krylov", C.c_int), ("restart", C.c_int)]
 
     @classmethod
     def default(cls, **kw):
-        o = cls(2000, 1e-2, 40, 8, 1, PRECOND_LUSGS, 0)
+        o = cls(2000, 1e-2, 40, 8, 1, PRECOND_LUSGS, 0, KRYLOV_FBICGSTAB, 0)
         for k, v in kw.items():
             setattr(o, k, v)
         return o

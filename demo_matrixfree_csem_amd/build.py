@@ -7,6 +7,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -39,7 +40,7 @@ def _run(cmd):
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    objs = []
+    cmds = []
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
         o = os.path.join(OBJ, os.path.basename(src) + ".o")
@@ -50,12 +51,15 @@ def build(verbose=False):
         # FP64 translation unit keeps the reference's unfused expressions
         flags = HIP_FLAGS if "mixed" not in src else [f if f != "-ffp-contract=off" else "-ffp-contract=on"
                                                       for f in HIP_FLAGS]
-        _run([HIPCC] + flags + ["-c", path, "-o", o])
-        objs.append(o)
+        cmds.append(([HIPCC] + flags + ["-c", path, "-o", o], o))
     for src in C_SOURCES:
         o = os.path.join(OBJ, os.path.basename(src) + ".o")
-        _run(["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
-        objs.append(o)
+        cmds.append((["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o], o))
+    # the translation units compile independently
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        for f in [ex.submit(_run, c) for c, _ in cmds]:
+            f.result()
+    objs = [o for _, o in cmds]
     tmp = LIB + ".tmp"
     # Linked by g++ with the static libgcc, not by hipcc: C99 complex
     # division in the host code (the GMRES back-substitution) then comes

@@ -36,7 +36,7 @@ static const char *KT_NAMES[KT_N] = {
   "k_sgs_ex<fwd>", "k_sgs_ey<fwd>", "k_sgs_ez<fwd>", "k_sgs_ez<bwd>", "k_sgs_ey<bwd>", "k_sgs_ex<bwd>",
   "k_dot_partial", "k_dot_final", "k_bicg_s", "k_bicg_xr", "k_bicg_p", "k_lu_pre", "k_lu_smith",
   "k_mx_apply", "k_mx_hz0", "k_mx_air_rows", "k_mx_air_cols", "k_mx_air_rows_inv", "k_mx_air_ghost",
-  "k_mx_lu_pre", "k_mx_sgs", "k_mx_krylov", "k_mx_dot", "k_mx_refine", "k_mx_unskew"};
+  "k_mx_lu_pre", "k_mx_sgs", "k_mx_krylov", "k_mx_dot", "k_mx_refine", "k_mx_unskew", "k_gm_orth"};
 
 struct KtPend {
   int id;

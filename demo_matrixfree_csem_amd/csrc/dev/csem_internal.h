@@ -45,7 +45,7 @@ enum {
   KT_SGS_EX_F, KT_SGS_EY_F, KT_SGS_EZ_F, KT_SGS_EZ_B, KT_SGS_EY_B, KT_SGS_EX_B,
   KT_DOT, KT_DOT_FIN, KT_BICG_S, KT_BICG_XR, KT_BICG_P, KT_LU_PRE, KT_LU_SMITH,
   KT_M_APPLY, KT_M_HZ0, KT_M_AIR_ROWS, KT_M_AIR_COLS, KT_M_AIR_RINV, KT_M_AIR_GHOST,
-  KT_M_PRE, KT_M_SGS, KT_M_VEC, KT_M_DOT, KT_M_REFINE, KT_M_UNSKEW, KT_N
+  KT_M_PRE, KT_M_SGS, KT_M_VEC, KT_M_DOT, KT_M_REFINE, KT_M_UNSKEW, KT_M_ORTH, KT_N
 };
 // An event pair for the next launch when timing is on and not paused
 // (graph capture pauses it), and the pair's record once launched.

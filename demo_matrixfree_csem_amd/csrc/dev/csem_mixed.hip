@@ -19,6 +19,7 @@
 #define CSEM_TYPES_ONLY
 #include "csem_internal.h"
 #include "csem_mixed_kernels.h"
+#include "csem_mixed_gmres.h"
 
 void kt_pause(int paused);
 
@@ -48,6 +49,10 @@ struct MixSlot {
   float4 *th;                               // LU-SGS records (T, h) per record slot and r
   int pc;                                   // preconditioner of the solve in flight (csem_mixed_opts)
   c2 *kv;                                   // Krylov vectors
+  c2 *xbest;                                // F-BiCGStab: the correction at the best recursive residual
+  c2 *gmV, *gmZ;                            // FGMRES basis and preconditioned basis (on first use)
+  double *gmpart, *gmhs;
+  int gm_m;                                 // restart length the FGMRES buffers hold
   double *part, *sc;
   int nch;
   z2 *res, *tmp64;                          // refinement
@@ -62,6 +67,7 @@ struct MixSlot {
   hipStream_t g_stream;
   MxFreq g_fr;  // the frequencies baked into the captured launches
   int g_pc;     // and the preconditioner
+  int g_kind;   // 0: one F-BiCGStab iteration; m > 0: one FGMRES(m) cycle
   std::vector<void *> allocs;
   long bytes;
 };
@@ -251,6 +257,7 @@ static int slot_new(csem_model *m, int R, MixSlot **out)
     MA(q, W, m->fE * R); MA(q, th, mx_pad(m).n * R); MA(q, V, mx_pad(m).n * R);
     q->nch = (int)((m->lE + RED_CHUNK - 1) / RED_CHUNK);
     MA(q, kv, 9L * R * m->lE); MA(q, part, (long)R * q->nch * 4); MA(q, sc, (long)R * MS_N);
+    MA(q, xbest, R * m->lE);
     MA(q, res, R * m->lE); MA(q, tmp64, m->lE); MA(q, npart, (long)R * q->nch); MA(q, norm, R);
     MA(q, xprev, R * m->lE);
     MA(q, scale, R);
@@ -623,11 +630,98 @@ static int mx_iteration(csem_model *m, MixState *x, MixSlot *q, int R, const MxF
   return 0;
 }
 
-// Capture one iteration as a graph (kernel launches only: the frequencies
-// and every scalar live in device memory or in the captured arguments).
-static int mx_graph(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, hipStream_t s)
+// ---------------------------------------------------------------- FGMRES
+static void slot_drop(MixSlot *q, void *p, long bytes)
 {
-  if (q->gexec && q->g_R == R && q->g_stream == s && q->g_pc == q->pc && !memcmp(&q->g_fr, &fr, sizeof fr)) return 0;
+  if (!p) return;
+  auto it = std::find(q->allocs.begin(), q->allocs.end(), p);
+  if (it != q->allocs.end()) q->allocs.erase(it);
+  dev_free(p);
+  q->bytes -= std::max(bytes, 256L);
+}
+
+// FGMRES(M) buffers of a slot: M + 1 basis vectors and M preconditioned
+// ones per right-hand side, the chunk partials and the Hessenberg state
+static int gm_alloc(const csem_model *m, MixSlot *q, int M)
+{
+  if (q->gm_m >= M) return 0;
+  const long n = m->lE, R = q->R, mo = q->gm_m;
+  if (mo) {
+    slot_drop(q, q->gmV, (mo + 1) * R * n * (long)sizeof(c2));
+    slot_drop(q, q->gmZ, mo * R * n * (long)sizeof(c2));
+    slot_drop(q, q->gmpart, R * q->nch * GM_PS * (long)sizeof(double));
+    slot_drop(q, q->gmhs, R * GM_HS * (long)sizeof(double));
+  }
+  q->gm_m = 0;
+  q->gmV = q->gmZ = NULL;
+  q->gmpart = q->gmhs = NULL;
+  int rc;
+  if ((rc = mx_dalloc(q, &q->gmV, (M + 1) * R * n)) || (rc = mx_dalloc(q, &q->gmZ, (long)M * R * n)) ||
+      (rc = mx_dalloc(q, &q->gmpart, R * q->nch * GM_PS)) || (rc = mx_dalloc(q, &q->gmhs, R * GM_HS)))
+    return rc;
+  q->gm_m = M;
+  return 0;
+}
+
+static MxGm mx_gmv(const csem_model *m, const MixSlot *q, int R, int M)
+{
+  const MxKry k = mx_kry(m, q);
+  MxGm g;
+  g.n = m->lE; g.nch = q->nch; g.m = M; g.R = R;
+  g.V = q->gmV; g.Z = q->gmZ; g.r = k.r; g.x = k.x;
+  g.part = q->gmpart; g.sc = q->sc; g.hs = q->gmhs;
+  return g;
+}
+
+// the three orthogonalisation passes of step j and their scalar steps
+// (bytes: the vectors each pass reads and writes)
+template <int MB>
+static void gm_orth(const MxGm &g, int R, int j, dim3 gr, dim3 ge, double rr, hipStream_t s)
+{
+  const double n = (double)g.n, nb = j + 2.0;
+  kt_launch(KT_M_ORTH, s, 8.0 * nb * rr * n, k_gm_dot<MB>, gr, RED_T, 0, g, j);
+  kt_launch(KT_M_ORTH, s, 0.0, k_gm_fin, R, RED_T, 0, g, j, 1);
+  kt_launch(KT_M_ORTH, s, 8.0 * (nb + 1) * rr * n, k_gm_upd1<MB>, gr, RED_T, 0, g, j);
+  kt_launch(KT_M_ORTH, s, 0.0, k_gm_fin, R, RED_T, 0, g, j, 2);
+  kt_launch(KT_M_ORTH, s, 8.0 * (nb + 1) * rr * n, k_gm_upd2<MB>, ge, 256, 0, g, j);
+}
+
+// One FGMRES(M) cycle from x = 0 on the inner right-hand side k.r for
+// every right-hand side taking part (solver.c:426-505 with CGS2); the
+// correction goes to k.x and MS_IT holds the steps taken.
+static int mx_gm_cycle(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, int M, hipStream_t s)
+{
+  const MxGm g = mx_gmv(m, q, R, M);
+  const long n = m->lE;
+  const dim3 gr(q->nch, R), ge((unsigned)((n + 255) / 256), R);
+  const double *live = R > 1 ? q->sc : nullptr;
+  const double rr = kt_rhs(R);
+  int rc;
+  kt_launch(KT_M_ORTH, s, 8.0 * rr * n, k_gm_norm, gr, RED_T, 0, g);
+  kt_launch(KT_M_ORTH, s, 0.0, k_gm_fin, R, RED_T, 0, g, 0, 0);
+  kt_launch(KT_M_VEC, s, 16.0 * rr * n, k_gm_v0, ge, 256, 0, g);
+  for (int j = 0; j < M; j++) {
+    c2 *vj = q->gmV + (long)j * R * n, *zj = q->gmZ + (long)j * R * n, *w = q->gmV + (long)(j + 1) * R * n;
+    if ((rc = mx_lusgs(m, x, q, R, fr, vj, zj, s, live))) return rc;
+    if ((rc = mx_apply(m, x, q, R, fr, zj, w, s, live))) return rc;
+    if (j < 4) gm_orth<4>(g, R, j, gr, ge, rr, s);
+    else if (j < 8) gm_orth<8>(g, R, j, gr, ge, rr, s);
+    else gm_orth<16>(g, R, j, gr, ge, rr, s);
+  }
+  kt_launch(KT_M_ORTH, s, 0.0, k_gm_solve, R, 64, 0, g);
+  kt_launch(KT_M_VEC, s, 8.0 * (M + 1) * rr * n, k_gm_xupd, ge, 256, 0, g);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+// Capture one F-BiCGStab iteration (kind 0) or one FGMRES(kind) cycle as a
+// graph (kernel launches only: the frequencies and every scalar live in
+// device memory or in the captured arguments).
+static int mx_graph(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, hipStream_t s, int kind)
+{
+  if (q->gexec && q->g_R == R && q->g_stream == s && q->g_pc == q->pc && q->g_kind == kind &&
+      !memcmp(&q->g_fr, &fr, sizeof fr))
+    return 0;
   if (q->gexec) { (void)hipGraphExecDestroy(q->gexec); q->gexec = NULL; }
   hipGraph_t g;
   int rc;
@@ -640,7 +734,7 @@ static int mx_graph(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
       kt_pause(0);
       return csem_fail(-1, "graph capture: %s", hipGetErrorString(e));
     }
-    rc = mx_iteration(m, x, q, R, fr, s);
+    rc = kind ? mx_gm_cycle(m, x, q, R, fr, kind, s) : mx_iteration(m, x, q, R, fr, s);
     e = hipStreamEndCapture(s, &g);
     kt_pause(0);
   }
@@ -653,6 +747,7 @@ static int mx_graph(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
   q->g_stream = s;
   q->g_fr = fr;
   q->g_pc = q->pc;
+  q->g_kind = kind;
   return 0;
 }
 
@@ -710,10 +805,16 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
   const unsigned gn = (unsigned)((n + 255) / 256);
   std::vector<double> bn(R), rn(R), hsc((size_t)R * MS_N), nb(R);
   std::vector<char> stalled(R, 0);  // a correction that did not lower the true residual was rejected
+  std::vector<char> cut(R, 0);      // this step's F-BiCGStab inner solve was cut for stagnation
+  std::vector<char> kept(R, 0);     // and its best iterate is in xbest
   std::vector<float> scl(R);
   int rc;
   if (o.precond != CSEM_PRECOND_LUSGS && o.precond != CSEM_PRECOND_RBSGS)
     return csem_fail(-3, "unknown preconditioner %d", o.precond);
+  if (o.krylov != CSEM_KRYLOV_FBICGSTAB && o.krylov != CSEM_KRYLOV_FGMRES)
+    return csem_fail(-3, "unknown inner Krylov method %d", o.krylov);
+  const int M = o.restart == 0 ? 10 : o.restart;
+  if (M < 1 || M > GM_MMAX) return csem_fail(-3, "FGMRES restart %d outside [1, %d]", M, GM_MMAX);
   q->pc = o.precond;
   // |b_r|, x = 0, res = b
   k_mx_norm_part<<<dim3(nch, R), RED_T, 0, s>>>(n, B, q->npart, nch);
@@ -724,97 +825,155 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
   HIP_OK(hipStreamSynchronize(s));
   for (int r = 0; r < R; r++) { bn[r] = sqrt(nb[r]); rn[r] = bn[r]; relres[r] = 1.0; iters[r] = 0; }
   if ((rc = mx_tune(m, x, q, R, fr, k.r, k.v, s))) return rc;  // operands: any R x n FP32 buffers
-  for (int outer = 0; outer <= o.max_outer; outer++) {
-    // which right-hand sides still need a correction
-    bool any = false;
-    for (int r = 0; r < R; r++) {
-      double *sc = &hsc[(size_t)r * MS_N];
-      memset(sc, 0, MS_N * sizeof(double));
-      const bool act = bn[r] > 0 && relres[r] >= tol[r] && iters[r] < o.max_inner && outer < o.max_outer && !stalled[r];
-      sc[MS_DONE] = act ? 0.0 : 1.0;
-      sc[MS_USED] = act ? 1.0 : 0.0;
-      // reduce this residual by what is still needed (with a margin), at most to inner_tol
-      sc[MS_TOL] = std::max(o.inner_tol, 0.3 * tol[r] / std::max(relres[r], 1e-300));
-      sc[MS_MAXIT] = o.max_inner - iters[r];
-      scl[r] = act ? (float)(1.0 / rn[r]) : 0.f;
-      any |= act;
-    }
-    if (!any) break;
-    int nact = 0;
-    for (int r = 0; r < R; r++) nact += hsc[(size_t)r * MS_N + MS_DONE] == 0.0;
-    g_kt_live = nact;
-    HIP_OK(hipMemcpyAsync(q->sc, hsc.data(), hsc.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(q->scale, scl.data(), R * sizeof(float), hipMemcpyHostToDevice, s));
-    kt_launch(KT_M_REFINE, s, 24.0 * R * n, k_mx_to32, dim3(gn, R), 256, 0, n, q->res, q->scale, k.r);
-    kt_launch(KT_M_DOT, s, 32.0 * kt_rhs(R) * n, k_mx_k_init, dim3(nch, R), RED_T, 0, k);
-    kt_launch(KT_M_DOT, s, 0.0, k_mx_k_fin, R, RED_T, 0, k, 0);
-    if (o.use_graph && (rc = mx_graph(m, x, q, R, fr, s))) return rc;
-    // inner iterations, convergence checked every check_every; an inner
-    // solve whose recursive residual has not improved by 1% in
-    // MX_STALL_ITERS iterations (FP32 stagnation on an ill-conditioned
-    // system) is ended, and the refinement step decides what it was worth
-    std::vector<double> best(R, 1.0);
-    std::vector<double> best_it(R, 0.0);
-    const int stall = o.stall_iters == 0 ? MX_STALL_ITERS : o.stall_iters;
-    for (;;) {
-      for (int w = 0; w < std::max(1, o.check_every); w++) {
-        if (o.use_graph) HIP_OK(hipGraphLaunch(q->gexec, s));
-        else if ((rc = mx_iteration(m, x, q, R, fr, s))) return rc;
+  // Phase 0 refines with the inner method asked for.  A right-hand side
+  // whose F-BiCGStab inner solve was cut for stagnation and whose
+  // correction the guard still rejected goes on in phase 1 with FGMRES(M)
+  // cycles from its best solution: GMRES minimises the residual over its
+  // Krylov space, so a cycle cannot diverge where BiCGStab in FP32 did.
+  std::vector<char> on(R, 1), cutrej(R, 0);
+  for (int phase = 0; phase < 2; phase++) {
+    const bool gm = phase == 1 || o.krylov == CSEM_KRYLOV_FGMRES;
+    if (phase == 1) {
+      bool any1 = false;
+      for (int r = 0; r < R; r++) {
+        on[r] = cutrej[r] && iters[r] < o.max_inner && relres[r] >= tol[r];
+        if (on[r]) stalled[r] = 0;
+        any1 |= on[r] != 0;
       }
-      HIP_OK(hipMemcpyAsync(hsc.data(), q->sc, hsc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-      HIP_OK(hipStreamSynchronize(s));
-      bool all = true;
+      if (!any1) break;
+    }
+    if (gm && (rc = gm_alloc(m, q, M))) return rc;
+    // FGMRES: each refinement step is one restart cycle of M steps, bounded
+    // by the inner budget alone
+    const int max_outer = gm ? o.max_inner + 1 : o.max_outer;
+    for (int outer = 0; outer <= max_outer; outer++) {
+      // which right-hand sides still need a correction
+      bool any = false;
       for (int r = 0; r < R; r++) {
         double *sc = &hsc[(size_t)r * MS_N];
-        if (sc[MS_DONE] == 0.0) {
-          if (sc[MS_REL] < 0.99 * best[r]) {
-            best[r] = sc[MS_REL];
-            best_it[r] = sc[MS_IT];
-          } else if (stall > 0 && sc[MS_IT] - best_it[r] >= stall) {
-            const double one = 1.0;
-            sc[MS_DONE] = 1.0;
-            HIP_OK(hipMemcpyAsync(q->sc + (size_t)r * MS_N + MS_DONE, &one, sizeof one, hipMemcpyHostToDevice, s));
-            HIP_OK(hipStreamSynchronize(s));
-          }
+        memset(sc, 0, MS_N * sizeof(double));
+        const bool act = on[r] && bn[r] > 0 && relres[r] >= tol[r] && iters[r] < o.max_inner && outer < max_outer &&
+                         !stalled[r];
+        sc[MS_DONE] = act ? 0.0 : 1.0;
+        sc[MS_USED] = act ? 1.0 : 0.0;
+        // reduce this residual by what is still needed (with a margin), at most to inner_tol
+        sc[MS_TOL] = std::max(o.inner_tol, 0.3 * tol[r] / std::max(relres[r], 1e-300));
+        sc[MS_MAXIT] = o.max_inner - iters[r];
+        scl[r] = act ? (float)(1.0 / rn[r]) : 0.f;
+        any |= act;
+      }
+      if (!any) break;
+      int nact = 0;
+      for (int r = 0; r < R; r++) nact += hsc[(size_t)r * MS_N + MS_DONE] == 0.0;
+      g_kt_live = nact;
+      HIP_OK(hipMemcpyAsync(q->sc, hsc.data(), hsc.size() * sizeof(double), hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(q->scale, scl.data(), R * sizeof(float), hipMemcpyHostToDevice, s));
+      kt_launch(KT_M_REFINE, s, 24.0 * R * n, k_mx_to32, dim3(gn, R), 256, 0, n, q->res, q->scale, k.r);
+      if (gm) {
+        // one FGMRES(M) cycle on the scaled residual
+        if (o.use_graph) {
+          if ((rc = mx_graph(m, x, q, R, fr, s, M))) return rc;
+          HIP_OK(hipGraphLaunch(q->gexec, s));
+        } else if ((rc = mx_gm_cycle(m, x, q, R, fr, M, s))) {
+          return rc;
         }
-        all &= sc[MS_DONE] != 0.0;
+        HIP_OK(hipMemcpyAsync(hsc.data(), q->sc, hsc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+      } else {
+        kt_launch(KT_M_DOT, s, 32.0 * kt_rhs(R) * n, k_mx_k_init, dim3(nch, R), RED_T, 0, k);
+        kt_launch(KT_M_DOT, s, 0.0, k_mx_k_fin, R, RED_T, 0, k, 0);
+        if (o.use_graph && (rc = mx_graph(m, x, q, R, fr, s, 0))) return rc;
+        // inner iterations, convergence checked every check_every; an inner
+        // solve whose recursive residual has not improved by 1% in
+        // stall_iters iterations (FP32 stagnation on an ill-conditioned
+        // system) is ended; its iterate at each 1% improvement is kept as
+        // a second candidate correction
+        std::vector<double> best(R, 1.0);
+        std::vector<double> best_it(R, 0.0);
+        std::fill(kept.begin(), kept.end(), 0);
+        std::fill(cut.begin(), cut.end(), 0);
+        const int stall = o.stall_iters == 0 ? MX_STALL_ITERS : o.stall_iters;
+        for (;;) {
+          for (int w = 0; w < std::max(1, o.check_every); w++) {
+            if (o.use_graph) HIP_OK(hipGraphLaunch(q->gexec, s));
+            else if ((rc = mx_iteration(m, x, q, R, fr, s))) return rc;
+          }
+          HIP_OK(hipMemcpyAsync(hsc.data(), q->sc, hsc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+          HIP_OK(hipStreamSynchronize(s));
+          bool all = true;
+          for (int r = 0; r < R; r++) {
+            double *sc = &hsc[(size_t)r * MS_N];
+            if (sc[MS_DONE] == 0.0) {
+              if (sc[MS_REL] < 0.99 * best[r]) {
+                best[r] = sc[MS_REL];
+                best_it[r] = sc[MS_IT];
+                if (stall > 0) {
+                  HIP_OK(hipMemcpyAsync(q->xbest + r * n, k.x + r * n, n * sizeof(c2), hipMemcpyDeviceToDevice, s));
+                  kept[r] = 1;
+                }
+              } else if (stall > 0 && sc[MS_IT] - best_it[r] >= stall) {
+                const double one = 1.0;
+                sc[MS_DONE] = 1.0;
+                cut[r] = 1;
+                HIP_OK(hipMemcpyAsync(q->sc + (size_t)r * MS_N + MS_DONE, &one, sizeof one, hipMemcpyHostToDevice, s));
+                HIP_OK(hipStreamSynchronize(s));
+              }
+            }
+            all &= sc[MS_DONE] != 0.0;
+          }
+          int nlive = 0;
+          for (int r = 0; r < R; r++) nlive += hsc[(size_t)r * MS_N + MS_DONE] == 0.0;
+          g_kt_live = nlive;
+          if (all) break;
+        }
       }
-      int nlive = 0;
-      for (int r = 0; r < R; r++) nlive += hsc[(size_t)r * MS_N + MS_DONE] == 0.0;
-      g_kt_live = nlive;
-      if (all) break;
-    }
-    for (int r = 0; r < R; r++)
-      if (hsc[(size_t)r * MS_N + MS_USED] != 0.0) iters[r] += (int)hsc[(size_t)r * MS_N + MS_IT];
-    // x += |res| * correction (FP64), then the true residual in FP64
-    HIP_OK(hipMemcpyAsync(q->xprev, X, (size_t)R * n * sizeof(z2), hipMemcpyDeviceToDevice, s));
-    kt_launch(KT_M_REFINE, s, 32.0 * R * n, k_mx_acc64, dim3(gn, R), 256, 0, n, X, k.x, q->sc, q->scale);
-    for (int r = 0; r < R; r++) {
-      if (hsc[(size_t)r * MS_N + MS_USED] == 0.0) continue;
-      if ((rc = run_apply_bufs(m, &f[r], X + r * n, q->tmp64, s, q->a64))) return rc;
-      kt_launch(KT_M_REFINE, s, 48.0 * n, k_mx_res64, nch, RED_T, 0, n, B + r * n, q->tmp64, q->res + r * n,
-                q->npart + (long)r * nch, nch);
-    }
-    k_mx_norm_fin<<<R, RED_T, 0, s>>>(q->npart, nch, q->norm);
-    HIP_OK(hipMemcpyAsync(nb.data(), q->norm, R * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    for (int r = 0; r < R; r++) {
-      if (hsc[(size_t)r * MS_N + MS_USED] == 0.0) continue;
-      const double rnew = sqrt(nb[r]);
-      if (!(rnew < rn[r])) {
-        // the FP32 inner solve did not help (stagnation or divergence on an
-        // ill-conditioned system): keep the previous solution and residual
-        // and stop refining this right-hand side; it is reported as not
-        // converged
-        stalled[r] = 1;
-        HIP_OK(hipMemcpyAsync(X + r * n, q->xprev + r * n, n * sizeof(z2), hipMemcpyDeviceToDevice, s));
+      for (int r = 0; r < R; r++)
+        if (hsc[(size_t)r * MS_N + MS_USED] != 0.0) iters[r] += (int)hsc[(size_t)r * MS_N + MS_IT];
+      // x += |res| * correction (FP64), then the true residual in FP64
+      HIP_OK(hipMemcpyAsync(q->xprev, X, (size_t)R * n * sizeof(z2), hipMemcpyDeviceToDevice, s));
+      kt_launch(KT_M_REFINE, s, 32.0 * R * n, k_mx_acc64, dim3(gn, R), 256, 0, n, X, k.x, q->sc, q->scale);
+      for (int r = 0; r < R; r++) {
+        if (hsc[(size_t)r * MS_N + MS_USED] == 0.0) continue;
         if ((rc = run_apply_bufs(m, &f[r], X + r * n, q->tmp64, s, q->a64))) return rc;
-        k_mx_res64<<<nch, RED_T, 0, s>>>(n, B + r * n, q->tmp64, q->res + r * n, q->npart + (long)r * nch, nch);
-        continue;
+        kt_launch(KT_M_REFINE, s, 48.0 * n, k_mx_res64, nch, RED_T, 0, n, B + r * n, q->tmp64, q->res + r * n,
+                  q->npart + (long)r * nch, nch);
       }
-      rn[r] = rnew;
-      relres[r] = bn[r] > 0 ? rn[r] / bn[r] : 0.0;
+      k_mx_norm_fin<<<R, RED_T, 0, s>>>(q->npart, nch, q->norm);
+      HIP_OK(hipMemcpyAsync(nb.data(), q->norm, R * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      for (int r = 0; r < R; r++) {
+        if (hsc[(size_t)r * MS_N + MS_USED] == 0.0) continue;
+        double rnew = sqrt(nb[r]);
+        if (!(rnew < rn[r]) && !gm && cut[r] && kept[r]) {
+          // the stagnating inner solve's last iterate did not help: try the
+          // iterate of its best recursive residual
+          HIP_OK(hipMemcpyAsync(X + r * n, q->xprev + r * n, n * sizeof(z2), hipMemcpyDeviceToDevice, s));
+          k_mx_acc64<<<dim3(gn, 1), 256, 0, s>>>(n, X + r * n, q->xbest + r * n, q->sc + r * MS_N, q->scale + r);
+          if ((rc = run_apply_bufs(m, &f[r], X + r * n, q->tmp64, s, q->a64))) return rc;
+          k_mx_res64<<<nch, RED_T, 0, s>>>(n, B + r * n, q->tmp64, q->res + r * n, q->npart + (long)r * nch, nch);
+          k_mx_norm_fin<<<1, RED_T, 0, s>>>(q->npart + (long)r * nch, nch, q->norm + r);
+          HIP_OK(hipMemcpyAsync(&nb[r], q->norm + r, sizeof(double), hipMemcpyDeviceToHost, s));
+          HIP_OK(hipStreamSynchronize(s));
+          rnew = sqrt(nb[r]);
+        }
+        if (!(rnew < rn[r])) {
+          // the FP32 inner solve did not help (stagnation or divergence on an
+          // ill-conditioned system): keep the previous solution and residual
+          // and stop refining this right-hand side (after a cut F-BiCGStab
+          // solve, FGMRES takes it over in phase 1); it is reported as not
+          // converged if nothing reaches tol
+          stalled[r] = 1;
+          cutrej[r] = !gm && cut[r];
+          HIP_OK(hipMemcpyAsync(X + r * n, q->xprev + r * n, n * sizeof(z2), hipMemcpyDeviceToDevice, s));
+          if ((rc = run_apply_bufs(m, &f[r], X + r * n, q->tmp64, s, q->a64))) return rc;
+          k_mx_res64<<<nch, RED_T, 0, s>>>(n, B + r * n, q->tmp64, q->res + r * n, q->npart + (long)r * nch, nch);
+          continue;
+        }
+        rn[r] = rnew;
+        relres[r] = bn[r] > 0 ? rn[r] / bn[r] : 0.0;
+      }
     }
+    if (gm) break;
   }
   g_kt_live = 0;
   HIP_OK(hipGetLastError());
@@ -841,7 +1000,7 @@ extern "C" int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *
     if (e != hipSuccess) { slot_release(x, q); return csem_fail(-1, "sync: %s", hipGetErrorString(e)); }
   }
   const hipStream_t s = stream ? (hipStream_t)stream : q->own;
-  const csem_mixed_opts dflt = {2000, 1e-2, 40, 8, 1, CSEM_PRECOND_LUSGS, 0};
+  const csem_mixed_opts dflt = {2000, 1e-2, 40, 8, 1, CSEM_PRECOND_LUSGS, 0, CSEM_KRYLOV_FBICGSTAB, 0};
   rc = mixed_solve_slot(m, x, q, R, f, (z2 *)xv, (const z2 *)bv, tol, opt ? *opt : dflt, iters, relres, s);
   slot_release(x, q);
   return rc;
@@ -920,7 +1079,7 @@ extern "C" long csem_mixed_slot_bytes(const csem_model *m, int R)
 {
   const long n = m->lE, c2b = sizeof(c2), zb = sizeof(z2);
   const long nch = (n + RED_CHUNK - 1) / RED_CHUNK;
-  long b = (m->fE * R + 3L * mx_pad(m).n * R + 9L * R * n) * c2b + (long)R * nch * 4 * 8 +
+  long b = (m->fE * R + 3L * mx_pad(m).n * R + 10L * R * n) * c2b + (long)R * nch * 4 * 8 +
            (long)R * MS_N * 8 + (2 * R * n + n) * zb + (long)R * nch * 8 + R * 8 + R * 4;
   if (m->airbc) {
     const long NN = (long)m->nfft * m->nfft, p1 = (long)m->n1 * m->n2, p2 = (long)(m->n1 + 1) * m->n2,

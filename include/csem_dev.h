@@ -198,14 +198,24 @@ int csem_vec_addscaled(long n, void *y, double re, double im, const void *x, voi
  * f[r].  x and b are R device vectors of csem_model_len complex doubles,
  * back to back.  Solves A_r x_r = b_r from x = 0 to a true FP64 relative
  * residual |b - A x| / |b| < tol[r] by FP64 iterative refinement around an
- * FP32 LU-SGS + F-BiCGStab inner solve with FP64 dot products.  Returns
+ * FP32 LU-SGS + F-BiCGStab (or FGMRES) inner solve with FP64 dot products.  Returns
  * the number of right-hand sides that reached tol, or < 0 on error;
  * iters[r] receives the inner iterations spent, relres[r] the final true
  * relative residual (host arrays).  opt may be NULL for the defaults.
- * An inner solve whose recursive residual has made no new 1% improvement
- * in stall_iters iterations is ended early (FP32 stagnation); its correction is
- * kept if it lowers the true residual and the next refinement step starts
- * afresh from that residual.  A refinement step that does not lower the true residual (the FP32 inner
+ * An F-BiCGStab inner solve whose recursive residual has made no new 1%
+ * improvement in stall_iters iterations is ended early (FP32 stagnation);
+ * its last iterate is the correction, or, if that does not lower the true
+ * residual, the iterate of its best recursive residual, and the next
+ * refinement step starts afresh from the new true residual; if neither
+ * lowers it, that right-hand side goes on with FGMRES(restart) refinement
+ * steps (below) from its best solution within the same budget.  With krylov = CSEM_KRYLOV_FGMRES each refinement
+ * step is one FGMRES(restart) cycle from x = 0 on the scaled true
+ * residual (classical Gram-Schmidt with one reorthogonalisation pass, the
+ * Hessenberg matrix in FP64 on the device, one captured graph per cycle);
+ * a cycle ends early once its residual estimate reaches the reduction the
+ * step needs, and max_inner bounds the cycles' steps in total (max_outer
+ * and check_every are not used).  Such a solve holds (2 restart + 1) R
+ * more complex-float vectors than csem_mixed_slot_bytes states.  A refinement step that does not lower the true residual (the FP32 inner
  * solve stagnating or diverging on an ill-conditioned system) is undone
  * and refinement of that right-hand side stops: x keeps the best solution
  * found and relres[r] its residual, never above 1.  A caller that needs
@@ -225,11 +235,16 @@ typedef struct {
                         splitting with the cells of each slice visited in
                         red-black order (fully parallel passes) */
   int stall_iters;   /* inner iterations without a new 1% residual
-                        improvement that end an inner solve (0 = the
-                        default, 100; < 0 = never) */
+                        improvement that end an F-BiCGStab inner solve (0 =
+                        the default, 100; < 0 = never) */
+  int krylov;        /* inner method: CSEM_KRYLOV_FBICGSTAB (solver.c:93-178)
+                        or CSEM_KRYLOV_FGMRES (solver.c:403-512) */
+  int restart;       /* FGMRES restart length m (0 = 10, at most 16) */
 } csem_mixed_opts;
 #define CSEM_PRECOND_LUSGS 0
 #define CSEM_PRECOND_RBSGS 1
+#define CSEM_KRYLOV_FBICGSTAB 0
+#define CSEM_KRYLOV_FGMRES 1
 int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *x, const void *b, const double *tol,
                      const csem_mixed_opts *opt, int *iters, double *relres, void *stream);
 /* The FP32 operator and preconditioner on R complex-float vectors back to
