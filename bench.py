@@ -247,6 +247,177 @@ def apply_large(reps=20, copy_gbs=None):
     return out
 
 
+def _rec_gap(a, b):
+    """per channel: max |a - b| over receivers / max |b| of the channel"""
+    sc = np.abs(b).max(axis=1)
+    return [float("%.2e" % (np.abs(a[c] - b[c]).max() / sc[c])) if sc[c] > 0 else 0.0 for c in range(a.shape[0])]
+
+
+def aniso_leg(freqs, niter):
+    """The anisotropic VTI model (180 x 120 x 120 cells, 11.1 M unknowns,
+    six channels) at `freqs`: LU-SGS preconditioned F-BiCGStab and
+    FGMRES(10), each in plain FP64 on the device (the reference's solvers,
+    one frequency after the other) and in mixed precision (FP32 inner
+    F-BiCGStab or FGMRES(10) cycles inside FP64 refinement, one solve per
+    frequency on three host threads).  Solves/s count the solves whose true
+    FP64 relative residual ends below tol; receivers of each mixed solve are
+    compared with the plain FP64 solve of the same method."""
+    from demo_matrixfree_csem_amd import surveys, _lib
+    from demo_matrixfree_csem_amd.session import Session, freq_consts
+    L = _lib.lib()
+    s = surveys.vti_marine()
+    work = tempfile.mkdtemp(prefix="csem_bench_vti_")
+    surveys.write_files(work, s)
+    ses = Session(s.argv(work))
+    model, n = ses.model, ses.n
+    R = len(freqs)
+    tols = [float(np.float32(np.float32(1e-8) / np.float32(f))) for f in freqs]
+    fr = (_lib.FreqT * R)(*[freq_consts(f) for f in freqs])
+    hb = []
+    for f in freqs:
+        ses.set_freq(f)
+        hb.append(ses.inject())
+    hb = np.ascontiguousarray(np.concatenate(hb), np.complex128)
+    db, dx, dw = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    for d, nb in ((db, R * n * 16), (dx, R * n * 16), (dw, n * 16)):
+        _lib.check(L.csem_malloc(C.byref(d), nb), "csem_malloc")
+    out = {"workload": "vti_marine 180x120x120 + nb=10 (n_op %d), lextend 120 km, LU-SGS, f=%s Hz, tol=1e-8/f"
+                       % (n, "/".join("%g" % f for f in freqs)), "n_op": n, "niter": niter}
+    xs = {}
+
+    def at(p, r):
+        return C.c_void_p(p.value + r * n * 16)
+
+    def true_rel(r):
+        L.csem_apply(model, C.byref(fr[r]), at(dx, r), dw, None)
+        w = np.zeros(n, np.complex128)
+        L.csem_memcpy_d2h(w.ctypes.data, dw, n * 16, None)
+        L.csem_sync(None)
+        br = hb[r * n:(r + 1) * n]
+        return float(np.linalg.norm(br - w) / np.linalg.norm(br))
+
+    def fetch(key):
+        h = np.zeros(R * n, np.complex128)
+        L.csem_memcpy_d2h(h.ctypes.data, dx, R * n * 16, None)
+        L.csem_sync(None)
+        xs[key] = h
+
+    try:
+        _lib.check(L.csem_memcpy_h2d(db, hb.ctypes.data, R * n * 16, None), "upload")
+        for name, solve in (
+                ("fp64_fbicgstab", lambda r: L.csem_bicgstab(model, C.byref(fr[r]), at(dx, r), at(db, r), niter,
+                                                             tols[r], 1, None, None)),
+                ("fp64_fgmres10", lambda r: L.csem_fgmres(model, C.byref(fr[r]), at(dx, r), at(db, r), niter // 10,
+                                                          tols[r], 10, 1, None, None))):
+            L.csem_memset(dx, 0, R * n * 16, None)
+            L.csem_sync(None)
+            t0 = time.perf_counter()
+            its = []
+            for r in range(R):
+                its.append(_lib.check(solve(r), name))
+            L.csem_sync(None)
+            dt = time.perf_counter() - t0
+            rel = [true_rel(r) for r in range(R)]
+            ok = sum(rel[r] < tols[r] for r in range(R))
+            out[name] = {"solves_per_s": round(ok / dt, 4), "wall_s": round(dt, 3), "iterations": its,
+                         "true_relres": [float("%.3e" % v) for v in rel], "converged": ok}
+            fetch(name)
+        for name, kry in (("mixed_fbicgstab", _lib.KRYLOV_FBICGSTAB), ("mixed_fgmres10", _lib.KRYLOV_FGMRES)):
+            opts = _lib.MixedOpts.default(max_inner=niter, krylov=kry, restart=10)
+            L.csem_memset(dx, 0, R * n * 16, None)
+            L.csem_sync(None)
+            res = [None] * R
+            nxt = [0]
+            mu = threading.Lock()
+
+            def worker():
+                while True:
+                    with mu:
+                        r = nxt[0]
+                        nxt[0] += 1
+                    if r >= R:
+                        return
+                    itr, rr = np.zeros(1, np.int32), np.zeros(1)
+                    tl = np.array([tols[r]])
+                    nc = L.csem_mixed_solve(model, 1, C.byref(fr[r]), at(dx, r), at(db, r), tl.ctypes.data,
+                                            C.byref(opts), itr.ctypes.data, rr.ctypes.data, None)
+                    res[r] = (nc, int(itr[0]), float(rr[0]))
+
+            t0 = time.perf_counter()
+            th = [threading.Thread(target=worker) for _ in range(min(3, R))]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+            L.csem_sync(None)
+            dt = time.perf_counter() - t0
+            for o in res:
+                _lib.check(o[0], name)
+            ok = sum(o[2] < tols[r] for r, o in enumerate(res))
+            out[name] = {"solves_per_s": round(ok / dt, 4), "wall_s": round(dt, 3),
+                         "inner_iterations": [o[1] for o in res], "true_relres": [float("%.3e" % o[2]) for o in res],
+                         "converged": ok}
+            fetch(name)
+        for mixed, plain in (("mixed_fbicgstab", "fp64_fbicgstab"), ("mixed_fgmres10", "fp64_fgmres10")):
+            gaps = []
+            for r, f in enumerate(freqs):
+                ses.set_freq(f)  # the H channels scale by 1 / (i w mu0)
+                a = ses.extract_double(xs[mixed][r * n:(r + 1) * n])
+                b = ses.extract_double(xs[plain][r * n:(r + 1) * n])
+                gaps.append(_rec_gap(a, b))
+            out[mixed]["receiver_err_vs_%s" % plain] = gaps
+        for m, p in (("mixed_fbicgstab", "fp64_fbicgstab"), ("mixed_fgmres10", "fp64_fgmres10")):
+            if out[p]["solves_per_s"] > 0:
+                out[m]["speedup_vs_%s" % p] = round(out[m]["solves_per_s"] / out[p]["solves_per_s"], 3)
+        out["channels"] = s.flags["chrec"]
+        out["receiver_err_what"] = "per frequency, per channel: max |mixed - FP64| / max |FP64| over the 121 receivers"
+    finally:
+        for d in (db, dx, dw):
+            L.csem_free(d)
+        ses.close()
+    return out
+
+
+def topo_leg(workers=3, rhs_per_solve=4):
+    """The batched topography workload: 8 Ex sources x 6 frequencies on the
+    125 x 125 x 100 seafloor-topography model, plus 16 solves on 64^3 and
+    96 x 96 x 80 grids of the 64^3 parity model's construction (4 sources x
+    2 frequencies each), served by the engine with every request at t = 0:
+    up to `rhs_per_solve` right-hand sides of one model per solve (hipGraph
+    replays, autotuned operator tile), `workers` solves concurrently on their
+    own streams, the three models resident (built beforehand)."""
+    from demo_matrixfree_csem_amd import surveys, serve
+    topo = surveys.topo_marine()
+    tfreqs = list(topo.freqs)
+    trec = [tuple(float(v) for v in r[:3]) for r in topo.acq.receivers]
+    reqs = [(("topo", 125, 125, 100), surveys.topo_marine, tfreqs, [tuple(float(v) for v in src[:3])], trec)
+            for src in topo.acq.sources]
+    for g in ((64, 64, 64), (96, 96, 80)):
+        v = surveys.layered_64_variant(*g)
+        reqs.append((("layered",) + g, (lambda g=g: surveys.layered_64_variant(*g)), list(v.freqs),
+                     [tuple(float(x) for x in src[:3]) for src in v.acq.sources],
+                     [tuple(float(x) for x in r[:3]) for r in v.acq.receivers]))
+    eng = serve.Engine(workers=workers, rhs_per_solve=rhs_per_solve)
+    try:
+        t0 = time.perf_counter()
+        eng.preload(reqs)
+        build_s = time.perf_counter() - t0
+        results, st = eng.run(reqs, [0.0] * len(reqs))
+    finally:
+        eng.close()
+    its, worst = {}, 0.0
+    for i, rq in enumerate(reqs):
+        for (s_i, f), (_, it, rel) in results[i]["receivers"].items():
+            its.setdefault(rq[0][0] if rq[0][0] == "topo" else "x".join(map(str, rq[0][1:])), []).append(int(it))
+            worst = max(worst, float(rel) / serve.solve_tol(f))
+    st.update({"solves_expected": 48 + 16, "model_build_s": round(build_s, 3),
+               "inner_iterations_mean": {k: round(float(np.mean(v)), 1) for k, v in its.items()},
+               "worst_relres_over_tol": float("%.3e" % worst),
+               "workload": "125x125x100 topography (8 Ex sources x 6 frequencies 0.125-1.25 Hz) + 64^3 and "
+                           "96x96x80 layered variants (4 sources x 0.25/0.75 Hz each); mixed precision, tol 1e-8/f"})
+    return st
+
+
 def cpu_baseline(survey, freq, iters, gpu_iters):
     """The oracle (serial C restatement of the reference path) on this
     host: `iters` LU-SGS F-BiCGStab iterations of one solve, scaled to
@@ -295,6 +466,14 @@ def main():
     ap.add_argument("--apply-only", action="store_true",
                     help="counter runs: only the operator launches on the 180x120x120 model and the copy, then exit")
     ap.add_argument("--no-large", action="store_true", help="skip the 180x120x120 operator measurement")
+    ap.add_argument("--aniso-freqs", default="0.75,1.25",
+                    help="frequencies of the anisotropic-model leg (both Krylov methods in both precisions), "
+                         "comma-separated, 'all' for the workload's six, 'none' to skip")
+    ap.add_argument("--aniso-niter", type=int, default=6000)
+    ap.add_argument("--no-topo", action="store_true",
+                    help="skip the batched topography leg (48 solves + 16 on two smaller grids)")
+    ap.add_argument("--legs-only", action="store_true",
+                    help="run only the anisotropic and topography legs and print their JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this path has no multi-GPU form; independent solves would "
@@ -353,6 +532,23 @@ def main():
         res = apply_large()
         copy_1gib()
         print(json.dumps({"apply_only": res, "copy_bytes_each_way": 2 ** 30}), flush=True)
+        return
+
+    def legs():
+        o = {}
+        if args.aniso_freqs != "none":
+            af = (surveys.vti_marine(2, 8, 8, 8, 3).freqs if args.aniso_freqs == "all"
+                  else [float(v) for v in args.aniso_freqs.split(",")])
+            print("leg: anisotropic model, f = %s" % af, file=sys.stderr, flush=True)
+            o["aniso"] = aniso_leg(af, args.aniso_niter)
+        if not args.no_topo:
+            print("leg: batched topography workload", file=sys.stderr, flush=True)
+            o["topo_batch"] = topo_leg()
+        return o
+
+    if args.legs_only:
+        ses.close()
+        print(json.dumps(legs()), flush=True)
         return
 
     def plain_step():
@@ -599,9 +795,10 @@ def main():
         out["serving_cold"] = cst
     if not args.no_cpu and plain:
         out["cpu_baseline"] = cpu_baseline(survey, freqs[0], args.cpu_iters, plain["iterations"])
+    ses.close()
     if not args.no_large:
-        ses.close()
         out["apply_large"] = apply_large(copy_gbs=copy_gbs)
+    out.update(legs())
     print(json.dumps(out), flush=True)
 
 

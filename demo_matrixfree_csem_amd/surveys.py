@@ -116,6 +116,28 @@ def layered_64():
                        lextend=40e3, chsrc="Ex", chrec="Ex,Ey,Ez"))
 
 
+def layered_64_variant(nx, ny, nz, sources=(-2000.0, -1000.0, 1000.0, 2000.0), freqs=(0.25, 0.75)):
+    """The 64^3 parity model's construction on another grid of 100 m cells:
+    x and y centred on 0, z from 0; seawater above 1000 m, 1 ohm-m below,
+    the 100 ohm-m slab at z 2000-2100 m for |x|, |y| <= 1500 m; Ex sources
+    at (x, 0, 950) for x in `sources`; receivers every 200 m on y = 0,
+    z = 1000 within the grid."""
+    x = (100.0 * (np.arange(nx + 1) - nx / 2)).astype(np.float32)
+    y = (100.0 * (np.arange(ny + 1) - ny / 2)).astype(np.float32)
+    z = (100.0 * np.arange(nz + 1)).astype(np.float32)
+    xc, yc, zc = _centers(x), _centers(y), _centers(z)
+    rho = np.where(zc[:, None, None] < 1000, RHO_WATER, 1.0) * np.ones((nz, ny, nx))
+    slab = (np.abs(xc)[None, None, :] <= 1500) & (np.abs(yc)[None, :, None] <= 1500) & \
+           (zc[:, None, None] > 2000) & (zc[:, None, None] < 2100)
+    rho = np.where(slab, 100.0, rho).astype(np.float32)
+    half = 100.0 * (nx // 2)
+    rec = [(xr, 0.0, 1000.0) for xr in np.arange(-half, half + 1, 200)]
+    return Survey("layered_%dx%dx%d" % (nx, ny, nz), Model(x, y, z, rho, rho.copy(), rho.copy()),
+                  Acquisition(_acq_table([(sx, 0.0, 950.0) for sx in sources]), _acq_table(rec)), list(freqs),
+                  dict(fvm=0, airbc=1, preco=1, algopt=1, tol=1e-8, niter=2000, nb=10,
+                       lextend=40e3, chsrc="Ex", chrec="Ex,Ey,Ez"))
+
+
 def layered_100(seed=1):
     """100^3 uniform 100 m cells on x,y in [-5000,5000], z in [0,10000]:
     water (0.3125 ohm-m) above 1000 m, then 8 layers of 1125 m with seeded
