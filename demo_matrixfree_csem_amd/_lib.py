@@ -115,7 +115,7 @@ DEV_SYMBOLS = [
     "csem_vec_sub", "csem_vec_divr", "csem_vec_divz", "csem_vec_subscaled_dev", "csem_vec_addscaled",
     "csem_op_apply", "csem_op_lusgs", "csem_op_aphi", "csem_op_gmres", "csem_op_host",
     "csem_gmres_prec_release", "csem_bicgstab_op", "csem_gmres_op",
-    "csem_mixed_solve", "csem_apply32", "csem_lusgs32", "csem_mixed_tuning", "csem_mixed_set_tile", "csem_mixed_bytes", "csem_mixed_slot_bytes", "csem_mem_info",
+    "csem_mixed_solve", "csem_apply32", "csem_lusgs32", "csem_mixed_tuning", "csem_mixed_set_tile", "csem_mixed_tuning_lu", "csem_mixed_set_lu_tiles", "csem_mixed_bytes", "csem_mixed_slot_bytes", "csem_mem_info",
 ]
 
 _lib = None
@@ -161,6 +161,8 @@ def lib():
     L.csem_lusgs32.argtypes = [vp, ip, C.POINTER(FreqT), vp, vp, vp]
     L.csem_mixed_tuning.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip)]
     L.csem_mixed_set_tile.argtypes = [vp, ip, ip, ip]
+    L.csem_mixed_tuning_lu.argtypes = [vp, ip, vp, C.POINTER(ip)]
+    L.csem_mixed_set_lu_tiles.argtypes = [vp, ip, vp, ip]
     L.csem_mixed_bytes.argtypes = [vp]
     L.csem_mixed_slot_bytes.argtypes = [vp, ip]
     L.csem_mixed_slot_bytes.restype = lp

@@ -29,9 +29,15 @@ struct MixState {
   float *s11, *s22, *s33, *c1m, *c1p, *c2m, *c2p, *c3m, *c3p;
   c2 *sH1, *sH2, *tw;
   float *wx, *wy, *gxw1, *gxw2, *gyw1, *gyw2;
-  int cw;
-  // autotuned operator tile per batch size: rows per block, planes per block
+  int cw;       // default columns per block of the FFT column pass
+  int cw_max;   // the largest candidate (the kernel's LDS attribute is set for it)
+  // autotuned per batch size: the operator tile (rows per block, planes per
+  // block), the six LU-SGS pre-passes' tiles (candidate index, in launch
+  // order: Ex, Ey, Ez forward, Ez, Ey, Ex backward) and the FFT column
+  // pass's columns per block
   int ap_tj[MX_RMAX + 1], ap_kc[MX_RMAX + 1];
+  int pre_v[MX_RMAX + 1][6];
+  int air_cw[MX_RMAX + 1];
   std::mutex mu;                 // guards slots and tuning
   std::vector<struct MixSlot *> slots;
   std::vector<void *> allocs;
@@ -183,6 +189,8 @@ static int mx_model(csem_model *m, MixState **out)
 {
   if (m->mx) { *out = m->mx; return 0; }
   MixState *x = new MixState();
+  for (int r = 0; r <= MX_RMAX; r++)
+    for (int k = 0; k < 6; k++) x->pre_v[r][k] = -1;  // untuned: the default tiles
   const int n1 = m->n1, n2 = m->n2, n3f = m->n3f;
   const hipStream_t s = 0;
   int rc;
@@ -202,7 +210,8 @@ static int mx_model(csem_model *m, MixState **out)
     const long NN = (long)N * N;
     if (N < 64 || 4096 % N) { rc = csem_fail(-3, "mixed path needs nfft in [64, 4096] (got %d)", N); goto fail; }
     x->cw = N >= 2048 ? 1 : 2048 / N;  // columns per column-pass block of N cw / 16 threads, 16 values each
-    const int lc = (int)((2 * (size_t)N * x->cw + N) * sizeof(c2)), lr = (int)(3 * (size_t)N * sizeof(c2));
+    x->cw_max = 4096 / N;              // candidates: N cw from 1024 to 4096 (64 to 256 threads)
+    const int lc = (int)((2 * (size_t)N * x->cw_max + N) * sizeof(c2)), lr = (int)(3 * (size_t)N * sizeof(c2));
     if (hipFuncSetAttribute((const void *)k_mx_air_cols, hipFuncAttributeMaxDynamicSharedMemorySize, lc) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_mx_air_rows, hipFuncAttributeMaxDynamicSharedMemorySize, lr) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_mx_air_rows_inv, hipFuncAttributeMaxDynamicSharedMemorySize, lr) != hipSuccess) {
@@ -318,11 +327,11 @@ static MxFreq mx_freq(const csem_freq *f, int R)
 }
 
 // ---------------------------------------------------------------- operator
-static MxAir mx_air(const csem_model *m, const MixState *x, const MixSlot *q)
+static MxAir mx_air(const csem_model *m, const MixState *x, const MixSlot *q, int R)
 {
   MxAir a;
   a.sc = nullptr;
-  a.N = m->nfft; a.n1 = m->n1; a.n2 = m->n2; a.cw = x->cw;
+  a.N = m->nfft; a.n1 = m->n1; a.n2 = m->n2; a.cw = x->air_cw[R] ? x->air_cw[R] : x->cw;
   a.hz = q->hz0; a.buf = q->buf; a.bufx = q->bufx; a.bufy = q->bufy;
   a.sH1 = x->sH1; a.sH2 = x->sH2; a.tw = x->tw;
   a.jx = m->jx; a.jy = m->jy; a.wx = x->wx; a.wy = x->wy;
@@ -418,14 +427,14 @@ static int mx_apply(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
 {
   if (m->airbc) {
     const int nh = m->n1 * m->n2, N = m->nfft;
-    MxAir a = mx_air(m, x, q);
+    MxAir a = mx_air(m, x, q, R);
     a.sc = sc;
     kt_launch(KT_M_HZ0, s, 24.0 * nh * kt_rhs(R), k_mx_hz0, dim3((nh + 255) / 256, R), 256, 0, mx_geo(m, x), in,
               q->hz0, sc);
     kt_launch(KT_M_AIR_ROWS, s, 8.0 * N * N * kt_rhs(R), k_mx_air_rows, dim3(N, R), 256,
               3 * (size_t)N * sizeof(c2), a);
-    kt_launch(KT_M_AIR_COLS, s, 40.0 * N * N * kt_rhs(R), k_mx_air_cols, dim3(8 * ((N / x->cw + 7) / 8), R),
-              N * x->cw / 16, (2 * (size_t)N * x->cw + N) * sizeof(c2), a);
+    kt_launch(KT_M_AIR_COLS, s, 40.0 * N * N * kt_rhs(R), k_mx_air_cols, dim3(8 * ((N / a.cw + 7) / 8), R),
+              N * a.cw / 16, (2 * (size_t)N * a.cw + N) * sizeof(c2), a);
     kt_launch(KT_M_AIR_RINV, s, 16.0 * N * (m->nrowx + m->nrowy) * kt_rhs(R), k_mx_air_rows_inv,
               dim3(m->nrowx + m->nrowy, R), 256, 3 * (size_t)N * sizeof(c2), a);
     const int ng = (m->n1 + 1) * m->n2 + m->n1 * (m->n2 + 1);
@@ -486,15 +495,46 @@ static void mx_sgs_launch(dim3 g, int tb, size_t lds, hipStream_t s, const MxLu 
   }
 }
 
+// one pre-pass launch with tile candidate v (MxPreTileV)
+template <int C, bool FWD, int R>
+static void mx_pre_launch(const SkewGeo &g, int v, hipStream_t s, const MxLu &a, double bytes)
+{
+  switch (v) {
+    case 1: kt_launch(KT_M_PRE, s, bytes, k_mx_pre<C, FWD, R, 1>, mx_tile_grid<MxPreTileV<C, FWD, 1>>(g, C), 256, 0, a); break;
+    case 2: kt_launch(KT_M_PRE, s, bytes, k_mx_pre<C, FWD, R, 2>, mx_tile_grid<MxPreTileV<C, FWD, 2>>(g, C), 256, 0, a); break;
+    default: kt_launch(KT_M_PRE, s, bytes, k_mx_pre<C, FWD, R, 0>, mx_tile_grid<MxPreTileV<C, FWD, 0>>(g, C), 256, 0, a); break;
+  }
+}
+// pre-pass number kp of an application (Ex, Ey, Ez forward, Ez, Ey, Ex
+// backward) with tile candidate v
 template <int R>
-static void mx_lusgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
+static void mx_pre_k(const SkewGeo &g, int kp, int v, hipStream_t s, const MxLu &a, double bytes)
+{
+  switch (kp) {
+    case 0: mx_pre_launch<0, true, R>(g, v, s, a, bytes); break;
+    case 1: mx_pre_launch<1, true, R>(g, v, s, a, bytes); break;
+    case 2: mx_pre_launch<2, true, R>(g, v, s, a, bytes); break;
+    case 3: mx_pre_launch<2, false, R>(g, v, s, a, bytes); break;
+    case 4: mx_pre_launch<1, false, R>(g, v, s, a, bytes); break;
+    default: mx_pre_launch<0, false, R>(g, v, s, a, bytes); break;
+  }
+}
+static void mx_pre_any(const SkewGeo &g, int R, int kp, int v, hipStream_t s, const MxLu &a)
+{
+  switch (R) {
+    case 1: mx_pre_k<1>(g, kp, v, s, a, 0.0); break;
+    case 2: mx_pre_k<2>(g, kp, v, s, a, 0.0); break;
+    case 3: mx_pre_k<3>(g, kp, v, s, a, 0.0); break;
+    default: mx_pre_k<4>(g, kp, v, s, a, 0.0); break;
+  }
+}
+
+template <int R>
+static void mx_lusgs_R(const csem_model *m, const MxLu &a, const int *pv, hipStream_t s)
 {
   const SkewGeo &g = m->g;
   const MxPad &pk = a.pk;
-  // tile grids of the pre-passes (MxPreTile, XCD-ordered) and the copies (MxCopyTile, 3-D)
-  const dim3 gq[3] = {mx_tile_grid<MxPreTile<0, false>>(g, 0), mx_tile_grid<MxPreTile<1, false>>(g, 1),
-                      mx_tile_grid<MxPreTile<2, false>>(g, 2)};
-  const dim3 gqx = mx_tile_grid<MxPreTile<0, true>>(g, 0);  // the forward Ex tile
+  // tile grids of the copies (MxCopyTile, 3-D)
   const dim3 gu[3] = {mx_tile_grid3<MxCopyTile<0, R>>(g, 0), mx_tile_grid3<MxCopyTile<1, R>>(g, 1),
                       mx_tile_grid3<MxCopyTile<2, R>>(g, 2)};
   int tb[3];
@@ -513,18 +553,18 @@ static void mx_lusgs_R(const csem_model *m, const MxLu &a, hipStream_t s)
   // value to W and (backward) y; for Ex it writes the value to V, and the
   // copy reads V and writes W (forward) or y (backward)
   const double r = kt_rhs(R);
-  kt_launch(KT_M_PRE, s, 16.0 * r * nc[0], k_mx_pre<0, true, R>, gqx, 256, 0, a);
+  mx_pre_launch<0, true, R>(g, pv[0], s, a, 16.0 * r * nc[0]);
   mx_sgs_launch<0, true>(gs[0], tb[0], lds[0], s, a, 24.0 * r * nc[0]);
   kt_launch(KT_M_UNSKEW, s, 16.0 * r * nc[0], k_mx_unskew<0, true, R>, gu[0], 256, 0, a);
-  kt_launch(KT_M_PRE, s, 24.0 * r * nc[1], k_mx_pre<1, true, R>, gq[1], 256, 0, a);
+  mx_pre_launch<1, true, R>(g, pv[1], s, a, 24.0 * r * nc[1]);
   mx_sgs_launch<1, true>(gs[1], tb[1], lds[1], s, a, 24.0 * r * nc[1]);
-  kt_launch(KT_M_PRE, s, 32.0 * r * nc[2], k_mx_pre<2, true, R>, gq[2], 256, 0, a);
+  mx_pre_launch<2, true, R>(g, pv[2], s, a, 32.0 * r * nc[2]);
   mx_sgs_launch<2, true>(gs[2], tb[2], lds[2], s, a, 24.0 * r * nc[2]);
-  kt_launch(KT_M_PRE, s, 40.0 * r * nc[2], k_mx_pre<2, false, R>, gq[2], 256, 0, a);
+  mx_pre_launch<2, false, R>(g, pv[3], s, a, 40.0 * r * nc[2]);
   mx_sgs_launch<2, false>(gs[2], tb[2], lds[2], s, a, 32.0 * r * nc[2]);
-  kt_launch(KT_M_PRE, s, 40.0 * r * nc[1], k_mx_pre<1, false, R>, gq[1], 256, 0, a);
+  mx_pre_launch<1, false, R>(g, pv[4], s, a, 40.0 * r * nc[1]);
   mx_sgs_launch<1, false>(gs[1], tb[1], lds[1], s, a, 32.0 * r * nc[1]);
-  kt_launch(KT_M_PRE, s, 40.0 * r * nc[0], k_mx_pre<0, false, R>, gq[0], 256, 0, a);
+  mx_pre_launch<0, false, R>(g, pv[5], s, a, 40.0 * r * nc[0]);
   mx_sgs_launch<0, false>(gs[0], tb[0], lds[0], s, a, 24.0 * r * nc[0]);
   kt_launch(KT_M_UNSKEW, s, 16.0 * r * nc[0], k_mx_unskew<0, false, R>, gu[0], 256, 0, a);
 }
@@ -569,13 +609,13 @@ static void mx_rbsgs_dispatch(const csem_model *m, int R, const MxLu &a, hipStre
   }
 }
 
-static void mx_lusgs_dispatch(const csem_model *m, int R, const MxLu &a, hipStream_t s)
+static void mx_lusgs_dispatch(const csem_model *m, int R, const MxLu &a, const int *pv, hipStream_t s)
 {
   switch (R) {
-    case 1: mx_lusgs_R<1>(m, a, s); break;
-    case 2: mx_lusgs_R<2>(m, a, s); break;
-    case 3: mx_lusgs_R<3>(m, a, s); break;
-    default: mx_lusgs_R<4>(m, a, s); break;
+    case 1: mx_lusgs_R<1>(m, a, pv, s); break;
+    case 2: mx_lusgs_R<2>(m, a, pv, s); break;
+    case 3: mx_lusgs_R<3>(m, a, pv, s); break;
+    default: mx_lusgs_R<4>(m, a, pv, s); break;
   }
 }
 
@@ -586,7 +626,63 @@ static int mx_lusgs(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq 
   MxLu a = mx_lu(m, x, q, fr, b, y);
   a.sc = sc;
   if (q->pc == CSEM_PRECOND_RBSGS) mx_rbsgs_dispatch(m, R, a, s);
-  else mx_lusgs_dispatch(m, R, a, s);
+  else mx_lusgs_dispatch(m, R, a, x->pre_v[R], s);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+// Autotuner, second part: each of the six LU-SGS pre-passes over its tile
+// candidates (MxPreTileV) and the FFT column pass over its columns per
+// block, timed on this model and batch size; the fastest are kept.  Neither
+// choice changes a result: a pre-pass record and a column's transform are
+// the same expressions whatever the tile.  The pre-passes only write the
+// slot's records (which every application rewrites before a sweep reads
+// them) and the column pass only the slot's transform scratch.
+static int mx_tune_lu(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, const c2 *in, c2 *out,
+                      hipStream_t s)
+{
+  std::lock_guard<std::mutex> lk(x->mu);
+  if (x->pre_v[R][0] >= 0 && x->air_cw[R]) return 0;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return csem_fail(-1, "event");
+  auto timed = [&](auto launch) {
+    launch();  // warm
+    (void)hipEventRecord(e0, s);
+    for (int w = 0; w < 3; w++) launch();
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    return ms;
+  };
+  kt_pause(1);
+  const MxLu a = mx_lu(m, x, q, fr, in, out);
+  for (int kp = 0; kp < 6; kp++) {
+    float best = 1e30f;
+    int bv = 0;
+    for (int v = 0; v < MX_PRE_NV; v++) {
+      const float ms = timed([&] { mx_pre_any(m->g, R, kp, v, s, a); });
+      if (ms < best) { best = ms; bv = v; }
+    }
+    x->pre_v[R][kp] = bv;
+  }
+  int bc = x->cw;
+  if (m->airbc) {
+    const int N = m->nfft;
+    MxAir ar = mx_air(m, x, q, R);
+    float best = 1e30f;
+    for (int c = std::max(1, 1024 / N); c <= x->cw_max; c *= 2) {
+      ar.cw = c;
+      const float ms = timed([&] {
+        k_mx_air_cols<<<dim3(8 * ((N / c + 7) / 8), R), N * c / 16, (2 * (size_t)N * c + N) * sizeof(c2), s>>>(ar);
+      });
+      if (ms < best) { best = ms; bc = c; }
+    }
+  }
+  x->air_cw[R] = bc;
+  kt_pause(0);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -824,7 +920,8 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
   HIP_OK(hipMemcpyAsync(nb.data(), q->norm, R * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   for (int r = 0; r < R; r++) { bn[r] = sqrt(nb[r]); rn[r] = bn[r]; relres[r] = 1.0; iters[r] = 0; }
-  if ((rc = mx_tune(m, x, q, R, fr, k.r, k.v, s))) return rc;  // operands: any R x n FP32 buffers
+  // operands: any R x n FP32 buffers
+  if ((rc = mx_tune(m, x, q, R, fr, k.r, k.v, s)) || (rc = mx_tune_lu(m, x, q, R, fr, k.r, k.v, s))) return rc;
   // Phase 0 refines with the inner method asked for.  A right-hand side
   // whose F-BiCGStab inner solve was cut for stagnation and whose
   // correction the guard still rejected goes on in phase 1 with FGMRES(M)
@@ -1071,6 +1168,43 @@ extern "C" int csem_mixed_set_tile(csem_model *m, int R, int rows, int planes)
   std::lock_guard<std::mutex> lk(x->mu);
   x->ap_kc[R] = planes;
   x->ap_tj[R] = rows;
+  return 0;
+}
+
+extern "C" int csem_mixed_tuning_lu(csem_model *m, int R, int *pre, int *cols)
+{
+  MixState *x;
+  {
+    std::lock_guard<std::mutex> lk(g_mx_init_mu);
+    x = m->mx;
+  }
+  if (!x || R < 1 || R > MX_RMAX) return -1;
+  std::lock_guard<std::mutex> lk(x->mu);
+  if (x->pre_v[R][0] < 0 || !x->air_cw[R]) return -1;
+  for (int k = 0; k < 6; k++) pre[k] = x->pre_v[R][k];
+  *cols = x->air_cw[R];
+  return 0;
+}
+
+extern "C" int csem_mixed_set_lu_tiles(csem_model *m, int R, const int *pre, int cols)
+{
+  MixState *x;
+  int rc;
+  if (R < 1 || R > MX_RMAX) return csem_fail(-3, "batch of %d right-hand sides outside [1, %d]", R, MX_RMAX);
+  for (int k = 0; k < 6; k++)
+    if (pre[k] < 0 || pre[k] >= MX_PRE_NV) return csem_fail(-3, "pre-pass tile candidate %d outside [0, %d)", pre[k], MX_PRE_NV);
+  {
+    std::lock_guard<std::mutex> lk(g_mx_init_mu);
+    if ((rc = mx_model(m, &x))) return rc;
+  }
+  if (m->airbc) {
+    const int N = m->nfft;
+    if (cols < std::max(1, 1024 / N) || cols > x->cw_max || (cols & (cols - 1)))
+      return csem_fail(-3, "FFT column pass: %d columns per block not available for a %d-point transform", cols, N);
+  }
+  std::lock_guard<std::mutex> lk(x->mu);
+  for (int k = 0; k < 6; k++) x->pre_v[R][k] = pre[k];
+  x->air_cw[R] = m->airbc ? cols : x->cw;
   return 0;
 }
 

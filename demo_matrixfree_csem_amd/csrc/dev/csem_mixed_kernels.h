@@ -566,6 +566,16 @@ template <int Q, int S, int F> struct MxTile3 { static constexpr int TQ = Q, TS 
 template <int C, bool FWD> struct MxPreTile : MxTile3<PRE_TQ, 16 / PRE_TQ, 64> {};
 template <> struct MxPreTile<0, false> : MxTile3<32, 8, 4> {};
 template <> struct MxPreTile<0, true> : MxTile3<16, 8, 8> {};
+// The candidates the autotuner times per model and batch size (V = 0 is
+// the tile above).  Every candidate holds 1024 cells (four per thread) and,
+// for Ex, a multiple of 256 / (TQ TF) rows, so the LDS arrays and the
+// thread mapping are the same; a cell's record does not depend on the tile.
+#define MX_PRE_NV 3
+template <int C, bool FWD, int V> struct MxPreTileV : MxPreTile<C, FWD> {};
+template <bool FWD> struct MxPreTileV<0, FWD, 1> : MxTile3<FWD ? 32 : 16, 8, FWD ? 4 : 8> {};
+template <bool FWD> struct MxPreTileV<0, FWD, 2> : MxTile3<8, 8, 16> {};
+template <int C, bool FWD> struct MxPreTileV<C, FWD, 1> : MxTile3<1, 16, 64> {};
+template <int C, bool FWD> struct MxPreTileV<C, FWD, 2> : MxTile3<4, 4, 64> {};
 // The copy from V (k_mx_unskew) reads diagonal runs and writes rows: its Ex
 // tile trades row width for run length the other way, 16 x 8 x 8, or
 // 8 x 8 x 16 for one right-hand side (measured 10% faster there, slower at
@@ -584,10 +594,11 @@ template <int R> struct MxCopyTile<0, R> : MxTile3<(R == 1 ? 8 : 16), 8, (R == 1
 // store phase writes whole records in runs: h = iwmu/aii with aii =
 // c iwmu - sigma, c the sum of the cell's own-axis coefficients (the fvm
 // factor scales aij and aii alike and cancels).
-template <int C, bool FWD, int R>
+template <int C, bool FWD, int R, int V>
 __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
 {
-  using TL = MxPreTile<C, FWD>;
+  using TL = MxPreTileV<C, FWD, V>;
+  static_assert(TL::TQ * TL::TS * TL::TF == 1024 && (C != 0 || 256 % (TL::TQ * TL::TF) == 0), "pre-pass tile");
   constexpr int TQ = TL::TQ, TS = TL::TS, TF = TL::TF, NC = TQ * TS * TF;
   __shared__ c2 tile[R][NC];
   __shared__ c2 csg[NC];    // (c, sigma) of each slot

@@ -258,6 +258,15 @@ int csem_mixed_tuning(csem_model *m, int R, int *rows, int *planes);
  * candidates: output rows per block 4 or 6 (R <= 3), 2 or 4 (R = 4);
  * planes per block 8, 10, 12 or 16. */
 int csem_mixed_set_tile(csem_model *m, int R, int rows, int planes);
+/* The second part of the autotuner: the tile each of the six FP32 LU-SGS
+ * pre-passes uses at batch size R (candidate 0, 1 or 2, in launch order Ex,
+ * Ey, Ez forward, Ez, Ey, Ex backward) and the columns per block of the
+ * FFT column pass; -1 before the first solve at that R.  A solve times the
+ * candidates once per model and R; csem_mixed_set_lu_tiles fixes them
+ * instead (cols a power of two with nfft * cols in [1024, 4096]).  No
+ * choice changes a result. */
+int csem_mixed_tuning_lu(csem_model *m, int R, int *pre, int *cols);
+int csem_mixed_set_lu_tiles(csem_model *m, int R, const int *pre, int cols);
 /* device bytes held by the mixed-precision state of a model (its FP32
  * copy and every solve slot), and the bytes one more slot for R
  * right-hand sides takes: what a scheduler admits solves against */

@@ -157,6 +157,7 @@ def test_apply32_every_tile(workdir):
             for r, f in enumerate(fq):
                 orc.set_freq(f)
                 ref.append(orc.apply(x[r].astype(np.complex128)))
+            first = None
             for rows in ((4, 6) if R <= 3 else (2, 4)):
                 for planes in (8, 10, 12, 16):
                     _lib.check(L.csem_mixed_set_tile(ses.model, R, rows, planes), "set_tile")
@@ -165,6 +166,10 @@ def test_apply32_every_tile(workdir):
                     for r in range(R):
                         err = np.linalg.norm(y[r] - ref[r]) / np.linalg.norm(ref[r])
                         assert err < 2e-6, (R, rows, planes, r, err)
+                    # the tile changes no arithmetic: every tile gives the same bits
+                    if first is None:
+                        first = y
+                    assert np.array_equal(y, first), (R, rows, planes)
             L.csem_free(dx)
             L.csem_free(dy)
     finally:
@@ -267,6 +272,55 @@ def test_fp32_kernels_same_alone_and_batched(workdir, shape):
                 L.csem_free(dy)
                 for r in range(R):
                     assert np.array_equal(y[r], alone[r]), (name, R, r, float(np.abs(y[r] - alone[r]).max()))
+    finally:
+        ses.close()
+
+
+@pytest.mark.parametrize("shape", [dict(), dict(nx=130, ny=75, nz=70)], ids=["tiny", "multi-wave"])
+def test_lu_tiles_and_fft_columns_change_nothing(workdir, shape):
+    """Every LU-SGS pre-pass tile and every FFT column-pass width the
+    autotuner may pick gives the same FP32 LU-SGS and operator, bit for bit,
+    at one and three right-hand sides; a solve leaves its picks readable."""
+    print("case lu tiles / fft columns %s" % shape, flush=True)
+    s = surveys.tiny(4, airbc=1, **shape)
+    s.flags.update(n1fft=64, n2fft=64)
+    s.freqs = [0.375, 0.75, 1.25]
+    surveys.write_files(workdir, s)
+    ses = Session(s.argv(workdir))
+    L = _lib.lib()
+    try:
+        n = ses.n
+        rng = np.random.default_rng(23)
+        for R in (1, 3):
+            x = (rng.standard_normal((R, n)) + 1j * rng.standard_normal((R, n))).astype(np.complex64)
+            fr = (_lib.FreqT * R)(*[freq_consts(f) for f in s.freqs[:R]])
+            dx, dy = _dev(x), _dev(np.zeros_like(x))
+            base = {}
+            try:
+                for v in range(3):
+                    for cols in (16, 32, 64):  # nfft 64: 64 * cols in [1024, 4096]
+                        pre = (C.c_int * 6)(*([v] * 6))
+                        _lib.check(L.csem_mixed_set_lu_tiles(ses.model, R, pre, cols), "set_lu_tiles")
+                        for name, fn in (("lusgs32", L.csem_lusgs32), ("apply32", L.csem_apply32)):
+                            _lib.check(fn(ses.model, R, fr, dx, dy, None), name)
+                            y = _host(dy, x)
+                            if name not in base:
+                                base[name] = y
+                            assert np.array_equal(y, base[name]), (R, v, cols, name)
+                bad = (C.c_int * 6)(*([3] * 6))
+                assert L.csem_mixed_set_lu_tiles(ses.model, R, bad, 32) < 0
+                assert L.csem_mixed_set_lu_tiles(ses.model, R, pre, 8) < 0
+            finally:
+                L.csem_free(dx)
+                L.csem_free(dy)
+        # a solve at a batch size not fixed above times the candidates
+        ses.set_freq(s.freqs[0])
+        b = ses.inject()
+        ses.solve_mixed([b, b], s.freqs[:2], [1e-3, 1e-3])
+        pre, cols = (C.c_int * 6)(), C.c_int()
+        assert L.csem_mixed_tuning_lu(ses.model, 2, pre, C.byref(cols)) == 0
+        print("tuned pre-pass tiles %s, FFT columns per block %d" % (list(pre), cols.value), flush=True)
+        assert all(0 <= v < 3 for v in pre) and cols.value in (16, 32, 64)
     finally:
         ses.close()
 
