@@ -115,6 +115,8 @@ DEV_SYMBOLS = [
     "csem_vec_sub", "csem_vec_divr", "csem_vec_divz", "csem_vec_subscaled_dev", "csem_vec_addscaled",
     "csem_op_apply", "csem_op_lusgs", "csem_op_aphi", "csem_op_gmres", "csem_op_host",
     "csem_gmres_prec_release", "csem_bicgstab_op", "csem_gmres_op",
+    "csem_csr_create", "csem_csr_destroy", "csem_csr_nnz", "csem_csr_rows", "csem_csr_download", "csem_op_csr",
+    "csem_op_csr_sgs",
     "csem_mixed_solve", "csem_apply32", "csem_lusgs32", "csem_mixed_tuning", "csem_mixed_set_tile", "csem_mixed_tuning_lu", "csem_mixed_set_lu_tiles", "csem_mixed_bytes", "csem_mixed_slot_bytes", "csem_mem_info",
 ]
 
@@ -162,6 +164,13 @@ def lib():
     L.csem_mixed_tuning.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip)]
     L.csem_mixed_set_tile.argtypes = [vp, ip, ip, ip]
     L.csem_mixed_tuning_lu.argtypes = [vp, ip, vp, C.POINTER(ip)]
+    L.csem_csr_create.argtypes = [C.POINTER(vp), ip, ip, ip] + [vp] * 9 + [dp, dp]
+    L.csem_csr_destroy.argtypes = [vp]
+    L.csem_csr_nnz.restype = C.c_long
+    L.csem_csr_nnz.argtypes = [vp]
+    L.csem_csr_rows.restype = C.c_long
+    L.csem_csr_rows.argtypes = [vp]
+    L.csem_csr_download.argtypes = [vp, vp, vp, vp]
     L.csem_mixed_set_lu_tiles.argtypes = [vp, ip, vp, ip]
     L.csem_mixed_bytes.argtypes = [vp]
     L.csem_mixed_slot_bytes.argtypes = [vp, ip]
@@ -196,6 +205,10 @@ def lib():
     L.gmres_free.argtypes = []
     L.gmres_apply.argtypes = [ip, vp, vp]
     L.Acsr_init.argtypes = [C.POINTER(EmfT), ip]
+    L.Acsr_apply.argtypes = [ip, vp, vp]
+    L.Acsr_lusgs_apply.argtypes = [ip, vp, vp]
+    L.Acsr_free.argtypes = []
+    L.Acoo_init.argtypes = [C.POINTER(EmfT), ip]
     L.airbc_apply.argtypes = [C.POINTER(EmfT), vp, vp, vp]
     op_t = C.CFUNCTYPE(None, ip, vp, vp)
     L.fbicgstab.argtypes = [ip, vp, vp, vp, vp, ip, dp, ip]

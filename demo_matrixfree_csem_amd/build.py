@@ -25,7 +25,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-ffp-contr
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 C_FLAGS = ["-O3", "-fPIC", "-std=gnu11", "-Wall", "-Wno-unused-result", "-ffp-contract=off"]
 
-HIP_SOURCES = ["dev/csem_dev.hip", "dev/csem_mixed.hip"]
+HIP_SOURCES = ["dev/csem_dev.hip", "dev/csem_mixed.hip", "dev/csem_acsr.hip"]
 C_SOURCES = ["host/csem_util.c", "host/csem_model.c", "host/csem_entry.c"]
 
 

@@ -665,6 +665,8 @@ static struct {
   int on;
 } GP; /* gmres_init / gmres_free state (gmres_init_free.c:27) */
 
+static csem_csr *GA; /* Acsr_init / Acsr_free state (Acsr.c:19) */
+
 static int map_op(op_t fn, int n, opmap_t *o)
 {
   memset(o, 0, sizeof *o);
@@ -672,6 +674,11 @@ static int map_op(op_t fn, int n, opmap_t *o)
   if (fn == fdfd_apply) o->op = csem_op_apply;
   else if (fn == lusgs_apply) o->op = csem_op_lusgs;
   else if (fn == aphi_apply) o->op = csem_op_aphi;
+  else if (fn == Acsr_apply || fn == Acsr_lusgs_apply) {
+    if (!GA) return csem_fail(-4, "Acsr_apply used before Acsr_init");
+    o->op = fn == Acsr_apply ? csem_op_csr : csem_op_csr_sgs;
+    o->ctx = GA;
+  }
   else if (fn == gmres_apply) {
     if (!GP.on) return csem_fail(-4, "gmres_apply used before gmres_init");
     o->op = csem_op_gmres;
@@ -888,20 +895,77 @@ void aphi_apply(int n, csem_complex *x, csem_complex *y)
   apply_staged("aphi_apply", csem_op_aphi, NULL, n, x, y, 1);
 }
 
-/* The matrix-storage variants of main.c's main2 / main3 (Acsr.c, Acoo.c):
- * not provided; each call reports an error. */
+/* The matrix-storage path of main.c's main2 (Acsr.c): the operator as a
+ * CSR matrix on the device (csem_csr_create), a brute-force cross-check of
+ * fdfd_apply / lusgs_apply.  Acsr_apply and Acsr_lusgs_apply are
+ * csem_op_csr / csem_op_csr_sgs on host copies; passed to the Krylov
+ * drivers they run on the device vectors.  The MUMPS direct solve of
+ * main3 (Acoo.c) is not provided: each call reports an error. */
+void Acsr_init(emf_t *emf, int ifreq)
+{
+  int rc;
+  if (!emf || !emf->freqs || ifreq < 0 || ifreq >= emf->nfreq || !emf->d1 || !emf->d3s || !emf->sigma11 ||
+      !emf->sigma22 || !emf->sigma33 || emf->n1 < 1 || emf->n2 < 1 || emf->n3 < 1) {
+    csem_set_status(-3);
+    err("Acsr_init: needs emf_init's frequencies and extend_model_init's grid and conductivities");
+    return;
+  }
+  /* Acsr.c:34-39: the frequency's i w mu0 and the vector lengths */
+  emf->I_omega_mu0 = I * 2. * CSEM_PI * emf->freqs[ifreq] * CSEM_MU0;
+  emf->lEx = emf->n1 * (emf->n2 + 1) * (emf->n3 + 1);
+  emf->lEy = (emf->n1 + 1) * emf->n2 * (emf->n3 + 1);
+  emf->lEz = (emf->n1 + 1) * (emf->n2 + 1) * emf->n3;
+  emf->lE = emf->lEx + emf->lEy + emf->lEz;
+  if (GA) csem_csr_destroy(GA);
+  GA = NULL;
+  rc = csem_csr_create(&GA, emf->n1, emf->n2, emf->n3, emf->d1, emf->d2, emf->d3, emf->d1s, emf->d2s, emf->d3s,
+                       &emf->sigma11[0][0][0], &emf->sigma22[0][0][0], &emf->sigma33[0][0][0],
+                       creal(emf->I_omega_mu0), cimag(emf->I_omega_mu0));
+  if (rc < 0) {
+    die_dev("Acsr_init", rc);
+    return;
+  }
+  if (emf->verb) printf("#nnz=%ld\n", csem_csr_nnz(GA));
+  csem_set_status(0);
+}
+
+void Acsr_free(void)
+{
+  csem_csr_destroy(GA);
+  GA = NULL;
+}
+
+static int acsr_ready(const char *who, int n)
+{
+  if (!GA) {
+    csem_set_status(-4);
+    err("%s called before Acsr_init", who);
+    return 0;
+  }
+  if (n != csem_csr_rows(GA)) {
+    csem_set_status(-3);
+    err("%s: n=%d but the matrix has %ld rows", who, n, csem_csr_rows(GA));
+    return 0;
+  }
+  return 1;
+}
+
+/* Acsr.c:704-716 */
+void Acsr_apply(int n, csem_complex *x, csem_complex *y)
+{
+  if (acsr_ready("Acsr_apply", n)) apply_staged("Acsr_apply", csem_op_csr, GA, n, x, y, 0);
+}
+
+/* Acsr.c:719-748 */
+void Acsr_lusgs_apply(int n, csem_complex *x, csem_complex *y)
+{
+  if (acsr_ready("Acsr_lusgs_apply", n)) apply_staged("Acsr_lusgs_apply", csem_op_csr_sgs, GA, n, x, y, 0);
+}
+
 static void not_provided(const char *who)
 {
   csem_set_status(-5);
-  err("%s: the matrix-storage path is not provided by this library (use fdfd_apply / lusgs_apply)", who);
-}
-void Acsr_init(emf_t *emf, int ifreq) { (void)emf; (void)ifreq; not_provided("Acsr_init"); }
-void Acsr_free(void) { not_provided("Acsr_free"); }
-void Acsr_apply(int n, csem_complex *x, csem_complex *y) { (void)n; (void)x; (void)y; not_provided("Acsr_apply"); }
-void Acsr_lusgs_apply(int n, csem_complex *x, csem_complex *y)
-{
-  (void)n; (void)x; (void)y;
-  not_provided("Acsr_lusgs_apply");
+  err("%s: the MUMPS direct solve (Acoo.c) is not provided by this library", who);
 }
 void Acoo_init(emf_t *emf, int ifreq) { (void)emf; (void)ifreq; not_provided("Acoo_init"); }
 void Acoo_free(void) { not_provided("Acoo_free"); }

@@ -125,6 +125,30 @@ int csem_fgmres(csem_model *m, const csem_freq *f, void *x, const void *b, int n
 int csem_gmres(csem_model *m, const csem_freq *f, void *x, const void *b, int nouter,
                double tol, int mrestart, double *hist, void *stream);
 
+/* ---- The operator as a CSR matrix (Acsr.c:22-748): a brute-force
+ * cross-check of the matrix-free kernels.  Built on the device from host
+ * grid arrays (the extended grid of extend_model_init: d1, d1s of n1 + 1
+ * values, d2, d2s of n2 + 1, d3, d3s of n3 + 1, the edge conductivities in
+ * the Ex / Ey / Ez layouts) at one frequency (i w mu0): iw mu0 times the
+ * curl-curl operator, boundary rows with truncated stencils, each row's
+ * entries in Acsr.c's order and its diagonal last.  Acsr.c sets each
+ * row's pointer before storing the row's diagonal, which puts every
+ * diagonal in the next row's range; here each row holds its own.
+ * csem_op_csr (y = A x) and csem_op_csr_sgs (y = M^-1 x, the symmetric
+ * Gauss-Seidel of Acsr.c:719-748 on the rows, in the serial order's
+ * arithmetic) take the matrix as ctx. */
+typedef struct csem_csr csem_csr;
+int csem_csr_create(csem_csr **A, int n1, int n2, int n3, const double *d1, const double *d2, const double *d3,
+                    const double *d1s, const double *d2s, const double *d3s, const double *s11,
+                    const double *s22, const double *s33, double iomu_re, double iomu_im);
+void csem_csr_destroy(csem_csr *A);
+long csem_csr_nnz(const csem_csr *A);
+long csem_csr_rows(const csem_csr *A);
+/* host arrays of rows + 1, nnz and nnz complex doubles (any may be NULL) */
+int csem_csr_download(const csem_csr *A, int *row_ptr, int *col, void *val);
+int csem_op_csr(void *ctx, csem_model *m, const csem_freq *f, const void *x, void *y, void *stream);
+int csem_op_csr_sgs(void *ctx, csem_model *m, const csem_freq *f, const void *x, void *y, void *stream);
+
 /* ---- Operators as arguments.  The reference's Krylov drivers are generic
  * over op_t (solver.h:4): any y = Op x.  Here an operator acts on device
  * vectors (interleaved complex doubles) as op(ctx, model, freq, x, y,

@@ -1650,3 +1650,156 @@ orc_c128 orc_extract(orc_t *m, int ch, float rx, float ry, float rz, const zc *x
   }
   return s;
 }
+
+/* ------------------------------------------------------ matrix storage */
+
+/* Acsr_init (Acsr.c:22-693) on the full grid at the current frequency:
+ * iw mu0 times the curl-curl operator, every row (boundary rows with the
+ * stencil truncated to the grid), its entries in Acsr.c's order and its
+ * diagonal last.  Acsr.c writes row_ptr[r + 1] before the row's diagonal
+ * (Acsr.c:425-428, 553-556, 680-683); here the range of row r includes
+ * it, so row_ptr[r + 1] is one more than the reference's.  Returns nnz;
+ * with non-NULL arrays (rows + 1, nnz, nnz) fills them. */
+long orc_csr_build(const orc_t *m, int *row_ptr, int *col, orc_c128 *val)
+{
+  const int n1 = m->n1, n2 = m->n2, n3 = m->n3f;
+  const long lEx = (long)n1 * (n2 + 1) * (n3 + 1), lEy = (long)(n1 + 1) * n2 * (n3 + 1);
+  const double *d1 = m->d1, *d2 = m->d2, *d3 = m->d3, *d1s = m->d1s, *d2s = m->d2s, *d3s = m->d3s;
+  long nnz = 0, row = 0;
+  int i, j, k;
+  double t;
+  zc aii; /* complex, as Acsr.c's: its imaginary part is +0 before the diagonal's subtraction */
+#define CX(k, j, i) ((long)(i) + (long)n1 * ((j) + (long)(n2 + 1) * (k)))
+#define CY(k, j, i) (lEx + (long)(i) + (long)(n1 + 1) * ((j) + (long)n2 * (k)))
+#define CZ(k, j, i) (lEx + lEy + (long)(i) + (long)(n1 + 1) * ((j) + (long)(n2 + 1) * (k)))
+#define PUT(c, v) do { if (col) { col[nnz] = (int)(c); val[nnz] = (v); } nnz++; } while (0)
+#define DIAG(s) do { PUT(row, aii - m->iomu * (s)); row++; if (row_ptr) row_ptr[row] = (int)nnz; } while (0)
+  if (row_ptr) row_ptr[0] = 0;
+  for (k = 0; k <= n3; k++)
+    for (j = 0; j <= n2; j++)
+      for (i = 0; i < n1; i++) { /* Ex rows, Acsr.c:305-428 */
+        aii = 0;
+        t = 1. / (d2[j] * d1s[i]);
+        if (i + 1 <= n1 && j < n2) PUT(CY(k, j, i + 1), t);
+        if (j < n2) PUT(CY(k, j, i), -t);
+        t = 1. / (d2[j] * d2s[j]);
+        if (j + 1 <= n2) PUT(CX(k, j + 1, i), -t);
+        aii += t;
+        t = 1. / (d2[j] * d1s[i]);
+        if (i + 1 <= n1 && j - 1 >= 0) PUT(CY(k, j - 1, i + 1), -t);
+        if (j - 1 >= 0) PUT(CY(k, j - 1, i), t);
+        t = 1. / (d2[j] * d2s[MAXI(j - 1, 0)]);
+        if (j - 1 >= 0) PUT(CX(k, j - 1, i), -t);
+        aii += t;
+        t = 1. / (d3[k] * d3s[k]);
+        if (k + 1 <= n3) PUT(CX(k + 1, j, i), -t);
+        aii += t;
+        t = 1. / (d3[k] * d1s[i]);
+        if (i + 1 <= n1 && k < n3) PUT(CZ(k, j, i + 1), t);
+        if (k < n3) PUT(CZ(k, j, i), -t);
+        t = 1. / (d3[k] * d3s[MAXI(k - 1, 0)]);
+        aii += t;
+        if (k - 1 >= 0) PUT(CX(k - 1, j, i), -t);
+        t = 1. / (d3[k] * d1s[i]);
+        if (i + 1 <= n1 && k - 1 >= 0) PUT(CZ(k - 1, j, i + 1), -t);
+        if (k - 1 >= 0) PUT(CZ(k - 1, j, i), t);
+        DIAG(m->s11[CX(k, j, i)]);
+      }
+  for (k = 0; k <= n3; k++)
+    for (j = 0; j < n2; j++)
+      for (i = 0; i <= n1; i++) { /* Ey rows, Acsr.c:436-556 */
+        aii = 0;
+        t = 1. / (d3[k] * d2s[j]);
+        if (j + 1 <= n2 && k < n3) PUT(CZ(k, j + 1, i), t);
+        if (k < n3) PUT(CZ(k, j, i), -t);
+        t = 1. / (d3[k] * d3s[k]);
+        if (k + 1 <= n3) PUT(CY(k + 1, j, i), -t);
+        aii += t;
+        t = 1. / (d3[k] * d2s[j]);
+        if (j + 1 <= n2 && k - 1 >= 0) PUT(CZ(k - 1, j + 1, i), -t);
+        if (k - 1 >= 0) PUT(CZ(k - 1, j, i), t);
+        t = 1. / (d3[k] * d3s[MAXI(k - 1, 0)]);
+        aii += t;
+        if (k - 1 >= 0) PUT(CY(k - 1, j, i), -t);
+        t = 1. / (d1[i] * d1s[i]);
+        if (i + 1 <= n1) PUT(CY(k, j, i + 1), -t);
+        aii += t;
+        t = 1. / (d1[i] * d2s[j]);
+        if (i < n1 && j + 1 <= n2) PUT(CX(k, j + 1, i), t);
+        if (i < n1) PUT(CX(k, j, i), -t);
+        t = 1. / (d1[i] * d1s[MAXI(i - 1, 0)]);
+        aii += t;
+        if (i - 1 >= 0) PUT(CY(k, j, i - 1), -t);
+        t = 1. / (d1[i] * d2s[j]);
+        if (i - 1 >= 0 && j + 1 <= n2) PUT(CX(k, j + 1, i - 1), -t);
+        if (i - 1 >= 0) PUT(CX(k, j, i - 1), t);
+        DIAG(m->s22[CY(k, j, i) - lEx]);
+      }
+  for (k = 0; k < n3; k++)
+    for (j = 0; j <= n2; j++)
+      for (i = 0; i <= n1; i++) { /* Ez rows, Acsr.c:562-683 */
+        aii = 0;
+        t = 1. / (d1[i] * d3s[k]);
+        if (i < n1 && k + 1 <= n3) PUT(CX(k + 1, j, i), t);
+        if (i < n1) PUT(CX(k, j, i), -t);
+        t = 1. / (d1[i] * d1s[i]);
+        if (i + 1 <= n1) PUT(CZ(k, j, i + 1), -t);
+        aii += t;
+        t = 1. / (d1[i] * d3s[k]);
+        if (i - 1 >= 0 && k + 1 <= n3) PUT(CX(k + 1, j, i - 1), -t);
+        if (i - 1 >= 0) PUT(CX(k, j, i - 1), t);
+        t = 1. / (d1[i] * d1s[MAXI(i - 1, 0)]);
+        aii += t;
+        if (i - 1 >= 0) PUT(CZ(k, j, i - 1), -t);
+        t = 1. / (d2[j] * d2s[j]);
+        if (j + 1 <= n2) PUT(CZ(k, j + 1, i), -t);
+        aii += t;
+        t = 1. / (d2[j] * d3s[k]);
+        if (j < n2 && k + 1 <= n3) PUT(CY(k + 1, j, i), t);
+        if (j < n2) PUT(CY(k, j, i), -t);
+        t = 1. / (d2[j] * d2s[MAXI(j - 1, 0)]);
+        aii += t;
+        if (j - 1 >= 0) PUT(CZ(k, j - 1, i), -t);
+        t = 1. / (d2[j] * d3s[k]);
+        if (j - 1 >= 0 && k + 1 <= n3) PUT(CY(k + 1, j - 1, i), -t);
+        if (j - 1 >= 0) PUT(CY(k, j - 1, i), t);
+        DIAG(m->s33[CZ(k, j, i) - lEx - lEy]);
+      }
+#undef CX
+#undef CY
+#undef CZ
+#undef PUT
+#undef DIAG
+  return nnz;
+}
+
+/* Acsr_apply (Acsr.c:704-716), each row over its own range */
+void orc_csr_apply(long nrow, const int *row_ptr, const int *col, const orc_c128 *val, const orc_c128 *x,
+                   orc_c128 *y)
+{
+  long r;
+  int e;
+  for (r = 0; r < nrow; r++) {
+    y[r] = 0;
+    for (e = row_ptr[r]; e < row_ptr[r + 1]; e++) y[r] += val[e] * x[col[e]];
+  }
+}
+
+/* Acsr_lusgs_apply (Acsr.c:719-748): y = 0, a forward Gauss-Seidel sweep
+ * over the rows, then a backward one */
+void orc_csr_sgs(long nrow, const int *row_ptr, const int *col, const orc_c128 *val, const orc_c128 *x,
+                 orc_c128 *y)
+{
+  long r;
+  int e, pass;
+  memset(y, 0, nrow * sizeof(zc));
+  for (pass = 0; pass < 2; pass++)
+    for (r = pass ? nrow - 1 : 0; pass ? r >= 0 : r < nrow; r += pass ? -1 : 1) {
+      zc s = 0, aii = 1;
+      for (e = row_ptr[r]; e < row_ptr[r + 1]; e++) {
+        if (col[e] == r) aii = val[e];
+        else s += val[e] * y[col[e]];
+      }
+      y[r] = (x[r] - s) / aii;
+    }
+}

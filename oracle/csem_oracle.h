@@ -90,6 +90,16 @@ void orc_inject(orc_t *m, int ch, float sx, float sy, float sz, orc_c128 *b);
  * float32 store of inject_extract.c:159,182,208 */
 orc_c128 orc_extract(orc_t *m, int ch, float rx, float ry, float rz, const orc_c128 *x);
 
+/* Matrix storage (Acsr.c): the CSR matrix on the full grid at the current
+ * frequency (returns nnz; fills row_ptr[rows + 1], col[nnz], val[nnz]
+ * when non-NULL; each row's range holds its own diagonal, last), the
+ * product and the symmetric Gauss-Seidel of Acsr.c:704-748. */
+long orc_csr_build(const orc_t *m, int *row_ptr, int *col, orc_c128 *val);
+void orc_csr_apply(long nrow, const int *row_ptr, const int *col, const orc_c128 *val, const orc_c128 *x,
+                   orc_c128 *y);
+void orc_csr_sgs(long nrow, const int *row_ptr, const int *col, const orc_c128 *val, const orc_c128 *x,
+                 orc_c128 *y);
+
 double orc_create_nugrid(int n, double len, double dx, double *x, int istretch);
 int orc_find_index(int n, const double *x, double val);
 #endif

@@ -67,6 +67,10 @@ def lib():
         L.orc_create_nugrid.argtypes = [ip, dp, dp, vp, ip]
         L.orc_find_index.restype = ip
         L.orc_find_index.argtypes = [ip, vp, dp]
+        L.orc_csr_build.restype = C.c_long
+        L.orc_csr_build.argtypes = [vp, vp, vp, vp]
+        L.orc_csr_apply.argtypes = [C.c_long, vp, vp, vp, vp, vp]
+        L.orc_csr_sgs.argtypes = [C.c_long, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -235,6 +239,30 @@ class Oracle:
         hist = np.zeros(nouter * m + 2)
         it = lib().orc_gmres(self.h, _p(x), _p(b), nouter, tol, m, _p(hist))
         return x, it, hist[:it + 1]
+
+    def csr(self):
+        """Acsr_init's matrix on the full grid at the current frequency:
+        (row_ptr, col, val), each row's range holding its own diagonal."""
+        L = lib()
+        nnz = L.orc_csr_build(self.h, None, None, None)
+        rows = int(self.fE)
+        rp = np.zeros(rows + 1, np.int32)
+        col = np.zeros(nnz, np.int32)
+        val = np.zeros(nnz, np.complex128)
+        L.orc_csr_build(self.h, _p(rp), _p(col), _p(val))
+        return rp, col, val
+
+    @staticmethod
+    def csr_apply(rp, col, val, x):
+        y = np.zeros(len(rp) - 1, np.complex128)
+        lib().orc_csr_apply(len(rp) - 1, _p(rp), _p(col), _p(val), _p(np.ascontiguousarray(x, np.complex128)), _p(y))
+        return y
+
+    @staticmethod
+    def csr_sgs(rp, col, val, x):
+        y = np.zeros(len(rp) - 1, np.complex128)
+        lib().orc_csr_sgs(len(rp) - 1, _p(rp), _p(col), _p(val), _p(np.ascontiguousarray(x, np.complex128)), _p(y))
+        return y
 
 
 def run_ref(workdir, flags, mode, hfreq=None, hin=None, hout="ref", timeout=600):

@@ -4,7 +4,7 @@
  *
  * Usage: ref_harness <reference key=value flags> hmode=<mode> [hfreq=f]
  *        [hin=file] [hout=prefix]
- * Modes: grid, apply, lusgs, aphi, gmresprec, inject, extract, solve.  Binary outputs are
+ * Modes: grid, apply, lusgs, aphi, gmresprec, inject, extract, solve, acsr.  Binary outputs are
  * raw little-endian doubles / complex128 / complex64 named <prefix>_<what>.
  * airbc=1 is accepted for lusgs/aphi/inject/extract (the index shift only);
  * apply and solve need airbc=0 because airbc_apply lives in the file this
@@ -29,6 +29,17 @@ void gmres_apply(int n, complex *b, complex *x);
 void inject_source(acq_t *acq, emf_t *emf, complex *b);
 void extract_emf(acq_t *acq, emf_t *emf, complex *x, int ifreq);
 void write_data(acq_t *acq, emf_t *emf, char *fname, float _Complex ***dcal_fd);
+void Acsr_init(emf_t *emf, int ifreq);
+void Acsr_free();
+void Acsr_apply(int n, complex *x, complex *y);
+void Acsr_lusgs_apply(int n, complex *x, complex *y);
+/* Acsr.c:12-19: the matrix it assembles (a global of that file) */
+typedef struct {
+  int nrow, ncol, nnz;
+  int *row_ptr, *col_ind;
+  complex *val;
+} ref_ccsr_t;
+extern ref_ccsr_t *Acsr;
 void bicgstab(int n, complex *x, complex *b, op_t Aop, int niter, double tol, int verb);
 void gmres(int n, complex *x, complex *b, op_t Aop, int niter, double tol, int m, int verb);
 void fbicgstab(int n, complex *x, complex *b, op_t Aop, op_t invMop, int niter, double tol, int verb);
@@ -199,6 +210,17 @@ int main(int argc, char **argv)
       extract_emf(acq, emf, emf->xe, ifreq);
     }
     write_data(acq, emf, "emf_0001.txt", emf->dcal_fd);
+  } else if (!strcmp(mode, "acsr")) { /* main.c:183-196 at freqs[0] */
+    if (airbc) err("acsr needs airbc=0");
+    Acsr_init(emf, 0);
+    dump("row_ptr", Acsr->row_ptr, (Acsr->nrow + 1) * sizeof(int));
+    dump("col", Acsr->col_ind, Acsr->nnz * sizeof(int));
+    dump("val", Acsr->val, Acsr->nnz * sizeof(complex));
+    Acsr_apply(emf->ne, x, y);
+    dump("y", y, emf->ne * sizeof(complex));
+    Acsr_lusgs_apply(emf->ne, x, y);
+    dump("ysgs", y, emf->ne * sizeof(complex));
+    Acsr_free();
   } else {
     err("unknown hmode=%s", mode);
   }

@@ -12,9 +12,11 @@ Two sources, both data only (inputs and expected outputs):
    its sources by `make -C oracle ref`, on small seeded models from
    demo_matrixfree_csem_amd.surveys.tiny (grid arrays, edge conductivities,
    fdfd_apply, lusgs_apply, inject_source, extract_emf and full solves).
+3. ref_acsr.npz -- the same build's matrix-storage path (Acsr.c): the
+   assembled CSR arrays and Acsr_apply / Acsr_lusgs_apply of a vector.
 
 Run in a container that holds /root/reference:
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [run_layered] [ref_tiny] [ref_acsr]
 """
 import os
 import sys
@@ -118,10 +120,42 @@ def ref_tiny():
     np.savez_compressed(os.path.join(HERE, "ref_tiny.npz"), **out)
 
 
+ACSR_CASES = [  # name, survey kwargs, flag overrides, frequency (airbc=0: Acsr.c assembles the explicit-air grid)
+    ("c0", dict(seed=8, nx=5, ny=4, nz=3, airbc=0), dict(nb=2, rho_air=3.0), 0.75),
+    ("c1", dict(seed=9, nx=4, ny=5, nz=4, airbc=0), dict(nb=2, rho_air=10.0), 1.25),
+]
+
+
+def ref_acsr():
+    """ref_acsr.npz: the reference's Acsr_init arrays (row_ptr, col_ind,
+    val as Acsr.c builds them) and its Acsr_apply / Acsr_lusgs_apply of a
+    seeded vector, on small explicit-air models."""
+    out = {}
+    for name, kw, fl, f in ACSR_CASES:
+        s = surveys.tiny(**kw)
+        s.flags.update(fl)
+        s.freqs = [f]
+        d = tempfile.mkdtemp()
+        surveys.write_files(d, s)
+        argv = s.argv(d)
+        O.run_ref(d, argv, "grid")
+        lE = int(np.fromfile(os.path.join(d, "ref_info"), np.float64)[6])
+        x = _rand(lE, 200)
+        x.tofile(os.path.join(d, "xin"))
+        O.run_ref(d, argv, "acsr", hin=os.path.join(d, "xin"))
+        out[name + "_x"] = x
+        out[name + "_row_ptr"] = np.fromfile(os.path.join(d, "ref_row_ptr"), np.int32)
+        out[name + "_col"] = np.fromfile(os.path.join(d, "ref_col"), np.int32)
+        out[name + "_val"] = np.fromfile(os.path.join(d, "ref_val"), np.complex128)
+        out[name + "_apply"] = np.fromfile(os.path.join(d, "ref_y"), np.complex128)
+        out[name + "_sgs"] = np.fromfile(os.path.join(d, "ref_ysgs"), np.complex128)
+    np.savez_compressed(os.path.join(HERE, "ref_acsr.npz"), **out)
+
+
 if __name__ == "__main__":
     if not os.path.isdir(REF):
         sys.exit("reference not found at %s" % REF)
-    run_layered()
-    ref_tiny()
-    for f in ("run_layered.npz", "ref_tiny.npz"):
-        print(f, os.path.getsize(os.path.join(HERE, f)))
+    which = sys.argv[1:] or ["run_layered", "ref_tiny", "ref_acsr"]
+    for w in which:
+        globals()[w]()
+        print(w + ".npz", os.path.getsize(os.path.join(HERE, w + ".npz")))

@@ -152,11 +152,18 @@ def test_reference_main_links_against_the_library(workdir):
 
 
 def test_matrix_storage_variants_report_errors():
-    """Acsr_* / Acoo_* (main.c:148-151, 221-223) exist for the linker and
-    report an error instead of computing."""
+    """Acsr_init without a grid, Acsr_apply before Acsr_init and the MUMPS
+    path Acoo_* (main.c:148-151, 221-223; not provided) report an error
+    instead of computing."""
     L = _lib.lib()
     L.csem_set_exit_on_error(0)
     e = _lib.EmfT()
     L.Acsr_init(C.byref(e), 0)
+    assert L.csem_last_status() < 0
+    assert b"extend_model_init" in L.csem_last_error()
+    x = np.zeros(4, np.complex128)
+    L.Acsr_apply(4, x.ctypes.data, x.ctypes.data)
+    assert L.csem_last_status() < 0 and b"before Acsr_init" in L.csem_last_error()
+    L.Acoo_init(C.byref(e), 0)
     assert L.csem_last_status() < 0
     assert b"not provided" in L.csem_last_error()
