@@ -154,3 +154,63 @@ def test_vti_refinement_guard_without_the_stall_cut(workdir):
         assert abs(res - rel) <= 1e-3 * rel, (res, rel)
     finally:
         ses.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stall", [0, 16], ids=["default-cut", "forced-cuts"])
+def test_vti_second_model_stall_handling(workdir, stall):
+    """The stagnation handling on an anisotropic model the threshold was
+    not tuned on (seed 5): with the default cut, and with a cut after 16
+    iterations without progress (which makes the refinement go through its
+    candidates -- the cut solve's last iterate, its best iterate, then
+    FGMRES cycles), every right-hand side either reaches tol with receivers
+    within 1e-6 of FP64 F-BiCGStab's, or stops with a finite solution of
+    relative residual at most 1 that the FP64 continuation finishes to the
+    same receivers."""
+    import ctypes as C
+    import threading
+    from demo_matrixfree_csem_amd import _lib, serve
+    from demo_matrixfree_csem_amd.session import Session
+    s = surveys.vti_marine(5, 36, 24, 24, 33)
+    surveys.write_files(workdir, s)
+    freqs = [0.25, 0.75]
+    print("case vti_marine seed 5, stall_iters %d" % stall, flush=True)
+    ses = Session(s.argv(workdir))
+    L = _lib.lib()
+    try:
+        bs, tols = [], []
+        for f in freqs:
+            ses.set_freq(f)
+            bs.append(ses.inject())
+            tols.append(_tol(f))
+        xs, its, rel, nconv = ses.solve_mixed(bs, freqs, tols, _lib.MixedOpts.default(max_inner=4000,
+                                                                                     stall_iters=stall))
+        print("mixed: iterations %s, relres %s, converged %d" % (list(its), list(rel), nconv), flush=True)
+        for r, f in enumerate(freqs):
+            assert np.all(np.isfinite(xs[r])) and rel[r] <= 1.0
+            x = xs[r]
+            if not rel[r] < tols[r]:
+                dx, db = C.c_void_p(), C.c_void_p()
+                for d, a in ((dx, x), (db, bs[r])):
+                    a = np.ascontiguousarray(a, np.complex128)
+                    _lib.check(L.csem_malloc(C.byref(d), a.nbytes), "malloc")
+                    _lib.check(L.csem_memcpy_h2d(d, a.ctypes.data, a.nbytes, None), "h2d")
+                try:
+                    itc, relc = serve.fp64_continue(ses.model, f, dx, db, tols[r], rel[r], 6000, threading.Lock())
+                    x = np.zeros(ses.n, np.complex128)
+                    _lib.check(L.csem_memcpy_d2h(x.ctypes.data, dx, x.nbytes, None), "d2h")
+                finally:
+                    L.csem_free(dx)
+                    L.csem_free(db)
+                print("f=%g continued in FP64: %d iterations, relres %.2e" % (f, itc, relc), flush=True)
+                assert relc < tols[r]
+            ses.set_freq(f)
+            res = np.linalg.norm(bs[r] - ses.fdfd_apply(x)) / np.linalg.norm(bs[r])
+            assert res < 2 * tols[r], (f, res)
+            xb, itb = ses.solve(bs[r], tols[r], niter=6000, algopt=1)
+            assert itb < 6000
+            e = _rec_err(ses.extract(x).astype(np.complex128), ses.extract(xb).astype(np.complex128))
+            print("f=%g: FP64 F-BiCGStab %d iterations; receiver error %.2e" % (f, itb, e), flush=True)
+            assert e < 1e-6, (f, e)
+    finally:
+        ses.close()
