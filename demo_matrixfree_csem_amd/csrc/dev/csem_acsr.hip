@@ -228,6 +228,12 @@ __global__ void k_csr_gs_level(CsrArgs a, const z2 *x, z2 *y, int c, int L, int 
   y[r] = zdiv(zsub(x[r], s), aii);
 }
 
+__global__ void k_csr_zero(long n, z2 *y)
+{
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e < n) y[e] = zmk(0.0, 0.0);
+}
+
 template <class T>
 static int csr_alloc(csem_csr *A, T **p, long count)
 {
@@ -266,6 +272,13 @@ extern "C" int csem_csr_create(csem_csr **out, int n1, int n2, int n3, const dou
   void *tmp = NULL;
   size_t tmp_bytes = 0;
   const long n = A->lE;
+  // the build runs on a stream of its own and waits for that stream only:
+  // a device-wide wait would invalidate another thread's graph capture
+  hipStream_t bs = NULL;
+  if (hipStreamCreateWithFlags(&bs, hipStreamNonBlocking) != hipSuccess) {
+    delete A;
+    return csem_fail(-1, "stream");
+  }
   const unsigned nb = (unsigned)((n + 1 + 255) / 256);
   struct { double **dst; const double *src; long cnt; } up[] = {
       {&A->d1, d1, n1 + 1}, {&A->d2, d2, n2 + 1}, {&A->d3, d3, n3 + 1}, {&A->d1s, d1s, n1 + 1},
@@ -278,33 +291,37 @@ extern "C" int csem_csr_create(csem_csr **out, int n1, int n2, int n3, const dou
   if ((rc = csr_alloc(A, &A->row_ptr, n + 1)) || (rc = csr_alloc(A, &cnt, n + 1))) goto fail;
   {
     CsrArgs a = csr_args(A);
-    k_csr_count<<<nb, 256>>>(a, cnt);
-    if (hipcub::DeviceScan::ExclusiveSum(NULL, tmp_bytes, cnt, A->row_ptr, (int)(n + 1)) != hipSuccess) {
+    k_csr_count<<<nb, 256, 0, bs>>>(a, cnt);
+    if (hipcub::DeviceScan::ExclusiveSum(NULL, tmp_bytes, cnt, A->row_ptr, (int)(n + 1), bs) != hipSuccess) {
       rc = csem_fail(-1, "scan size");
       goto fail;
     }
     if ((rc = csr_alloc(A, (char **)&tmp, (long)tmp_bytes))) goto fail;
-    if (hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, A->row_ptr, (int)(n + 1)) != hipSuccess) {
+    if (hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, A->row_ptr, (int)(n + 1), bs) != hipSuccess) {
       rc = csem_fail(-1, "scan");
       goto fail;
     }
     int total = 0;
-    if (hipMemcpy(&total, A->row_ptr + n, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpyAsync(&total, A->row_ptr + n, sizeof(int), hipMemcpyDeviceToHost, bs) != hipSuccess ||
+        hipStreamSynchronize(bs) != hipSuccess) {
       rc = csem_fail(-1, "row count");
       goto fail;
     }
     A->nnz = total;
     if ((rc = csr_alloc(A, &A->col, A->nnz)) || (rc = csr_alloc(A, &A->val, A->nnz))) goto fail;
     a = csr_args(A);
-    k_csr_fill<<<nb, 256>>>(a);
-    if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess) {
+    k_csr_fill<<<nb, 256, 0, bs>>>(a);
+    if (hipStreamSynchronize(bs) != hipSuccess || hipGetLastError() != hipSuccess) {
       rc = csem_fail(-1, "CSR assembly failed");
       goto fail;
     }
   }
+  (void)hipStreamDestroy(bs);
   *out = A;
   return 0;
 fail:
+  (void)hipStreamSynchronize(bs);
+  (void)hipStreamDestroy(bs);
   csem_csr_destroy(A);
   return rc;
 }
@@ -315,6 +332,7 @@ extern "C" long csem_csr_rows(const csem_csr *A) { return A ? A->lE : -1; }
 extern "C" int csem_csr_download(const csem_csr *A, int *row_ptr, int *col, void *val)
 {
   if (!A) return csem_fail(-3, "no matrix");
+  std::lock_guard<std::mutex> lk(g_devwide_mu);  // synchronous copies: not during another thread's capture
   if ((row_ptr && hipMemcpy(row_ptr, A->row_ptr, (A->lE + 1) * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) ||
       (col && hipMemcpy(col, A->col, A->nnz * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) ||
       (val && hipMemcpy(val, A->val, A->nnz * sizeof(z2), hipMemcpyDeviceToHost) != hipSuccess))
@@ -341,7 +359,8 @@ extern "C" int csem_op_csr_sgs(void *ctx, csem_model *, const csem_freq *, const
   if (!A) return csem_fail(-3, "no matrix");
   const hipStream_t s = (hipStream_t)stream;
   const CsrArgs a = csr_args(A);
-  HIP_OK(hipMemsetAsync(y, 0, A->lE * sizeof(z2), s));
+  // y = 0 by a fill kernel (a memset node is not safe in a caller's captured graph)
+  k_csr_zero<<<(unsigned)((A->lE + 255) / 256), 256, 0, s>>>(A->lE, (z2 *)y);
   const int n1 = A->n1, n2 = A->n2, n3 = A->n3;
   // per component: the extents of its coupled axes (u, v) and free axis
   const int NU[3] = {n2, n1, n1}, NV[3] = {n3, n3, n2}, NW[3] = {n1, n2, n3};
