@@ -500,14 +500,17 @@ static int model_create(const csem_model_desc *d, const csem_model_raw *raw, cse
     g.A[0] = n3f - 1; g.B[0] = n2 - 1; g.ns[0] = n1;
     g.A[1] = n3f - 1; g.B[1] = n1 - 1; g.ns[1] = n2;
     g.A[2] = n2 - 1;  g.B[2] = n1 - 1; g.ns[2] = n3f;
-    long off = 0, mx = 0;
+    long off = 0, mx = 0, ncell = 0;
     for (int c = 0; c < 3; c++) {
-      const long nc = (long)g.ns[c] * g.A[c] * g.B[c];
+      g.AP[c] = 64 * ((g.A[c] + 63) / 64);
+      g.SL[c] = (long)(g.A[c] + g.B[c] - 1) * g.AP[c];
+      const long nc = (long)g.ns[c] * g.SL[c];
       g.base[c] = off;
       off += nc;
       mx = std::max(mx, nc);
+      ncell += (long)g.ns[c] * g.A[c] * g.B[c];
     }
-    m->nsk = off;
+    m->nsk = ncell;
     if ((rc = dalloc(m, &m->smith, off))) goto fail;
     if ((rc = dalloc(m, &m->rec, mx))) goto fail;
   }
@@ -686,6 +689,24 @@ static void (*sgs2_kernel(bool fvm, int nw))(LuArgs)
   }
 }
 
+// Dynamic LDS above 64 KB needs the kernel's limit raised first: once per
+// sweep instance, to the largest request seen.
+static int sg2_lds_attr(const void *k, size_t bytes)
+{
+  static std::mutex mu;
+  static std::vector<std::pair<const void *, size_t>> raised;
+  if (bytes <= 64 * 1024) return 0;
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto &p : raised)
+    if (p.first == k && p.second >= bytes) return 0;
+  if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+    return csem_fail(-1, "LU-SGS sweep: %zu bytes of LDS not available", bytes);
+  for (auto &p : raised)
+    if (p.first == k) { p.second = bytes; return 0; }
+  raised.push_back({k, bytes});
+  return 0;
+}
+
 static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipStream_t s)
 {
   LuArgs a;
@@ -718,15 +739,24 @@ static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   size_t lds[3];
   unsigned gs[3];
   for (int c = 0; c < 3; c++) {
-    tb[c] = ((g.A[c] + 63) / 64) * 64;
+    tb[c] = g.AP[c];  // one thread per row slot: the sweep's slots per diagonal
     lds[c] = (size_t)sg2_lds_bytes(g.A[c], g.B[c], tb[c] / 64);
     gs[c] = 8 * ((g.ns[c] + 7) / 8);
+    if (lds[c] > 160 * 1024)
+      return csem_fail(-3, "LU-SGS sweep of %d x %d cells needs %zu bytes of LDS", g.A[c], g.B[c], lds[c]);
   }
   double nc[3];
   for (int c = 0; c < 3; c++) nc[c] = (double)g.ns[c] * g.A[c] * g.B[c];
   // the fvm scaling is a template argument: a runtime branch in the sweep
   // loop would make the compiler drain the prefetched loads every step
 #define SWEEP(C, F) sgs2_kernel<C, F>(m->fvm != 0, tb[C] / 64)
+  {
+    int rc = 0;
+    if ((rc = sg2_lds_attr((const void *)SWEEP(0, true), lds[0])) || (rc = sg2_lds_attr((const void *)SWEEP(0, false), lds[0])) ||
+        (rc = sg2_lds_attr((const void *)SWEEP(1, true), lds[1])) || (rc = sg2_lds_attr((const void *)SWEEP(1, false), lds[1])) ||
+        (rc = sg2_lds_attr((const void *)SWEEP(2, true), lds[2])) || (rc = sg2_lds_attr((const void *)SWEEP(2, false), lds[2])))
+      return rc;
+  }
   // bytes per swept cell each kernel must move: the pre-pass reads b and
   // the other components' W values it differences and writes the record
   // (only b for Ex forward); the sweep reads the record and the diagonal

@@ -271,8 +271,10 @@ __global__ void k_hz0(ApplyArgs a, z2 *hz)
 //
 // Layout: a sweep's per-cell inputs (record, right-hand side, diagonal)
 // are stored diagonal-major ("skewed"): slice q, then diagonal d = s + f,
-// then row s, packed without holes, so the lanes of a wave (consecutive
-// rows of one diagonal) load one contiguous run.  The state W (forward
+// then row s, AP slots per diagonal (row s at slot s - 1), so the lanes of a
+// wave (consecutive rows of one diagonal) load one contiguous run and a
+// lane's next diagonal is a fixed stride away; the slots of rows not on a
+// diagonal are never written or read.  The state W (forward
 // values, then the backward values of Ey and Ez) and the result y keep the
 // reference layout; the pre-passes gather their neighbours from W.
 
@@ -282,7 +284,9 @@ __global__ void k_hz0(ApplyArgs a, z2 *hz)
 //   Ez: q = k (n3f slices), s = j (A = n2-1), f = i (B = n1-1)
 struct SkewGeo {
   int A[3], B[3], ns[3];
-  long base[3];  // component offsets in the skewed diagonal cache
+  long base[3];  // component offsets in the diagonal cache (the Smith part)
+  int AP[3];     // FP64 sweep slots per diagonal: A rounded up to a whole wave
+  long SL[3];    // FP64 sweep slots per slice: (A + B - 1) diagonals of AP
 };
 
 __host__ __device__ __forceinline__ int sk_tri(int m) { return m > 0 ? m * (m + 1) / 2 : 0; }
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(256) void k_lu_smith(LuArgs a)
   if (fabs(cc) < fabs(dd)) { ratio = cc / dd; denom = (cc * ratio) + dd; code = 2; }
   else { ratio = dd / cc; denom = (dd * ratio) + cc; code = 1; }
   if (denom < 0) code = -code;
-  ((double4 *)a.smith)[a.g.base[C] + (long)q * a.g.A[C] * a.g.B[C] + p] =
+  ((double4 *)a.smith)[a.g.base[C] + (long)q * a.g.SL[C] + (long)(s + f - 2) * a.g.AP[C] + (s - 1)] =
       make_double4(ratio, fabs(denom), fabs(1.0 / denom), code);
 }
 
@@ -409,8 +413,8 @@ __global__ __launch_bounds__(256) void k_lu_pre(LuArgs a)
   __shared__ z2 tile[NC * NU];
   __shared__ long pidx[NC];  // global record index of each slot, -1 off the slices
   const int n1 = a.n1, n2 = a.n2, n3 = a.n3f;
-  const int A = a.g.A[C], B = a.g.B[C], ns = a.g.ns[C];
-  const long AB = (long)A * B;
+  const int A = a.g.A[C], B = a.g.B[C], ns = a.g.ns[C], AP = a.g.AP[C];
+  const long SL = a.g.SL[C];
   const int t = threadIdx.x;
   int q0, s0, f0;
   if (C == 0) { q0 = blockIdx.x * TQ; f0 = 1 + blockIdx.y * TF; s0 = 1 + blockIdx.z * TS; }
@@ -428,10 +432,7 @@ __global__ __launch_bounds__(256) void k_lu_pre(LuArgs a)
       pidx[cell] = -1;
       continue;
     }
-    {
-      const int d = s + f;
-      pidx[cell] = q * AB + sk_off(d, A, B) + (s - max(1, d - B));
-    }
+    pidx[cell] = q * SL + (long)(s + f - 2) * AP + (s - 1);
     int k, j, i;
     lu_kji<C>(q, s, f, k, j, i);
     const int kop = k - a.kofs;
@@ -522,10 +523,19 @@ __global__ __launch_bounds__(256) void k_lu_pre(LuArgs a)
 //     slot is rewritten only SG2_RING steps later, while the barriers keep
 //     every wave within SG2_BP steps of the others);
 //   - each step's record and diagonal are loaded SG2_P steps ahead into
-//     registers by raw buffer loads (a lane whose cell is off the slice
-//     gets an offset past the buffer: the load reads 0 and issues no
-//     traffic, and its result is discarded); its fast-axis spacings SG2_PC
-//     steps ahead from the LDS table;
+//     registers by raw buffer loads at a fixed lane offset plus a fixed
+//     stride per step (a lane whose cell is off the slice gets an offset
+//     past the buffer: the load reads 0 and issues no traffic); its
+//     fast-axis spacings SG2_PC steps ahead from an LDS table padded with
+//     zeros, so that no index is clamped;
+//   - a lane off the slice computes with zeros: before its row starts every
+//     input it sees (record, diagonal, spacings or its own ones, the values
+//     of its row and of the neighbouring row, which has not started
+//     either) is zero and every operation of the update maps +0 operands to
+//     +0, so the row's boundary value stays the reference's +0 without a
+//     mask; after its row ends, or on a row past A in a forward sweep, it
+//     computes values no cell of the slice reads (in a backward sweep the
+//     rows past A, whose values row A reads, see only zeros);
 //   - the values do not go to memory from the computing waves: a wave's
 //     vector-memory counter retires loads and stores in issue order, so
 //     waiting for a prefetched record would also wait for every earlier
@@ -538,14 +548,24 @@ __global__ __launch_bounds__(256) void k_lu_pre(LuArgs a)
 #define SG2_PD 4
 #define SG2_L (SG2_BP + SG2_PD - 1)
 #define SG2_RING 32
-#define SG2_P 4
 #define SG2_PC 4
 #define SG2_U 32
 #define SG2_OOB ((int)0x80000000)  // a buffer offset past every buffer (all shorter than 2 GiB)
+#define SG2_SP 66  // staging slots per step: 64 rows, padded so that the store wave's reads of eight
+                   // steps of eight rows (Ey, Ez) meet no LDS bank conflict
 static_assert(SG2_RING > SG2_L - SG2_PD + SG2_BP, "edge ring slot rewritten before it is read");
-static_assert(SG2_U % SG2_RING == 0 && SG2_U % SG2_BP == 0 && SG2_U % SG2_P == 0 && SG2_U % SG2_PD == 0 &&
-                  SG2_U % SG2_PC == 0,
+static_assert(SG2_U % SG2_RING == 0 && SG2_U % SG2_BP == 0 && SG2_U % SG2_PD == 0 && SG2_U % SG2_PC == 0,
               "sweep unroll");
+// Record units a sweep loads per step (b and the Smith part always) and
+// how many steps ahead it loads them.  Ex forward sees Ey = Ez = 0, so its
+// record terms are +0 and only b is read; Ey forward sees Ez = 0, so its
+// first two terms (t1 and the known part of t2) are +0.  The fewer
+// registers a step's loads take, the deeper the prefetch.
+template <int C, bool FWD> struct Sg2Load {
+  static constexpr bool R01 = !FWD || C == 2, R23 = !FWD || C != 0;
+  static constexpr int P = (R01 && R23) ? 4 : 8;
+  static_assert(SG2_U % P == 0, "sweep unroll");
+};
 typedef unsigned sg2_v4u __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t sg2_rsrc(const void *p, long len)
@@ -577,14 +597,25 @@ __device__ __forceinline__ z2 sg2_zdpp_keep(z2 old, z2 v)
   return zmk(sg2_dpp_keep<CTRL>(old.x, v.x), sg2_dpp_keep<CTRL>(old.y, v.y));
 }
 
+// The fast-axis table: entry u holds column f = u - sg2_toff(nw) (zeros
+// off [1, B]); the offset and length cover every column a lane of any wave
+// reads from its first step to the last prefetch of the last loop
+// iteration, forward and backward.
+__host__ __device__ __forceinline__ int sg2_toff(int nw)
+{
+  return 64 * nw + (nw - 1) * SG2_L + SG2_U + SG2_PC + SG2_BP + 4;
+}
+__host__ __device__ __forceinline__ int sg2_tlen(int A, int B, int nw)
+{
+  return sg2_toff(nw) + A + B + (nw - 1) * SG2_L + SG2_BP + SG2_U + SG2_PC + 4;
+}
 // LDS of a block of nw computing waves: the edge rings and the junk slots
 // the other lanes' ring stores land in, the value staging (2 halves x nw
-// waves x SG2_BP steps x 64 lanes), the fast table (B + 2 double4 and
-// B + 2 doubles) and the diagonal table (A + B + 1 ints)
+// waves x SG2_BP steps x SG2_SP slots) and the fast table (double4 spacings
+// and a double cell length per entry)
 __host__ __device__ __forceinline__ long sg2_lds_bytes(int A, int B, int nw)
 {
-  return (long)nw * (2 * SG2_RING + 64) * 16 + 2L * nw * SG2_BP * 64 * 16 + (long)(B + 2) * 40 +
-         (long)(A + B + 1) * 4;
+  return (long)nw * (2 * SG2_RING + 64) * 16 + 2L * nw * SG2_BP * SG2_SP * 16 + (long)sg2_tlen(A, B, nw) * 40;
 }
 
 // Backward sweeps: the operator-grid cells of slice q outside the sweep
@@ -627,25 +658,25 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_sgs2(LuArgs a)
   constexpr int nt = 64 * (NW + 1), lagmax = (NW - 1) * SG2_L;  // wave NW stores
   const int t = threadIdx.x, lane = t & 63;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int A = a.g.A[C], B = a.g.B[C], AB = A * B;
+  const int A = a.g.A[C], B = a.g.B[C];
   const int nst = A + B - 1;
   const int s = 1 + t, sc = min(s, A);
+  constexpr int AP = 64 * NW;  // slots per diagonal (the host sets a.g.AP[C] to this)
+  const int OFF = sg2_toff(NW), TL = sg2_tlen(A, B, NW);
   z2 *ring = xs2;                               // [NW][SG2_RING]
   z2 *junk = xs2 + NW * SG2_RING;               // [NW][SG2_RING + 64]
-  z2 *stg = junk + NW * (SG2_RING + 64);        // [2][NW][SG2_BP][64]
-  double4 *ftab = (double4 *)(stg + 2 * NW * SG2_BP * 64);
-  double *vtab = (double *)(ftab + (B + 2));
-  int *dtab = (int *)(vtab + (B + 2));
+  z2 *stg = junk + NW * (SG2_RING + 64);        // [2][NW][SG2_BP][SG2_SP]
+  double4 *ftab = (double4 *)(stg + 2 * NW * SG2_BP * SG2_SP);  // [TL]
+  double *vtab = (double *)(ftab + TL);                     // [TL]
   {
     const double *g0 = (C == 0) ? a.d2s : a.d1s, *g1 = (C == 0) ? a.d2 : a.d1;
     const double *g2 = (C == 0) ? a.r2s : a.r1s, *g3 = (C == 0) ? a.r2 : a.r1;
-    for (int u = t; u <= B + 1; u += nt) {
-      const int fs = FWD ? max(u - 1, 0) : min(u, B);
-      const int fc = min(u, B);
-      ftab[u] = make_double4(g0[fs], g2[fs], g1[fc], g3[fc]);
-      vtab[u] = (C == 0) ? a.d2[fc] : a.d1s[fc];
+    for (int u = t; u < TL; u += nt) {
+      const int f = u - OFF, fs = FWD ? f - 1 : f;
+      const bool in = f >= 1 && f <= B;
+      ftab[u] = in ? make_double4(g0[fs], g2[fs], g1[f], g3[f]) : make_double4(0, 0, 0, 0);
+      vtab[u] = in ? ((C == 0) ? a.d2[f] : a.d1s[f]) : 0.0;
     }
-    for (int d = t; d <= A + B; d += nt) dtab[d] = d >= 2 ? sk_off(d, A, B) - max(1, d - B) : 0;
     for (int u = t; u < NW * SG2_RING; u += nt) ring[u] = zmk(0, 0);
   }
   const double *h0 = (C == 2) ? a.d2s : a.d3s, *h1 = (C == 2) ? a.d2 : a.d3;
@@ -655,8 +686,9 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_sgs2(LuArgs a)
   const double vA = (C == 0) ? a.d1s[q] : (C == 1) ? a.d2[q] : a.d2[sc];
   const double vB = (C == 2) ? a.d3[q] : a.d3[sc];
   // this slice's records and diagonals as raw buffers
-  const __amdgpu_buffer_rsrc_t Rb = sg2_rsrc(a.rec + (long)q * AB, (long)AB * (long)sizeof(LuRec));
-  const __amdgpu_buffer_rsrc_t Sb = sg2_rsrc(a.smith + a.g.base[C] + (long)q * AB, (long)AB * (long)sizeof(double4));
+  const long SL = a.g.SL[C];
+  const __amdgpu_buffer_rsrc_t Rb = sg2_rsrc(a.rec + (long)q * SL, SL * (long)sizeof(LuRec));
+  const __amdgpu_buffer_rsrc_t Sb = sg2_rsrc(a.smith + a.g.base[C] + (long)q * SL, SL * (long)sizeof(double4));
   // natural offset of cell (q, s, f) = nb + f*FS within the component
   const long PL = (C == 0) ? (long)n1 * (n2 + 1) : (C == 1) ? (long)(n1 + 1) * n2 : (long)(n1 + 1) * (n2 + 1);
   const long SS = (C == 2) ? (long)(n1 + 1) : PL;
@@ -680,9 +712,43 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_sgs2(LuArgs a)
     // computing waves fill the other half
     __syncthreads();  // the tables and rings are ready
     const int nhalf = nex / SG2_BP;
+    if (C != 0) {
+      // Ey, Ez: rows along x, a row's cells on consecutive diagonals are
+      // consecutive in the grid.  Lane l takes step l % 8 of row 8 k + l / 8
+      // of a wave, so eight lanes write one row's run of eight consecutive
+      // cells and a store covers eight rows' runs (about eight cache lines,
+      // against 64 for a step of 64 rows).
+      static_assert(SG2_BP == 8, "eight lanes per row, eight rows per store");
+      const int u = lane % SG2_BP, rl = lane / SG2_BP;
+      const long qb = (C == 1) ? (long)(n1 + 1) * q : PL * q;
+      for (int h = 0; h < nhalf; h++) {
+        __syncthreads();
+        const z2 *src = stg + (h & 1) * NW * SG2_BP * SG2_SP + u * SG2_SP + rl;
+        const int tau = h * SG2_BP + u;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+          const int lw = (FWD ? w : NW - 1 - w) * SG2_L;
+          const int d = FWD ? 2 + tau - lw : A + B - tau + lw;
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            const int sw = 1 + 64 * w + 8 * k + rl, scw = min(sw, A), f = d - sw;
+            const bool ok = sw <= A && f >= 1 && f <= B;
+            const long cell = qb + scw * SS + f;
+            const int kopw = ((C == 2) ? q : scw) - a.kofs;
+            const z2 v = src[w * SG2_BP * SG2_SP + 8 * k];
+            sg2_st(v, Wb, ok ? (int)cell * (int)sizeof(z2) : SG2_OOB);
+            if (!FWD)
+              sg2_st(v, Yb, ok && kopw >= 0 ? (int)(cell - (long)a.kofs * PL) * (int)sizeof(z2) : SG2_OOB);
+          }
+        }
+      }
+      if (!FWD) sg2_zero_frame<C>(a, q, t, nt);
+      return;
+    }
+    // Ex: rows a plane apart, columns a row apart: one step of 64 rows per store
     for (int h = 0; h < nhalf; h++) {
       __syncthreads();
-      const z2 *src = stg + (h & 1) * NW * SG2_BP * 64 + lane;
+      const z2 *src = stg + (h & 1) * NW * SG2_BP * SG2_SP + lane;
 #pragma unroll
       for (int w = 0; w < NW; w++) {
         const int sw = 1 + 64 * w + lane, scw = min(sw, A);
@@ -694,7 +760,7 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_sgs2(LuArgs a)
           const int tau = h * SG2_BP + u;
           const int d = FWD ? 2 + tau - lw : A + B - tau + lw, f = d - sw;
           const bool ok = sw <= A && f >= 1 && f <= B;
-          const z2 v = src[(w * SG2_BP + u) * 64];
+          const z2 v = src[(w * SG2_BP + u) * SG2_SP];
           if (FWD || C != 0) sg2_st(v, Wb, ok ? (int)(nbw + (long)f * FS) * (int)sizeof(z2) : SG2_OOB);
           if (!FWD)
             sg2_st(v, Yb, ok && kopw >= 0 ? (int)(nbw - (long)a.kofs * PL + (long)f * FS) * (int)sizeof(z2) : SG2_OOB);
@@ -708,23 +774,28 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_sgs2(LuArgs a)
   const int zl = __builtin_amdgcn_mbcnt_lo(0u, 0u);  // a lane-varying zero: the ring address stays per lane
   z2 *ring_in = ring + wv * SG2_RING + zl;
   z2 *ring_st = send ? ring + (FWD ? wv + 1 : wv - 1) * SG2_RING : junk + wv * (SG2_RING + 64) + lane;
-  z2 *stg_w = stg + wv * SG2_BP * 64 + lane;  // this lane's staging slot of step u of half 0
-  constexpr bool NOREC = (C == 0) && FWD;  // Ex forward: Ey = Ez = 0, so R = 0 (only b is stored)
-  // diagonal of step tau and whether this lane's cell (s, d - s) is on the slice
-  auto diag = [&](int tau) { return FWD ? 2 + tau - lag : A + B - tau + lag; };
-  auto onslice = [&](int d) {
-    const int f = d - s;
-    return s <= A && f >= 1 && f <= B;
-  };
+  z2 *stg_w = stg + wv * SG2_BP * SG2_SP + lane;  // this lane's staging slot of step u of half 0
+  using LD = Sg2Load<C, FWD>;
+  constexpr int SG2_P = LD::P;
+  // step tau is diagonal d = 2 + tau - lag (forward) or A + B - tau + lag
+  // (backward), this lane's cell (s, f = d - s) and its slot (d - 2) AP +
+  // s - 1: the slot's byte offsets are rv0 + tau RSTEP and sv0 + tau SSTEP,
+  // the fast table entry ib + tau (forward) or ib - tau, and the cell is on
+  // the slice when f - 1 = fm0 + tau (forward) or fm0 - tau is in [0, B)
+  const int d0 = FWD ? 2 - lag : A + B + lag, sl0 = (d0 - 2) * AP + s - 1;
+  constexpr int RSTEP = (FWD ? 1 : -1) * AP * (int)sizeof(LuRec), SSTEP = (FWD ? 1 : -1) * AP * (int)sizeof(double4);
+  const int rv0 = sl0 * (int)sizeof(LuRec), sv0 = sl0 * (int)sizeof(double4);
+  const int ib = d0 - s + OFF;
+  const int fm0 = s <= A ? d0 - s - 1 : (1 << 30);
   struct Rg { z2 r0, r1, r2, r3, b; double4 S; };
   auto load = [&](Rg &g, int tau) {
-    const int d = diag(tau);
-    const bool ok = onslice(d);
-    const int e = dtab[min(max(d, 2), A + B)] + s;
-    const int ro = ok ? e * (int)sizeof(LuRec) : SG2_OOB, so = ok ? e * (int)sizeof(double4) : SG2_OOB;
-    if (!NOREC) {
+    const bool ok = (unsigned)(FWD ? fm0 + tau : fm0 - tau) < (unsigned)B;
+    const int ro = ok ? rv0 + tau * RSTEP : SG2_OOB, so = ok ? sv0 + tau * SSTEP : SG2_OOB;
+    if (LD::R01) {
       g.r0 = sg2_ld(Rb, ro);
       g.r1 = sg2_ld(Rb, ro + 16);
+    }
+    if (LD::R23) {
       g.r2 = sg2_ld(Rb, ro + 32);
       g.r3 = sg2_ld(Rb, ro + 48);
     }
@@ -732,11 +803,12 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_sgs2(LuArgs a)
     const z2 s01 = sg2_ld(Sb, so), s23 = sg2_ld(Sb, so + 16);
     g.S = make_double4(s01.x, s01.y, s23.x, s23.y);
   };
-  __syncthreads();  // the tables and rings are ready (load reads the diagonal table)
+  __syncthreads();  // the tables and rings are ready
   Rg rg[SG2_P];
 #pragma unroll
   for (int u = 0; u < SG2_P; u++) {
-    if (NOREC) rg[u].r0 = rg[u].r1 = rg[u].r2 = rg[u].r3 = z0;
+    if (!LD::R01) rg[u].r0 = rg[u].r1 = z0;
+    if (!LD::R23) rg[u].r2 = rg[u].r3 = z0;
     load(rg[u], u);
   }
   double4 ftr[SG2_PC];
@@ -744,22 +816,22 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_sgs2(LuArgs a)
   z2 edg[SG2_PD];
 #pragma unroll
   for (int u = 0; u < SG2_PC; u++) {
-    const int fi = min(max(diag(u) - s, 0), B + 1);
-    ftr[u] = ftab[fi];
-    vtr[u] = FVM ? vtab[fi] : 0.0;
+    ftr[u] = ftab[FWD ? ib + u : ib - u];
+    vtr[u] = FVM ? vtab[FWD ? ib + u : ib - u] : 0.0;
   }
 #pragma unroll
   for (int u = 0; u < SG2_PD; u++) edg[u] = ring_in[(u - SG2_L - 1) & (SG2_RING - 1)];
   z2 prev = z0;
   const bool afast = (C != 1);
   for (int tk0 = 0; tk0 < ntk; tk0 += SG2_U) {
+    // the fast table from step tk0 + SG2_PC on at literal offsets
+    const double4 *ct = ftab + (FWD ? ib + tk0 : ib - tk0 - SG2_U - SG2_PC);
+    const double *cv = vtab + (FWD ? ib + tk0 : ib - tk0 - SG2_U - SG2_PC);
 #pragma unroll
     for (int uu = 0; uu < SG2_U; uu++) {
       const int tau = tk0 + uu;
       const Rg &c = rg[uu % SG2_P];
       const double4 ft = ftr[uu % SG2_PC];
-      const int d = diag(tau), f = d - s;
-      const bool ok = onslice(d);
       // new value of (s-1, f) (forward) or (s+1, f) (backward)
       const z2 nbv = FWD ? sg2_zdpp_keep<DPP_WAVE_SHR1>(edg[uu % SG2_PD], prev)
                          : sg2_zdpp_keep<DPP_WAVE_SHL1>(edg[uu % SG2_PD], prev);
@@ -779,20 +851,16 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_sgs2(LuArgs a)
       z2 aij = zmul(zsub(tA, tB), a.iwmu);
       if (FVM) aij = zmul(aij, zrmul((vtr[uu % SG2_PC] * vA) * vB, a.iomu));
       const z2 v = smith_finish(zsub(c.b, aij), c.S);
-      stg_w[((uu / SG2_BP) & 1) * NW * SG2_BP * 64 + (uu % SG2_BP) * 64] = v;
-      prev.x = ok ? v.x : prev.x;
-      prev.y = ok ? v.y : prev.y;
+      stg_w[((uu / SG2_BP) & 1) * NW * SG2_BP * SG2_SP + (uu % SG2_BP) * SG2_SP] = v;
+      prev = v;
       ring_st[uu & (SG2_RING - 1)] = prev;
       if (uu % SG2_BP == SG2_BP - 1) {
         __syncthreads();
         if (tau + 1 >= nex) goto swept;  // the same step for every wave
       }
       edg[uu % SG2_PD] = ring_in[(uu + SG2_PD - SG2_L - 1) & (SG2_RING - 1)];
-      {
-        const int fi = min(max(diag(tau + SG2_PC) - s, 0), B + 1);
-        ftr[uu % SG2_PC] = ftab[fi];
-        if (FVM) vtr[uu % SG2_PC] = vtab[fi];
-      }
+      ftr[uu % SG2_PC] = ct[FWD ? uu + SG2_PC : SG2_U - uu];
+      if (FVM) vtr[uu % SG2_PC] = cv[FWD ? uu + SG2_PC : SG2_U - uu];
       load(rg[uu % SG2_P], tau + SG2_P);
     }
   }

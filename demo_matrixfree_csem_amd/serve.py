@@ -249,9 +249,11 @@ class Engine:
 
     def _estimate(self, survey):
         """Device bytes a model of `survey` will take: its FP64 arrays (about
-        96 bytes per unknown of the full grid -- conductivities, LU-SGS
-        records, Smith cache, pre-pass buffer -- and 80 per airwave
-        transform point: two multipliers, three work planes) plus half that for the
+        160 bytes per unknown of the full grid -- conductivities, LU-SGS
+        records and Smith cache in the sweeps' diagonal-major layout, which
+        gives every diagonal a whole wave's slots, about twice the cells,
+        pre-pass buffer -- and 80 per airwave transform point: two
+        multipliers, three work planes) plus half the FP64 arrays for the
         FP32 copy, times the largest ratio of taken to estimated bytes seen
         so far.  Returns (estimate, unscaled estimate)."""
         nx, ny, nz = (len(a) - 1 for a in (survey.model.x1node, survey.model.x2node, survey.model.x3node))
@@ -259,7 +261,7 @@ class Engine:
         n1, n2, n3f = nx + 2 * nb, ny + 2 * nb, nz + 2 * nb
         fE = n1 * (n2 + 1) * (n3f + 1) + (n1 + 1) * n2 * (n3f + 1) + (n1 + 1) * (n2 + 1) * n3f
         nfft = int(survey.flags.get("n1fft", 1024)) if int(survey.flags.get("airbc", 0)) else 0
-        base = 1.5 * (96 * fE + 80 * nfft * nfft)
+        base = 1.5 * (160 * fE + 80 * nfft * nfft)
         return int(self.est_scale * base), base
 
     def run(self, requests, arrivals):
