@@ -270,11 +270,13 @@ __global__ void k_hz0(ApplyArgs a, z2 *hz)
 // term is dropped.  Divisions by spacings use ddiv.
 //
 // Layout: a sweep's per-cell inputs (record, right-hand side, diagonal)
-// are stored diagonal-major ("skewed"): slice q, then diagonal d = s + f,
-// then row s, AP slots per diagonal (row s at slot s - 1), so the lanes of a
-// wave (consecutive rows of one diagonal) load one contiguous run and a
-// lane's next diagonal is a fixed stride away; the slots of rows not on a
-// diagonal are never written or read.  The state W (forward
+// are stored diagonal-major ("skewed") in 16-byte planes: slice q, then
+// one plane per 16-byte unit of the record (r0..r3, b) or of the Smith part
+// (two), then diagonal d = s + f, then row s, AP slots per diagonal (row s
+// at slot s - 1).  The lanes of a wave (consecutive rows of one diagonal)
+// load each unit as one contiguous run, every unit of a step from the same
+// lane offset, and a lane's next diagonal is a fixed stride away; the
+// slots of rows not on a diagonal are never written or read.  The state W (forward
 // values, then the backward values of Ey and Ez) and the result y keep the
 // reference layout; the pre-passes gather their neighbours from W.
 
@@ -298,6 +300,15 @@ __host__ __device__ __forceinline__ int sk_off(int d, int A, int B)
 }
 
 struct LuRec { z2x4 R; z2 b; };  // 80 B per swept cell
+
+// The record planes of sweep (C, FWD): NP per slice, unit u in plane u
+// (Ex forward: b alone, in plane 0); W01: whether units 0 and 1 are written
+// (Ey forward sees Ez = 0, so its t1 and the known part of t2 are +0 and
+// its sweep does not read them).
+template <int C, bool FWD> struct LuPlanes {
+  static constexpr int NP = (C == 0 && FWD) ? 1 : 5;
+  static constexpr bool W01 = !FWD || C == 2;
+};
 
 struct LuArgs {
   const z2 *x;  // right-hand side, operator grid
@@ -370,8 +381,10 @@ __global__ __launch_bounds__(256) void k_lu_smith(LuArgs a)
   if (fabs(cc) < fabs(dd)) { ratio = cc / dd; denom = (cc * ratio) + dd; code = 2; }
   else { ratio = dd / cc; denom = (dd * ratio) + cc; code = 1; }
   if (denom < 0) code = -code;
-  ((double4 *)a.smith)[a.g.base[C] + (long)q * a.g.SL[C] + (long)(s + f - 2) * a.g.AP[C] + (s - 1)] =
-      make_double4(ratio, fabs(denom), fabs(1.0 / denom), code);
+  const long SL = a.g.SL[C];
+  z2 *S = (z2 *)a.smith + 2 * (a.g.base[C] + (long)q * SL) + (long)(s + f - 2) * a.g.AP[C] + (s - 1);
+  S[0] = zmk(ratio, fabs(denom));
+  S[SL] = zmk(fabs(1.0 / denom), code);
 }
 
 // Second half of zdiv: n / aii from the stored diagonal.  Both branches
@@ -432,7 +445,7 @@ __global__ __launch_bounds__(256) void k_lu_pre(LuArgs a)
       pidx[cell] = -1;
       continue;
     }
-    pidx[cell] = q * SL + (long)(s + f - 2) * AP + (s - 1);
+    pidx[cell] = q * LuPlanes<C, FWD>::NP * SL + (long)(s + f - 2) * AP + (s - 1);
     int k, j, i;
     lu_kji<C>(q, s, f, k, j, i);
     const int kop = k - a.kofs;
@@ -480,22 +493,20 @@ __global__ __launch_bounds__(256) void k_lu_pre(LuArgs a)
     }
   }
   __syncthreads();
-  // store the tile's records unit by unit (Ex forward: only b, the unit its
-  // sweep reads)
+  // store the tile's records plane by plane, each diagonal's run of the
+  // tile contiguous (Ex forward: b alone; Ey forward: not the +0 units its
+  // sweep does not read)
   z2 *rec = (z2 *)a.rec;
-  if (C == 0 && FWD) {
+  using LP = LuPlanes<C, FWD>;
+#pragma unroll
+  for (int c = 0; c < NU; c++) {
+    if ((C == 0 && FWD && c != 4) || (!LP::W01 && c < 2)) continue;
+    const long po = (LP::NP == 1) ? 0 : (long)c * SL;
 #pragma unroll
     for (int it = 0; it < NC / 256; it++) {
       const int e = t + it * 256;
       const long p = pidx[e];
-      if (p >= 0) rec[p * NU + 4] = tile[e * NU + 4];
-    }
-  } else {
-#pragma unroll
-    for (int it = 0; it < NC * NU / 256; it++) {
-      const int u = t + it * 256, e = u / NU, c = u % NU;
-      const long p = pidx[e];
-      if (p >= 0) rec[p * NU + c] = tile[u];
+      if (p >= 0) rec[p + po] = tile[e * NU + c];
     }
   }
 }
@@ -547,9 +558,9 @@ __global__ __launch_bounds__(256) void k_lu_pre(LuArgs a)
 #define SG2_BP 8
 #define SG2_PD 4
 #define SG2_L (SG2_BP + SG2_PD - 1)
-#define SG2_RING 32
+#define SG2_RING 16
 #define SG2_PC 4
-#define SG2_U 32
+#define SG2_U 48
 #define SG2_OOB ((int)0x80000000)  // a buffer offset past every buffer (all shorter than 2 GiB)
 #define SG2_SP 66  // staging slots per step: 64 rows, padded so that the store wave's reads of eight
                    // steps of eight rows (Ey, Ez) meet no LDS bank conflict
@@ -563,7 +574,9 @@ static_assert(SG2_U % SG2_RING == 0 && SG2_U % SG2_BP == 0 && SG2_U % SG2_PD == 
 // registers a step's loads take, the deeper the prefetch.
 template <int C, bool FWD> struct Sg2Load {
   static constexpr bool R01 = !FWD || C == 2, R23 = !FWD || C != 0;
-  static constexpr int P = (R01 && R23) ? 4 : 8;
+  static constexpr int P = (R01 && R23) ? 6 : 8;
+  static constexpr int NP = LuPlanes<C, FWD>::NP, BP = (NP == 1) ? 0 : 4;  // planes per slice, b's plane
+  static_assert(R01 == LuPlanes<C, FWD>::W01 || !R01, "the sweep reads only units its pre-pass writes");
   static_assert(SG2_U % P == 0, "sweep unroll");
 };
 typedef unsigned sg2_v4u __attribute__((ext_vector_type(4)));
@@ -572,9 +585,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t sg2_rsrc(const void *p, long l
 {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)len, 0x00020000);
 }
-__device__ __forceinline__ z2 sg2_ld(__amdgpu_buffer_rsrc_t r, int ofs)
+__device__ __forceinline__ z2 sg2_ld(__amdgpu_buffer_rsrc_t r, int ofs, int sofs = 0)
 {
-  return __builtin_bit_cast(z2, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, 0, 0));
+  return __builtin_bit_cast(z2, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, sofs, 0));
 }
 __device__ __forceinline__ void sg2_st(z2 v, __amdgpu_buffer_rsrc_t r, int ofs)
 {
@@ -687,8 +700,10 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_sgs2(LuArgs a)
   const double vB = (C == 2) ? a.d3[q] : a.d3[sc];
   // this slice's records and diagonals as raw buffers
   const long SL = a.g.SL[C];
-  const __amdgpu_buffer_rsrc_t Rb = sg2_rsrc(a.rec + (long)q * SL, SL * (long)sizeof(LuRec));
-  const __amdgpu_buffer_rsrc_t Sb = sg2_rsrc(a.smith + a.g.base[C] + (long)q * SL, SL * (long)sizeof(double4));
+  using LD = Sg2Load<C, FWD>;
+  const __amdgpu_buffer_rsrc_t Rb = sg2_rsrc((const z2 *)a.rec + (long)q * LD::NP * SL, LD::NP * SL * 16L);
+  const __amdgpu_buffer_rsrc_t Sb = sg2_rsrc((const z2 *)a.smith + 2 * (a.g.base[C] + (long)q * SL), 2 * SL * 16L);
+  const int PS = (int)SL * 16;  // bytes per plane
   // natural offset of cell (q, s, f) = nb + f*FS within the component
   const long PL = (C == 0) ? (long)n1 * (n2 + 1) : (C == 1) ? (long)(n1 + 1) * n2 : (long)(n1 + 1) * (n2 + 1);
   const long SS = (C == 2) ? (long)(n1 + 1) : PL;
@@ -775,32 +790,31 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_sgs2(LuArgs a)
   z2 *ring_in = ring + wv * SG2_RING + zl;
   z2 *ring_st = send ? ring + (FWD ? wv + 1 : wv - 1) * SG2_RING : junk + wv * (SG2_RING + 64) + lane;
   z2 *stg_w = stg + wv * SG2_BP * SG2_SP + lane;  // this lane's staging slot of step u of half 0
-  using LD = Sg2Load<C, FWD>;
   constexpr int SG2_P = LD::P;
   // step tau is diagonal d = 2 + tau - lag (forward) or A + B - tau + lag
   // (backward), this lane's cell (s, f = d - s) and its slot (d - 2) AP +
-  // s - 1: the slot's byte offsets are rv0 + tau RSTEP and sv0 + tau SSTEP,
+  // s - 1: the slot's byte offset in every plane is rv0 + tau RSTEP,
   // the fast table entry ib + tau (forward) or ib - tau, and the cell is on
   // the slice when f - 1 = fm0 + tau (forward) or fm0 - tau is in [0, B)
   const int d0 = FWD ? 2 - lag : A + B + lag, sl0 = (d0 - 2) * AP + s - 1;
-  constexpr int RSTEP = (FWD ? 1 : -1) * AP * (int)sizeof(LuRec), SSTEP = (FWD ? 1 : -1) * AP * (int)sizeof(double4);
-  const int rv0 = sl0 * (int)sizeof(LuRec), sv0 = sl0 * (int)sizeof(double4);
+  constexpr int RSTEP = (FWD ? 1 : -1) * AP * 16;
+  const int rv0 = sl0 * 16;
   const int ib = d0 - s + OFF;
   const int fm0 = s <= A ? d0 - s - 1 : (1 << 30);
   struct Rg { z2 r0, r1, r2, r3, b; double4 S; };
   auto load = [&](Rg &g, int tau) {
     const bool ok = (unsigned)(FWD ? fm0 + tau : fm0 - tau) < (unsigned)B;
-    const int ro = ok ? rv0 + tau * RSTEP : SG2_OOB, so = ok ? sv0 + tau * SSTEP : SG2_OOB;
+    const int ro = ok ? rv0 + tau * RSTEP : SG2_OOB;  // every plane's lane offset
     if (LD::R01) {
       g.r0 = sg2_ld(Rb, ro);
-      g.r1 = sg2_ld(Rb, ro + 16);
+      g.r1 = sg2_ld(Rb, ro, PS);
     }
     if (LD::R23) {
-      g.r2 = sg2_ld(Rb, ro + 32);
-      g.r3 = sg2_ld(Rb, ro + 48);
+      g.r2 = sg2_ld(Rb, ro, 2 * PS);
+      g.r3 = sg2_ld(Rb, ro, 3 * PS);
     }
-    g.b = sg2_ld(Rb, ro + 64);
-    const z2 s01 = sg2_ld(Sb, so), s23 = sg2_ld(Sb, so + 16);
+    g.b = sg2_ld(Rb, ro, LD::BP * PS);
+    const z2 s01 = sg2_ld(Sb, ro), s23 = sg2_ld(Sb, ro, PS);
     g.S = make_double4(s01.x, s01.y, s23.x, s23.y);
   };
   __syncthreads();  // the tables and rings are ready
