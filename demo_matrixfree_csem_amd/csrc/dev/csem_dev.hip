@@ -759,15 +759,16 @@ static int run_lusgs(csem_model *m, const csem_freq *f, const z2 *x, z2 *y, hipS
   }
   // bytes per swept cell each kernel must move: the pre-pass reads b and
   // the other components' W values it differences and writes the record
-  // (only b for Ex forward); the sweep reads the record and the diagonal
-  // and writes W (not in the last sweep) and y (backward sweeps).  The
+  // (only b for Ex forward, not the two +0 units for Ey forward); the sweep
+  // reads the record units it uses and the diagonal and writes W (not in
+  // the last sweep) and y (backward sweeps).  The
   // records' round trip is the price of the parallel pre-pass: the whole
   // application needs 144 bytes per unknown (forward Ex 40, Ey 56, Ez 72,
   // backward 88 each: b, sigma, the values read, the value written).
   kt_launch(KT_LU_PRE, s, 32.0 * nc[0], k_lu_pre<0, true>, gq[0], bq, 0, a);
   kt_launch(KT_SGS_EX_F, s, 64.0 * nc[0], SWEEP(0, true), gs[0], tb[0] + 64, lds[0], a);
-  kt_launch(KT_LU_PRE, s, 112.0 * nc[1], k_lu_pre<1, true>, gq[1], bq, 0, a);
-  kt_launch(KT_SGS_EY_F, s, 128.0 * nc[1], SWEEP(1, true), gs[1], tb[1] + 64, lds[1], a);
+  kt_launch(KT_LU_PRE, s, 80.0 * nc[1], k_lu_pre<1, true>, gq[1], bq, 0, a);
+  kt_launch(KT_SGS_EY_F, s, 96.0 * nc[1], SWEEP(1, true), gs[1], tb[1] + 64, lds[1], a);
   kt_launch(KT_LU_PRE, s, 128.0 * nc[2], k_lu_pre<2, true>, gq[2], bq, 0, a);
   kt_launch(KT_SGS_EZ_F, s, 128.0 * nc[2], SWEEP(2, true), gs[2], tb[2] + 64, lds[2], a);
   kt_launch(KT_LU_PRE, s, 144.0 * nc[2], k_lu_pre<2, false>, gq[2], bq, 0, a);
