@@ -52,7 +52,9 @@ struct MixSlot {
   c2 *buf, *bufx, *bufy, *hz0, *hxg, *hyg;  // FP32 airwave scratch, R of each
   AirBufs a64;                              // FP64 airwave scratch (true residual)
   c2 *W, *V;                                // LU-SGS state; a sweep's values in record-slot order
-  float4 *th;                               // LU-SGS records (T, h) per record slot and r
+  c2 *tt, *hh;                              // LU-SGS records per record slot and r: T, h = iwmu/aii
+  c2 h_iw[MX_RMAX];                         // the iwmu each right-hand side's h was computed for
+  char h_ok[MX_RMAX];                       // whether it was
   int pc;                                   // preconditioner of the solve in flight (csem_mixed_opts)
   c2 *kv;                                   // Krylov vectors
   c2 *xbest;                                // F-BiCGStab: the correction at the best recursive residual
@@ -263,7 +265,7 @@ static int slot_new(csem_model *m, int R, MixSlot **out)
     MA(q, a64.hz0, (long)n1 * n2); MA(q, a64.hxg, (long)(n1 + 1) * n2); MA(q, a64.hyg, (long)n1 * (n2 + 1));
   }
   {
-    MA(q, W, m->fE * R); MA(q, th, mx_pad(m).n * R); MA(q, V, mx_pad(m).n * R);
+    MA(q, W, m->fE * R); MA(q, tt, mx_pad(m).n * R); MA(q, hh, mx_pad(m).n * R); MA(q, V, mx_pad(m).n * R);
     q->nch = (int)((m->lE + RED_CHUNK - 1) / RED_CHUNK);
     MA(q, kv, 9L * R * m->lE); MA(q, part, (long)R * q->nch * 4); MA(q, sc, (long)R * MS_N);
     MA(q, xbest, R * m->lE);
@@ -273,7 +275,8 @@ static int slot_new(csem_model *m, int R, MixSlot **out)
   }
   // records of slots off the slices stay zero (the pre-passes write the cells' records)
   if (dev_memset(q->W, 0, m->fE * R * sizeof(c2)) != hipSuccess ||
-      dev_memset(q->th, 0, mx_pad(m).n * R * sizeof(float4)) != hipSuccess ||
+      dev_memset(q->tt, 0, mx_pad(m).n * R * sizeof(c2)) != hipSuccess ||
+      dev_memset(q->hh, 0, mx_pad(m).n * R * sizeof(c2)) != hipSuccess ||
       dev_memset(q->sc, 0, (long)R * MS_N * sizeof(double)) != hipSuccess) {
     rc = csem_fail(-1, "memset");
     goto fail;
@@ -453,7 +456,7 @@ static MxLu mx_lu(const csem_model *m, const MixState *x, MixSlot *q, const MxFr
   MxLu a;
   a.g = mx_geo(m, x); a.sk = m->g;
   a.b = b; a.y = y; a.W = q->W; a.fE = m->fE; a.fEx = m->fEx; a.fEy = m->fEy;
-  a.th = q->th; a.V = q->V; a.pk = mx_pad(m);
+  a.T = q->tt; a.h = q->hh; a.V = q->V; a.pk = mx_pad(m);
   a.c1m = x->c1m; a.c1p = x->c1p; a.c2m = x->c2m; a.c2p = x->c2p; a.c3m = x->c3m; a.c3p = x->c3p;
   a.fr = fr;
   a.sc = nullptr;
@@ -619,12 +622,34 @@ static void mx_lusgs_dispatch(const csem_model *m, int R, const MxLu &a, const i
   }
 }
 
+// The records' per-frequency half (h) of a slot for the frequencies fr,
+// computed when a right-hand side's frequency differs from the one its h
+// holds.  A solve calls this before it captures an iteration, so a
+// captured graph never holds it.
+static int mx_hrec(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, hipStream_t s)
+{
+  bool need = false;
+  for (int r = 0; r < R; r++)
+    need = need || !q->h_ok[r] || q->h_iw[r].x != fr.iwmu[r].x || q->h_iw[r].y != fr.iwmu[r].y;
+  if (!need) return 0;
+  const MxLu a = mx_lu(m, x, q, fr, nullptr, nullptr);
+  const SkewGeo &g = m->g;
+  k_mx_hrec<0><<<dim3((unsigned)((a.pk.SL[0] + 255) / 256), g.ns[0]), 256, 0, s>>>(a, R);
+  k_mx_hrec<1><<<dim3((unsigned)((a.pk.SL[1] + 255) / 256), g.ns[1]), 256, 0, s>>>(a, R);
+  k_mx_hrec<2><<<dim3((unsigned)((a.pk.SL[2] + 255) / 256), g.ns[2]), 256, 0, s>>>(a, R);
+  HIP_OK(hipGetLastError());
+  for (int r = 0; r < R; r++) { q->h_iw[r] = fr.iwmu[r]; q->h_ok[r] = 1; }
+  return 0;
+}
+
 // sc: the inner solve's scalars (right-hand sides done are skipped), or NULL
 static int mx_lusgs(csem_model *m, MixState *x, MixSlot *q, int R, const MxFreq &fr, const c2 *b, c2 *y,
                     hipStream_t s, const double *sc = nullptr)
 {
   MxLu a = mx_lu(m, x, q, fr, b, y);
   a.sc = sc;
+  int rc;
+  if (q->pc != CSEM_PRECOND_RBSGS && (rc = mx_hrec(m, x, q, R, fr, s))) return rc;
   if (q->pc == CSEM_PRECOND_RBSGS) mx_rbsgs_dispatch(m, R, a, s);
   else mx_lusgs_dispatch(m, R, a, x->pre_v[R], s);
   HIP_OK(hipGetLastError());
@@ -922,6 +947,7 @@ static int mixed_solve_slot(csem_model *m, MixState *x, MixSlot *q, int R, const
   for (int r = 0; r < R; r++) { bn[r] = sqrt(nb[r]); rn[r] = bn[r]; relres[r] = 1.0; iters[r] = 0; }
   // operands: any R x n FP32 buffers
   if ((rc = mx_tune(m, x, q, R, fr, k.r, k.v, s)) || (rc = mx_tune_lu(m, x, q, R, fr, k.r, k.v, s))) return rc;
+  if (o.precond == CSEM_PRECOND_LUSGS && (rc = mx_hrec(m, x, q, R, fr, s))) return rc;
   // Phase 0 refines with the inner method asked for.  A right-hand side
   // whose F-BiCGStab inner solve was cut for stagnation and whose
   // correction the guard still rejected goes on in phase 1 with FGMRES(M)

@@ -498,9 +498,11 @@ __global__ void k_mx_air_ghost(MxAir a)
 // coefficient, cs the slow axis', prev the row's previous cell, nbv the
 // neighbour row.
 //
-// Per-cell records (T, h) are stored per right-hand side (r-major), one
-// float4 each, written whole by each pre-pass; a sweep step loads both with
-// one 16-byte load.
+// Per-cell records are stored per right-hand side (r-major) in two planes
+// of complex floats: T, written by each pre-pass, and h = iwmu/aii, which
+// depends only on the frequency and is written once for a slot's
+// frequencies (k_mx_hrec); a sweep step loads both, each wave's step two
+// contiguous runs.
 // Each slice of component C is a block of A + B + 1 diagonals of AP rows
 // (AP = A rounded up to whole waves): diagonal d = s + f (2..A+B) at
 // position d - 1, row s at s - 1.  The first and last diagonals and every
@@ -522,7 +524,8 @@ struct MxLu {
   c2 *y;         // R x n results
   c2 *W;         // R x fE state on the full grid (Ex | Ey | Ez)
   long fE, fEx, fEy;
-  float4 *th;    // (T, h = iwmu/aii) per record slot, R x pk.n
+  c2 *T;         // T per record slot, R x pk.n (every pre-pass)
+  c2 *h;         // h = iwmu/aii per record slot, R x pk.n (once per frequency)
   c2 *V;         // a sweep's values per record slot, R x pk.n
   const float *c1m, *c1p, *c2m, *c2p, *c3m, *c3p;  // own-axis coefficients (z on the full grid)
   MxFreq fr;
@@ -588,12 +591,9 @@ template <int R> struct MxCopyTile<0, R> : MxTile3<(R == 1 ? 8 : 16), 8, (R == 1
 #define MW1(r, k, j, i) a.W[(r) * a.fE + a.fEx + id2(n1, n2, k, j, i)]
 #define MW2(r, k, j, i) a.W[(r) * a.fE + a.fEx + a.fEy + id3(n1, n2, k, j, i)]
 
-// Pre-pass of sweep (C, FWD) for every r: the records (T, h) of a tile of
-// cells.  Each cell goes to LDS at its position in the tile's skewed order,
-// with its global record index and the two numbers its h needs, so the
-// store phase writes whole records in runs: h = iwmu/aii with aii =
-// c iwmu - sigma, c the sum of the cell's own-axis coefficients (the fvm
-// factor scales aij and aii alike and cancels).
+// Pre-pass of sweep (C, FWD) for every r: the record T of a tile of cells.
+// Each cell goes to LDS at its position in the tile's skewed order, with
+// its global record index, so the store phase writes the records in runs.
 template <int C, bool FWD, int R, int V>
 __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
 {
@@ -601,7 +601,6 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
   static_assert(TL::TQ * TL::TS * TL::TF == 1024 && (C != 0 || 256 % (TL::TQ * TL::TF) == 0), "pre-pass tile");
   constexpr int TQ = TL::TQ, TS = TL::TS, TF = TL::TF, NC = TQ * TS * TF;
   __shared__ c2 tile[R][NC];
-  __shared__ c2 csg[NC];    // (c, sigma) of each slot
   __shared__ int pidx[NC];  // global record index of each slot, -1 off the slices
   const int n1 = a.g.n1, n2 = a.g.n2, n3 = a.g.n3f;
   const int A = a.sk.A[C], B = a.sk.B[C], ns = a.sk.ns[C];
@@ -633,9 +632,6 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
     const long xo = (C == 0) ? id1(n1, n2, kop, j, i) : (C == 1) ? a.g.lEx + id2(n1, n2, kop, j, i)
                                                                    : a.g.lEx + a.g.lEy + id3(n1, n2, kop, j, i);
     const float vol = a.g.d1s[i] * a.g.d2[j] * a.g.d3[k];
-    if (C == 0) csg[cell] = cmk(a.c2p[j] + a.c2m[j] + a.c3p[k] + a.c3m[k], a.g.s11[id1(n1, n2, k, j, i)]);
-    else if (C == 1) csg[cell] = cmk(a.c3p[k] + a.c3m[k] + a.c1p[i] + a.c1m[i], a.g.s22[id2(n1, n2, k, j, i)]);
-    else csg[cell] = cmk(a.c1p[i] + a.c1m[i] + a.c2p[j] + a.c2m[j], a.g.s33[id3(n1, n2, k, j, i)]);
 #pragma unroll
     for (int r = 0; r < R; r++) {
       if (R > 1 && !mx_live(a.sc, r)) continue;
@@ -673,16 +669,39 @@ __global__ __launch_bounds__(256) void k_mx_pre(MxLu a)
     const int e = t + it * 256;
     const int p = pidx[e];
     if (p < 0) continue;
-    const c2 cs = csg[e];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      if (R > 1 && !mx_live(a.sc, r)) continue;
-      const c2 iw = a.fr.iwmu[r], T = tile[r][e];
-      const c2 aii = cmk(cs.x * iw.x - cs.y, cs.x * iw.y);
-      const float den = 1.f / (aii.x * aii.x + aii.y * aii.y);
-      a.th[r * a.pk.n + p] =
-          make_float4(T.x, T.y, (iw.x * aii.x + iw.y * aii.y) * den, (iw.y * aii.x - iw.x * aii.y) * den);
-    }
+    for (int r = 0; r < R; r++)
+      if (R == 1 || mx_live(a.sc, r)) a.T[r * a.pk.n + p] = tile[r][e];
+  }
+}
+
+// The per-frequency half of the records of component C for r < R: h =
+// iwmu/aii of every swept cell, aii = c iwmu - sigma with c the sum of the
+// cell's own-axis coefficients (the fvm factor scales aij and aii alike and
+// cancels).  One thread per record slot of a slice (coalesced stores); the
+// slots off the slice keep their zeros.
+template <int C>
+__global__ __launch_bounds__(256) void k_mx_hrec(MxLu a, int R)
+{
+  const int A = a.sk.A[C], B = a.sk.B[C], AP = a.pk.AP[C];
+  const int q = blockIdx.y;
+  const long u = (long)blockIdx.x * 256 + threadIdx.x;
+  if (u >= a.pk.SL[C]) return;
+  const int s = (int)(u % AP) + 1, f = (int)(u / AP) + 1 - s;  // slot (d - 1) AP + s - 1, d = s + f
+  if (s > A || f < 1 || f > B) return;
+  int k, j, i;
+  lu_kji<C>(q, s, f, k, j, i);
+  const int n1 = a.g.n1, n2 = a.g.n2;
+  c2 cs;
+  if (C == 0) cs = cmk(a.c2p[j] + a.c2m[j] + a.c3p[k] + a.c3m[k], a.g.s11[id1(n1, n2, k, j, i)]);
+  else if (C == 1) cs = cmk(a.c3p[k] + a.c3m[k] + a.c1p[i] + a.c1m[i], a.g.s22[id2(n1, n2, k, j, i)]);
+  else cs = cmk(a.c1p[i] + a.c1m[i] + a.c2p[j] + a.c2m[j], a.g.s33[id3(n1, n2, k, j, i)]);
+  const long p = a.pk.base[C] + (long)q * a.pk.SL[C] + u;
+  for (int r = 0; r < R; r++) {
+    const c2 iw = a.fr.iwmu[r];
+    const c2 aii = cmk(cs.x * iw.x - cs.y, cs.x * iw.y);
+    const float den = 1.f / (aii.x * aii.x + aii.y * aii.y);
+    a.h[r * a.pk.n + p] = cmk((iw.x * aii.x + iw.y * aii.y) * den, (iw.y * aii.x - iw.x * aii.y) * den);
   }
 }
 
@@ -975,10 +994,12 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_mx_sgs(MxLu a)
   }
   const float *gs = (C == 2) ? (FWD ? a.c2m : a.c2p) : (FWD ? a.c3m : a.c3p);
   const float cs = gs[sc];
-  // this slice's records as a raw buffer (the hardware checks the vector
-  // offset: a load past the end reads zero, a store past the end is dropped)
-  const __amdgpu_buffer_rsrc_t Rb = mx_rsrc(a.th + r * a.pk.n + a.pk.base[C] + (long)q * a.pk.SL[C],
-                                            a.pk.SL[C] * (long)sizeof(float4));
+  // this slice's records (T and h planes) as raw buffers (the hardware
+  // checks the vector offset: a load past the end reads zero, a store past
+  // the end is dropped)
+  const long rb0 = r * a.pk.n + a.pk.base[C] + (long)q * a.pk.SL[C];
+  const __amdgpu_buffer_rsrc_t Tb = mx_rsrc(a.T + rb0, a.pk.SL[C] * (long)sizeof(c2));
+  const __amdgpu_buffer_rsrc_t Hb = mx_rsrc(a.h + rb0, a.pk.SL[C] * (long)sizeof(c2));
   // the values go to V in the records' slot order: a wave's step is one
   // contiguous run (a lane off the slice stores its exact 0 into a slot no
   // reader uses, a lagged step's run lies outside the slice and is dropped)
@@ -1055,16 +1076,19 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_mx_sgs(MxLu a)
   }
   // table index of this lane's column at step tau: ib + tau forward, ib - tau backward
   const int ib = (FWD ? 2 - lag : A + B + lag) - s + OFF;
-  // byte offset of this lane's record at step tau: rv0 + tau * RSTEP
-  constexpr int RSTEP = (FWD ? 1 : -1) * AP * (int)sizeof(float4);
-  const int rv0 = (s - 1) * (int)sizeof(float4) + (FWD ? 1 - lag : nst + lag) * AP * (int)sizeof(float4);
+  // byte offset of this lane's slot at step tau in the T, h and V planes:
+  // rv0 + tau * RSTEP
+  constexpr int RSTEP = (FWD ? 1 : -1) * AP * (int)sizeof(c2);
+  const int rv0 = (s - 1) * (int)sizeof(c2) + (FWD ? 1 - lag : nst + lag) * AP * (int)sizeof(c2);
   auto rec_at = [&](int vofs) -> float4 {
-    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(Rb, vofs, 0, 0));
+    const c2 T = __builtin_bit_cast(c2, __builtin_amdgcn_raw_buffer_load_b64(Tb, vofs, 0, 0));
+    const c2 h = __builtin_bit_cast(c2, __builtin_amdgcn_raw_buffer_load_b64(Hb, vofs, 0, 0));
+    return make_float4(T.x, T.y, h.x, h.y);
   };
   // a lane whose cell at step tau is off the slice (f - 1 = d - s - 1 outside
-  // [0, B) or s > A) neither loads nor stores: its offsets are past the
-  // buffers (the load reads 0, the store is dropped), and the V offset is
-  // the record offset halved, so the load's mask carries over to the store
+  // [0, B) or s > A) neither loads nor stores: its offset is past the
+  // buffers (the loads read 0, the store is dropped), the same offset for
+  // the records and V
   const int fm0 = s <= A ? (FWD ? 1 - lag : A + B + lag - 1) - s : (1 << 30);  // f - 1 at step 0
   auto rofs = [&](int tau, int ofs) {
     return (MXS_MASK == 0 || (unsigned)(FWD ? fm0 + tau : fm0 - tau) < (unsigned)B) ? ofs : MX_OOB;
@@ -1100,7 +1124,7 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_mx_sgs(MxLu a)
       if (SW)  // MXS_U / MXS_BP is even: the half of step tk0 + uu is that of uu
         stg_w[((uu / MXS_BP) & 1) * NW * MXS_BP * MXS_SP + (uu % MXS_BP) * MXS_SP] = prev;
       else
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, prev), Vb, (int)((unsigned)rof[uu % P] >> 1), 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, prev), Vb, rof[uu % P], 0, 0);
       ring_st[uu & (MXS_RING - 1)] = prev;
       if (uu % MXS_BP == MXS_BP - 1) {
         __syncthreads();
