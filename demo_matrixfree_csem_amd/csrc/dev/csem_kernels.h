@@ -727,58 +727,79 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_sgs2(LuArgs a)
     // computing waves fill the other half
     __syncthreads();  // the tables and rings are ready
     const int nhalf = nex / SG2_BP;
+    // Offsets are precomputed per lane (f - 1 of the lane's cell at half 0,
+    // past any column on a row past A; the cell's byte offsets in W and y,
+    // 1 GiB where the row has no y plane) and blended per store: a store
+    // wave that issued as many instructions as a computing wave would hold
+    // every barrier.
     if (C != 0) {
       // Ey, Ez: rows along x, a row's cells on consecutive diagonals are
       // consecutive in the grid.  Lane l takes step l % 8 of row 8 k + l / 8
       // of a wave, so eight lanes write one row's run of eight consecutive
       // cells and a store covers eight rows' runs (about eight cache lines,
-      // against 64 for a step of 64 rows).
+      // against 64 for a step of 64 rows).  Half h moves f by SG2_BP.
       static_assert(SG2_BP == 8, "eight lanes per row, eight rows per store");
       const int u = lane % SG2_BP, rl = lane / SG2_BP;
       const long qb = (C == 1) ? (long)(n1 + 1) * q : PL * q;
+      int fm1[NW][8], wo[NW][8], yo[NW][8];
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        const int lw = (FWD ? w : NW - 1 - w) * SG2_L;
+        const int d = FWD ? 2 + u - lw : A + B - u + lw;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const int sw = 1 + 64 * w + 8 * k + rl, scw = min(sw, A), f = d - sw;
+          const long cell = qb + scw * SS + f;
+          const int kopw = ((C == 2) ? q : scw) - a.kofs;
+          fm1[w][k] = sw <= A ? f - 1 : (1 << 30);
+          wo[w][k] = (int)cell * (int)sizeof(z2);
+          yo[w][k] = (FWD || kopw < 0) ? (1 << 30) : (int)(cell - (long)a.kofs * PL) * (int)sizeof(z2);
+        }
+      }
+      constexpr int FST = FWD ? SG2_BP : -SG2_BP, OST = FST * (int)sizeof(z2);
       for (int h = 0; h < nhalf; h++) {
         __syncthreads();
         const z2 *src = stg + (h & 1) * NW * SG2_BP * SG2_SP + u * SG2_SP + rl;
-        const int tau = h * SG2_BP + u;
 #pragma unroll
         for (int w = 0; w < NW; w++) {
-          const int lw = (FWD ? w : NW - 1 - w) * SG2_L;
-          const int d = FWD ? 2 + tau - lw : A + B - tau + lw;
 #pragma unroll
           for (int k = 0; k < 8; k++) {
-            const int sw = 1 + 64 * w + 8 * k + rl, scw = min(sw, A), f = d - sw;
-            const bool ok = sw <= A && f >= 1 && f <= B;
-            const long cell = qb + scw * SS + f;
-            const int kopw = ((C == 2) ? q : scw) - a.kofs;
+            const int m = -(int)((unsigned)(fm1[w][k] + h * FST) < (unsigned)B);
             const z2 v = src[w * SG2_BP * SG2_SP + 8 * k];
-            sg2_st(v, Wb, ok ? (int)cell * (int)sizeof(z2) : SG2_OOB);
-            if (!FWD)
-              sg2_st(v, Yb, ok && kopw >= 0 ? (int)(cell - (long)a.kofs * PL) * (int)sizeof(z2) : SG2_OOB);
+            sg2_st(v, Wb, ((wo[w][k] + h * OST) & m) | (SG2_OOB & ~m));
+            if (!FWD) sg2_st(v, Yb, ((yo[w][k] + h * OST) & m) | (SG2_OOB & ~m));
           }
         }
       }
       if (!FWD) sg2_zero_frame<C>(a, q, t, nt);
       return;
     }
-    // Ex: rows a plane apart, columns a row apart: one step of 64 rows per store
+    // Ex: rows a plane apart, columns a row apart: one step of 64 rows per
+    // store; step tau moves f by one, the offsets by FS cells
+    int fm1[NW], wo[NW], yo[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+      const int sw = 1 + 64 * w + lane, scw = min(sw, A);
+      const int lw = (FWD ? w : NW - 1 - w) * SG2_L;
+      const long nbw = (long)q + scw * SS;
+      const int kopw = scw - a.kofs, f = (FWD ? 2 - lw : A + B + lw) - sw;
+      fm1[w] = sw <= A ? f - 1 : (1 << 30);
+      wo[w] = (int)(nbw + (long)f * FS) * (int)sizeof(z2);
+      yo[w] = kopw < 0 ? (1 << 30) : (int)(nbw - (long)a.kofs * PL + (long)f * FS) * (int)sizeof(z2);
+    }
+    const int TST = (FWD ? 1 : -1) * FS * (int)sizeof(z2);  // bytes per step
     for (int h = 0; h < nhalf; h++) {
       __syncthreads();
       const z2 *src = stg + (h & 1) * NW * SG2_BP * SG2_SP + lane;
 #pragma unroll
       for (int w = 0; w < NW; w++) {
-        const int sw = 1 + 64 * w + lane, scw = min(sw, A);
-        const int lw = (FWD ? w : NW - 1 - w) * SG2_L;
-        const long nbw = ((C == 0) ? (long)q : (C == 1) ? (long)(n1 + 1) * q : PL * q) + scw * SS;
-        const int kopw = ((C == 2) ? q : scw) - a.kofs;
 #pragma unroll
         for (int u = 0; u < SG2_BP; u++) {
           const int tau = h * SG2_BP + u;
-          const int d = FWD ? 2 + tau - lw : A + B - tau + lw, f = d - sw;
-          const bool ok = sw <= A && f >= 1 && f <= B;
+          const int m = -(int)((unsigned)(fm1[w] + (FWD ? tau : -tau)) < (unsigned)B);
           const z2 v = src[(w * SG2_BP + u) * SG2_SP];
-          if (FWD || C != 0) sg2_st(v, Wb, ok ? (int)(nbw + (long)f * FS) * (int)sizeof(z2) : SG2_OOB);
-          if (!FWD)
-            sg2_st(v, Yb, ok && kopw >= 0 ? (int)(nbw - (long)a.kofs * PL + (long)f * FS) * (int)sizeof(z2) : SG2_OOB);
+          if (FWD) sg2_st(v, Wb, ((wo[w] + tau * TST) & m) | (SG2_OOB & ~m));
+          else sg2_st(v, Yb, ((yo[w] + tau * TST) & m) | (SG2_OOB & ~m));
         }
       }
     }

@@ -489,7 +489,7 @@ static dim3 mx_tile_grid3(const SkewGeo &g, int c)
 template <int C, bool FWD>
 static void mx_sgs_launch(dim3 g, int tb, size_t lds, hipStream_t s, const MxLu &a, double bytes)
 {
-  const int nt = tb + (C != 0 ? 64 : 0);  // Ey, Ez: and the store wave
+  const int nt = tb + (C != 0 ? 64 * mxs_store_waves(tb / 64) : 0);  // Ey, Ez: and the store waves
   switch (tb / 64) {
     case 1: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 1>, g, nt, lds, a); break;
     case 2: kt_launch(KT_M_SGS, s, bytes, k_mx_sgs<C, FWD, 2>, g, nt, lds, a); break;

@@ -914,7 +914,9 @@ __device__ __forceinline__ void mx_zero_frame(c2 *base, int nu, int nv, long su,
 // computing waves put each step's values into an LDS staging area (two
 // halves of MXS_BP steps), and one more wave of the block stores each half
 // to W and y while the others fill the other half, eight lanes per row, so
-// a store instruction writes eight rows' runs of MXS_BP values.  Ex (rows a
+// a store instruction writes eight rows' runs of MXS_BP values; a store
+// wave that issued as many instructions as a computing wave would hold
+// every barrier, so its offsets are precomputed per lane and blended.  Ex (rows a
 // plane apart, runs only across slices): the values go to V in the records'
 // slot order, one contiguous run per wave and step, and k_mx_unskew copies
 // them to W and y through tiles that span slices.  Every value a step needs comes from registers
@@ -952,6 +954,10 @@ __host__ __device__ __forceinline__ int mxs_table_len(int A, int B, int AP, int 
 {
   return mxs_table_off(AP, nw) + A + B + (nw - 1) * MXS_L + MXS_U + MXS_PC + 4;
 }
+// store waves of a sweep block of nw computing waves (Ey, Ez): one (a
+// second one, on the SIMD a block of at most two computing waves leaves
+// free, measured no faster)
+__host__ __device__ constexpr int mxs_store_waves(int) { return 1; }
 #define MXS_SP 72  // staging slots per step: 64 rows, padded so that the store wave's reads (eight
                    // steps of eight rows) fall in at most two-way bank conflicts
 __host__ __device__ __forceinline__ long mxs_lds_bytes(int A, int B, int AP, int c)
@@ -960,7 +966,7 @@ __host__ __device__ __forceinline__ long mxs_lds_bytes(int A, int B, int AP, int
   return (long)nw * (2 * MXS_RING + 64) * 8 + (c != 0 ? 2L * nw * MXS_BP * MXS_SP * 8 : 0) +
          (long)mxs_table_len(A, B, AP, nw) * 4;
 }
-// the computing waves of a block of AP rows, and for Ey, Ez the store wave
+// the computing waves of a block of AP rows, and for Ey, Ez the store waves
 template <int C, bool FWD, int NW>
 __global__ __launch_bounds__(64 * (NW + 1)) void k_mx_sgs(MxLu a)
 {
@@ -972,7 +978,8 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_mx_sgs(MxLu a)
   if (q >= nslice || !mx_live(a.sc, r)) return;
   const int n1 = a.g.n1, n2 = a.g.n2;
   constexpr bool SW = (C != 0);  // a store wave writes W and y (Ex: V and k_mx_unskew)
-  constexpr int nt = 64 * (NW + (SW ? 1 : 0)), AP = 64 * NW, lagmax = (NW - 1) * MXS_L;
+  constexpr int NSW = SW ? mxs_store_waves(NW) : 0;
+  constexpr int nt = 64 * (NW + NSW), AP = 64 * NW, lagmax = (NW - 1) * MXS_L;
   constexpr int P = MXS_P;
   const int t = threadIdx.x, lane = t & 63;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -1030,7 +1037,7 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_mx_sgs(MxLu a)
       mx_zero_frame(y0 + a.g.lEx + a.g.lEy + (long)(n1 + 1) * (n2 + 1) * (q - a.g.kofs), n2 + 1, n1 + 1, n1 + 1, 1, 1,
                     n2 - 1, t, nt);
   };
-  if (SW && wv == NW) {
+  if (SW && wv >= NW) {
     // the store wave: after the barrier that ends half h (steps MXS_BP h ..
     // MXS_BP h + MXS_BP - 1), store that half's values to W and (backward)
     // y while the computing waves fill the other half.  Lane l takes step
@@ -1047,28 +1054,45 @@ __global__ __launch_bounds__(64 * (NW + 1)) void k_mx_sgs(MxLu a)
     const __amdgpu_buffer_rsrc_t Yb = mx_rsrc(a.y + r * a.g.n + xofs, ylen * (long)sizeof(c2));
     const int u = lane % MXS_BP, rl = lane / MXS_BP;
     static_assert(MXS_BP == 8, "eight lanes per row, eight rows per store");
+    // per lane and (w, k), for half 0: f - 1 of the lane's cell (past any
+    // column on a row past A) and the cell's byte offsets in W and y (1 GiB,
+    // past the buffer whatever half, where the row has no y plane); half h
+    // moves f by MXS_BP, forward or backward.  This wave issues about as many
+    // instructions per half as a computing wave, so a store costs a handful
+    // of 32-bit operations.
+    // Store wave sw0 takes the groups of eight rows g = NSW j + sw0 (wave
+    // g / 8, rows 8 (g % 8) .. of it).
+    constexpr int NG = NW * 8 / (NSW > 0 ? NSW : 1);
+    const int sw0 = wv - NW;
+    int fm1[NG], wo[NG], yo[NG];
+#pragma unroll
+    for (int j = 0; j < NG; j++) {
+      const int g = NSW * j + sw0, w = g / 8, k = g % 8;
+      const int lw = (FWD ? w : NW - 1 - w) * MXS_L;
+      const int d = FWD ? 2 + u - lw : A + B - u + lw;
+      const int sw = 1 + 64 * w + 8 * k + rl, scw = min(sw, A), f = d - sw;
+      const long cell = qb + scw * SS + f;
+      const int kopw = ((C == 2) ? q : scw) - a.g.kofs;
+      fm1[j] = sw <= A ? f - 1 : (1 << 30);
+      wo[j] = (int)cell * (int)sizeof(c2);
+      yo[j] = (FWD || kopw < 0) ? (1 << 30) : (int)(cell - (long)a.g.kofs * PL) * (int)sizeof(c2);
+    }
+    constexpr int FST = FWD ? MXS_BP : -MXS_BP, OST = FST * (int)sizeof(c2);
     __syncthreads();  // the computing waves' first barrier
     const int nhalf = nex / MXS_BP;
     for (int h = 0; h < nhalf; h++) {
       __syncthreads();
       const c2 *src = stg + (h & 1) * NW * MXS_BP * MXS_SP + u * MXS_SP + rl;
-      const int tau = h * MXS_BP + u;
 #pragma unroll
-      for (int w = 0; w < NW; w++) {
-        const int lw = (FWD ? w : NW - 1 - w) * MXS_L;
-        const int d = FWD ? 2 + tau - lw : A + B - tau + lw;
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-          const int sw = 1 + 64 * w + 8 * k + rl, scw = min(sw, A), f = d - sw;
-          const bool ok = sw <= A && f >= 1 && f <= B;
-          const long cell = qb + scw * SS + f;
-          const int kopw = ((C == 2) ? q : scw) - a.g.kofs;
-          const v2u v = __builtin_bit_cast(v2u, src[w * MXS_BP * MXS_SP + 8 * k]);
-          __builtin_amdgcn_raw_buffer_store_b64(v, Wb, ok ? (int)cell * (int)sizeof(c2) : MX_OOB, 0, 0);
-          if (!FWD)
-            __builtin_amdgcn_raw_buffer_store_b64(
-                v, Yb, ok && kopw >= 0 ? (int)(cell - (long)a.g.kofs * PL) * (int)sizeof(c2) : MX_OOB, 0, 0);
-        }
+      for (int j = 0; j < NG; j++) {
+        const int g = NSW * j + sw0;
+        // a lane off the slice stores past the buffers (a blend, not a
+        // branch: every lane issues both stores)
+        const int m = -(int)((unsigned)(fm1[j] + h * FST) < (unsigned)B);
+        const v2u v = __builtin_bit_cast(v2u, src[(g / 8) * MXS_BP * MXS_SP + 8 * (g % 8)]);
+        __builtin_amdgcn_raw_buffer_store_b64(v, Wb, ((wo[j] + h * OST) & m) | (MX_OOB & ~m), 0, 0);
+        if (!FWD)
+          __builtin_amdgcn_raw_buffer_store_b64(v, Yb, ((yo[j] + h * OST) & m) | (MX_OOB & ~m), 0, 0);
       }
     }
     if (!FWD) zero_frame();
