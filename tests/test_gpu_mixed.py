@@ -199,6 +199,38 @@ def test_lusgs32_close_to_fp64(workdir):
         ses.close()
 
 
+def test_lusgs32_follows_frequency_changes(workdir):
+    """The FP32 records' per-frequency half (h = iwmu/aii) is kept per
+    solve slot and rebuilt when a right-hand side's frequency changes: the
+    preconditioner at 0.25, 1.25 and again 0.25 Hz on one slot gives the
+    first result bit for bit the third time and matches the oracle at each
+    frequency (an h left from the other frequency misses by far more)."""
+    print("case lusgs32 frequency changes", flush=True)
+    s, ses, orc = _setup(workdir)
+    L = _lib.lib()
+    try:
+        n = ses.n
+        rng = np.random.default_rng(21)
+        x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+        dx, dy = _dev(x), _dev(np.zeros_like(x))
+        ys = []
+        for f in (0.25, 1.25, 0.25):
+            fr = (_lib.FreqT * 1)(freq_consts(f))
+            _lib.check(L.csem_lusgs32(ses.model, 1, fr, dx, dy, None), "lusgs32")
+            ys.append(_host(dy, x))
+            orc.set_freq(f)
+            y64 = orc.lusgs(x.astype(np.complex128))
+            err = np.linalg.norm(ys[-1] - y64) / np.linalg.norm(y64)
+            print("f=%g lusgs32 vs oracle %.2e" % (f, err), flush=True)
+            assert err < 1e-4, (f, err)
+        assert np.array_equal(ys[0].view(np.float32), ys[2].view(np.float32))
+        assert not np.array_equal(ys[0].view(np.float32), ys[1].view(np.float32))
+        L.csem_free(dx)
+        L.csem_free(dy)
+    finally:
+        ses.close()
+
+
 def test_fp32_kernels_multi_tile(workdir):
     """Several operator tiles along x and y, LU-SGS slices of more than 64
     rows (several waves per slice, lagged edge exchange): the FP32 operator
