@@ -117,6 +117,7 @@ DEV_SYMBOLS = [
     "csem_gmres_prec_release", "csem_bicgstab_op", "csem_gmres_op",
     "csem_csr_create", "csem_csr_destroy", "csem_csr_nnz", "csem_csr_rows", "csem_csr_download", "csem_op_csr",
     "csem_op_csr_sgs",
+    "csem_model_apply_planes", "csem_model_set_apply_planes",
     "csem_mixed_solve", "csem_apply32", "csem_lusgs32", "csem_mixed_tuning", "csem_mixed_set_tile", "csem_mixed_tuning_lu", "csem_mixed_set_lu_tiles", "csem_mixed_bytes", "csem_mixed_slot_bytes", "csem_mem_info",
 ]
 
@@ -163,6 +164,8 @@ def lib():
     L.csem_lusgs32.argtypes = [vp, ip, C.POINTER(FreqT), vp, vp, vp]
     L.csem_mixed_tuning.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip)]
     L.csem_mixed_set_tile.argtypes = [vp, ip, ip, ip]
+    L.csem_model_apply_planes.argtypes = [vp]
+    L.csem_model_set_apply_planes.argtypes = [vp, ip]
     L.csem_mixed_tuning_lu.argtypes = [vp, ip, vp, C.POINTER(ip)]
     L.csem_csr_create.argtypes = [C.POINTER(vp), ip, ip, ip] + [vp] * 9 + [dp, dp]
     L.csem_csr_destroy.argtypes = [vp]

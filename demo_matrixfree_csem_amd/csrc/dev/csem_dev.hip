@@ -257,6 +257,71 @@ static int pick_apply_planes(csem_model *m)
   return 0;
 }
 
+static ApplyArgs apply_args(const csem_model *m, const csem_freq *f, const z2 *x, z2 *y);
+
+// Every plane depth gives the same values (a tile computes each cell in the
+// same order whatever its depth), so the depth is picked by speed: each
+// candidate's k_apply is timed on this model (one warm and three timed
+// launches on zeroed vectors of the model's length) once, at creation,
+// before any other thread can reach the model.  The cost model's pick
+// above stays when the two scratch vectors do not fit.  The launches go on
+// the null stream under the device-wide lock, which graph captures on
+// other threads hold too.
+template <int KC>
+static float time_apply_kc(csem_model *m, const z2 *x, z2 *y, hipEvent_t e0, hipEvent_t e1)
+{
+  const csem_freq f = {0.0, 1.0, 0.0, -1.0};
+  m->kc = KC;
+  const ApplyArgs a = apply_args(m, &f, x, y);
+  const int ntiles = a.nti * a.ntj * a.ntk;
+  const dim3 blk(64, AP_NW), grd(8 * ((ntiles + 7) / 8));
+  k_apply<KC><<<grd, blk>>>(a);
+  (void)hipEventRecord(e0, 0);
+  for (int w = 0; w < 3; w++) k_apply<KC><<<grd, blk>>>(a);
+  (void)hipEventRecord(e1, 0);
+  float ms = 1e30f;
+  if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 1e30f;
+  return ms;
+}
+static int time_apply_planes(csem_model *m)
+{
+  const long bytes = m->lE * (long)sizeof(z2);
+  z2 *x = nullptr, *y = nullptr;
+  if (dev_malloc((void **)&x, bytes) != hipSuccess || dev_malloc((void **)&y, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    if (x) dev_free(x);
+    return 0;
+  }
+  const int pick = m->kc;
+  int kc = pick;
+  hipError_t e = hipSuccess;
+  {
+    std::lock_guard<std::mutex> lk(g_devwide_mu);
+    hipEvent_t e0, e1;
+    e = hipMemset(x, 0, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) {
+      if ((e = hipEventCreate(&e1)) == hipSuccess) {
+        const int kcs[5] = {6, 8, 10, 12, 16};
+        const float t[5] = {time_apply_kc<6>(m, x, y, e0, e1), time_apply_kc<8>(m, x, y, e0, e1),
+                            time_apply_kc<10>(m, x, y, e0, e1), time_apply_kc<12>(m, x, y, e0, e1),
+                            time_apply_kc<16>(m, x, y, e0, e1)};
+        float best = 1e30f;
+        for (int c = 0; c < 5; c++)
+          if (t[c] < best) { best = t[c]; kc = kcs[c]; }
+        (void)hipEventDestroy(e1);
+      }
+      (void)hipEventDestroy(e0);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+  }
+  m->kc = e == hipSuccess ? kc : pick;
+  dev_free(x);
+  dev_free(y);
+  if (e != hipSuccess) return csem_fail(-1, "timing the operator tile depths: %s", hipGetErrorString(e));
+  return 0;
+}
+
 extern "C" void csem_model_destroy(csem_model *m)
 {
   if (!m) return;
@@ -576,6 +641,7 @@ static int model_create(const csem_model_desc *d, const csem_model_raw *raw, cse
 #undef UP
   if ((rc = set_lds_attrs())) goto fail;
   if ((rc = pick_apply_planes(m))) goto fail;
+  if ((rc = time_apply_planes(m))) goto fail;
   *out = m;
   return 0;
 fail:
@@ -586,6 +652,14 @@ fail:
 extern "C" long csem_model_len(const csem_model *m) { return m->lE; }
 extern "C" long csem_model_full_len(const csem_model *m) { return m->fE; }
 extern "C" long csem_model_bytes(const csem_model *m) { return m->bytes; }
+extern "C" int csem_model_apply_planes(const csem_model *m) { return m->kc; }
+extern "C" int csem_model_set_apply_planes(csem_model *m, int planes)
+{
+  if (planes != 6 && planes != 8 && planes != 10 && planes != 12 && planes != 16)
+    return csem_fail(-3, "operator planes per tile %d: 6, 8, 10, 12 or 16", planes);
+  m->kc = planes;
+  return 0;
+}
 
 static inline hipStream_t S(void *s) { return (hipStream_t)s; }
 

@@ -275,6 +275,12 @@ int csem_mixed_solve(csem_model *m, int R, const csem_freq *f, void *x, const vo
  * back (length R * csem_model_len), for tests and custom drivers. */
 int csem_apply32(csem_model *m, int R, const csem_freq *f, const void *x, void *y, void *stream);
 int csem_lusgs32(csem_model *m, int R, const csem_freq *f, const void *x, void *y, void *stream);
+/* Planes per tile of the FP64 operator (6, 8, 10, 12 or 16): the fastest on
+ * the model, timed once when the model is created; every depth gives the
+ * same values.  csem_model_set_apply_planes fixes another depth; it must
+ * not run while another thread applies the model's operator. */
+int csem_model_apply_planes(const csem_model *m);
+int csem_model_set_apply_planes(csem_model *m, int planes);
 /* Tile shape the autotuner picked for the FP32 operator at batch size R
  * (rows per block, planes per block); -1 before the first solve. */
 int csem_mixed_tuning(csem_model *m, int R, int *rows, int *planes);

@@ -86,6 +86,18 @@ def test_apply_lusgs_bitwise_multi_tile(workdir, airbc):
         y = ses.fdfd_apply(x)
         yo = orc.apply(x)
         assert np.array_equal(y, yo), np.abs(y - yo).max() / np.abs(yo).max()
+        # the timed plane depth is one of the candidates, and every depth
+        # gives the oracle's values
+        from demo_matrixfree_csem_amd import _lib
+        L = _lib.lib()
+        timed = L.csem_model_apply_planes(ses.model)
+        assert timed in (6, 8, 10, 12, 16), timed
+        for kc in (6, 8, 10, 12, 16):
+            assert L.csem_model_set_apply_planes(ses.model, kc) == 0
+            y = ses.fdfd_apply(x)
+            assert np.array_equal(y, yo), (kc, np.abs(y - yo).max() / np.abs(yo).max())
+        assert L.csem_model_set_apply_planes(ses.model, 7) != 0
+        assert L.csem_model_set_apply_planes(ses.model, timed) == 0
         y = ses.lusgs_apply(x)
         yo = orc.lusgs(x)
         assert np.array_equal(y, yo), np.abs(y - yo).max() / np.abs(yo).max()
